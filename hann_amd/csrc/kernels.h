@@ -1,0 +1,27 @@
+// kernels.h -- host-side launchers of the hann_amd device kernels.
+#pragma once
+
+#include "common.h"
+
+namespace hann {
+
+// Grow-only device scratch, in numbered slots, owned by one stream's user.
+struct Workspace {
+    DevBuf<char> slots[12];
+    template <typename T>
+    T *get(int slot, size_t count) {
+        slots[slot].resize(count * sizeof(T) + 256);
+        return reinterpret_cast<T *>(slots[slot].ptr);
+    }
+};
+
+int kp_for(int k);
+
+// distance.hip
+void launch_distance_batch(int metric, const float *queries, int64_t nq, const float *base,
+                           int64_t nb, int dim, double *out, hipStream_t st);
+void launch_topk_exact(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
+                       int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws,
+                       hipStream_t st);
+
+}  // namespace hann

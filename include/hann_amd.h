@@ -1,0 +1,241 @@
+/* hann_amd.h -- C-ABI of the MI355X (gfx950) library behind hann's
+ * batch-distance and PQIVF hot path.
+ *
+ * Every entry point takes plain pointers and sizes. Pointers are host
+ * pointers unless the name ends in _dev, in which case they are device
+ * pointers and `stream` is a hipStream_t passed as void* (NULL = the
+ * library's own stream for that call). Host buffers are borrowed only for the
+ * duration of the call; the library never keeps a caller pointer (the rule of
+ * core/distance.go:69-72). Calls that return int return a hann_status; the
+ * message of the last failure on the calling thread is hann_last_error().
+ *
+ * Reference interfaces replaced (habedi/hann):
+ *   simd_* / hann_cpu_init          core/simd_distance.h:26-35, core/simd_ops.h:11-14
+ *   Metric.DistanceBatch/RankBatch  core/metric.go:64-100
+ *   pqivf.New + With* options       pqivf/index.go:56-146
+ *   Index / BulkIndex / Trainer     core/index.go:14-69, pqivf/index.go:212-1101
+ *   IndexStats                      core/index.go:131-139
+ *   test hooks of export_test.go    pqivf/export_test.go:5-31
+ */
+#ifndef HANN_AMD_H
+#define HANN_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status -- */
+typedef enum {
+    HANN_OK = 0,
+    HANN_ERR_INVALID_ARG = 1,  /* validation failure (dimension, k, ids ...) */
+    HANN_ERR_NOT_TRAINED = 2,  /* search on an untrained index */
+    HANN_ERR_EMPTY = 3,        /* "index is empty", too few vectors to train */
+    HANN_ERR_DUPLICATE_ID = 4, /* "id %d already exists ..." */
+    HANN_ERR_NOT_FOUND = 5,    /* "id %d not found" */
+    HANN_ERR_OOM = 6,          /* device or host allocation failed, or over budget */
+    HANN_ERR_DEVICE = 7,       /* HIP runtime error, or no gfx950 device */
+    HANN_ERR_FORMAT = 8,       /* bad or newer serialized index */
+    HANN_ERR_UNSUPPORTED = 9   /* a parameter outside what the kernels support */
+} hann_status;
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *hann_last_error(void);
+
+/* Version string of the library. */
+const char *hann_version(void);
+
+/* Select and initialise a device (replaces hann_cpu_init's role as the
+ * one-time set-up, core/simd_ops.c:58-76). Idempotent. */
+int hann_init(int device);
+
+/* Drop-in for hann_cpu_init (core/simd_ops.h:11, called from
+ * core/cpu_check.go:34-44). The SIMD level argument is accepted and ignored:
+ * there is one code path, on the GPU. Initialises device 0. */
+void hann_cpu_init(int support_level);
+
+/* ------------------------------------------- reference distance symbols --
+ * Same names and signatures as core/simd_distance.h:26-34, host pointers,
+ * synchronous. Results are bit-identical to the reference's AVX2+FMA tier
+ * (AVX tier for Manhattan, which that tier aliases). */
+float simd_euclidean(const float *a, const float *b, size_t n);
+float simd_squared_euclidean(const float *a, const float *b, size_t n);
+float simd_manhattan(const float *a, const float *b, size_t n);
+float simd_cosine_distance(const float *a, const float *b, size_t n);
+void simd_euclidean_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
+void simd_squared_euclidean_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
+void simd_manhattan_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
+void simd_cosine_distance_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
+
+/* ------------------------------------------------------ batch distance -- */
+typedef enum {
+    HANN_METRIC_SQEUCLIDEAN = 0,
+    HANN_METRIC_EUCLIDEAN = 1,
+    HANN_METRIC_MANHATTAN = 2,
+    HANN_METRIC_COSINE = 3
+} hann_metric;
+
+typedef enum {
+    HANN_PREC_FP32 = 0, /* exact reference-order FP32 (VALU) */
+    HANN_PREC_BF16 = 1, /* BF16 MFMA candidate pass + exact FP32 rerank */
+} hann_precision;
+
+/* Metric.DistanceBatch / RankBatch (core/metric.go:64-100) for nq queries at
+ * once: out[q*nb + i] = metric(queries[q], base[i]) as the reference computes
+ * it and widens to double. rank != 0 gives the rank value (squared distance
+ * for Euclidean, core/metric.go:155-158). Validation as batchSize
+ * (core/metric.go:104-117): dim == 0 is an error; nb == 0 is a no-op. */
+int hann_distance_batch(int metric, int rank, const float *queries, size_t nq, const float *base,
+                        size_t nb, size_t dim, double *out);
+int hann_distance_batch_dev(int metric, int rank, const float *queries, size_t nq,
+                            const float *base, size_t nb, size_t dim, double *out, void *stream);
+
+/* Exact top-k: for each query the k base rows with the smallest rank value,
+ * ordered by (rank, row index). out_ids[q*k + j] is a row index (-1 when
+ * nb < k), out_dist is the true distance in double (FromRank applied).
+ * precision HANN_PREC_BF16 runs an MFMA pass over BF16 copies of the inputs
+ * that keeps `overfetch` candidates per query (0 = 4k), then reranks those
+ * in exact FP32; it supports the (squared) Euclidean metrics. */
+int hann_batch_topk(int metric, const float *queries, size_t nq, const float *base, size_t nb,
+                    size_t dim, size_t k, int precision, size_t overfetch, int64_t *out_ids,
+                    double *out_dist);
+int hann_batch_topk_dev(int metric, const float *queries, size_t nq, const float *base, size_t nb,
+                        size_t dim, size_t k, int precision, size_t overfetch, int64_t *out_ids,
+                        double *out_dist, void *stream);
+
+/* ------------------------------------------------------------- k-means --
+ * assignPoints (pqivf/index.go:738-789): out[i] = nearest centroid of row i,
+ * lowest index on a tie. precision HANN_PREC_BF16 uses an MFMA pass and
+ * rechecks the two best candidates in exact FP32, falling back to the exact
+ * scan for any point whose margin does not certify the winner; results are
+ * identical to HANN_PREC_FP32. */
+int hann_assign_points(const float *data, size_t n, size_t d, const float *centroids, size_t k,
+                       int precision, int32_t *out);
+int hann_assign_points_dev(const float *data, size_t n, size_t d, const float *centroids, size_t k,
+                           int precision, int32_t *out, void *stream);
+
+/* runKMeans (pqivf/index.go:795-847) with the library's own random stream:
+ * `seed` selects the stream, init 0 = random permutation (as the reference),
+ * 1 = k-means++ (extension). Writes min(k, n) centroids and returns that
+ * count in *k_out. */
+int hann_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uint64_t seed,
+                int init, int precision, float *out_centroids, size_t *k_out);
+
+/* --------------------------------------------------------------- PQIVF -- */
+typedef struct hann_pqivf hann_pqivf;
+
+/* Options of pqivf.New (pqivf/index.go:56-146). A zero field takes the
+ * reference default: coarse_k 16, pq_k 16, kmeans_iters 10,
+ * candidate_clusters 3, num_subquantizers = largest of 8,4,2,1 dividing dim.
+ * To pass an explicit non-positive value (which New rejects), set the
+ * matching bit in `explicit_mask` (bit 0 coarse_k, 1 num_subquantizers,
+ * 2 pq_k, 3 kmeans_iters, 4 candidate_clusters). */
+typedef struct {
+    int64_t coarse_k;
+    int64_t num_subquantizers;
+    int64_t pq_k;
+    int64_t kmeans_iters;
+    int64_t candidate_clusters;
+    int32_t bruteforce_fallback; /* 1 = on (default), 0 = off; see explicit_mask bit 5 */
+    uint32_t explicit_mask;
+    uint64_t seed;               /* random stream seed (the HANN_SEED role, core/utils.go:12-20) */
+    /* build-side extensions */
+    int32_t train_precision;     /* hann_precision for k-means assignment */
+    int32_t kmeans_init;         /* 0 perm (reference), 1 k-means++ */
+    int64_t train_sample;        /* 0 = train on all vectors; else a seeded sample size */
+} hann_pqivf_options;
+
+void hann_pqivf_default_options(hann_pqivf_options *opts);
+
+/* pqivf.New: validation order and messages as pqivf/index.go:106-146. */
+int hann_pqivf_new(int64_t dim, const hann_pqivf_options *opts, hann_pqivf **out);
+void hann_pqivf_free(hann_pqivf *idx);
+
+/* Add (pqivf/index.go:212-260). `len` is the vector's length, which the
+ * call validates against the index dimension. */
+int hann_pqivf_add(hann_pqivf *idx, int64_t id, const float *vec, size_t len);
+/* BulkAdd (266-297): vectors packed back to back in `flat`, vector i has
+ * lens[i] floats (lens == NULL: every vector has dim floats). Applied in
+ * ascending id order and not atomic, exactly as the reference. */
+int hann_pqivf_bulk_add(hann_pqivf *idx, const int64_t *ids, const float *flat, const size_t *lens,
+                        size_t n);
+int hann_pqivf_delete(hann_pqivf *idx, int64_t id);                        /* 300-345 */
+int hann_pqivf_bulk_delete(hann_pqivf *idx, const int64_t *ids, size_t n); /* 350-384 */
+int hann_pqivf_update(hann_pqivf *idx, int64_t id, const float *vec, size_t len); /* 402-413 */
+int hann_pqivf_bulk_update(hann_pqivf *idx, const int64_t *ids, const float *flat,
+                           const size_t *lens, size_t n);                 /* 419-444 */
+int hann_pqivf_train(hann_pqivf *idx);                                     /* 451-605 */
+
+/* Scan modes of the inverted-list scan. */
+typedef enum {
+    HANN_SCAN_EXACT = 0,   /* decode + reference-order FP32 distance: bit-identical to
+                              pqivf/index.go:905-936 on the AVX2 tier */
+    HANN_SCAN_ADC_F32 = 1, /* FP32 ADC lookup tables */
+    HANN_SCAN_ADC_F16 = 2  /* FP16 ADC lookup tables, FP32 accumulation */
+} hann_scan_mode;
+
+typedef struct {
+    int32_t scan_mode;        /* hann_scan_mode */
+    int32_t coarse_precision; /* hann_precision for the coarse step (BF16 reranks) */
+    int32_t rerank;           /* 0: PQ distances (reference); R>0: rerank R*k candidates
+                                 with exact FP32 distances on the stored vectors */
+    int32_t fused;            /* 1: persistent fused scan over (query, list) work items */
+} hann_search_params;
+
+void hann_search_default_params(hann_search_params *p);
+
+/* Search (pqivf/index.go:850-949) of one query. Writes up to k results to
+ * out_ids/out_dist in non-decreasing distance order, *out_count results.
+ * Validation order and messages as the reference. */
+int hann_pqivf_search(hann_pqivf *idx, const float *query, size_t len, int64_t k, int64_t *out_ids,
+                      double *out_dist, int64_t *out_count);
+
+/* Batched search: nq queries of dim floats, per-query k[q] and nprobe[q]
+ * (nprobe == NULL: the index's candidate_clusters for every query).
+ * Outputs are [nq][kmax] with kmax = max k; counts[q] results for query q. */
+int hann_pqivf_search_batch(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
+                            const int32_t *nprobe, const hann_search_params *params,
+                            int64_t *out_ids, double *out_dist, int32_t *out_counts);
+/* Same with device-resident queries, k, nprobe and outputs (out_dist in FP32
+ * true distance here); asynchronous on `stream`. */
+int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
+                                const int32_t *nprobe, int32_t kmax, int32_t max_nprobe,
+                                const hann_search_params *params, int64_t *out_ids,
+                                float *out_dist, int32_t *out_counts, void *stream);
+
+/* IndexStats (core/index.go:131-139). */
+typedef struct {
+    int64_t count;
+    int64_t dimension;
+    int64_t fallback_searches;
+    char distance[32]; /* "euclidean" */
+} hann_index_stats;
+int hann_pqivf_stats(hann_pqivf *idx, hann_index_stats *out);
+
+/* Save / Load (pqivf/index.go:968-1101) in the library's binary format.
+ * save: *buf is malloc'ed, release with hann_free_buffer. */
+int hann_pqivf_save(hann_pqivf *idx, void **buf, size_t *len);
+int hann_pqivf_load(hann_pqivf *idx, const void *buf, size_t len);
+void hann_free_buffer(void *buf);
+
+/* ------------------------------------------ test hooks (export_test.go) -- */
+int hann_pqivf_is_trained(hann_pqivf *idx);
+/* sizes: *k = coarse centroids, *m subquantizers, *p = max codebook rows */
+int hann_pqivf_shape(hann_pqivf *idx, int64_t *k, int64_t *m, int64_t *p, int64_t *dim);
+int hann_pqivf_coarse_centroids(hann_pqivf *idx, float *out /* [k][dim] */);
+/* codebooks [m][p][dim/m]; rows beyond cb_len[i] are zero */
+int hann_pqivf_codebooks(hann_pqivf *idx, float *out, int64_t *cb_len);
+/* list l's ids in storage order: call with out == NULL to get the length */
+int hann_pqivf_list_ids(hann_pqivf *idx, int64_t list, int64_t *out, int64_t *len);
+/* list l's PQ codes [len][m] in storage order (-1 rows: entry has no codes) */
+int hann_pqivf_list_codes(hann_pqivf *idx, int64_t list, int32_t *out, int64_t *len);
+/* encodeVector (pqivf/index.go:669-675) */
+int hann_pqivf_encode_vector(hann_pqivf *idx, const float *vec, size_t len, int64_t cluster,
+                             int32_t *codes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

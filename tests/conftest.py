@@ -1,0 +1,48 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and libhann_amd.so")
+
+
+@pytest.fixture(scope="session")
+def oracle():
+    import oracle as o
+    o.lib()
+    return o
+
+
+@pytest.fixture(scope="session")
+def hann():
+    """The product library, loaded once per session on the GPU box."""
+    import hann_amd
+    from hann_amd import _lib
+    lib = _lib.load()
+    _lib.check(lib.hann_init(0))
+    return hann_amd
+
+
+@pytest.fixture(scope="session")
+def distance_golden():
+    return np.load(os.path.join(GOLDEN, "distance_golden.npz"))
+
+
+@pytest.fixture(scope="session")
+def gob_fixture():
+    import json
+    with open(os.path.join(GOLDEN, "index_v1.json")) as fh:
+        return json.load(fh)
+
+
+def bits_to_f32(a):
+    return np.array(a, np.uint32).view(np.float32)

@@ -1,0 +1,290 @@
+"""Pins of the CPU oracle (oracle/oracle.c) -- CPU only.
+
+Each check is chosen so that a plausible slip in the restatement (a wrong
+accumulation order, an unfused tail, a transposed operand, a dropped term)
+fails one of them.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import bits_to_f32
+
+METRICS = [("squared_euclidean", 0), ("euclidean", 1), ("manhattan", 2), ("cosine_distance", 3)]
+
+
+def split_pairs(g):
+    dims = g["pair_dims"]
+    off = np.concatenate([[0], np.cumsum(dims)])
+    a, b = g["pair_a"], g["pair_b"]
+    return [(int(d), a[off[i]:off[i + 1]], b[off[i]:off[i + 1]]) for i, d in enumerate(dims)]
+
+
+@pytest.mark.parametrize("name,mid", METRICS)
+def test_pairs_bit_exact_vs_reference_avx2(oracle, distance_golden, name, mid):
+    """Golden bits from the reference's own C (AVX2 tier, AVX for Manhattan)."""
+    want = distance_golden[f"pair_{name}_avx2"]
+    for i, (d, a, b) in enumerate(split_pairs(distance_golden)):
+        got = oracle.pair(mid, a, b)
+        assert got.view(np.uint32) == want[i].view(np.uint32), (name, d, i, got, want[i])
+
+
+def test_tiers_differ_so_the_pin_is_tier_specific(distance_golden):
+    """The fallback tier rounds differently at some dims: the golden file
+    really distinguishes summation orders."""
+    a = distance_golden["pair_squared_euclidean_avx2"].view(np.uint32)
+    f = distance_golden["pair_squared_euclidean_fallback"].view(np.uint32)
+    assert (a != f).any()
+
+
+@pytest.mark.parametrize("name,mid", METRICS)
+def test_batches_bit_exact_vs_reference(oracle, distance_golden, name, mid):
+    for si, (dim, n) in enumerate(distance_golden["batch_shapes"]):
+        q = distance_golden[f"batch{si}_q"]
+        flat = distance_golden[f"batch{si}_flat"]
+        got = oracle.batch(mid, q, flat, int(dim))
+        want = distance_golden[f"batch{si}_{name}_avx2"]
+        assert np.array_equal(got.view(np.uint64), want.view(np.uint64)), (name, dim, n)
+
+
+def test_closed_forms(oracle):
+    """core/distance_test.go:14-123 (tolerance 1e-6 as there)."""
+    cases = [
+        ([1, 2, 3, 4, 5, 6], [1, 2, 3, 4, 5, 6], 0, 0, 0, 0),
+        ([1, 2, 3, 4, 5, 6], [6, 5, 4, 3, 2, 1], math.sqrt(70), 70, 18, 1 - 56 / 91),
+        ([1, 0, 0, 1, 0, 1], [0, 1, 1, 0, 1, 0], math.sqrt(6), 6, 6, 1),
+        ([-1, -2, -3, -4, -5, -6], [-6, -5, -4, -3, -2, -1], math.sqrt(70), 70, 18, 1 - 56 / 91),
+        ([0, 0, 0, 0, 0, 0], [1, 2, 3, 4, 5, 6], math.sqrt(91), 91, 21, 1),
+        ([1, 2, 3, 4, 5, 6], [1, 2, 3, 4, 5, 6.000001], 1e-6, 1e-12, 1e-6, 0),
+    ]
+    for a, b, eu, sq, man, cos in cases:
+        assert abs(oracle.pair(1, a, b) - eu) < 1e-6
+        assert abs(oracle.pair(0, a, b) - sq) < 1e-6
+        assert abs(oracle.pair(2, a, b) - man) < 1e-6
+        assert abs(oracle.pair(3, a, b) - cos) < 1e-6
+
+
+def test_fp64_differential(oracle):
+    """core/distance_test.go:203-227: 1e-3 relative against float64."""
+    rng = np.random.default_rng(7)
+    for dim in [1, 2, 3, 4, 5, 6, 7, 8, 9, 15, 16, 17, 25, 31, 32, 33, 128, 200, 784]:
+        for _ in range(5):
+            a = (rng.random(dim) * 20 - 10).astype(np.float32)
+            b = (rng.random(dim) * 20 - 10).astype(np.float32)
+            a64, b64 = a.astype(np.float64), b.astype(np.float64)
+            sq = float(((a64 - b64) ** 2).sum())
+            ref = {0: sq, 1: math.sqrt(sq), 2: float(np.abs(a64 - b64).sum()),
+                   3: 1 - max(-1, min(1, float(a64 @ b64) / (np.linalg.norm(a64) * np.linalg.norm(b64))))}
+            for mid, want in ref.items():
+                got = float(oracle.pair(mid, a, b))
+                assert abs(got - want) <= 1e-3 * max(1, abs(want)), (mid, dim)
+
+
+def test_integer_inputs_are_exact(oracle):
+    """Integer-valued inputs: every partial sum is an integer below 2^24, so
+    the reference order equals exact arithmetic."""
+    rng = np.random.default_rng(3)
+    for dim in [8, 64, 96, 128]:
+        a = rng.integers(0, 256, dim).astype(np.float32)
+        b = rng.integers(0, 256, dim).astype(np.float32)
+        assert float(oracle.pair(0, a, b)) == float(((a.astype(np.int64) - b.astype(np.int64)) ** 2).sum())
+        assert float(oracle.pair(2, a, b)) == float(np.abs(a.astype(np.int64) - b.astype(np.int64)).sum())
+
+
+def fma32(a, b, c):
+    """float32 fused multiply-add: the exact a*b+c rounded once to nearest-even."""
+    from fractions import Fraction
+    v = Fraction(float(a)) * Fraction(float(b)) + Fraction(float(c))
+    f = np.float32(float(v))
+    best = f
+    for g in (np.nextafter(f, np.float32(np.inf)), np.nextafter(f, np.float32(-np.inf))):
+        dg, db = abs(Fraction(float(g)) - v), abs(Fraction(float(best)) - v)
+        if dg < db or (dg == db and int(g.view(np.uint32)) % 2 == 0):
+            best = g
+    return np.float32(best)
+
+
+def test_s4_and_s8_special_orders(oracle):
+    """s=4 is a plain fused chain; s=8 is one cleanup vector reduced by the
+    horizontal-sum tree."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        a = rng.standard_normal(4).astype(np.float32)
+        b = rng.standard_normal(4).astype(np.float32)
+        d = (a - b).astype(np.float32)
+        want = np.float32(d[0] * d[0])
+        for i in range(1, 4):
+            want = fma32(d[i], d[i], want)
+        assert oracle.pair(0, a, b) == want
+        a8 = rng.standard_normal(8).astype(np.float32)
+        b8 = rng.standard_normal(8).astype(np.float32)
+        s = ((a8 - b8).astype(np.float32) ** 2).astype(np.float32)
+        f = np.float32
+        want8 = f(f(f(s[0] + s[4]) + f(s[1] + s[5])) + f(f(s[2] + s[6]) + f(s[3] + s[7])))
+        assert oracle.pair(0, a8, b8) == want8
+
+
+# ----------------------------------------------------------- gob fixture --
+def fixture_arrays(g):
+    C = bits_to_f32(g["CoarseCentroidsBits"])
+    CB = bits_to_f32(g["CodebooksBits"])
+    ents = [e for c, es in sorted(g["InvertedLists"].items(), key=lambda x: int(x[0])) for e in es]
+    V = {e["ID"]: bits_to_f32(g["VectorBits"][str(e["ID"])]) for e in ents}
+    return C, CB, ents, V
+
+
+def test_fixture_assign_matches_reference(oracle, gob_fixture):
+    """Stored Cluster of every entry = or_assign over the stored centroids."""
+    C, _, ents, V = fixture_arrays(gob_fixture)
+    data = np.stack([V[e["ID"]] for e in ents])
+    got = oracle.assign(data, C)
+    assert got.tolist() == [e["Cluster"] for e in ents]
+
+
+def test_fixture_encode_matches_reference(oracle, gob_fixture):
+    """Stored Codes of every entry = or_encode of its residual."""
+    C, CB, ents, V = fixture_arrays(gob_fixture)
+    m = gob_fixture["NumSubquantizers"]
+    for e in ents:
+        codes = oracle.encode(V[e["ID"]], C[e["Cluster"]], m, CB, [CB.shape[1]] * m)
+        assert codes.tolist() == e["Codes"], e["ID"]
+
+
+def test_fixture_codebooks_are_point_order_means(oracle, gob_fixture):
+    """The stored codebooks are bit-for-bit the float32 point-order means the
+    update step computes (training data in cluster-ascending list order,
+    pqivf/index.go:520-541, assignment = the stored codes)."""
+    C, CB, ents, V = fixture_arrays(gob_fixture)
+    m = gob_fixture["NumSubquantizers"]
+    d = gob_fixture["Dimension"]
+    s = d // m
+    for i in range(m):
+        sub = np.stack([(V[e["ID"]] - C[e["Cluster"]]).astype(np.float32)[i * s:(i + 1) * s] for e in ents])
+        assign = np.array([e["Codes"][i] for e in ents], np.int32)
+        got, _ = oracle.kmeans_update(sub, assign, CB.shape[1], np.zeros_like(CB[i]), 1)
+        assert np.array_equal(got.view(np.uint32), CB[i].view(np.uint32)), i
+
+
+def test_fixture_coarse_centroids_are_member_means(oracle, gob_fixture):
+    """The legacy file was trained on map-ordered data, so the stored coarse
+    centroids equal the member means to rounding only (order unknown)."""
+    C, _, ents, V = fixture_arrays(gob_fixture)
+    data = np.stack([V[e["ID"]] for e in ents])
+    assign = np.array([e["Cluster"] for e in ents], np.int32)
+    got, _ = oracle.kmeans_update(data, assign, C.shape[0], np.zeros_like(C), 1)
+    assert np.max(np.abs(got - C)) <= 2e-6 * max(1.0, float(np.abs(C).max()))
+
+
+def test_update_is_sequential_float32(oracle):
+    """Cluster sums are sequential float32 sums in point order: equal to the
+    last element of numpy's float32 cumsum, and (on this data) different
+    from a pairwise sum, so the test sees the order."""
+    rng = np.random.default_rng(5)
+    n, d, k = 3000, 16, 3
+    data = (rng.standard_normal((n, d)) * 1000 + 0.1).astype(np.float32)
+    assign = rng.integers(0, k, n).astype(np.int32)
+    got, _ = oracle.kmeans_update(data, assign, k, np.zeros((k, d), np.float32), 1)
+    differs = False
+    for c in range(k):
+        mem = data[assign == c]
+        seq = np.cumsum(mem, axis=0, dtype=np.float32)[-1]
+        want = (seq / np.float32(len(mem))).astype(np.float32)
+        assert np.array_equal(got[c].view(np.uint32), want.view(np.uint32))
+        # numpy reduces a contiguous axis pairwise
+        pair = (np.ascontiguousarray(mem.T).sum(axis=1, dtype=np.float32) / np.float32(len(mem))).astype(np.float32)
+        differs |= not np.array_equal(pair, want)
+    assert differs
+
+
+def test_empty_cluster_reseeds_from_stream(oracle):
+    data = np.arange(40, dtype=np.float32).reshape(10, 4)
+    assign = np.zeros(10, np.int32)
+    state = 12345
+    got, st2 = oracle.kmeans_update(data, assign, 3, np.zeros((3, 4), np.float32), state)
+    s = np.uint64(state)
+    import ctypes
+    st = (ctypes.c_uint64 * 1)(state)
+    i1 = oracle.lib().or_rng_bounded(st, 10)
+    i2 = oracle.lib().or_rng_bounded(st, 10)
+    assert np.array_equal(got[1], data[i1]) and np.array_equal(got[2], data[i2])
+    assert st2 == st[0]
+    assert np.array_equal(got[0], data.mean(axis=0, dtype=np.float32)) or np.allclose(got[0], data.mean(0))
+    del s
+
+
+def test_rng_perm_is_a_permutation_and_deterministic(oracle):
+    p1, s1 = oracle.perm(99, 1000)
+    p2, s2 = oracle.perm(99, 1000)
+    assert sorted(p1.tolist()) == list(range(1000))
+    assert np.array_equal(p1, p2) and s1 == s2
+    p3, _ = oracle.perm(100, 1000)
+    assert not np.array_equal(p1, p3)
+
+
+def test_assign_tie_goes_to_lowest_index(oracle):
+    """pqivf/index_test.go:1005-1066: duplicate centroids tie, lowest wins."""
+    cents = np.array([[1, 1], [0, 0], [0, 0], [5, 5]], np.float32)
+    data = np.array([[0, 0], [0.1, 0], [4, 4]], np.float32)
+    assert oracle.assign(data, cents).tolist() == [1, 1, 3]
+
+
+def test_assign_matches_fp64_brute_force_when_separated(oracle):
+    rng = np.random.default_rng(8)
+    cents = (rng.standard_normal((64, 32)) * 10).astype(np.float32)
+    data = (cents[rng.integers(0, 64, 2000)] + rng.standard_normal((2000, 32)) * 0.5).astype(np.float32)
+    d64 = ((data[:, None, :].astype(np.float64) - cents[None].astype(np.float64)) ** 2).sum(-1)
+    srt = np.sort(d64, axis=1)
+    ok = (srt[:, 1] - srt[:, 0]) > 1e-3 * srt[:, 1]
+    got = oracle.assign(data, cents, nthreads=4)
+    assert np.array_equal(got[ok], d64.argmin(1)[ok])
+
+
+def test_batch_topk_matches_fp64_sort(oracle):
+    rng = np.random.default_rng(9)
+    base = rng.integers(0, 256, (3000, 24)).astype(np.float32)
+    q = rng.integers(0, 256, (5, 24)).astype(np.float32)
+    ids, rank = oracle.batch_topk(0, q, base, 17, nthreads=2)
+    d = ((q[:, None, :].astype(np.int64) - base[None].astype(np.int64)) ** 2).sum(-1)
+    for i in range(5):
+        want = np.lexsort((np.arange(3000), d[i]))[:17]
+        assert ids[i].tolist() == want.tolist()
+        assert np.array_equal(rank[i], d[i][want].astype(np.float32))
+
+
+def test_search_matches_fp64_on_reconstruction(oracle):
+    """or_search ranks by the float32 reconstruction c + cb (decodePQCode +
+    vectorAdd); its order matches FP64 distances to the same reconstruction
+    except where they are within float32 rounding."""
+    rng = np.random.default_rng(10)
+    n, d = 2000, 16
+    cent = (rng.random((8, d)) * 20 - 10).astype(np.float32)
+    x = (cent[rng.integers(0, 8, n)] + rng.standard_normal((n, d))).astype(np.float32)
+    ids = np.arange(n, dtype=np.int64) * 3 + 1
+    ix = oracle.train(ids, x, 8, 4, 16, 5, seed=1)
+    q = (x[:20] + rng.standard_normal((20, d)).astype(np.float32) * 0.1).astype(np.float32)
+    got_ids, got_d, cnt = oracle.search_many(ix, q, 10, 3)
+    s = d // 4
+    recon = np.concatenate([ix.codebooks[i][ix.codes[:, i]] for i in range(4)], axis=1)
+    cl = np.repeat(np.arange(len(ix.list_off) - 1), np.diff(ix.list_off))
+    recon = (ix.coarse[cl] + recon).astype(np.float32)
+    for qi in range(20):
+        assert cnt[qi] == 10
+        dq = np.sqrt(((recon.astype(np.float64) - q[qi].astype(np.float64)) ** 2).sum(1))
+        pos = {int(i): j for j, i in enumerate(ix.ids)}
+        for r in range(10):
+            j = pos[int(got_ids[qi, r])]
+            assert abs(dq[j] - got_d[qi, r]) <= 1e-5 * max(1, dq[j])
+        assert np.all(np.diff(got_d[qi]) >= 0)
+    del s
+
+
+def test_train_lists_in_ascending_id_order(oracle):
+    """pqivf/index_test.go:1072-1097"""
+    rng = np.random.default_rng(12)
+    ids = rng.permutation(500).astype(np.int64)
+    x = rng.standard_normal((500, 8)).astype(np.float32)
+    ix = oracle.train(ids, x, 4, 2, 8, 3, seed=2)
+    for c in range(len(ix.list_off) - 1):
+        seg = ix.ids[ix.list_off[c]:ix.list_off[c + 1]]
+        assert np.all(np.diff(seg) > 0)
