@@ -1,0 +1,120 @@
+"""GPU parity of the batch-distance and batch top-k kernels (C-ABI calls).
+
+Bar: bit-exact against the golden vectors written by the reference's own C
+kernels and against the oracle; top-k ids and rank values identical to the
+oracle's exact (rank, row) ordering."""
+import numpy as np
+import pytest
+
+from test_oracle import split_pairs
+
+pytestmark = pytest.mark.gpu
+
+NAMES = [("squared_euclidean", 0), ("euclidean", 1), ("manhattan", 2), ("cosine_distance", 3)]
+
+
+def test_version(hann):
+    from hann_amd import _lib
+    assert b"gfx950" in _lib.load().hann_version()
+
+
+@pytest.mark.parametrize("name,mid", NAMES)
+def test_drop_in_pair_symbols_bit_exact(hann, distance_golden, name, mid):
+    from hann_amd import _lib
+    lib = _lib.load()
+    fn = {0: lib.simd_squared_euclidean, 1: lib.simd_euclidean, 2: lib.simd_manhattan,
+          3: lib.simd_cosine_distance}[mid]
+    want = distance_golden[f"pair_{name}_avx2"]
+    for i, (d, a, b) in enumerate(split_pairs(distance_golden)):
+        got = np.float32(fn(_lib.fptr(a), _lib.fptr(b), d))
+        assert got.view(np.uint32) == want[i].view(np.uint32), (name, d, i, got, want[i])
+
+
+@pytest.mark.parametrize("name,mid", NAMES)
+def test_drop_in_batch_symbols_bit_exact(hann, distance_golden, name, mid):
+    from hann_amd import _lib
+    lib = _lib.load()
+    fn = {0: lib.simd_squared_euclidean_batch, 1: lib.simd_euclidean_batch, 2: lib.simd_manhattan_batch,
+          3: lib.simd_cosine_distance_batch}[mid]
+    for si, (dim, n) in enumerate(distance_golden["batch_shapes"]):
+        q = distance_golden[f"batch{si}_q"]
+        flat = distance_golden[f"batch{si}_flat"]
+        out = np.empty(int(n), np.float64)
+        fn(_lib.fptr(q), _lib.fptr(flat), int(dim), int(n), _lib.dptr(out))
+        want = distance_golden[f"batch{si}_{name}_avx2"]
+        assert np.array_equal(out.view(np.uint64), want.view(np.uint64)), (name, dim, n)
+
+
+def test_metric_api_semantics(hann):
+    """core/metric_batch_test.go:76-100 and core/distance_test.go:125-153."""
+    from hann_amd import core
+    with pytest.raises(hann.HannError, match="same length"):
+        core.Euclidean.Distance([1], [1, 2])
+    assert core.Cosine.Distance([], []) == 0.0
+    with pytest.raises(hann.HannError, match="query must not be empty"):
+        core.Euclidean.DistanceBatch([], [1, 2], np.zeros(1))
+    with pytest.raises(hann.HannError, match="not a multiple"):
+        core.Euclidean.RankBatch([1, 2], [1, 2, 3], np.zeros(1))
+    with pytest.raises(hann.HannError, match="does not match the candidate count"):
+        core.Euclidean.DistanceBatch([1, 2], [1, 2, 3, 4], np.zeros(3))
+    core.Euclidean.DistanceBatch([1, 2], [], np.zeros(0))
+    assert core.Euclidean.Rank([0, 0], [3, 4]) == 25.0
+    assert core.Euclidean.FromRank(25.0) == 5.0
+    assert core.Cosine.Distance([0] * 8, list(range(1, 9))) == 1.0
+
+
+@pytest.mark.parametrize("dim", [1, 7, 8, 24, 31, 32, 33, 64, 96, 128, 200])
+@pytest.mark.parametrize("mid", [0, 1, 2, 3])
+def test_distance_matrix_matches_oracle(hann, oracle, dim, mid):
+    from hann_amd import core
+    rng = np.random.default_rng(dim * 10 + mid)
+    q = (rng.random((5, dim)) * 20 - 10).astype(np.float32)
+    base = (rng.random((300, dim)) * 20 - 10).astype(np.float32)
+    m = {0: core.SquaredEuclidean, 1: core.Euclidean, 2: core.Manhattan, 3: core.Cosine}[mid]
+    got = m.distance_matrix(q, base)
+    for i in range(5):
+        want = oracle.batch(mid, q[i], base, dim)
+        assert np.array_equal(got[i].view(np.uint64), want.view(np.uint64))
+
+
+@pytest.mark.parametrize("nq,nb,dim,k", [(1, 1, 8, 1), (3, 50, 8, 10), (37, 5000, 128, 100),
+                                          (17, 3000, 96, 64), (5, 2000, 100, 33), (20, 70, 16, 100),
+                                          (64, 20000, 64, 10), (2, 4096, 784, 5)])
+def test_batch_topk_exact_matches_oracle(hann, oracle, nq, nb, dim, k):
+    from hann_amd import core
+    rng = np.random.default_rng(nq * 1000 + nb)
+    q = (rng.random((nq, dim)) * 20 - 10).astype(np.float32)
+    base = (rng.random((nb, dim)) * 20 - 10).astype(np.float32)
+    ids, dist = core.batch_topk(core.Euclidean, q, base, k)
+    wid, wrank = oracle.batch_topk(0, q, base, k, nthreads=8)
+    assert np.array_equal(ids, wid)
+    kk = min(k, nb)
+    assert np.array_equal(dist[:, :kk], np.sqrt(wrank[:, :kk].astype(np.float64)))
+    if kk < k:
+        assert np.all(ids[:, kk:] == -1) and np.all(np.isinf(dist[:, kk:]))
+
+
+def test_batch_topk_integer_ties_ordered_by_row(hann, oracle):
+    """Config-2 style integer data: huge tie groups, ordered by (rank, row)."""
+    from hann_amd import core
+    rng = np.random.default_rng(4)
+    lam = rng.uniform(10, 30, 32)
+    base = np.minimum(rng.poisson(lam, (20000, 32)), 255).astype(np.float32)
+    q = np.minimum(rng.poisson(lam, (40, 32)), 255).astype(np.float32)
+    ids, dist = core.batch_topk(core.SquaredEuclidean, q, base, 100)
+    wid, wrank = oracle.batch_topk(0, q, base, 100, nthreads=8)
+    assert np.array_equal(ids, wid)
+    assert np.array_equal(dist, wrank.astype(np.float64))
+
+
+@pytest.mark.parametrize("mid", [2, 3])
+def test_batch_topk_other_metrics(hann, oracle, mid):
+    from hann_amd import core
+    rng = np.random.default_rng(mid)
+    q = (rng.random((9, 40)) * 20 - 10).astype(np.float32)
+    base = (rng.random((3000, 40)) * 20 - 10).astype(np.float32)
+    m = core.Manhattan if mid == 2 else core.Cosine
+    ids, dist = core.batch_topk(m, q, base, 20)
+    wid, wrank = oracle.batch_topk(mid, q, base, 20, nthreads=8)
+    assert np.array_equal(ids, wid)
+    assert np.array_equal(dist, wrank.astype(np.float64))
