@@ -51,6 +51,7 @@ class PQIVFOptions(ctypes.Structure):
         ("coarse_k", i64), ("num_subquantizers", i64), ("pq_k", i64), ("kmeans_iters", i64),
         ("candidate_clusters", i64), ("bruteforce_fallback", i32), ("explicit_mask", ctypes.c_uint32),
         ("seed", u64), ("train_precision", i32), ("kmeans_init", i32), ("train_sample", i64),
+        ("pq_train_sample", i64),
     ]
 
 
@@ -106,6 +107,9 @@ SIGNATURES = {
     "hann_pqivf_save": (cint, [vp, ctypes.POINTER(vp), c_size_p]),
     "hann_pqivf_load": (cint, [vp, vp, sz]),
     "hann_free_buffer": (None, [vp]),
+    "hann_pqivf_profile": (cint, [vp, cint]),
+    "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),
+    "hann_pqivf_export_lists": (cint, [vp, c_int64_p, c_int64_p, c_int32_p, c_float_p]),
     "hann_pqivf_is_trained": (cint, [vp]),
     "hann_pqivf_shape": (cint, [vp, c_int64_p, c_int64_p, c_int64_p, c_int64_p]),
     "hann_pqivf_coarse_centroids": (cint, [vp, c_float_p]),

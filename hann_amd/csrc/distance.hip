@@ -272,8 +272,8 @@ static void topk_launch(int metric, const float *queries, int64_t nq, const floa
     if (rows_per_split < 64) rows_per_split = 64;
     nsplit = ceil_div(nb, rows_per_split);
     if (nsplit < 1) nsplit = 1;
-    float *pkey = ws.get<float>(0, (size_t)(nq * nsplit * k));
-    int64_t *pid = ws.get<int64_t>(1, (size_t)(nq * nsplit * k));
+    float *pkey = ws.get<float>(14, (size_t)(nq * nsplit * k));
+    int64_t *pid = ws.get<int64_t>(15, (size_t)(nq * nsplit * k));
     for (int64_t qt0 = 0; qt0 < qtiles; qt0 += 65535) {
         const int64_t qtn = qtiles - qt0 < 65535 ? qtiles - qt0 : 65535;
         const int64_t qb = qt0 * qt;

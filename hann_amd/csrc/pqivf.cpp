@@ -1,0 +1,1217 @@
+// pqivf.cpp -- the PQIVF index behind the C-ABI: host bookkeeping with the
+// reference's semantics (pqivf/index.go), GPU training and GPU search.
+//
+// Host side (authoritative state, same rules as the reference):
+//   lists[c]      entries of coarse list c in storage order (ids, vectors,
+//                 codes); `exists` mirrors the presence of the map key
+//                 invertedLists[c] (pqivf/index.go:43)
+//   id_to_cluster id -> list (idToCluster)
+//   pending       vectors added before training (pendingVectors)
+// Device side (rebuilt from the host state when it changes):
+//   list_start/list_len per list, ids, PQ codes (uint8, or uint16 when a
+//   codebook has more than 256 rows), a has-codes flag, the raw vectors,
+//   the coarse centroids and the codebooks.
+#include "pqivf.h"
+
+#include <algorithm>
+#include <memory>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "batch.h"
+#include "ivf.h"
+#include "kmeans.h"
+#include "rng.h"
+
+namespace hann {
+
+// ------------------------------------------------------------ k-means --
+int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
+                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st) {
+    const int64_t keff = n < k ? n : k;
+    if (keff <= 0) return 0;
+    // seeds: the first keff entries of a random permutation (pqivf/index.go:804-810)
+    std::vector<int64_t> init((size_t)groups * keff);
+    std::vector<int64_t> perm;
+    for (int g = 0; g < groups; ++g) {
+        rng_perm(states[g], n, perm);
+        std::copy(perm.begin(), perm.begin() + keff, init.begin() + (size_t)g * keff);
+    }
+    DevBuf<int64_t> dinit(init.size());
+    HANN_HIP(hipMemcpyAsync(dinit.get(), init.data(), init.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    launch_gather_group(data, row_stride, dsub, (int)keff, groups, dinit.get(), cents, st);
+    sc.reserve(n, (int)keff, groups);
+    DevBuf<int32_t> assign((size_t)groups * n);
+    std::vector<int64_t> offs((size_t)groups * (keff + 1));
+    std::vector<int64_t> reseed((size_t)groups * keff);
+    for (int it = 0; it < iters; ++it) {
+        if (dsub < 32) {
+            launch_assign_groups(data, n, row_stride, cents, keff * dsub, nullptr, (int)keff, dsub, groups,
+                                 assign.get(), n, st);
+        } else {
+            for (int g = 0; g < groups; ++g)
+                launch_assign(data + (int64_t)g * dsub, n, row_stride, cents + (int64_t)g * keff * dsub, (int)keff,
+                              dsub, assign.get() + (int64_t)g * n, st);
+        }
+        launch_group_members(assign.get(), n, n, (int)keff, groups, sc, st);
+        HANN_HIP(hipMemcpyAsync(offs.data(), sc.offs.get(), offs.size() * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+        HANN_HIP(hipStreamSynchronize(st));
+        // empty clusters draw a replacement point, in ascending cluster order
+        // (pqivf/index.go:829-837)
+        for (int g = 0; g < groups; ++g)
+            for (int64_t c = 0; c < keff; ++c) {
+                const int64_t *o = offs.data() + (size_t)g * (keff + 1);
+                reseed[(size_t)g * keff + c] = o[c + 1] == o[c] ? (int64_t)rng_bounded(states[g], (uint64_t)n) : -1;
+            }
+        HANN_HIP(hipMemcpyAsync(sc.reseed.get(), reseed.data(), reseed.size() * sizeof(int64_t),
+                                hipMemcpyHostToDevice, st));
+        launch_kmeans_sums(data, n, row_stride, dsub, (int)keff, groups, cents, keff * dsub, sc, st);
+    }
+    HANN_HIP(hipStreamSynchronize(st));
+    return keff;
+}
+
+// -------------------------------------------------------------- index --
+static int64_t default_num_subquantizers(int64_t dim) {
+    // pqivf/index.go:150-157
+    for (int64_t n : {8, 4, 2})
+        if (dim % n == 0) return n;
+    return 1;
+}
+
+PQIVF::PQIVF() = default;
+
+PQIVF::~PQIVF() {
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void PQIVF::ensure_stream() {
+    if (!stream) {
+        ensure_device();
+        HANN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        d_fb_counter.resize(1);
+        HANN_HIP(hipMemsetAsync(d_fb_counter.get(), 0, sizeof(unsigned long long), stream));
+    }
+}
+
+void PQIVF::validate_vector(size_t len) const {
+    // pqivf/index.go:388-396
+    if (dim == 0) fail(HANN_ERR_INVALID_ARG, "cannot add to a zero-dimension index");
+    if ((int64_t)len != dim)
+        fail(HANN_ERR_INVALID_ARG, strf("vector dimension %zu does not match index dimension %lld", len, (long long)dim));
+}
+
+bool PQIVF::codes_valid(const int32_t *codes) const {
+    if (!have_codebooks) return false;
+    for (int64_t i = 0; i < m; ++i)
+        if (codes[i] < 0 || codes[i] >= cb_len[i]) return false;
+    return true;
+}
+
+int64_t PQIVF::nearest_centroid(const float *vec) {
+    // nearestCentroid (pqivf/index.go:162-179) on the device
+    if (kc == 0) fail(HANN_ERR_INVALID_ARG, "no coarse centroids available");
+    ensure_stream();
+    sync_device();
+    scratch_f.resize(dim);
+    HANN_HIP(hipMemcpyAsync(scratch_f.get(), vec, dim * sizeof(float), hipMemcpyHostToDevice, stream));
+    DevBuf<int32_t> out(1);
+    launch_assign(scratch_f.get(), 1, dim, d_coarse.get(), (int)kc, (int)dim, out.get(), stream);
+    int32_t c = 0;
+    HANN_HIP(hipMemcpyAsync(&c, out.get(), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipStreamSynchronize(stream));
+    return c;
+}
+
+void PQIVF::encode_rows(const float *vecs, int64_t n, const int32_t *clusters, int32_t *codes_out) {
+    // encodeWithCodebooks (pqivf/index.go:641-666) for n rows on the device
+    if (n == 0) return;
+    ensure_stream();
+    sync_device();
+    DevBuf<float> dv((size_t)n * dim), resid((size_t)n * dim);
+    DevBuf<int32_t> dc((size_t)n), dcodes((size_t)n * m);
+    HANN_HIP(hipMemcpyAsync(dv.get(), vecs, (size_t)n * dim * sizeof(float), hipMemcpyHostToDevice, stream));
+    HANN_HIP(hipMemcpyAsync(dc.get(), clusters, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    launch_encode(dv.get(), n, (int)dim, d_coarse.get(), dc.get(), (int)m, d_cb.get(), (int)pmax, d_cb_len.get(),
+                  dcodes.get(), resid.get(), stream);
+    HANN_HIP(hipMemcpyAsync(codes_out, dcodes.get(), (size_t)n * m * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipStreamSynchronize(stream));
+}
+
+void PQIVF::assign_locked(const std::vector<int64_t> &ids, const std::vector<const float *> &vecs) {
+    // assignLocked (pqivf/index.go:245-260) for a run of new entries, batched:
+    // nearest centroid and codes on the GPU, appended in the given order.
+    const int64_t n = (int64_t)ids.size();
+    if (n == 0) return;
+    if (kc == 0) fail(HANN_ERR_INVALID_ARG, "no coarse centroids available");
+    ensure_stream();
+    sync_device();
+    std::vector<float> flat((size_t)n * dim);
+    for (int64_t i = 0; i < n; ++i) std::memcpy(&flat[(size_t)i * dim], vecs[i], dim * sizeof(float));
+    DevBuf<float> dv(flat.size());
+    DevBuf<int32_t> dc((size_t)n);
+    HANN_HIP(hipMemcpyAsync(dv.get(), flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    launch_assign(dv.get(), n, dim, d_coarse.get(), (int)kc, (int)dim, dc.get(), stream);
+    std::vector<int32_t> cl((size_t)n);
+    std::vector<int32_t> codes;
+    bool can_encode = have_codebooks;
+    for (int64_t i = 0; i < m && can_encode; ++i)
+        if (cb_len[i] == 0) can_encode = false;   // "failed to encode sub-vector": stored without codes
+    if (can_encode) {
+        DevBuf<float> resid(flat.size());
+        DevBuf<int32_t> dcodes((size_t)n * m);
+        launch_encode(dv.get(), n, (int)dim, d_coarse.get(), dc.get(), (int)m, d_cb.get(), (int)pmax,
+                      d_cb_len.get(), dcodes.get(), resid.get(), stream);
+        codes.resize((size_t)n * m);
+        HANN_HIP(hipMemcpyAsync(codes.data(), dcodes.get(), codes.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                stream));
+    }
+    HANN_HIP(hipMemcpyAsync(cl.data(), dc.get(), (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipStreamSynchronize(stream));
+    for (int64_t i = 0; i < n; ++i) {
+        ListData &L = lists[cl[i]];
+        L.exists = true;
+        L.ids.push_back(ids[i]);
+        L.vecs.insert(L.vecs.end(), vecs[i], vecs[i] + dim);
+        if (can_encode) {
+            L.codes.insert(L.codes.end(), codes.begin() + (size_t)i * m, codes.begin() + (size_t)(i + 1) * m);
+            L.has_codes.push_back(1);
+        } else {
+            L.codes.insert(L.codes.end(), (size_t)m, -1);
+            L.has_codes.push_back(0);
+        }
+        id_to_cluster[ids[i]] = cl[i];
+    }
+    mark_dirty();
+}
+
+void PQIVF::add_locked(int64_t id, const float *vec, size_t len) {
+    // addLocked (pqivf/index.go:221-237)
+    validate_vector(len);
+    if (id_to_cluster.count(id)) fail(HANN_ERR_DUPLICATE_ID, strf("id %lld already exists in a cluster", (long long)id));
+    if (pending.count(id)) fail(HANN_ERR_DUPLICATE_ID, strf("id %lld already exists in pending vectors", (long long)id));
+    if (trained) {
+        assign_locked({id}, {vec});
+        return;
+    }
+    pending.put(id, vec);
+}
+
+void PQIVF::bulk_add(const int64_t *ids, const float *flat, const size_t *lens, size_t n) {
+    // BulkAdd (pqivf/index.go:266-297): ascending id order, not atomic. On a
+    // trained index the GPU assigns each run of valid vectors in one batch;
+    // the first invalid id stops the call after the run before it is in.
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + (lens ? lens[i] : (size_t)dim);
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ids[a] < ids[b]; });
+    std::vector<int64_t> run_ids;
+    std::vector<const float *> run_vecs;
+    std::unordered_map<int64_t, int> in_run;
+    auto flush = [&] {
+        if (!run_ids.empty()) assign_locked(run_ids, run_vecs);
+        run_ids.clear();
+        run_vecs.clear();
+        in_run.clear();
+    };
+    for (size_t oi = 0; oi < n; ++oi) {
+        const size_t i = order[oi];
+        const int64_t id = ids[i];
+        const size_t len = off[i + 1] - off[i];
+        const float *vec = flat + off[i];
+        if ((int64_t)len != dim) {
+            flush();
+            fail(HANN_ERR_INVALID_ARG, strf("vector dimension %zu does not match index dimension %lld for id %lld", len,
+                                            (long long)dim, (long long)id));
+        }
+        if (id_to_cluster.count(id) || in_run.count(id)) {
+            flush();
+            fail(HANN_ERR_DUPLICATE_ID, strf("id %lld already exists in a cluster", (long long)id));
+        }
+        if (pending.count(id)) {
+            flush();
+            fail(HANN_ERR_DUPLICATE_ID, strf("id %lld already exists in pending vectors", (long long)id));
+        }
+        if (trained) {
+            run_ids.push_back(id);
+            run_vecs.push_back(vec);
+            in_run[id] = 1;
+        } else {
+            pending.put(id, vec);
+        }
+    }
+    flush();
+}
+
+void PQIVF::remove_from_list(int64_t id, int64_t cluster) {
+    ListData &L = lists[cluster];
+    size_t w = 0;
+    const size_t len = L.ids.size();
+    for (size_t r = 0; r < len; ++r) {
+        if (L.ids[r] == id) continue;
+        if (w != r) {
+            L.ids[w] = L.ids[r];
+            std::memcpy(&L.vecs[w * dim], &L.vecs[r * dim], dim * sizeof(float));
+            std::memcpy(&L.codes[w * m], &L.codes[r * m], m * sizeof(int32_t));
+            L.has_codes[w] = L.has_codes[r];
+        }
+        ++w;
+    }
+    L.ids.resize(w);
+    L.vecs.resize(w * dim);
+    L.codes.resize(w * m);
+    L.has_codes.resize(w);
+}
+
+void PQIVF::delete_locked(int64_t id) {
+    // deleteLocked (pqivf/index.go:313-345): order-preserving removal
+    if (pending.erase(id)) return;
+    auto it = id_to_cluster.find(id);
+    if (it == id_to_cluster.end()) fail(HANN_ERR_NOT_FOUND, strf("id %lld not found", (long long)id));
+    const int64_t c = it->second;
+    if (c < 0 || c >= (int64_t)lists.size() || !lists[c].exists)
+        fail(HANN_ERR_NOT_FOUND, strf("inconsistent state: cluster %lld not found for id %lld", (long long)c, (long long)id));
+    const size_t before = lists[c].ids.size();
+    remove_from_list(id, c);
+    if (lists[c].ids.size() == before)
+        fail(HANN_ERR_NOT_FOUND, strf("id %lld not found in cluster %lld", (long long)id, (long long)c));
+    id_to_cluster.erase(it);
+    mark_dirty();
+}
+
+void PQIVF::bulk_delete(const int64_t *ids, size_t n) {
+    // BulkDelete (pqivf/index.go:350-384): unknown ids are skipped
+    std::vector<int64_t> sorted(ids, ids + n);
+    std::sort(sorted.begin(), sorted.end());
+    for (int64_t id : sorted) {
+        if (pending.erase(id)) continue;
+        auto it = id_to_cluster.find(id);
+        if (it == id_to_cluster.end()) continue;
+        const int64_t c = it->second;
+        if (c < 0 || c >= (int64_t)lists.size() || !lists[c].exists) continue;
+        remove_from_list(id, c);
+        id_to_cluster.erase(it);
+        mark_dirty();
+    }
+}
+
+void PQIVF::update(int64_t id, const float *vec, size_t len) {
+    // Update (pqivf/index.go:402-413)
+    validate_vector(len);
+    delete_locked(id);
+    add_locked(id, vec, len);
+}
+
+void PQIVF::bulk_update(const int64_t *ids, const float *flat, const size_t *lens, size_t n) {
+    // BulkUpdate (pqivf/index.go:419-444): validate all, then delete + add in
+    // ascending id order
+    std::vector<size_t> off(n + 1, 0);
+    for (size_t i = 0; i < n; ++i) off[i + 1] = off[i] + (lens ? lens[i] : (size_t)dim);
+    std::vector<size_t> order(n);
+    std::iota(order.begin(), order.end(), 0);
+    std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return ids[a] < ids[b]; });
+    for (size_t i : order) {
+        const size_t len = off[i + 1] - off[i];
+        try {
+            validate_vector(len);
+        } catch (const Error &e) {
+            fail(e.code, strf("invalid vector for id %lld: %s", (long long)ids[i], e.what()));
+        }
+    }
+    for (size_t i : order) {
+        delete_locked(ids[i]);
+        add_locked(ids[i], flat + off[i], off[i + 1] - off[i]);
+    }
+}
+
+void PQIVF::train() {
+    // Train (pqivf/index.go:451-605)
+    std::vector<int64_t> all_ids;
+    std::vector<const float *> all_vecs;
+    for (const ListData &L : lists)
+        for (size_t r = 0; r < L.ids.size(); ++r) {
+            all_ids.push_back(L.ids[r]);
+            all_vecs.push_back(&L.vecs[r * dim]);
+        }
+    for (size_t i = 0; i < pending.size(); ++i) {
+        all_ids.push_back(pending.ids[i]);
+        all_vecs.push_back(&pending.vecs[i * dim]);
+    }
+    const int64_t n = (int64_t)all_ids.size();
+    if (n < coarse_k)
+        fail(HANN_ERR_EMPTY, strf("not enough vectors (%lld) to train coarse quantizer with %lld clusters", (long long)n,
+                                  (long long)coarse_k));
+    std::vector<size_t> order((size_t)n);
+    std::iota(order.begin(), order.end(), 0);
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return all_ids[a] < all_ids[b]; });
+    std::vector<int64_t> ids((size_t)n);
+    std::vector<float> X((size_t)n * dim);
+    for (int64_t i = 0; i < n; ++i) {
+        ids[i] = all_ids[order[i]];
+        std::memcpy(&X[(size_t)i * dim], all_vecs[order[i]], dim * sizeof(float));
+    }
+    all_vecs.clear();
+    ensure_stream();
+    const uint64_t epoch = train_epoch;
+    const int d = (int)dim;
+    DevBuf<float> dX(X.size());
+    HANN_HIP(hipMemcpyAsync(dX.get(), X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+
+    // coarse quantizer, optionally on a seeded sample (build extension)
+    KMeansScratch sc;
+    const float *train_src = dX.get();
+    int64_t ntrain = n;
+    DevBuf<float> dS;
+    if (coarse_sample > 0 && coarse_sample < n) {
+        uint64_t st = stream_seed(seed, epoch, kRoleCoarseSample);
+        std::vector<int64_t> perm;
+        rng_perm(st, n, perm);
+        std::vector<int64_t> pick(perm.begin(), perm.begin() + coarse_sample);
+        std::sort(pick.begin(), pick.end());
+        DevBuf<int64_t> dpick(pick.size());
+        HANN_HIP(hipMemcpyAsync(dpick.get(), pick.data(), pick.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+        dS.resize((size_t)coarse_sample * dim);
+        launch_gather_rows(dX.get(), d, dpick.get(), coarse_sample, dS.get(), stream);
+        HANN_HIP(hipStreamSynchronize(stream));
+        train_src = dS.get();
+        ntrain = coarse_sample;
+    }
+    DevBuf<float> dC((size_t)coarse_k * dim);
+    std::vector<uint64_t> st0{stream_seed(seed, epoch, 0)};
+    const int64_t new_kc = run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream);
+    dS.release();
+
+    // final assignment of every vector; lists in ascending id order
+    DevBuf<int32_t> dA((size_t)n);
+    launch_assign(dX.get(), n, dim, dC.get(), (int)new_kc, d, dA.get(), stream);
+    std::vector<int32_t> A((size_t)n);
+    HANN_HIP(hipMemcpyAsync(A.data(), dA.get(), A.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+
+    // PQ training rows: residuals in cluster-ascending, list order
+    // (pqivf/index.go:520-541) = the stable grouping of points by cluster
+    KMeansScratch sg;
+    sg.reserve(n, (int)new_kc, 1);
+    launch_group_members(dA.get(), n, n, (int)new_kc, 1, sg, stream);
+    int64_t npq = n;
+    const int32_t *pq_order = sg.members.get();
+    DevBuf<int32_t> dsub_order;
+    std::vector<int32_t> full, sub;   // host staging of the sample order, alive until the stream syncs
+    if (pq_sample > 0 && pq_sample < n) {
+        uint64_t st = stream_seed(seed, epoch, kRolePQSample);
+        std::vector<int64_t> perm;
+        rng_perm(st, n, perm);
+        std::vector<int64_t> pick(perm.begin(), perm.begin() + pq_sample);
+        std::sort(pick.begin(), pick.end());
+        full.resize((size_t)n);
+        sub.resize((size_t)pq_sample);
+        HANN_HIP(hipMemcpyAsync(full.data(), sg.members.get(), (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+        HANN_HIP(hipStreamSynchronize(stream));
+        for (int64_t i = 0; i < pq_sample; ++i) sub[i] = full[pick[i]];
+        dsub_order.resize(sub.size());
+        HANN_HIP(hipMemcpyAsync(dsub_order.get(), sub.data(), sub.size() * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+        pq_order = dsub_order.get();
+        npq = pq_sample;
+    }
+    DevBuf<float> dR((size_t)npq * dim);
+    launch_residuals(dX.get(), d, dC.get(), dA.get(), pq_order, npq, dR.get(), stream);
+    const int s = (int)(dim / m);
+    DevBuf<float> dCB((size_t)m * pq_k * s);
+    std::vector<uint64_t> sts((size_t)m);
+    for (int64_t i = 0; i < m; ++i) sts[i] = stream_seed(seed, epoch, 1 + (uint64_t)i);
+    KMeansScratch sp;
+    const int64_t pk = run_kmeans(dR.get(), npq, dim, s, (int)pq_k, (int)iters, (int)m, sts, dCB.get(), sp, stream);
+    dR.release();
+
+    // encode every vector (pqivf/index.go:554-601)
+    std::vector<int64_t> cbl((size_t)m, pk);
+    DevBuf<int64_t> dcbl((size_t)m);
+    HANN_HIP(hipMemcpyAsync(dcbl.get(), cbl.data(), cbl.size() * sizeof(int64_t), hipMemcpyHostToDevice, stream));
+    DevBuf<int32_t> dcodes((size_t)n * m);
+    {
+        DevBuf<float> resid(s >= 32 ? (size_t)n * dim : 1);
+        launch_encode(dX.get(), n, d, dC.get(), dA.get(), (int)m, dCB.get(), (int)pk, dcbl.get(), dcodes.get(),
+                      resid.get(), stream);
+    }
+    std::vector<int32_t> codes((size_t)n * m);
+    std::vector<float> C((size_t)new_kc * dim), CB((size_t)m * pk * s);
+    HANN_HIP(hipMemcpyAsync(codes.data(), dcodes.get(), codes.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipMemcpyAsync(C.data(), dC.get(), C.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipMemcpyAsync(CB.data(), dCB.get(), CB.size() * sizeof(float), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipStreamSynchronize(stream));
+
+    // commit the new state
+    train_epoch = epoch + 1;
+    kc = new_kc;
+    coarse.swap(C);
+    codebooks.swap(CB);
+    pmax = pk;
+    cb_len = cbl;
+    have_codebooks = true;
+    lists.assign((size_t)kc, ListData());
+    id_to_cluster.clear();
+    id_to_cluster.reserve((size_t)n);
+    std::vector<int64_t> cnt((size_t)kc, 0);
+    for (int64_t i = 0; i < n; ++i) cnt[A[i]]++;
+    for (int64_t c = 0; c < kc; ++c) {
+        lists[c].ids.reserve(cnt[c]);
+        lists[c].vecs.reserve(cnt[c] * dim);
+        lists[c].codes.reserve(cnt[c] * m);
+        lists[c].has_codes.reserve(cnt[c]);
+    }
+    for (int64_t i = 0; i < n; ++i) {
+        ListData &L = lists[A[i]];
+        L.exists = true;
+        L.ids.push_back(ids[i]);
+        L.vecs.insert(L.vecs.end(), &X[(size_t)i * dim], &X[(size_t)(i + 1) * dim]);
+        L.codes.insert(L.codes.end(), &codes[(size_t)i * m], &codes[(size_t)(i + 1) * m]);
+        L.has_codes.push_back(1);
+        id_to_cluster[ids[i]] = A[i];
+    }
+    pending.clear();
+    trained = true;
+    mark_dirty();
+}
+
+// ------------------------------------------------------- device mirror --
+void PQIVF::mark_dirty() { dirty = true; }
+
+void PQIVF::sync_device() {
+    ensure_stream();
+    if (!dirty) return;
+    const int s = m > 0 ? (int)(dim / m) : 0;
+    int64_t N = 0;
+    for (const ListData &L : lists) N += (int64_t)L.ids.size();
+    n_entries = N;
+    code_bytes = pmax > 256 ? 2 : 1;
+    std::vector<int64_t> start(lists.size()), len(lists.size());
+    std::vector<int64_t> hids((size_t)N);
+    std::vector<uint8_t> hcodes((size_t)N * m * code_bytes);
+    std::vector<uint8_t> hhas((size_t)N);
+    std::vector<float> hv((size_t)N * dim);
+    int64_t pos = 0;
+    any_uncoded = false;
+    for (size_t c = 0; c < lists.size(); ++c) {
+        const ListData &L = lists[c];
+        start[c] = pos;
+        len[c] = (int64_t)L.ids.size();
+        for (size_t r = 0; r < L.ids.size(); ++r, ++pos) {
+            hids[pos] = L.ids[r];
+            std::memcpy(&hv[(size_t)pos * dim], &L.vecs[r * dim], dim * sizeof(float));
+            const int32_t *cr = &L.codes[r * m];
+            const bool ok = L.has_codes[r] && codes_valid(cr);
+            hhas[pos] = ok ? 1 : 0;
+            if (!ok) any_uncoded = true;
+            for (int64_t i = 0; i < m; ++i) {
+                const int32_t v = ok ? cr[i] : 0;
+                if (code_bytes == 1)
+                    hcodes[(size_t)pos * m + i] = (uint8_t)v;
+                else
+                    reinterpret_cast<uint16_t *>(hcodes.data())[(size_t)pos * m + i] = (uint16_t)v;
+            }
+        }
+    }
+    auto up = [&](auto &buf, const auto &host) {
+        using T = typename std::decay_t<decltype(host)>::value_type;
+        buf.resize(host.size() + 1);
+        if (!host.empty())
+            HANN_HIP(hipMemcpyAsync(buf.get(), host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice, stream));
+    };
+    up(d_list_start, start);
+    up(d_list_len, len);
+    up(d_ids, hids);
+    up(d_codes, hcodes);
+    up(d_has_codes, hhas);
+    up(d_vecs, hv);
+    up(d_coarse, coarse);
+    up(d_cb, codebooks);
+    up(d_cb_len, cb_len);
+    (void)s;
+    HANN_HIP(hipStreamSynchronize(stream));
+    dirty = false;
+}
+
+// --------------------------------------------------------------- search --
+void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,
+                          const hann_search_params &p, float *out_key, int64_t *out_id, int32_t *out_cnt,
+                          hipStream_t st) {
+    // the caller holds the index lock and has synced the device mirror
+    if (nq == 0) return;
+    const int d = (int)dim;
+    const int K = (int)lists.size();
+    int np = pmax_probe < K ? pmax_probe : K;
+    if (np < 1) np = 1;
+    const int R = p.rerank > 0 ? p.rerank : 0;
+    // coarse step: the np nearest centroids by (rank, index); nearestCentroids
+    // (pqivf/index.go:184-206). Probing every list needs no selection.
+    std::array<hipEvent_t, 4> ev{};
+    if (profile) {
+        for (auto &e : ev) HANN_HIP(hipEventCreate(&e));
+        HANN_HIP(hipEventRecord(ev[0], st));
+    }
+    int64_t *probes;
+    int probe_stride;
+    if (np >= K) {
+        if ((int64_t)all_lists.n < (int64_t)K) {
+            std::vector<int64_t> h((size_t)K);
+            std::iota(h.begin(), h.end(), 0);
+            all_lists.resize((size_t)K);
+            HANN_HIP(hipMemcpyAsync(all_lists.get(), h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        }
+        probes = all_lists.get();
+        probe_stride = 0;
+    } else {
+        if (np > 512) fail(HANN_ERR_UNSUPPORTED, strf("nprobe %d above 512 (and below coarseK %d) is not supported", np, K));
+        probes = ws.get<int64_t>(5, (size_t)nq * np);
+        double *pd = ws.get<double>(6, (size_t)nq * np);
+        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, pd, ws, st);
+        probe_stride = np;
+    }
+    const int selk = R > 0 ? kmax * R : kmax;
+    const int cap = 2 * kp_for(selk);
+    ScanArgs a{};
+    a.queries = dq;
+    a.d = d;
+    a.m = (int)m;
+    a.s = (int)(dim / m);
+    a.pmax_cb = (int)(pmax > 0 ? pmax : 1);
+    a.K = K;
+    a.probes = probes;
+    a.probe_stride = probe_stride;
+    a.probe_cap = np;
+    a.nprobe_q = dnp;
+    a.nprobe_default = (int)nprobe;
+    a.k_q = dk;
+    a.k_default = kmax;
+    a.rerank = R;
+    a.allow_fallback = fallback ? 1 : 0;
+    a.cap = cap;
+    a.list_start = d_list_start.get();
+    a.list_len = d_list_len.get();
+    a.ids = d_ids.get();
+    a.codes = d_codes.get();
+    a.has_codes = d_has_codes.get();
+    a.any_uncoded = any_uncoded ? 1 : 0;
+    a.vecs = d_vecs.get();
+    a.coarse = d_coarse.get();
+    a.cb = d_cb.get();
+    a.cb_len = d_cb_len.get();
+    a.fb_counter = d_fb_counter.get();
+    a.fell_back = nullptr;
+    if (profile) HANN_HIP(hipEventRecord(ev[1], st));
+    int mode = p.scan_mode;
+    if (!have_codebooks) {
+        // no codebooks: every entry ranks by its exact vector
+        a.any_uncoded = 1;
+        mode = SCAN_MODE_ADC_F32;
+    }
+    if (R == 0) {
+        a.out_key = out_key;
+        a.out_id = out_id;
+        a.out_cnt = out_cnt;
+        a.out_stride = kmax;
+        launch_scan(mode, code_bytes, a, nq, st);
+        if (profile) {
+            HANN_HIP(hipEventRecord(ev[2], st));
+            HANN_HIP(hipEventRecord(ev[3], st));
+        }
+    } else {
+        float *ck = ws.get<float>(7, (size_t)nq * selk);
+        int64_t *cpos = ws.get<int64_t>(8, (size_t)nq * selk);
+        int32_t *ccnt = ws.get<int32_t>(9, (size_t)nq);
+        a.out_key = ck;
+        a.out_id = cpos;
+        a.out_cnt = ccnt;
+        a.out_stride = selk;
+        launch_scan(mode, code_bytes, a, nq, st);
+        if (profile) HANN_HIP(hipEventRecord(ev[2], st));
+        launch_rerank(dq, d, nq, cpos, ccnt, selk, dk, kmax, kmax, d_vecs.get(), d_ids.get(), out_key, out_id, out_cnt,
+                      kmax, st);
+        if (profile) HANN_HIP(hipEventRecord(ev[3], st));
+    }
+    if (profile) t_events.push_back(ev);
+}
+
+void PQIVF::resolve_timers() {
+    for (auto &ev : t_events) {
+        HANN_HIP(hipEventSynchronize(ev[3]));
+        float a = 0, b = 0, c = 0;
+        HANN_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
+        HANN_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
+        HANN_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+        t_coarse_ms += a;
+        t_scan_ms += b;
+        t_rerank_ms += c;
+        ++t_calls;
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    }
+    t_events.clear();
+}
+
+void PQIVF::check_searchable(int64_t k, size_t len) const {
+    // Search's validation order (pqivf/index.go:854-872)
+    if (k <= 0) fail(HANN_ERR_INVALID_ARG, "k must be positive");
+    if (!trained)
+        fail(HANN_ERR_NOT_TRAINED, "PQIVF index is not trained. Call Train() after adding data before searching");
+    if ((int64_t)len != dim)
+        fail(HANN_ERR_INVALID_ARG,
+             strf("query dimension %zu does not match index dimension %lld", len, (long long)dim));
+    bool any = false;
+    for (const ListData &L : lists) any = any || L.exists;
+    if (!any) fail(HANN_ERR_EMPTY, "index is empty");
+}
+
+void PQIVF::search_host(const float *queries, int64_t nq, const int32_t *k, const int32_t *np,
+                        const hann_search_params &p, int64_t *out_ids, double *out_dist, int32_t *out_counts) {
+    int kmax = 0, pmx = (int)nprobe;
+    for (int64_t q = 0; q < nq; ++q) {
+        kmax = std::max(kmax, k ? k[q] : 0);
+        if (np) pmx = std::max(pmx, np[q]);
+    }
+    if (np) {
+        pmx = 0;
+        for (int64_t q = 0; q < nq; ++q) pmx = std::max(pmx, np[q]);
+    }
+    for (int64_t q = 0; q < nq; ++q) check_searchable(k ? k[q] : 0, (size_t)dim);
+    if (nq == 0) return;
+    sync_device();
+    hipStream_t st = stream;
+    float *dq = ws.get<float>(0, (size_t)nq * dim);
+    int32_t *dk = ws.get<int32_t>(1, (size_t)nq);
+    int32_t *dn = np ? ws.get<int32_t>(2, (size_t)nq) : nullptr;
+    float *ok = ws.get<float>(3, (size_t)nq * kmax);
+    int64_t *oi = ws.get<int64_t>(4, (size_t)nq * kmax);
+    int32_t *oc = ws.get<int32_t>(10, (size_t)nq);
+    HANN_HIP(hipMemcpyAsync(dq, queries, (size_t)nq * dim * sizeof(float), hipMemcpyHostToDevice, st));
+    HANN_HIP(hipMemcpyAsync(dk, k, (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    if (np) HANN_HIP(hipMemcpyAsync(dn, np, (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice, st));
+    search_device(dq, nq, dk, dn, kmax, pmx, p, ok, oi, oc, st);
+    std::vector<float> hk((size_t)nq * kmax);
+    HANN_HIP(hipMemcpyAsync(hk.data(), ok, hk.size() * sizeof(float), hipMemcpyDeviceToHost, st));
+    HANN_HIP(hipMemcpyAsync(out_ids, oi, (size_t)nq * kmax * sizeof(int64_t), hipMemcpyDeviceToHost, st));
+    HANN_HIP(hipMemcpyAsync(out_counts, oc, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HANN_HIP(hipStreamSynchronize(st));
+    // FromRank: float64 square root of the widened float32 rank
+    // (pqivf/index.go:945-947, core/metric.go:156)
+    for (size_t i = 0; i < hk.size(); ++i) out_dist[i] = std::sqrt((double)hk[i]);
+}
+
+int64_t PQIVF::fallback_count() {
+    int64_t devc = 0;
+    if (stream && d_fb_counter.get()) {
+        unsigned long long v = 0;
+        HANN_HIP(hipMemcpyAsync(&v, d_fb_counter.get(), sizeof(v), hipMemcpyDeviceToHost, stream));
+        HANN_HIP(hipStreamSynchronize(stream));
+        devc = (int64_t)v;
+    }
+    return devc;
+}
+
+int64_t PQIVF::count() const {
+    int64_t c = (int64_t)pending.size();
+    for (const ListData &L : lists) c += (int64_t)L.ids.size();
+    return c;
+}
+
+// ------------------------------------------------------------ save/load --
+namespace {
+constexpr char kMagic[8] = {'H', 'A', 'N', 'N', 'P', 'Q', 'I', 'V'};
+constexpr uint32_t kFormatVersion = 1;
+
+struct Writer {
+    std::vector<unsigned char> b;
+    template <typename T>
+    void put(const T &v) {
+        const unsigned char *p = reinterpret_cast<const unsigned char *>(&v);
+        b.insert(b.end(), p, p + sizeof(T));
+    }
+    void raw(const void *p, size_t n) {
+        const unsigned char *c = reinterpret_cast<const unsigned char *>(p);
+        b.insert(b.end(), c, c + n);
+    }
+};
+
+struct Reader {
+    const unsigned char *p;
+    size_t n, at = 0;
+    template <typename T>
+    T get() {
+        if (at + sizeof(T) > n) fail(HANN_ERR_FORMAT, "serialized index is truncated");
+        T v;
+        std::memcpy(&v, p + at, sizeof(T));
+        at += sizeof(T);
+        return v;
+    }
+    void raw(void *dst, size_t bytes) {
+        if (at + bytes > n || bytes > n) fail(HANN_ERR_FORMAT, "serialized index is truncated");
+        std::memcpy(dst, p + at, bytes);
+        at += bytes;
+    }
+    int64_t count(size_t elem) {
+        int64_t c = get<int64_t>();
+        if (c < 0 || (elem && (uint64_t)c > (n - at) / elem + 1)) fail(HANN_ERR_FORMAT, "serialized index is corrupt");
+        return c;
+    }
+};
+}  // namespace
+
+std::vector<unsigned char> PQIVF::save() const {
+    // serializedPQIVF (pqivf/index.go:969-984) in a binary layout
+    Writer w;
+    w.raw(kMagic, 8);
+    w.put<uint32_t>(kFormatVersion);
+    w.put<int64_t>(dim);
+    w.put<int64_t>(coarse_k);
+    w.put<int64_t>(m);
+    w.put<int64_t>(pq_k);
+    w.put<int64_t>(iters);
+    w.put<int64_t>(nprobe);
+    w.put<uint8_t>(fallback ? 1 : 0);
+    w.put<uint8_t>(trained ? 1 : 0);
+    w.put<uint64_t>(seed);
+    w.put<uint64_t>(train_epoch);
+    w.put<int64_t>(kc);
+    w.raw(coarse.data(), coarse.size() * sizeof(float));
+    w.put<uint8_t>(have_codebooks ? 1 : 0);
+    w.put<int64_t>(pmax);
+    for (int64_t i = 0; i < m; ++i) w.put<int64_t>(i < (int64_t)cb_len.size() ? cb_len[i] : 0);
+    w.raw(codebooks.data(), codebooks.size() * sizeof(float));
+    int64_t nl = 0;
+    for (const ListData &L : lists) nl += L.exists ? 1 : 0;
+    w.put<int64_t>(nl);
+    for (size_t c = 0; c < lists.size(); ++c) {
+        const ListData &L = lists[c];
+        if (!L.exists) continue;
+        w.put<int64_t>((int64_t)c);
+        w.put<int64_t>((int64_t)L.ids.size());
+        w.raw(L.ids.data(), L.ids.size() * sizeof(int64_t));
+        w.raw(L.has_codes.data(), L.has_codes.size());
+        w.raw(L.codes.data(), L.codes.size() * sizeof(int32_t));
+        w.raw(L.vecs.data(), L.vecs.size() * sizeof(float));
+    }
+    w.put<int64_t>((int64_t)pending.size());
+    for (size_t i = 0; i < pending.size(); ++i) {
+        w.put<int64_t>(pending.ids[i]);
+        w.put<int64_t>(dim);
+        w.raw(&pending.vecs[i * dim], dim * sizeof(float));
+    }
+    return std::move(w.b);
+}
+
+void PQIVF::load(const void *buf, size_t len) {
+    // GobDecode (pqivf/index.go:1020-1084): version and parameter checks,
+    // then the new state replaces the old one
+    Reader r{reinterpret_cast<const unsigned char *>(buf), len};
+    char magic[8];
+    r.raw(magic, 8);
+    if (std::memcmp(magic, kMagic, 8) != 0) fail(HANN_ERR_FORMAT, "not a hann PQIVF index file");
+    const uint32_t ver = r.get<uint32_t>();
+    if (ver > kFormatVersion)
+        fail(HANN_ERR_FORMAT, strf("index file has format version %u, but this build supports up to %u", ver, kFormatVersion));
+    const int64_t ddim = r.get<int64_t>(), dck = r.get<int64_t>(), dm = r.get<int64_t>(), dpk = r.get<int64_t>();
+    const int64_t dit = r.get<int64_t>(), dnp = r.get<int64_t>();
+    const bool dfb = r.get<uint8_t>() != 0, dtr = r.get<uint8_t>() != 0;
+    const uint64_t dseed = r.get<uint64_t>(), depoch = r.get<uint64_t>();
+    if (ddim <= 0) fail(HANN_ERR_FORMAT, strf("serialized index has invalid dimension %lld", (long long)ddim));
+    if (dm <= 0 || ddim % dm != 0)
+        fail(HANN_ERR_FORMAT, strf("serialized index has invalid subquantizer count %lld for dimension %lld", (long long)dm,
+                                   (long long)ddim));
+    if (dck <= 0 || dpk <= 0 || dit <= 0)
+        fail(HANN_ERR_FORMAT, strf("serialized index has invalid training parameters: coarseK %lld, pqK %lld, kMeansIters %lld",
+                                   (long long)dck, (long long)dpk, (long long)dit));
+    const int64_t dkc = r.count((size_t)ddim * 4);
+    std::vector<float> C((size_t)(dkc * ddim));
+    r.raw(C.data(), C.size() * sizeof(float));
+    const bool dhave = r.get<uint8_t>() != 0;
+    const int64_t dpmax = r.count(0);
+    std::vector<int64_t> cbl((size_t)dm);
+    r.raw(cbl.data(), cbl.size() * sizeof(int64_t));
+    const int64_t s = ddim / dm;
+    if (dpmax > (int64_t)((len - r.at) / 4 / std::max<int64_t>(1, dm * s) + 1)) fail(HANN_ERR_FORMAT, "serialized index is corrupt");
+    std::vector<float> CB((size_t)(dm * dpmax * s));
+    r.raw(CB.data(), CB.size() * sizeof(float));
+    for (int64_t i = 0; i < dm; ++i)
+        if (cbl[i] < 0 || cbl[i] > dpmax) fail(HANN_ERR_FORMAT, "serialized index has an invalid codebook length");
+    std::vector<ListData> L((size_t)dkc);
+    std::unordered_map<int64_t, int64_t> i2c;
+    const int64_t nl = r.count(16);
+    for (int64_t j = 0; j < nl; ++j) {
+        const int64_t c = r.get<int64_t>();
+        const int64_t n = r.count((size_t)(8 + 1 + 4 * dm + 4 * ddim));
+        if (c < 0 || c >= dkc) fail(HANN_ERR_FORMAT, strf("serialized list %lld has no coarse centroid", (long long)c));
+        ListData &x = L[c];
+        x.exists = true;
+        x.ids.resize(n);
+        x.has_codes.resize(n);
+        x.codes.resize(n * dm);
+        x.vecs.resize(n * ddim);
+        r.raw(x.ids.data(), n * 8);
+        r.raw(x.has_codes.data(), n);
+        r.raw(x.codes.data(), n * dm * 4);
+        r.raw(x.vecs.data(), n * ddim * 4);
+        for (int64_t id : x.ids) i2c[id] = c;
+    }
+    PendingStore pend;
+    pend.dim = ddim;
+    const int64_t npend = r.count(16);
+    std::vector<float> v((size_t)ddim);
+    for (int64_t j = 0; j < npend; ++j) {
+        const int64_t id = r.get<int64_t>();
+        const int64_t n = r.count(4);
+        if (n != ddim) fail(HANN_ERR_FORMAT, strf("pending vector %lld has %lld values, expected %lld", (long long)id,
+                                                  (long long)n, (long long)ddim));
+        r.raw(v.data(), n * 4);
+        if (!pend.count(id)) pend.put(id, v.data());
+    }
+    dim = ddim;
+    coarse_k = dck;
+    m = dm;
+    pq_k = dpk;
+    iters = dit;
+    // a file without a probe count keeps the configured one, else 3
+    // (pqivf/index.go:1070-1074)
+    if (dnp > 0) nprobe = dnp;
+    else if (nprobe <= 0) nprobe = 3;
+    fallback = dfb;
+    trained = dtr;
+    seed = dseed;
+    train_epoch = depoch;
+    kc = dkc;
+    coarse.swap(C);
+    have_codebooks = dhave;
+    pmax = dpmax;
+    cb_len.swap(cbl);
+    codebooks.swap(CB);
+    lists.swap(L);
+    id_to_cluster.swap(i2c);
+    std::swap(pending, pend);
+    mark_dirty();
+}
+
+}  // namespace hann
+
+// =================================================================== C-ABI
+using namespace hann;
+
+struct hann_pqivf {
+    PQIVF ix;
+};
+
+extern "C" {
+
+void hann_pqivf_default_options(hann_pqivf_options *o) {
+    std::memset(o, 0, sizeof(*o));
+    o->bruteforce_fallback = 1;
+    o->seed = 0;
+}
+
+int hann_pqivf_new(int64_t dim, const hann_pqivf_options *opts, hann_pqivf **out) {
+    HANN_API_BEGIN
+    *out = nullptr;
+    // pqivf.New (pqivf/index.go:106-146)
+    if (dim <= 0) fail(HANN_ERR_INVALID_ARG, strf("dimension (%lld) must be positive", (long long)dim));
+    hann_pqivf_options o;
+    hann_pqivf_default_options(&o);
+    if (opts) o = *opts;
+    auto pick = [&](int64_t v, int bit, int64_t def) { return (v != 0 || (o.explicit_mask >> bit) & 1) ? v : def; };
+    auto h = std::make_unique<hann_pqivf>();
+    PQIVF &x = h->ix;
+    x.dim = dim;
+    x.pending.dim = dim;
+    x.coarse_k = pick(o.coarse_k, 0, 16);
+    x.m = pick(o.num_subquantizers, 1, 0);
+    if (x.m == 0 && !((o.explicit_mask >> 1) & 1)) x.m = default_num_subquantizers(dim);
+    x.pq_k = pick(o.pq_k, 2, 16);
+    x.iters = pick(o.kmeans_iters, 3, 10);
+    x.nprobe = pick(o.candidate_clusters, 4, 3);
+    x.fallback = ((o.explicit_mask >> 5) & 1) ? o.bruteforce_fallback != 0 : true;
+    x.seed = o.seed;
+    x.train_precision = o.train_precision;
+    x.kmeans_init = o.kmeans_init;
+    x.coarse_sample = o.train_sample;
+    x.pq_sample = o.pq_train_sample;
+    if (x.coarse_k <= 0) fail(HANN_ERR_INVALID_ARG, strf("coarseK (%lld) must be positive", (long long)x.coarse_k));
+    if (x.pq_k <= 0) fail(HANN_ERR_INVALID_ARG, strf("pqK (%lld) must be positive", (long long)x.pq_k));
+    if (x.iters <= 0) fail(HANN_ERR_INVALID_ARG, strf("kMeansIters (%lld) must be positive", (long long)x.iters));
+    if (x.m <= 0) fail(HANN_ERR_INVALID_ARG, strf("numSubquantizers (%lld) must be positive", (long long)x.m));
+    if (dim % x.m != 0)
+        fail(HANN_ERR_INVALID_ARG, strf("dimension (%lld) must be divisible by numSubquantizers (%lld)", (long long)dim,
+                                        (long long)x.m));
+    *out = h.release();
+    HANN_API_END
+}
+
+void hann_pqivf_free(hann_pqivf *idx) { delete idx; }
+
+#define IDX_CHECK()                                                    \
+    if (!idx) fail(HANN_ERR_INVALID_ARG, "index handle is null");      \
+    PQIVF &x = idx->ix
+
+int hann_pqivf_add(hann_pqivf *idx, int64_t id, const float *vec, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.add_locked(id, vec, len);
+    HANN_API_END
+}
+
+int hann_pqivf_bulk_add(hann_pqivf *idx, const int64_t *ids, const float *flat, const size_t *lens, size_t n) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.bulk_add(ids, flat, lens, n);
+    HANN_API_END
+}
+
+int hann_pqivf_delete(hann_pqivf *idx, int64_t id) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.delete_locked(id);
+    HANN_API_END
+}
+
+int hann_pqivf_bulk_delete(hann_pqivf *idx, const int64_t *ids, size_t n) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.bulk_delete(ids, n);
+    HANN_API_END
+}
+
+int hann_pqivf_update(hann_pqivf *idx, int64_t id, const float *vec, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.update(id, vec, len);
+    HANN_API_END
+}
+
+int hann_pqivf_bulk_update(hann_pqivf *idx, const int64_t *ids, const float *flat, const size_t *lens, size_t n) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.bulk_update(ids, flat, lens, n);
+    HANN_API_END
+}
+
+int hann_pqivf_train(hann_pqivf *idx) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.train();
+    HANN_API_END
+}
+
+void hann_search_default_params(hann_search_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->scan_mode = HANN_SCAN_EXACT;
+}
+
+int hann_pqivf_search(hann_pqivf *idx, const float *query, size_t len, int64_t k, int64_t *out_ids, double *out_dist,
+                      int64_t *out_count) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    *out_count = 0;
+    std::unique_lock<std::shared_mutex> lk(x.mu);   // the device mirror may be rebuilt
+    x.check_searchable(k, len);
+    if (k > 1 << 20) fail(HANN_ERR_UNSUPPORTED, "k above 2^20 is not supported");
+    hann_search_params p;
+    hann_search_default_params(&p);
+    int32_t kk = (int32_t)k, cnt = 0;
+    x.search_host(query, 1, &kk, nullptr, p, out_ids, out_dist, &cnt);
+    *out_count = cnt;
+    HANN_API_END
+}
+
+int hann_pqivf_search_batch(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k, const int32_t *nprobe,
+                            const hann_search_params *params, int64_t *out_ids, double *out_dist, int32_t *out_counts) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    if (!k) fail(HANN_ERR_INVALID_ARG, "k array is null");
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    hann_search_params p;
+    hann_search_default_params(&p);
+    if (params) p = *params;
+    x.search_host(queries, (int64_t)nq, k, nprobe, p, out_ids, out_dist, out_counts);
+    HANN_API_END
+}
+
+int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
+                                const int32_t *nprobe, int32_t kmax, int32_t max_nprobe,
+                                const hann_search_params *params, int64_t *out_ids, float *out_dist,
+                                int32_t *out_counts, void *stream) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.check_searchable(kmax, (size_t)x.dim);
+    hann_search_params p;
+    hann_search_default_params(&p);
+    if (params) p = *params;
+    x.sync_device();
+    hipStream_t st = stream ? (hipStream_t)stream : x.stream;
+    float *key = x.ws.get<float>(11, nq * (size_t)kmax);
+    x.search_device(queries, (int64_t)nq, k, nprobe, kmax, max_nprobe > 0 ? max_nprobe : (int)x.nprobe, p, key, out_ids,
+                    out_counts, st);
+    launch_finalize(key, (int64_t)nq * kmax, out_dist, st);
+    if (!stream) HANN_HIP(hipStreamSynchronize(st));
+    HANN_API_END
+}
+
+int hann_pqivf_stats(hann_pqivf *idx, hann_index_stats *out) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    std::memset(out, 0, sizeof(*out));
+    out->count = x.count();
+    out->dimension = x.dim;
+    out->fallback_searches = x.fallback_count();
+    std::strncpy(out->distance, "euclidean", sizeof(out->distance) - 1);
+    HANN_API_END
+}
+
+int hann_pqivf_save(hann_pqivf *idx, void **buf, size_t *len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    std::vector<unsigned char> b = x.save();
+    void *p = std::malloc(b.size() ? b.size() : 1);
+    if (!p) fail(HANN_ERR_OOM, "host allocation failed");
+    std::memcpy(p, b.data(), b.size());
+    *buf = p;
+    *len = b.size();
+    HANN_API_END
+}
+
+int hann_pqivf_load(hann_pqivf *idx, const void *buf, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.load(buf, len);
+    HANN_API_END
+}
+
+int hann_pqivf_profile(hann_pqivf *idx, int enable) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.profile = enable != 0;
+    HANN_API_END
+}
+
+int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, double *rerank_ms, int64_t *calls) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.resolve_timers();
+    *coarse_ms = x.t_coarse_ms;
+    *scan_ms = x.t_scan_ms;
+    *rerank_ms = x.t_rerank_ms;
+    *calls = x.t_calls;
+    x.t_coarse_ms = x.t_scan_ms = x.t_rerank_ms = 0;
+    x.t_calls = 0;
+    HANN_API_END
+}
+
+int hann_pqivf_export_lists(hann_pqivf *idx, int64_t *list_off, int64_t *ids, int32_t *codes, float *vecs) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    int64_t pos = 0;
+    for (size_t c = 0; c < x.lists.size(); ++c) {
+        const ListData &L = x.lists[c];
+        if (list_off) list_off[c] = pos;
+        const size_t n = L.ids.size();
+        if (ids) std::memcpy(ids + pos, L.ids.data(), n * sizeof(int64_t));
+        if (codes)
+            for (size_t r = 0; r < n; ++r)
+                for (int64_t i = 0; i < x.m; ++i)
+                    codes[(pos + (int64_t)r) * x.m + i] = L.has_codes[r] ? L.codes[r * x.m + i] : -1;
+        if (vecs) std::memcpy(vecs + pos * x.dim, L.vecs.data(), L.vecs.size() * sizeof(float));
+        pos += (int64_t)n;
+    }
+    if (list_off) list_off[x.lists.size()] = pos;
+    HANN_API_END
+}
+
+int hann_pqivf_is_trained(hann_pqivf *idx) {
+    if (!idx) return 0;
+    std::shared_lock<std::shared_mutex> lk(idx->ix.mu);
+    return idx->ix.trained ? 1 : 0;
+}
+
+int hann_pqivf_shape(hann_pqivf *idx, int64_t *k, int64_t *m, int64_t *p, int64_t *dim) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    *k = x.kc;
+    *m = x.m;
+    *p = x.pmax;
+    *dim = x.dim;
+    HANN_API_END
+}
+
+int hann_pqivf_coarse_centroids(hann_pqivf *idx, float *out) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    std::memcpy(out, x.coarse.data(), x.coarse.size() * sizeof(float));
+    HANN_API_END
+}
+
+int hann_pqivf_codebooks(hann_pqivf *idx, float *out, int64_t *cb_len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    std::memcpy(out, x.codebooks.data(), x.codebooks.size() * sizeof(float));
+    for (int64_t i = 0; i < x.m; ++i) cb_len[i] = x.have_codebooks ? x.cb_len[i] : 0;
+    HANN_API_END
+}
+
+int hann_pqivf_list_ids(hann_pqivf *idx, int64_t list, int64_t *out, int64_t *len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    if (list < 0 || list >= (int64_t)x.lists.size() || !x.lists[list].exists) {
+        *len = -1;   // no such list key
+        return HANN_OK;
+    }
+    const ListData &L = x.lists[list];
+    *len = (int64_t)L.ids.size();
+    if (out) std::memcpy(out, L.ids.data(), L.ids.size() * sizeof(int64_t));
+    HANN_API_END
+}
+
+int hann_pqivf_list_codes(hann_pqivf *idx, int64_t list, int32_t *out, int64_t *len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    if (list < 0 || list >= (int64_t)x.lists.size() || !x.lists[list].exists) {
+        *len = -1;
+        return HANN_OK;
+    }
+    const ListData &L = x.lists[list];
+    *len = (int64_t)L.ids.size();
+    if (out)
+        for (size_t r = 0; r < L.ids.size(); ++r)
+            for (int64_t i = 0; i < x.m; ++i) out[r * x.m + i] = L.has_codes[r] ? L.codes[r * x.m + i] : -1;
+    HANN_API_END
+}
+
+int hann_pqivf_encode_vector(hann_pqivf *idx, const float *vec, size_t len, int64_t cluster, int32_t *codes) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    // encodeVector (pqivf/index.go:669-675)
+    if (!x.have_codebooks) fail(HANN_ERR_INVALID_ARG, "codebooks not trained");
+    if ((int64_t)len != x.dim) fail(HANN_ERR_INVALID_ARG, "vector lengths do not match");
+    if (cluster < 0 || cluster >= x.kc) fail(HANN_ERR_INVALID_ARG, "cluster out of range");
+    for (int64_t i = 0; i < x.m; ++i)
+        if (x.cb_len[i] == 0) fail(HANN_ERR_INVALID_ARG, "failed to encode sub-vector");
+    int32_t c = (int32_t)cluster;
+    x.encode_rows(vec, 1, &c, codes);
+    HANN_API_END
+}
+
+}  // extern "C"

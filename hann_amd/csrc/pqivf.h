@@ -1,0 +1,134 @@
+// pqivf.h -- the PQIVF index runtime (pqivf.cpp).
+#pragma once
+
+#include <array>
+#include <atomic>
+#include <map>
+#include <shared_mutex>
+#include <unordered_map>
+#include <vector>
+
+#include "kmeans.h"
+#include "runtime.h"
+
+namespace hann {
+
+// Entries of one coarse list in storage order.
+struct ListData {
+    std::vector<int64_t> ids;
+    std::vector<float> vecs;        // [len][dim]
+    std::vector<int32_t> codes;     // [len][m]
+    std::vector<uint8_t> has_codes; // entry carries PQ codes
+    bool exists = false;            // the map key invertedLists[c] is present
+};
+
+// Vectors added before training (pendingVectors), stored contiguously:
+// Train sorts them by id, so their storage order carries no meaning.
+struct PendingStore {
+    int64_t dim = 0;
+    std::unordered_map<int64_t, int64_t> slot;
+    std::vector<int64_t> ids;
+    std::vector<float> vecs;   // [size][dim]
+    bool count(int64_t id) const { return slot.count(id) != 0; }
+    size_t size() const { return ids.size(); }
+    void put(int64_t id, const float *v) {
+        slot[id] = (int64_t)ids.size();
+        ids.push_back(id);
+        vecs.insert(vecs.end(), v, v + dim);
+    }
+    bool erase(int64_t id) {
+        auto it = slot.find(id);
+        if (it == slot.end()) return false;
+        const int64_t s = it->second, last = (int64_t)ids.size() - 1;
+        slot.erase(it);
+        if (s != last) {
+            ids[s] = ids[last];
+            std::copy(vecs.begin() + last * dim, vecs.begin() + (last + 1) * dim, vecs.begin() + s * dim);
+            slot[ids[s]] = s;
+        }
+        ids.pop_back();
+        vecs.resize((size_t)last * dim);
+        return true;
+    }
+    void clear() {
+        slot.clear();
+        ids.clear();
+        vecs.clear();
+        ids.shrink_to_fit();
+        vecs.shrink_to_fit();
+    }
+};
+
+// runKMeans over `groups` problems on device data; returns min(k, n)
+int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
+                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st);
+
+struct PQIVF {
+    // parameters (pqivf/index.go:36-54)
+    int64_t dim = 0, coarse_k = 16, m = 1, pq_k = 16, iters = 10, nprobe = 3;
+    bool fallback = true;
+    uint64_t seed = 0;
+    int train_precision = 0, kmeans_init = 0;
+    int64_t coarse_sample = 0, pq_sample = 0;
+    // state
+    bool trained = false;
+    int64_t kc = 0;                   // len(coarseCentroids)
+    std::vector<float> coarse;        // [kc][dim]
+    bool have_codebooks = false;
+    int64_t pmax = 0;                 // codebook rows (stride)
+    std::vector<int64_t> cb_len;      // [m]
+    std::vector<float> codebooks;     // [m][pmax][dim/m]
+    std::vector<ListData> lists;      // [kc]
+    std::unordered_map<int64_t, int64_t> id_to_cluster;
+    PendingStore pending;
+    uint64_t train_epoch = 0;
+    std::shared_mutex mu;
+
+    // device mirror
+    hipStream_t stream = nullptr;
+    Workspace ws;
+    bool dirty = true;
+    bool any_uncoded = false;
+    int code_bytes = 1;
+    int64_t n_entries = 0;
+    DevBuf<int64_t> d_list_start, d_list_len, d_ids, d_cb_len, all_lists;
+    DevBuf<uint8_t> d_codes, d_has_codes;
+    DevBuf<float> d_vecs, d_coarse, d_cb, scratch_f;
+    DevBuf<unsigned long long> d_fb_counter;
+
+    PQIVF();
+    ~PQIVF();
+    void ensure_stream();
+    void validate_vector(size_t len) const;
+    bool codes_valid(const int32_t *codes) const;
+    int64_t nearest_centroid(const float *vec);
+    void encode_rows(const float *vecs, int64_t n, const int32_t *clusters, int32_t *codes_out);
+    void assign_locked(const std::vector<int64_t> &ids, const std::vector<const float *> &vecs);
+    void add_locked(int64_t id, const float *vec, size_t len);
+    void bulk_add(const int64_t *ids, const float *flat, const size_t *lens, size_t n);
+    void remove_from_list(int64_t id, int64_t cluster);
+    void delete_locked(int64_t id);
+    void bulk_delete(const int64_t *ids, size_t n);
+    void update(int64_t id, const float *vec, size_t len);
+    void bulk_update(const int64_t *ids, const float *flat, const size_t *lens, size_t n);
+    void train();
+    void mark_dirty();
+    void sync_device();
+    void check_searchable(int64_t k, size_t len) const;
+    void search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,
+                       const hann_search_params &p, float *out_key, int64_t *out_id, int32_t *out_cnt, hipStream_t st);
+    void search_host(const float *queries, int64_t nq, const int32_t *k, const int32_t *np,
+                     const hann_search_params &p, int64_t *out_ids, double *out_dist, int32_t *out_counts);
+    int64_t fallback_count();
+    // stage timers (HIP events on the search stream), accumulated when enabled
+    bool profile = false;
+    double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;
+    int64_t t_calls = 0;
+    std::vector<std::array<hipEvent_t, 4>> t_events;   // pending, resolved by stage_times()
+    void resolve_timers();
+    int64_t count() const;
+    std::vector<unsigned char> save() const;
+    void load(const void *buf, size_t len);
+};
+
+}  // namespace hann

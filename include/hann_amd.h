@@ -131,20 +131,23 @@ typedef struct hann_pqivf hann_pqivf;
  * candidate_clusters 3, num_subquantizers = largest of 8,4,2,1 dividing dim.
  * To pass an explicit non-positive value (which New rejects), set the
  * matching bit in `explicit_mask` (bit 0 coarse_k, 1 num_subquantizers,
- * 2 pq_k, 3 kmeans_iters, 4 candidate_clusters). */
+ * 2 pq_k, 3 kmeans_iters, 4 candidate_clusters). The fallback is on unless
+ * bit 5 is set, in which case bruteforce_fallback decides
+ * (WithBruteForceFallback, pqivf/index.go:92-94). */
 typedef struct {
     int64_t coarse_k;
     int64_t num_subquantizers;
     int64_t pq_k;
     int64_t kmeans_iters;
     int64_t candidate_clusters;
-    int32_t bruteforce_fallback; /* 1 = on (default), 0 = off; see explicit_mask bit 5 */
+    int32_t bruteforce_fallback; /* read when explicit_mask bit 5 is set */
     uint32_t explicit_mask;
     uint64_t seed;               /* random stream seed (the HANN_SEED role, core/utils.go:12-20) */
     /* build-side extensions */
     int32_t train_precision;     /* hann_precision for k-means assignment */
     int32_t kmeans_init;         /* 0 perm (reference), 1 k-means++ */
-    int64_t train_sample;        /* 0 = train on all vectors; else a seeded sample size */
+    int64_t train_sample;        /* coarse k-means on a seeded sample of this size (0 = all) */
+    int64_t pq_train_sample;     /* PQ k-means on a seeded residual sample (0 = all) */
 } hann_pqivf_options;
 
 void hann_pqivf_default_options(hann_pqivf_options *opts);
@@ -219,6 +222,22 @@ int hann_pqivf_stats(hann_pqivf *idx, hann_index_stats *out);
 int hann_pqivf_save(hann_pqivf *idx, void **buf, size_t *len);
 int hann_pqivf_load(hann_pqivf *idx, const void *buf, size_t len);
 void hann_free_buffer(void *buf);
+
+/* Stage timers of the search path (measurement; no counterpart in the
+ * reference). When enabled, every search records HIP events on its stream
+ * around the coarse step, the list scan and the rerank.
+ * hann_pqivf_stage_times waits for them, returns the accumulated
+ * milliseconds and the number of searches, and resets the totals. */
+int hann_pqivf_profile(hann_pqivf *idx, int enable);
+int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, double *rerank_ms,
+                           int64_t *calls);
+
+/* Bulk export of the clustered entries in list order (CSR): list_off
+ * [k+1], ids [N], codes [N][m] (rows of -1: entry without codes), vecs
+ * [N][dim], N = list_off[k]. Any output may be NULL. Build extension for
+ * tools that need the whole trained state at once (what Save holds). */
+int hann_pqivf_export_lists(hann_pqivf *idx, int64_t *list_off, int64_t *ids, int32_t *codes,
+                            float *vecs);
 
 /* ------------------------------------------ test hooks (export_test.go) -- */
 int hann_pqivf_is_trained(hann_pqivf *idx);

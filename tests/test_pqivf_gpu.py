@@ -1,0 +1,488 @@
+"""GPU parity of the PQIVF pipeline through the C-ABI.
+
+Bars (stated per test):
+  * k-means centroids, cluster assignments, list membership and order, PQ
+    codebooks and codes: bit-exact against the oracle's restatement of
+    pqivf/index.go on identical seeded inputs.
+  * Search in HANN_SCAN_EXACT mode (the reference's decode-and-measure
+    distance): ids and distances identical to the oracle.
+  * ADC lookup-table modes: distances within the stated FP32 / FP16 bounds of
+    the oracle, ids equal except at near-ties inside that bound.
+  * The reference's behavioural tests (pqivf/index_test.go), restated.
+"""
+import json
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN, bits_to_f32
+
+pytestmark = pytest.mark.gpu
+
+
+def P():
+    from hann_amd import pqivf
+    return pqivf
+
+
+def build_pair(oracle, x, ids, coarse_k, m, pq_k, iters, seed):
+    pq = P()
+    idx = pq.New(x.shape[1], pq.WithCoarseK(coarse_k), pq.WithNumSubquantizers(m), pq.WithPQK(pq_k),
+                 pq.WithKMeansIters(iters), pq.WithSeed(seed))
+    idx.bulk_add_arrays(ids, x)
+    idx.Train()
+    ref = oracle.train(ids, x, coarse_k, m, pq_k, iters, seed=seed, epoch=0, nthreads=8)
+    return idx, ref
+
+
+# ------------------------------------------------------------- k-means --
+@pytest.mark.parametrize("n,d,k", [(1, 4, 1), (7, 8, 3), (500, 16, 8), (1000, 32, 129), (800, 40, 17),
+                                   (3000, 64, 64), (2000, 96, 100), (300, 130, 5)])
+def test_assign_points_matches_oracle(hann, oracle, n, d, k):
+    """pqivf/index_test.go:1005-1066 shapes plus wider ones: bit-exact."""
+    rng = np.random.default_rng(n + d + k)
+    data = rng.standard_normal((n, d)).astype(np.float32) * 3
+    cents = rng.standard_normal((k, d)).astype(np.float32) * 3
+    got = P().AssignPoints(data, cents)
+    assert np.array_equal(got, oracle.assign(data, cents, nthreads=8))
+
+
+def test_assign_points_tie_lowest_index(hann):
+    """Duplicate centroids tie; the lowest index wins."""
+    cents = np.array([[1, 1, 1, 1], [0, 0, 0, 0], [0, 0, 0, 0], [5, 5, 5, 5]], np.float32)
+    data = np.array([[0, 0, 0, 0], [0.1, 0, 0, 0], [4, 4, 4, 4]], np.float32)
+    assert P().AssignPoints(data, cents).tolist() == [1, 1, 3]
+    cents64 = np.zeros((70, 64), np.float32)
+    data64 = np.ones((5, 64), np.float32)
+    assert P().AssignPoints(data64, cents64).tolist() == [0] * 5
+
+
+@pytest.mark.parametrize("n,d,k,iters", [(2000, 16, 16, 5), (3000, 64, 32, 6), (500, 40, 7, 4),
+                                         (300, 8, 400, 3), (5000, 4, 256, 5), (1200, 33, 10, 3)])
+def test_kmeans_matches_oracle_bit_exact(hann, oracle, n, d, k, iters):
+    rng = np.random.default_rng(n * 7 + d)
+    centers = rng.uniform(-10, 10, (max(k // 2, 1), d)).astype(np.float32)
+    data = (centers[rng.integers(0, len(centers), n)] + rng.standard_normal((n, d))).astype(np.float32)
+    seed = 1234 + n
+    got = P().KMeans(data, k, iters, seed)
+    want, _ = oracle.kmeans(data, k, iters, oracle.stream_seed(seed, 0, 0), nthreads=8)
+    assert got.shape == want.shape
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_kmeans_empty_clusters_reseed_like_oracle(hann, oracle):
+    """Many duplicate points force empty clusters every iteration."""
+    base = np.array([[0, 0, 0, 0], [10, 10, 10, 10], [20, 0, 0, 0]], np.float32)
+    data = np.repeat(base, 200, axis=0)
+    got = P().KMeans(data, 12, 4, 77)
+    want, _ = oracle.kmeans(data, 12, 4, oracle.stream_seed(77, 0, 0))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_kmeans_large_single_cluster(hann, oracle):
+    """One cluster holding many points (the stable grouping's long runs)."""
+    rng = np.random.default_rng(3)
+    data = (rng.standard_normal((20000, 8)) * 0.01).astype(np.float32)
+    data[:5] += 100
+    got = P().KMeans(data, 2, 3, 5)
+    want, _ = oracle.kmeans(data, 2, 3, oracle.stream_seed(5, 0, 0))
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+# --------------------------------------------------------------- train --
+def check_train_parity(idx, ref):
+    pq = P()
+    C = pq.CoarseCentroids(idx)
+    assert np.array_equal(C.view(np.uint32), ref.coarse.view(np.uint32))
+    CB, cbl = pq.Codebooks(idx)
+    assert np.array_equal(cbl, ref.cb_len.astype(np.int64))
+    p = int(cbl.max())
+    assert np.array_equal(CB[:, :p].view(np.uint32), ref.codebooks[:, :p].view(np.uint32))
+    lists = pq.InvertedIDs(idx)
+    codes = pq.InvertedCodes(idx)
+    for c in range(len(ref.list_off) - 1):
+        seg = ref.ids[ref.list_off[c]:ref.list_off[c + 1]]
+        if len(seg) == 0:
+            assert c not in lists
+            continue
+        assert lists[c] == seg.tolist()
+        assert np.array_equal(codes[c], ref.codes[ref.list_off[c]:ref.list_off[c + 1]])
+
+
+def test_train_config1_bit_exact(hann, oracle):
+    """Config 1 (20k x 64, K=64, 10 iterations, PQ 8 x 256): coarse
+    centroids, codebooks, assignments, list order and codes bit-exact."""
+    from hann_amd import synthetic
+    x, _ = synthetic.config1()
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    idx, ref = build_pair(oracle, x, ids, 64, 8, 256, 10, seed=11)
+    check_train_parity(idx, ref)
+
+
+def test_train_shuffled_ids_and_small_dims(hann, oracle):
+    rng = np.random.default_rng(21)
+    x = (rng.uniform(-10, 10, (8, 12))[rng.integers(0, 8, 3000)] + rng.standard_normal((3000, 12))).astype(np.float32)
+    ids = rng.permutation(100000)[:3000].astype(np.int64)
+    idx, ref = build_pair(oracle, x, ids, 16, 4, 16, 6, seed=5)
+    check_train_parity(idx, ref)
+
+
+def test_train_wide_subspaces(hann, oracle):
+    """d=128, m=2: 64-dim sub-spaces take the tiled assignment path."""
+    rng = np.random.default_rng(22)
+    x = (rng.uniform(-5, 5, (10, 128))[rng.integers(0, 10, 1500)] + rng.standard_normal((1500, 128))).astype(np.float32)
+    ids = np.arange(1500, dtype=np.int64) * 2
+    idx, ref = build_pair(oracle, x, ids, 8, 2, 32, 4, seed=9)
+    check_train_parity(idx, ref)
+
+
+# -------------------------------------------------------------- search --
+def test_search_exact_mode_identical_to_oracle(hann, oracle):
+    """Config 1: 200 queries, k=10, nprobe=8, the reference's own distance."""
+    from hann_amd import synthetic, _lib
+    x, qs = synthetic.config1()
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    idx, ref = build_pair(oracle, x, ids, 64, 8, 256, 10, seed=11)
+    got_ids, got_d, cnt = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_EXACT)
+    want_ids, want_d, wcnt = oracle.search_many(ref, qs, 10, 8, nthreads=8)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got_ids, want_ids)
+    assert np.array_equal(got_d.view(np.uint64), want_d.view(np.uint64))
+    # the single-query drop-in path (the index's default nprobe, 3) agrees too
+    w3_ids, w3_d, _ = oracle.search_many(ref, qs, 10, 3, nthreads=8)
+    for qi in range(0, 200, 37):
+        r = idx.Search(qs[qi], 10)
+        assert [n.ID for n in r] == w3_ids[qi].tolist()
+        assert [n.Distance for n in r] == w3_d[qi].tolist()
+
+
+def adc_bound(ref, q, ids_found, d_ref, f16):
+    """Per-candidate bound on |ADC - reference| distance: both round the same
+    real ||q - (c + cb)||^2; FP32 terms are bounded by (d + m + 8) ulps of
+    the magnitudes involved, FP16 tables add 2^-11 of the distance."""
+    d = q.size
+    scale = d_ref + float(np.dot(q, q)) + 1.0
+    b = (d + ref.m + 8) * 2.0 ** -23 * scale * 4
+    if f16:
+        b += 2.0 ** -10 * d_ref + 2.0 ** -10
+    return b
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_search_adc_modes_within_bound(hann, oracle, mode):
+    from hann_amd import synthetic
+    x, qs = synthetic.config1()
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    idx, ref = build_pair(oracle, x, ids, 64, 8, 256, 10, seed=11)
+    k = 10
+    got_ids, got_d, cnt = idx.search_batch(qs, k, nprobe=8, scan_mode=mode)
+    want_ids, want_d, _ = oracle.search_many(ref, qs, k, 8, nthreads=8)
+    # reference distance of any entry: the float32 reconstruction, in FP64
+    pos = {int(i): j for j, i in enumerate(ref.ids)}
+    cl = np.repeat(np.arange(len(ref.list_off) - 1), np.diff(ref.list_off))
+    recon = ref.coarse[cl] + np.concatenate([ref.codebooks[i][ref.codes[:, i]] for i in range(ref.m)], axis=1)
+    recon = recon.astype(np.float32)
+    mism = 0
+    for qi in range(len(qs)):
+        q = qs[qi].astype(np.float64)
+        kth = want_d[qi, k - 1] ** 2
+        for r in range(k):
+            gd2 = got_d[qi, r] ** 2
+            j = pos[int(got_ids[qi, r])]
+            true2 = float(((recon[j].astype(np.float64) - q) ** 2).sum())
+            tol = adc_bound(ref, qs[qi], None, true2, mode == 2)
+            assert abs(gd2 - true2) <= tol, (qi, r, gd2, true2, tol)
+            # every returned id is within the bound of the reference's k-th
+            assert true2 <= kth + 2 * tol
+        if set(got_ids[qi]) != set(want_ids[qi]):
+            mism += 1
+            # a differing id must sit inside the tie band at the k-th distance
+            for gid in set(got_ids[qi]) - set(want_ids[qi]):
+                j = pos[int(gid)]
+                true2 = float(((recon[j].astype(np.float64) - q) ** 2).sum())
+                assert abs(true2 - kth) <= 2 * adc_bound(ref, qs[qi], None, kth, mode == 2)
+    assert mism <= len(qs) // 10
+
+
+def test_search_rerank_is_exact_fp32(hann, oracle):
+    """rerank R: exact reference-order distances on the stored vectors."""
+    from hann_amd import synthetic
+    x, qs = synthetic.config1()
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    idx, ref = build_pair(oracle, x, ids, 64, 8, 256, 10, seed=11)
+    got_ids, got_d, cnt = idx.search_batch(qs[:50], 10, nprobe=8, scan_mode=1, rerank=4)
+    for qi in range(50):
+        for r in range(10):
+            j = int(got_ids[qi, r])
+            want = np.sqrt(np.float64(oracle.pair(0, qs[qi], x[j])))
+            assert got_d[qi, r] == want
+        assert np.all(np.diff(got_d[qi]) >= 0)
+
+
+def test_per_query_k_and_nprobe(hann, oracle):
+    from hann_amd import synthetic
+    x, qs = synthetic.config1(n=6000, q=40)
+    ids = np.arange(x.shape[0], dtype=np.int64)
+    idx, ref = build_pair(oracle, x, ids, 32, 8, 64, 5, seed=3)
+    rng = np.random.default_rng(0)
+    ks = rng.choice([1, 5, 10, 50], 40).astype(np.int32)
+    nps = rng.choice([1, 2, 8, 32], 40).astype(np.int32)
+    got_ids, got_d, cnt = idx.search_batch(qs, ks, nprobe=nps, scan_mode=0)
+    for qi in range(40):
+        wi, wd, wc = oracle.search_many(ref, qs[qi:qi + 1], int(ks[qi]), int(nps[qi]))
+        assert cnt[qi] == wc[0]
+        assert got_ids[qi, :ks[qi]].tolist() == wi[0].tolist()
+        assert np.array_equal(got_d[qi, :ks[qi]], wd[0])
+
+
+# ---------------------------------------------- reference behaviour tests --
+def new_small(dim=4, coarse_k=2, m=2, pq_k=2, iters=5, *extra):
+    pq = P()
+    return pq.New(dim, pq.WithCoarseK(coarse_k), pq.WithNumSubquantizers(m), pq.WithPQK(pq_k),
+                  pq.WithKMeansIters(iters), pq.WithSeed(42), *extra)
+
+
+def test_search_basic(hann):
+    """pqivf/index_test.go:68-105"""
+    idx = new_small(6, 2, 2, 2, 5)
+    vecs = {1: [1, 2, 3, 4, 5, 6], 2: [6, 5, 4, 3, 2, 1], 3: [1, 1, 1, 1, 1, 1], 4: [2, 2, 2, 2, 2, 2]}
+    for i, v in vecs.items():
+        idx.Add(i, v)
+    idx.Train()
+    res = idx.Search([1, 2, 3, 4, 5, 6], 2)
+    assert len(res) == 2 and res[0].ID == 1
+
+
+def trained_index():
+    idx = new_small(4, 2, 2, 2, 5)
+    idx.BulkAdd({1: [0, 0, 0, 0], 2: [1, 1, 1, 1], 3: [10, 10, 10, 10], 4: [11, 11, 11, 11]})
+    idx.Train()
+    return idx
+
+
+def ids_of(idx, q, k):
+    return {n.ID for n in idx.Search(q, k)}
+
+
+def test_add_bulkadd_delete_update_after_train(hann):
+    """pqivf/index_test.go:293-359"""
+    idx = trained_index()
+    idx.Add(5, [0.5, 0.5, 0.5, 0.5])
+    assert 5 in ids_of(idx, [0.5, 0.5, 0.5, 0.5], 5)
+    idx.BulkAdd({6: [10.5] * 4, 7: [0.2] * 4})
+    got = ids_of(idx, [10.5] * 4, 7)
+    assert 6 in got and 7 in got
+    idx.Delete(1)
+    got = ids_of(idx, [0, 0, 0, 0], 10)
+    assert 1 not in got
+    idx.Update(2, [10.2] * 4)
+    assert 2 in ids_of(idx, [10.2] * 4, 3)
+
+
+def test_edge_k_larger_than_count(hann):
+    """pqivf/index_test.go:371-453"""
+    idx = new_small(4, 2, 2, 2, 5)
+    idx.BulkAdd({1: [1, 1, 1, 1], 2: [2, 2, 2, 2], 3: [3, 3, 3, 3]})
+    idx.Train()
+    assert len(idx.Search([1, 1, 1, 1], 10)) == 3
+    with pytest.raises(hann.HannError, match="query dimension 2 does not match index dimension 4"):
+        idx.Search([1, 1], 1)
+
+
+def clustered_vectors():
+    """pqivf/index_test.go:552-563"""
+    out = {}
+    i = 0
+    for base in (0, 100, 200, 300):
+        for j in range(3):
+            off = base + j
+            out[i] = [off] * 4
+            i += 1
+    return out
+
+
+def test_candidate_clusters_and_fallback(hann):
+    """pqivf/index_test.go:570-641"""
+    pq = P()
+    vecs = clustered_vectors()
+    k = len(vecs)
+    r = pq.New(4, pq.WithCoarseK(4), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithKMeansIters(10),
+               pq.WithCandidateClusters(1), pq.WithBruteForceFallback(False), pq.WithSeed(1))
+    r.BulkAdd(vecs)
+    r.Train()
+    got = r.Search(vecs[0], k)
+    assert 0 < len(got) < k
+    assert 0 in {n.ID for n in got}
+    assert r.Stats().FallbackSearches == 0
+    f = pq.New(4, pq.WithCoarseK(4), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithKMeansIters(10),
+               pq.WithCandidateClusters(1), pq.WithSeed(1))
+    f.BulkAdd(vecs)
+    f.Train()
+    assert len(f.Search(vecs[0], k)) == k
+    assert f.Stats().FallbackSearches >= 1
+
+
+def test_fallback_counter(hann):
+    """pqivf/index_test.go:819-842"""
+    idx = new_small(4, 2, 2, 2, 5)
+    for i in range(4):
+        idx.Add(i, [i * 10] * 4)
+    idx.Train()
+    assert idx.Stats().FallbackSearches == 0
+    idx.Search([0, 0, 0, 0], 4)
+    assert idx.Stats().FallbackSearches >= 1
+
+
+def test_delete_all_then_search(hann):
+    """pqivf/index_test.go:847-888"""
+    idx = new_small(4, 2, 2, 2, 5)
+    idx.BulkAdd({i: [i] * 4 for i in range(4)})
+    idx.Train()
+    for i in range(4):
+        idx.Delete(i)
+    assert idx.Stats().Count == 0
+    assert idx.Search([0, 0, 0, 0], 3) == []
+    idx.Add(5, [1, 1, 1, 1])
+    res = idx.Search([1, 1, 1, 1], 1)
+    assert [n.ID for n in res] == [5]
+
+
+def test_save_load_trained_same_results(hann, tmp_path):
+    """pqivf/index_test.go:170-205 and the save/load differential
+    (internal/testutil/differential.go): identical distances rank by rank."""
+    from hann_amd import synthetic
+    pq = P()
+    x, qs = synthetic.config1(n=3000, q=20)
+    idx = pq.New(64, pq.WithCoarseK(16), pq.WithPQK(16), pq.WithCandidateClusters(4), pq.WithSeed(8))
+    idx.bulk_add_arrays(np.arange(3000), x)
+    idx.Train()
+    a_ids, a_d, _ = idx.search_batch(qs, 10)
+    path = tmp_path / "idx.bin"
+    with open(path, "wb") as fh:
+        idx.Save(fh)
+    other = pq.New(64)
+    with open(path, "rb") as fh:
+        other.Load(fh)
+    b_ids, b_d, _ = other.search_batch(qs, 10)
+    assert np.array_equal(a_ids, b_ids) and np.array_equal(a_d, b_d)
+    assert other.Stats().Count == 3000
+
+
+def test_candidate_clusters_survive_save_load(hann):
+    """pqivf/index_test.go:894-938"""
+    pq = P()
+    vecs = clustered_vectors()
+    idx = pq.New(4, pq.WithCoarseK(4), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithCandidateClusters(4),
+                 pq.WithSeed(2))
+    idx.BulkAdd(vecs)
+    idx.Train()
+    data = idx.save_bytes()
+    other = pq.New(4)
+    other.load_bytes(data)
+    got = other.Search(vecs[1], 1)
+    assert got[0].ID == 1
+    assert other.Stats().FallbackSearches == 0
+
+
+def test_train_lists_ascending_and_retrain(hann):
+    """pqivf/index_test.go:1072-1097, and Train as a refresh"""
+    pq = P()
+    rng = np.random.default_rng(1)
+    ids = rng.permutation(400)
+    x = rng.standard_normal((400, 8)).astype(np.float32)
+    idx = pq.New(8, pq.WithCoarseK(4), pq.WithSeed(3))
+    idx.bulk_add_arrays(ids, x)
+    idx.Train()
+    idx.Delete(int(ids[0]))
+    idx.Add(1000, x[0])
+    idx.Train()
+    for c, lst in pq.InvertedIDs(idx).items():
+        assert lst == sorted(lst)
+    assert idx.Stats().Count == 400
+
+
+def test_synthetic_recall_floor(hann):
+    """pqivf/quality_test.go:11-56: recall@10 >= 0.10 on a 2000 x 16,
+    16-cluster set (coarseK 16, m 2, pqK 16, 10 iterations)."""
+    from hann_amd import synthetic
+    pq = P()
+    x, _ = synthetic.gaussian_mixture(2000, 16, 16, seed=42)
+    qs = synthetic.clustered_queries(x, 20, seed=43)
+    idx = pq.New(16, pq.WithCoarseK(16), pq.WithNumSubquantizers(2), pq.WithPQK(16), pq.WithKMeansIters(10),
+                 pq.WithSeed(42))
+    idx.bulk_add_arrays(np.arange(2000), x)
+    idx.Train()
+    total = 0.0
+    for q in qs:
+        d = ((x.astype(np.float64) - q) ** 2).sum(1)
+        want = np.lexsort((np.arange(2000), d))[:10]
+        got = {n.ID for n in idx.Search(q, 10)}
+        total += len(got & set(want.tolist())) / 10
+    assert total / 20 >= 0.10
+
+
+# ------------------------------------------- the reference's trained file --
+def fixture_index_bytes(g, nprobe=0):
+    """Our binary format holding the state of pqivf/testdata/index_v1.gob."""
+    C = bits_to_f32(g["CoarseCentroidsBits"])
+    CB = bits_to_f32(g["CodebooksBits"])
+    d, m = g["Dimension"], g["NumSubquantizers"]
+    out = bytearray(b"HANNPQIV")
+    out += struct.pack("<I", 1)
+    out += struct.pack("<6q", d, g["CoarseK"], m, g["PqK"], g["KMeansIters"], nprobe)
+    out += struct.pack("<BB", 1 if g["AllowBruteForceFallback"] else 0, 1 if g["Trained"] else 0)
+    out += struct.pack("<QQ", 0, 1)
+    out += struct.pack("<q", C.shape[0]) + C.tobytes()
+    out += struct.pack("<B", 1) + struct.pack("<q", CB.shape[1])
+    out += struct.pack(f"<{m}q", *[CB.shape[1]] * m) + CB.tobytes()
+    lists = g["InvertedLists"]
+    out += struct.pack("<q", len(lists))
+    for c, es in sorted(lists.items(), key=lambda t: int(t[0])):
+        out += struct.pack("<qq", int(c), len(es))
+        out += np.array([e["ID"] for e in es], np.int64).tobytes()
+        out += bytes([1] * len(es))
+        out += np.array([e["Codes"] for e in es], np.int32).tobytes()
+        out += np.concatenate([bits_to_f32(g["VectorBits"][str(e["ID"])]) for e in es]).tobytes()
+    out += struct.pack("<q", 0)
+    return bytes(out)
+
+
+def test_reference_trained_file_loads_and_searches(hann, oracle, gob_fixture):
+    """pqivf/golden_test.go:95-146: the file's index answers with Count 29 and
+    deterministically; its search equals the oracle's on the same state
+    (the golden queries come from Go's math/rand and cannot be rebuilt)."""
+    pq = P()
+    idx = pq.New(8, pq.WithCoarseK(3), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithKMeansIters(5))
+    idx.load_bytes(fixture_index_bytes(gob_fixture))
+    assert idx.Stats().Count == 29
+    with open(os.path.join(GOLDEN, "index_v1.json")) as fh:
+        g = json.load(fh)
+    ents = [e for c, es in sorted(g["InvertedLists"].items(), key=lambda t: int(t[0])) for e in es]
+
+    class Ref:
+        pass
+    ref = Ref()
+    ref.m = 2
+    ref.coarse = bits_to_f32(g["CoarseCentroidsBits"])
+    ref.codebooks = bits_to_f32(g["CodebooksBits"])
+    ref.cb_len = np.array([4, 4], np.uint64)
+    sizes = [len(g["InvertedLists"][str(c)]) for c in range(3)]
+    ref.list_off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    ref.ids = np.array([e["ID"] for e in ents], np.int64)
+    ref.vecs = np.stack([bits_to_f32(g["VectorBits"][str(e["ID"])]) for e in ents]).astype(np.float32)
+    ref.codes = np.array([e["Codes"] for e in ents], np.int32)
+    ref.has_codes = np.ones(29, np.uint8)
+    rng = np.random.default_rng(8)
+    qs = (ref.vecs[rng.integers(0, 29, 3)] + rng.standard_normal((3, 8)) * 0.1).astype(np.float32)
+    for _ in range(2):
+        got_ids, got_d, cnt = idx.search_batch(qs, 5)   # nprobe 3 (legacy default)
+        want_ids, want_d, wcnt = oracle.search_many(ref, qs, 5, 3)
+        assert np.array_equal(got_ids, want_ids) and np.array_equal(got_d, want_d)
+    # stored cluster and codes equal the GPU's assignment and encoding
+    assign = pq.AssignPoints(ref.vecs, ref.coarse)
+    cl = np.repeat(np.arange(3), sizes)
+    assert np.array_equal(assign, cl)
+    for j, e in enumerate(ents):
+        assert pq.EncodeVector(idx, ref.vecs[j], int(cl[j])) == e["Codes"]
