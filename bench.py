@@ -10,7 +10,9 @@ from the same mixture, each with its own k in {10, 50, 100} and nprobe in
 Two search paths are measured on the same index and the same batches:
   plain   the FP32 drop-in: the reference's decode-and-measure PQ distance
           (bit-identical to pqivf/index.go:905-936) + an exact FP32 rerank
-  chosen  FP16 ADC lookup tables with FP32 accumulation + the same rerank
+  chosen  the persistent fused scan: FP16 ADC lookup tables built from
+          precomputed per-list tables, FP32 accumulation, autotuned waves
+          per query, + the same rerank
 The rerank depth of each path is the smallest of --rerank-grid whose
 recall@10 against FP64 ground truth reaches 0.95. `value` is the chosen
 path's queries per second.
@@ -84,9 +86,9 @@ class DevBatch:
         self.cnt = torch.empty(self.nq, dtype=torch.int32, device="cuda")
 
 
-def run_search(idx, b, stream, mode, rerank):
+def run_search(idx, b, stream, mode, rerank, fused):
     idx.search_batch_dev(b.q.data_ptr(), b.nq, b.k.data_ptr(), b.np.data_ptr(), b.kmax, b.pmax, b.ids.data_ptr(),
-                         b.dist.data_ptr(), b.cnt.data_ptr(), stream, scan_mode=mode, rerank=rerank)
+                         b.dist.data_ptr(), b.cnt.data_ptr(), stream, scan_mode=mode, rerank=rerank, fused=fused)
 
 
 def probe_bytes(torch, lib, idx, b, list_sizes, m):
@@ -203,26 +205,26 @@ def main():
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
 
-    def recall_of(mode, rerank):
+    def recall_of(mode, rerank, fused):
         b = dev_batches[0]
-        run_search(idx, b, sh, mode, rerank)
+        run_search(idx, b, sh, mode, rerank, fused)
         torch.cuda.synchronize()
         return recall_at_10(b.ids.cpu().numpy()[:ngt], gt)
 
     grid = [int(v) for v in a.rerank_grid.split(",")]
-    paths = {"plain": _lib.SCAN_EXACT, "chosen": _lib.SCAN_ADC_F16}
+    paths = {"plain": (_lib.SCAN_EXACT, 0), "chosen": (_lib.SCAN_ADC_F16, 1)}
     chosen_r, recalls = {}, {}
-    for name, mode in paths.items():
+    for name, (mode, fused) in paths.items():
         for r in grid:
-            rec = recall_of(mode, r)
+            rec = recall_of(mode, r, fused)
             log(f"{name}: rerank {r} recall@10 {rec:.4f}")
             chosen_r[name], recalls[name] = r, rec
             if rec >= 0.95:
                 break
 
-    def timed(mode, rerank, profile):
+    def timed(mode, rerank, fused, profile):
         for i in range(a.warmup):
-            run_search(idx, dev_batches[i], sh, mode, rerank)
+            run_search(idx, dev_batches[i], sh, mode, rerank, fused)
         torch.cuda.synchronize()
         idx.set_profile(profile)
         idx.stage_times()
@@ -231,7 +233,7 @@ def main():
         t = time.perf_counter()
         for i in range(a.steps):
             evs[i][0].record(stream)
-            run_search(idx, dev_batches[a.warmup + i], sh, mode, rerank)
+            run_search(idx, dev_batches[a.warmup + i], sh, mode, rerank, fused)
             evs[i][1].record(stream)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
@@ -241,8 +243,8 @@ def main():
         return el, lat, st
 
     res = {}
-    for name, mode in paths.items():
-        el, lat, st = timed(mode, chosen_r[name], True)
+    for name, (mode, fused) in paths.items():
+        el, lat, st = timed(mode, chosen_r[name], fused, True)
         qps = a.steps * a.batch / el
         res[name] = {"qps": qps, "ms_per_step": 1000 * el / a.steps, "p50_ms": float(np.percentile(lat, 50)),
                      "p99_ms": float(np.percentile(lat, 99)), "stages": st, "rerank": chosen_r[name],
@@ -261,7 +263,7 @@ def main():
     achieved = bytes_per_launch / (scan_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_scan<ADC_F16>", "scan_ms_per_launch": round(scan_ms, 4),
+                "kernel": "k_scan_fused<ADC_F16>", "scan_ms_per_launch": round(scan_ms, 4),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "pair_bytes_per_launch": int(pair_b / a.steps)}
 
@@ -284,7 +286,8 @@ def main():
         "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
         "recall_at_10": round(ch["recall"], 4),
         "chosen": {"qps": round(ch["qps"], 1), "recall_at_10": round(ch["recall"], 4), "rerank": ch["rerank"],
-                   "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3), "path": "adc-f16 + fp32 rerank",
+                   "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
+                   "path": "persistent fused adc-f16 scan (list tables, autotuned waves/query) + fp32 rerank",
                    "stage_ms_per_step": [round(v / max(ch["stages"][3], 1), 4) for v in ch["stages"][:3]]},
         "plain": {"qps": round(pl["qps"], 1), "recall_at_10": round(pl["recall"], 4), "rerank": pl["rerank"],
                   "p50_ms": round(pl["p50_ms"], 3), "p99_ms": round(pl["p99_ms"], 3),

@@ -59,6 +59,15 @@ class SearchParams(ctypes.Structure):
     _fields_ = [("scan_mode", i32), ("coarse_precision", i32), ("rerank", i32), ("fused", i32)]
 
 
+class ServerOptions(ctypes.Structure):
+    _fields_ = [("max_batch", i32), ("slots", i32), ("hbm_budget", i64), ("params", SearchParams)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [("completed_queries", i64), ("completed_requests", i64), ("batches", i64), ("mutations", i64),
+                ("queue_depth", i64), ("in_flight", i64), ("budget_in_use", i64), ("budget_peak", i64)]
+
+
 class IndexStats(ctypes.Structure):
     _fields_ = [("count", i64), ("dimension", i64), ("fallback_searches", i64),
                 ("distance", ctypes.c_char * 32)]
@@ -110,6 +119,17 @@ SIGNATURES = {
     "hann_pqivf_profile": (cint, [vp, cint]),
     "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),
     "hann_pqivf_export_lists": (cint, [vp, c_int64_p, c_int64_p, c_int32_p, c_float_p]),
+    "hann_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),
+    "hann_server_create": (cint, [vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(vp)]),
+    "hann_server_free": (None, [vp]),
+    "hann_server_submit": (cint, [vp, c_float_p, sz, c_int32_p, c_int32_p, ctypes.POINTER(u64)]),
+    "hann_server_submit_bulk_add": (cint, [vp, c_int64_p, c_float_p, sz, ctypes.POINTER(u64)]),
+    "hann_server_submit_bulk_delete": (cint, [vp, c_int64_p, sz, ctypes.POINTER(u64)]),
+    "hann_server_poll": (cint, [vp, u64, cint, c_int64_p, c_double_p, c_int32_p, c_double_p]),
+    "hann_server_stats": (cint, [vp, ctypes.POINTER(ServerCounters)]),
+    "hann_server_replay": (cint, [vp, sz, c_double_p, c_int32_p, c_int64_p, c_int32_p, c_float_p, c_int32_p,
+                                  c_int32_p, c_int64_p, c_float_p, c_int64_p, i32, c_int64_p, c_double_p,
+                                  c_double_p, c_double_p]),
     "hann_pqivf_is_trained": (cint, [vp]),
     "hann_pqivf_shape": (cint, [vp, c_int64_p, c_int64_p, c_int64_p, c_int64_p]),
     "hann_pqivf_coarse_centroids": (cint, [vp, c_float_p]),

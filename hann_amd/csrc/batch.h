@@ -1,6 +1,7 @@
 // batch.h -- batch top-k dispatch (exact FP32, or BF16 MFMA + FP32 rerank).
 #pragma once
 
+#include "kmeans.h"
 #include "runtime.h"
 
 namespace hann {
@@ -9,5 +10,15 @@ namespace hann {
 void batch_topk(Context &c, int metric, const float *queries, int64_t nq, const float *base,
                 int64_t nb, int dim, int k, int precision, int overfetch, int64_t *out_ids,
                 double *out_dist, hipStream_t st);
+
+// Certified BF16 top-k over (squared) Euclidean ranks: identical to the
+// exact path. hint (k == 1 only, may be null): a row per query whose exact
+// rank bounds the query's nearest, e.g. the previous k-means assignment.
+void certified_topk(Context &c, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
+                    const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st);
+
+// assignPoints on the BF16 path: out[i] = nearest centroid, lowest index on ties
+void assign_bf16(Context &c, const float *data, int64_t n, const float *cents, int kc, int d, const int32_t *hint,
+                 int32_t *out, hipStream_t st);
 
 }  // namespace hann

@@ -1,14 +1,122 @@
-// batch.hip -- batch top-k dispatch.
+// batch.hip -- batch top-k dispatch: exact FP32, or the certified BF16
+// MFMA candidate pass followed by an exact FP32 ranking.
+#include <algorithm>
+#include <vector>
+
 #include "batch.h"
+#include "mfma.h"
 
 namespace hann {
 
-void batch_topk(Context &c, int metric, const float *queries, int64_t nq, const float *base,
-                int64_t nb, int dim, int k, int precision, int overfetch, int64_t *out_ids,
-                double *out_dist, hipStream_t st) {
+// Certified BF16 path for the (squared) Euclidean metrics. Results equal the
+// exact path's: the filter keeps every row whose exact FP32 rank can be at
+// most the query's sample k-th rank, and those rows are ranked exactly.
+// Queries whose candidate list overflows are rerun on the exact path.
+void certified_topk(Context &c, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
+                    const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st) {
+    Workspace &w = c.ws_bf16;
+    const int dp = (int)ceil_div(dim, 32) * 32;
+    const int metric = sqrt_out ? HANN_METRIC_EUCLIDEAN : HANN_METRIC_SQEUCLIDEAN;
+    // BF16 copies, norms, max norm and exactness flag
+    __bf16 *xb = w.get<__bf16>(0, (size_t)nb * dp);
+    float *xn = w.get<float>(1, (size_t)nb);
+    float *xf = w.get<float>(2, (size_t)nb);
+    unsigned *stat = w.get<unsigned>(3, 4);   // [0] max norm bits, [1] inexact flag
+    HANN_HIP(hipMemsetAsync(stat, 0, 4 * sizeof(unsigned), st));
+    launch_to_bf16(base, nb, dim, dp, xb, xn, xf, stat, reinterpret_cast<int *>(stat + 1), st);
+    __bf16 *qb = w.get<__bf16>(4, (size_t)nq * dp);
+    float *qn = w.get<float>(5, (size_t)nq);
+    float *qf = w.get<float>(6, (size_t)nq);
+    launch_to_bf16(queries, nq, dim, dp, qb, qn, qf, stat + 2, reinterpret_cast<int *>(stat + 1), st);
+    // an upper bound of each query's k-th rank: the hint row's exact rank, or
+    // the exact k-th rank over a sample of rows (every stride-th)
+    double *srank = w.get<double>(10, (size_t)nq * k);
+    int64_t ns = nb;
+    std::vector<int64_t> pick;
+    if (hint && k == 1) {
+        launch_hint_rank(queries, nq, dim, base, hint, srank, st);
+        ns = std::max<int64_t>(1, nb / 64);   // sizes the candidate room only
+    } else {
+        const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
+        const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+        ns = ceil_div(nb, stride);
+        pick.resize((size_t)ns);
+        for (int64_t i = 0; i < ns; ++i) pick[i] = i * stride;
+        int64_t *dpick = w.get<int64_t>(7, (size_t)ns);
+        HANN_HIP(hipMemcpyAsync(dpick, pick.data(), (size_t)ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        float *sample = w.get<float>(8, (size_t)ns * dim);
+        launch_gather_rows(base, dim, dpick, ns, sample, st);
+        int64_t *sid = w.get<int64_t>(9, (size_t)nq * k);
+        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, queries, nq, sample, ns, dim, k, sid, srank, c.ws, st);
+    }
+    float *thr = w.get<float>(11, (size_t)nq);
+    launch_thresholds(srank, k, (int)nq, qf, stat, reinterpret_cast<int *>(stat + 1), dim, thr, st);
+    // candidates: expected about k * nb / ns per query; room for 8x that
+    const int cap = (int)std::min<int64_t>(nb, std::max<int64_t>(256, 8 * ceil_div((int64_t)k * nb, ns)));
+    const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
+    int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
+    int32_t *cand = w.get<int32_t>(13, (size_t)chunk * cap);
+    HANN_HIP(hipMemsetAsync(cnt, 0, (size_t)nq * sizeof(int32_t), st));
+    for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
+        const int64_t qn_ = std::min(chunk, nq - q0);
+        launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, cnt + q0, cand, cap, st);
+        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, cnt + q0, cand, cap, k, sqrt_out, out_ids + q0 * k,
+                               out_dist + q0 * k, st);
+    }
+    // overflowed queries (and queries with fewer than k rows) take the exact path
+    std::vector<int32_t> hc((size_t)nq);
+    HANN_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HANN_HIP(hipStreamSynchronize(st));
+    std::vector<int64_t> redo;
+    for (int64_t q = 0; q < nq; ++q)
+        if (hc[q] > cap) redo.push_back(q);
+    c.bf16_fallback_queries += (int64_t)redo.size();
+    if (redo.empty()) return;
+    const int64_t nr = (int64_t)redo.size();
+    int64_t *dredo = w.get<int64_t>(7, (size_t)nr);
+    HANN_HIP(hipMemcpyAsync(dredo, redo.data(), (size_t)nr * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    float *rq = w.get<float>(8, (size_t)nr * dim);
+    launch_gather_rows(queries, dim, dredo, nr, rq, st);
+    int64_t *rid = w.get<int64_t>(9, (size_t)nr * k);
+    double *rdist = w.get<double>(10, (size_t)nr * k);
+    launch_topk_exact(metric, rq, nr, base, nb, dim, k, rid, rdist, c.ws, st);
+    for (int64_t i = 0; i < nr; ++i) {
+        HANN_HIP(hipMemcpyAsync(out_ids + redo[i] * k, rid + i * k, (size_t)k * sizeof(int64_t),
+                                hipMemcpyDeviceToDevice, st));
+        HANN_HIP(hipMemcpyAsync(out_dist + redo[i] * k, rdist + i * k, (size_t)k * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+    }
+    HANN_HIP(hipStreamSynchronize(st));
+}
+
+__global__ void k_ids_to_assign(const int64_t *__restrict__ ids, int64_t n, int32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (int32_t)ids[i];
+}
+
+void assign_bf16(Context &c, const float *data, int64_t n, const float *cents, int kc, int d, const int32_t *hint,
+                 int32_t *out, hipStream_t st) {
+    if (n == 0) return;
+    int64_t *ids = c.ws_bf16.get<int64_t>(14, (size_t)n);
+    double *dist = c.ws_bf16.get<double>(15, (size_t)n);
+    certified_topk(c, data, n, cents, kc, d, 1, hint, 0, ids, dist, st);
+    hipLaunchKernelGGL(k_ids_to_assign, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, ids, n, out);
+    HANN_LAUNCH_CHECK();
+}
+
+void batch_topk(Context &c, int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim,
+                int k, int precision, int overfetch, int64_t *out_ids, double *out_dist, hipStream_t st) {
     (void)overfetch;
-    if (precision == HANN_PREC_FP32) {
+    if (precision == HANN_PREC_FP32 || nb == 0) {
         launch_topk_exact(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, c.ws, st);
+        return;
+    }
+    if (precision == HANN_PREC_BF16) {
+        if (metric != HANN_METRIC_SQEUCLIDEAN && metric != HANN_METRIC_EUCLIDEAN)
+            fail(HANN_ERR_UNSUPPORTED, "BF16 batch top-k supports the (squared) Euclidean metrics");
+        if (nb >= (1ll << 31)) fail(HANN_ERR_UNSUPPORTED, "BF16 batch top-k needs fewer than 2^31 rows");
+        certified_topk(c, queries, nq, base, nb, dim, k, nullptr, metric == HANN_METRIC_EUCLIDEAN ? 1 : 0, out_ids,
+                       out_dist, st);
         return;
     }
     fail(HANN_ERR_UNSUPPORTED, strf("precision %d is not supported for batch top-k", precision));

@@ -9,7 +9,7 @@ namespace hann {
 // Slot map: 0-4 and 10-11 host-facing staging of the entry points, 5-9 the
 // search pipeline, 14-15 launch_topk_exact's partial lists.
 struct Workspace {
-    DevBuf<char> slots[16];
+    DevBuf<char> slots[20];
     template <typename T>
     T *get(int slot, size_t count) {
         slots[slot].resize(count * sizeof(T) + 256);

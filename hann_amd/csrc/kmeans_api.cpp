@@ -2,6 +2,7 @@
 #include <cstring>
 
 #include "kmeans.h"
+#include "batch.h"
 #include "pqivf.h"
 #include "rng.h"
 #include "runtime.h"
@@ -15,12 +16,12 @@ int hann_assign_points_dev(const float *data, size_t n, size_t d, const float *c
     HANN_API_BEGIN
     if (d == 0) fail(HANN_ERR_INVALID_ARG, "query must not be empty");
     if (k == 0) fail(HANN_ERR_INVALID_ARG, "no centroids");
-    if (precision != HANN_PREC_FP32)
+    if (precision != HANN_PREC_FP32 && precision != HANN_PREC_BF16)
         fail(HANN_ERR_UNSUPPORTED, strf("precision %d is not supported for assignment", precision));
     if (n == 0) return HANN_OK;
     Context &c = global_ctx();
     hipStream_t st = stream ? (hipStream_t)stream : c.stream;
-    launch_assign(data, (int64_t)n, (int64_t)d, centroids, (int)k, (int)d, out, st);
+    assign_points(data, (int64_t)n, centroids, (int)k, (int)d, precision, nullptr, out, st);
     if (!stream) HANN_HIP(hipStreamSynchronize(st));
     HANN_API_END
 }
@@ -30,16 +31,15 @@ int hann_assign_points(const float *data, size_t n, size_t d, const float *centr
     HANN_API_BEGIN
     if (d == 0) fail(HANN_ERR_INVALID_ARG, "query must not be empty");
     if (k == 0) fail(HANN_ERR_INVALID_ARG, "no centroids");
-    if (precision != HANN_PREC_FP32)
+    if (precision != HANN_PREC_FP32 && precision != HANN_PREC_BF16)
         fail(HANN_ERR_UNSUPPORTED, strf("precision %d is not supported for assignment", precision));
     if (n == 0) return HANN_OK;
     Context &c = global_ctx();
-    std::lock_guard<std::mutex> lk(c.mu);
     DevBuf<float> dd(n * d), dc(k * d);
     DevBuf<int32_t> da(n);
     h2d(dd.get(), data, n * d * sizeof(float), c.stream);
     h2d(dc.get(), centroids, k * d * sizeof(float), c.stream);
-    launch_assign(dd.get(), (int64_t)n, (int64_t)d, dc.get(), (int)k, (int)d, da.get(), c.stream);
+    assign_points(dd.get(), (int64_t)n, dc.get(), (int)k, (int)d, precision, nullptr, da.get(), c.stream);
     d2h(out, da.get(), n * sizeof(int32_t), c.stream);
     sync(c.stream);
     HANN_API_END
@@ -53,15 +53,15 @@ int hann_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uint
     if (d == 0) fail(HANN_ERR_INVALID_ARG, "query must not be empty");
     if (k == 0) fail(HANN_ERR_INVALID_ARG, "k must be positive");
     if (init != 0) fail(HANN_ERR_UNSUPPORTED, strf("k-means init %d is not supported", init));
-    if (precision != HANN_PREC_FP32)
+    if (precision != HANN_PREC_FP32 && precision != HANN_PREC_BF16)
         fail(HANN_ERR_UNSUPPORTED, strf("precision %d is not supported for k-means", precision));
     Context &c = global_ctx();
-    std::lock_guard<std::mutex> lk(c.mu);
     DevBuf<float> dd(n * d), dc(k * d);
     h2d(dd.get(), data, n * d * sizeof(float), c.stream);
     std::vector<uint64_t> st{stream_seed(seed, 0, 0)};
     KMeansScratch sc;
-    const int64_t kk = run_kmeans(dd.get(), (int64_t)n, (int64_t)d, (int)d, (int)k, iters, 1, st, dc.get(), sc, c.stream);
+    const int64_t kk =
+        run_kmeans(dd.get(), (int64_t)n, (int64_t)d, (int)d, (int)k, iters, 1, st, dc.get(), sc, c.stream, precision);
     d2h(out_centroids, dc.get(), (size_t)kk * d * sizeof(float), c.stream);
     sync(c.stream);
     *k_out = (size_t)kk;

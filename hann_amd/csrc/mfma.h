@@ -1,0 +1,26 @@
+// mfma.h -- launchers of the BF16 MFMA candidate pass (mfma.hip).
+#pragma once
+
+#include "common.h"
+#include "kernels.h"
+
+namespace hann {
+
+// rows -> BF16 [n][dp] (dp a multiple of 32), ||x_b||^2, ||x||, max ||x||, inexact flag
+void launch_to_bf16(const float *x, int64_t n, int d, int dp, __bf16 *out, float *norm_b, float *norm_f,
+                    unsigned *max_norm, int *inexact, hipStream_t st);
+// tau_rank[q] = exact rank of (query q, row hint[q]) -- an upper bound of its nearest row
+void launch_hint_rank(const float *queries, int64_t nq, int d, const float *base, const int32_t *hint,
+                      double *tau_rank, hipStream_t st);
+// filter thresholds from the sample's exact ranks tau_rank[q*k + k-1]
+void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm_f, const unsigned *max_norm,
+                       const int *inexact, int d, float *thr, hipStream_t st);
+// append every row with approx distance <= thr[q] to q's candidate list
+void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
+                        const float *xn, const float *thr, int32_t *cnt, int32_t *cand_row, int cap, hipStream_t st);
+// exact reference-order ranking of the candidates (queries with cnt <= cap)
+void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
+                            const int32_t *cand_row, int cap, int k, int sqrt_out, int64_t *out_ids,
+                            double *out_dist, hipStream_t st);
+
+}  // namespace hann

@@ -1,0 +1,259 @@
+// mfma.hip -- BF16 MFMA candidate pass for certified exact top-k (gfx950).
+//
+// The squared distance ||q||^2 + ||x||^2 - 2 q.x is evaluated on BF16 copies
+// of the inputs with v_mfma_f32_32x32x16_bf16 (FP32 accumulation). Every row
+// whose approximate distance is within the threshold T_q of a query becomes a
+// candidate; T_q is the query's exact k-th distance over a row sample (an
+// upper bound of its true k-th distance) widened by a rigorous bound on
+// |approx - exact FP32| (mfma.h). The candidates are then ranked in the
+// reference's exact FP32 order, so the result equals the exact path's.
+#include "common.h"
+#include "mfma.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+namespace hann {
+namespace dev {
+
+// FP32 rows -> BF16 rows padded with zeros to dp columns; norm_b = ||x_b||^2
+// of the BF16 values (FP32 sum), norm_f = ||x|| of the FP32 row; max_norm
+// (float bits, non-negative) and inexact flag gathered with atomics.
+__global__ void k_to_bf16(const float *__restrict__ x, int64_t n, int d, int dp, __bf16 *__restrict__ out,
+                          float *__restrict__ norm_b, float *__restrict__ norm_f, unsigned *__restrict__ max_norm,
+                          int *__restrict__ inexact) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float *r = x + i * d;
+    __bf16 *o = out + i * dp;
+    float sb = 0.f, sf = 0.f;
+    int bad = 0;
+    for (int e = 0; e < dp; ++e) {
+        float v = e < d ? r[e] : 0.f;
+        __bf16 b = (__bf16)v;
+        float vb = (float)b;
+        bad |= vb != v;
+        o[e] = b;
+        sb = __builtin_fmaf(vb, vb, sb);
+        sf = __builtin_fmaf(v, v, sf);
+    }
+    norm_b[i] = sb;
+    const float nf = __builtin_sqrtf(sf);
+    norm_f[i] = nf;
+    atomicMax(max_norm, __float_as_uint(nf));
+    if (bad) atomicOr(inexact, 1);
+}
+
+// Per-query filter threshold from the sample's exact k-th rank tau.
+//   e <= tau (exact FP32, reference order) implies D <= tau' = tau/(1-(d+2)u)
+//   BF16 rounding moves sqrt(D) by at most delta = 2^-9 (|q| + max|x|)
+//   (zero when every input is a BF16 value), FP32 accumulation of the
+//   expansion adds at most (d+2) u (|q|+max|x|)^2 (1+2^-8)^2 for any
+//   summation order; E2 takes twice that, since the MFMA unit's internal
+//   accumulation order is not specified
+//   so approx <= (sqrt(tau') + delta)^2 + E2 =: T, widened by 2^-20.
+__global__ void k_thresholds(const double *__restrict__ tau_rank, int k, int nq, const float *__restrict__ qnorm_f,
+                             const unsigned *__restrict__ max_norm, const int *__restrict__ inexact, int d,
+                             float *__restrict__ thr) {
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const double t = tau_rank[(int64_t)q * k + (k - 1)];
+    if (!(t < 1e300)) {
+        thr[q] = __builtin_inff();
+        return;
+    }
+    const double u = 1.0 / 16777216.0;
+    const double a = qnorm_f[q], b = (double)__uint_as_float(*max_norm);
+    const double tp = t / (1.0 - (d + 2) * u);
+    const double delta = *inexact ? (a + b) / 512.0 : 0.0;
+    const double e2 = 2.0 * (d + 2) * u * (a + b) * (a + b) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
+    const double s = __builtin_sqrt(tp) + delta;
+    thr[q] = (float)((s * s + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
+}
+
+// tau_rank[q] = exact reference-order rank of (query q, row hint[q])
+__global__ void k_hint_rank(const float *__restrict__ queries, int64_t nq, int d, const float *__restrict__ base,
+                            const int32_t *__restrict__ hint, double *__restrict__ tau_rank) {
+    const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    tau_rank[q] = (double)sane_key(
+        sqdiff_exact(PtrLoad{queries + q * d}, PtrLoad{base + (int64_t)hint[q] * d}, d));
+}
+
+constexpr int MF_TILE = 128;   // queries and rows per workgroup tile
+constexpr int MF_KC = 128;     // K chunk held in LDS (bf16 columns)
+constexpr int MF_PAD = 8;      // LDS row padding (bf16) against bank conflicts
+constexpr int MF_LD = MF_KC + MF_PAD;
+
+// One workgroup: 128 queries x 128 rows, 4 waves of 64 x 64 (2 x 2 tiles of
+// 32 x 32). Lane layout of v_mfma_f32_32x32x16_bf16: A lane l holds row
+// l%32, k = 8*(l/32) .. +7; B lane l holds column l%32, the same k; D
+// register r of lane l is row 8*(r/4) + 4*(l/32) + r%4, column l%32.
+__global__ void __launch_bounds__(256) k_mfma_filter(const __bf16 *__restrict__ Q, int64_t nq,
+                                                     const __bf16 *__restrict__ X, int64_t nb, int dp,
+                                                     const float *__restrict__ qn, const float *__restrict__ xn,
+                                                     const float *__restrict__ thr, int32_t *__restrict__ cnt,
+                                                     int32_t *__restrict__ cand_row, int cap) {
+    __shared__ __attribute__((aligned(16))) __bf16 sA[MF_TILE * MF_LD];
+    __shared__ __attribute__((aligned(16))) __bf16 sB[MF_TILE * MF_LD];
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int wm = wave / 2, wn = wave % 2;
+    const int64_t r0 = (int64_t)blockIdx.x * MF_TILE;
+    const int64_t q0 = (int64_t)blockIdx.y * MF_TILE;
+
+    f32x16 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+    for (int k0 = 0; k0 < dp; k0 += MF_KC) {
+        const int kc = dp - k0 < MF_KC ? dp - k0 : MF_KC;   // multiple of 16
+        const int vecs = kc / 8;
+        __syncthreads();
+        for (int v = tid; v < MF_TILE * vecs; v += 256) {
+            const int row = v / vecs, c8 = v % vecs;
+            const int64_t gq = q0 + row, gr = r0 + row;
+            bf16x8 a = {}, b = {};
+            if (gq < nq) a = *reinterpret_cast<const bf16x8 *>(Q + gq * dp + k0 + c8 * 8);
+            if (gr < nb) b = *reinterpret_cast<const bf16x8 *>(X + gr * dp + k0 + c8 * 8);
+            *reinterpret_cast<bf16x8 *>(&sA[row * MF_LD + c8 * 8]) = a;
+            *reinterpret_cast<bf16x8 *>(&sB[row * MF_LD + c8 * 8]) = b;
+        }
+        __syncthreads();
+        const int kl = (lane / 32) * 8;
+        for (int kk = 0; kk < kc; kk += 16) {
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+                a[i] = *reinterpret_cast<const bf16x8 *>(&sA[(wm * 64 + i * 32 + lane % 32) * MF_LD + kk + kl]);
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                b[j] = *reinterpret_cast<const bf16x8 *>(&sB[(wn * 64 + j * 32 + lane % 32) * MF_LD + kk + kl]);
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int j = 0; j < 2; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
+    }
+    // epilogue: approximate distance, threshold test, append
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int64_t gr = r0 + wn * 64 + j * 32 + lane % 32;
+            const float xr = gr < nb ? xn[gr] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const int64_t gq = q0 + wm * 64 + i * 32 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                if (gq >= nq || gr >= nb) continue;
+                const float approx = (qn[gq] + xr) - 2.f * acc[i][j][r];
+                if (approx <= thr[gq]) {
+                    const int slot = atomicAdd(&cnt[gq], 1);
+                    if (slot < cap) cand_row[gq * cap + slot] = (int32_t)gr;
+                }
+            }
+        }
+}
+
+// Exact reference-order rank of every candidate, then the k best by
+// (rank, row). One wave per query. Queries whose candidate list overflowed
+// write nothing (the host reruns them on the exact path).
+__global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict__ queries, int d,
+                                                        const float *__restrict__ base,
+                                                        const int32_t *__restrict__ cnt,
+                                                        const int32_t *__restrict__ cand_row, int cap, int k,
+                                                        int ccap, int sqrt_out, int64_t *__restrict__ out_ids,
+                                                        double *__restrict__ out_dist) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    float *s_keys = reinterpret_cast<float *>(s_raw);
+    int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + ccap);
+    const int lane = threadIdx.x;
+    const int64_t q = blockIdx.x;
+    const int n = cnt[q];
+    if (n > cap) return;
+    WaveTopK sel;
+    sel.init(s_keys, s_ids, k, ccap);
+    const float *qv = queries + q * d;
+    for (int b = 0; b < n; b += kWave) {
+        const int t = b + lane;
+        const bool valid = t < n;
+        float key = 0.f;
+        int64_t row = 0;
+        if (valid) {
+            row = cand_row[q * cap + t];
+            key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
+        }
+        sel.push(key, row, valid);
+    }
+    const int got = sel.finish();
+    for (int t = lane; t < k; t += kWave) {
+        double dd = t < got ? (double)sel.keys[t] : __builtin_inf();
+        if (sqrt_out && t < got) dd = __builtin_sqrt(dd);
+        out_ids[q * k + t] = t < got ? sel.ids[t] : (int64_t)-1;
+        out_dist[q * k + t] = dd;
+    }
+}
+
+}  // namespace dev
+
+void launch_to_bf16(const float *x, int64_t n, int d, int dp, __bf16 *out, float *norm_b, float *norm_f,
+                    unsigned *max_norm, int *inexact, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(dev::k_to_bf16, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, x, n, d, dp, out, norm_b,
+                       norm_f, max_norm, inexact);
+    HANN_LAUNCH_CHECK();
+}
+
+void launch_hint_rank(const float *queries, int64_t nq, int d, const float *base, const int32_t *hint,
+                      double *tau_rank, hipStream_t st) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(dev::k_hint_rank, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, st, queries, nq, d, base,
+                       hint, tau_rank);
+    HANN_LAUNCH_CHECK();
+}
+
+void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm_f, const unsigned *max_norm,
+                       const int *inexact, int d, float *thr, hipStream_t st) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(dev::k_thresholds, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, st, tau_rank, k, nq,
+                       qnorm_f, max_norm, inexact, d, thr);
+    HANN_LAUNCH_CHECK();
+}
+
+void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
+                        const float *xn, const float *thr, int32_t *cnt, int32_t *cand_row, int cap, hipStream_t st) {
+    if (nq == 0 || nb == 0) return;
+    const int64_t rt = ceil_div(nb, dev::MF_TILE), qt = ceil_div(nq, dev::MF_TILE);
+    for (int64_t y0 = 0; y0 < qt; y0 += 65535) {
+        const int64_t yn = qt - y0 < 65535 ? qt - y0 : 65535;
+        hipLaunchKernelGGL(dev::k_mfma_filter, dim3((unsigned)rt, (unsigned)yn), dim3(256), 0, st,
+                           Q + y0 * dev::MF_TILE * dp, nq - y0 * dev::MF_TILE, X, nb, dp, qn + y0 * dev::MF_TILE, xn,
+                           thr + y0 * dev::MF_TILE, cnt + y0 * dev::MF_TILE,
+                           cand_row + y0 * dev::MF_TILE * (int64_t)cap, cap);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
+                            const int32_t *cand_row, int cap, int k, int sqrt_out, int64_t *out_ids,
+                            double *out_dist, hipStream_t st) {
+    if (nq == 0) return;
+    const int ccap = 2 * kp_for(k);
+    const size_t shm = (size_t)ccap * 12 + 16;
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
+    for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
+        const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
+        hipLaunchKernelGGL(dev::k_rank_candidates, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d, base,
+                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, k, ccap, sqrt_out, out_ids + q0 * k,
+                           out_dist + q0 * k);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+}  // namespace hann

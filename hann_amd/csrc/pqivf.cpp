@@ -23,12 +23,24 @@
 #include "ivf.h"
 #include "kmeans.h"
 #include "rng.h"
+#include "scan_fused.h"
 
 namespace hann {
 
 // ------------------------------------------------------------ k-means --
+void assign_points(const float *data, int64_t n, const float *cents, int k, int d, int precision, const int32_t *hint,
+                   int32_t *out, hipStream_t st) {
+    if (precision == HANN_PREC_BF16 && d >= 32) {
+        Context &c = global_ctx();
+        std::lock_guard<std::mutex> lk(c.mu);
+        assign_bf16(c, data, n, cents, k, d, hint, out, st);
+        return;
+    }
+    launch_assign(data, n, d, cents, k, d, out, st);
+}
+
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
-                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st) {
+                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st, int precision) {
     const int64_t keff = n < k ? n : k;
     if (keff <= 0) return 0;
     // seeds: the first keff entries of a random permutation (pqivf/index.go:804-810)
@@ -45,8 +57,13 @@ int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, i
     DevBuf<int32_t> assign((size_t)groups * n);
     std::vector<int64_t> offs((size_t)groups * (keff + 1));
     std::vector<int64_t> reseed((size_t)groups * keff);
+    const bool bf16 = precision == HANN_PREC_BF16 && groups == 1 && dsub >= 32 && row_stride == dsub;
     for (int it = 0; it < iters; ++it) {
-        if (dsub < 32) {
+        if (bf16) {
+            // the previous assignment's centroid bounds every point's nearest
+            assign_points(data, n, cents, (int)keff, dsub, HANN_PREC_BF16, it > 0 ? assign.get() : nullptr,
+                          assign.get(), st);
+        } else if (dsub < 32) {
             launch_assign_groups(data, n, row_stride, cents, keff * dsub, nullptr, (int)keff, dsub, groups,
                                  assign.get(), n, st);
         } else {
@@ -380,12 +397,13 @@ void PQIVF::train() {
     }
     DevBuf<float> dC((size_t)coarse_k * dim);
     std::vector<uint64_t> st0{stream_seed(seed, epoch, 0)};
-    const int64_t new_kc = run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream);
+    const int64_t new_kc =
+        run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream, train_precision);
     dS.release();
 
     // final assignment of every vector; lists in ascending id order
     DevBuf<int32_t> dA((size_t)n);
-    launch_assign(dX.get(), n, dim, dC.get(), (int)new_kc, d, dA.get(), stream);
+    assign_points(dX.get(), n, dC.get(), (int)new_kc, d, train_precision, nullptr, dA.get(), stream);
     std::vector<int32_t> A((size_t)n);
     HANN_HIP(hipMemcpyAsync(A.data(), dA.get(), A.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
 
@@ -528,14 +546,32 @@ void PQIVF::sync_device() {
     up(d_cb, codebooks);
     up(d_cb_len, cb_len);
     (void)s;
+    tables_ready = false;
     HANN_HIP(hipStreamSynchronize(stream));
     dirty = false;
 }
 
 // --------------------------------------------------------------- search --
+__global__ void k_min_i32(const int32_t *__restrict__ v, int64_t n, int32_t *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) atomicMin(out, v[i]);
+}
+
+int PQIVF::device_min(const int32_t *v, int64_t n, hipStream_t st, Workspace &w) {
+    int32_t *m = w.get<int32_t>(16, 1);
+    HANN_HIP(hipMemsetAsync(m, 0x7F, sizeof(int32_t), st));   // 0x7F7F7F7F: above any nprobe
+    hipLaunchKernelGGL(k_min_i32, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, v, n, m);
+    HANN_LAUNCH_CHECK();
+    int32_t r = 0;
+    HANN_HIP(hipMemcpyAsync(&r, m, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HANN_HIP(hipStreamSynchronize(st));
+    return r;
+}
+
 void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,
                           const hann_search_params &p, float *out_key, int64_t *out_id, int32_t *out_cnt,
-                          hipStream_t st) {
+                          hipStream_t st, int min_nprobe, Workspace *wsp) {
+    Workspace &w = wsp ? *wsp : ws;
     // the caller holds the index lock and has synced the device mirror
     if (nq == 0) return;
     const int d = (int)dim;
@@ -544,7 +580,9 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     if (np < 1) np = 1;
     const int R = p.rerank > 0 ? p.rerank : 0;
     // coarse step: the np nearest centroids by (rank, index); nearestCentroids
-    // (pqivf/index.go:184-206). Probing every list needs no selection.
+    // (pqivf/index.go:184-206). When every query probes every list no
+    // selection is needed; otherwise the per-query prefix of the sorted
+    // probe list is each query's set.
     std::array<hipEvent_t, 4> ev{};
     if (profile) {
         for (auto &e : ev) HANN_HIP(hipEventCreate(&e));
@@ -552,20 +590,28 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     }
     int64_t *probes;
     int probe_stride;
-    if (np >= K) {
+    double *probe_rank = nullptr;
+    bool all_probe = np >= K;
+    if (all_probe && dnp) {
+        if (min_nprobe < 0) min_nprobe = device_min(dnp, nq, st, w);
+        all_probe = min_nprobe >= K;
+    }
+    if (all_probe) {
         if ((int64_t)all_lists.n < (int64_t)K) {
             std::vector<int64_t> h((size_t)K);
             std::iota(h.begin(), h.end(), 0);
             all_lists.resize((size_t)K);
             HANN_HIP(hipMemcpyAsync(all_lists.get(), h.data(), h.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+            HANN_HIP(hipStreamSynchronize(st));
         }
         probes = all_lists.get();
         probe_stride = 0;
+        np = K;
     } else {
         if (np > 512) fail(HANN_ERR_UNSUPPORTED, strf("nprobe %d above 512 (and below coarseK %d) is not supported", np, K));
-        probes = ws.get<int64_t>(5, (size_t)nq * np);
-        double *pd = ws.get<double>(6, (size_t)nq * np);
-        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, pd, ws, st);
+        probes = w.get<int64_t>(5, (size_t)nq * np);
+        probe_rank = w.get<double>(6, (size_t)nq * np);
+        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, probe_rank, w, st);
         probe_stride = np;
     }
     const int selk = R > 0 ? kmax * R : kmax;
@@ -606,31 +652,97 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         a.any_uncoded = 1;
         mode = SCAN_MODE_ADC_F32;
     }
+    const bool fused = p.fused && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
+    auto scan = [&](const ScanArgs &args) {
+        if (fused)
+            run_fused_scan(args, nq, mode == SCAN_MODE_ADC_F16 ? 1 : 0, np, probe_rank, st, w);
+        else
+            launch_scan(mode, code_bytes, args, nq, st);
+    };
     if (R == 0) {
         a.out_key = out_key;
         a.out_id = out_id;
         a.out_cnt = out_cnt;
         a.out_stride = kmax;
-        launch_scan(mode, code_bytes, a, nq, st);
+        scan(a);
         if (profile) {
             HANN_HIP(hipEventRecord(ev[2], st));
             HANN_HIP(hipEventRecord(ev[3], st));
         }
     } else {
-        float *ck = ws.get<float>(7, (size_t)nq * selk);
-        int64_t *cpos = ws.get<int64_t>(8, (size_t)nq * selk);
-        int32_t *ccnt = ws.get<int32_t>(9, (size_t)nq);
+        float *ck = w.get<float>(7, (size_t)nq * selk);
+        int64_t *cpos = w.get<int64_t>(8, (size_t)nq * selk);
+        int32_t *ccnt = w.get<int32_t>(9, (size_t)nq);
         a.out_key = ck;
         a.out_id = cpos;
         a.out_cnt = ccnt;
         a.out_stride = selk;
-        launch_scan(mode, code_bytes, a, nq, st);
+        scan(a);
         if (profile) HANN_HIP(hipEventRecord(ev[2], st));
         launch_rerank(dq, d, nq, cpos, ccnt, selk, dk, kmax, kmax, d_vecs.get(), d_ids.get(), out_key, out_id, out_cnt,
                       kmax, st);
         if (profile) HANN_HIP(hipEventRecord(ev[3], st));
     }
     if (profile) t_events.push_back(ev);
+}
+
+void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int lut16, int maxp, const double *probe_rank,
+                           hipStream_t st, Workspace &w) {
+    const int K = (int)lists.size();
+    if (!tables_ready) {
+        d_T.resize((size_t)K * m * pmax + 1);
+        launch_list_tables(d_coarse.get(), K, (int)dim, (int)m, (int)pmax, d_cb.get(), d_cb_len.get(), d_T.get(), st);
+        tables_ready = true;
+    }
+    if (num_cus == 0) {
+        int dev = 0;
+        HANN_HIP(hipGetDevice(&dev));
+        HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    FusedArgs f{};
+    f.T = d_T.get();
+    f.probe_rank = a.probe_stride > 0 ? probe_rank : nullptr;
+    f.order = w.get<int32_t>(12, (size_t)nq);
+    f.counter = w.get<int32_t>(13, 1);
+    auto grid_for = [&](int wpq) {
+        const size_t shm = fused_lds_bytes(wpq, lut16 ? 2 : 4, (int)dim, (int)m, (int)pmax, a.cap);
+        int per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(shm, 1));
+        per_cu = std::min(per_cu, 16 / wpq);
+        return (int)std::min<int64_t>(nq, (int64_t)num_cus * per_cu);
+    };
+    auto fits = [&](int wpq) { return fused_lds_bytes(wpq, lut16 ? 2 : 4, (int)dim, (int)m, (int)pmax, a.cap) <= 160 * 1024; };
+    // the tuner: time every fitting waves-per-query choice once on this batch
+    // shape, keep the fastest (results are identical for every choice)
+    const uint64_t key = ((uint64_t)a.cap << 20) ^ ((uint64_t)lut16 << 16) ^ (uint64_t)(maxp >= 64 ? 2 : maxp >= 16 ? 1 : 0);
+    auto it = tuned_wpq.find(key);
+    int wpq = 0;
+    if (it != tuned_wpq.end()) {
+        wpq = it->second;
+    } else {
+        float best = 1e30f;
+        for (int w : {1, 2, 4}) {
+            if (!fits(w)) continue;
+            hipEvent_t e0, e1;
+            HANN_HIP(hipEventCreate(&e0));
+            HANN_HIP(hipEventCreate(&e1));
+            HANN_HIP(hipEventRecord(e0, st));
+            launch_scan_fused(a, f, nq, w, lut16, code_bytes, maxp, grid_for(w), st);
+            HANN_HIP(hipEventRecord(e1, st));
+            HANN_HIP(hipEventSynchronize(e1));
+            float ms = 0;
+            HANN_HIP(hipEventElapsedTime(&ms, e0, e1));
+            (void)hipEventDestroy(e0);
+            (void)hipEventDestroy(e1);
+            if (ms < best) {
+                best = ms;
+                wpq = w;
+            }
+        }
+        if (wpq == 0) fail(HANN_ERR_UNSUPPORTED, "fused scan: no configuration fits in LDS");
+        tuned_wpq[key] = wpq;
+        last_tune = strf("fused scan tuned: waves/query %d (%.3f ms)", wpq, best);
+    }
+    launch_scan_fused(a, f, nq, wpq, lut16, code_bytes, maxp, grid_for(wpq), st);
 }
 
 void PQIVF::resolve_timers() {
@@ -686,7 +798,10 @@ void PQIVF::search_host(const float *queries, int64_t nq, const int32_t *k, cons
     HANN_HIP(hipMemcpyAsync(dq, queries, (size_t)nq * dim * sizeof(float), hipMemcpyHostToDevice, st));
     HANN_HIP(hipMemcpyAsync(dk, k, (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice, st));
     if (np) HANN_HIP(hipMemcpyAsync(dn, np, (size_t)nq * sizeof(int32_t), hipMemcpyHostToDevice, st));
-    search_device(dq, nq, dk, dn, kmax, pmx, p, ok, oi, oc, st);
+    int pmin = np ? INT32_MAX : (int)nprobe;
+    if (np)
+        for (int64_t q = 0; q < nq; ++q) pmin = std::min(pmin, np[q]);
+    search_device(dq, nq, dk, dn, kmax, pmx, p, ok, oi, oc, st, pmin);
     std::vector<float> hk((size_t)nq * kmax);
     HANN_HIP(hipMemcpyAsync(hk.data(), ok, hk.size() * sizeof(float), hipMemcpyDeviceToHost, st));
     HANN_HIP(hipMemcpyAsync(out_ids, oi, (size_t)nq * kmax * sizeof(int64_t), hipMemcpyDeviceToHost, st));
@@ -893,10 +1008,6 @@ void PQIVF::load(const void *buf, size_t len) {
 
 // =================================================================== C-ABI
 using namespace hann;
-
-struct hann_pqivf {
-    PQIVF ix;
-};
 
 extern "C" {
 

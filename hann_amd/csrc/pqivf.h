@@ -5,9 +5,11 @@
 #include <atomic>
 #include <map>
 #include <shared_mutex>
+#include <string>
 #include <unordered_map>
 #include <vector>
 
+#include "ivf.h"
 #include "kmeans.h"
 #include "runtime.h"
 
@@ -60,8 +62,14 @@ struct PendingStore {
 };
 
 // runKMeans over `groups` problems on device data; returns min(k, n)
+// precision HANN_PREC_BF16 (single group, dsub >= 32): certified BF16
+// assignment, identical results
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
-                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st);
+                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st,
+                   int precision = HANN_PREC_FP32);
+// assignPoints in either precision (BF16 needs a contiguous [n][d] layout)
+void assign_points(const float *data, int64_t n, const float *cents, int k, int d, int precision,
+                   const int32_t *hint, int32_t *out, hipStream_t st);
 
 struct PQIVF {
     // parameters (pqivf/index.go:36-54)
@@ -95,6 +103,12 @@ struct PQIVF {
     DevBuf<uint8_t> d_codes, d_has_codes;
     DevBuf<float> d_vecs, d_coarse, d_cb, scratch_f;
     DevBuf<unsigned long long> d_fb_counter;
+    // fused scan: list tables T[K][m][P] (built on first use) and tuned waves per query
+    DevBuf<float> d_T;
+    bool tables_ready = false;
+    std::map<uint64_t, int> tuned_wpq;   // key: (cap, lut16, maxp bucket)
+    int num_cus = 0;
+    std::string last_tune;
 
     PQIVF();
     ~PQIVF();
@@ -116,10 +130,14 @@ struct PQIVF {
     void sync_device();
     void check_searchable(int64_t k, size_t len) const;
     void search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,
-                       const hann_search_params &p, float *out_key, int64_t *out_id, int32_t *out_cnt, hipStream_t st);
+                       const hann_search_params &p, float *out_key, int64_t *out_id, int32_t *out_cnt, hipStream_t st,
+                       int min_nprobe = -1, Workspace *wsp = nullptr);
+    int device_min(const int32_t *v, int64_t n, hipStream_t st, Workspace &w);
     void search_host(const float *queries, int64_t nq, const int32_t *k, const int32_t *np,
                      const hann_search_params &p, int64_t *out_ids, double *out_dist, int32_t *out_counts);
     int64_t fallback_count();
+    void run_fused_scan(const ScanArgs &a, int64_t nq, int lut16, int maxp, const double *probe_rank,
+                        hipStream_t st, Workspace &w);
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;
@@ -132,3 +150,8 @@ struct PQIVF {
 };
 
 }  // namespace hann
+
+// the C-ABI handle (include/hann_amd.h)
+struct hann_pqivf {
+    hann::PQIVF ix;
+};

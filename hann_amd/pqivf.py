@@ -80,6 +80,11 @@ def WithPQTrainSample(n):
     return _opt("pq_train_sample", n)
 
 
+def WithTrainPrecision(p):
+    """_lib.PREC_BF16: certified BF16 MFMA assignment in Train (same results)."""
+    return _opt("train_precision", p)
+
+
 def New(dimension, *opts):
     """pqivf.New (pqivf/index.go:106-146)."""
     lib = _lib.load()

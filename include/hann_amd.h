@@ -239,6 +239,55 @@ int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, 
 int hann_pqivf_export_lists(hann_pqivf *idx, int64_t *list_off, int64_t *ids, int32_t *codes,
                             float *vecs);
 
+/* ------------------------------------------------------------ serving --
+ * A request loop over a trained, device-resident index (build extension;
+ * the reference has no server). Requests of a few queries each, with their
+ * own k and nprobe, are batched continuously: up to `slots` batches run on
+ * the device at once, and a batch is admitted when a slot is free and its
+ * device buffers fit in the HBM budget. Mutations submitted to the server
+ * run in arrival order after every earlier batch and before every later one
+ * (the reader/writer rule of pqivf/index.go:37). */
+typedef struct hann_server hann_server;
+
+typedef struct {
+    int32_t max_batch;            /* queries per launched batch (default 4096) */
+    int32_t slots;                /* batches in flight (default 2, at most 8) */
+    int64_t hbm_budget;           /* bytes of batch buffers in flight (default 1 GiB) */
+    hann_search_params params;    /* scan of every batch (default fused ADC FP16) */
+} hann_server_options;
+
+typedef struct {
+    int64_t completed_queries, completed_requests, batches, mutations;
+    int64_t queue_depth, in_flight, budget_in_use, budget_peak;
+} hann_server_counters;
+
+void hann_server_default_options(hann_server_options *o);
+int hann_server_create(hann_pqivf *idx, const hann_server_options *o, hann_server **out);
+/* drains every queued and in-flight request, then frees */
+void hann_server_free(hann_server *srv);
+int hann_server_submit(hann_server *srv, const float *queries, size_t nq, const int32_t *k,
+                       const int32_t *nprobe, uint64_t *ticket);
+int hann_server_submit_bulk_add(hann_server *srv, const int64_t *ids, const float *flat, size_t n,
+                                uint64_t *ticket);
+int hann_server_submit_bulk_delete(hann_server *srv, const int64_t *ids, size_t n, uint64_t *ticket);
+/* Results of a finished request ([nq][kmax], kmax = the request's largest
+ * k; distances are true distances). wait == 0 returns HANN_ERR_EMPTY while
+ * the request runs. A ticket can be collected once. latency_ms (may be NULL)
+ * is submit-to-completion time. */
+int hann_server_poll(hann_server *srv, uint64_t ticket, int wait, int64_t *out_ids, double *out_dist,
+                     int32_t *out_counts, double *latency_ms);
+int hann_server_stats(hann_server *srv, hann_server_counters *out);
+/* Open-loop load driver: request r is submitted at at_us[r] after the call
+ * starts (kind 0 search of q_cnt[r] queries from q_off[r], 1 bulk add of the
+ * next q_cnt[r] rows of add_ids/add_vecs, 2 bulk delete of the next q_cnt[r]
+ * del_ids). Writes latencies, search results ([query][kmax_out]) and the
+ * wall time of the run. */
+int hann_server_replay(hann_server *srv, size_t nreq, const double *at_us, const int32_t *kind,
+                       const int64_t *q_off, const int32_t *q_cnt, const float *queries, const int32_t *k,
+                       const int32_t *nprobe, const int64_t *add_ids, const float *add_vecs,
+                       const int64_t *del_ids, int32_t kmax_out, int64_t *out_ids, double *out_dist,
+                       double *latency_ms, double *wall_s);
+
 /* ------------------------------------------ test hooks (export_test.go) -- */
 int hann_pqivf_is_trained(hann_pqivf *idx);
 /* sizes: *k = coarse centroids, *m subquantizers, *p = max codebook rows */

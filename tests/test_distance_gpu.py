@@ -118,3 +118,25 @@ def test_batch_topk_other_metrics(hann, oracle, mid):
     wid, wrank = oracle.batch_topk(mid, q, base, 20, nthreads=8)
     assert np.array_equal(ids, wid)
     assert np.array_equal(dist, wrank.astype(np.float64))
+
+
+@pytest.mark.parametrize("nq,nb,dim,k,kind", [(300, 20000, 128, 100, "int"), (130, 50000, 96, 10, "real"),
+                                              (64, 4096, 100, 50, "real"), (5, 300, 32, 10, "real"),
+                                              (200, 30000, 784, 20, "real"), (1000, 8000, 64, 1, "int")])
+def test_batch_topk_bf16_certified_equals_exact(hann, oracle, nq, nb, dim, k, kind):
+    """BF16 MFMA candidate pass + exact FP32 ranking: ids and distances are
+    identical to the exact path (the filter threshold is a proven bound)."""
+    from hann_amd import _lib, core
+    rng = np.random.default_rng(nq + nb + dim)
+    if kind == "int":
+        lam = rng.uniform(10, 30, dim)
+        base = np.minimum(rng.poisson(lam, (nb, dim)), 255).astype(np.float32)
+        q = np.minimum(rng.poisson(lam, (nq, dim)), 255).astype(np.float32)
+    else:
+        cen = rng.standard_normal((50, dim)).astype(np.float32) * 4
+        base = (cen[rng.integers(0, 50, nb)] + rng.standard_normal((nb, dim))).astype(np.float32)
+        q = (cen[rng.integers(0, 50, nq)] + rng.standard_normal((nq, dim))).astype(np.float32)
+    ids, dist = core.batch_topk(core.Euclidean, q, base, k, precision=_lib.PREC_BF16)
+    wid, wrank = oracle.batch_topk(0, q, base, k, nthreads=8)
+    assert np.array_equal(ids, wid)
+    assert np.array_equal(dist, np.sqrt(wrank.astype(np.float64)))

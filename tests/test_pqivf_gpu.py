@@ -486,3 +486,79 @@ def test_reference_trained_file_loads_and_searches(hann, oracle, gob_fixture):
     assert np.array_equal(assign, cl)
     for j, e in enumerate(ents):
         assert pq.EncodeVector(idx, ref.vecs[j], int(cl[j])) == e["Codes"]
+
+
+# ------------------------------------------------------ BF16 assignment --
+@pytest.mark.parametrize("n,d,k,spread", [(5000, 128, 256, 5.0), (3000, 96, 1000, 1.0), (2000, 64, 64, 20.0),
+                                          (700, 33, 70, 3.0), (4096, 128, 4096, 5.0)])
+def test_assign_points_bf16_identical(hann, oracle, n, d, k, spread):
+    """Certified BF16 MFMA assignment equals the exact FP32 argmin, ties to
+    the lowest index included."""
+    from hann_amd import _lib
+    rng = np.random.default_rng(n + k)
+    cen = rng.standard_normal((max(k // 4, 1), d)).astype(np.float32) * spread
+    cents = (cen[rng.integers(0, len(cen), k)] + rng.standard_normal((k, d)) * 0.5).astype(np.float32)
+    cents[1] = cents[0]          # a duplicated centroid: ties must go to index 0
+    data = (cents[rng.integers(0, k, n)] + rng.standard_normal((n, d))).astype(np.float32)
+    got = P().AssignPoints(data, cents, precision=_lib.PREC_BF16)
+    assert np.array_equal(got, oracle.assign(data, cents, nthreads=8))
+
+
+def test_kmeans_bf16_identical(hann, oracle):
+    from hann_amd import _lib
+    rng = np.random.default_rng(77)
+    cen = rng.standard_normal((64, 64)).astype(np.float32) * 5
+    data = (cen[rng.integers(0, 64, 20000)] + rng.standard_normal((20000, 64))).astype(np.float32)
+    got = P().KMeans(data, 128, 6, 99, precision=_lib.PREC_BF16)
+    want, _ = oracle.kmeans(data, 128, 6, oracle.stream_seed(99, 0, 0), nthreads=8)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_train_bf16_identical(hann, oracle):
+    """Train with the BF16 coarse assignment: the same index, bit for bit."""
+    from hann_amd import _lib, synthetic
+    pq = P()
+    x, _ = synthetic.config1(n=8000)
+    ids = np.arange(8000, dtype=np.int64)
+    idx = pq.New(64, pq.WithCoarseK(64), pq.WithNumSubquantizers(8), pq.WithPQK(64), pq.WithKMeansIters(6),
+                 pq.WithSeed(13), pq.WithTrainPrecision(_lib.PREC_BF16))
+    idx.bulk_add_arrays(ids, x)
+    idx.Train()
+    ref = oracle.train(ids, x, 64, 8, 64, 6, seed=13, epoch=0, nthreads=8)
+    check_train_parity(idx, ref)
+
+
+@pytest.mark.parametrize("mode,fused,rerank", [(1, 1, 0), (2, 1, 0), (1, 1, 3), (2, 1, 2)])
+def test_fused_scan_matches_unfused(hann, oracle, mode, fused, rerank):
+    """The persistent scan with precomputed list tables computes the same
+    distances as the per-query ADC scan up to FP32 (or FP16) rounding; with
+    a rerank both return the exact reference-order distances."""
+    from hann_amd import synthetic
+    pq = P()
+    x, qs = synthetic.config1(n=20000, q=300)
+    idx = pq.New(64, pq.WithCoarseK(64), pq.WithNumSubquantizers(8), pq.WithPQK(256), pq.WithKMeansIters(5),
+                 pq.WithSeed(4))
+    idx.bulk_add_arrays(np.arange(20000), x)
+    idx.Train()
+    rng = np.random.default_rng(3)
+    ks = rng.choice([1, 10, 50, 100], 300).astype(np.int32)
+    nps = rng.choice([1, 4, 8, 16, 64], 300).astype(np.int32)
+    a_ids, a_d, a_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=0)
+    b_ids, b_d, b_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=fused)
+    assert np.array_equal(a_c, b_c)
+    for qi in range(300):
+        kq = int(ks[qi])
+        if rerank:
+            # both return exact reference-order distances for their ids; the
+            # over-fetched candidate sets may differ at the margin
+            for r in range(0, kq, max(1, kq // 5)):
+                j = int(b_ids[qi, r])
+                assert b_d[qi, r] == np.sqrt(np.float64(oracle.pair(0, qs[qi], x[j])))
+            assert np.all(np.diff(b_d[qi, :kq]) >= 0)
+        else:
+            tol = 1e-4 * (1 + a_d[qi, :kq] ** 2) if mode == 1 else 4e-3 * (1 + a_d[qi, :kq] ** 2)
+            assert np.all(np.abs(a_d[qi, :kq] ** 2 - b_d[qi, :kq] ** 2) <= tol)
+        agree = len(set(a_ids[qi, :kq].tolist()) & set(b_ids[qi, :kq].tolist())) / kq
+        assert agree >= 0.8
+    both = [len(set(a_ids[q, :ks[q]].tolist()) & set(b_ids[q, :ks[q]].tolist())) / ks[q] for q in range(300)]
+    assert np.mean(both) >= 0.95
