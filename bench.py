@@ -149,9 +149,111 @@ def cpu_baseline(idx, q, k, nprobe, budget_s):
                       f"(decode-and-measure, no rerank), {threads} threads, {el:.1f} s"}
 
 
+def bench_topk(a, torch, lib):
+    """Config 2: 1,000,000 x 128 integer-valued float32 base, 10,000 queries,
+    squared Euclidean, k = 100. Exact FP32 (reference order) vs the certified
+    BF16 MFMA candidate pass + exact FP32 ranking (identical results)."""
+    from hann_amd import _lib, synthetic
+    import oracle
+    base, qs = synthetic.config2(n=a.n_topk, q=a.q_topk)
+    bd = torch.from_numpy(base).cuda()
+    qd = torch.from_numpy(qs).cuda()
+    k = 100
+    out = {}
+    stream = torch.cuda.Stream()
+    for name, prec in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16)):
+        ids = torch.empty((qs.shape[0], k), dtype=torch.int64, device="cuda")
+        dist = torch.empty((qs.shape[0], k), dtype=torch.float64, device="cuda")
+        def run():
+            _lib.check(lib.hann_batch_topk_dev(0, qd.data_ptr(), qs.shape[0], bd.data_ptr(), base.shape[0], 128, k,
+                                               prec, 0, ids.data_ptr(), dist.data_ptr(), stream.cuda_stream))
+        run()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(a.steps):
+            run()
+        torch.cuda.synchronize()
+        el = (time.perf_counter() - t) / a.steps
+        out[name] = {"s_per_run": round(el, 4), "queries_per_s": round(qs.shape[0] / el, 1),
+                     "tflops": round(2.0 * qs.shape[0] * base.shape[0] * 128 / el / 1e12, 2),
+                     "ids": ids.cpu().numpy(), "dist": dist.cpu().numpy()}
+        log(f"topk {name}: {el * 1000:.1f} ms per run")
+    same = bool(np.array_equal(out["fp32"]["ids"], out["bf16"]["ids"]) and
+                np.array_equal(out["fp32"]["dist"], out["bf16"]["dist"]))
+    # error of the returned distances against FP64 (integer data: exact)
+    sq = out["bf16"]["dist"][:100]
+    ids = out["bf16"]["ids"][:100]
+    d64 = np.stack([((base[ids[i]].astype(np.float64) - qs[i]) ** 2).sum(1) for i in range(100)])
+    err = float(np.max(np.abs(sq - d64) / np.maximum(d64, 1)))
+    # CPU baseline: the oracle on a bounded sample of queries
+    threads = min(os.cpu_count() or 1, 16)
+    ns = max(threads, 16)
+    t = time.perf_counter()
+    wid, _ = oracle.batch_topk(0, qs[:ns], base, k, nthreads=threads)
+    cpu_el = time.perf_counter() - t
+    assert np.array_equal(wid, out["fp32"]["ids"][:ns])
+    fl = out["bf16"]
+    return {"metric": "batch top-k queries/s (1M x 128 int base, 10k queries, k=100)", "value": fl["queries_per_s"],
+            "unit": "queries/s", "n_gpus": 1, "steps": a.steps, "warmup": 1, "ms_per_step": round(fl["s_per_run"] * 1e3, 3),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32", "data": "synthetic",
+            "config": {"workload": f"exact batch top-k {base.shape[0]}x128 integer data, {qs.shape[0]} queries, k=100"},
+            "fp32": {k2: v for k2, v in out["fp32"].items() if k2 not in ("ids", "dist")},
+            "bf16": {k2: v for k2, v in out["bf16"].items() if k2 not in ("ids", "dist")},
+            "bf16_identical_to_fp32": same, "max_rel_err_vs_fp64": err,
+            "roofline": {"bound": "mfma", "achieved": fl["tflops"], "peak": 2500.0, "unit": "TFLOP/s",
+                         "frac": round(fl["tflops"] / 2500.0, 4), "traffic": None},
+            "cpu_baseline": {"value": round(ns / cpu_el, 2), "unit": "queries/s", "cores": threads, "kind": "port",
+                             "sample": f"{ns} queries, oracle exact top-100 over the 1M base, {threads} threads"}}
+
+
+def bench_train(a, torch, lib):
+    """Config 3: 1,000,000 x 128 from a 1,024-component mixture (centers
+    N(0, 25 I), std 1); coarse k-means K = 4,096, 20 iterations; PQ 16 x 256
+    on a 65,536-residual sample; encode all. FP32 vs BF16 assignment."""
+    from hann_amd import _lib, pqivf
+    rng = np.random.default_rng(5)
+    centers = (rng.standard_normal((1024, 128)) * 5).astype(np.float32)
+    n = a.n_train
+    x = np.empty((n, 128), np.float32)
+    for s0 in range(0, n, 250000):
+        e = min(n, s0 + 250000)
+        x[s0:e] = centers[rng.integers(0, 1024, e - s0)] + rng.standard_normal((e - s0, 128), dtype=np.float32)
+    res = {}
+    lists = {}
+    for name, prec in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16)):
+        idx = pqivf.New(128, pqivf.WithCoarseK(4096), pqivf.WithNumSubquantizers(16), pqivf.WithPQK(256),
+                        pqivf.WithKMeansIters(a.train_iters), pqivf.WithSeed(3), pqivf.WithPQTrainSample(65536),
+                        pqivf.WithTrainPrecision(prec))
+        idx.bulk_add_arrays(np.arange(n), x)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        idx.Train()
+        el = time.perf_counter() - t
+        res[name] = round(el, 3)
+        off, ids, codes, _ = idx.export_lists(with_vectors=False)
+        lists[name] = (off, ids, codes, pqivf.CoarseCentroids(idx))
+        log(f"train {name}: {el:.2f} s")
+        del idx
+    same = all(np.array_equal(lists["fp32"][i], lists["bf16"][i]) for i in range(3)) and \
+        np.array_equal(lists["fp32"][3].view(np.uint32), lists["bf16"][3].view(np.uint32))
+    flops = 2.0 * n * 4096 * 128 * (a.train_iters + 1)
+    return {"metric": "PQIVF train seconds (1M x 128, K=4096, 20 iters, PQ 16x256)", "value": res["bf16"],
+            "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": res["bf16"] * 1e3,
+            "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32",
+            "data": "synthetic", "config": {"workload": f"train {n}x128 K=4096 iters={a.train_iters} PQ16x256"},
+            "fp32_s": res["fp32"], "bf16_s": res["bf16"], "bf16_identical_to_fp32": bool(same),
+            "assign_tflops_fp32": round(flops / res["fp32"] / 1e12, 2),
+            "roofline": None, "cpu_baseline": None}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--workload", default="pqivf", choices=["pqivf", "topk", "train"])
+    ap.add_argument("--n-topk", type=int, default=1_000_000)
+    ap.add_argument("--q-topk", type=int, default=10_000)
+    ap.add_argument("--n-train", type=int, default=1_000_000)
+    ap.add_argument("--train-iters", type=int, default=20)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--n", type=int, default=10_000_000)
@@ -167,6 +269,8 @@ def main():
     ap.add_argument("--rerank-grid", default="1,2,4,8")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rate", type=float, default=0.0, help="offered queries/s (0: half the chosen QPS)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {a.gpus}).",
@@ -178,6 +282,12 @@ def main():
     lib = _lib.load()
     _lib.check(lib.hann_init(0))
     torch.cuda.init()
+    if a.workload == "topk":
+        print(json.dumps(bench_topk(a, torch, lib)), flush=True)
+        return
+    if a.workload == "train":
+        print(json.dumps(bench_train(a, torch, lib)), flush=True)
+        return
 
     t0 = time.time()
     x, centers = synthetic.normalized_mixture(a.n, a.dim, a.components, a.sigma, seed=7)
@@ -267,6 +377,55 @@ def main():
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "pair_bytes_per_launch": int(pair_b / a.steps)}
 
+    # serving: open-loop Poisson arrivals through the continuous-batching loop
+    serve = {}
+    if a.serve_seconds > 0:
+        from hann_amd import serving
+        rate = a.serve_rate if a.serve_rate > 0 else min(200000.0, 0.5 * res["chosen"]["qps"])
+        q0, k0, n0 = batches[0]
+        rng = np.random.default_rng(9)
+        mean_size = float(np.mean(np.minimum(rng.lognormal(np.log(8), 1.0, 100000).astype(int) + 1, 1024)))
+        n_req = max(100, int(rate * a.serve_seconds / mean_size))
+        sizes = np.minimum(rng.lognormal(np.log(8), 1.0, n_req).astype(int) + 1, 1024).astype(np.int32)
+        at = np.cumsum(rng.exponential(1e6 * mean_size / rate, n_req))
+        kind = np.zeros(n_req, np.int32)
+        mut = rng.random(n_req) < 0.01
+        kind[mut] = rng.integers(1, 3, int(mut.sum()))
+        sizes[mut] = rng.integers(1000, 10001, int(mut.sum()))
+        nq_total = int(sizes[kind == 0].sum())
+        pool = np.concatenate([q0] * (nq_total // q0.shape[0] + 1))[:nq_total]
+        kk = np.concatenate([k0] * (nq_total // k0.shape[0] + 1))[:nq_total]
+        pp = np.concatenate([n0] * (nq_total // n0.shape[0] + 1))[:nq_total]
+        q_off = np.zeros(n_req, np.int64)
+        q_off[kind == 0] = np.concatenate([[0], np.cumsum(sizes[kind == 0])[:-1]])
+        n_add = int(sizes[kind == 1].sum())
+        add_vecs, _ = synthetic.normalized_mixture(max(n_add, 1), a.dim, a.components, a.sigma, seed=4242,
+                                                   centers=centers)
+        add_ids = np.arange(a.n, a.n + max(n_add, 1), dtype=np.int64)
+        n_del = int(sizes[kind == 2].sum())
+        del_ids = rng.choice(a.n, max(n_del, 1), replace=False).astype(np.int64)
+        for name, (mode, fused) in paths.items():
+            srv = serving.Server(idx, max_batch=a.batch, slots=2, hbm_budget=2 << 30, scan_mode=mode,
+                                 rerank=chosen_r[name], fused=fused)
+            try:
+                ids_out, _, lat, wall = srv.replay(at, kind, q_off, sizes, pool, kk, pp, add_ids, add_vecs,
+                                                   del_ids, kmax_out=10)
+                st = srv.stats()
+            finally:
+                srv.close()
+            # undo the run's mutations so the next path sees the same index
+            idx.BulkDelete(add_ids[:n_add].tolist())
+            idx.bulk_add_arrays(np.sort(del_ids[:n_del]), x[np.sort(del_ids[:n_del])])
+            sl = lat[kind == 0]
+            first = ids_out[:min(ngt, nq_total)]
+            serve[name] = {"offered_qps": round(rate, 1), "achieved_qps": round(nq_total / wall, 1),
+                           "p50_ms": round(float(np.percentile(sl, 50)), 3),
+                           "p99_ms": round(float(np.percentile(sl, 99)), 3),
+                           "requests": int(n_req), "mutations": int(mut.sum()), "batches": int(st["batches"]),
+                           "budget_peak_bytes": int(st["budget_peak"]),
+                           "recall_at_10_first_pool": round(recall_at_10(first, gt[:first.shape[0]]), 4)}
+            log(f"serve {name}: {serve[name]}")
+
     cpu = None
     if not a.no_cpu:
         q0, k0, n0 = batches[0]
@@ -294,6 +453,7 @@ def main():
                   "path": "reference-order fp32 decode + fp32 rerank",
                   "stage_ms_per_step": [round(v / max(pl["stages"][3], 1), 4) for v in pl["stages"][:3]]},
         "chosen_vs_plain": round(ch["qps"] / pl["qps"], 3),
+        "serving": serve,
         "train_s": round(train_s, 2),
         "roofline": roofline,
         "cpu_baseline": cpu,

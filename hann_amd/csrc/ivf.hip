@@ -127,9 +127,10 @@ __global__ void __launch_bounds__(64) k_scan(ScanArgs a) {
     const int got = sel.finish();
     float *ok = a.out_key + q * a.out_stride;
     int64_t *oi = a.out_id + q * a.out_stride;
-    for (int t = lane; t < selk; t += kWave) {
-        ok[t] = t < got ? sel.keys[t] : __builtin_inff();
-        oi[t] = t < got ? sel.ids[t] : (int64_t)-1;
+    for (int t = lane; t < a.out_stride; t += kWave) {
+        const bool okk = t < got && t < selk;
+        ok[t] = okk ? sel.keys[t] : __builtin_inff();
+        oi[t] = okk ? sel.ids[t] : (int64_t)-1;
     }
     if (lane == 0) a.out_cnt[q] = got < selk ? got : selk;
 }
@@ -170,9 +171,10 @@ __global__ void __launch_bounds__(64) k_rerank(const float *__restrict__ queries
         sel.push(key, id, valid);
     }
     const int got = sel.finish();
-    for (int t = lane; t < kq; t += kWave) {
-        out_key[q * out_stride + t] = t < got ? sel.keys[t] : __builtin_inff();
-        out_id[q * out_stride + t] = t < got ? sel.ids[t] : (int64_t)-1;
+    for (int t = lane; t < out_stride; t += kWave) {
+        const bool okk = t < got && t < kq;
+        out_key[q * out_stride + t] = okk ? sel.keys[t] : __builtin_inff();
+        out_id[q * out_stride + t] = okk ? sel.ids[t] : (int64_t)-1;
     }
     if (lane == 0) out_cnt[q] = got < kq ? got : kq;
 }

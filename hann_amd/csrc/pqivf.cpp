@@ -187,6 +187,7 @@ void PQIVF::assign_locked(const std::vector<int64_t> &ids, const std::vector<con
     HANN_HIP(hipMemcpyAsync(cl.data(), dc.get(), (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HANN_HIP(hipStreamSynchronize(stream));
     for (int64_t i = 0; i < n; ++i) {
+        touch_append(cl[i]);
         ListData &L = lists[cl[i]];
         L.exists = true;
         L.ids.push_back(ids[i]);
@@ -200,7 +201,6 @@ void PQIVF::assign_locked(const std::vector<int64_t> &ids, const std::vector<con
         }
         id_to_cluster[ids[i]] = cl[i];
     }
-    mark_dirty();
 }
 
 void PQIVF::add_locked(int64_t id, const float *vec, size_t len) {
@@ -263,6 +263,7 @@ void PQIVF::bulk_add(const int64_t *ids, const float *flat, const size_t *lens, 
 }
 
 void PQIVF::remove_from_list(int64_t id, int64_t cluster) {
+    touch_other(cluster);
     ListData &L = lists[cluster];
     size_t w = 0;
     const size_t len = L.ids.size();
@@ -295,7 +296,6 @@ void PQIVF::delete_locked(int64_t id) {
     if (lists[c].ids.size() == before)
         fail(HANN_ERR_NOT_FOUND, strf("id %lld not found in cluster %lld", (long long)id, (long long)c));
     id_to_cluster.erase(it);
-    mark_dirty();
 }
 
 void PQIVF::bulk_delete(const int64_t *ids, size_t n) {
@@ -310,7 +310,6 @@ void PQIVF::bulk_delete(const int64_t *ids, size_t n) {
         if (c < 0 || c >= (int64_t)lists.size() || !lists[c].exists) continue;
         remove_from_list(id, c);
         id_to_cluster.erase(it);
-        mark_dirty();
     }
 }
 
@@ -493,43 +492,150 @@ void PQIVF::train() {
 }
 
 // ------------------------------------------------------- device mirror --
-void PQIVF::mark_dirty() { dirty = true; }
+void PQIVF::mark_dirty() {
+    dirty = true;
+    full_rebuild = true;
+}
+
+void PQIVF::touch_append(int64_t l) {
+    dirty = true;
+    if (full_rebuild) return;
+    if (touch_flags[l] == 0) {
+        touch_flags[l] = 1;
+        append_from[l] = (int64_t)lists[l].ids.size();
+        touched_lists.push_back(l);
+    }
+}
+
+void PQIVF::touch_other(int64_t l) {
+    dirty = true;
+    if (full_rebuild) return;
+    if (touch_flags[l] == 0) touched_lists.push_back(l);
+    touch_flags[l] |= 2;
+}
+
+// dst[i] = slot of packed entry i: copy its id, codes, flag and vector there
+__global__ void k_scatter_entries(int64_t n, int m, int code_bytes, int d, const int64_t *__restrict__ dst,
+                                  const int64_t *__restrict__ ids, const uint8_t *__restrict__ codes,
+                                  const uint8_t *__restrict__ has, const float *__restrict__ vecs,
+                                  int64_t *__restrict__ o_ids, uint8_t *__restrict__ o_codes,
+                                  uint8_t *__restrict__ o_has, float *__restrict__ o_vecs) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * d) return;
+    const int64_t i = t / d;
+    const int e = (int)(t % d);
+    const int64_t p = dst[i];
+    o_vecs[p * d + e] = vecs[t];
+    if (e == 0) {
+        o_ids[p] = ids[i];
+        o_has[p] = has[i];
+        const int cb = m * code_bytes;
+        for (int b = 0; b < cb; ++b) o_codes[p * cb + b] = codes[i * cb + b];
+    }
+}
 
 void PQIVF::sync_device() {
     ensure_stream();
     if (!dirty) return;
     const int s = m > 0 ? (int)(dim / m) : 0;
-    int64_t N = 0;
-    for (const ListData &L : lists) N += (int64_t)L.ids.size();
-    n_entries = N;
     code_bytes = pmax > 256 ? 2 : 1;
-    std::vector<int64_t> start(lists.size()), len(lists.size());
-    std::vector<int64_t> hids((size_t)N);
-    std::vector<uint8_t> hcodes((size_t)N * m * code_bytes);
-    std::vector<uint8_t> hhas((size_t)N);
-    std::vector<float> hv((size_t)N * dim);
-    int64_t pos = 0;
-    any_uncoded = false;
-    for (size_t c = 0; c < lists.size(); ++c) {
-        const ListData &L = lists[c];
-        start[c] = pos;
-        len[c] = (int64_t)L.ids.size();
-        for (size_t r = 0; r < L.ids.size(); ++r, ++pos) {
-            hids[pos] = L.ids[r];
-            std::memcpy(&hv[(size_t)pos * dim], &L.vecs[r * dim], dim * sizeof(float));
-            const int32_t *cr = &L.codes[r * m];
-            const bool ok = L.has_codes[r] && codes_valid(cr);
-            hhas[pos] = ok ? 1 : 0;
-            if (!ok) any_uncoded = true;
-            for (int64_t i = 0; i < m; ++i) {
-                const int32_t v = ok ? cr[i] : 0;
-                if (code_bytes == 1)
-                    hcodes[(size_t)pos * m + i] = (uint8_t)v;
-                else
-                    reinterpret_cast<uint16_t *>(hcodes.data())[(size_t)pos * m + i] = (uint16_t)v;
+    auto pack_entry = [&](const ListData &L, size_t r, int64_t pos, std::vector<int64_t> &hids,
+                          std::vector<uint8_t> &hcodes, std::vector<uint8_t> &hhas, std::vector<float> &hv) {
+        hids[pos] = L.ids[r];
+        std::memcpy(&hv[(size_t)pos * dim], &L.vecs[r * dim], dim * sizeof(float));
+        const int32_t *cr = &L.codes[r * m];
+        const bool ok = L.has_codes[r] && codes_valid(cr);
+        hhas[pos] = ok ? 1 : 0;
+        if (!ok) any_uncoded = true;
+        for (int64_t i = 0; i < m; ++i) {
+            const int32_t v = ok ? cr[i] : 0;
+            if (code_bytes == 1)
+                hcodes[(size_t)pos * m + i] = (uint8_t)v;
+            else
+                reinterpret_cast<uint16_t *>(hcodes.data())[(size_t)pos * m + i] = (uint16_t)v;
+        }
+    };
+    // incremental: upload appended tails, or whole changed lists, into the
+    // lists' existing device slots
+    if (!full_rebuild && dev_start.size() == lists.size()) {
+        bool fits = true;
+        int64_t total = 0;
+        for (int64_t l : touched_lists) {
+            const int64_t len = (int64_t)lists[l].ids.size();
+            if (len > dev_cap[l]) fits = false;
+            total += len - (touch_flags[l] == 1 ? append_from[l] : 0);
+        }
+        if (fits) {
+            std::vector<int64_t> hids((size_t)total), dst((size_t)total);
+            std::vector<uint8_t> hcodes((size_t)total * m * code_bytes), hhas((size_t)total);
+            std::vector<float> hv((size_t)total * dim);
+            int64_t pos = 0;
+            for (int64_t l : touched_lists) {
+                const ListData &L = lists[l];
+                const int64_t from = touch_flags[l] == 1 ? append_from[l] : 0;
+                for (int64_t r = from; r < (int64_t)L.ids.size(); ++r, ++pos) {
+                    pack_entry(L, (size_t)r, pos, hids, hcodes, hhas, hv);
+                    dst[pos] = dev_start[l] + r;
+                }
+                dev_len[l] = (int64_t)L.ids.size();
+                touch_flags[l] = 0;
             }
+            touched_lists.clear();
+            if (total > 0) {
+                DevBuf<int64_t> d_dst((size_t)total), d_i((size_t)total);
+                DevBuf<uint8_t> d_c(hcodes.size()), d_h((size_t)total);
+                DevBuf<float> d_v(hv.size());
+                HANN_HIP(hipMemcpyAsync(d_dst.get(), dst.data(), total * 8, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_i.get(), hids.data(), total * 8, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_c.get(), hcodes.data(), hcodes.size(), hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_h.get(), hhas.data(), total, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_v.get(), hv.data(), hv.size() * 4, hipMemcpyHostToDevice, stream));
+                hipLaunchKernelGGL(k_scatter_entries, dim3((unsigned)ceil_div(total * dim, 256)), dim3(256), 0, stream,
+                                   total, (int)m, code_bytes, (int)dim, d_dst.get(), d_i.get(), d_c.get(), d_h.get(),
+                                   d_v.get(), d_ids.get(), d_codes.get(), d_has_codes.get(), d_vecs.get());
+                HANN_LAUNCH_CHECK();
+                HANN_HIP(hipMemcpyAsync(d_list_len.get(), dev_len.data(), dev_len.size() * 8, hipMemcpyHostToDevice,
+                                        stream));
+                HANN_HIP(hipStreamSynchronize(stream));
+            }
+            n_entries = 0;
+            for (int64_t v : dev_len) n_entries += v;
+            ++incremental_syncs;
+            dirty = false;
+            return;
         }
     }
+    // full rebuild: every list gets slack capacity for later appends
+    const size_t K = lists.size();
+    dev_start.assign(K, 0);
+    dev_cap.assign(K, 0);
+    dev_len.assign(K, 0);
+    append_from.assign(K, 0);
+    touch_flags.assign(K, 0);
+    touched_lists.clear();
+    int64_t N = 0;
+    for (size_t c = 0; c < K; ++c) {
+        const int64_t len = (int64_t)lists[c].ids.size();
+        dev_start[c] = N;
+        dev_len[c] = len;
+        dev_cap[c] = len + std::max<int64_t>(8, len / 8);
+        N += dev_cap[c];
+    }
+    n_entries = 0;
+    for (int64_t v : dev_len) n_entries += v;
+    std::vector<int64_t> hids((size_t)N, -1);
+    std::vector<uint8_t> hcodes((size_t)N * m * code_bytes, 0);
+    std::vector<uint8_t> hhas((size_t)N, 0);
+    std::vector<float> hv((size_t)N * dim, 0.f);
+    any_uncoded = false;
+    for (size_t c = 0; c < K; ++c) {
+        const ListData &L = lists[c];
+        for (size_t r = 0; r < L.ids.size(); ++r) pack_entry(L, r, dev_start[c] + (int64_t)r, hids, hcodes, hhas, hv);
+    }
+    std::vector<int64_t> &start = dev_start;
+    std::vector<int64_t> &len = dev_len;
+    full_rebuild = false;
+    ++full_syncs;
     auto up = [&](auto &buf, const auto &host) {
         using T = typename std::decay_t<decltype(host)>::value_type;
         buf.resize(host.size() + 1);

@@ -96,6 +96,13 @@ struct PQIVF {
     hipStream_t stream = nullptr;
     Workspace ws;
     bool dirty = true;
+    bool full_rebuild = true;
+    // incremental mirror updates: lists have slack capacity on the device;
+    // appended tails or whole changed lists are uploaded at the next sync
+    std::vector<int64_t> dev_start, dev_cap, dev_len, append_from;
+    std::vector<uint8_t> touch_flags;   // bit 0: appends only so far, bit 1: other change
+    std::vector<int64_t> touched_lists;
+    int64_t incremental_syncs = 0, full_syncs = 0;
     bool any_uncoded = false;
     int code_bytes = 1;
     int64_t n_entries = 0;
@@ -126,7 +133,9 @@ struct PQIVF {
     void update(int64_t id, const float *vec, size_t len);
     void bulk_update(const int64_t *ids, const float *flat, const size_t *lens, size_t n);
     void train();
-    void mark_dirty();
+    void mark_dirty();           // the next sync rebuilds the whole device mirror
+    void touch_append(int64_t l);   // entries are about to be appended to list l
+    void touch_other(int64_t l);    // list l changed in any other way
     void sync_device();
     void check_searchable(int64_t k, size_t len) const;
     void search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,

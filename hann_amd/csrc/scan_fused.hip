@@ -88,8 +88,29 @@ __device__ __forceinline__ float lut_make<float>(float v) { return v; }
 template <>
 __device__ __forceinline__ __half lut_make<__half>(float v) { return __float2half(v); }
 
+// Sum of an entry's table values in subquantizer order, starting at qc.
+// M > 0 (uint8 codes, M % 4 == 0): the codes are read as M/4 dwords.
+template <int M, typename CodeT, typename LutT>
+__device__ __forceinline__ float adc_sum(const void *codes, int64_t e, int m, int P, const LutT *lut, float qc) {
+    float acc = qc;
+    if constexpr (M > 0 && sizeof(CodeT) == 1 && M % 4 == 0) {
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes) + e * (M / 4);
+        uint32_t w[M / 4];
+#pragma unroll
+        for (int j = 0; j < M / 4; ++j) w[j] = cw[j];
+#pragma unroll
+        for (int j = 0; j < M / 4; ++j)
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc = acc + lut_get<LutT>(lut, (4 * j + b) * P + (int)((w[j] >> (8 * b)) & 0xFFu));
+    } else {
+        const CodeT *ce = reinterpret_cast<const CodeT *>(codes) + e * m;
+        for (int i = 0; i < m; ++i) acc = acc + lut_get<LutT>(lut, i * P + (int)ce[i]);
+    }
+    return acc;
+}
+
 // LDS: q[d] | S_q[m*P] (LutT) | per wave: lut[m*P] (LutT), keys[cap], ids[cap]
-template <int WPQ, typename CodeT, typename LutT>
+template <int WPQ, typename CodeT, typename LutT, int M>
 __global__ void __launch_bounds__(64 * WPQ) k_scan_fused(ScanArgs a, FusedArgs f) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     __shared__ int s_item;
@@ -168,10 +189,7 @@ __global__ void __launch_bounds__(64 * WPQ) k_scan_fused(ScanArgs a, FusedArgs f
                 float key = 0.f;
                 if (valid) {
                     if (!a.any_uncoded || a.has_codes[e]) {
-                        const CodeT *ce = reinterpret_cast<const CodeT *>(a.codes) + e * m;
-                        float acc = qc;
-                        for (int i = 0; i < m; ++i) acc = acc + lut_get<LutT>(s_lut, i * P + (int)ce[i]);
-                        key = acc;
+                        key = adc_sum<M, CodeT, LutT>(a.codes, e, m, P, s_lut, qc);
                     } else {
                         key = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.vecs + e * d}, d);
                     }
@@ -200,9 +218,11 @@ __global__ void __launch_bounds__(64 * WPQ) k_scan_fused(ScanArgs a, FusedArgs f
             got = sel.finish();
             float *okey = a.out_key + q * a.out_stride;
             int64_t *oid = a.out_id + q * a.out_stride;
-            for (int t = lane; t < selk; t += kWave) {
-                okey[t] = t < got ? sel.keys[t] : __builtin_inff();
-                oid[t] = t < got ? sel.ids[t] : (int64_t)-1;
+            // every slot of the output row is written: the result, then (inf, -1)
+            for (int t = lane; t < a.out_stride; t += kWave) {
+                const bool ok = t < got && t < selk;
+                okey[t] = ok ? sel.keys[t] : __builtin_inff();
+                oid[t] = ok ? sel.ids[t] : (int64_t)-1;
             }
             if (lane == 0) a.out_cnt[q] = got < selk ? got : selk;
         }
@@ -233,26 +253,39 @@ void launch_scan_fused(const ScanArgs &a, FusedArgs f, int64_t nq, int wpq, int 
     HANN_LAUNCH_CHECK();
     const size_t shm = fused_lds_bytes(wpq, lut16 ? 2 : 4, a.d, a.m, a.pmax_cb, a.cap);
     if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("fused scan needs %zu bytes of LDS", shm));
-#define HANN_FUSED(W, CT, LT)                                                                             \
+#define HANN_FUSED(W, CT, LT, M)                                                                          \
     do {                                                                                                  \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_fused<W, CT, LT>,                          \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_fused<W, CT, LT, M>,                       \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));              \
-        hipLaunchKernelGGL((dev::k_scan_fused<W, CT, LT>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, f); \
+        hipLaunchKernelGGL((dev::k_scan_fused<W, CT, LT, M>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, f); \
         HANN_LAUNCH_CHECK();                                                                              \
     } while (0)
-#define HANN_FUSED_W(CT, LT)                         \
-    do {                                             \
-        if (wpq == 1) HANN_FUSED(1, CT, LT);         \
-        else if (wpq == 2) HANN_FUSED(2, CT, LT);    \
-        else HANN_FUSED(4, CT, LT);                  \
+#define HANN_FUSED_W(CT, LT, M)                         \
+    do {                                                \
+        if (wpq == 1) HANN_FUSED(1, CT, LT, M);         \
+        else if (wpq == 2) HANN_FUSED(2, CT, LT, M);    \
+        else HANN_FUSED(4, CT, LT, M);                  \
+    } while (0)
+#define HANN_FUSED_M(LT)                                                    \
+    do {                                                                    \
+        switch (a.m) {                                                      \
+        case 8: HANN_FUSED_W(uint8_t, LT, 8); break;                        \
+        case 16: HANN_FUSED_W(uint8_t, LT, 16); break;                      \
+        case 24: HANN_FUSED_W(uint8_t, LT, 24); break;                      \
+        case 32: HANN_FUSED_W(uint8_t, LT, 32); break;                      \
+        case 48: HANN_FUSED_W(uint8_t, LT, 48); break;                      \
+        case 64: HANN_FUSED_W(uint8_t, LT, 64); break;                      \
+        default: HANN_FUSED_W(uint8_t, LT, 0); break;                       \
+        }                                                                   \
     } while (0)
     if (code_bytes == 1) {
-        if (lut16) HANN_FUSED_W(uint8_t, __half);
-        else HANN_FUSED_W(uint8_t, float);
+        if (lut16) HANN_FUSED_M(__half);
+        else HANN_FUSED_M(float);
     } else {
-        if (lut16) HANN_FUSED_W(uint16_t, __half);
-        else HANN_FUSED_W(uint16_t, float);
+        if (lut16) HANN_FUSED_W(uint16_t, __half, 0);
+        else HANN_FUSED_W(uint16_t, float, 0);
     }
+#undef HANN_FUSED_M
 #undef HANN_FUSED_W
 #undef HANN_FUSED
 }

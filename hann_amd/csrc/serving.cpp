@@ -271,7 +271,8 @@ struct Server {
                     r->out_dist.assign((size_t)r->nq * r->kmax, INFINITY);
                     r->out_cnt.assign((size_t)r->nq, 0);
                     for (int64_t i = 0; i < r->nq; ++i) {
-                        const int32_t c = s->h_cnt[o + i];
+                        // a query never has more results than its own k
+                        const int32_t c = std::min(s->h_cnt[o + i], r->k[i]);
                         r->out_cnt[i] = c;
                         for (int j = 0; j < c && j < r->kmax; ++j) {
                             r->out_ids[(size_t)i * r->kmax + j] = s->h_id[(o + i) * s->kmax + j];

@@ -325,14 +325,15 @@ def test_candidate_clusters_and_fallback(hann):
 
 
 def test_fallback_counter(hann):
-    """pqivf/index_test.go:819-842"""
-    idx = new_small(4, 2, 2, 2, 5)
-    for i in range(4):
-        idx.Add(i, [i * 10] * 4)
+    """pqivf/index_test.go:819-842: 8 lists, 3 probed, k = 40 of 40 entries."""
+    idx = new_small(4, 8, 2, 2, 5)
+    for i in range(40):
+        idx.Add(i, [i, i + 1, i + 2, i + 3])
     idx.Train()
     assert idx.Stats().FallbackSearches == 0
-    idx.Search([0, 0, 0, 0], 4)
+    res = idx.Search([0, 1, 2, 3], 40)
     assert idx.Stats().FallbackSearches >= 1
+    assert sorted(n.ID for n in res) == list(range(40))
 
 
 def test_delete_all_then_search(hann):
@@ -562,3 +563,35 @@ def test_fused_scan_matches_unfused(hann, oracle, mode, fused, rerank):
         assert agree >= 0.8
     both = [len(set(a_ids[q, :ks[q]].tolist()) & set(b_ids[q, :ks[q]].tolist())) / ks[q] for q in range(300)]
     assert np.mean(both) >= 0.95
+
+
+def test_incremental_mirror_equals_rebuild(hann):
+    """Mutations after training reach the device incrementally (appended
+    tails, re-uploaded lists, capacity overflow -> full rebuild); searches
+    equal those of the same state rebuilt from scratch (save -> load)."""
+    from hann_amd import synthetic
+    pq = P()
+    x, qs = synthetic.config1(n=12000, q=200)
+    idx = pq.New(64, pq.WithCoarseK(32), pq.WithPQK(64), pq.WithKMeansIters(4), pq.WithSeed(6))
+    idx.bulk_add_arrays(np.arange(10000), x[:10000])
+    idx.Train()
+    idx.search_batch(qs, 10, nprobe=4)            # first sync: full build
+    rng = np.random.default_rng(0)
+    for step in range(6):
+        lo = 10000 + step * 300
+        idx.bulk_add_arrays(np.arange(lo, lo + 300), x[lo:lo + 300])   # appends
+        idx.search_batch(qs[:10], 10, nprobe=4)
+        idx.BulkDelete(rng.choice(lo, 200, replace=False).tolist())      # whole-list re-uploads
+        idx.Update(int(lo + 5), x[int(rng.integers(0, 12000))])         # delete + append
+        got = idx.search_batch(qs, 10, nprobe=6, scan_mode=0)
+        other = pq.New(64)
+        other.load_bytes(idx.save_bytes())
+        want = other.search_batch(qs, 10, nprobe=6, scan_mode=0)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # a burst that overflows list capacity forces a full rebuild
+    idx.bulk_add_arrays(np.arange(20000, 22000), np.repeat(x[:1], 2000, axis=0) + np.float32(1e-3))
+    got = idx.search_batch(qs, 10, nprobe=6, scan_mode=0)
+    other = pq.New(64)
+    other.load_bytes(idx.save_bytes())
+    want = other.search_batch(qs, 10, nprobe=6, scan_mode=0)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])

@@ -46,8 +46,12 @@ def test_alone_equals_mixed(hann, index, mode, rerank):
         st = srv.stats()
     finally:
         srv.close()
-    for a, b in zip(alone, mixed):
-        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]) and np.array_equal(a[2], b[2])
+    for r, (a, b) in enumerate(zip(alone, mixed)):
+        sl = slice(off[r], off[r + 1])
+        info = f"request {r}: k {ks[sl].tolist()} nprobe {nps[sl].tolist()} counts alone {a[2].tolist()} mixed {b[2].tolist()}"
+        assert np.array_equal(a[2], b[2]), info
+        assert np.array_equal(a[0], b[0]), info
+        assert np.array_equal(a[1], b[1]), info
     assert st["batches"] < 2 * n_req   # the mixed phase did batch requests together
     # and both equal the direct batched search
     ids, dist, cnt = idx.search_batch(qs[:off[n_req]], ks[:off[n_req]], nprobe=nps[:off[n_req]], scan_mode=mode,
