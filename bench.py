@@ -392,6 +392,13 @@ def main():
         mut = rng.random(n_req) < 0.01
         kind[mut] = rng.integers(1, 3, int(mut.sum()))
         sizes[mut] = rng.integers(1000, 10001, int(mut.sum()))
+        # keep each mutation kind within 2% of the index so the run measures
+        # search under churn, not a rebuild
+        for kd in (1, 2):
+            sel = kind == kd
+            tot = int(sizes[sel].sum())
+            if tot > a.n // 50:
+                sizes[sel] = np.maximum(1, sizes[sel].astype(np.int64) * (a.n // 50) // tot).astype(np.int32)
         nq_total = int(sizes[kind == 0].sum())
         pool = np.concatenate([q0] * (nq_total // q0.shape[0] + 1))[:nq_total]
         kk = np.concatenate([k0] * (nq_total // k0.shape[0] + 1))[:nq_total]
@@ -403,7 +410,7 @@ def main():
                                                    centers=centers)
         add_ids = np.arange(a.n, a.n + max(n_add, 1), dtype=np.int64)
         n_del = int(sizes[kind == 2].sum())
-        del_ids = rng.choice(a.n, max(n_del, 1), replace=False).astype(np.int64)
+        del_ids = rng.permutation(a.n)[:max(n_del, 1)].astype(np.int64)
         for name, (mode, fused) in paths.items():
             srv = serving.Server(idx, max_batch=a.batch, slots=2, hbm_budget=2 << 30, scan_mode=mode,
                                  rerank=chosen_r[name], fused=fused)

@@ -263,7 +263,7 @@ void PQIVF::bulk_add(const int64_t *ids, const float *flat, const size_t *lens, 
 }
 
 void PQIVF::remove_from_list(int64_t id, int64_t cluster) {
-    touch_other(cluster);
+    touch_delete(cluster, id);
     ListData &L = lists[cluster];
     size_t w = 0;
     const size_t len = L.ids.size();
@@ -300,16 +300,46 @@ void PQIVF::delete_locked(int64_t id) {
 
 void PQIVF::bulk_delete(const int64_t *ids, size_t n) {
     // BulkDelete (pqivf/index.go:350-384): unknown ids are skipped
+    // Removing the ids one at a time (as the reference does) or per list in
+    // one pass leaves the same lists: each removal keeps the order of the rest.
     std::vector<int64_t> sorted(ids, ids + n);
     std::sort(sorted.begin(), sorted.end());
+    std::vector<std::pair<int64_t, int64_t>> by_list;   // (cluster, id)
     for (int64_t id : sorted) {
         if (pending.erase(id)) continue;
         auto it = id_to_cluster.find(id);
         if (it == id_to_cluster.end()) continue;
         const int64_t c = it->second;
         if (c < 0 || c >= (int64_t)lists.size() || !lists[c].exists) continue;
-        remove_from_list(id, c);
+        by_list.emplace_back(c, id);
         id_to_cluster.erase(it);
+    }
+    std::sort(by_list.begin(), by_list.end());
+    std::vector<int64_t> drop;
+    for (size_t i = 0; i < by_list.size();) {
+        const int64_t c = by_list[i].first;
+        drop.clear();
+        for (; i < by_list.size() && by_list[i].first == c; ++i) {
+            drop.push_back(by_list[i].second);
+            touch_delete(c, by_list[i].second);
+        }
+        ListData &L = lists[c];
+        size_t w = 0;
+        const size_t len = L.ids.size();
+        for (size_t r = 0; r < len; ++r) {
+            if (std::binary_search(drop.begin(), drop.end(), L.ids[r])) continue;
+            if (w != r) {
+                L.ids[w] = L.ids[r];
+                std::memcpy(&L.vecs[w * dim], &L.vecs[r * dim], dim * sizeof(float));
+                std::memcpy(&L.codes[w * m], &L.codes[r * m], m * sizeof(int32_t));
+                L.has_codes[w] = L.has_codes[r];
+            }
+            ++w;
+        }
+        L.ids.resize(w);
+        L.vecs.resize(w * dim);
+        L.codes.resize(w * m);
+        L.has_codes.resize(w);
     }
 }
 
@@ -511,7 +541,89 @@ void PQIVF::touch_other(int64_t l) {
     dirty = true;
     if (full_rebuild) return;
     if (touch_flags[l] == 0) touched_lists.push_back(l);
+    touch_flags[l] |= 4;
+}
+
+void PQIVF::touch_delete(int64_t l, int64_t id) {
+    dirty = true;
+    if (full_rebuild) return;
+    if (touch_flags[l] == 0) touched_lists.push_back(l);
     touch_flags[l] |= 2;
+    deleted_ids[l].push_back(id);
+}
+
+// Stable removal of deleted ids from lists on the device. Job j: list
+// slots [start[j], start[j] + len[j]), ids to drop del[doff[j] .. doff[j+1])
+// (sorted); kept entries go, in order, to temp rows [toff[j], ...).
+__global__ void __launch_bounds__(256) k_compact_lists(const int64_t *__restrict__ start, const int64_t *__restrict__ len,
+                                                       const int64_t *__restrict__ doff, const int64_t *__restrict__ del,
+                                                       const int64_t *__restrict__ toff, int m, int code_bytes, int d,
+                                                       const int64_t *__restrict__ ids, const uint8_t *__restrict__ codes,
+                                                       const uint8_t *__restrict__ has, const float *__restrict__ vecs,
+                                                       int64_t *__restrict__ t_ids, uint8_t *__restrict__ t_codes,
+                                                       uint8_t *__restrict__ t_has, float *__restrict__ t_vecs) {
+    __shared__ int s_scan[256];
+    const int j = blockIdx.x;
+    const int64_t s0 = start[j], n = len[j];
+    const int64_t *dl = del + doff[j];
+    const int64_t nd = doff[j + 1] - doff[j];
+    const int cb = m * code_bytes;
+    int64_t run = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += 256) {
+        const int64_t e = c0 + threadIdx.x;
+        int keep = 0;
+        if (e < n) {
+            const int64_t id = ids[s0 + e];
+            int64_t lo = 0, hi = nd;
+            while (lo < hi) {
+                const int64_t mid = (lo + hi) >> 1;
+                if (dl[mid] < id) lo = mid + 1; else hi = mid;
+            }
+            keep = !(lo < nd && dl[lo] == id);
+        }
+        s_scan[threadIdx.x] = keep;
+        __syncthreads();
+        for (int o = 1; o < 256; o <<= 1) {
+            const int v = (int)threadIdx.x >= o ? s_scan[threadIdx.x - o] : 0;
+            __syncthreads();
+            s_scan[threadIdx.x] += v;
+            __syncthreads();
+        }
+        if (keep) {
+            const int64_t dst = toff[j] + run + s_scan[threadIdx.x] - 1;
+            const int64_t src = s0 + e;
+            t_ids[dst] = ids[src];
+            t_has[dst] = has[src];
+            for (int b = 0; b < cb; ++b) t_codes[dst * cb + b] = codes[src * cb + b];
+            for (int x = 0; x < d; ++x) t_vecs[dst * d + x] = vecs[src * d + x];
+        }
+        run += s_scan[255];
+        __syncthreads();
+    }
+}
+
+// copy job j's kept rows back to its list slots
+__global__ void k_copy_back(const int64_t *__restrict__ start, const int64_t *__restrict__ toff,
+                            const int64_t *__restrict__ newlen, int njobs, int m, int code_bytes, int d,
+                            const int64_t *__restrict__ t_ids, const uint8_t *__restrict__ t_codes,
+                            const uint8_t *__restrict__ t_has, const float *__restrict__ t_vecs,
+                            int64_t *__restrict__ ids, uint8_t *__restrict__ codes, uint8_t *__restrict__ has,
+                            float *__restrict__ vecs) {
+    const int j = blockIdx.y;
+    if (j >= njobs) return;
+    const int cb = m * code_bytes;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < newlen[j] * d;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t r = t / d;
+        const int x = (int)(t % d);
+        const int64_t src = toff[j] + r, dst = start[j] + r;
+        vecs[dst * d + x] = t_vecs[src * d + x];
+        if (x == 0) {
+            ids[dst] = t_ids[src];
+            has[dst] = t_has[src];
+            for (int b = 0; b < cb; ++b) codes[dst * cb + b] = t_codes[src * cb + b];
+        }
+    }
 }
 
 // dst[i] = slot of packed entry i: copy its id, codes, flag and vector there
@@ -560,10 +672,67 @@ void PQIVF::sync_device() {
     if (!full_rebuild && dev_start.size() == lists.size()) {
         bool fits = true;
         int64_t total = 0;
+        std::vector<int64_t> jobs;   // delete-only lists: compacted on the device
         for (int64_t l : touched_lists) {
             const int64_t len = (int64_t)lists[l].ids.size();
             if (len > dev_cap[l]) fits = false;
+            if (touch_flags[l] == 2) {
+                jobs.push_back(l);
+                continue;
+            }
             total += len - (touch_flags[l] == 1 ? append_from[l] : 0);
+        }
+        if (fits && !jobs.empty()) {
+            const size_t J = jobs.size();
+            std::vector<int64_t> jstart(J), jlen(J), doff(J + 1, 0), toff(J + 1, 0), newlen(J), dels;
+            for (size_t j = 0; j < J; ++j) {
+                const int64_t l = jobs[j];
+                jstart[j] = dev_start[l];
+                jlen[j] = dev_len[l];
+                std::vector<int64_t> &dv = deleted_ids[l];
+                std::sort(dv.begin(), dv.end());
+                dels.insert(dels.end(), dv.begin(), dv.end());
+                doff[j + 1] = (int64_t)dels.size();
+                toff[j + 1] = toff[j] + jlen[j];
+                newlen[j] = (int64_t)lists[l].ids.size();
+            }
+            const int64_t T = toff[J];
+            DevBuf<int64_t> d_js(J), d_jl(J), d_do(J + 1), d_del(dels.size() + 1), d_to(J + 1), d_nl(J), t_ids((size_t)T + 1);
+            DevBuf<uint8_t> t_codes((size_t)T * m * code_bytes + 1), t_has((size_t)T + 1);
+            DevBuf<float> t_vecs((size_t)T * dim + 1);
+            HANN_HIP(hipMemcpyAsync(d_js.get(), jstart.data(), J * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_jl.get(), jlen.data(), J * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_do.get(), doff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
+            if (!dels.empty())
+                HANN_HIP(hipMemcpyAsync(d_del.get(), dels.data(), dels.size() * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_to.get(), toff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_nl.get(), newlen.data(), J * 8, hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_compact_lists, dim3((unsigned)J), dim3(256), 0, stream, d_js.get(), d_jl.get(),
+                               d_do.get(), d_del.get(), d_to.get(), (int)m, code_bytes, (int)dim, d_ids.get(),
+                               d_codes.get(), d_has_codes.get(), d_vecs.get(), t_ids.get(), t_codes.get(), t_has.get(),
+                               t_vecs.get());
+            HANN_LAUNCH_CHECK();
+            for (size_t j0 = 0; j0 < J; j0 += 65535) {
+                const size_t jn = std::min<size_t>(65535, J - j0);
+                hipLaunchKernelGGL(k_copy_back, dim3(64, (unsigned)jn), dim3(256), 0, stream, d_js.get() + j0,
+                                   d_to.get() + j0, d_nl.get() + j0, (int)jn, (int)m, code_bytes, (int)dim, t_ids.get(),
+                                   t_codes.get(), t_has.get(), t_vecs.get(), d_ids.get(), d_codes.get(),
+                                   d_has_codes.get(), d_vecs.get());
+                HANN_LAUNCH_CHECK();
+            }
+            HANN_HIP(hipStreamSynchronize(stream));
+            for (size_t j = 0; j < J; ++j) {
+                dev_len[jobs[j]] = newlen[j];
+                deleted_ids[jobs[j]].clear();
+                touch_flags[jobs[j]] = 0;
+            }
+            HANN_HIP(hipMemcpyAsync(d_list_len.get(), dev_len.data(), dev_len.size() * 8, hipMemcpyHostToDevice,
+                                    stream));
+            HANN_HIP(hipStreamSynchronize(stream));
+            std::vector<int64_t> rest;
+            for (int64_t l : touched_lists)
+                if (touch_flags[l] != 0) rest.push_back(l);
+            touched_lists.swap(rest);
         }
         if (fits) {
             std::vector<int64_t> hids((size_t)total), dst((size_t)total);
@@ -579,6 +748,7 @@ void PQIVF::sync_device() {
                 }
                 dev_len[l] = (int64_t)L.ids.size();
                 touch_flags[l] = 0;
+                deleted_ids[l].clear();
             }
             touched_lists.clear();
             if (total > 0) {
@@ -612,6 +782,7 @@ void PQIVF::sync_device() {
     dev_len.assign(K, 0);
     append_from.assign(K, 0);
     touch_flags.assign(K, 0);
+    deleted_ids.assign(K, {});
     touched_lists.clear();
     int64_t N = 0;
     for (size_t c = 0; c < K; ++c) {

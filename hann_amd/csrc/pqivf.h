@@ -100,7 +100,8 @@ struct PQIVF {
     // incremental mirror updates: lists have slack capacity on the device;
     // appended tails or whole changed lists are uploaded at the next sync
     std::vector<int64_t> dev_start, dev_cap, dev_len, append_from;
-    std::vector<uint8_t> touch_flags;   // bit 0: appends only so far, bit 1: other change
+    std::vector<uint8_t> touch_flags;   // bit 0 appends, bit 1 deletes, bit 2 other changes
+    std::vector<std::vector<int64_t>> deleted_ids;   // per list, since the last sync
     std::vector<int64_t> touched_lists;
     int64_t incremental_syncs = 0, full_syncs = 0;
     bool any_uncoded = false;
@@ -136,6 +137,7 @@ struct PQIVF {
     void mark_dirty();           // the next sync rebuilds the whole device mirror
     void touch_append(int64_t l);   // entries are about to be appended to list l
     void touch_other(int64_t l);    // list l changed in any other way
+    void touch_delete(int64_t l, int64_t id);   // id was removed from list l
     void sync_device();
     void check_searchable(int64_t k, size_t len) const;
     void search_device(const float *dq, int64_t nq, const int32_t *dk, const int32_t *dnp, int kmax, int pmax_probe,

@@ -588,6 +588,16 @@ def test_incremental_mirror_equals_rebuild(hann):
         other.load_bytes(idx.save_bytes())
         want = other.search_batch(qs, 10, nprobe=6, scan_mode=0)
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    # delete-only lists are compacted in place on the device
+    for step in range(3):
+        live = np.concatenate([np.asarray(v, np.int64) for v in pq.InvertedIDs(idx).values()])
+        idx.BulkDelete(rng.choice(live, 400, replace=False).tolist() + [10 ** 9])   # unknown id skipped
+        other = pq.New(64)
+        other.load_bytes(idx.save_bytes())
+        for mode, fused in ((0, 0), (2, 1), (1, 0)):
+            got = idx.search_batch(qs, 10, nprobe=6, scan_mode=mode, fused=fused)
+            want = other.search_batch(qs, 10, nprobe=6, scan_mode=mode, fused=fused)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]), (step, mode)
     # a burst that overflows list capacity forces a full rebuild
     idx.bulk_add_arrays(np.arange(20000, 22000), np.repeat(x[:1], 2000, axis=0) + np.float32(1e-3))
     got = idx.search_batch(qs, 10, nprobe=6, scan_mode=0)
