@@ -10,9 +10,9 @@ from the same mixture, each with its own k in {10, 50, 100} and nprobe in
 Two search paths are measured on the same index and the same batches:
   plain   the FP32 drop-in: the reference's decode-and-measure PQ distance
           (bit-identical to pqivf/index.go:905-936) + an exact FP32 rerank
-  chosen  the persistent fused scan: FP16 ADC lookup tables built from
-          precomputed per-list tables, FP32 accumulation, autotuned waves
-          per query, + the same rerank
+  chosen  the certified BF16 MFMA coarse step (identical probes), then the
+          persistent split-table scan: one FP32 table per query plus a
+          per-entry list term, autotuned waves per query, + the same rerank
 The rerank depth of each path is the smallest of --rerank-grid whose
 recall@10 against FP64 ground truth reaches 0.95. `value` is the chosen
 path's queries per second.
@@ -86,9 +86,10 @@ class DevBatch:
         self.cnt = torch.empty(self.nq, dtype=torch.int32, device="cuda")
 
 
-def run_search(idx, b, stream, mode, rerank, fused):
+def run_search(idx, b, stream, mode, rerank, fused, coarse):
     idx.search_batch_dev(b.q.data_ptr(), b.nq, b.k.data_ptr(), b.np.data_ptr(), b.kmax, b.pmax, b.ids.data_ptr(),
-                         b.dist.data_ptr(), b.cnt.data_ptr(), stream, scan_mode=mode, rerank=rerank, fused=fused)
+                         b.dist.data_ptr(), b.cnt.data_ptr(), stream, scan_mode=mode, rerank=rerank, fused=fused,
+                         coarse_precision=coarse)
 
 
 def probe_bytes(torch, lib, idx, b, list_sizes, m):
@@ -315,26 +316,26 @@ def main():
     stream = torch.cuda.Stream()
     sh = stream.cuda_stream
 
-    def recall_of(mode, rerank, fused):
+    def recall_of(mode, rerank, fused, coarse):
         b = dev_batches[0]
-        run_search(idx, b, sh, mode, rerank, fused)
+        run_search(idx, b, sh, mode, rerank, fused, coarse)
         torch.cuda.synchronize()
         return recall_at_10(b.ids.cpu().numpy()[:ngt], gt)
 
     grid = [int(v) for v in a.rerank_grid.split(",")]
-    paths = {"plain": (_lib.SCAN_EXACT, 0), "chosen": (_lib.SCAN_ADC_F16, 1)}
+    paths = {"plain": (_lib.SCAN_EXACT, 0, _lib.PREC_FP32), "chosen": (_lib.SCAN_ADC_F32, 1, _lib.PREC_BF16)}
     chosen_r, recalls = {}, {}
-    for name, (mode, fused) in paths.items():
+    for name, (mode, fused, coarse) in paths.items():
         for r in grid:
-            rec = recall_of(mode, r, fused)
+            rec = recall_of(mode, r, fused, coarse)
             log(f"{name}: rerank {r} recall@10 {rec:.4f}")
             chosen_r[name], recalls[name] = r, rec
             if rec >= 0.95:
                 break
 
-    def timed(mode, rerank, fused, profile):
+    def timed(mode, rerank, fused, coarse, profile):
         for i in range(a.warmup):
-            run_search(idx, dev_batches[i], sh, mode, rerank, fused)
+            run_search(idx, dev_batches[i], sh, mode, rerank, fused, coarse)
         torch.cuda.synchronize()
         idx.set_profile(profile)
         idx.stage_times()
@@ -343,7 +344,7 @@ def main():
         t = time.perf_counter()
         for i in range(a.steps):
             evs[i][0].record(stream)
-            run_search(idx, dev_batches[a.warmup + i], sh, mode, rerank, fused)
+            run_search(idx, dev_batches[a.warmup + i], sh, mode, rerank, fused, coarse)
             evs[i][1].record(stream)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
@@ -353,8 +354,8 @@ def main():
         return el, lat, st
 
     res = {}
-    for name, (mode, fused) in paths.items():
-        el, lat, st = timed(mode, chosen_r[name], fused, True)
+    for name, (mode, fused, coarse) in paths.items():
+        el, lat, st = timed(mode, chosen_r[name], fused, coarse, True)
         qps = a.steps * a.batch / el
         res[name] = {"qps": qps, "ms_per_step": 1000 * el / a.steps, "p50_ms": float(np.percentile(lat, 50)),
                      "p99_ms": float(np.percentile(lat, 99)), "stages": st, "rerank": chosen_r[name],
@@ -373,7 +374,7 @@ def main():
     achieved = bytes_per_launch / (scan_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_scan_fused<ADC_F16>", "scan_ms_per_launch": round(scan_ms, 4),
+                "kernel": "k_scan_split<ADC_F32>", "scan_ms_per_launch": round(scan_ms, 4),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "pair_bytes_per_launch": int(pair_b / a.steps)}
 
@@ -411,9 +412,9 @@ def main():
         add_ids = np.arange(a.n, a.n + max(n_add, 1), dtype=np.int64)
         n_del = int(sizes[kind == 2].sum())
         del_ids = rng.permutation(a.n)[:max(n_del, 1)].astype(np.int64)
-        for name, (mode, fused) in paths.items():
+        for name, (mode, fused, coarse) in paths.items():
             srv = serving.Server(idx, max_batch=a.batch, slots=2, hbm_budget=2 << 30, scan_mode=mode,
-                                 rerank=chosen_r[name], fused=fused)
+                                 rerank=chosen_r[name], fused=fused, coarse_precision=coarse)
             try:
                 ids_out, _, lat, wall = srv.replay(at, kind, q_off, sizes, pool, kk, pp, add_ids, add_vecs,
                                                    del_ids, kmax_out=10)
@@ -443,7 +444,7 @@ def main():
     out = {
         "metric": METRIC, "value": round(ch["qps"], 1), "unit": "queries/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": round(ch["ms_per_step"], 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f16-lut+f32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None, "dtype": "bf16+f32", "data": "synthetic",
         "config": {"workload": f"pqivf search {a.n}x{a.dim} nlist {a.nlist} PQ{a.m}x256, batch {a.batch}, "
                                f"k in {{10,50,100}}, nprobe in {{8..128}}",
                    "n": a.n, "dim": a.dim, "nlist": a.nlist, "m": a.m, "pq_k": 256, "batch": a.batch,
@@ -453,7 +454,7 @@ def main():
         "recall_at_10": round(ch["recall"], 4),
         "chosen": {"qps": round(ch["qps"], 1), "recall_at_10": round(ch["recall"], 4), "rerank": ch["rerank"],
                    "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
-                   "path": "persistent fused adc-f16 scan (list tables, autotuned waves/query) + fp32 rerank",
+                   "path": "certified bf16 coarse + persistent split-table fp32 adc scan (autotuned waves/query) + fp32 rerank",
                    "stage_ms_per_step": [round(v / max(ch["stages"][3], 1), 4) for v in ch["stages"][:3]]},
         "plain": {"qps": round(pl["qps"], 1), "recall_at_10": round(pl["recall"], 4), "rerank": pl["rerank"],
                   "p50_ms": round(pl["p50_ms"], 3), "p99_ms": round(pl["p99_ms"], 3),

@@ -728,6 +728,7 @@ void PQIVF::sync_device() {
             }
             HANN_HIP(hipMemcpyAsync(d_list_len.get(), dev_len.data(), dev_len.size() * 8, hipMemcpyHostToDevice,
                                     stream));
+            refresh_terms(&jobs);
             HANN_HIP(hipStreamSynchronize(stream));
             std::vector<int64_t> rest;
             for (int64_t l : touched_lists)
@@ -735,6 +736,7 @@ void PQIVF::sync_device() {
             touched_lists.swap(rest);
         }
         if (fits) {
+            const std::vector<int64_t> changed = touched_lists;
             std::vector<int64_t> hids((size_t)total), dst((size_t)total);
             std::vector<uint8_t> hcodes((size_t)total * m * code_bytes), hhas((size_t)total);
             std::vector<float> hv((size_t)total * dim);
@@ -766,6 +768,7 @@ void PQIVF::sync_device() {
                 HANN_LAUNCH_CHECK();
                 HANN_HIP(hipMemcpyAsync(d_list_len.get(), dev_len.data(), dev_len.size() * 8, hipMemcpyHostToDevice,
                                         stream));
+                refresh_terms(&changed);
                 HANN_HIP(hipStreamSynchronize(stream));
             }
             n_entries = 0;
@@ -823,9 +826,29 @@ void PQIVF::sync_device() {
     up(d_cb, codebooks);
     up(d_cb_len, cb_len);
     (void)s;
-    tables_ready = false;
+    d_term.resize((size_t)N + 1);
+    refresh_terms(nullptr);
     HANN_HIP(hipStreamSynchronize(stream));
     dirty = false;
+}
+
+void PQIVF::refresh_terms(const std::vector<int64_t> *which) {
+    if (!have_codebooks || lists.empty()) return;
+    int64_t max_len = 0;
+    if (which) {
+        if (which->empty()) return;
+        for (int64_t l : *which) max_len = std::max(max_len, dev_len[l]);
+        d_term_lists.resize(which->size());
+        HANN_HIP(hipMemcpyAsync(d_term_lists.get(), which->data(), which->size() * sizeof(int64_t),
+                                hipMemcpyHostToDevice, stream));
+    } else {
+        for (int64_t v : dev_len) max_len = std::max(max_len, v);
+    }
+    launch_entry_terms(which ? d_term_lists.get() : nullptr, which ? (int64_t)which->size() : (int64_t)lists.size(),
+                       max_len, d_list_start.get(), d_list_len.get(), d_codes.get(), code_bytes, d_has_codes.get(),
+                       (int)dim, (int)m, (int)pmax, d_coarse.get(), d_cb.get(), d_term.get(), stream);
+    // the host vector behind an asynchronous copy must outlive it
+    HANN_HIP(hipStreamSynchronize(stream));
 }
 
 // --------------------------------------------------------------- search --
@@ -888,7 +911,13 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         if (np > 512) fail(HANN_ERR_UNSUPPORTED, strf("nprobe %d above 512 (and below coarseK %d) is not supported", np, K));
         probes = w.get<int64_t>(5, (size_t)nq * np);
         probe_rank = w.get<double>(6, (size_t)nq * np);
-        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, probe_rank, w, st);
+        if (p.coarse_precision == HANN_PREC_BF16)
+            // identical probes and ranks from the certified BF16 pass
+            certified_topk(coarse_ctx, dq, nq, d_coarse.get(), kc, d, np, nullptr, 0, probes, probe_rank, st);
+        else if (p.coarse_precision == HANN_PREC_FP32)
+            launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, probe_rank, w, st);
+        else
+            fail(HANN_ERR_UNSUPPORTED, strf("coarse precision %d is not supported", p.coarse_precision));
         probe_stride = np;
     }
     const int selk = R > 0 ? kmax * R : kmax;
@@ -932,7 +961,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     const bool fused = p.fused && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
     auto scan = [&](const ScanArgs &args) {
         if (fused)
-            run_fused_scan(args, nq, mode == SCAN_MODE_ADC_F16 ? 1 : 0, np, probe_rank, st, w);
+            run_fused_scan(args, nq, np, probe_rank, st, w);
         else
             launch_scan(mode, code_bytes, args, nq, st);
     };
@@ -963,47 +992,45 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     if (profile) t_events.push_back(ev);
 }
 
-void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int lut16, int maxp, const double *probe_rank,
-                           hipStream_t st, Workspace &w) {
-    const int K = (int)lists.size();
-    if (!tables_ready) {
-        d_T.resize((size_t)K * m * pmax + 1);
-        launch_list_tables(d_coarse.get(), K, (int)dim, (int)m, (int)pmax, d_cb.get(), d_cb_len.get(), d_T.get(), st);
-        tables_ready = true;
-    }
+void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double *probe_rank, hipStream_t st,
+                           Workspace &w) {
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
         HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
     FusedArgs f{};
-    f.T = d_T.get();
+    f.term = d_term.get();
     f.probe_rank = a.probe_stride > 0 ? probe_rank : nullptr;
     f.order = w.get<int32_t>(12, (size_t)nq);
     f.counter = w.get<int32_t>(13, 1);
-    auto grid_for = [&](int wpq) {
-        const size_t shm = fused_lds_bytes(wpq, lut16 ? 2 : 4, (int)dim, (int)m, (int)pmax, a.cap);
+    // per-probe tables in LDS when each query has its own probe row
+    f.lds_probes = a.probe_stride > 0 ? a.probe_cap : 0;
+    const int kp = a.cap / 2;
+    auto lds_for = [&](int wv) { return block_lds_bytes((int)dim, (int)m, (int)pmax, block_cap(kp, wv), f.lds_probes); };
+    auto grid_for = [&](int wv) {
+        const size_t shm = lds_for(wv);
         int per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(shm, 1));
-        per_cu = std::min(per_cu, 16 / wpq);
+        per_cu = std::min(per_cu, 32 / wv);
         return (int)std::min<int64_t>(nq, (int64_t)num_cus * per_cu);
     };
-    auto fits = [&](int wpq) { return fused_lds_bytes(wpq, lut16 ? 2 : 4, (int)dim, (int)m, (int)pmax, a.cap) <= 160 * 1024; };
+    auto fits = [&](int wv) { return lds_for(wv) <= 160 * 1024; };
     // the tuner: time every fitting waves-per-query choice once on this batch
     // shape, keep the fastest (results are identical for every choice)
-    const uint64_t key = ((uint64_t)a.cap << 20) ^ ((uint64_t)lut16 << 16) ^ (uint64_t)(maxp >= 64 ? 2 : maxp >= 16 ? 1 : 0);
+    const uint64_t key = ((uint64_t)a.cap << 20) ^ (uint64_t)(maxp >= 64 ? 2 : maxp >= 16 ? 1 : 0);
     auto it = tuned_wpq.find(key);
     int wpq = 0;
     if (it != tuned_wpq.end()) {
         wpq = it->second;
     } else {
         float best = 1e30f;
-        for (int w : {1, 2, 4}) {
-            if (!fits(w)) continue;
+        for (int wv : {2, 4, 8}) {
+            if (!fits(wv)) continue;
             hipEvent_t e0, e1;
             HANN_HIP(hipEventCreate(&e0));
             HANN_HIP(hipEventCreate(&e1));
             HANN_HIP(hipEventRecord(e0, st));
-            launch_scan_fused(a, f, nq, w, lut16, code_bytes, maxp, grid_for(w), st);
+            launch_scan_block(a, f, nq, wv, kp, code_bytes, maxp, grid_for(wv), st);
             HANN_HIP(hipEventRecord(e1, st));
             HANN_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -1012,14 +1039,14 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int lut16, int maxp, c
             (void)hipEventDestroy(e1);
             if (ms < best) {
                 best = ms;
-                wpq = w;
+                wpq = wv;
             }
         }
-        if (wpq == 0) fail(HANN_ERR_UNSUPPORTED, "fused scan: no configuration fits in LDS");
+        if (wpq == 0) fail(HANN_ERR_UNSUPPORTED, "persistent scan: no configuration fits in LDS");
         tuned_wpq[key] = wpq;
-        last_tune = strf("fused scan tuned: waves/query %d (%.3f ms)", wpq, best);
+        last_tune = strf("persistent scan tuned: waves/query %d (%.3f ms)", wpq, best);
     }
-    launch_scan_fused(a, f, nq, wpq, lut16, code_bytes, maxp, grid_for(wpq), st);
+    launch_scan_block(a, f, nq, wpq, kp, code_bytes, maxp, grid_for(wpq), st);
 }
 
 void PQIVF::resolve_timers() {

@@ -111,10 +111,13 @@ struct PQIVF {
     DevBuf<uint8_t> d_codes, d_has_codes;
     DevBuf<float> d_vecs, d_coarse, d_cb, scratch_f;
     DevBuf<unsigned long long> d_fb_counter;
-    // fused scan: list tables T[K][m][P] (built on first use) and tuned waves per query
-    DevBuf<float> d_T;
-    bool tables_ready = false;
-    std::map<uint64_t, int> tuned_wpq;   // key: (cap, lut16, maxp bucket)
+    // persistent scan: per-entry list terms, kept current by every sync
+    DevBuf<float> d_term;
+    DevBuf<int64_t> d_term_lists;
+    void refresh_terms(const std::vector<int64_t> *lists);   // null: every list
+    // the coarse step's BF16 candidate pass (used under the index lock)
+    Context coarse_ctx;
+    std::map<uint64_t, int> tuned_wpq;   // persistent scan waves per query; key: (cap, maxp bucket)
     int num_cus = 0;
     std::string last_tune;
 
@@ -147,8 +150,8 @@ struct PQIVF {
     void search_host(const float *queries, int64_t nq, const int32_t *k, const int32_t *np,
                      const hann_search_params &p, int64_t *out_ids, double *out_dist, int32_t *out_counts);
     int64_t fallback_count();
-    void run_fused_scan(const ScanArgs &a, int64_t nq, int lut16, int maxp, const double *probe_rank,
-                        hipStream_t st, Workspace &w);
+    void run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double *probe_rank, hipStream_t st,
+                        Workspace &w);
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;

@@ -6,18 +6,26 @@
 namespace hann {
 
 struct FusedArgs {
-    const float *T;            // [K][m][P] list tables
+    const float *term;         // [entries] sum_i |cb_i|^2 + 2 <c_l,i, cb_i> of each entry's codes
     const double *probe_rank;  // [nq][probe_stride] exact coarse ranks of the probes, or null
     int32_t *order;            // [nq] work queue (written by the launch)
     int32_t *counter;          // work counter (reset by the launch)
     int nq;
+    int lds_probes;            // probes per query kept in LDS (0: read from global memory)
 };
 
-void launch_list_tables(const float *coarse, int K, int d, int m, int P, const float *cb, const int64_t *cb_len,
-                        float *T, hipStream_t st);
-size_t fused_lds_bytes(int wpq, int lut_bytes, int d, int m, int P, int cap);
-// wpq waves per query (1, 2 or 4); grid = persistent workgroups
-void launch_scan_fused(const ScanArgs &a, FusedArgs f, int64_t nq, int wpq, int lut16, int code_bytes, int maxp,
+size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes);
+// top-k room of the scan: a power of two >= max(2 kp, kp + 64 w)
+int block_cap(int kp, int w);
+// per-entry terms of the split scan for lists[0..nlists) (every list when
+// lists is null; nlists is then K); max_len bounds the lists' lengths
+void launch_entry_terms(const int64_t *lists, int64_t nlists, int64_t max_len, const int64_t *list_start,
+                        const int64_t *list_len, const void *codes, int code_bytes, const uint8_t *has, int d, int m,
+                        int P, const float *coarse, const float *cb, float *term, hipStream_t st);
+// the persistent scan (FP32 query table S_q, per-entry terms): w waves
+// (2, 4 or 8) per query sharing one top-k of room block_cap(kp, w), with
+// kp = kp_for(largest selection size); grid = persistent workgroups
+void launch_scan_block(const ScanArgs &a, FusedArgs f, int64_t nq, int w, int kp, int code_bytes, int maxp,
                        int grid, hipStream_t st);
 
 }  // namespace hann

@@ -1,12 +1,15 @@
 // scan_fused.hip -- persistent ADC list scan over a query work queue (gfx950).
 //
-// The per-(query, list) lookup table uses the expansion
-//   |q - c_l - cb_ij|^2 = |q - c_l|^2 + sum_i ( T_l[i][j] + S_q[i][j] )
-//   T_l[i][j] = |cb_ij|^2 + 2 <c_l,i , cb_ij>      (per list, built at sync)
-//   S_q[i][j] = -2 <q_i , cb_ij>                   (per query, built once)
-// so a list's table costs m*P additions instead of m*P*s multiply-adds.
-// |q - c_l|^2 is the coarse step's exact rank of list l (or is recomputed
-// for lists reached by the fallback).
+// The scan never builds a per-(query, list) lookup table. With y_e the PQ
+// reconstruction of entry e of list l (code_e,i picks cb_i,code),
+//   |q - c_l - y_e|^2 = |q - c_l|^2 + t_e + sum_i S_q[i][code_e,i]
+//   t_e       = sum_i ( |cb_i,code|^2 + 2 <c_l,i , cb_i,code> )   (per entry, at sync)
+//   S_q[i][j] = -2 <q_i , cb_ij>                                  (per query)
+// so the only table is the query's S_q, built once per query (m*P entries,
+// FP32 in LDS) and read by every list the query probes. |q - c_l|^2 is the
+// coarse step's exact rank of list l (or is recomputed for lists reached by
+// the fallback). S_q stays FP32: its entries are not small beside the
+// distance, so rounding them to FP16 would cost accuracy.
 //
 // Workgroups are persistent: each pulls the next query from an atomic
 // counter over a cost-ordered queue (largest nprobe first), so the ragged
@@ -19,27 +22,6 @@
 
 namespace hann {
 namespace dev {
-
-// T[l][i][j] for every list (grid: K x m)
-__global__ void k_list_tables(const float *__restrict__ coarse, int d, int m, int s, int P,
-                              const float *__restrict__ cb, const int64_t *__restrict__ cb_len,
-                              float *__restrict__ T) {
-    const int l = blockIdx.x, i = blockIdx.y;
-    const float *c = coarse + (int64_t)l * d + i * s;
-    for (int j = threadIdx.x; j < P; j += blockDim.x) {
-        float v = __builtin_inff();
-        if (j < (int)cb_len[i]) {
-            const float *w = cb + ((int64_t)i * P + j) * s;
-            float nn = 0.f, dc = 0.f;
-            for (int t = 0; t < s; ++t) {
-                nn = __builtin_fmaf(w[t], w[t], nn);
-                dc = __builtin_fmaf(c[t], w[t], dc);
-            }
-            v = nn + 2.f * dc;
-        }
-        T[((int64_t)l * m + i) * P + j] = v;
-    }
-}
 
 // queue order: queries by descending nprobe (counting sort, one workgroup)
 __global__ void __launch_bounds__(1024) k_cost_order(const int32_t *__restrict__ nprobe_q, int nprobe_default,
@@ -74,26 +56,51 @@ __global__ void __launch_bounds__(1024) k_cost_order(const int32_t *__restrict__
     }
 }
 
-template <typename LutT>
-__device__ __forceinline__ float lut_get(const LutT *lut, int idx);
-template <>
-__device__ __forceinline__ float lut_get<float>(const float *lut, int idx) { return lut[idx]; }
-template <>
-__device__ __forceinline__ float lut_get<__half>(const __half *lut, int idx) { return __half2float(lut[idx]); }
+// t_e for the entries of the given lists (grid: x over entries, y over lists)
+template <typename CodeT>
+__global__ void k_entry_terms(const int64_t *__restrict__ lists, int64_t j0, const int64_t *__restrict__ list_start,
+                              const int64_t *__restrict__ list_len, const CodeT *__restrict__ codes,
+                              const uint8_t *__restrict__ has, int d, int m, int P, const float *__restrict__ coarse,
+                              const float *__restrict__ cb, float *__restrict__ term) {
+    const int64_t j = j0 + blockIdx.y;
+    const int64_t l = lists ? lists[j] : j;
+    const int s = d / m;
+    const float *c = coarse + l * d;
+    const int64_t e0 = list_start[l], n = list_len[l];
+    for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t e = e0 + r;
+        float t = 0.f;
+        if (has[e]) {
+            for (int i = 0; i < m; ++i) {
+                const float *w = cb + ((int64_t)i * P + (int)codes[e * m + i]) * s;
+                float nn = 0.f, dc = 0.f;
+                for (int u = 0; u < s; ++u) {
+                    nn = __builtin_fmaf(w[u], w[u], nn);
+                    dc = __builtin_fmaf(c[i * s + u], w[u], dc);
+                }
+                t = t + (nn + 2.f * dc);
+            }
+        }
+        term[e] = t;
+    }
+}
 
-template <typename LutT>
-__device__ __forceinline__ LutT lut_make(float v);
-template <>
-__device__ __forceinline__ float lut_make<float>(float v) { return v; }
-template <>
-__device__ __forceinline__ __half lut_make<__half>(float v) { return __float2half(v); }
-
-// Sum of an entry's table values in subquantizer order, starting at qc.
-// M > 0 (uint8 codes, M % 4 == 0): the codes are read as M/4 dwords.
-template <int M, typename CodeT, typename LutT>
-__device__ __forceinline__ float adc_sum(const void *codes, int64_t e, int m, int P, const LutT *lut, float qc) {
-    float acc = qc;
-    if constexpr (M > 0 && sizeof(CodeT) == 1 && M % 4 == 0) {
+// acc + sum_i S[i][code_i] in subquantizer order (uint8 codes)
+template <int M>
+__device__ __forceinline__ float split_sum(const uint8_t *codes, int64_t e, int m, int P, const float *S, float acc) {
+    if constexpr (M > 0 && M % 8 == 0) {
+        const uint2 *cw = reinterpret_cast<const uint2 *>(codes) + e * (M / 8);
+        uint2 w[M / 8];
+#pragma unroll
+        for (int j = 0; j < M / 8; ++j) w[j] = cw[j];
+#pragma unroll
+        for (int j = 0; j < M / 8; ++j) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc = acc + S[(8 * j + b) * P + (int)((w[j].x >> (8 * b)) & 0xFFu)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) acc = acc + S[(8 * j + 4 + b) * P + (int)((w[j].y >> (8 * b)) & 0xFFu)];
+        }
+    } else if constexpr (M > 0 && M % 4 == 0) {
         const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes) + e * (M / 4);
         uint32_t w[M / 4];
 #pragma unroll
@@ -101,30 +108,99 @@ __device__ __forceinline__ float adc_sum(const void *codes, int64_t e, int m, in
 #pragma unroll
         for (int j = 0; j < M / 4; ++j)
 #pragma unroll
-            for (int b = 0; b < 4; ++b) acc = acc + lut_get<LutT>(lut, (4 * j + b) * P + (int)((w[j] >> (8 * b)) & 0xFFu));
+            for (int b = 0; b < 4; ++b) acc = acc + S[(4 * j + b) * P + (int)((w[j] >> (8 * b)) & 0xFFu)];
     } else {
-        const CodeT *ce = reinterpret_cast<const CodeT *>(codes) + e * m;
-        for (int i = 0; i < m; ++i) acc = acc + lut_get<LutT>(lut, i * P + (int)ce[i]);
+        const uint8_t *ce = codes + e * m;
+        for (int i = 0; i < m; ++i) acc = acc + S[i * P + (int)ce[i]];
     }
     return acc;
 }
 
-// LDS: q[d] | S_q[m*P] (LutT) | per wave: lut[m*P] (LutT), keys[cap], ids[cap]
-template <int WPQ, typename CodeT, typename LutT, int M>
-__global__ void __launch_bounds__(64 * WPQ) k_scan_fused(ScanArgs a, FusedArgs f) {
+template <int M>
+__device__ __forceinline__ float split_sum16(const uint16_t *codes, int64_t e, int m, int P, const float *S,
+                                             float acc) {
+    const uint16_t *ce = codes + e * m;
+    for (int i = 0; i < m; ++i) acc = acc + S[i * P + (int)ce[i]];
+    return acc;
+}
+
+// One query's top-k shared by the whole workgroup: LDS keys/ids[cap], the
+// fill n and the threshold (tk, ti) are block-uniform. Rounds keep every
+// wave in step, so compaction can use block barriers.
+struct BlockTopK {
+    float *keys;
+    int64_t *ids;
+    int cap, k, n;
+    float tk;
+    int64_t ti;
+
+    __device__ void init(float *kb, int64_t *ib, int k_, int cap_) {
+        keys = kb;
+        ids = ib;
+        k = k_;
+        cap = cap_;
+        n = 0;
+        tk = __builtin_inff();
+        ti = INT64_MAX;
+    }
+
+    // sort all slots (those at and above n padded with +inf), keep the best
+    // k; every thread of the block calls it
+    __device__ void compact() {
+        const int tid = threadIdx.x, nt = blockDim.x;
+        for (int t = n + tid; t < cap; t += nt) {
+            keys[t] = __builtin_inff();
+            ids[t] = INT64_MAX;
+        }
+        __syncthreads();
+        for (int size = 2; size <= cap; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = tid; t < (cap >> 1); t += nt) {
+                    const int lo = 2 * t - (t & (stride - 1));
+                    const int hi = lo + stride;
+                    const bool up = (lo & size) == 0;
+                    const float ka = keys[lo], kb = keys[hi];
+                    const int64_t ia = ids[lo], ib = ids[hi];
+                    const bool swap = up ? cand_less(kb, ib, ka, ia) : cand_less(ka, ia, kb, ib);
+                    if (swap) {
+                        keys[lo] = kb;
+                        keys[hi] = ka;
+                        ids[lo] = ib;
+                        ids[hi] = ia;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        if (n > k) n = k;
+        if (n == k) {
+            tk = keys[k - 1];
+            ti = ids[k - 1];
+        }
+    }
+};
+
+// LDS: q[d] | S_q[m*P] (FP32) | keys[cap] | ids[cap] | per probe: list, chunks, |q-c|^2
+template <int W, typename CodeT, int M>
+__global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     __shared__ int s_item;
+    __shared__ int64_t s_red[2][W];
+    __shared__ int s_cnt[2][W];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
     const int d = a.d, m = a.m, s = a.s, P = a.pmax_cb, MP = m * P;
+    const int cap = a.cap;
     float *s_q = reinterpret_cast<float *>(s_raw);
     size_t off = ((size_t)d * 4 + 15) & ~(size_t)15;
-    LutT *s_S = reinterpret_cast<LutT *>(s_raw + off);
-    off += ((size_t)MP * sizeof(LutT) + 15) & ~(size_t)15;
-    const size_t per_wave = (((size_t)MP * sizeof(LutT) + 15) & ~(size_t)15) + (size_t)a.cap * 12;
-    unsigned char *wbase = s_raw + off + wave * per_wave;
-    LutT *s_lut = reinterpret_cast<LutT *>(wbase);
-    float *s_keys = reinterpret_cast<float *>(wbase + (((size_t)MP * sizeof(LutT) + 15) & ~(size_t)15));
-    int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + a.cap);
+    float *s_S = reinterpret_cast<float *>(s_raw + off);
+    off += ((size_t)MP * 4 + 15) & ~(size_t)15;
+    float *s_keys = reinterpret_cast<float *>(s_raw + off);
+    off += ((size_t)cap * 4 + 15) & ~(size_t)15;
+    int64_t *s_ids = reinterpret_cast<int64_t *>(s_raw + off);
+    off += (size_t)cap * 8;
+    int32_t *s_lst = reinterpret_cast<int32_t *>(s_raw + off);
+    int32_t *s_nch = s_lst + f.lds_probes;
+    float *s_qc = reinterpret_cast<float *>(s_nch + f.lds_probes);
 
     for (;;) {
         __syncthreads();
@@ -142,152 +218,234 @@ __global__ void __launch_bounds__(64 * WPQ) k_scan_fused(ScanArgs a, FusedArgs f
         if (npq > a.probe_cap) npq = a.probe_cap;
         if (npq < 0) npq = 0;
         const int64_t *probes = a.probes + q * a.probe_stride;
+        const bool in_lds = npq <= f.lds_probes;
 
         for (int e = tid; e < d; e += blockDim.x) s_q[e] = a.queries[q * d + e];
         __syncthreads();
         // S_q[i][j] = -2 <q_i, cb_ij>
         for (int t = tid; t < MP; t += blockDim.x) {
-            const int i = t / P, j = t % P;
+            const int i = t / P, j = t - i * P;
             const float *w = a.cb + ((int64_t)i * P + j) * s;
             float dc = 0.f;
             for (int u = 0; u < s; ++u) dc = __builtin_fmaf(s_q[i * s + u], w[u], dc);
-            s_S[t] = lut_make<LutT>(-2.f * dc);
+            s_S[t] = -2.f * dc;
         }
-        int64_t total = 0;
-        for (int j = tid; j < npq; j += blockDim.x) total += a.list_len[probes[j]];
-        for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o);
-        // sum the per-wave partial counts through LDS
-        __shared__ int64_t s_tot[16];
-        if (lane == 0) s_tot[wave] = total;
-        __syncthreads();
-        total = 0;
-        for (int w = 0; w < WPQ; ++w) total += s_tot[w];
-        const bool fb = total < kq && a.allow_fallback;
+        // probed lists: entries and 64-entry chunks
+        int64_t tot_e = 0, tot_c = 0;
+        for (int li = tid; li < npq; li += blockDim.x) {
+            const int64_t l = probes[li];
+            const int64_t len = a.list_len[l];
+            tot_e += len;
+            tot_c += (len + kWave - 1) / kWave;
+            if (in_lds) {
+                s_lst[li] = (int32_t)l;
+                s_nch[li] = (int32_t)((len + kWave - 1) / kWave);
+                s_qc[li] = f.probe_rank ? (float)f.probe_rank[q * a.probe_stride + li]
+                                        : sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.coarse + l * d}, d);
+            }
+        }
+        for (int o = 32; o > 0; o >>= 1) {
+            tot_e += __shfl_xor(tot_e, o);
+            tot_c += __shfl_xor(tot_c, o);
+        }
+        if (lane == 0) {
+            s_red[0][wave] = tot_e;
+            s_red[1][wave] = tot_c;
+        }
+        __syncthreads();   // S_q, the probe table and the partial counts are complete
+        tot_e = 0;
+        tot_c = 0;
+        for (int w = 0; w < W; ++w) {
+            tot_e += s_red[0][w];
+            tot_c += s_red[1][w];
+        }
+        const bool fb = tot_e < kq && a.allow_fallback;
         if (tid == 0 && fb && a.fb_counter) atomicAdd(a.fb_counter, 1ull);
         const int nl = fb ? a.K : npq;
+        const bool tab = in_lds && !fb;
+        if (fb) {
+            // every list: count its chunks (rare; lists read from global memory)
+            tot_c = 0;
+            for (int l = tid; l < a.K; l += blockDim.x) tot_c += (a.list_len[l] + kWave - 1) / kWave;
+            for (int o = 32; o > 0; o >>= 1) tot_c += __shfl_xor(tot_c, o);
+            __syncthreads();
+            if (lane == 0) s_red[1][wave] = tot_c;
+            __syncthreads();
+            tot_c = 0;
+            for (int w = 0; w < W; ++w) tot_c += s_red[1][w];
+        }
 
-        WaveTopK sel;
-        sel.init(s_keys, s_ids, selk, a.cap);
-        wave_sync();
-        for (int li = wave; li < nl; li += WPQ) {
-            const int64_t l = fb ? li : probes[li];
-            const int64_t e0 = a.list_start[l], e1 = e0 + a.list_len[l];
-            if (e1 == e0) continue;
-            // |q - c_l|^2: the coarse rank when the list was probed
-            float qc;
-            if (!fb && f.probe_rank) {
-                qc = (float)f.probe_rank[q * a.probe_stride + li];
+        BlockTopK sel;
+        sel.init(s_keys, s_ids, selk, cap);
+        // this wave's work item: chunk c of probe li; items go round-robin
+        // over the waves, one per wave per round
+        int li = -1, c = wave;
+        int64_t l = 0, e0 = 0, len = 0, nch = 0;
+        float qc = 0.f;
+        auto enter = [&](int at) {
+            li = at;
+            if (li >= nl) return;
+            if (tab) {
+                l = s_lst[li];
+                nch = s_nch[li];
+                qc = s_qc[li];
             } else {
-                qc = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.coarse + l * d}, d);
+                l = fb ? li : probes[li];
+                nch = (a.list_len[l] + kWave - 1) / kWave;
+                qc = (!fb && f.probe_rank) ? (float)f.probe_rank[q * a.probe_stride + li]
+                                           : sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.coarse + l * d}, d);
             }
-            const float *Tl = f.T + l * (int64_t)MP;
-            for (int t = lane; t < MP; t += kWave) s_lut[t] = lut_make<LutT>(Tl[t] + lut_get<LutT>(s_S, t));
-            wave_sync();
-            for (int64_t b = e0; b < e1; b += kWave) {
-                const int64_t e = b + lane;
-                const bool valid = e < e1;
-                float key = 0.f;
-                if (valid) {
-                    if (!a.any_uncoded || a.has_codes[e]) {
-                        key = adc_sum<M, CodeT, LutT>(a.codes, e, m, P, s_lut, qc);
-                    } else {
-                        key = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.vecs + e * d}, d);
-                    }
+            e0 = a.list_start[l];
+            len = a.list_len[l];
+        };
+        enter(0);
+        const int64_t rounds = (tot_c + W - 1) / W;
+        for (int64_t r = 0; r < rounds; ++r) {
+            while (li < nl && c >= nch) {
+                c -= (int)nch;
+                enter(li + 1);
+            }
+            const int64_t e = e0 + (int64_t)c * kWave + lane;
+            const bool valid = li < nl && (int64_t)c * kWave + lane < len;
+            float key = 0.f;
+            int64_t id = 0;
+            if (valid) {
+                if (!a.any_uncoded || a.has_codes[e]) {
+                    if constexpr (sizeof(CodeT) == 1)
+                        key = split_sum<M>(reinterpret_cast<const uint8_t *>(a.codes), e, m, P, s_S, qc + f.term[e]);
+                    else
+                        key = split_sum16<M>(reinterpret_cast<const uint16_t *>(a.codes), e, m, P, s_S,
+                                             qc + f.term[e]);
+                } else {
+                    key = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.vecs + e * d}, d);
                 }
-                const int64_t id = a.rerank > 0 ? e : (valid ? a.ids[e] : 0);
-                sel.push(key, id, valid);
+                key = sane_key(key);
+                id = a.rerank > 0 ? e : a.ids[e];
             }
-            wave_sync();
+            c += W;
+            // append the round's survivors; the counts go through LDS so
+            // that every thread knows the block's fill without atomics
+            bool pass = valid && cand_less(key, id, sel.tk, sel.ti);
+            uint64_t mask = __ballot(pass);
+            const int par = (int)(r & 1);
+            if (lane == 0) s_cnt[par][wave] = __popcll(mask);
+            __syncthreads();
+            int sum = 0, base = 0;
+            for (int w = 0; w < W; ++w) {
+                const int v = s_cnt[par][w];
+                base += w < wave ? v : 0;
+                sum += v;
+            }
+            if (sel.n + sum > sel.cap) {
+                sel.compact();   // cap >= k + 64 W: the round now fits
+                __syncthreads();
+                pass = pass && cand_less(key, id, sel.tk, sel.ti);
+                mask = __ballot(pass);
+                if (lane == 0) s_cnt[par][wave] = __popcll(mask);
+                __syncthreads();
+                sum = 0;
+                base = 0;
+                for (int w = 0; w < W; ++w) {
+                    const int v = s_cnt[par][w];
+                    base += w < wave ? v : 0;
+                    sum += v;
+                }
+                __syncthreads();   // the counts are read before the next round with this parity
+            }
+            if (pass) {
+                const int pos = sel.n + base +
+                                (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+                s_keys[pos] = key;
+                s_ids[pos] = id;
+            }
+            sel.n += sum;
         }
-        int got = sel.finish();
-        // merge the waves' lists into wave 0's
-        __shared__ int s_got[16];
-        if (lane == 0) s_got[wave] = got;
         __syncthreads();
-        if (wave == 0) {
-            for (int w = 1; w < WPQ; ++w) {
-                const float *ok = reinterpret_cast<const float *>(
-                    s_raw + off + w * per_wave + (((size_t)MP * sizeof(LutT) + 15) & ~(size_t)15));
-                const int64_t *oi = reinterpret_cast<const int64_t *>(ok + a.cap);
-                const int n = s_got[w];
-                for (int b = 0; b < n; b += kWave) {
-                    const int t = b + lane;
-                    sel.push(t < n ? ok[t] : 0.f, t < n ? oi[t] : 0, t < n);
-                }
-            }
-            got = sel.finish();
-            float *okey = a.out_key + q * a.out_stride;
-            int64_t *oid = a.out_id + q * a.out_stride;
-            // every slot of the output row is written: the result, then (inf, -1)
-            for (int t = lane; t < a.out_stride; t += kWave) {
-                const bool ok = t < got && t < selk;
-                okey[t] = ok ? sel.keys[t] : __builtin_inff();
-                oid[t] = ok ? sel.ids[t] : (int64_t)-1;
-            }
-            if (lane == 0) a.out_cnt[q] = got < selk ? got : selk;
+        sel.compact();
+        __syncthreads();
+        const int got = sel.n;
+        float *okey = a.out_key + q * a.out_stride;
+        int64_t *oid = a.out_id + q * a.out_stride;
+        // every slot of the output row is written: the result, then (inf, -1)
+        for (int t = tid; t < a.out_stride; t += blockDim.x) {
+            const bool ok = t < got && t < selk;
+            okey[t] = ok ? s_keys[t] : __builtin_inff();
+            oid[t] = ok ? s_ids[t] : (int64_t)-1;
         }
+        if (tid == 0) a.out_cnt[q] = got < selk ? got : selk;
     }
 }
 
 }  // namespace dev
 
-void launch_list_tables(const float *coarse, int K, int d, int m, int P, const float *cb, const int64_t *cb_len,
-                        float *T, hipStream_t st) {
-    if (K == 0) return;
-    hipLaunchKernelGGL(dev::k_list_tables, dim3((unsigned)K, (unsigned)m), dim3(256), 0, st, coarse, d, m, d / m, P,
-                       cb, cb_len, T);
-    HANN_LAUNCH_CHECK();
+size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes) {
+    return (((size_t)d * 4 + 15) & ~(size_t)15) + (((size_t)m * P * 4 + 15) & ~(size_t)15) +
+           (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (size_t)lds_probes * 12 + 16;
 }
 
-size_t fused_lds_bytes(int wpq, int lut_bytes, int d, int m, int P, int cap) {
-    const size_t mp = (((size_t)m * P * lut_bytes) + 15) & ~(size_t)15;
-    return (((size_t)d * 4 + 15) & ~(size_t)15) + mp + (size_t)wpq * (mp + (size_t)cap * 12) + 16;
+int block_cap(int kp, int w) {
+    int c = 64;
+    while (c < 2 * kp || c < kp + 64 * w) c <<= 1;
+    return c;
 }
 
-void launch_scan_fused(const ScanArgs &a, FusedArgs f, int64_t nq, int wpq, int lut16, int code_bytes, int maxp,
+void launch_entry_terms(const int64_t *lists, int64_t nlists, int64_t max_len, const int64_t *list_start,
+                        const int64_t *list_len, const void *codes, int code_bytes, const uint8_t *has, int d, int m,
+                        int P, const float *coarse, const float *cb, float *term, hipStream_t st) {
+    if (nlists <= 0 || max_len <= 0) return;
+    const unsigned gx = (unsigned)std::min<int64_t>(64, ceil_div(max_len, 256));
+    for (int64_t j0 = 0; j0 < nlists; j0 += 65535) {
+        const unsigned gy = (unsigned)std::min<int64_t>(65535, nlists - j0);
+        if (code_bytes == 1)
+            hipLaunchKernelGGL(dev::k_entry_terms<uint8_t>, dim3(gx, gy), dim3(256), 0, st, lists, j0, list_start,
+                               list_len, reinterpret_cast<const uint8_t *>(codes), has, d, m, P, coarse, cb, term);
+        else
+            hipLaunchKernelGGL(dev::k_entry_terms<uint16_t>, dim3(gx, gy), dim3(256), 0, st, lists, j0, list_start,
+                               list_len, reinterpret_cast<const uint16_t *>(codes), has, d, m, P, coarse, cb, term);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+void launch_scan_block(const ScanArgs &a_in, FusedArgs f, int64_t nq, int w, int kp, int code_bytes, int maxp,
                        int grid, hipStream_t st) {
     if (nq == 0) return;
+    ScanArgs a = a_in;
+    a.cap = block_cap(kp, w);
     f.nq = (int)nq;
     hipLaunchKernelGGL(dev::k_cost_order, dim3(1), dim3(1024), 0, st, a.nprobe_q, a.nprobe_default, (int)nq, maxp,
                        f.order, f.counter);
     HANN_LAUNCH_CHECK();
-    const size_t shm = fused_lds_bytes(wpq, lut16 ? 2 : 4, a.d, a.m, a.pmax_cb, a.cap);
-    if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("fused scan needs %zu bytes of LDS", shm));
-#define HANN_FUSED(W, CT, LT, M)                                                                          \
+    const size_t shm = block_lds_bytes(a.d, a.m, a.pmax_cb, a.cap, f.lds_probes);
+    if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("list scan needs %zu bytes of LDS", shm));
+#define HANN_BLOCK(W, CT, M)                                                                              \
     do {                                                                                                  \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_fused<W, CT, LT, M>,                       \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_block<W, CT, M>,                           \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));              \
-        hipLaunchKernelGGL((dev::k_scan_fused<W, CT, LT, M>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, f); \
+        hipLaunchKernelGGL((dev::k_scan_block<W, CT, M>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, f); \
         HANN_LAUNCH_CHECK();                                                                              \
     } while (0)
-#define HANN_FUSED_W(CT, LT, M)                         \
-    do {                                                \
-        if (wpq == 1) HANN_FUSED(1, CT, LT, M);         \
-        else if (wpq == 2) HANN_FUSED(2, CT, LT, M);    \
-        else HANN_FUSED(4, CT, LT, M);                  \
-    } while (0)
-#define HANN_FUSED_M(LT)                                                    \
-    do {                                                                    \
-        switch (a.m) {                                                      \
-        case 8: HANN_FUSED_W(uint8_t, LT, 8); break;                        \
-        case 16: HANN_FUSED_W(uint8_t, LT, 16); break;                      \
-        case 24: HANN_FUSED_W(uint8_t, LT, 24); break;                      \
-        case 32: HANN_FUSED_W(uint8_t, LT, 32); break;                      \
-        case 48: HANN_FUSED_W(uint8_t, LT, 48); break;                      \
-        case 64: HANN_FUSED_W(uint8_t, LT, 64); break;                      \
-        default: HANN_FUSED_W(uint8_t, LT, 0); break;                       \
-        }                                                                   \
+#define HANN_BLOCK_W(CT, M)                         \
+    do {                                            \
+        if (w == 2) HANN_BLOCK(2, CT, M);           \
+        else if (w == 4) HANN_BLOCK(4, CT, M);      \
+        else HANN_BLOCK(8, CT, M);                  \
     } while (0)
     if (code_bytes == 1) {
-        if (lut16) HANN_FUSED_M(__half);
-        else HANN_FUSED_M(float);
+        switch (a.m) {
+        case 8: HANN_BLOCK_W(uint8_t, 8); break;
+        case 16: HANN_BLOCK_W(uint8_t, 16); break;
+        case 24: HANN_BLOCK_W(uint8_t, 24); break;
+        case 32: HANN_BLOCK_W(uint8_t, 32); break;
+        case 48: HANN_BLOCK_W(uint8_t, 48); break;
+        case 64: HANN_BLOCK_W(uint8_t, 64); break;
+        default: HANN_BLOCK_W(uint8_t, 0); break;
+        }
     } else {
-        if (lut16) HANN_FUSED_W(uint16_t, __half, 0);
-        else HANN_FUSED_W(uint16_t, float, 0);
+        HANN_BLOCK_W(uint16_t, 0);
     }
-#undef HANN_FUSED_M
-#undef HANN_FUSED_W
-#undef HANN_FUSED
+#undef HANN_BLOCK_W
+#undef HANN_BLOCK
 }
 
 }  // namespace hann

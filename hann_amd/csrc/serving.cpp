@@ -315,7 +315,7 @@ void hann_server_default_options(hann_server_options *o) {
     o->slots = 2;
     o->hbm_budget = 1ll << 30;
     hann_search_default_params(&o->params);
-    o->params.scan_mode = HANN_SCAN_ADC_F16;
+    o->params.scan_mode = HANN_SCAN_ADC_F32;
     o->params.fused = 1;
 }
 

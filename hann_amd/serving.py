@@ -13,7 +13,7 @@ from . import _lib
 
 
 class Server:
-    def __init__(self, index, max_batch=4096, slots=2, hbm_budget=1 << 30, scan_mode=_lib.SCAN_ADC_F16, rerank=0,
+    def __init__(self, index, max_batch=4096, slots=2, hbm_budget=1 << 30, scan_mode=_lib.SCAN_ADC_F32, rerank=0,
                  fused=1, coarse_precision=_lib.PREC_FP32):
         self._lib = _lib.load()
         o = _lib.ServerOptions()

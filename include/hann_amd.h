@@ -181,10 +181,13 @@ typedef enum {
 
 typedef struct {
     int32_t scan_mode;        /* hann_scan_mode */
-    int32_t coarse_precision; /* hann_precision for the coarse step (BF16 reranks) */
+    int32_t coarse_precision; /* hann_precision of the coarse step: HANN_PREC_BF16 is the
+                                 certified MFMA pass, whose probes and ranks equal FP32's */
     int32_t rerank;           /* 0: PQ distances (reference); R>0: rerank R*k candidates
                                  with exact FP32 distances on the stored vectors */
-    int32_t fused;            /* 1: persistent fused scan over (query, list) work items */
+    int32_t fused;            /* 1 (ADC modes): the persistent scan with one FP32 table per
+                                 query and a per-entry list term; scan_mode ADC_F32 and
+                                 ADC_F16 both take it */
 } hann_search_params;
 
 void hann_search_default_params(hann_search_params *p);
@@ -253,7 +256,7 @@ typedef struct {
     int32_t max_batch;            /* queries per launched batch (default 4096) */
     int32_t slots;                /* batches in flight (default 2, at most 8) */
     int64_t hbm_budget;           /* bytes of batch buffers in flight (default 1 GiB) */
-    hann_search_params params;    /* scan of every batch (default fused ADC FP16) */
+    hann_search_params params;    /* scan of every batch (default: the persistent ADC scan) */
 } hann_server_options;
 
 typedef struct {

@@ -170,14 +170,16 @@ def adc_bound(ref, q, ids_found, d_ref, f16):
     return b
 
 
-@pytest.mark.parametrize("mode", [1, 2])
-def test_search_adc_modes_within_bound(hann, oracle, mode):
+@pytest.mark.parametrize("mode,fused", [(1, 0), (2, 0), (1, 1), (2, 1)])
+def test_search_adc_modes_within_bound(hann, oracle, mode, fused):
+    """ADC modes: per-(query, list) tables (fused 0) and the persistent scan
+    with one FP32 table per query plus per-entry list terms (fused 1)."""
     from hann_amd import synthetic
     x, qs = synthetic.config1()
     ids = np.arange(x.shape[0], dtype=np.int64)
     idx, ref = build_pair(oracle, x, ids, 64, 8, 256, 10, seed=11)
     k = 10
-    got_ids, got_d, cnt = idx.search_batch(qs, k, nprobe=8, scan_mode=mode)
+    got_ids, got_d, cnt = idx.search_batch(qs, k, nprobe=8, scan_mode=mode, fused=fused)
     want_ids, want_d, _ = oracle.search_many(ref, qs, k, 8, nthreads=8)
     # reference distance of any entry: the float32 reconstruction, in FP64
     pos = {int(i): j for j, i in enumerate(ref.ids)}
@@ -531,9 +533,9 @@ def test_train_bf16_identical(hann, oracle):
 
 @pytest.mark.parametrize("mode,fused,rerank", [(1, 1, 0), (2, 1, 0), (1, 1, 3), (2, 1, 2)])
 def test_fused_scan_matches_unfused(hann, oracle, mode, fused, rerank):
-    """The persistent scan with precomputed list tables computes the same
-    distances as the per-query ADC scan up to FP32 (or FP16) rounding; with
-    a rerank both return the exact reference-order distances."""
+    """The persistent scan (query table + per-entry list terms) computes the
+    same distances as the per-(query, list) table scan up to FP32 (or FP16)
+    rounding; with a rerank both return the exact reference-order distances."""
     from hann_amd import synthetic
     pq = P()
     x, qs = synthetic.config1(n=20000, q=300)
@@ -605,3 +607,25 @@ def test_incremental_mirror_equals_rebuild(hann):
     other.load_bytes(idx.save_bytes())
     want = other.search_batch(qs, 10, nprobe=6, scan_mode=0)
     assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+@pytest.mark.parametrize("mode,fused,rerank", [(0, 0, 0), (1, 1, 4), (2, 1, 2)])
+def test_bf16_coarse_step_identical(hann, mode, fused, rerank):
+    """The certified BF16 coarse pass returns the same probes and ranks as the
+    exact one, so whole searches are identical; mixed k and nprobe."""
+    from hann_amd import synthetic, _lib
+    pq = P()
+    x, qs = synthetic.config1(n=20000, q=300)
+    idx = pq.New(64, pq.WithCoarseK(256), pq.WithNumSubquantizers(8), pq.WithPQK(64), pq.WithKMeansIters(4),
+                 pq.WithSeed(8))
+    idx.bulk_add_arrays(np.arange(20000), x)
+    idx.Train()
+    rng = np.random.default_rng(5)
+    ks = rng.choice([1, 10, 50], 300).astype(np.int32)
+    nps = rng.choice([1, 8, 32, 128], 300).astype(np.int32)
+    a = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=fused,
+                         coarse_precision=_lib.PREC_FP32)
+    b = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=fused,
+                         coarse_precision=_lib.PREC_BF16)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
