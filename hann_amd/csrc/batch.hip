@@ -28,38 +28,51 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     float *qn = w.get<float>(5, (size_t)nq);
     float *qf = w.get<float>(6, (size_t)nq);
     launch_to_bf16(queries, nq, dim, dp, qb, qn, qf, stat + 2, reinterpret_cast<int *>(stat + 1), st);
-    // an upper bound of each query's k-th rank: the hint row's exact rank, or
-    // the exact k-th rank over a sample of rows (every stride-th)
-    double *srank = w.get<double>(10, (size_t)nq * k);
+    // filter thresholds: from the hint row's exact rank (k == 1), or from
+    // the k-th smallest approximate distance over a row sample (every
+    // stride-th row), which bounds the query's k-th exact rank
+    float *thr = w.get<float>(11, (size_t)nq);
     int64_t ns = nb;
-    std::vector<int64_t> pick;
     if (hint && k == 1) {
+        double *srank = w.get<double>(10, (size_t)nq);
         launch_hint_rank(queries, nq, dim, base, hint, srank, st);
+        launch_thresholds(srank, 1, (int)nq, qf, stat, reinterpret_cast<int *>(stat + 1), dim, thr, st);
         ns = std::max<int64_t>(1, nb / 64);   // sizes the candidate room only
     } else {
         const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
         const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
         ns = ceil_div(nb, stride);
-        pick.resize((size_t)ns);
+        std::vector<int64_t> pick((size_t)ns);
         for (int64_t i = 0; i < ns; ++i) pick[i] = i * stride;
         int64_t *dpick = w.get<int64_t>(7, (size_t)ns);
         HANN_HIP(hipMemcpyAsync(dpick, pick.data(), (size_t)ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
-        float *sample = w.get<float>(8, (size_t)ns * dim);
-        launch_gather_rows(base, dim, dpick, ns, sample, st);
-        int64_t *sid = w.get<int64_t>(9, (size_t)nq * k);
-        launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, queries, nq, sample, ns, dim, k, sid, srank, c.ws, st);
+        __bf16 *sb = w.get<__bf16>(8, (size_t)ns * dp);
+        float *sn = w.get<float>(9, (size_t)ns);
+        launch_gather_bf16(xb, dp, xn, dpick, ns, sb, sn, st);
+        const int64_t qchunk = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(256ll << 20) / (4 * ns)));
+        float *dense = w.get<float>(10, (size_t)qchunk * ns);
+        for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
+            const int64_t qn_ = std::min(qchunk, nq - q0);
+            launch_mfma_dense(qb + q0 * dp, qn_, sb, ns, dp, qn + q0, sn, dense, st);
+            launch_sample_thresholds(dense, qn_, ns, k, qf + q0, stat, reinterpret_cast<int *>(stat + 1), dim,
+                                     thr + q0, st);
+        }
+        // the host copy of pick must outlive its transfer
+        HANN_HIP(hipStreamSynchronize(st));
     }
-    float *thr = w.get<float>(11, (size_t)nq);
-    launch_thresholds(srank, k, (int)nq, qf, stat, reinterpret_cast<int *>(stat + 1), dim, thr, st);
     // candidates: expected about k * nb / ns per query; room for 8x that
-    const int cap = (int)std::min<int64_t>(nb, std::max<int64_t>(256, 8 * ceil_div((int64_t)k * nb, ns)));
+    const int64_t expect = std::max<int64_t>(k, ceil_div((int64_t)k * nb, ns));
+    const int cap = (int)std::min<int64_t>(nb, std::max<int64_t>(256, 8 * expect));
+    // rows per workgroup: about 8 expected candidates per query and
+    // workgroup, against 32 slots
+    const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb / (16 * expect)));
     const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
     int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
     int32_t *cand = w.get<int32_t>(13, (size_t)chunk * cap);
     HANN_HIP(hipMemsetAsync(cnt, 0, (size_t)nq * sizeof(int32_t), st));
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
         const int64_t qn_ = std::min(chunk, nq - q0);
-        launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, cnt + q0, cand, cap, st);
+        launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cap, st);
         launch_rank_candidates(queries + q0 * dim, qn_, dim, base, cnt + q0, cand, cap, k, sqrt_out, out_ids + q0 * k,
                                out_dist + q0 * k, st);
     }

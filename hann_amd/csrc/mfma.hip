@@ -3,10 +3,12 @@
 // The squared distance ||q||^2 + ||x||^2 - 2 q.x is evaluated on BF16 copies
 // of the inputs with v_mfma_f32_32x32x16_bf16 (FP32 accumulation). Every row
 // whose approximate distance is within the threshold T_q of a query becomes a
-// candidate; T_q is the query's exact k-th distance over a row sample (an
-// upper bound of its true k-th distance) widened by a rigorous bound on
-// |approx - exact FP32| (mfma.h). The candidates are then ranked in the
-// reference's exact FP32 order, so the result equals the exact path's.
+// candidate. T_q comes from an upper bound of the query's true k-th exact
+// distance -- the k-th smallest approximate distance over a row sample,
+// widened by the error bound below, or the exact rank of a hint row -- and
+// is widened again by a rigorous bound on |approx - exact FP32|. The
+// candidates are then ranked in the reference's exact FP32 order, so the
+// result equals the exact path's.
 #include "common.h"
 #include "mfma.h"
 
@@ -83,81 +85,218 @@ __global__ void k_hint_rank(const float *__restrict__ queries, int64_t nq, int d
 constexpr int MF_TILE = 128;   // queries and rows per workgroup tile
 constexpr int MF_KC = 128;     // K chunk held in LDS (bf16 columns)
 constexpr int MF_PAD = 8;      // LDS row padding (bf16) against bank conflicts
-constexpr int MF_LD = MF_KC + MF_PAD;
+constexpr int MF_SLOTS = 32;   // per-query candidate slots of a workgroup
 
-// One workgroup: 128 queries x 128 rows, 4 waves of 64 x 64 (2 x 2 tiles of
-// 32 x 32). Lane layout of v_mfma_f32_32x32x16_bf16: A lane l holds row
-// l%32, k = 8*(l/32) .. +7; B lane l holds column l%32, the same k; D
-// register r of lane l is row 8*(r/4) + 4*(l/32) + r%4, column l%32.
-__global__ void __launch_bounds__(256) k_mfma_filter(const __bf16 *__restrict__ Q, int64_t nq,
-                                                     const __bf16 *__restrict__ X, int64_t nb, int dp,
-                                                     const float *__restrict__ qn, const float *__restrict__ xn,
-                                                     const float *__restrict__ thr, int32_t *__restrict__ cnt,
-                                                     int32_t *__restrict__ cand_row, int cap) {
-    __shared__ __attribute__((aligned(16))) __bf16 sA[MF_TILE * MF_LD];
-    __shared__ __attribute__((aligned(16))) __bf16 sB[MF_TILE * MF_LD];
+// One workgroup: 128 queries x (rt_per_wg x 128) rows, 4 waves of 64 x 64
+// (2 x 2 tiles of 32 x 32) per row tile. Lane layout of
+// v_mfma_f32_32x32x16_bf16: A lane l holds row l%32, k = 8*(l/32) .. +7; B
+// lane l holds column l%32, the same k; D register r of lane l is row
+// 8*(r/4) + 4*(l/32) + r%4, column l%32.
+// DENSE: write every approximate distance to dense[q * ld_dense + row].
+// Otherwise rows within thr[q] are gathered per query in LDS and appended
+// to the query's global list once per workgroup; a query whose LDS slots
+// overflow gets cnt > cap, which sends it to the exact path.
+template <bool DENSE>
+__global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q, int64_t nq,
+                                                   const __bf16 *__restrict__ X, int64_t nb, int dp,
+                                                   const float *__restrict__ qn, const float *__restrict__ xn,
+                                                   const float *__restrict__ thr, int rt_per_wg,
+                                                   int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row, int cap,
+                                                   float *__restrict__ dense, int64_t ld_dense) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    const int kc_max = dp < MF_KC ? dp : MF_KC;
+    const int ld = kc_max + MF_PAD;
+    __bf16 *sA = reinterpret_cast<__bf16 *>(s_raw);
+    __bf16 *sB = sA + MF_TILE * ld;
+    int32_t *s_cc = reinterpret_cast<int32_t *>(sB + MF_TILE * ld);
+    int32_t *s_cr = s_cc + MF_TILE;
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int wm = wave / 2, wn = wave % 2;
-    const int64_t r0 = (int64_t)blockIdx.x * MF_TILE;
     const int64_t q0 = (int64_t)blockIdx.y * MF_TILE;
+    const int64_t rbase = (int64_t)blockIdx.x * MF_TILE * rt_per_wg;
+    if (!DENSE)
+        for (int t = tid; t < MF_TILE; t += 256) s_cc[t] = 0;
 
-    f32x16 acc[2][2];
+    for (int rt = 0; rt < rt_per_wg; ++rt) {
+        const int64_t r0 = rbase + (int64_t)rt * MF_TILE;
+        if (r0 >= nb) break;
+        f32x16 acc[2][2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
-#pragma unroll
-        for (int j = 0; j < 2; ++j)
-#pragma unroll
-            for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
-
-    for (int k0 = 0; k0 < dp; k0 += MF_KC) {
-        const int kc = dp - k0 < MF_KC ? dp - k0 : MF_KC;   // multiple of 16
-        const int vecs = kc / 8;
-        __syncthreads();
-        for (int v = tid; v < MF_TILE * vecs; v += 256) {
-            const int row = v / vecs, c8 = v % vecs;
-            const int64_t gq = q0 + row, gr = r0 + row;
-            bf16x8 a = {}, b = {};
-            if (gq < nq) a = *reinterpret_cast<const bf16x8 *>(Q + gq * dp + k0 + c8 * 8);
-            if (gr < nb) b = *reinterpret_cast<const bf16x8 *>(X + gr * dp + k0 + c8 * 8);
-            *reinterpret_cast<bf16x8 *>(&sA[row * MF_LD + c8 * 8]) = a;
-            *reinterpret_cast<bf16x8 *>(&sB[row * MF_LD + c8 * 8]) = b;
-        }
-        __syncthreads();
-        const int kl = (lane / 32) * 8;
-        for (int kk = 0; kk < kc; kk += 16) {
-            bf16x8 a[2], b[2];
-#pragma unroll
-            for (int i = 0; i < 2; ++i)
-                a[i] = *reinterpret_cast<const bf16x8 *>(&sA[(wm * 64 + i * 32 + lane % 32) * MF_LD + kk + kl]);
+        for (int i = 0; i < 2; ++i)
 #pragma unroll
             for (int j = 0; j < 2; ++j)
-                b[j] = *reinterpret_cast<const bf16x8 *>(&sB[(wn * 64 + j * 32 + lane % 32) * MF_LD + kk + kl]);
 #pragma unroll
-            for (int i = 0; i < 2; ++i)
+                for (int r = 0; r < 16; ++r) acc[i][j][r] = 0.f;
+
+        for (int k0 = 0; k0 < dp; k0 += MF_KC) {
+            const int kc = dp - k0 < MF_KC ? dp - k0 : MF_KC;   // multiple of 16
+            const int vecs = kc / 8;
+            __syncthreads();
+            for (int v = tid; v < MF_TILE * vecs; v += 256) {
+                const int row = v / vecs, c8 = v % vecs;
+                const int64_t gq = q0 + row, gr = r0 + row;
+                bf16x8 a = {}, b = {};
+                if (gq < nq) a = *reinterpret_cast<const bf16x8 *>(Q + gq * dp + k0 + c8 * 8);
+                if (gr < nb) b = *reinterpret_cast<const bf16x8 *>(X + gr * dp + k0 + c8 * 8);
+                *reinterpret_cast<bf16x8 *>(&sA[row * ld + c8 * 8]) = a;
+                *reinterpret_cast<bf16x8 *>(&sB[row * ld + c8 * 8]) = b;
+            }
+            __syncthreads();
+            const int kl = (lane / 32) * 8;
+            for (int kk = 0; kk < kc; kk += 16) {
+                bf16x8 a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+                    a[i] = *reinterpret_cast<const bf16x8 *>(&sA[(wm * 64 + i * 32 + lane % 32) * ld + kk + kl]);
 #pragma unroll
                 for (int j = 0; j < 2; ++j)
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-    }
-    // epilogue: approximate distance, threshold test, append
+                    b[j] = *reinterpret_cast<const bf16x8 *>(&sB[(wn * 64 + j * 32 + lane % 32) * ld + kk + kl]);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < 2; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            const int64_t gr = r0 + wn * 64 + j * 32 + lane % 32;
-            const float xr = gr < nb ? xn[gr] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int64_t gq = q0 + wm * 64 + i * 32 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                if (gq >= nq || gr >= nb) continue;
-                const float approx = (qn[gq] + xr) - 2.f * acc[i][j][r];
-                if (approx <= thr[gq]) {
-                    const int slot = atomicAdd(&cnt[gq], 1);
-                    if (slot < cap) cand_row[gq * cap + slot] = (int32_t)gr;
-                }
+                    for (int j = 0; j < 2; ++j)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
             }
         }
+        // epilogue: approximate distance; threshold test or dense store
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int64_t gr = r0 + wn * 64 + j * 32 + lane % 32;
+                const float xr = gr < nb ? xn[gr] : 0.f;
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int ql = wm * 64 + i * 32 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
+                    const int64_t gq = q0 + ql;
+                    if (gq >= nq || gr >= nb) continue;
+                    const float approx = (qn[gq] + xr) - 2.f * acc[i][j][r];
+                    if (DENSE) {
+                        dense[gq * ld_dense + gr] = approx;
+                    } else if (approx <= thr[gq]) {
+                        const int slot = atomicAdd(&s_cc[ql], 1);
+                        if (slot < MF_SLOTS) s_cr[ql * MF_SLOTS + slot] = (int32_t)gr;
+                    }
+                }
+            }
+    }
+    if (DENSE) return;
+    __syncthreads();
+    for (int ql = tid; ql < MF_TILE; ql += 256) {
+        const int64_t gq = q0 + ql;
+        const int n = s_cc[ql];
+        if (gq >= nq || n == 0) continue;
+        if (n > MF_SLOTS) {
+            atomicMax(&cnt[gq], cap + 1);
+            continue;
+        }
+        const int base = atomicAdd(&cnt[gq], n);
+        for (int i = 0; i < n && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[ql * MF_SLOTS + i];
+    }
+}
+
+// out rows b = X rows pick[b] (BF16, dp columns) and their BF16 norms
+__global__ void k_gather_bf16(const __bf16 *__restrict__ X, int dp, const float *__restrict__ xn,
+                              const int64_t *__restrict__ pick, int64_t n, __bf16 *__restrict__ out,
+                              float *__restrict__ out_n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * dp) return;
+    const int64_t b = t / dp;
+    const int c = (int)(t % dp);
+    const int64_t src = pick[b];
+    out[t] = X[src * dp + c];
+    if (c == 0) out_n[b] = xn[src];
+}
+
+// order-preserving map of floats (after NaN -> +inf) onto unsigned integers
+__device__ __forceinline__ unsigned float_key(float v) {
+    const unsigned u = __float_as_uint(sane_key(v));
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key_float(unsigned k) {
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
+}
+
+// Per query: A = k-th smallest approximate distance over its ns sample rows
+// (radix select, 8 bits a pass). Each of those k rows has
+//   D <= (sqrt(max(A, 0) + E2) + delta)^2  and exact rank e <= D / (1-(d+2)u)
+// (the bounds of k_thresholds, read the other way), which bounds the query's
+// true k-th exact rank tau; the filter threshold is then formed from tau as
+// in k_thresholds. One workgroup of 256 threads per query.
+__global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t ns, int k,
+                                                           const float *__restrict__ qnorm_f,
+                                                           const unsigned *__restrict__ max_norm,
+                                                           const int *__restrict__ inexact, int d,
+                                                           float *__restrict__ thr) {
+    __shared__ unsigned hist[256];
+    __shared__ unsigned s_prefix;
+    __shared__ int s_k;
+    const int tid = threadIdx.x, lane = tid % 64;
+    const int64_t q = blockIdx.x;
+    if (k > ns) {
+        if (tid == 0) thr[q] = __builtin_inff();
+        return;
+    }
+    const float *a = approx + q * ns;
+    unsigned prefix = 0, mask = 0;
+    int kk = k;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
+        hist[tid] = 0;
+        __syncthreads();
+        for (int64_t i = tid; i < ns; i += 256) {
+            const unsigned u = float_key(a[i]);
+            if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (tid < 64) {
+            // lane l owns bins 4l .. 4l+3; find the bin holding the kk-th
+            const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                           h3 = hist[4 * lane + 3];
+            const unsigned own = h0 + h1 + h2 + h3;
+            unsigned incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            const unsigned excl = incl - own;
+            const bool here = excl < (unsigned)kk && (unsigned)kk <= incl;
+            if (here) {
+                unsigned run = excl, bin = 4 * lane;
+                const unsigned hs[4] = {h0, h1, h2, h3};
+                for (int j = 0; j < 4; ++j) {
+                    if (run + hs[j] >= (unsigned)kk) {
+                        bin = 4 * lane + j;
+                        break;
+                    }
+                    run += hs[j];
+                }
+                s_prefix = prefix | (bin << shift);
+                s_k = kk - (int)run;
+            }
+        }
+        __syncthreads();
+        prefix = s_prefix;
+        kk = s_k;
+        mask |= 255u << shift;
+        __syncthreads();
+    }
+    if (tid != 0) return;
+    const float A = key_float(prefix);
+    if (!(A < __builtin_inff())) {
+        thr[q] = __builtin_inff();
+        return;
+    }
+    const double u = 1.0 / 16777216.0;
+    const double qa = qnorm_f[q], qb = (double)__uint_as_float(*max_norm);
+    const double delta = *inexact ? (qa + qb) / 512.0 : 0.0;
+    const double e2 = 2.0 * (d + 2) * u * (qa + qb) * (qa + qb) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
+    const double s0 = __builtin_sqrt((A > 0.f ? (double)A : 0.0) + e2) + delta;
+    const double tau = s0 * s0 / (1.0 - (d + 2) * u) * (1.0 + 1.0 / 1048576.0);
+    const double tp = tau / (1.0 - (d + 2) * u);
+    const double s1 = __builtin_sqrt(tp) + delta;
+    thr[q] = (float)((s1 * s1 + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
 }
 
 // Exact reference-order rank of every candidate, then the k best by
@@ -225,18 +364,61 @@ void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm
     HANN_LAUNCH_CHECK();
 }
 
+static size_t mfma_lds(int dp, bool dense) {
+    const int ld = (dp < dev::MF_KC ? dp : dev::MF_KC) + dev::MF_PAD;
+    return (size_t)2 * dev::MF_TILE * ld * 2 + (dense ? 0 : (size_t)dev::MF_TILE * (1 + dev::MF_SLOTS) * 4);
+}
+
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                        const float *xn, const float *thr, int32_t *cnt, int32_t *cand_row, int cap, hipStream_t st) {
+                        const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row, int cap,
+                        hipStream_t st) {
     if (nq == 0 || nb == 0) return;
-    const int64_t rt = ceil_div(nb, dev::MF_TILE), qt = ceil_div(nq, dev::MF_TILE);
+    if (rt_per_wg < 1) rt_per_wg = 1;
+    const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt_per_wg), qt = ceil_div(nq, dev::MF_TILE);
+    const size_t shm = mfma_lds(dp, false);
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
     for (int64_t y0 = 0; y0 < qt; y0 += 65535) {
         const int64_t yn = qt - y0 < 65535 ? qt - y0 : 65535;
-        hipLaunchKernelGGL(dev::k_mfma_filter, dim3((unsigned)rt, (unsigned)yn), dim3(256), 0, st,
+        hipLaunchKernelGGL(dev::k_mfma_pass<false>, dim3((unsigned)rg, (unsigned)yn), dim3(256), shm, st,
                            Q + y0 * dev::MF_TILE * dp, nq - y0 * dev::MF_TILE, X, nb, dp, qn + y0 * dev::MF_TILE, xn,
-                           thr + y0 * dev::MF_TILE, cnt + y0 * dev::MF_TILE,
-                           cand_row + y0 * dev::MF_TILE * (int64_t)cap, cap);
+                           thr + y0 * dev::MF_TILE, rt_per_wg, cnt + y0 * dev::MF_TILE,
+                           cand_row + y0 * dev::MF_TILE * (int64_t)cap, cap, nullptr, (int64_t)0);
         HANN_LAUNCH_CHECK();
     }
+}
+
+void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
+                       const float *xn, float *dense, hipStream_t st) {
+    if (nq == 0 || nb == 0) return;
+    const int rt = 4;
+    const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt), qt = ceil_div(nq, dev::MF_TILE);
+    const size_t shm = mfma_lds(dp, true);
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
+    for (int64_t y0 = 0; y0 < qt; y0 += 65535) {
+        const int64_t yn = qt - y0 < 65535 ? qt - y0 : 65535;
+        hipLaunchKernelGGL(dev::k_mfma_pass<true>, dim3((unsigned)rg, (unsigned)yn), dim3(256), shm, st,
+                           Q + y0 * dev::MF_TILE * dp, nq - y0 * dev::MF_TILE, X, nb, dp, qn + y0 * dev::MF_TILE, xn,
+                           nullptr, rt, nullptr, nullptr, 0, dense + y0 * dev::MF_TILE * nb, nb);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t *pick, int64_t n, __bf16 *out,
+                        float *out_n, hipStream_t st) {
+    if (n == 0) return;
+    hipLaunchKernelGGL(dev::k_gather_bf16, dim3((unsigned)ceil_div(n * dp, 256)), dim3(256), 0, st, X, dp, xn, pick,
+                       n, out, out_n);
+    HANN_LAUNCH_CHECK();
+}
+
+void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const float *qnorm_f,
+                              const unsigned *max_norm, const int *inexact, int d, float *thr, hipStream_t st) {
+    if (nq == 0) return;
+    hipLaunchKernelGGL(dev::k_sample_thresholds, dim3((unsigned)nq), dim3(256), 0, st, approx, ns, k, qnorm_f,
+                       max_norm, inexact, d, thr);
+    HANN_LAUNCH_CHECK();
 }
 
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,

@@ -122,13 +122,20 @@ def test_batch_topk_other_metrics(hann, oracle, mid):
 
 @pytest.mark.parametrize("nq,nb,dim,k,kind", [(300, 20000, 128, 100, "int"), (130, 50000, 96, 10, "real"),
                                               (64, 4096, 100, 50, "real"), (5, 300, 32, 10, "real"),
-                                              (200, 30000, 784, 20, "real"), (1000, 8000, 64, 1, "int")])
+                                              (200, 30000, 784, 20, "real"), (1000, 8000, 64, 1, "int"),
+                                              (50, 5000, 64, 10, "dup"), (700, 16384, 96, 128, "real")])
 def test_batch_topk_bf16_certified_equals_exact(hann, oracle, nq, nb, dim, k, kind):
     """BF16 MFMA candidate pass + exact FP32 ranking: ids and distances are
     identical to the exact path (the filter threshold is a proven bound)."""
     from hann_amd import _lib, core
     rng = np.random.default_rng(nq + nb + dim)
-    if kind == "int":
+    if kind == "dup":
+        # three distinct rows: every query ties with thousands of rows, the
+        # candidate lists overflow and those queries take the exact path
+        rows = rng.standard_normal((3, dim)).astype(np.float32)
+        base = rows[rng.integers(0, 3, nb)]
+        q = rows[rng.integers(0, 3, nq)] + np.float32(0.01)
+    elif kind == "int":
         lam = rng.uniform(10, 30, dim)
         base = np.minimum(rng.poisson(lam, (nb, dim)), 255).astype(np.float32)
         q = np.minimum(rng.poisson(lam, (nq, dim)), 255).astype(np.float32)
