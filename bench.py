@@ -10,9 +10,12 @@ from the same mixture, each with its own k in {10, 50, 100} and nprobe in
 Two search paths are measured on the same index and the same batches:
   plain   the FP32 drop-in: the reference's decode-and-measure PQ distance
           (bit-identical to pqivf/index.go:905-936) + an exact FP32 rerank
-  chosen  the certified BF16 MFMA coarse step (identical probes), then the
-          persistent split-table scan: one FP32 table per query plus a
-          per-entry list term, autotuned waves per query, + the same rerank
+  chosen  the fastest at recall@10 >= 0.95 of three candidates, each with the
+          certified BF16 MFMA coarse step (probes identical to FP32) and the
+          same rerank: the persistent FP32 table scan (one table per query,
+          per-entry list terms), or the list-major scan that decodes each
+          probed list once and scores it against all its queries with BF16
+          or split-BF16 MFMA
 The rerank depth of each path is the smallest of --rerank-grid whose
 recall@10 against FP64 ground truth reaches 0.95. `value` is the chosen
 path's queries per second.
@@ -272,6 +275,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered queries/s (0: half the chosen QPS)")
+    ap.add_argument("--paths", default="", help="comma-separated subset of the chosen-path candidates")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {a.gpus}).",
@@ -323,7 +327,15 @@ def main():
         return recall_at_10(b.ids.cpu().numpy()[:ngt], gt)
 
     grid = [int(v) for v in a.rerank_grid.split(",")]
-    paths = {"plain": (_lib.SCAN_EXACT, 0, _lib.PREC_FP32), "chosen": (_lib.SCAN_ADC_F32, 1, _lib.PREC_BF16)}
+    # the plain FP32 drop-in, and the candidates for the chosen path (all
+    # with the certified BF16 coarse step and an exact FP32 rerank)
+    paths = {"plain": (_lib.SCAN_EXACT, 0, _lib.PREC_FP32),
+             "table-f32": (_lib.SCAN_ADC_F32, 1, _lib.PREC_BF16),
+             "list-bf16": (_lib.SCAN_LIST_BF16, 1, _lib.PREC_BF16),
+             "list-bf16x3": (_lib.SCAN_LIST_BF16X3, 1, _lib.PREC_BF16)}
+    if a.paths:
+        keep = set(a.paths.split(",")) | {"plain"}
+        paths = {k: v for k, v in paths.items() if k in keep}
     chosen_r, recalls = {}, {}
     for name, (mode, fused, coarse) in paths.items():
         for r in grid:
@@ -350,17 +362,33 @@ def main():
         el = time.perf_counter() - t
         idx.set_profile(False)
         st = idx.stage_times()
+        kern = idx.scan_kernel_time()
         lat = np.array([s.elapsed_time(e) for s, e in evs])
-        return el, lat, st
+        return el, lat, st, kern
 
     res = {}
     for name, (mode, fused, coarse) in paths.items():
-        el, lat, st = timed(mode, chosen_r[name], fused, coarse, True)
+        el, lat, st, kern = timed(mode, chosen_r[name], fused, coarse, True)
         qps = a.steps * a.batch / el
         res[name] = {"qps": qps, "ms_per_step": 1000 * el / a.steps, "p50_ms": float(np.percentile(lat, 50)),
                      "p99_ms": float(np.percentile(lat, 99)), "stages": st, "rerank": chosen_r[name],
-                     "recall": recalls[name]}
-        log(f"{name}: {qps:.0f} q/s, {1000 * el / a.steps:.2f} ms/step, stages {st}")
+                     "recall": recalls[name], "kernel_ms": kern / max(st[3], 1)}
+        log(f"{name}: {qps:.0f} q/s, {1000 * el / a.steps:.2f} ms/step, stages {st}, scan kernel {kern:.3f} ms")
+    # chosen: the fastest candidate path at recall@10 >= 0.95 (else the most accurate)
+    cands = [n for n in res if n != "plain"]
+    ok = [n for n in cands if res[n]["recall"] >= 0.95]
+    chosen_name = max(ok, key=lambda n: res[n]["qps"]) if ok else max(cands, key=lambda n: res[n]["recall"])
+    res["chosen"] = res[chosen_name]
+    paths["chosen"] = paths[chosen_name]
+    chosen_r["chosen"] = chosen_r[chosen_name]
+    path_desc = {"table-f32": "certified bf16 coarse + persistent fp32 table scan (one table per query, per-entry "
+                              "list terms, autotuned waves/query) + fp32 rerank",
+                 "list-bf16": "certified bf16 coarse + list-major decode + bf16 mfma scan (table-scan threshold "
+                              "pre-pass) + fp32 rerank",
+                 "list-bf16x3": "certified bf16 coarse + list-major decode + split-bf16 mfma scan (table-scan "
+                                "threshold pre-pass) + fp32 rerank"}
+    kernel_name = {"table-f32": "k_scan_block", "list-bf16": "k_scan_list<bf16>",
+                   "list-bf16x3": "k_scan_list<bf16x3>"}
 
     # roofline of the dominant kernel: the chosen path's list scan
     dist_b, pair_b = 0, 0
@@ -368,13 +396,13 @@ def main():
         db, pb = probe_bytes(torch, lib, idx, dev_batches[a.warmup + i], list_sizes, a.m)
         dist_b += db
         pair_b += pb
-    st = res["chosen"]["stages"]
-    scan_ms = st[1] / max(st[3], 1)
+    kern_ms = res["chosen"]["kernel_ms"]
     bytes_per_launch = dist_b / a.steps
-    achieved = bytes_per_launch / (scan_ms * 1e-3) / 1e9
+    achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "kernel": "k_scan_split<ADC_F32>", "scan_ms_per_launch": round(scan_ms, 4),
+                "kernel": kernel_name[chosen_name], "kernel_ms_per_launch": round(kern_ms, 4),
+                "scan_stage_ms_per_launch": round(res["chosen"]["stages"][1] / max(res["chosen"]["stages"][3], 1), 4),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
                 "pair_bytes_per_launch": int(pair_b / a.steps)}
 
@@ -412,7 +440,8 @@ def main():
         add_ids = np.arange(a.n, a.n + max(n_add, 1), dtype=np.int64)
         n_del = int(sizes[kind == 2].sum())
         del_ids = rng.permutation(a.n)[:max(n_del, 1)].astype(np.int64)
-        for name, (mode, fused, coarse) in paths.items():
+        for name in ("plain", "chosen"):
+            mode, fused, coarse = paths[name]
             srv = serving.Server(idx, max_batch=a.batch, slots=2, hbm_budget=2 << 30, scan_mode=mode,
                                  rerank=chosen_r[name], fused=fused, coarse_precision=coarse)
             try:
@@ -454,8 +483,13 @@ def main():
         "recall_at_10": round(ch["recall"], 4),
         "chosen": {"qps": round(ch["qps"], 1), "recall_at_10": round(ch["recall"], 4), "rerank": ch["rerank"],
                    "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
-                   "path": "certified bf16 coarse + persistent split-table fp32 adc scan (autotuned waves/query) + fp32 rerank",
+                   "path": path_desc[chosen_name], "name": chosen_name,
                    "stage_ms_per_step": [round(v / max(ch["stages"][3], 1), 4) for v in ch["stages"][:3]]},
+        "candidates": {n: {"qps": round(res[n]["qps"], 1), "recall_at_10": round(res[n]["recall"], 4),
+                           "rerank": res[n]["rerank"], "p99_ms": round(res[n]["p99_ms"], 3),
+                           "scan_kernel_ms": round(res[n]["kernel_ms"], 4),
+                           "stage_ms_per_step": [round(v / max(res[n]["stages"][3], 1), 4)
+                                                 for v in res[n]["stages"][:3]]} for n in cands},
         "plain": {"qps": round(pl["qps"], 1), "recall_at_10": round(pl["recall"], 4), "rerank": pl["rerank"],
                   "p50_ms": round(pl["p50_ms"], 3), "p99_ms": round(pl["p99_ms"], 3),
                   "path": "reference-order fp32 decode + fp32 rerank",

@@ -22,7 +22,7 @@ HANN_ERR_UNSUPPORTED = 9
 
 METRIC_SQEUCLIDEAN, METRIC_EUCLIDEAN, METRIC_MANHATTAN, METRIC_COSINE = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
-SCAN_EXACT, SCAN_ADC_F32, SCAN_ADC_F16 = 0, 1, 2
+SCAN_EXACT, SCAN_ADC_F32, SCAN_ADC_F16, SCAN_LIST_BF16, SCAN_LIST_BF16X3 = 0, 1, 2, 3, 4
 
 
 class HannError(Exception):
@@ -118,6 +118,7 @@ SIGNATURES = {
     "hann_free_buffer": (None, [vp]),
     "hann_pqivf_profile": (cint, [vp, cint]),
     "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),
+    "hann_pqivf_scan_kernel_time": (cint, [vp, c_double_p]),
     "hann_pqivf_export_lists": (cint, [vp, c_int64_p, c_int64_p, c_int32_p, c_float_p]),
     "hann_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),
     "hann_server_create": (cint, [vp, ctypes.POINTER(ServerOptions), ctypes.POINTER(vp)]),

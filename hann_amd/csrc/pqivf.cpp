@@ -24,6 +24,7 @@
 #include "kmeans.h"
 #include "rng.h"
 #include "scan_fused.h"
+#include "scan_list.h"
 
 namespace hann {
 
@@ -843,10 +844,14 @@ void PQIVF::refresh_terms(const std::vector<int64_t> *which) {
                                 hipMemcpyHostToDevice, stream));
     } else {
         for (int64_t v : dev_len) max_len = std::max(max_len, v);
+        // bounds start over with a full rebuild (entries only grow them otherwise)
+        d_ymax_tmax.resize(2);
+        HANN_HIP(hipMemsetAsync(d_ymax_tmax.get(), 0, 2 * sizeof(unsigned), stream));
     }
     launch_entry_terms(which ? d_term_lists.get() : nullptr, which ? (int64_t)which->size() : (int64_t)lists.size(),
                        max_len, d_list_start.get(), d_list_len.get(), d_codes.get(), code_bytes, d_has_codes.get(),
-                       (int)dim, (int)m, (int)pmax, d_coarse.get(), d_cb.get(), d_term.get(), stream);
+                       (int)dim, (int)m, (int)pmax, d_coarse.get(), d_cb.get(), d_term.get(), d_ymax_tmax.get(),
+                       stream);
     // the host vector behind an asynchronous copy must outlive it
     HANN_HIP(hipStreamSynchronize(stream));
 }
@@ -883,11 +888,20 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     // (pqivf/index.go:184-206). When every query probes every list no
     // selection is needed; otherwise the per-query prefix of the sorted
     // probe list is each query's set.
-    std::array<hipEvent_t, 4> ev{};
+    std::array<hipEvent_t, 6> ev{};
     if (profile) {
         for (auto &e : ev) HANN_HIP(hipEventCreate(&e));
         HANN_HIP(hipEventRecord(ev[0], st));
+        // [4], [5] start out recorded so that a search without a scan
+        // kernel measures zero
+        HANN_HIP(hipEventRecord(ev[4], st));
+        HANN_HIP(hipEventRecord(ev[5], st));
+        kmark = &ev[4];
     }
+    struct KmarkReset {
+        hipEvent_t *&k;
+        ~KmarkReset() { k = nullptr; }
+    } kmark_reset{kmark};
     int64_t *probes;
     int probe_stride;
     double *probe_rank = nullptr;
@@ -953,17 +967,36 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     a.fell_back = nullptr;
     if (profile) HANN_HIP(hipEventRecord(ev[1], st));
     int mode = p.scan_mode;
+    if (mode < SCAN_MODE_EXACT || mode > SCAN_MODE_LIST_BF16X3)
+        fail(HANN_ERR_UNSUPPORTED, strf("scan mode %d is not supported", mode));
+    bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
+    const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
+    if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
+                      !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
+                      list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax) > 160 * 1024)) {
+        // the list-major scan needs a rerank, codes for every entry, a probe
+        // row per query and a supported shape; otherwise the persistent
+        // table scan runs
+        list_scan = false;
+        mode = SCAN_MODE_ADC_F32;
+    }
+    if (list_scan) mode = SCAN_MODE_ADC_F32;   // the pre-pass and overflow scans
     if (!have_codebooks) {
         // no codebooks: every entry ranks by its exact vector
         a.any_uncoded = 1;
         mode = SCAN_MODE_ADC_F32;
     }
-    const bool fused = p.fused && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
+    const bool fused = (p.fused || list_scan) && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
     auto scan = [&](const ScanArgs &args) {
-        if (fused)
+        if (list_scan)
+            run_list_scan(args, nq, x3, np, probe_rank, st, w);
+        else if (fused)
             run_fused_scan(args, nq, np, probe_rank, st, w);
-        else
+        else {
+            mark_kernel(0, st);
             launch_scan(mode, code_bytes, args, nq, st);
+            mark_kernel(1, st);
+        }
     };
     if (R == 0) {
         a.out_key = out_key;
@@ -993,7 +1026,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
 }
 
 void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double *probe_rank, hipStream_t st,
-                           Workspace &w) {
+                           Workspace &w, bool main_scan) {
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
@@ -1046,7 +1079,126 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
         tuned_wpq[key] = wpq;
         last_tune = strf("persistent scan tuned: waves/query %d (%.3f ms)", wpq, best);
     }
+    if (main_scan) mark_kernel(0, st);
     launch_scan_block(a, f, nq, wpq, kp, code_bytes, maxp, grid_for(wpq), st);
+    if (main_scan) mark_kernel(1, st);
+}
+
+void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank,
+                          hipStream_t st, Workspace &w) {
+    const int K = (int)lists.size();
+    const int selk = a.out_stride;
+    const int R = a.rerank;
+    const int np = a.probe_cap;
+    const int kp = a.cap / 2;
+    const int cap = std::max(1024, 8 * kp);
+    const int64_t pairs_max = nq * (int64_t)np;
+    // arena (slot 17) for the batch's small buffers
+    size_t off = 0;
+    auto carve = [&](size_t bytes) {
+        const size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
+                 o_lfill = carve((size_t)K * 4), o_tot = carve(16), o_qtot = carve((size_t)nq * 8),
+                 o_tau = carve((size_t)nq * 4), o_cnt = carve((size_t)nq * 4), o_ovf = carve((size_t)nq * 4),
+                 o_ctr = carve(16), o_pq = carve((size_t)pairs_max * 4), o_pqc = carve((size_t)pairs_max * 4),
+                 o_items = carve((size_t)pairs_max * 16), o_pkey = carve((size_t)nq * selk * 4),
+                 o_ppos = carve((size_t)nq * selk * 8), o_pcnt = carve((size_t)nq * 4);
+    char *base = w.get<char>(17, off);
+    auto at = [&](size_t o) { return base + o; };
+    int32_t *lcnt = reinterpret_cast<int32_t *>(at(o_lcnt));
+    int32_t *lstart = reinterpret_cast<int32_t *>(at(o_lstart));
+    int32_t *ioff = reinterpret_cast<int32_t *>(at(o_ioff));
+    int32_t *lfill = reinterpret_cast<int32_t *>(at(o_lfill));
+    int32_t *totals = reinterpret_cast<int32_t *>(at(o_tot));
+    int64_t *qtot = reinterpret_cast<int64_t *>(at(o_qtot));
+    float *tau = reinterpret_cast<float *>(at(o_tau));
+    int32_t *cnt = reinterpret_cast<int32_t *>(at(o_cnt));
+    int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
+    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [0] items taken, [1] overflowed, [2] fallback queue
+    int32_t *pair_q = reinterpret_cast<int32_t *>(at(o_pq));
+    float *pair_qc = reinterpret_cast<float *>(at(o_pqc));
+    int4 *items = reinterpret_cast<int4 *>(at(o_items));
+    float *pkey = reinterpret_cast<float *>(at(o_pkey));
+    int64_t *ppos = reinterpret_cast<int64_t *>(at(o_ppos));
+    int32_t *pcnt = reinterpret_cast<int32_t *>(at(o_pcnt));
+    float *cand_key = w.get<float>(18, (size_t)nq * cap);
+    int32_t *cand_pos = w.get<int32_t>(19, (size_t)nq * cap);
+
+    // 1. pre-pass: the table scan over each query's nearest lists bounds its
+    //    selk-th distance
+    ScanArgs pre = a;
+    pre.probe_cap = std::min(np, 2);
+    pre.allow_fallback = 0;
+    pre.fb_counter = nullptr;
+    pre.out_key = pkey;
+    pre.out_id = ppos;
+    pre.out_cnt = pcnt;
+    run_fused_scan(pre, nq, std::min(maxp, 2), probe_rank, st, w, false);
+    launch_list_thresholds(a, nq, pkey, pcnt, selk, probe_rank, reinterpret_cast<const float *>(d_ymax_tmax.get()),
+                           x3, tau, st);
+    // 2. the batch's (query, list) pairs grouped by list, as work items
+    launch_fill_i32(lcnt, K, 0, st);
+    launch_pair_count(a, nq, lcnt, qtot, st);
+    launch_pair_offsets(lcnt, K, lstart, ioff, lfill, totals, st);
+    launch_pair_scatter(a, nq, probe_rank, lstart, lfill, pair_q, pair_qc, st);
+    launch_make_items(lcnt, lstart, ioff, K, items, st);
+    // 3. the scan
+    launch_fill_i32(cnt, nq, 0, st);
+    launch_fill_i32(ctr, 4, 0, st);
+    if (num_cus == 0) {
+        int dev = 0;
+        HANN_HIP(hipGetDevice(&dev));
+        HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
+    }
+    ListScanArgs la{};
+    la.queries = a.queries;
+    la.d = (int)dim;
+    la.m = (int)m;
+    la.s = (int)(dim / m);
+    la.P = (int)pmax;
+    la.dk = (int)ceil_div(dim, 16) * 16;
+    la.list_start = d_list_start.get();
+    la.list_len = d_list_len.get();
+    la.codes = d_codes.get();
+    la.term = d_term.get();
+    la.cb = d_cb.get();
+    la.pair_q = pair_q;
+    la.pair_qc = pair_qc;
+    la.items = items;
+    la.n_items = totals + 1;
+    la.item_counter = ctr;
+    la.tau = tau;
+    la.cnt = cnt;
+    la.cand_key = cand_key;
+    la.cand_pos = cand_pos;
+    la.cap = cap;
+    const size_t shm = list_scan_lds_bytes(x3, la.d, la.m, la.P);
+    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / shm));
+    mark_kernel(0, st);
+    launch_scan_list(la, x3, num_cus * per_cu, st);
+    mark_kernel(1, st);
+    // 4. the selk best candidates per query, as the rerank's rows
+    launch_select_candidates(cand_key, cand_pos, cnt, cap, qtot, a.k_q, a.k_default, R, a.allow_fallback, nq,
+                             a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
+    // 5. overflowed queries and brute-force fallbacks: the table scan over
+    //    all their probes (the reference's rules, counter included)
+    FusedArgs f{};
+    f.term = d_term.get();
+    f.probe_rank = probe_rank;
+    f.order = ovf;
+    f.counter = ctr + 2;
+    f.nq_dev = ctr + 1;
+    f.lds_probes = a.probe_cap;
+    ScanArgs fa = a;
+    fa.out_key = pkey;
+    const int wv = 4;
+    const size_t fshm = block_lds_bytes((int)dim, (int)m, (int)pmax, block_cap(kp, wv), f.lds_probes);
+    if (fshm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major scan: the overflow scan does not fit in LDS");
+    const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus * std::max<size_t>(1, (160 * 1024) / fshm));
+    launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
 }
 
 void PQIVF::resolve_timers() {
@@ -1056,9 +1208,12 @@ void PQIVF::resolve_timers() {
         HANN_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
         HANN_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
         HANN_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
+        float kt = 0;
+        HANN_HIP(hipEventElapsedTime(&kt, ev[4], ev[5]));
         t_coarse_ms += a;
         t_scan_ms += b;
         t_rerank_ms += c;
+        t_kernel_ms += kt;
         ++t_calls;
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -1526,6 +1681,16 @@ int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, 
     *calls = x.t_calls;
     x.t_coarse_ms = x.t_scan_ms = x.t_rerank_ms = 0;
     x.t_calls = 0;
+    HANN_API_END
+}
+
+int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.resolve_timers();
+    *kernel_ms = x.t_kernel_ms;
+    x.t_kernel_ms = 0;
     HANN_API_END
 }
 

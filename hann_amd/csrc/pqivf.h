@@ -114,6 +114,7 @@ struct PQIVF {
     // persistent scan: per-entry list terms, kept current by every sync
     DevBuf<float> d_term;
     DevBuf<int64_t> d_term_lists;
+    DevBuf<unsigned> d_ymax_tmax;   // max |y_e|, max |t_e| (float bits) over the entries
     void refresh_terms(const std::vector<int64_t> *lists);   // null: every list
     // the coarse step's BF16 candidate pass (used under the index lock)
     Context coarse_ctx;
@@ -151,12 +152,23 @@ struct PQIVF {
                      const hann_search_params &p, int64_t *out_ids, double *out_dist, int32_t *out_counts);
     int64_t fallback_count();
     void run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double *probe_rank, hipStream_t st,
-                        Workspace &w);
+                        Workspace &w, bool main_scan = true);
+    // list-major MFMA scan; a.out_* are the candidate rows of the rerank
+    void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
+                       Workspace &w);
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;
     int64_t t_calls = 0;
-    std::vector<std::array<hipEvent_t, 4>> t_events;   // pending, resolved by stage_times()
+    // per search: stage boundaries [0..3]; [4], [5] bracket the main scan
+    // kernel's launch (k_scan_list or k_scan_block or k_scan) when recorded
+    std::vector<std::array<hipEvent_t, 6>> t_events;   // pending, resolved by stage_times()
+    hipEvent_t *kmark = nullptr;   // the current search's [4], [5] while it launches
+    double t_kernel_ms = 0;
+    int64_t t_kernel_calls = 0;
+    void mark_kernel(int which, hipStream_t st) {
+        if (kmark) HANN_HIP(hipEventRecord(kmark[which], st));
+    }
     void resolve_timers();
     int64_t count() const;
     std::vector<unsigned char> save() const;

@@ -61,7 +61,8 @@ template <typename CodeT>
 __global__ void k_entry_terms(const int64_t *__restrict__ lists, int64_t j0, const int64_t *__restrict__ list_start,
                               const int64_t *__restrict__ list_len, const CodeT *__restrict__ codes,
                               const uint8_t *__restrict__ has, int d, int m, int P, const float *__restrict__ coarse,
-                              const float *__restrict__ cb, float *__restrict__ term) {
+                              const float *__restrict__ cb, float *__restrict__ term,
+                              unsigned *__restrict__ ymax_tmax) {
     const int64_t j = j0 + blockIdx.y;
     const int64_t l = lists ? lists[j] : j;
     const int s = d / m;
@@ -69,7 +70,7 @@ __global__ void k_entry_terms(const int64_t *__restrict__ lists, int64_t j0, con
     const int64_t e0 = list_start[l], n = list_len[l];
     for (int64_t r = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; r < n; r += (int64_t)gridDim.x * blockDim.x) {
         const int64_t e = e0 + r;
-        float t = 0.f;
+        float t = 0.f, yy = 0.f;
         if (has[e]) {
             for (int i = 0; i < m; ++i) {
                 const float *w = cb + ((int64_t)i * P + (int)codes[e * m + i]) * s;
@@ -79,9 +80,16 @@ __global__ void k_entry_terms(const int64_t *__restrict__ lists, int64_t j0, con
                     dc = __builtin_fmaf(c[i * s + u], w[u], dc);
                 }
                 t = t + (nn + 2.f * dc);
+                yy = yy + nn;
             }
         }
         term[e] = t;
+        // upper bounds of |y_e| and |t_e| over the index (bits of
+        // non-negative floats order like the floats)
+        if (ymax_tmax) {
+            atomicMax(&ymax_tmax[0], __float_as_uint(__builtin_sqrtf(yy) * 1.0001f));
+            atomicMax(&ymax_tmax[1], __float_as_uint(__builtin_fabsf(t) * 1.0001f + 1e-30f));
+        }
     }
 }
 
@@ -202,12 +210,13 @@ __global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) 
     int32_t *s_nch = s_lst + f.lds_probes;
     float *s_qc = reinterpret_cast<float *>(s_nch + f.lds_probes);
 
+    const int nq_all = f.nq_dev ? *f.nq_dev : f.nq;
     for (;;) {
         __syncthreads();
         if (tid == 0) s_item = atomicAdd(f.counter, 1);
         __syncthreads();
         const int item = s_item;
-        if (item >= f.nq) break;
+        if (item >= nq_all) break;
         const int64_t q = f.order[item];
 
         int kq = a.k_q ? a.k_q[q] : a.k_default;
@@ -392,17 +401,20 @@ int block_cap(int kp, int w) {
 
 void launch_entry_terms(const int64_t *lists, int64_t nlists, int64_t max_len, const int64_t *list_start,
                         const int64_t *list_len, const void *codes, int code_bytes, const uint8_t *has, int d, int m,
-                        int P, const float *coarse, const float *cb, float *term, hipStream_t st) {
+                        int P, const float *coarse, const float *cb, float *term, unsigned *ymax_tmax,
+                        hipStream_t st) {
     if (nlists <= 0 || max_len <= 0) return;
     const unsigned gx = (unsigned)std::min<int64_t>(64, ceil_div(max_len, 256));
     for (int64_t j0 = 0; j0 < nlists; j0 += 65535) {
         const unsigned gy = (unsigned)std::min<int64_t>(65535, nlists - j0);
         if (code_bytes == 1)
             hipLaunchKernelGGL(dev::k_entry_terms<uint8_t>, dim3(gx, gy), dim3(256), 0, st, lists, j0, list_start,
-                               list_len, reinterpret_cast<const uint8_t *>(codes), has, d, m, P, coarse, cb, term);
+                               list_len, reinterpret_cast<const uint8_t *>(codes), has, d, m, P, coarse, cb, term,
+                               ymax_tmax);
         else
             hipLaunchKernelGGL(dev::k_entry_terms<uint16_t>, dim3(gx, gy), dim3(256), 0, st, lists, j0, list_start,
-                               list_len, reinterpret_cast<const uint16_t *>(codes), has, d, m, P, coarse, cb, term);
+                               list_len, reinterpret_cast<const uint16_t *>(codes), has, d, m, P, coarse, cb, term,
+                               ymax_tmax);
         HANN_LAUNCH_CHECK();
     }
 }
@@ -413,9 +425,11 @@ void launch_scan_block(const ScanArgs &a_in, FusedArgs f, int64_t nq, int w, int
     ScanArgs a = a_in;
     a.cap = block_cap(kp, w);
     f.nq = (int)nq;
-    hipLaunchKernelGGL(dev::k_cost_order, dim3(1), dim3(1024), 0, st, a.nprobe_q, a.nprobe_default, (int)nq, maxp,
-                       f.order, f.counter);
-    HANN_LAUNCH_CHECK();
+    if (!f.nq_dev) {
+        hipLaunchKernelGGL(dev::k_cost_order, dim3(1), dim3(1024), 0, st, a.nprobe_q, a.nprobe_default, (int)nq,
+                           maxp, f.order, f.counter);
+        HANN_LAUNCH_CHECK();
+    }
     const size_t shm = block_lds_bytes(a.d, a.m, a.pmax_cb, a.cap, f.lds_probes);
     if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("list scan needs %zu bytes of LDS", shm));
 #define HANN_BLOCK(W, CT, M)                                                                              \

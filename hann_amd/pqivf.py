@@ -234,6 +234,13 @@ class Index:
                                                     ctypes.byref(n)))
         return a.value, b.value, c.value, n.value
 
+    def scan_kernel_time(self):
+        """Milliseconds in the main scan kernel over the profiled searches
+        (call after stage_times)."""
+        v = ctypes.c_double()
+        _lib.check(self._lib.hann_pqivf_scan_kernel_time(self._h, ctypes.byref(v)))
+        return v.value
+
     def export_lists(self, with_vectors=True):
         """(list_off [k+1], ids [N], codes [N][m], vecs [N][dim] or None)"""
         k, m, _, d = self.shape()

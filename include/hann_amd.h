@@ -176,7 +176,14 @@ typedef enum {
     HANN_SCAN_EXACT = 0,   /* decode + reference-order FP32 distance: bit-identical to
                               pqivf/index.go:905-936 on the AVX2 tier */
     HANN_SCAN_ADC_F32 = 1, /* FP32 ADC lookup tables */
-    HANN_SCAN_ADC_F16 = 2  /* FP16 ADC lookup tables, FP32 accumulation */
+    HANN_SCAN_ADC_F16 = 2, /* FP16 ADC lookup tables, FP32 accumulation */
+    HANN_SCAN_LIST_BF16 = 3,   /* list-major scan: each probed list decoded once per batch
+                                  and scored against all its queries on the matrix cores
+                                  (BF16 operands, FP32 accumulation); needs rerank > 0,
+                                  otherwise (and for indexes with entries lacking codes)
+                                  the persistent FP32 table scan runs */
+    HANN_SCAN_LIST_BF16X3 = 4  /* the same with split BF16 operands (hi + lo, three
+                                  products): inner products to about FP32 accuracy */
 } hann_scan_mode;
 
 typedef struct {
@@ -234,6 +241,11 @@ void hann_free_buffer(void *buf);
 int hann_pqivf_profile(hann_pqivf *idx, int enable);
 int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, double *rerank_ms,
                            int64_t *calls);
+/* Milliseconds spent in the main list-scan kernel of the profiled searches
+ * (one launch per search: k_scan_list, k_scan_block or k_scan), from HIP
+ * events on the search stream; resets the total. Call after
+ * hann_pqivf_stage_times, which resolves the events (build extension). */
+int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms);
 
 /* Bulk export of the clustered entries in list order (CSR): list_off
  * [k+1], ids [N], codes [N][m] (rows of -1: entry without codes), vecs

@@ -629,3 +629,105 @@ def test_bf16_coarse_step_identical(hann, mode, fused, rerank):
                          coarse_precision=_lib.PREC_BF16)
     for u, v in zip(a, b):
         assert np.array_equal(u, v)
+
+
+# ------------------------------------------------- list-major MFMA scan --
+def list_index(n=20000, coarse_k=64, seed=4):
+    from hann_amd import synthetic
+    pq = P()
+    x, qs = synthetic.config1(n=n, q=300)
+    idx = pq.New(64, pq.WithCoarseK(coarse_k), pq.WithNumSubquantizers(8), pq.WithPQK(256),
+                 pq.WithKMeansIters(5), pq.WithSeed(seed))
+    idx.bulk_add_arrays(np.arange(n), x)
+    idx.Train()
+    return idx, x, qs
+
+
+@pytest.mark.parametrize("mode,rerank", [(3, 4), (4, 4), (4, 2)])
+def test_list_scan_matches_table_scan(hann, oracle, mode, rerank):
+    """The list-major MFMA scan selects candidates by BF16 (mode 3) or split
+    BF16 (mode 4) inner products, then reranks in exact FP32: returned
+    distances are the exact reference-order ones, and the results agree
+    with the FP32 table scan + rerank (all but near-ties for the split
+    operands, >= 95 % overlap for plain BF16)."""
+    from hann_amd import _lib
+    idx, x, qs = list_index()
+    rng = np.random.default_rng(7)
+    ks = rng.choice([1, 10, 50], 300).astype(np.int32)
+    nps = rng.choice([1, 4, 8, 16, 64], 300).astype(np.int32)
+    a_ids, a_d, a_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_ADC_F32, rerank=rerank, fused=1)
+    b_ids, b_d, b_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=1)
+    assert np.array_equal(a_c, b_c)
+    same = 0
+    for qi in range(300):
+        kq = int(ks[qi])
+        for r in range(0, kq, max(1, kq // 4)):
+            j = int(b_ids[qi, r])
+            assert b_d[qi, r] == np.sqrt(np.float64(oracle.pair(0, qs[qi], x[j])))
+        assert np.all(np.diff(b_d[qi, :kq]) >= 0)
+        same += int(np.array_equal(a_ids[qi, :kq], b_ids[qi, :kq]))
+    both = [len(set(a_ids[q, :ks[q]].tolist()) & set(b_ids[q, :ks[q]].tolist())) / ks[q] for q in range(300)]
+    if mode == 4:
+        assert same >= 295
+    assert np.mean(both) >= 0.95
+
+
+def test_list_scan_overflow_and_open_threshold(hann):
+    """Lists smaller than k*R leave the pre-pass without a threshold: every
+    probed entry becomes a candidate, and queries whose candidates exceed
+    the room take the table scan. Results equal the table scan's when the
+    candidates hold every probed entry."""
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
+    for nprobe in (16, 64):
+        a = idx.search_batch(qs, 50, nprobe=nprobe, scan_mode=_lib.SCAN_ADC_F32, rerank=4, fused=1)
+        b = idx.search_batch(qs, 50, nprobe=nprobe, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
+        assert np.array_equal(a[2], b[2])
+        agree = np.mean([len(set(a[0][q].tolist()) & set(b[0][q].tolist())) / 50 for q in range(len(qs))])
+        assert agree >= 0.99, (nprobe, agree)
+
+
+def test_list_scan_fallback_counter(hann):
+    """The brute-force fallback (fewer than k probed entries) keeps the
+    reference's rule and counter under the list-major scan."""
+    from hann_amd import _lib
+    idx = new_small(4, 8, 2, 2, 5)
+    for i in range(40):
+        idx.Add(i, [i, i + 1, i + 2, i + 3])
+    idx.Train()
+    q = np.array([[0, 1, 2, 3]], np.float32)
+    ids, d, c = idx.search_batch(q, 40, nprobe=3, scan_mode=_lib.SCAN_LIST_BF16, rerank=2, fused=1)
+    assert idx.Stats().FallbackSearches >= 1
+    assert c[0] == 40 and sorted(ids[0].tolist()) == list(range(40))
+
+
+def test_list_scan_without_rerank_uses_table_scan(hann):
+    """Without a rerank the list modes return the FP32 table scan's results."""
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=6000, coarse_k=32, seed=2)
+    a = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_ADC_F32, rerank=0, fused=1)
+    b = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16, rerank=0, fused=1)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def test_list_scan_after_mutations(hann):
+    """Appended, deleted and updated entries reach the list-major scan (its
+    per-entry terms are refreshed with the device mirror)."""
+    from hann_amd import _lib
+    pq = P()
+    idx, x, qs = list_index(n=12000, coarse_k=32, seed=3)
+    idx.BulkDelete(list(range(0, 12000, 7)))
+    rng = np.random.default_rng(1)
+    idx.bulk_add_arrays(np.arange(20000, 20500), x[:500] + rng.standard_normal((500, 64)).astype(np.float32) * 0.1)
+    other = pq.New(64)
+    other.load_bytes(idx.save_bytes())
+    a = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
+    b = other.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
+    assert np.array_equal(a[2], b[2])
+    # entry positions differ between the two device layouts, which can only
+    # move candidates tied at the selection margin
+    agree = np.mean([len(set(a[0][q].tolist()) & set(b[0][q].tolist())) / 10 for q in range(len(qs))])
+    assert agree >= 0.99
+    live = set(np.concatenate([np.asarray(v) for v in pq.InvertedIDs(idx).values()]).tolist())
+    assert set(a[0][a[0] >= 0].tolist()) <= live
