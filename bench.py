@@ -276,6 +276,8 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered queries/s (0: half the chosen QPS)")
     ap.add_argument("--paths", default="", help="comma-separated subset of the chosen-path candidates")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="run one device copy of this many MB first (a known-size kernel for counter scaling)")
     a = ap.parse_args()
     if a.gpus != 1:
         print(f"bench.py measures one GPU: this build runs on a single MI355X (asked for --gpus {a.gpus}).",
@@ -287,6 +289,13 @@ def main():
     lib = _lib.load()
     _lib.check(lib.hann_init(0))
     torch.cuda.init()
+    if a.copy_probe_mb > 0:
+        src = torch.ones(a.copy_probe_mb << 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {a.copy_probe_mb} MB read, {a.copy_probe_mb} MB written")
+        del src, dst
     if a.workload == "topk":
         print(json.dumps(bench_topk(a, torch, lib)), flush=True)
         return

@@ -7,9 +7,11 @@ namespace hann {
 
 // Grow-only device scratch, in numbered slots, owned by one stream's user.
 // Slot map: 0-4 and 10-11 host-facing staging of the entry points, 5-9 the
-// search pipeline, 14-15 launch_topk_exact's partial lists.
+// search pipeline, 12-13 the persistent scan's queue, 14-15
+// launch_topk_exact's partial lists, 16 device_min, 17-19 the list-major
+// scan, 20 the batch's query tables.
 struct Workspace {
-    DevBuf<char> slots[20];
+    DevBuf<char> slots[24];
     template <typename T>
     T *get(int slot, size_t count) {
         slots[slot].resize(count * sizeof(T) + 256);

@@ -1039,6 +1039,12 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
     f.counter = w.get<int32_t>(13, 1);
     // per-probe tables in LDS when each query has its own probe row
     f.lds_probes = a.probe_stride > 0 ? a.probe_cap : 0;
+    // the batch's query tables in one pass (up to 512 MB of them at once)
+    if ((size_t)nq * m * pmax * 4 <= (512ull << 20)) {
+        float *S = w.get<float>(20, (size_t)nq * m * pmax);
+        launch_query_tables(a.queries, nq, (int)dim, (int)m, (int)pmax, d_cb.get(), S, st);
+        f.S = S;
+    }
     const int kp = a.cap / 2;
     auto lds_for = [&](int wv) { return block_lds_bytes((int)dim, (int)m, (int)pmax, block_cap(kp, wv), f.lds_probes); };
     auto grid_for = [&](int wv) {

@@ -13,10 +13,14 @@ struct FusedArgs {
     int nq;
     int lds_probes;            // probes per query kept in LDS (0: read from global memory)
     const int32_t *nq_dev;     // or null: queries order[0 .. *nq_dev) (the caller zeroes counter)
+    const float *S;            // [nq][m][P] query tables, or null: built per query in LDS
 };
 
 size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes);
-// top-k room of the scan: a power of two >= max(2 kp, kp + 64 w)
+// S[q][i][j] = -2 <q_i, cb_ij> for every query of a batch
+void launch_query_tables(const float *queries, int64_t nq, int d, int m, int P, const float *cb, float *S,
+                         hipStream_t st);
+// top-k room of the scan: a power of two >= max(2 kp, kp + 128 w)
 int block_cap(int kp, int w);
 // per-entry terms of the scan for lists[0..nlists) (every list when lists is
 // null; nlists is then K); max_len bounds the lists' lengths; ymax_tmax (or

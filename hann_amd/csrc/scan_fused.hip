@@ -93,6 +93,27 @@ __global__ void k_entry_terms(const int64_t *__restrict__ lists, int64_t j0, con
     }
 }
 
+// S[q][i][j] = -2 <q_i, cb_ij> (the same FMA chain as the in-kernel build);
+// grid: (ceil(nq / 16), m), codebook i staged in LDS
+__global__ void __launch_bounds__(256) k_query_tables(const float *__restrict__ queries, int64_t nq, int d, int m,
+                                                      int P, const float *__restrict__ cb, float *__restrict__ S) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    float *s_cb = reinterpret_cast<float *>(s_raw);
+    const int i = blockIdx.y, s = d / m;
+    const int64_t q0 = (int64_t)blockIdx.x * 16;
+    for (int t = threadIdx.x; t < P * s; t += 256) s_cb[t] = cb[(int64_t)i * P * s + t];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 16 * P; t += 256) {
+        const int ql = t / P, j = t - ql * P;
+        const int64_t q = q0 + ql;
+        if (q >= nq) break;
+        const float *qi = queries + q * d + i * s;
+        float dc = 0.f;
+        for (int u = 0; u < s; ++u) dc = __builtin_fmaf(qi[u], s_cb[j * s + u], dc);
+        S[(q * m + i) * P + j] = -2.f * dc;
+    }
+}
+
 // acc + sum_i S[i][code_i] in subquantizer order (uint8 codes)
 template <int M>
 __device__ __forceinline__ float split_sum(const uint8_t *codes, int64_t e, int m, int P, const float *S, float acc) {
@@ -189,14 +210,15 @@ struct BlockTopK {
 };
 
 // LDS: q[d] | S_q[m*P] (FP32) | keys[cap] | ids[cap] | per probe: list, chunks, |q-c|^2
-template <int W, typename CodeT, int M>
-__global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) {
+// PC: the codebook stride P when known at compile time (256), else 0
+template <int W, typename CodeT, int M, int PC>
+__global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_scan_block(ScanArgs a, FusedArgs f) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     __shared__ int s_item;
     __shared__ int64_t s_red[2][W];
     __shared__ int s_cnt[2][W];
     const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
-    const int d = a.d, m = a.m, s = a.s, P = a.pmax_cb, MP = m * P;
+    const int d = a.d, m = a.m, s = a.s, P = PC ? PC : a.pmax_cb, MP = m * P;
     const int cap = a.cap;
     float *s_q = reinterpret_cast<float *>(s_raw);
     size_t off = ((size_t)d * 4 + 15) & ~(size_t)15;
@@ -231,13 +253,23 @@ __global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) 
 
         for (int e = tid; e < d; e += blockDim.x) s_q[e] = a.queries[q * d + e];
         __syncthreads();
-        // S_q[i][j] = -2 <q_i, cb_ij>
-        for (int t = tid; t < MP; t += blockDim.x) {
-            const int i = t / P, j = t - i * P;
-            const float *w = a.cb + ((int64_t)i * P + j) * s;
-            float dc = 0.f;
-            for (int u = 0; u < s; ++u) dc = __builtin_fmaf(s_q[i * s + u], w[u], dc);
-            s_S[t] = -2.f * dc;
+        // S_q[i][j] = -2 <q_i, cb_ij>: from the batch's tables, or built here
+        if (f.S) {
+            const float *src = f.S + q * MP;
+            if (MP % 4 == 0) {
+                for (int t = tid; t < MP / 4; t += blockDim.x)
+                    reinterpret_cast<float4 *>(s_S)[t] = reinterpret_cast<const float4 *>(src)[t];
+            } else {
+                for (int t = tid; t < MP; t += blockDim.x) s_S[t] = src[t];
+            }
+        } else {
+            for (int t = tid; t < MP; t += blockDim.x) {
+                const int i = t / P, j = t - i * P;
+                const float *w = a.cb + ((int64_t)i * P + j) * s;
+                float dc = 0.f;
+                for (int u = 0; u < s; ++u) dc = __builtin_fmaf(s_q[i * s + u], w[u], dc);
+                s_S[t] = -2.f * dc;
+            }
         }
         // probed lists: entries and 64-entry chunks
         int64_t tot_e = 0, tot_c = 0;
@@ -309,31 +341,15 @@ __global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) 
         };
         enter(0);
         const int64_t rounds = (tot_c + W - 1) / W;
-        for (int64_t r = 0; r < rounds; ++r) {
+        auto walk = [&]() {
             while (li < nl && c >= nch) {
                 c -= (int)nch;
                 enter(li + 1);
             }
-            const int64_t e = e0 + (int64_t)c * kWave + lane;
-            const bool valid = li < nl && (int64_t)c * kWave + lane < len;
-            float key = 0.f;
-            int64_t id = 0;
-            if (valid) {
-                if (!a.any_uncoded || a.has_codes[e]) {
-                    if constexpr (sizeof(CodeT) == 1)
-                        key = split_sum<M>(reinterpret_cast<const uint8_t *>(a.codes), e, m, P, s_S, qc + f.term[e]);
-                    else
-                        key = split_sum16<M>(reinterpret_cast<const uint16_t *>(a.codes), e, m, P, s_S,
-                                             qc + f.term[e]);
-                } else {
-                    key = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.vecs + e * d}, d);
-                }
-                key = sane_key(key);
-                id = a.rerank > 0 ? e : a.ids[e];
-            }
-            c += W;
-            // append the round's survivors; the counts go through LDS so
-            // that every thread knows the block's fill without atomics
+        };
+        // append one round's survivors; the counts go through LDS so that
+        // every thread knows the block's fill without atomics
+        auto push_round = [&](float key, int64_t id, bool valid, int64_t r) {
             bool pass = valid && cand_less(key, id, sel.tk, sel.ti);
             uint64_t mask = __ballot(pass);
             const int par = (int)(r & 1);
@@ -369,6 +385,126 @@ __global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) 
                 s_ids[pos] = id;
             }
             sel.n += sum;
+        };
+        constexpr bool PIPE = sizeof(CodeT) == 1 && M > 0 && M % 8 == 0;
+        if (PIPE && !a.any_uncoded) {
+            // two chunks per wave per round (two independent gather chains,
+            // half the barriers), software-pipelined: the next round's codes,
+            // terms and ids are in flight while this round gathers from S_q
+            constexpr int NW8 = PIPE ? M / 8 : 1;
+            const uint2 *cw = reinterpret_cast<const uint2 *>(a.codes);
+            struct Slot {
+                uint2 w[NW8];
+                float t, qc;
+                int64_t e, id;
+                bool v;
+            };
+            Slot nx[2];
+            auto locate = [&](Slot &x) {
+                walk();
+                x.e = e0 + (int64_t)c * kWave + lane;
+                x.v = li < nl && (int64_t)c * kWave + lane < len;
+                x.qc = qc;
+                if (x.v) {
+#pragma unroll
+                    for (int j = 0; j < NW8; ++j) x.w[j] = cw[x.e * NW8 + j];
+                    x.t = f.term[x.e];
+                    x.id = a.rerank > 0 ? x.e : a.ids[x.e];
+                }
+                c += W;
+            };
+            auto gather = [&](const Slot &x) {
+                float acc = x.qc + x.t;
+#pragma unroll
+                for (int j = 0; j < NW8; ++j) {
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) acc = acc + s_S[(8 * j + b) * P + (int)((x.w[j].x >> (8 * b)) & 0xFFu)];
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        acc = acc + s_S[(8 * j + 4 + b) * P + (int)((x.w[j].y >> (8 * b)) & 0xFFu)];
+                }
+                return sane_key(acc);
+            };
+            locate(nx[0]);
+            locate(nx[1]);
+            const int64_t rounds2 = (tot_c + 2 * W - 1) / (2 * W);
+            for (int64_t r = 0; r < rounds2; ++r) {
+                const Slot cu0 = nx[0], cu1 = nx[1];
+                locate(nx[0]);
+                locate(nx[1]);
+                const float k0 = cu0.v ? gather(cu0) : 0.f;
+                const float k1 = cu1.v ? gather(cu1) : 0.f;
+                const int64_t i0 = cu0.v ? cu0.id : 0, i1 = cu1.v ? cu1.id : 0;
+                // append both chunks' survivors (counts through LDS)
+                bool p0 = cu0.v && cand_less(k0, i0, sel.tk, sel.ti);
+                bool p1 = cu1.v && cand_less(k1, i1, sel.tk, sel.ti);
+                uint64_t m0 = __ballot(p0), m1 = __ballot(p1);
+                const int par = (int)(r & 1);
+                if (lane == 0) s_cnt[par][wave] = __popcll(m0) + __popcll(m1);
+                __syncthreads();
+                int sum = 0, base = 0;
+                for (int w = 0; w < W; ++w) {
+                    const int v = s_cnt[par][w];
+                    base += w < wave ? v : 0;
+                    sum += v;
+                }
+                if (sel.n + sum > sel.cap) {
+                    sel.compact();   // cap >= k + 128 W: the round now fits
+                    __syncthreads();
+                    p0 = p0 && cand_less(k0, i0, sel.tk, sel.ti);
+                    p1 = p1 && cand_less(k1, i1, sel.tk, sel.ti);
+                    m0 = __ballot(p0);
+                    m1 = __ballot(p1);
+                    if (lane == 0) s_cnt[par][wave] = __popcll(m0) + __popcll(m1);
+                    __syncthreads();
+                    sum = 0;
+                    base = 0;
+                    for (int w = 0; w < W; ++w) {
+                        const int v = s_cnt[par][w];
+                        base += w < wave ? v : 0;
+                        sum += v;
+                    }
+                    __syncthreads();
+                }
+                const int below0 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m0, 0));
+                const int below1 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m1 >> 32),
+                                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)m1, 0));
+                if (p0) {
+                    s_keys[sel.n + base + below0] = k0;
+                    s_ids[sel.n + base + below0] = i0;
+                }
+                if (p1) {
+                    const int pos = sel.n + base + __popcll(m0) + below1;
+                    s_keys[pos] = k1;
+                    s_ids[pos] = i1;
+                }
+                sel.n += sum;
+            }
+        } else {
+            for (int64_t r = 0; r < rounds; ++r) {
+                walk();
+                const int64_t e = e0 + (int64_t)c * kWave + lane;
+                const bool valid = li < nl && (int64_t)c * kWave + lane < len;
+                float key = 0.f;
+                int64_t id = 0;
+                if (valid) {
+                    if (!a.any_uncoded || a.has_codes[e]) {
+                        if constexpr (sizeof(CodeT) == 1)
+                            key = split_sum<M>(reinterpret_cast<const uint8_t *>(a.codes), e, m, P, s_S,
+                                               qc + f.term[e]);
+                        else
+                            key = split_sum16<M>(reinterpret_cast<const uint16_t *>(a.codes), e, m, P, s_S,
+                                                 qc + f.term[e]);
+                    } else {
+                        key = sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.vecs + e * d}, d);
+                    }
+                    key = sane_key(key);
+                    id = a.rerank > 0 ? e : a.ids[e];
+                }
+                c += W;
+                push_round(key, id, valid, r);
+            }
         }
         __syncthreads();
         sel.compact();
@@ -388,6 +524,16 @@ __global__ void __launch_bounds__(64 * W) k_scan_block(ScanArgs a, FusedArgs f) 
 
 }  // namespace dev
 
+void launch_query_tables(const float *queries, int64_t nq, int d, int m, int P, const float *cb, float *S,
+                         hipStream_t st) {
+    if (nq == 0) return;
+    const size_t shm = (size_t)P * (d / m) * 4;
+    if (shm > 64 * 1024) fail(HANN_ERR_UNSUPPORTED, "query tables: codebook too large");
+    hipLaunchKernelGGL(dev::k_query_tables, dim3((unsigned)ceil_div(nq, 16), (unsigned)m), dim3(256), shm, st, queries,
+                       nq, d, m, P, cb, S);
+    HANN_LAUNCH_CHECK();
+}
+
 size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes) {
     return (((size_t)d * 4 + 15) & ~(size_t)15) + (((size_t)m * P * 4 + 15) & ~(size_t)15) +
            (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (size_t)lds_probes * 12 + 16;
@@ -395,7 +541,7 @@ size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes) {
 
 int block_cap(int kp, int w) {
     int c = 64;
-    while (c < 2 * kp || c < kp + 64 * w) c <<= 1;
+    while (c < 2 * kp || c < kp + 128 * w) c <<= 1;
     return c;
 }
 
@@ -432,12 +578,20 @@ void launch_scan_block(const ScanArgs &a_in, FusedArgs f, int64_t nq, int w, int
     }
     const size_t shm = block_lds_bytes(a.d, a.m, a.pmax_cb, a.cap, f.lds_probes);
     if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("list scan needs %zu bytes of LDS", shm));
-#define HANN_BLOCK(W, CT, M)                                                                              \
+#define HANN_BLOCK_P(W, CT, M, PP)                                                                        \
     do {                                                                                                  \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_block<W, CT, M>,                           \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_block<W, CT, M, PP>,                       \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));              \
-        hipLaunchKernelGGL((dev::k_scan_block<W, CT, M>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, f); \
+        hipLaunchKernelGGL((dev::k_scan_block<W, CT, M, PP>), dim3((unsigned)grid), dim3(64 * W), shm, st, a, \
+                           f);                                                                            \
         HANN_LAUNCH_CHECK();                                                                              \
+    } while (0)
+#define HANN_BLOCK(W, CT, M)                                        \
+    do {                                                            \
+        if (sizeof(CT) == 1 && M > 0 && a.pmax_cb == 256)           \
+            HANN_BLOCK_P(W, CT, M, 256);                            \
+        else                                                        \
+            HANN_BLOCK_P(W, CT, M, 0);                              \
     } while (0)
 #define HANN_BLOCK_W(CT, M)                         \
     do {                                            \
