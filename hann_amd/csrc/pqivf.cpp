@@ -972,6 +972,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
     const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
     if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
+                      reinterpret_cast<uintptr_t>(dq) % 16 != 0 ||
                       !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
                       list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax) > 160 * 1024)) {
         // the list-major scan needs a rerank, codes for every entry, a probe
@@ -1133,8 +1134,8 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     float *cand_key = w.get<float>(18, (size_t)nq * cap);
     int32_t *cand_pos = w.get<int32_t>(19, (size_t)nq * cap);
 
-    // 1. pre-pass: the table scan over each query's nearest lists bounds its
-    //    selk-th distance
+    // 1. pre-pass: the table scan over each query's nearest lists, for an
+    //    estimate of its selk-th distance
     ScanArgs pre = a;
     pre.probe_cap = std::min(np, 2);
     pre.allow_fallback = 0;
@@ -1143,11 +1144,11 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     pre.out_id = ppos;
     pre.out_cnt = pcnt;
     run_fused_scan(pre, nq, std::min(maxp, 2), probe_rank, st, w, false);
-    launch_list_thresholds(a, nq, pkey, pcnt, selk, probe_rank, reinterpret_cast<const float *>(d_ymax_tmax.get()),
-                           x3, tau, st);
     // 2. the batch's (query, list) pairs grouped by list, as work items
     launch_fill_i32(lcnt, K, 0, st);
     launch_pair_count(a, nq, lcnt, qtot, st);
+    launch_list_thresholds(a, nq, pkey, pcnt, selk, pre.probe_cap, qtot, probe_rank,
+                           reinterpret_cast<const float *>(d_ymax_tmax.get()), x3, tau, st);
     launch_pair_offsets(lcnt, K, lstart, ioff, lfill, totals, st);
     launch_pair_scatter(a, nq, probe_rank, lstart, lfill, pair_q, pair_qc, st);
     launch_make_items(lcnt, lstart, ioff, K, items, st);

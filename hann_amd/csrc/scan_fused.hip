@@ -209,7 +209,8 @@ struct BlockTopK {
     }
 };
 
-// LDS: q[d] | S_q[m*P] (FP32) | keys[cap] | ids[cap] | per probe: list, chunks, |q-c|^2
+// LDS: q[d] | S_q[m*P] (FP32) | keys[cap] | ids[cap] | per probe: first slot,
+// length, list, chunks, |q-c|^2
 // PC: the codebook stride P when known at compile time (256), else 0
 template <int W, typename CodeT, int M, int PC>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_scan_block(ScanArgs a, FusedArgs f) {
@@ -228,7 +229,9 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     off += ((size_t)cap * 4 + 15) & ~(size_t)15;
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_raw + off);
     off += (size_t)cap * 8;
-    int32_t *s_lst = reinterpret_cast<int32_t *>(s_raw + off);
+    int64_t *s_e0 = reinterpret_cast<int64_t *>(s_raw + off);   // per probe: first slot, length
+    int32_t *s_len = reinterpret_cast<int32_t *>(s_e0 + f.lds_probes);
+    int32_t *s_lst = s_len + f.lds_probes;
     int32_t *s_nch = s_lst + f.lds_probes;
     float *s_qc = reinterpret_cast<float *>(s_nch + f.lds_probes);
 
@@ -279,6 +282,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             tot_e += len;
             tot_c += (len + kWave - 1) / kWave;
             if (in_lds) {
+                s_e0[li] = a.list_start[l];
+                s_len[li] = (int32_t)len;
                 s_lst[li] = (int32_t)l;
                 s_nch[li] = (int32_t)((len + kWave - 1) / kWave);
                 s_qc[li] = f.probe_rank ? (float)f.probe_rank[q * a.probe_stride + li]
@@ -330,14 +335,16 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                 l = s_lst[li];
                 nch = s_nch[li];
                 qc = s_qc[li];
+                e0 = s_e0[li];
+                len = s_len[li];
             } else {
                 l = fb ? li : probes[li];
                 nch = (a.list_len[l] + kWave - 1) / kWave;
                 qc = (!fb && f.probe_rank) ? (float)f.probe_rank[q * a.probe_stride + li]
                                            : sqdiff_exact(PtrLoad{s_q}, PtrLoad{a.coarse + l * d}, d);
+                e0 = a.list_start[l];
+                len = a.list_len[l];
             }
-            e0 = a.list_start[l];
-            len = a.list_len[l];
         };
         enter(0);
         const int64_t rounds = (tot_c + W - 1) / W;
@@ -536,7 +543,7 @@ void launch_query_tables(const float *queries, int64_t nq, int d, int m, int P, 
 
 size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes) {
     return (((size_t)d * 4 + 15) & ~(size_t)15) + (((size_t)m * P * 4 + 15) & ~(size_t)15) +
-           (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (size_t)lds_probes * 12 + 16;
+           (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (size_t)lds_probes * 24 + 16;
 }
 
 int block_cap(int kp, int w) {

@@ -38,11 +38,12 @@ void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank
                          int32_t *lfill, int32_t *pair_q, float *pair_qc, hipStream_t st);
 void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, int K, int4 *items,
                        hipStream_t st);
-// tau[q] from the pre-pass (the selk-th key over the nearest lists, a
-// per-table-scan distance) widened by the bound on |MFMA - table| distance
+// tau[q] from the pre-pass over the nearest lists (its keys in ascending
+// order): the selk-th key, widened by the bound on |MFMA - table| distance
+// (pre_lists and qtot are kept for threshold estimates that use them)
 void launch_list_thresholds(const ScanArgs &a, int64_t nq, const float *pre_key, const int32_t *pre_cnt,
-                            int pre_stride, const double *probe_rank, const float *ymax_tmax, int x3, float *tau,
-                            hipStream_t st);
+                            int pre_stride, int pre_lists, const int64_t *qtot, const double *probe_rank,
+                            const float *ymax_tmax, int x3, float *tau, hipStream_t st);
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);

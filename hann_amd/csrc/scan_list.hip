@@ -12,11 +12,13 @@
 // v_mfma_f32_32x32x16_bf16 over d. Each list's codes are read once per item
 // instead of once per (query, list).
 //
-// Candidates: every entry within the query's threshold tau_q. tau_q is the
-// selk-th smallest table-scan distance over the query's nearest lists (a
-// cheap pre-pass), widened by a bound on |MFMA - table| distance, so each of
-// those selk entries passes and the candidate list holds the selk best by
-// the MFMA distance. The selk best are then reranked in exact FP32.
+// Candidates: every entry within the query's threshold tau_q: the selk-th
+// table-scan distance over its nearest lists (a pre-pass), widened by a
+// bound on |MFMA - table| distance. At least selk entries pass, and the
+// selk best by MFMA distance are among them (each is at most the selk-th
+// passing distance); queries with more than the candidate room (or, as a
+// guard, fewer than selk) are redone by the table scan. The selk best are
+// then reranked in exact FP32.
 #include "common.h"
 #include "scan_list.h"
 
@@ -124,9 +126,9 @@ __global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__
 
 // one thread per query
 __global__ void k_list_thresholds(ScanArgs a, int64_t nq, const float *__restrict__ pre_key,
-                                  const int32_t *__restrict__ pre_cnt, int pre_stride,
-                                  const double *__restrict__ probe_rank, const float *__restrict__ ymax_tmax,
-                                  float coef, float *__restrict__ tau) {
+                                  const int32_t *__restrict__ pre_cnt, int pre_stride, int pre_lists,
+                                  const int64_t *__restrict__ qtot, const double *__restrict__ probe_rank,
+                                  const float *__restrict__ ymax_tmax, float coef, float *__restrict__ tau) {
     const int64_t q = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (q >= nq) return;
     int kq = a.k_q ? a.k_q[q] : a.k_default;
@@ -137,6 +139,10 @@ __global__ void k_list_thresholds(ScanArgs a, int64_t nq, const float *__restric
         tau[q] = __builtin_inff();
         return;
     }
+    // the selk-th key of the nearest lists bounds the selk-th over all
+    // probed lists (a superset): at least selk entries pass
+    (void)pre_lists;
+    (void)qtot;
     const double t0 = pre_key[q * pre_stride + selk - 1];
     double qq = 0.0;
     for (int j = 0; j < a.d; ++j) {
@@ -151,21 +157,28 @@ __global__ void k_list_thresholds(ScanArgs a, int64_t nq, const float *__restric
     tau[q] = (float)((t0 + bound) * (1.0 + 1.0 / 1048576.0) + 1e-30);
 }
 
-// NW waves per workgroup; s == 4 (d = 4 M) and uint8 codes. The codebook
-// sits in LDS as BF16 (hi, and lo for the split mode) for the kernel's
-// lifetime; the MFMA's A operand of lane l for k-step j is the two codewords
-// of subquantizers 4j + 2h and 4j + 2h + 1 (h = l / 32) of entry row l % 32,
-// gathered straight from LDS into registers.
-constexpr int LNW = 8;
+// LNW = 4 waves per workgroup; s == 4 (d = 4 M) and uint8 codes. The
+// codebook sits in LDS as BF16 (hi, and lo for the split mode) for the
+// kernel's lifetime; the MFMA's A operand of lane l for k-step j is the two
+// codewords of subquantizers 4j + 2h and 4j + 2h + 1 (h = l / 32) of entry
+// row l % 32, gathered straight from LDS into registers.
+//
+// Latency: a workgroup takes its next item from the queue while it starts
+// the current one, and every thread prefetches its share of the next item
+// (its query row slice, the pair's rank and threshold, and the first code
+// slab) into registers; inside an item each wave loads its next slab's codes
+// and terms while it computes the current one.
+constexpr int LNW = 4;
 
 template <bool X3, int M>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     extern __shared__ __align__(16) unsigned char s_raw[];
-    __shared__ int s_item, s_nitems;
+    __shared__ int s_next, s_nitems;
     __shared__ int s_cnt[LQ];
     __shared__ float s_qc[LQ], s_tau[LQ];
     __shared__ int s_qid[LQ];
-    constexpr int D = 4 * M, LD = D + 8;   // LD: Q row stride in BF16 (16-byte rows, fewer bank conflicts)
+    constexpr int D = 4 * M, LD = D + 8;   // LD: Q row stride in BF16 (16-byte rows)
+    constexpr int QV = M / 8;              // float4 per thread of a 32 x D query tile (8 threads a row)
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int P = la.P;
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
@@ -195,62 +208,110 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             sCBl[t] = g;
         }
     }
-    if (tid == 0) s_nitems = *la.n_items;
+    if (tid == 0) {
+        s_nitems = *la.n_items;
+        s_next = atomicAdd(la.item_counter, 1);
+    }
     __syncthreads();
     const int n_items = s_nitems;
     const int col = lane & 31, half = lane >> 5;
+    const int qrow = tid >> 3, qsub = tid & 7;   // this thread's slice of the query tile
+
+    // prefetched state of the next item
+    int4 nh = make_int4(0, 0, 0, 0);
+    int nqid = -1;
+    float nqc = 0.f, ntau = 0.f;
+    float4 nqv[QV];
+    int64_t ne0 = 0, nn = 0;
+    uint32_t nw[M / 4];
+    float nt = 0.f;
+    auto load_codes = [&](int64_t e0, int64_t n, int64_t b, uint32_t *w, float &t) {
+        const int64_t e = b + col;   // this lane's A row
+        if (e < n) {
+            const uint2 *cw = reinterpret_cast<const uint2 *>(codes + (e0 + e) * M);
+#pragma unroll
+            for (int j = 0; j < M / 8; ++j) {
+                const uint2 v = cw[j];
+                w[2 * j] = v.x;
+                w[2 * j + 1] = v.y;
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < M / 4; ++j) w[j] = 0;
+        }
+        t = (lane < 32 && b + lane < n) ? la.term[e0 + b + lane] : 0.f;
+    };
+    auto prefetch = [&](int item) {
+        if (item >= n_items) return;
+        nh = la.items[item];
+        nqid = -1;
+        if (qrow < nh.z) {
+            nqid = la.pair_q[nh.y + qrow];
+            const float4 *src = reinterpret_cast<const float4 *>(la.queries + (int64_t)nqid * D) + qsub * QV;
+#pragma unroll
+            for (int v = 0; v < QV; ++v) nqv[v] = src[v];
+            if (qsub == 0) {
+                nqc = la.pair_qc[nh.y + qrow];
+                ntau = la.tau[nqid];
+            }
+        }
+        ne0 = la.list_start[nh.x];
+        nn = la.list_len[nh.x];
+        load_codes(ne0, nn, (int64_t)wave * 32, nw, nt);
+    };
+    int item = s_next;
+    prefetch(item);
 
     for (;;) {
-        if (tid == 0) s_item = atomicAdd(la.item_counter, 1);
-        __syncthreads();
-        const int item = s_item;
         if (item >= n_items) break;
-        const int4 it = la.items[item];
-        const int64_t l = it.x;
-        const int p0 = it.y, pc = it.z;
-        if (tid < LQ) {
-            if (tid < pc) {
-                const int qid = la.pair_q[p0 + tid];
-                s_qid[tid] = qid;
-                s_qc[tid] = la.pair_qc[p0 + tid];
-                s_tau[tid] = la.tau[qid];
-            } else {
-                s_qid[tid] = -1;
-                s_qc[tid] = 0.f;
-                s_tau[tid] = -__builtin_inff();
-            }
-            s_cnt[tid] = 0;
-        }
-        __syncthreads();
-        // the item's queries; rows >= pc are zero so that their (discarded)
-        // columns stay finite
-        for (int i = tid; i < LQ * D; i += 64 * LNW) {
-            const int t = i / D, j = i - t * D;
-            const float v = t < pc ? la.queries[(int64_t)s_qid[t] * D + j] : 0.f;
-            const __bf16 h = (__bf16)v;
-            sQh[t * LD + j] = h;
-            if (X3) sQl[t * LD + j] = (__bf16)(v - (float)h);
-        }
-        __syncthreads();
-        const float qc_r = s_qc[col], tau_r = s_tau[col];
-        const int64_t e0 = la.list_start[l], n = la.list_len[l];
-        for (int64_t base = (int64_t)wave * 32; base < n; base += 32 * LNW) {
-            const int64_t e = base + col;   // this lane's A row
-            uint32_t w[M / 4];
-            if (e < n) {
-                const uint2 *cw = reinterpret_cast<const uint2 *>(codes + (e0 + e) * M);
+        const int pc = nh.z;
+        const int64_t e0 = ne0, n = nn;
+        // the item's queries into LDS (rows >= pc zero, so that their
+        // discarded columns stay finite)
+        {
+            __bf16 *dh = sQh + qrow * LD + qsub * (4 * QV);
+            __bf16 *dl = sQl + qrow * LD + qsub * (4 * QV);
 #pragma unroll
-                for (int j = 0; j < M / 8; ++j) {
-                    const uint2 v = cw[j];
-                    w[2 * j] = v.x;
-                    w[2 * j + 1] = v.y;
+            for (int v = 0; v < QV; ++v) {
+                const float4 x = qrow < pc ? nqv[v] : make_float4(0.f, 0.f, 0.f, 0.f);
+                bf16x4 h;
+                h[0] = (__bf16)x.x;
+                h[1] = (__bf16)x.y;
+                h[2] = (__bf16)x.z;
+                h[3] = (__bf16)x.w;
+                *reinterpret_cast<bf16x4 *>(dh + 4 * v) = h;
+                if (X3) {
+                    bf16x4 g;
+                    g[0] = (__bf16)(x.x - (float)h[0]);
+                    g[1] = (__bf16)(x.y - (float)h[1]);
+                    g[2] = (__bf16)(x.z - (float)h[2]);
+                    g[3] = (__bf16)(x.w - (float)h[3]);
+                    *reinterpret_cast<bf16x4 *>(dl + 4 * v) = g;
                 }
-                if (M % 8) w[M / 4 - 1] = reinterpret_cast<const uint32_t *>(codes + (e0 + e) * M)[M / 4 - 1];
-            } else {
-#pragma unroll
-                for (int j = 0; j < M / 4; ++j) w[j] = 0;
             }
-            if (lane < 32) s_term[lane] = base + lane < n ? la.term[e0 + base + lane] : 0.f;
+            if (qsub == 0) {
+                s_qid[qrow] = qrow < pc ? nqid : -1;
+                s_qc[qrow] = qrow < pc ? nqc : 0.f;
+                s_tau[qrow] = qrow < pc ? ntau : -__builtin_inff();
+                s_cnt[qrow] = 0;
+            }
+        }
+        uint32_t wn[M / 4];
+#pragma unroll
+        for (int j = 0; j < M / 4; ++j) wn[j] = nw[j];
+        float tn = nt;
+        if (tid == 0) s_next = atomicAdd(la.item_counter, 1);
+        __syncthreads();
+        const int next_item = s_next;
+        prefetch(next_item);   // in flight during this item
+        const float qc_r = s_qc[col], tau_r = s_tau[col];
+        for (int64_t base = (int64_t)wave * 32; base < n; base += 32 * LNW) {
+            uint32_t w[M / 4];
+#pragma unroll
+            for (int j = 0; j < M / 4; ++j) w[j] = wn[j];
+            const float tc = tn;
+            load_codes(e0, n, base + 32 * LNW, wn, tn);
+            if (lane < 32) s_term[lane] = tc;
             f32x16 acc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = 0.f;
@@ -298,20 +359,21 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         }
         __syncthreads();
         for (int t = wave; t < pc; t += LNW) {
-            int nt = s_cnt[t];
-            nt = nt < LSLOTS ? nt : LSLOTS;
-            if (nt == 0) continue;
+            int ntk = s_cnt[t];
+            ntk = ntk < LSLOTS ? ntk : LSLOTS;
+            if (ntk == 0) continue;
             const int64_t qid = s_qid[t];
             int g0 = 0;
-            if (lane == 0) g0 = atomicAdd(&la.cnt[qid], nt);
+            if (lane == 0) g0 = atomicAdd(&la.cnt[qid], ntk);
             g0 = __shfl(g0, 0);
-            for (int j = lane; j < nt; j += 64)
+            for (int j = lane; j < ntk; j += 64)
                 if (g0 + j < la.cap) {
                     la.cand_key[qid * la.cap + g0 + j] = s_ck[t * LSLOTS + j];
                     la.cand_pos[qid * la.cap + g0 + j] = s_cp[t * LSLOTS + j];
                 }
         }
-        __syncthreads();
+        __syncthreads();   // the next item rewrites the query tile and the slots
+        item = next_item;
     }
 }
 
@@ -341,7 +403,9 @@ __global__ void __launch_bounds__(256) k_select_candidates(const float *__restri
     const int selk = kq * R;
     const int c = cnt[q];
     int64_t *orow = out_pos + q * out_stride;
-    if (c > cap || (allow_fallback && qtot[q] < kq)) {
+    // overflow, too few candidates for the selection while the probed lists
+    // hold enough, or the brute-force fallback: the table scan redoes it
+    if (c > cap || (c < selk && qtot[q] > c) || (allow_fallback && qtot[q] < kq)) {
         if (tid == 0) ovf[atomicAdd(ovf_cnt, 1)] = (int32_t)q;
         return;
     }
@@ -441,8 +505,8 @@ void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t
 }
 
 void launch_list_thresholds(const ScanArgs &a, int64_t nq, const float *pre_key, const int32_t *pre_cnt,
-                            int pre_stride, const double *probe_rank, const float *ymax_tmax, int x3, float *tau,
-                            hipStream_t st) {
+                            int pre_stride, int pre_lists, const int64_t *qtot, const double *probe_rank,
+                            const float *ymax_tmax, int x3, float *tau, hipStream_t st) {
     if (nq == 0) return;
     // 2 |<q, y>_MFMA - <q, y>| <= coef |q| |y|. With eps the BF16 rounding
     // (2^-9 relative to nearest, 2^-8 if truncated): products of rounded
@@ -451,7 +515,7 @@ void launch_list_thresholds(const ScanArgs &a, int64_t nq, const float *pre_key,
     // take the truncating eps and round up.
     const float coef = x3 ? 2.5e-4f : 1.6e-2f;
     hipLaunchKernelGGL(dev::k_list_thresholds, dim3((unsigned)ceil_div(nq, 256)), dim3(256), 0, st, a, nq, pre_key,
-                       pre_cnt, pre_stride, probe_rank, ymax_tmax, coef, tau);
+                       pre_cnt, pre_stride, pre_lists, qtot, probe_rank, ymax_tmax, coef, tau);
     HANN_LAUNCH_CHECK();
 }
 
