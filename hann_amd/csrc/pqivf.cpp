@@ -18,6 +18,7 @@
 #include <cmath>
 #include <cstring>
 #include <numeric>
+#include <thread>
 
 #include "batch.h"
 #include "ivf.h"
@@ -134,10 +135,10 @@ int64_t PQIVF::nearest_centroid(const float *vec) {
     sync_device();
     scratch_f.resize(dim);
     HANN_HIP(hipMemcpyAsync(scratch_f.get(), vec, dim * sizeof(float), hipMemcpyHostToDevice, stream));
-    DevBuf<int32_t> out(1);
-    launch_assign(scratch_f.get(), 1, dim, d_coarse.get(), (int)kc, (int)dim, out.get(), stream);
+    int32_t *out = mws.get<int32_t>(19, 1);
+    launch_assign(scratch_f.get(), 1, dim, d_coarse.get(), (int)kc, (int)dim, out, stream);
     int32_t c = 0;
-    HANN_HIP(hipMemcpyAsync(&c, out.get(), sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipMemcpyAsync(&c, out, sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HANN_HIP(hipStreamSynchronize(stream));
     return c;
 }
@@ -147,13 +148,13 @@ void PQIVF::encode_rows(const float *vecs, int64_t n, const int32_t *clusters, i
     if (n == 0) return;
     ensure_stream();
     sync_device();
-    DevBuf<float> dv((size_t)n * dim), resid((size_t)n * dim);
-    DevBuf<int32_t> dc((size_t)n), dcodes((size_t)n * m);
-    HANN_HIP(hipMemcpyAsync(dv.get(), vecs, (size_t)n * dim * sizeof(float), hipMemcpyHostToDevice, stream));
-    HANN_HIP(hipMemcpyAsync(dc.get(), clusters, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
-    launch_encode(dv.get(), n, (int)dim, d_coarse.get(), dc.get(), (int)m, d_cb.get(), (int)pmax, d_cb_len.get(),
-                  dcodes.get(), resid.get(), stream);
-    HANN_HIP(hipMemcpyAsync(codes_out, dcodes.get(), (size_t)n * m * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    float *dv = mws.get<float>(0, (size_t)n * dim), *resid = mws.get<float>(2, (size_t)n * dim);
+    int32_t *dc = mws.get<int32_t>(1, (size_t)n), *dcodes = mws.get<int32_t>(3, (size_t)n * m);
+    HANN_HIP(hipMemcpyAsync(dv, vecs, (size_t)n * dim * sizeof(float), hipMemcpyHostToDevice, stream));
+    HANN_HIP(hipMemcpyAsync(dc, clusters, (size_t)n * sizeof(int32_t), hipMemcpyHostToDevice, stream));
+    launch_encode(dv, n, (int)dim, d_coarse.get(), dc, (int)m, d_cb.get(), (int)pmax, d_cb_len.get(), dcodes, resid,
+                  stream);
+    HANN_HIP(hipMemcpyAsync(codes_out, dcodes, (size_t)n * m * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HANN_HIP(hipStreamSynchronize(stream));
 }
 
@@ -167,25 +168,24 @@ void PQIVF::assign_locked(const std::vector<int64_t> &ids, const std::vector<con
     sync_device();
     std::vector<float> flat((size_t)n * dim);
     for (int64_t i = 0; i < n; ++i) std::memcpy(&flat[(size_t)i * dim], vecs[i], dim * sizeof(float));
-    DevBuf<float> dv(flat.size());
-    DevBuf<int32_t> dc((size_t)n);
-    HANN_HIP(hipMemcpyAsync(dv.get(), flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, stream));
-    launch_assign(dv.get(), n, dim, d_coarse.get(), (int)kc, (int)dim, dc.get(), stream);
+    float *dv = mws.get<float>(0, flat.size());
+    int32_t *dc = mws.get<int32_t>(1, (size_t)n);
+    HANN_HIP(hipMemcpyAsync(dv, flat.data(), flat.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    launch_assign(dv, n, dim, d_coarse.get(), (int)kc, (int)dim, dc, stream);
     std::vector<int32_t> cl((size_t)n);
     std::vector<int32_t> codes;
     bool can_encode = have_codebooks;
     for (int64_t i = 0; i < m && can_encode; ++i)
         if (cb_len[i] == 0) can_encode = false;   // "failed to encode sub-vector": stored without codes
     if (can_encode) {
-        DevBuf<float> resid(flat.size());
-        DevBuf<int32_t> dcodes((size_t)n * m);
-        launch_encode(dv.get(), n, (int)dim, d_coarse.get(), dc.get(), (int)m, d_cb.get(), (int)pmax,
-                      d_cb_len.get(), dcodes.get(), resid.get(), stream);
+        float *resid = mws.get<float>(2, flat.size());
+        int32_t *dcodes = mws.get<int32_t>(3, (size_t)n * m);
+        launch_encode(dv, n, (int)dim, d_coarse.get(), dc, (int)m, d_cb.get(), (int)pmax, d_cb_len.get(), dcodes,
+                      resid, stream);
         codes.resize((size_t)n * m);
-        HANN_HIP(hipMemcpyAsync(codes.data(), dcodes.get(), codes.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                stream));
+        HANN_HIP(hipMemcpyAsync(codes.data(), dcodes, codes.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     }
-    HANN_HIP(hipMemcpyAsync(cl.data(), dc.get(), (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
+    HANN_HIP(hipMemcpyAsync(cl.data(), dc, (size_t)n * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
     HANN_HIP(hipStreamSynchronize(stream));
     for (int64_t i = 0; i < n; ++i) {
         touch_append(cl[i]);
@@ -316,31 +316,52 @@ void PQIVF::bulk_delete(const int64_t *ids, size_t n) {
         id_to_cluster.erase(it);
     }
     std::sort(by_list.begin(), by_list.end());
-    std::vector<int64_t> drop;
+    // runs of one list: (list, first, end) in by_list; the bookkeeping is
+    // serial, the per-list compactions are independent and run on threads
+    std::vector<std::array<size_t, 3>> runs;
     for (size_t i = 0; i < by_list.size();) {
         const int64_t c = by_list[i].first;
-        drop.clear();
-        for (; i < by_list.size() && by_list[i].first == c; ++i) {
-            drop.push_back(by_list[i].second);
-            touch_delete(c, by_list[i].second);
-        }
-        ListData &L = lists[c];
-        size_t w = 0;
-        const size_t len = L.ids.size();
-        for (size_t r = 0; r < len; ++r) {
-            if (std::binary_search(drop.begin(), drop.end(), L.ids[r])) continue;
-            if (w != r) {
-                L.ids[w] = L.ids[r];
-                std::memcpy(&L.vecs[w * dim], &L.vecs[r * dim], dim * sizeof(float));
-                std::memcpy(&L.codes[w * m], &L.codes[r * m], m * sizeof(int32_t));
-                L.has_codes[w] = L.has_codes[r];
+        const size_t b = i;
+        for (; i < by_list.size() && by_list[i].first == c; ++i) touch_delete(c, by_list[i].second);
+        runs.push_back({(size_t)c, b, i});
+    }
+    auto compact = [&](size_t r0, size_t r1) {
+        for (size_t k = r0; k < r1; ++k) {
+            ListData &L = lists[runs[k][0]];
+            const auto first = by_list.begin() + (ptrdiff_t)runs[k][1], last = by_list.begin() + (ptrdiff_t)runs[k][2];
+            auto dropped = [&](int64_t id) {
+                auto it = std::lower_bound(first, last, std::make_pair((int64_t)runs[k][0], id));
+                return it != last && it->second == id;
+            };
+            size_t w = 0;
+            const size_t len = L.ids.size();
+            for (size_t r = 0; r < len; ++r) {
+                if (dropped(L.ids[r])) continue;
+                if (w != r) {
+                    L.ids[w] = L.ids[r];
+                    std::memcpy(&L.vecs[w * dim], &L.vecs[r * dim], dim * sizeof(float));
+                    std::memcpy(&L.codes[w * m], &L.codes[r * m], m * sizeof(int32_t));
+                    L.has_codes[w] = L.has_codes[r];
+                }
+                ++w;
             }
-            ++w;
+            L.ids.resize(w);
+            L.vecs.resize(w * dim);
+            L.codes.resize(w * m);
+            L.has_codes.resize(w);
         }
-        L.ids.resize(w);
-        L.vecs.resize(w * dim);
-        L.codes.resize(w * m);
-        L.has_codes.resize(w);
+    };
+    const size_t nthreads = std::min<size_t>(runs.size() / 8, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    if (nthreads <= 1) {
+        compact(0, runs.size());
+    } else {
+        std::vector<std::thread> pool;
+        const size_t per = (runs.size() + nthreads - 1) / nthreads;
+        for (size_t t = 0; t < nthreads; ++t) {
+            const size_t r0 = t * per, r1 = std::min(runs.size(), r0 + per);
+            if (r0 < r1) pool.emplace_back(compact, r0, r1);
+        }
+        for (auto &th : pool) th.join();
     }
 }
 
@@ -371,6 +392,16 @@ void PQIVF::bulk_update(const int64_t *ids, const float *flat, const size_t *len
         delete_locked(ids[i]);
         add_locked(ids[i], flat + off[i], off[i + 1] - off[i]);
     }
+}
+
+void PQIVF::reserve_slack(ListData &L, int64_t len) const {
+    // appends after a build fill this room before any reallocation (which
+    // would copy the whole list) -- the host side of the device mirror's slack
+    const int64_t cap = len + std::max<int64_t>(8, len / 8);
+    L.ids.reserve(cap);
+    L.vecs.reserve(cap * dim);
+    L.codes.reserve(cap * m);
+    L.has_codes.reserve(cap);
 }
 
 void PQIVF::train() {
@@ -502,12 +533,7 @@ void PQIVF::train() {
     id_to_cluster.reserve((size_t)n);
     std::vector<int64_t> cnt((size_t)kc, 0);
     for (int64_t i = 0; i < n; ++i) cnt[A[i]]++;
-    for (int64_t c = 0; c < kc; ++c) {
-        lists[c].ids.reserve(cnt[c]);
-        lists[c].vecs.reserve(cnt[c] * dim);
-        lists[c].codes.reserve(cnt[c] * m);
-        lists[c].has_codes.reserve(cnt[c]);
-    }
+    for (int64_t c = 0; c < kc; ++c) reserve_slack(lists[c], cnt[c]);
     for (int64_t i = 0; i < n; ++i) {
         ListData &L = lists[A[i]];
         L.exists = true;
@@ -553,22 +579,20 @@ void PQIVF::touch_delete(int64_t l, int64_t id) {
     deleted_ids[l].push_back(id);
 }
 
-// Stable removal of deleted ids from lists on the device. Job j: list
-// slots [start[j], start[j] + len[j]), ids to drop del[doff[j] .. doff[j+1])
-// (sorted); kept entries go, in order, to temp rows [toff[j], ...).
-__global__ void __launch_bounds__(256) k_compact_lists(const int64_t *__restrict__ start, const int64_t *__restrict__ len,
-                                                       const int64_t *__restrict__ doff, const int64_t *__restrict__ del,
-                                                       const int64_t *__restrict__ toff, int m, int code_bytes, int d,
-                                                       const int64_t *__restrict__ ids, const uint8_t *__restrict__ codes,
-                                                       const uint8_t *__restrict__ has, const float *__restrict__ vecs,
-                                                       int64_t *__restrict__ t_ids, uint8_t *__restrict__ t_codes,
-                                                       uint8_t *__restrict__ t_has, float *__restrict__ t_vecs) {
+// Stable removal of deleted ids from lists on the device, in two steps.
+// k_compact_map: job j covers list slots [start[j], start[j] + len[j]) and
+// drops the ids del[doff[j] .. doff[j+1]) (sorted); its kept entries, in
+// order, get rows koff[j] .. of a packed map: smap = their slot, dmap = the
+// slot they move to (start[j] + rank among the kept).
+__global__ void __launch_bounds__(256) k_compact_map(const int64_t *__restrict__ start, const int64_t *__restrict__ len,
+                                                     const int64_t *__restrict__ doff, const int64_t *__restrict__ del,
+                                                     const int64_t *__restrict__ koff, const int64_t *__restrict__ ids,
+                                                     int64_t *__restrict__ smap, int64_t *__restrict__ dmap) {
     __shared__ int s_scan[256];
     const int j = blockIdx.x;
     const int64_t s0 = start[j], n = len[j];
     const int64_t *dl = del + doff[j];
     const int64_t nd = doff[j + 1] - doff[j];
-    const int cb = m * code_bytes;
     int64_t run = 0;
     for (int64_t c0 = 0; c0 < n; c0 += 256) {
         const int64_t e = c0 + threadIdx.x;
@@ -591,39 +615,40 @@ __global__ void __launch_bounds__(256) k_compact_lists(const int64_t *__restrict
             __syncthreads();
         }
         if (keep) {
-            const int64_t dst = toff[j] + run + s_scan[threadIdx.x] - 1;
-            const int64_t src = s0 + e;
-            t_ids[dst] = ids[src];
-            t_has[dst] = has[src];
-            for (int b = 0; b < cb; ++b) t_codes[dst * cb + b] = codes[src * cb + b];
-            for (int x = 0; x < d; ++x) t_vecs[dst * d + x] = vecs[src * d + x];
+            const int64_t rank = run + s_scan[threadIdx.x] - 1;
+            smap[koff[j] + rank] = s0 + e;
+            dmap[koff[j] + rank] = s0 + rank;
         }
         run += s_scan[255];
         __syncthreads();
     }
 }
 
-// copy job j's kept rows back to its list slots
-__global__ void k_copy_back(const int64_t *__restrict__ start, const int64_t *__restrict__ toff,
-                            const int64_t *__restrict__ newlen, int njobs, int m, int code_bytes, int d,
-                            const int64_t *__restrict__ t_ids, const uint8_t *__restrict__ t_codes,
-                            const uint8_t *__restrict__ t_has, const float *__restrict__ t_vecs,
-                            int64_t *__restrict__ ids, uint8_t *__restrict__ codes, uint8_t *__restrict__ has,
-                            float *__restrict__ vecs) {
-    const int j = blockIdx.y;
-    if (j >= njobs) return;
-    const int cb = m * code_bytes;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < newlen[j] * d;
-         t += (int64_t)gridDim.x * blockDim.x) {
+// Entries rows [0, n): from slot smap[i] (or row i) to slot dmap[i] (or
+// row i); vectors and codes move coalesced (a thread per element).
+__global__ void k_move_entries(int64_t n, int cb, int d, const int64_t *__restrict__ smap,
+                               const int64_t *__restrict__ dmap, const int64_t *__restrict__ ids,
+                               const uint8_t *__restrict__ codes, const uint8_t *__restrict__ has,
+                               const float *__restrict__ vecs, int64_t *__restrict__ o_ids, uint8_t *__restrict__ o_codes,
+                               uint8_t *__restrict__ o_has, float *__restrict__ o_vecs) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int64_t t = t0; t < n * d; t += stride) {
         const int64_t r = t / d;
-        const int x = (int)(t % d);
-        const int64_t src = toff[j] + r, dst = start[j] + r;
-        vecs[dst * d + x] = t_vecs[src * d + x];
-        if (x == 0) {
-            ids[dst] = t_ids[src];
-            has[dst] = t_has[src];
-            for (int b = 0; b < cb; ++b) codes[dst * cb + b] = t_codes[src * cb + b];
-        }
+        const int x = (int)(t - r * d);
+        const int64_t src = smap ? smap[r] : r, dst = dmap ? dmap[r] : r;
+        o_vecs[dst * d + x] = vecs[src * d + x];
+    }
+    for (int64_t t = t0; t < n * cb; t += stride) {
+        const int64_t r = t / cb;
+        const int x = (int)(t - r * cb);
+        const int64_t src = smap ? smap[r] : r, dst = dmap ? dmap[r] : r;
+        o_codes[dst * cb + x] = codes[src * cb + x];
+    }
+    for (int64_t r = t0; r < n; r += stride) {
+        const int64_t src = smap ? smap[r] : r, dst = dmap ? dmap[r] : r;
+        o_ids[dst] = ids[src];
+        o_has[dst] = has[src];
     }
 }
 
@@ -685,7 +710,7 @@ void PQIVF::sync_device() {
         }
         if (fits && !jobs.empty()) {
             const size_t J = jobs.size();
-            std::vector<int64_t> jstart(J), jlen(J), doff(J + 1, 0), toff(J + 1, 0), newlen(J), dels;
+            std::vector<int64_t> jstart(J), jlen(J), doff(J + 1, 0), koff(J + 1, 0), newlen(J), dels;
             for (size_t j = 0; j < J; ++j) {
                 const int64_t l = jobs[j];
                 jstart[j] = dev_start[l];
@@ -694,31 +719,37 @@ void PQIVF::sync_device() {
                 std::sort(dv.begin(), dv.end());
                 dels.insert(dels.end(), dv.begin(), dv.end());
                 doff[j + 1] = (int64_t)dels.size();
-                toff[j + 1] = toff[j] + jlen[j];
                 newlen[j] = (int64_t)lists[l].ids.size();
+                koff[j + 1] = koff[j] + newlen[j];
             }
-            const int64_t T = toff[J];
-            DevBuf<int64_t> d_js(J), d_jl(J), d_do(J + 1), d_del(dels.size() + 1), d_to(J + 1), d_nl(J), t_ids((size_t)T + 1);
-            DevBuf<uint8_t> t_codes((size_t)T * m * code_bytes + 1), t_has((size_t)T + 1);
-            DevBuf<float> t_vecs((size_t)T * dim + 1);
-            HANN_HIP(hipMemcpyAsync(d_js.get(), jstart.data(), J * 8, hipMemcpyHostToDevice, stream));
-            HANN_HIP(hipMemcpyAsync(d_jl.get(), jlen.data(), J * 8, hipMemcpyHostToDevice, stream));
-            HANN_HIP(hipMemcpyAsync(d_do.get(), doff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
+            const int64_t T = koff[J];   // kept entries over the jobs
+            int64_t *d_js = mws.get<int64_t>(4, J), *d_jl = mws.get<int64_t>(5, J), *d_do = mws.get<int64_t>(6, J + 1),
+                    *d_del = mws.get<int64_t>(7, dels.size() + 1), *d_ko = mws.get<int64_t>(8, J + 1),
+                    *smap = mws.get<int64_t>(9, (size_t)T + 1), *dmap = mws.get<int64_t>(20, (size_t)T + 1),
+                    *t_ids = mws.get<int64_t>(10, (size_t)T + 1);
+            uint8_t *t_codes = mws.get<uint8_t>(11, (size_t)T * m * code_bytes + 1),
+                    *t_has = mws.get<uint8_t>(12, (size_t)T + 1);
+            float *t_vecs = mws.get<float>(13, (size_t)T * dim + 1);
+            HANN_HIP(hipMemcpyAsync(d_js, jstart.data(), J * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_jl, jlen.data(), J * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_do, doff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
             if (!dels.empty())
-                HANN_HIP(hipMemcpyAsync(d_del.get(), dels.data(), dels.size() * 8, hipMemcpyHostToDevice, stream));
-            HANN_HIP(hipMemcpyAsync(d_to.get(), toff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
-            HANN_HIP(hipMemcpyAsync(d_nl.get(), newlen.data(), J * 8, hipMemcpyHostToDevice, stream));
-            hipLaunchKernelGGL(k_compact_lists, dim3((unsigned)J), dim3(256), 0, stream, d_js.get(), d_jl.get(),
-                               d_do.get(), d_del.get(), d_to.get(), (int)m, code_bytes, (int)dim, d_ids.get(),
-                               d_codes.get(), d_has_codes.get(), d_vecs.get(), t_ids.get(), t_codes.get(), t_has.get(),
-                               t_vecs.get());
+                HANN_HIP(hipMemcpyAsync(d_del, dels.data(), dels.size() * 8, hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipMemcpyAsync(d_ko, koff.data(), (J + 1) * 8, hipMemcpyHostToDevice, stream));
+            hipLaunchKernelGGL(k_compact_map, dim3((unsigned)J), dim3(256), 0, stream, d_js, d_jl, d_do, d_del, d_ko,
+                               d_ids.get(), smap, dmap);
             HANN_LAUNCH_CHECK();
-            for (size_t j0 = 0; j0 < J; j0 += 65535) {
-                const size_t jn = std::min<size_t>(65535, J - j0);
-                hipLaunchKernelGGL(k_copy_back, dim3(64, (unsigned)jn), dim3(256), 0, stream, d_js.get() + j0,
-                                   d_to.get() + j0, d_nl.get() + j0, (int)jn, (int)m, code_bytes, (int)dim, t_ids.get(),
-                                   t_codes.get(), t_has.get(), t_vecs.get(), d_ids.get(), d_codes.get(),
-                                   d_has_codes.get(), d_vecs.get());
+            if (T > 0) {
+                const int cbw = (int)(m * code_bytes);
+                const unsigned g = (unsigned)std::min<int64_t>(65535, ceil_div(T * dim, 256));
+                // kept entries to packed rows, then back to their new slots
+                hipLaunchKernelGGL(k_move_entries, dim3(g), dim3(256), 0, stream, T, cbw, (int)dim, smap,
+                                   (const int64_t *)nullptr, d_ids.get(), d_codes.get(), d_has_codes.get(),
+                                   d_vecs.get(), t_ids, t_codes, t_has, t_vecs);
+                HANN_LAUNCH_CHECK();
+                hipLaunchKernelGGL(k_move_entries, dim3(g), dim3(256), 0, stream, T, cbw, (int)dim,
+                                   (const int64_t *)nullptr, dmap, t_ids, t_codes, t_has, t_vecs, d_ids.get(),
+                                   d_codes.get(), d_has_codes.get(), d_vecs.get());
                 HANN_LAUNCH_CHECK();
             }
             HANN_HIP(hipStreamSynchronize(stream));
@@ -755,17 +786,17 @@ void PQIVF::sync_device() {
             }
             touched_lists.clear();
             if (total > 0) {
-                DevBuf<int64_t> d_dst((size_t)total), d_i((size_t)total);
-                DevBuf<uint8_t> d_c(hcodes.size()), d_h((size_t)total);
-                DevBuf<float> d_v(hv.size());
-                HANN_HIP(hipMemcpyAsync(d_dst.get(), dst.data(), total * 8, hipMemcpyHostToDevice, stream));
-                HANN_HIP(hipMemcpyAsync(d_i.get(), hids.data(), total * 8, hipMemcpyHostToDevice, stream));
-                HANN_HIP(hipMemcpyAsync(d_c.get(), hcodes.data(), hcodes.size(), hipMemcpyHostToDevice, stream));
-                HANN_HIP(hipMemcpyAsync(d_h.get(), hhas.data(), total, hipMemcpyHostToDevice, stream));
-                HANN_HIP(hipMemcpyAsync(d_v.get(), hv.data(), hv.size() * 4, hipMemcpyHostToDevice, stream));
+                int64_t *d_dst = mws.get<int64_t>(14, (size_t)total), *d_i = mws.get<int64_t>(15, (size_t)total);
+                uint8_t *d_c = mws.get<uint8_t>(16, hcodes.size()), *d_h = mws.get<uint8_t>(17, (size_t)total);
+                float *d_v = mws.get<float>(18, hv.size());
+                HANN_HIP(hipMemcpyAsync(d_dst, dst.data(), total * 8, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_i, hids.data(), total * 8, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_c, hcodes.data(), hcodes.size(), hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_h, hhas.data(), total, hipMemcpyHostToDevice, stream));
+                HANN_HIP(hipMemcpyAsync(d_v, hv.data(), hv.size() * 4, hipMemcpyHostToDevice, stream));
                 hipLaunchKernelGGL(k_scatter_entries, dim3((unsigned)ceil_div(total * dim, 256)), dim3(256), 0, stream,
-                                   total, (int)m, code_bytes, (int)dim, d_dst.get(), d_i.get(), d_c.get(), d_h.get(),
-                                   d_v.get(), d_ids.get(), d_codes.get(), d_has_codes.get(), d_vecs.get());
+                                   total, (int)m, code_bytes, (int)dim, d_dst, d_i, d_c, d_h, d_v, d_ids.get(),
+                                   d_codes.get(), d_has_codes.get(), d_vecs.get());
                 HANN_LAUNCH_CHECK();
                 HANN_HIP(hipMemcpyAsync(d_list_len.get(), dev_len.data(), dev_len.size() * 8, hipMemcpyHostToDevice,
                                         stream));
@@ -1423,6 +1454,10 @@ void PQIVF::load(const void *buf, size_t len) {
         if (c < 0 || c >= dkc) fail(HANN_ERR_FORMAT, strf("serialized list %lld has no coarse centroid", (long long)c));
         ListData &x = L[c];
         x.exists = true;
+        x.ids.reserve(n + std::max<int64_t>(8, n / 8));
+        x.has_codes.reserve(n + std::max<int64_t>(8, n / 8));
+        x.codes.reserve((n + std::max<int64_t>(8, n / 8)) * dm);
+        x.vecs.reserve((n + std::max<int64_t>(8, n / 8)) * ddim);
         x.ids.resize(n);
         x.has_codes.resize(n);
         x.codes.resize(n * dm);

@@ -111,6 +111,7 @@ struct PQIVF {
     DevBuf<uint8_t> d_codes, d_has_codes;
     DevBuf<float> d_vecs, d_coarse, d_cb, scratch_f;
     DevBuf<unsigned long long> d_fb_counter;
+    Workspace mws;   // grow-only device scratch of the mutation paths (no allocation per call)
     // persistent scan: per-entry list terms, kept current by every sync
     DevBuf<float> d_term;
     DevBuf<int64_t> d_term_lists;
@@ -138,6 +139,7 @@ struct PQIVF {
     void update(int64_t id, const float *vec, size_t len);
     void bulk_update(const int64_t *ids, const float *flat, const size_t *lens, size_t n);
     void train();
+    void reserve_slack(ListData &L, int64_t len) const;
     void mark_dirty();           // the next sync rebuilds the whole device mirror
     void touch_append(int64_t l);   // entries are about to be appended to list l
     void touch_other(int64_t l);    // list l changed in any other way
