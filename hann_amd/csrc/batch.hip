@@ -13,17 +13,41 @@ namespace hann {
 // most the query's sample k-th rank, and those rows are ranked exactly.
 // Queries whose candidate list overflows are rerun on the exact path.
 void certified_topk(Context &c, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
-                    const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st) {
+                    const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st,
+                    const int32_t *k_q, BF16Base *cache) {
     Workspace &w = c.ws_bf16;
     const int dp = (int)ceil_div(dim, 32) * 32;
     const int metric = sqrt_out ? HANN_METRIC_EUCLIDEAN : HANN_METRIC_SQEUCLIDEAN;
-    // BF16 copies, norms, max norm and exactness flag
-    __bf16 *xb = w.get<__bf16>(0, (size_t)nb * dp);
-    float *xn = w.get<float>(1, (size_t)nb);
-    float *xf = w.get<float>(2, (size_t)nb);
-    unsigned *stat = w.get<unsigned>(3, 4);   // [0] max norm bits, [1] inexact flag
-    HANN_HIP(hipMemsetAsync(stat, 0, 4 * sizeof(unsigned), st));
-    launch_to_bf16(base, nb, dim, dp, xb, xn, xf, stat, reinterpret_cast<int *>(stat + 1), st);
+    // BF16 copies, norms, max norm and exactness flag of the base (cached)
+    // and of the queries
+    __bf16 *xb;
+    float *xn, *xf;
+    unsigned *stat = w.get<unsigned>(3, 4);   // [0] max norm bits, [1] inexact flag, [2] queries' max norm
+    if (cache) {
+        if (!cache->ready) {
+            cache->xb.resize((size_t)nb * dp);
+            cache->xn.resize((size_t)nb);
+            cache->xf.resize((size_t)nb);
+            cache->stat.resize(4);
+            HANN_HIP(hipMemsetAsync(cache->stat.get(), 0, 4 * sizeof(unsigned), st));
+            launch_to_bf16(base, nb, dim, dp, cache->xb.get(), cache->xn.get(), cache->xf.get(), cache->stat.get(),
+                           reinterpret_cast<int *>(cache->stat.get() + 1), st);
+            // other streams may read the cache once this call returns: it
+            // synchronizes its stream before returning (the overflow check)
+            cache->ready = true;
+        }
+        xb = cache->xb.get();
+        xn = cache->xn.get();
+        xf = cache->xf.get();
+        HANN_HIP(hipMemcpyAsync(stat, cache->stat.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+    } else {
+        xb = w.get<__bf16>(0, (size_t)nb * dp);
+        xn = w.get<float>(1, (size_t)nb);
+        xf = w.get<float>(2, (size_t)nb);
+        HANN_HIP(hipMemsetAsync(stat, 0, 4 * sizeof(unsigned), st));
+        launch_to_bf16(base, nb, dim, dp, xb, xn, xf, stat, reinterpret_cast<int *>(stat + 1), st);
+    }
+    (void)xf;
     __bf16 *qb = w.get<__bf16>(4, (size_t)nq * dp);
     float *qn = w.get<float>(5, (size_t)nq);
     float *qf = w.get<float>(6, (size_t)nq);
@@ -54,8 +78,8 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
             const int64_t qn_ = std::min(qchunk, nq - q0);
             launch_mfma_dense(qb + q0 * dp, qn_, sb, ns, dp, qn + q0, sn, dense, st);
-            launch_sample_thresholds(dense, qn_, ns, k, qf + q0, stat, reinterpret_cast<int *>(stat + 1), dim,
-                                     thr + q0, st);
+            launch_sample_thresholds(dense, qn_, ns, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
+                                     reinterpret_cast<int *>(stat + 1), dim, thr + q0, st);
         }
         // the host copy of pick must outlive its transfer
         HANN_HIP(hipStreamSynchronize(st));
@@ -73,8 +97,8 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
         const int64_t qn_ = std::min(chunk, nq - q0);
         launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cap, st);
-        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, cnt + q0, cand, cap, k, sqrt_out, out_ids + q0 * k,
-                               out_dist + q0 * k, st);
+        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, cnt + q0, cand, cap, k, k_q ? k_q + q0 : nullptr,
+                               sqrt_out, out_ids + q0 * k, out_dist + q0 * k, st);
     }
     // overflowed queries (and queries with fewer than k rows) take the exact path
     std::vector<int32_t> hc((size_t)nq);

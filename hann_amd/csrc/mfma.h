@@ -29,11 +29,13 @@ void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t 
                         float *out_n, hipStream_t st);
 // filter thresholds from the k-th smallest approximate distance of each
 // query's ns sample rows (approx[q * ns + j])
-void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const float *qnorm_f,
-                              const unsigned *max_norm, const int *inexact, int d, float *thr, hipStream_t st);
+// (k_q: each query's own k, clamped to [1, k], or null)
+void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const int32_t *k_q,
+                              const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, float *thr,
+                              hipStream_t st);
 // exact reference-order ranking of the candidates (queries with cnt <= cap)
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
-                            const int32_t *cand_row, int cap, int k, int sqrt_out, int64_t *out_ids,
-                            double *out_dist, hipStream_t st);
+                            const int32_t *cand_row, int cap, int k, const int32_t *k_q, int sqrt_out,
+                            int64_t *out_ids, double *out_dist, hipStream_t st);
 
 }  // namespace hann

@@ -224,7 +224,8 @@ __device__ __forceinline__ float key_float(unsigned k) {
 // (the bounds of k_thresholds, read the other way), which bounds the query's
 // true k-th exact rank tau; the filter threshold is then formed from tau as
 // in k_thresholds. One workgroup of 256 threads per query.
-__global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t ns, int k,
+__global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t ns, int kmax,
+                                                           const int32_t *__restrict__ k_q,
                                                            const float *__restrict__ qnorm_f,
                                                            const unsigned *__restrict__ max_norm,
                                                            const int *__restrict__ inexact, int d,
@@ -234,6 +235,8 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
     __shared__ int s_k;
     const int tid = threadIdx.x, lane = tid % 64;
     const int64_t q = blockIdx.x;
+    int k = k_q ? k_q[q] : kmax;
+    k = k < 1 ? 1 : (k > kmax ? kmax : k);
     if (k > ns) {
         if (tid == 0) thr[q] = __builtin_inff();
         return;
@@ -306,8 +309,8 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
                                                         const float *__restrict__ base,
                                                         const int32_t *__restrict__ cnt,
                                                         const int32_t *__restrict__ cand_row, int cap, int k,
-                                                        int ccap, int sqrt_out, int64_t *__restrict__ out_ids,
-                                                        double *__restrict__ out_dist) {
+                                                        const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
+                                                        int64_t *__restrict__ out_ids, double *__restrict__ out_dist) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     float *s_keys = reinterpret_cast<float *>(s_raw);
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + ccap);
@@ -315,8 +318,10 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
     const int64_t q = blockIdx.x;
     const int n = cnt[q];
     if (n > cap) return;
+    int kq = k_q ? k_q[q] : k;
+    kq = kq < 1 ? 1 : (kq > k ? k : kq);
     WaveTopK sel;
-    sel.init(s_keys, s_ids, k, ccap);
+    sel.init(s_keys, s_ids, kq, ccap);
     const float *qv = queries + q * d;
     for (int b = 0; b < n; b += kWave) {
         const int t = b + lane;
@@ -329,7 +334,8 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
         }
         sel.push(key, row, valid);
     }
-    const int got = sel.finish();
+    int got = sel.finish();
+    got = got < kq ? got : kq;
     for (int t = lane; t < k; t += kWave) {
         double dd = t < got ? (double)sel.keys[t] : __builtin_inf();
         if (sqrt_out && t < got) dd = __builtin_sqrt(dd);
@@ -413,17 +419,18 @@ void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t 
     HANN_LAUNCH_CHECK();
 }
 
-void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const float *qnorm_f,
-                              const unsigned *max_norm, const int *inexact, int d, float *thr, hipStream_t st) {
+void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const int32_t *k_q,
+                              const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, float *thr,
+                              hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_sample_thresholds, dim3((unsigned)nq), dim3(256), 0, st, approx, ns, k, qnorm_f,
+    hipLaunchKernelGGL(dev::k_sample_thresholds, dim3((unsigned)nq), dim3(256), 0, st, approx, ns, k, k_q, qnorm_f,
                        max_norm, inexact, d, thr);
     HANN_LAUNCH_CHECK();
 }
 
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
-                            const int32_t *cand_row, int cap, int k, int sqrt_out, int64_t *out_ids,
-                            double *out_dist, hipStream_t st) {
+                            const int32_t *cand_row, int cap, int k, const int32_t *k_q, int sqrt_out,
+                            int64_t *out_ids, double *out_dist, hipStream_t st) {
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = (size_t)ccap * 12 + 16;
@@ -432,7 +439,8 @@ void launch_rank_candidates(const float *queries, int64_t nq, int d, const float
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d, base,
-                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, k, ccap, sqrt_out, out_ids + q0 * k,
+                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, k, k_q ? k_q + q0 : nullptr, ccap, sqrt_out,
+                           out_ids + q0 * k,
                            out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
     }

@@ -858,6 +858,7 @@ void PQIVF::sync_device() {
     up(d_cb, codebooks);
     up(d_cb_len, cb_len);
     (void)s;
+    coarse_bf16.ready = false;
     d_term.resize((size_t)N + 1);
     refresh_terms(nullptr);
     HANN_HIP(hipStreamSynchronize(stream));
@@ -958,7 +959,8 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         probe_rank = w.get<double>(6, (size_t)nq * np);
         if (p.coarse_precision == HANN_PREC_BF16)
             // identical probes and ranks from the certified BF16 pass
-            certified_topk(coarse_ctx, dq, nq, d_coarse.get(), kc, d, np, nullptr, 0, probes, probe_rank, st);
+            certified_topk(coarse_ctx, dq, nq, d_coarse.get(), kc, d, np, nullptr, 0, probes, probe_rank, st, dnp,
+                           &coarse_bf16);
         else if (p.coarse_precision == HANN_PREC_FP32)
             launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, probe_rank, w, st);
         else

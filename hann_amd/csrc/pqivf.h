@@ -9,6 +9,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "batch.h"
 #include "ivf.h"
 #include "kmeans.h"
 #include "runtime.h"
@@ -117,8 +118,10 @@ struct PQIVF {
     DevBuf<int64_t> d_term_lists;
     DevBuf<unsigned> d_ymax_tmax;   // max |y_e|, max |t_e| (float bits) over the entries
     void refresh_terms(const std::vector<int64_t> *lists);   // null: every list
-    // the coarse step's BF16 candidate pass (used under the index lock)
+    // the coarse step's BF16 candidate pass (used under the index lock) and
+    // the centroids' BF16 copy (rebuilt after a full sync)
     Context coarse_ctx;
+    BF16Base coarse_bf16;
     std::map<uint64_t, int> tuned_wpq;   // persistent scan waves per query; key: (cap, maxp bucket)
     int num_cus = 0;
     std::string last_tune;
