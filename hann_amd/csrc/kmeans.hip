@@ -608,4 +608,132 @@ void launch_gather_rows(const float *src, int d, const int64_t *idx, int64_t n, 
     HANN_LAUNCH_CHECK();
 }
 
+
+// ---------------------------------------------------------- k-means++ --
+namespace dev {
+constexpr int PP_BLOCK = 256;
+
+// D[i] = min(D[i], |x_i - c|^2) (first: D[i] = |x_i - c|^2); per-block sum
+// of D in FP64 (fixed tree order)
+template <bool FIRST>
+__global__ void __launch_bounds__(PP_BLOCK) k_pp_update(const float *__restrict__ data, int64_t n, int64_t row_stride,
+                                                        int d, const float *__restrict__ c, float *__restrict__ D,
+                                                        double *__restrict__ bsum) {
+    __shared__ double s_sum[PP_BLOCK];
+    const int64_t i = (int64_t)blockIdx.x * PP_BLOCK + threadIdx.x;
+    double v = 0.0;
+    if (i < n) {
+        const float e = sane_key(sqdiff_exact(PtrLoad{data + i * row_stride}, PtrLoad{c}, d));
+        const float dv = FIRST ? e : (e < D[i] ? e : D[i]);
+        D[i] = dv;
+        v = dv;
+    }
+    s_sum[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = PP_BLOCK / 2; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) s_sum[threadIdx.x] += s_sum[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = s_sum[0];
+}
+
+// one workgroup: the row whose D-prefix first exceeds u * total; copy it to
+// the next center slot
+__global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsum, int64_t nblocks,
+                                                  const float *__restrict__ D, int64_t n, const double *__restrict__ u,
+                                                  const float *__restrict__ data, int64_t row_stride, int d,
+                                                  float *__restrict__ cent_out) {
+    __shared__ double s_tot[1024];
+    __shared__ int64_t s_pick;
+    const int t = threadIdx.x;
+    // per-thread chunk sums in block order, then an inclusive scan over threads
+    const int64_t per = (nblocks + 1023) / 1024;
+    const int64_t b0 = t * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    double mine = 0.0;
+    for (int64_t b = b0; b < b1; ++b) mine += bsum[b];
+    s_tot[t] = mine;
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        const double v = t >= o ? s_tot[t - o] : 0.0;
+        __syncthreads();
+        s_tot[t] += v;
+        __syncthreads();
+    }
+    const double total = s_tot[1023];
+    if (t == 0) s_pick = -1;
+    __syncthreads();
+    if (!(total > 0.0)) {
+        if (t == 0) {
+            int64_t r = (int64_t)(*u * (double)n);
+            s_pick = r < n ? r : n - 1;
+        }
+    } else {
+        const double target = *u * total;
+        // [previous inclusive sum, own inclusive sum): disjoint intervals,
+        // so exactly one thread holds the target
+        const double excl = t > 0 ? s_tot[t - 1] : 0.0;
+        if (excl <= target && target < s_tot[t]) {
+            // this thread's chunk holds the crossing: walk its blocks, then the rows
+            double run = excl;
+            int64_t pick = -1, last_pos = -1;
+            for (int64_t b = b0; b < b1 && pick < 0; ++b) {
+                if (run + bsum[b] <= target) {
+                    run += bsum[b];
+                    continue;
+                }
+                const int64_t r1 = (b + 1) * PP_BLOCK < n ? (b + 1) * PP_BLOCK : n;
+                for (int64_t r = b * PP_BLOCK; r < r1; ++r) {
+                    if (D[r] > 0.f) last_pos = r;
+                    run += (double)D[r];
+                    if (run > target && D[r] > 0.f) {
+                        pick = r;
+                        break;
+                    }
+                }
+            }
+            s_pick = pick >= 0 ? pick : last_pos;
+        }
+    }
+    __syncthreads();
+    if (s_pick < 0 && t == 0) {
+        // rounding left the crossing unresolved: the last row with D > 0
+        for (int64_t r = n - 1; r >= 0; --r)
+            if (D[r] > 0.f) {
+                s_pick = r;
+                break;
+            }
+        if (s_pick < 0) s_pick = 0;
+    }
+    __syncthreads();
+    const int64_t p = s_pick;
+    for (int e = t; e < d; e += 1024) cent_out[e] = data[p * row_stride + e];
+}
+
+}  // namespace dev
+
+void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, int k, int64_t first,
+                     const double *u_dev, float *cents, hipStream_t st) {
+    using dev::PP_BLOCK;
+    if (k <= 0 || n <= 0) return;
+    const int64_t nb = ceil_div(n, (int64_t)PP_BLOCK);
+    DevBuf<float> D((size_t)n);
+    DevBuf<double> bsum((size_t)nb);
+    // the first center
+    HANN_HIP(hipMemcpy2DAsync(cents, d * sizeof(float), data + first * row_stride, d * sizeof(float),
+                              d * sizeof(float), 1, hipMemcpyDeviceToDevice, st));
+    for (int j = 1; j < k; ++j) {
+        if (j == 1)
+            hipLaunchKernelGGL(dev::k_pp_update<true>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride, d,
+                               cents, D.get(), bsum.get());
+        else
+            hipLaunchKernelGGL(dev::k_pp_update<false>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride, d,
+                               cents + (int64_t)(j - 1) * d, D.get(), bsum.get());
+        HANN_LAUNCH_CHECK();
+        hipLaunchKernelGGL(dev::k_pp_pick, dim3(1), dim3(1024), 0, st, bsum.get(), nb, D.get(), n, u_dev + (j - 1), data,
+                           row_stride, d, cents + (int64_t)j * d);
+        HANN_LAUNCH_CHECK();
+    }
+    HANN_HIP(hipStreamSynchronize(st));
+}
+
 }  // namespace hann

@@ -52,7 +52,7 @@ int hann_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uint
     if (n == 0) fail(HANN_ERR_EMPTY, "no data for k-means training");
     if (d == 0) fail(HANN_ERR_INVALID_ARG, "query must not be empty");
     if (k == 0) fail(HANN_ERR_INVALID_ARG, "k must be positive");
-    if (init != 0) fail(HANN_ERR_UNSUPPORTED, strf("k-means init %d is not supported", init));
+    if (init != 0 && init != 1) fail(HANN_ERR_UNSUPPORTED, strf("k-means init %d is not supported", init));
     if (precision != HANN_PREC_FP32 && precision != HANN_PREC_BF16)
         fail(HANN_ERR_UNSUPPORTED, strf("precision %d is not supported for k-means", precision));
     Context &c = global_ctx();
@@ -61,7 +61,8 @@ int hann_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uint
     std::vector<uint64_t> st{stream_seed(seed, 0, 0)};
     KMeansScratch sc;
     const int64_t kk =
-        run_kmeans(dd.get(), (int64_t)n, (int64_t)d, (int)d, (int)k, iters, 1, st, dc.get(), sc, c.stream, precision);
+        run_kmeans(dd.get(), (int64_t)n, (int64_t)d, (int)d, (int)k, iters, 1, st, dc.get(), sc, c.stream, precision,
+                   init);
     d2h(out_centroids, dc.get(), (size_t)kk * d * sizeof(float), c.stream);
     sync(c.stream);
     *k_out = (size_t)kk;

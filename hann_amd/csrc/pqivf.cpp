@@ -42,19 +42,35 @@ void assign_points(const float *data, int64_t n, const float *cents, int k, int 
 }
 
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
-                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st, int precision) {
+                   std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st, int precision,
+                   int init_mode) {
     const int64_t keff = n < k ? n : k;
     if (keff <= 0) return 0;
-    // seeds: the first keff entries of a random permutation (pqivf/index.go:804-810)
-    std::vector<int64_t> init((size_t)groups * keff);
-    std::vector<int64_t> perm;
-    for (int g = 0; g < groups; ++g) {
-        rng_perm(states[g], n, perm);
-        std::copy(perm.begin(), perm.begin() + keff, init.begin() + (size_t)g * keff);
+    if (init_mode == 1) {
+        // k-means++ (build extension): per group, a uniform first row and
+        // keff - 1 uniform draws for the D^2 sampling, from the group's stream
+        std::vector<double> u((size_t)keff);
+        DevBuf<double> du((size_t)keff);
+        for (int g = 0; g < groups; ++g) {
+            const int64_t first = (int64_t)rng_bounded(states[g], (uint64_t)n);
+            for (int64_t j = 0; j + 1 < keff; ++j) u[j] = (double)(rng_next(states[g]) >> 11) * (1.0 / 9007199254740992.0);
+            HANN_HIP(hipMemcpyAsync(du.get(), u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, st));
+            launch_kmeanspp(data + (int64_t)g * dsub, n, row_stride, dsub, (int)keff, first, du.get(),
+                            cents + (int64_t)g * keff * dsub, st);
+        }
+    } else {
+        // seeds: the first keff entries of a random permutation (pqivf/index.go:804-810)
+        std::vector<int64_t> init((size_t)groups * keff);
+        std::vector<int64_t> perm;
+        for (int g = 0; g < groups; ++g) {
+            rng_perm(states[g], n, perm);
+            std::copy(perm.begin(), perm.begin() + keff, init.begin() + (size_t)g * keff);
+        }
+        DevBuf<int64_t> dinit(init.size());
+        HANN_HIP(hipMemcpyAsync(dinit.get(), init.data(), init.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        launch_gather_group(data, row_stride, dsub, (int)keff, groups, dinit.get(), cents, st);
+        HANN_HIP(hipStreamSynchronize(st));
     }
-    DevBuf<int64_t> dinit(init.size());
-    HANN_HIP(hipMemcpyAsync(dinit.get(), init.data(), init.size() * sizeof(int64_t), hipMemcpyHostToDevice, st));
-    launch_gather_group(data, row_stride, dsub, (int)keff, groups, dinit.get(), cents, st);
     sc.reserve(n, (int)keff, groups);
     DevBuf<int32_t> assign((size_t)groups * n);
     std::vector<int64_t> offs((size_t)groups * (keff + 1));
@@ -459,7 +475,8 @@ void PQIVF::train() {
     DevBuf<float> dC((size_t)coarse_k * dim);
     std::vector<uint64_t> st0{stream_seed(seed, epoch, 0)};
     const int64_t new_kc =
-        run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream, train_precision);
+        run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream, train_precision,
+                   kmeans_init);
     dS.release();
 
     // final assignment of every vector; lists in ascending id order
@@ -500,7 +517,8 @@ void PQIVF::train() {
     std::vector<uint64_t> sts((size_t)m);
     for (int64_t i = 0; i < m; ++i) sts[i] = stream_seed(seed, epoch, 1 + (uint64_t)i);
     KMeansScratch sp;
-    const int64_t pk = run_kmeans(dR.get(), npq, dim, s, (int)pq_k, (int)iters, (int)m, sts, dCB.get(), sp, stream);
+    const int64_t pk = run_kmeans(dR.get(), npq, dim, s, (int)pq_k, (int)iters, (int)m, sts, dCB.get(), sp, stream,
+                                  HANN_PREC_FP32, kmeans_init);
     dR.release();
 
     // encode every vector (pqivf/index.go:554-601)
@@ -1542,6 +1560,8 @@ int hann_pqivf_new(int64_t dim, const hann_pqivf_options *opts, hann_pqivf **out
     x.fallback = ((o.explicit_mask >> 5) & 1) ? o.bruteforce_fallback != 0 : true;
     x.seed = o.seed;
     x.train_precision = o.train_precision;
+    if (o.kmeans_init != 0 && o.kmeans_init != 1)
+        fail(HANN_ERR_UNSUPPORTED, strf("k-means init %d is not supported", (int)o.kmeans_init));
     x.kmeans_init = o.kmeans_init;
     x.coarse_sample = o.train_sample;
     x.pq_sample = o.pq_train_sample;

@@ -67,7 +67,7 @@ struct PendingStore {
 // assignment, identical results
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
                    std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st,
-                   int precision = HANN_PREC_FP32);
+                   int precision = HANN_PREC_FP32, int init_mode = 0);
 // assignPoints in either precision (BF16 needs a contiguous [n][d] layout)
 void assign_points(const float *data, int64_t n, const float *cents, int k, int d, int precision,
                    const int32_t *hint, int32_t *out, hipStream_t st);

@@ -80,6 +80,13 @@ def WithPQTrainSample(n):
     return _opt("pq_train_sample", n)
 
 
+def WithKMeansInit(init):
+    """Build extension: 0 = permutation seeding (the reference), 1 = k-means++."""
+    def f(o):
+        o.kmeans_init = int(init)
+    return f
+
+
 def WithTrainPrecision(p):
     """_lib.PREC_BF16: certified BF16 MFMA assignment in Train (same results)."""
     return _opt("train_precision", p)
@@ -287,12 +294,13 @@ def AssignPoints(data, centroids, precision=_lib.PREC_FP32):
     return out
 
 
-def KMeans(data, k, iters, seed, precision=_lib.PREC_FP32):
+def KMeans(data, k, iters, seed, precision=_lib.PREC_FP32, init=0):
+    """runKMeans; init 0 = the reference's permutation seeding, 1 = k-means++."""
     data = np.ascontiguousarray(data, np.float32)
     n, d = data.shape
     out = np.zeros((max(min(k, n), 1), d), np.float32)
     kk = ctypes.c_size_t()
-    _lib.check(_lib.load().hann_kmeans(_lib.fptr(data), n, d, k, iters, seed, 0, precision, _lib.fptr(out),
+    _lib.check(_lib.load().hann_kmeans(_lib.fptr(data), n, d, k, iters, seed, init, precision, _lib.fptr(out),
                                        ctypes.byref(kk)))
     return out[:kk.value]
 

@@ -47,6 +47,8 @@ _SIGS = {
     "or_assign": (None, [_fp, _sz, _sz, _fp, _sz, _int, _i32p]),
     "or_kmeans_update": (None, [_fp, _sz, _sz, _i32p, _sz, _u64p, _fp]),
     "or_kmeans": (_sz, [_fp, _sz, _sz, _sz, _int, _u64p, _int, _fp]),
+    "or_kmeanspp_seed": (None, [_fp, _sz, _sz, _sz, _u64p, _fp]),
+    "or_kmeans_init": (_sz, [_fp, _sz, _sz, _sz, _int, _u64p, _int, _int, _fp]),
     "or_encode": (_int, [_fp, _fp, _sz, _sz, _fp, _szp, _sz, _i32p]),
     "or_train": (_int, [_fp, _sz, _sz, _sz, _sz, _sz, _int, _u64, _u64, _int, _fp, _szp, _i32p,
                         _fp, _szp, _i32p]),
@@ -130,6 +132,16 @@ def kmeans(data, k, iters, rng_state, nthreads=1):
     out = np.zeros((max(min(k, n), 1), d), np.float32)
     st = (ctypes.c_uint64 * 1)(rng_state)
     kk = lib().or_kmeans(data.ctypes.data_as(_fp), n, d, k, iters, st, nthreads, out.ctypes.data_as(_fp))
+    return out[:kk], st[0]
+
+
+def kmeans_init(data, k, iters, rng_state, init, nthreads=1):
+    """or_kmeans_init: init 0 (permutation, the reference) or 1 (k-means++)."""
+    data = f32(data)
+    n, d = data.shape
+    out = np.zeros((max(min(k, n), 1), d), np.float32)
+    st = (ctypes.c_uint64 * 1)(rng_state)
+    kk = lib().or_kmeans_init(data.ctypes.data_as(_fp), n, d, k, iters, st, init, nthreads, out.ctypes.data_as(_fp))
     return out[:kk], st[0]
 
 

@@ -360,6 +360,110 @@ size_t or_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uin
     return k;
 }
 
+/* k-means++ seeding (a build extension: the reference seeds with a
+ * permutation). A restatement of the device's order of operations, so that
+ * seeds compare bit for bit: D^2 in the reference FP32 order (NaN as +inf),
+ * per-256-row FP64 tree sums, 1024 sequential chunk sums, a Hillis-Steele
+ * inclusive scan over the chunks, and a walk of the chunk, blocks and rows
+ * that hold u * total. */
+static float or_sane(float v) { return v != v ? INFINITY : v; }
+
+void or_kmeanspp_seed(const float *data, size_t n, size_t d, size_t k, uint64_t *rng, float *out) {
+    if (n == 0 || k == 0) return;
+    const size_t first = (size_t)or_rng_bounded(rng, (uint64_t)n);
+    double *u = (double *)malloc(sizeof(double) * (k > 1 ? k - 1 : 1));
+    for (size_t j = 0; j + 1 < k; j++) u[j] = (double)(or_rng_next(rng) >> 11) * (1.0 / 9007199254740992.0);
+    memcpy(out, data + first * d, d * sizeof(float));
+    const size_t B = 256, nb = (n + B - 1) / B, per = (nb + 1023) / 1024;
+    float *D = (float *)malloc(sizeof(float) * n);
+    double *bsum = (double *)malloc(sizeof(double) * nb);
+    double s[1024], blk[256];
+    for (size_t j = 1; j < k; j++) {
+        const float *c = out + (j - 1) * d;
+        for (size_t i = 0; i < n; i++) {
+            const float e = or_sane(or_sqdiff(data + i * d, c, d));
+            D[i] = j == 1 ? e : (e < D[i] ? e : D[i]);
+        }
+        for (size_t b = 0; b < nb; b++) {
+            for (size_t t = 0; t < B; t++) blk[t] = b * B + t < n ? (double)D[b * B + t] : 0.0;
+            for (size_t o = B / 2; o > 0; o >>= 1)
+                for (size_t t = 0; t < o; t++) blk[t] += blk[t + o];
+            bsum[b] = blk[0];
+        }
+        for (size_t t = 0; t < 1024; t++) {
+            const size_t b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
+            double m = 0.0;
+            for (size_t b = b0; b < b1; b++) m += bsum[b];
+            s[t] = m;
+        }
+        for (size_t o = 1; o < 1024; o <<= 1) {
+            double v[1024];
+            for (size_t t = 0; t < 1024; t++) v[t] = t >= o ? s[t - o] : 0.0;
+            for (size_t t = 0; t < 1024; t++) s[t] += v[t];
+        }
+        const double total = s[1023];
+        int64_t pick = -1;
+        if (!(total > 0.0)) {
+            int64_t r = (int64_t)(u[j - 1] * (double)n);
+            pick = r < (int64_t)n ? r : (int64_t)n - 1;
+        } else {
+            const double target = u[j - 1] * total;
+            for (size_t t = 0; t < 1024 && pick < 0; t++) {
+                const double excl = t > 0 ? s[t - 1] : 0.0;
+                if (!(excl <= target && target < s[t])) continue;
+                const size_t b0 = t * per, b1 = b0 + per < nb ? b0 + per : nb;
+                double run = excl;
+                int64_t found = -1, last_pos = -1;
+                for (size_t b = b0; b < b1 && found < 0; b++) {
+                    if (run + bsum[b] <= target) {
+                        run += bsum[b];
+                        continue;
+                    }
+                    const size_t r1 = (b + 1) * B < n ? (b + 1) * B : n;
+                    for (size_t r = b * B; r < r1; r++) {
+                        if (D[r] > 0.f) last_pos = (int64_t)r;
+                        run += (double)D[r];
+                        if (run > target && D[r] > 0.f) {
+                            found = (int64_t)r;
+                            break;
+                        }
+                    }
+                }
+                pick = found >= 0 ? found : last_pos;
+                if (pick < 0) pick = -2;   /* the claiming chunk found no row */
+            }
+            if (pick < 0) {
+                pick = -1;
+                for (int64_t r = (int64_t)n - 1; r >= 0; r--)
+                    if (D[r] > 0.f) {
+                        pick = r;
+                        break;
+                    }
+                if (pick < 0) pick = 0;
+            }
+        }
+        memcpy(out + j * d, data + (size_t)pick * d, d * sizeof(float));
+    }
+    free(D);
+    free(bsum);
+    free(u);
+}
+
+size_t or_kmeans_init(const float *data, size_t n, size_t d, size_t k, int iters, uint64_t *rng, int init,
+                      int nthreads, float *out_cents) {
+    if (init == 0) return or_kmeans(data, n, d, k, iters, rng, nthreads, out_cents);
+    if (n == 0) return 0;
+    if (n < k) k = n;
+    or_kmeanspp_seed(data, n, d, k, rng, out_cents);
+    int32_t *assign = (int32_t *)malloc(sizeof(int32_t) * n);
+    for (int it = 0; it < iters; it++) {
+        or_assign(data, n, d, out_cents, k, nthreads, assign);
+        or_kmeans_update(data, n, d, assign, k, rng, out_cents);
+    }
+    free(assign);
+    return k;
+}
+
 /* -------------------------------------------------------------------- PQ */
 int or_encode(const float *vec, const float *centroid, size_t d, size_t m,
               const float *codebooks, const size_t *cb_len, size_t pmax, int32_t *codes) {

@@ -45,6 +45,11 @@ void or_kmeans_update(const float *data, size_t n, size_t d, const int32_t *assi
                       uint64_t *rng, float *cents /* in: old, out: new */);
 size_t or_kmeans(const float *data, size_t n, size_t d, size_t k, int iters, uint64_t *rng,
                  int nthreads, float *out_cents /* [min(k,n)][d] */);
+/* k-means++ seeding (build extension; device order of operations) and
+ * runKMeans with init 0 (the reference's permutation) or 1 (k-means++) */
+void or_kmeanspp_seed(const float *data, size_t n, size_t d, size_t k, uint64_t *rng, float *out);
+size_t or_kmeans_init(const float *data, size_t n, size_t d, size_t k, int iters, uint64_t *rng, int init,
+                      int nthreads, float *out_cents);
 
 /* PQ encode of one vector (pqivf/index.go:641-666). codebooks: [m][pmax][s],
  * cb_len[i] rows valid in subquantizer i. returns 0, or -1 on an empty codebook */

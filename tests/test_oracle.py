@@ -288,3 +288,48 @@ def test_train_lists_in_ascending_id_order(oracle):
     for c in range(len(ix.list_off) - 1):
         seg = ix.ids[ix.list_off[c]:ix.list_off[c + 1]]
         assert np.all(np.diff(seg) > 0)
+
+
+# ------------------------------------------- k-means++ (build extension) --
+def _pp_draws(oracle, state, n, k):
+    """The seeding's draws from one stream: the first row, then k-1 uniforms."""
+    import ctypes
+    st = (ctypes.c_uint64 * 1)(state)
+    lib = oracle.lib()
+    first = lib.or_rng_bounded(st, n)
+    u = [(lib.or_rng_next(st) >> 11) * (1.0 / 9007199254740992.0) for _ in range(k - 1)]
+    return first, u
+
+
+def test_kmeanspp_seed_follows_d2_sampling(oracle):
+    """Seeds are data rows: the first is the stream's bounded draw, each
+    next one the row where the running D^2 sum passes u * total (the rows'
+    D^2 values are spread far apart here, so rounding cannot move the pick)."""
+    rng = np.random.default_rng(4)
+    x = (rng.standard_normal((200, 3)) * np.array([100.0, 1.0, 0.01])).astype(np.float32)
+    k, state = 6, 12345
+    seeds, _ = oracle.kmeans_init(x, k, 0, state, init=1)
+    first, u = _pp_draws(oracle, state, x.shape[0], k)
+    assert np.array_equal(seeds[0], x[first])
+    chosen = [x[first].astype(np.float64)]
+    for j in range(1, k):
+        d2 = np.min([((x.astype(np.float64) - c) ** 2).sum(1) for c in chosen], axis=0)
+        cum = np.cumsum(d2)
+        r = int(np.searchsorted(cum, u[j - 1] * cum[-1], side="right"))
+        # the pick lies within one row of the exact crossing (float32 D^2)
+        rows = {max(r - 1, 0), r, min(r + 1, len(x) - 1)}
+        assert any(np.array_equal(seeds[j], x[i]) for i in rows), j
+        chosen.append(seeds[j].astype(np.float64))
+
+
+def test_kmeanspp_seeds_distinct_and_deterministic(oracle):
+    rng = np.random.default_rng(5)
+    cen = rng.uniform(-20, 20, (8, 16)).astype(np.float32)
+    x = (cen[rng.integers(0, 8, 3000)] + rng.standard_normal((3000, 16))).astype(np.float32)
+    a, _ = oracle.kmeans_init(x, 32, 0, 99, init=1)
+    b, _ = oracle.kmeans_init(x, 32, 0, 99, init=1)
+    assert np.array_equal(a, b)
+    assert len({r.tobytes() for r in a}) == 32
+    # every one of the 8 separated clusters gets a seed
+    near = np.argmin(((a[:, None, :] - cen[None]) ** 2).sum(-1), axis=1)
+    assert set(near.tolist()) == set(range(8))

@@ -731,3 +731,33 @@ def test_list_scan_after_mutations(hann):
     assert agree >= 0.99
     live = set(np.concatenate([np.asarray(v) for v in pq.InvertedIDs(idx).values()]).tolist())
     assert set(a[0][a[0] >= 0].tolist()) <= live
+
+
+# ------------------------------------------- k-means++ (build extension) --
+@pytest.mark.parametrize("n,d,k,iters", [(3000, 16, 32, 0), (3000, 16, 32, 3), (20000, 64, 100, 2),
+                                         (700, 8, 256, 2), (300, 4, 300, 1), (5000, 128, 64, 1)])
+def test_kmeanspp_matches_oracle(hann, oracle, n, d, k, iters):
+    """k-means++ seeding on the device equals the oracle's restatement of its
+    order of operations bit for bit, seeds and Lloyd iterations alike."""
+    rng = np.random.default_rng(n + d + k)
+    cen = rng.uniform(-10, 10, (max(k // 3, 1), d)).astype(np.float32)
+    x = (cen[rng.integers(0, len(cen), n)] + rng.standard_normal((n, d))).astype(np.float32)
+    got = P().KMeans(x, k, iters, 7 + n, init=1)
+    want, _ = oracle.kmeans_init(x, k, iters, oracle.stream_seed(7 + n, 0, 0), init=1, nthreads=8)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_train_with_kmeanspp(hann, oracle):
+    """WithKMeansInit(1): the coarse centroids are the oracle's k-means++ run
+    over the id-sorted training vectors; an unknown init is rejected."""
+    from hann_amd import synthetic
+    pq = P()
+    x, _ = synthetic.config1(n=8000, q=10)
+    idx = pq.New(64, pq.WithCoarseK(32), pq.WithNumSubquantizers(8), pq.WithPQK(64), pq.WithKMeansIters(4),
+                 pq.WithSeed(21), pq.WithKMeansInit(1))
+    idx.bulk_add_arrays(np.arange(8000), x)
+    idx.Train()
+    want, _ = oracle.kmeans_init(x, 32, 4, oracle.stream_seed(21, 0, 0), init=1, nthreads=8)
+    assert np.array_equal(pq.CoarseCentroids(idx).view(np.uint32), want.view(np.uint32))
+    with pytest.raises(Exception):
+        pq.New(64, pq.WithKMeansInit(7))
