@@ -173,18 +173,20 @@ struct BlockTopK {
         ti = INT64_MAX;
     }
 
-    // sort all slots (those at and above n padded with +inf), keep the best
-    // k; every thread of the block calls it
+    // sort the filled slots (padded with +inf up to a power of two), keep the
+    // best k; every thread of the block calls it
     __device__ void compact() {
         const int tid = threadIdx.x, nt = blockDim.x;
-        for (int t = n + tid; t < cap; t += nt) {
+        int span = 2;
+        while (span < n) span <<= 1;   // n <= cap, a power of two
+        for (int t = n + tid; t < span; t += nt) {
             keys[t] = __builtin_inff();
             ids[t] = INT64_MAX;
         }
         __syncthreads();
-        for (int size = 2; size <= cap; size <<= 1) {
+        for (int size = 2; size <= span; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int t = tid; t < (cap >> 1); t += nt) {
+                for (int t = tid; t < (span >> 1); t += nt) {
                     const int lo = 2 * t - (t & (stride - 1));
                     const int hi = lo + stride;
                     const bool up = (lo & size) == 0;
