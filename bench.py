@@ -36,6 +36,21 @@ if ROOT not in sys.path:
 
 METRIC = "QPS at recall@10≥0.95, 10M×96 PQIVF, chosen vs plain GPU; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
+# rocprofv3 counter summary of the table scan on the 10M x 96 workload: FETCH_SIZE
+# and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB copy kernel
+COUNTERS = os.path.join(ROOT, "profiles", "k_scan_block_pqivf_10m_x96_nlist16384_counters.txt")
+
+
+def profiled_traffic(kernel):
+    """HBM-side bytes per launch of `kernel` from the committed counter summary, or None."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                if line.startswith(f"traffic_bytes_per_launch {kernel}:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def log(*a):
@@ -411,8 +426,10 @@ def main():
     kern_ms = res["chosen"]["kernel_ms"]
     bytes_per_launch = dist_b / a.steps
     achieved = bytes_per_launch / (kern_ms * 1e-3) / 1e9
+    # traffic is the counter-measured figure for the 10M x 96 workload only
+    traffic = profiled_traffic(kernel_name[chosen_name]) if (a.n, a.dim, a.nlist) == (10_000_000, 96, 16384) else None
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kernel_name[chosen_name], "kernel_ms_per_launch": round(kern_ms, 4),
                 "scan_stage_ms_per_launch": round(res["chosen"]["stages"][1] / max(res["chosen"]["stages"][3], 1), 4),
                 "algorithmic_bytes_per_launch": int(bytes_per_launch),
