@@ -20,6 +20,8 @@
 #include "ivf.h"
 #include "scan_fused.h"
 
+#include <type_traits>
+
 namespace hann {
 namespace dev {
 
@@ -153,6 +155,13 @@ __device__ __forceinline__ float split_sum16(const uint16_t *codes, int64_t e, i
     return acc;
 }
 
+__device__ __forceinline__ int64_t uniform_i64(int64_t v) {
+    const uint64_t u = (uint64_t)v;
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(u >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 // One query's top-k shared by the whole workgroup: LDS keys/ids[cap], the
 // fill n and the threshold (tk, ti) are block-uniform. Rounds keep every
 // wave in step, so compaction can use block barriers.
@@ -205,13 +214,14 @@ struct BlockTopK {
         }
         if (n > k) n = k;
         if (n == k) {
-            tk = keys[k - 1];
-            ti = ids[k - 1];
+            // block-uniform: keep them in scalar registers
+            tk = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(keys[k - 1])));
+            ti = uniform_i64(ids[k - 1]);
         }
     }
 };
 
-// LDS: q[d] | S_q[m*P] (FP32) | keys[cap] | ids[cap] | per probe: first slot,
+// LDS: S_q[m*P] (FP32) | q[d] | keys[cap] | ids[cap] | per probe: first slot,
 // length, list, chunks, |q-c|^2
 // PC: the codebook stride P when known at compile time (256), else 0
 template <int W, typename CodeT, int M, int PC>
@@ -220,13 +230,15 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     __shared__ int s_item;
     __shared__ int64_t s_red[2][W];
     __shared__ int s_cnt[2][W];
-    const int tid = threadIdx.x, wave = tid / kWave, lane = tid % kWave;
+    const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = tid % kWave;
     const int d = a.d, m = a.m, s = a.s, P = PC ? PC : a.pmax_cb, MP = m * P;
     const int cap = a.cap;
-    float *s_q = reinterpret_cast<float *>(s_raw);
-    size_t off = ((size_t)d * 4 + 15) & ~(size_t)15;
-    float *s_S = reinterpret_cast<float *>(s_raw + off);
-    off += ((size_t)MP * 4 + 15) & ~(size_t)15;
+    // S_q first: its LDS address is then a compile-time constant, which the
+    // gathers fold into their instruction offsets
+    float *s_S = reinterpret_cast<float *>(s_raw);
+    size_t off = ((size_t)MP * 4 + 15) & ~(size_t)15;
+    float *s_q = reinterpret_cast<float *>(s_raw + off);
+    off += ((size_t)d * 4 + 15) & ~(size_t)15;
     float *s_keys = reinterpret_cast<float *>(s_raw + off);
     off += ((size_t)cap * 4 + 15) & ~(size_t)15;
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_raw + off);
@@ -396,53 +408,82 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
             sel.n += sum;
         };
         constexpr bool PIPE = sizeof(CodeT) == 1 && M > 0 && M % 8 == 0;
-        if (PIPE && !a.any_uncoded) {
-            // two chunks per wave per round (two independent gather chains,
-            // half the barriers), software-pipelined: the next round's codes,
-            // terms and ids are in flight while this round gathers from S_q
+        // two chunks per wave per round (two independent gather chains, half
+        // the barriers), software-pipelined: the next round's codes, terms and
+        // ids are in flight while this round gathers from S_q. SLOT_IDS: the
+        // ids kept are entry slots (for the rerank), not user ids.
+        auto pipe_rounds = [&](auto slot_ids) {
+            constexpr bool SLOT_IDS = decltype(slot_ids)::value;
             constexpr int NW8 = PIPE ? M / 8 : 1;
             const uint2 *cw = reinterpret_cast<const uint2 *>(a.codes);
             struct Slot {
                 uint2 w[NW8];
                 float t, qc;
-                int64_t e, id;
+                int64_t id;
                 bool v;
             };
-            Slot nx[2];
+            // the next chunk of the probe table's lists (LDS reads only)
+            auto walk_tab = [&]() {
+                while (li < nl && c >= nch) {
+                    c -= (int)nch;
+                    ++li;
+                    if (li < nl) {
+                        nch = s_nch[li];
+                        qc = s_qc[li];
+                        e0 = s_e0[li];
+                        len = s_len[li];
+                    }
+                }
+            };
+            // lanes past their list's end load slot 0 (it exists: some probed
+            // list has entries) and are marked invalid; any code byte indexes
+            // inside the block's LDS
             auto locate = [&](Slot &x) {
-                walk();
-                x.e = e0 + (int64_t)c * kWave + lane;
+                walk_tab();
+                const int64_t e = e0 + (int64_t)c * kWave + lane;
                 x.v = li < nl && (int64_t)c * kWave + lane < len;
                 x.qc = qc;
-                if (x.v) {
+                const int64_t es = x.v ? e : 0;
 #pragma unroll
-                    for (int j = 0; j < NW8; ++j) x.w[j] = cw[x.e * NW8 + j];
-                    x.t = f.term[x.e];
-                    x.id = a.rerank > 0 ? x.e : a.ids[x.e];
-                }
+                for (int j = 0; j < NW8; ++j) x.w[j] = cw[es * NW8 + j];
+                x.t = f.term[es];
+                if constexpr (SLOT_IDS)
+                    x.id = e;
+                else
+                    x.id = a.ids[es];
                 c += W;
             };
-            auto gather = [&](const Slot &x) {
-                float acc = x.qc + x.t;
+            // both chunks' lookups are issued before any is summed (two
+            // independent chains of M LDS reads in flight); the sums keep
+            // subquantizer order
+            auto gather2 = [&](const Slot &x0, const Slot &x1, float &k0, float &k1) {
+                constexpr int NS = NW8 * 8;
+                constexpr int GS = NS <= 24 ? NS : 8;   // lookups per chain in flight
+                float a0 = x0.qc + x0.t, a1 = x1.qc + x1.t;
 #pragma unroll
-                for (int j = 0; j < NW8; ++j) {
+                for (int g = 0; g < NS; g += GS) {
+                    float v0[GS], v1[GS];
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) acc = acc + s_S[(8 * j + b) * P + (int)((x.w[j].x >> (8 * b)) & 0xFFu)];
+                    for (int u = 0; u < GS; ++u) {
+                        const int i = g + u, sh = 8 * (i & 3);
+                        const uint32_t w0 = (i & 4) ? x0.w[i >> 3].y : x0.w[i >> 3].x;
+                        const uint32_t w1 = (i & 4) ? x1.w[i >> 3].y : x1.w[i >> 3].x;
+                        v0[u] = s_S[i * P + (int)((w0 >> sh) & 0xFFu)];
+                        v1[u] = s_S[i * P + (int)((w1 >> sh) & 0xFFu)];
+                    }
 #pragma unroll
-                    for (int b = 0; b < 4; ++b)
-                        acc = acc + s_S[(8 * j + 4 + b) * P + (int)((x.w[j].y >> (8 * b)) & 0xFFu)];
+                    for (int u = 0; u < GS; ++u) {
+                        a0 = a0 + v0[u];
+                        a1 = a1 + v1[u];
+                    }
                 }
-                return sane_key(acc);
+                k0 = sane_key(a0);
+                k1 = sane_key(a1);
             };
-            locate(nx[0]);
-            locate(nx[1]);
-            const int64_t rounds2 = (tot_c + 2 * W - 1) / (2 * W);
-            for (int64_t r = 0; r < rounds2; ++r) {
-                const Slot cu0 = nx[0], cu1 = nx[1];
-                locate(nx[0]);
-                locate(nx[1]);
-                const float k0 = cu0.v ? gather(cu0) : 0.f;
-                const float k1 = cu1.v ? gather(cu1) : 0.f;
+            // one round: score chunks cu0 and cu1 and append their survivors
+            auto round = [&](const Slot &cu0, const Slot &cu1, int64_t r) {
+                float k0, k1;
+                gather2(cu0, cu1, k0, k1);
                 const int64_t i0 = cu0.v ? cu0.id : 0, i1 = cu1.v ? cu1.id : 0;
                 // append both chunks' survivors (counts through LDS)
                 bool p0 = cu0.v && cand_less(k0, i0, sel.tk, sel.ti);
@@ -457,6 +498,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     base += w < wave ? v : 0;
                     sum += v;
                 }
+                sum = __builtin_amdgcn_readfirstlane(sum);
+                base = __builtin_amdgcn_readfirstlane(base);
                 if (sel.n + sum > sel.cap) {
                     sel.compact();   // cap >= k + 128 W: the round now fits
                     __syncthreads();
@@ -473,6 +516,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         base += w < wave ? v : 0;
                         sum += v;
                     }
+                    sum = __builtin_amdgcn_readfirstlane(sum);
+                    base = __builtin_amdgcn_readfirstlane(base);
                     __syncthreads();
                 }
                 const int below0 = (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m0 >> 32),
@@ -489,7 +534,32 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                     s_ids[pos] = i1;
                 }
                 sel.n += sum;
+            };
+            // ping-pong slot sets: a round's loads land in the set that the
+            // following round reads, with no register copies between them
+            // (a copy would wait for the loads it copies)
+            Slot a0, a1, b0, b1;
+            locate(a0);
+            locate(a1);
+            const int64_t rounds2 = (tot_c + 2 * W - 1) / (2 * W);
+            for (int64_t r = 0; r < rounds2; r += 2) {
+                locate(b0);
+                locate(b1);
+                round(a0, a1, r);
+                if (r + 1 >= rounds2) break;
+                locate(a0);
+                locate(a1);
+                round(b0, b1, r + 1);
             }
+        };
+        // the pipelined rounds need the probe table in LDS: their list walk
+        // then issues no global loads, and every round issues the same loads,
+        // so each wait covers only the loads it needs
+        if (PIPE && !a.any_uncoded && tab && tot_c > 0) {
+            if (a.rerank > 0)
+                pipe_rounds(std::true_type{});
+            else
+                pipe_rounds(std::false_type{});
         } else {
             for (int64_t r = 0; r < rounds; ++r) {
                 walk();
