@@ -1022,17 +1022,19 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         fail(HANN_ERR_UNSUPPORTED, strf("scan mode %d is not supported", mode));
     bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
     const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
+    int64_t rows_max = 0;
     if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
-                      reinterpret_cast<uintptr_t>(dq) % 16 != 0 ||
+                      reinterpret_cast<uintptr_t>(dq) % 16 != 0 || np > select_rows_max_probes() ||
                       !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
-                      list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax) > 160 * 1024)) {
+                      list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax) > 160 * 1024 ||
+                      (rows_max = list_rows_bound(nq, np)) > kListRowsMax)) {
         // the list-major scan needs a rerank, codes for every entry, a probe
-        // row per query and a supported shape; otherwise the persistent
-        // table scan runs
+        // row per query, a supported shape and room for the batch's
+        // distance rows; otherwise the persistent table scan runs
         list_scan = false;
         mode = SCAN_MODE_ADC_F32;
     }
-    if (list_scan) mode = SCAN_MODE_ADC_F32;   // the pre-pass and overflow scans
+    if (list_scan) mode = SCAN_MODE_ADC_F32;   // the fallback scans
     if (!have_codebooks) {
         // no codebooks: every entry ranks by its exact vector
         a.any_uncoded = 1;
@@ -1041,7 +1043,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     const bool fused = (p.fused || list_scan) && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
     auto scan = [&](const ScanArgs &args) {
         if (list_scan)
-            run_list_scan(args, nq, x3, np, probe_rank, st, w);
+            run_list_scan(args, nq, x3, np, probe_rank, st, w, rows_max);
         else if (fused)
             run_fused_scan(args, nq, np, probe_rank, st, w);
         else {
@@ -1143,13 +1145,11 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
 }
 
 void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank,
-                          hipStream_t st, Workspace &w) {
+                          hipStream_t st, Workspace &w, int64_t rows_max) {
     const int K = (int)lists.size();
     const int selk = a.out_stride;
-    const int R = a.rerank;
     const int np = a.probe_cap;
     const int kp = a.cap / 2;
-    const int cap = std::max(1024, 8 * kp);
     const int64_t pairs_max = nq * (int64_t)np;
     // arena (slot 17) for the batch's small buffers
     size_t off = 0;
@@ -1160,10 +1160,10 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     };
     const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
                  o_lfill = carve((size_t)K * 4), o_tot = carve(16), o_qtot = carve((size_t)nq * 8),
-                 o_tau = carve((size_t)nq * 4), o_cnt = carve((size_t)nq * 4), o_ovf = carve((size_t)nq * 4),
+                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_qrange = carve((size_t)nq * 8), o_ovf = carve((size_t)nq * 4),
                  o_ctr = carve(16), o_pq = carve((size_t)pairs_max * 4), o_pqc = carve((size_t)pairs_max * 4),
-                 o_items = carve((size_t)pairs_max * 16), o_pkey = carve((size_t)nq * selk * 4),
-                 o_ppos = carve((size_t)nq * selk * 8), o_pcnt = carve((size_t)nq * 4);
+                 o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)pairs_max * 16),
+                 o_pkey = carve((size_t)nq * selk * 4);
     char *base = w.get<char>(17, off);
     auto at = [&](size_t o) { return base + o; };
     int32_t *lcnt = reinterpret_cast<int32_t *>(at(o_lcnt));
@@ -1172,45 +1172,40 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int32_t *lfill = reinterpret_cast<int32_t *>(at(o_lfill));
     int32_t *totals = reinterpret_cast<int32_t *>(at(o_tot));
     int64_t *qtot = reinterpret_cast<int64_t *>(at(o_qtot));
-    float *tau = reinterpret_cast<float *>(at(o_tau));
-    int32_t *cnt = reinterpret_cast<int32_t *>(at(o_cnt));
+    int64_t *qslots = reinterpret_cast<int64_t *>(at(o_qslots));
+    int64_t *roff = reinterpret_cast<int64_t *>(at(o_roff));
+    unsigned *qrange = reinterpret_cast<unsigned *>(at(o_qrange));
     int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
-    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [0] items taken, [1] overflowed, [2] fallback queue
+    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [0] items taken, [1] fallbacks, [2] fallback queue
     int32_t *pair_q = reinterpret_cast<int32_t *>(at(o_pq));
     float *pair_qc = reinterpret_cast<float *>(at(o_pqc));
+    int64_t *pair_off = reinterpret_cast<int64_t *>(at(o_poff));
     int4 *items = reinterpret_cast<int4 *>(at(o_items));
     float *pkey = reinterpret_cast<float *>(at(o_pkey));
-    int64_t *ppos = reinterpret_cast<int64_t *>(at(o_ppos));
-    int32_t *pcnt = reinterpret_cast<int32_t *>(at(o_pcnt));
-    float *cand_key = w.get<float>(18, (size_t)nq * cap);
-    int32_t *cand_pos = w.get<int32_t>(19, (size_t)nq * cap);
-
-    // 1. pre-pass: the table scan over each query's nearest lists, for an
-    //    estimate of its selk-th distance
-    ScanArgs pre = a;
-    pre.probe_cap = std::min(np, 2);
-    pre.allow_fallback = 0;
-    pre.fb_counter = nullptr;
-    pre.out_key = pkey;
-    pre.out_id = ppos;
-    pre.out_cnt = pcnt;
-    run_fused_scan(pre, nq, std::min(maxp, 2), probe_rank, st, w, false);
-    // 2. the batch's (query, list) pairs grouped by list, as work items
-    launch_fill_i32(lcnt, K, 0, st);
-    launch_pair_count(a, nq, lcnt, qtot, st);
-    launch_list_thresholds(a, nq, pkey, pcnt, selk, pre.probe_cap, qtot, probe_rank,
-                           reinterpret_cast<const float *>(d_ymax_tmax.get()), x3, tau, st);
-    launch_pair_offsets(lcnt, K, lstart, ioff, lfill, totals, st);
-    launch_pair_scatter(a, nq, probe_rank, lstart, lfill, pair_q, pair_qc, st);
-    launch_make_items(lcnt, lstart, ioff, K, items, st);
-    // 3. the scan
-    launch_fill_i32(cnt, nq, 0, st);
-    launch_fill_i32(ctr, 4, 0, st);
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
         HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
+    const size_t shm = list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax);
+    const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / shm));
+    // every (query, entry) distance of the batch: query rows, probes in
+    // order; then the junk words of the scan's masked lanes
+    const size_t junk = (size_t)grid * list_scan_threads() * 4;
+    float *dist = w.get<float>(18, (size_t)rows_max + junk);
+
+    // 1. the batch's (query, list) pairs grouped by list, as work items, and
+    //    each query's row
+    launch_fill_i32(lcnt, K, 0, st);
+    launch_pair_count(a, nq, lcnt, qtot, qslots, st);
+    launch_row_offsets(qslots, nq, roff, st);
+    launch_pair_offsets(lcnt, K, lstart, ioff, lfill, totals, st);
+    launch_pair_scatter(a, nq, probe_rank, lstart, roff, lfill, pair_q, pair_qc, pair_off, st);
+    launch_make_items(lcnt, lstart, ioff, K, items, st);
+    // 2. the scan
+    launch_fill_i32(ctr, 4, 0, st);
+    launch_fill_i32(reinterpret_cast<int32_t *>(qrange), nq, -1, st);
+    launch_fill_i32(reinterpret_cast<int32_t *>(qrange) + nq, nq, 0, st);
     ListScanArgs la{};
     la.queries = a.queries;
     la.d = (int)dim;
@@ -1218,6 +1213,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.s = (int)(dim / m);
     la.P = (int)pmax;
     la.dk = (int)ceil_div(dim, 16) * 16;
+    la.nq = nq;
     la.list_start = d_list_start.get();
     la.list_len = d_list_len.get();
     la.codes = d_codes.get();
@@ -1225,24 +1221,20 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.cb = d_cb.get();
     la.pair_q = pair_q;
     la.pair_qc = pair_qc;
+    la.pair_off = pair_off;
     la.items = items;
     la.n_items = totals + 1;
     la.item_counter = ctr;
-    la.tau = tau;
-    la.cnt = cnt;
-    la.cand_key = cand_key;
-    la.cand_pos = cand_pos;
-    la.cap = cap;
-    const size_t shm = list_scan_lds_bytes(x3, la.d, la.m, la.P);
-    const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / shm));
+    la.dist = dist;
+    la.junk = dist + rows_max;
+    la.qrange = qrange;
     mark_kernel(0, st);
-    launch_scan_list(la, x3, num_cus * per_cu, st);
+    launch_scan_list(la, x3, grid, st);
     mark_kernel(1, st);
-    // 4. the selk best candidates per query, as the rerank's rows
-    launch_select_candidates(cand_key, cand_pos, cnt, cap, qtot, a.k_q, a.k_default, R, a.allow_fallback, nq,
-                             a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
-    // 5. overflowed queries and brute-force fallbacks: the table scan over
-    //    all their probes (the reference's rules, counter included)
+    // 3. the selk best row entries per query, as the rerank's rows
+    launch_select_rows(a, nq, dist, roff, qtot, qslots, qrange, a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
+    // 4. brute-force fallbacks: the table scan over all lists (the
+    //    reference's rule, counter included)
     FusedArgs f{};
     f.term = d_term.get();
     f.probe_rank = probe_rank;
@@ -1254,9 +1246,21 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     fa.out_key = pkey;
     const int wv = 4;
     const size_t fshm = block_lds_bytes((int)dim, (int)m, (int)pmax, block_cap(kp, wv), f.lds_probes);
-    if (fshm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major scan: the overflow scan does not fit in LDS");
+    if (fshm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major scan: the fallback scan does not fit in LDS");
     const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus * std::max<size_t>(1, (160 * 1024) / fshm));
     launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
+}
+
+int64_t PQIVF::list_rows_bound(int64_t nq, int np) const {
+    // each query's row holds at most its np longest probed lists, each
+    // rounded up to 4 slots
+    std::vector<int64_t> len(dev_len);
+    for (int64_t &v : len) v = (v + 3) & ~(int64_t)3;
+    const size_t top = std::min(len.size(), (size_t)std::max(np, 0));
+    std::nth_element(len.begin(), len.begin() + (ptrdiff_t)top, len.end(), std::greater<int64_t>());
+    int64_t s = 0;
+    for (size_t i = 0; i < top; ++i) s += len[i];
+    return nq * s;
 }
 
 void PQIVF::resolve_timers() {

@@ -160,7 +160,11 @@ struct PQIVF {
                         Workspace &w, bool main_scan = true);
     // list-major MFMA scan; a.out_* are the candidate rows of the rerank
     void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
-                       Workspace &w);
+                       Workspace &w, int64_t rows_max);
+    // entries in a batch's distance rows at most: nq times the np longest lists
+    int64_t list_rows_bound(int64_t nq, int np) const;
+    // the list-major scan's distance rows stay within 2^31 floats (8 GiB)
+    static constexpr int64_t kListRowsMax = int64_t(1) << 31;
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;

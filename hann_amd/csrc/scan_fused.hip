@@ -155,6 +155,42 @@ __device__ __forceinline__ float split_sum16(const uint16_t *codes, int64_t e, i
     return acc;
 }
 
+// 4 * byte B of w in one VALU op (SDWA operand select): the LDS byte offset
+// of code B of a packed code word in a 256-entry FP32 table
+template <int B>
+__device__ __forceinline__ uint32_t byte_x4(uint32_t w) {
+    uint32_t r;
+    if constexpr (B == 0)
+        asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_0"
+            : "=v"(r) : "v"(w));
+    else if constexpr (B == 1)
+        asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_1"
+            : "=v"(r) : "v"(w));
+    else if constexpr (B == 2)
+        asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_2"
+            : "=v"(r) : "v"(w));
+    else
+        asm("v_lshlrev_b32_sdwa %0, 2, %1 dst_sel:DWORD dst_unused:UNUSED_PAD src0_sel:DWORD src1_sel:BYTE_3"
+            : "=v"(r) : "v"(w));
+    return r;
+}
+
+// b known once the caller's loop is unrolled
+__device__ __forceinline__ uint32_t byte_x4_at(uint32_t w, int b) {
+    switch (b) {
+    case 0: return byte_x4<0>(w);
+    case 1: return byte_x4<1>(w);
+    case 2: return byte_x4<2>(w);
+    default: return byte_x4<3>(w);
+    }
+}
+
+typedef __attribute__((address_space(3))) const float lds_f32;
+
+// the FP32 word at LDS byte address off (an LDS address, not a pointer into
+// the dynamic area)
+__device__ __forceinline__ float lds_at(uint32_t off) { return *(lds_f32 *)(uintptr_t)off; }
+
 __device__ __forceinline__ int64_t uniform_i64(int64_t v) {
     const uint64_t u = (uint64_t)v;
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)u);
@@ -227,9 +263,6 @@ struct BlockTopK {
 template <int W, typename CodeT, int M, int PC>
 __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))) k_scan_block(ScanArgs a, FusedArgs f) {
     extern __shared__ __align__(16) unsigned char s_raw[];
-    __shared__ int s_item;
-    __shared__ int64_t s_red[2][W];
-    __shared__ int s_cnt[2][W];
     const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid / kWave), lane = tid % kWave;
     const int d = a.d, m = a.m, s = a.s, P = PC ? PC : a.pmax_cb, MP = m * P;
     const int cap = a.cap;
@@ -248,6 +281,12 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
     int32_t *s_lst = s_len + f.lds_probes;
     int32_t *s_nch = s_lst + f.lds_probes;
     float *s_qc = reinterpret_cast<float *>(s_nch + f.lds_probes);
+    // block bookkeeping last: the kernel has no static LDS, so S_q sits at
+    // LDS address 0
+    int64_t(*s_red)[W] = reinterpret_cast<int64_t(*)[W]>(
+        (reinterpret_cast<uintptr_t>(s_qc + f.lds_probes) + 15) & ~(uintptr_t)15);
+    int(*s_cnt)[W] = reinterpret_cast<int(*)[W]>(&s_red[2][0]);
+    int &s_item = *reinterpret_cast<int *>(&s_cnt[2][0]);
 
     const int nq_all = f.nq_dev ? *f.nq_dev : f.nq;
     for (;;) {
@@ -468,8 +507,14 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
                         const int i = g + u, sh = 8 * (i & 3);
                         const uint32_t w0 = (i & 4) ? x0.w[i >> 3].y : x0.w[i >> 3].x;
                         const uint32_t w1 = (i & 4) ? x1.w[i >> 3].y : x1.w[i >> 3].x;
-                        v0[u] = s_S[i * P + (int)((w0 >> sh) & 0xFFu)];
-                        v1[u] = s_S[i * P + (int)((w1 >> sh) & 0xFFu)];
+                        if constexpr (PC == 256) {
+                            // S_q at LDS address 0: table i at byte 1024 i
+                            v0[u] = lds_at(byte_x4_at(w0, i & 3) + 1024u * i);
+                            v1[u] = lds_at(byte_x4_at(w1, i & 3) + 1024u * i);
+                        } else {
+                            v0[u] = s_S[i * P + (int)((w0 >> sh) & 0xFFu)];
+                            v1[u] = s_S[i * P + (int)((w1 >> sh) & 0xFFu)];
+                        }
                     }
 #pragma unroll
                     for (int u = 0; u < GS; ++u) {
@@ -555,7 +600,8 @@ __global__ void __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(4))
         // the pipelined rounds need the probe table in LDS: their list walk
         // then issues no global loads, and every round issues the same loads,
         // so each wait covers only the loads it needs
-        if (PIPE && !a.any_uncoded && tab && tot_c > 0) {
+        // (the P = 256 gathers address S_q from LDS address 0)
+        if (PIPE && !a.any_uncoded && tab && tot_c > 0 && (PC != 256 || __builtin_amdgcn_groupstaticsize() == 0)) {
             if (a.rerank > 0)
                 pipe_rounds(std::true_type{});
             else
@@ -613,9 +659,13 @@ void launch_query_tables(const float *queries, int64_t nq, int d, int m, int P, 
     HANN_LAUNCH_CHECK();
 }
 
+// s_red[2][8] (int64), s_cnt[2][8], s_item, rounded up
+constexpr size_t kBlockBookkeeping = 256;
+
 size_t block_lds_bytes(int d, int m, int P, int cap, int lds_probes) {
     return (((size_t)d * 4 + 15) & ~(size_t)15) + (((size_t)m * P * 4 + 15) & ~(size_t)15) +
-           (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (size_t)lds_probes * 24 + 16;
+           (((size_t)cap * 4 + 15) & ~(size_t)15) + (size_t)cap * 8 + (((size_t)lds_probes * 24 + 15) & ~(size_t)15) +
+           kBlockBookkeeping;
 }
 
 int block_cap(int kp, int w) {

@@ -672,11 +672,9 @@ def test_list_scan_matches_table_scan(hann, oracle, mode, rerank):
     assert np.mean(both) >= 0.95
 
 
-def test_list_scan_overflow_and_open_threshold(hann):
-    """Lists smaller than k*R leave the pre-pass without a threshold: every
-    probed entry becomes a candidate, and queries whose candidates exceed
-    the room take the table scan. Results equal the table scan's when the
-    candidates hold every probed entry."""
+def test_list_scan_small_lists(hann):
+    """Many short lists (rows of about 1,250 and 5,000 entries against
+    k * R = 200): the list-major selection agrees with the table scan's."""
     from hann_amd import _lib
     idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
     for nprobe in (16, 64):
@@ -685,6 +683,44 @@ def test_list_scan_overflow_and_open_threshold(hann):
         assert np.array_equal(a[2], b[2])
         agree = np.mean([len(set(a[0][q].tolist()) & set(b[0][q].tolist())) / 50 for q in range(len(qs))])
         assert agree >= 0.99, (nprobe, agree)
+
+
+@pytest.mark.parametrize("mode", [3, 4])
+def test_list_scan_rows_within_selection_equal_table_scan(hann, mode):
+    """When k * R covers every probed entry, both scans rerank every entry
+    in exact FP32: ids, distances and counts are identical."""
+    idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
+    longest = max(len(v) for v in P().InvertedIDs(idx).values())
+    assert 2 * longest <= 50 * 16
+    rng = np.random.default_rng(3)
+    ks = rng.choice([50, 100], len(qs)).astype(np.int32)
+    from hann_amd import _lib
+    a = idx.search_batch(qs, ks, nprobe=2, scan_mode=_lib.SCAN_ADC_F32, rerank=16, fused=1)
+    b = idx.search_batch(qs, ks, nprobe=2, scan_mode=mode, rerank=16, fused=1)
+    for u, v in zip(a, b):
+        assert np.array_equal(u, v)
+
+
+def test_list_scan_exact_ties_select_by_position(hann):
+    """Thousands of identical vectors share one key: the selection narrows
+    its histogram to that key and keeps the smallest positions, as the
+    table scan does (its ties go by position), so results are identical."""
+    from hann_amd import _lib
+    pq = P()
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((3000, 64)).astype(np.float32)
+    dup = np.repeat(rng.standard_normal((1, 64)).astype(np.float32), 6000, axis=0)
+    data = np.concatenate([x, dup])
+    idx = pq.New(64, pq.WithCoarseK(8), pq.WithNumSubquantizers(8), pq.WithPQK(64), pq.WithKMeansIters(3),
+                 pq.WithSeed(2))
+    idx.bulk_add_arrays(np.arange(len(data)), data)
+    idx.Train()
+    qs = dup[:8] + 0.01 * rng.standard_normal((8, 64)).astype(np.float32)
+    for mode in (3, 4):
+        a = idx.search_batch(qs, 10, nprobe=2, scan_mode=_lib.SCAN_ADC_F32, rerank=4, fused=1)
+        b = idx.search_batch(qs, 10, nprobe=2, scan_mode=mode, rerank=4, fused=1)
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
 
 
 def test_list_scan_fallback_counter(hann):
