@@ -353,8 +353,14 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             if (X3) bl[j] = *reinterpret_cast<const bf16x8 *>(&sQl[col * LD + 16 * j + 8 * half]);
         }
         const float qc_r = s_qc[col];
-        const bool col_ok = col < pc;
-        unsigned kmin = 0xFFFFFFFFu, kmax = 0u;
+        // this lane's queries in the store phase are 8t + lane / 8: their
+        // smallest and largest keys so far (NaN padding is ignored by min/max)
+        float lmin[4], lmax[4];
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            lmin[t] = __builtin_inff();
+            lmax[t] = -__builtin_inff();
+        }
         // one slab: 32 entries of the list against the item's queries. Every
         // slab issues the same memory operations (masked lanes store to the
         // junk words), so each wait covers only what it needs.
@@ -382,26 +388,32 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const int row = 8 * (r >> 2) + 4 * half + (r & 3);
-                const float key = sane_key((qc_r + s_term[row]) - 2.f * acc[r]);
-                s_tile[col * LT + row] = key;
-                const bool ok = col_ok && base + row < n;
-                const unsigned u = key_bits(key);
-                kmin = (ok && u < kmin) ? u : kmin;
-                kmax = (ok && u > kmax) ? u : kmax;
+                // fminf maps a NaN to +inf
+                s_tile[col * LT + row] = fminf((qc_r + s_term[row]) - 2.f * acc[r], __builtin_inff());
             }
             wave_sync();   // the tile
             // 16-byte runs of the row: lane -> query 8t + lane / 8, entries
             // base + 4 (lane % 8) .. + 3; slots past the list's end get the
-            // padding sentinel, and masked lanes store to this wave's junk
+            // padding sentinel (last slab only), and masked lanes store to
+            // this wave's junk
+            const bool full = base + 32 <= n;
 #pragma unroll
             for (int t = 0; t < 4; ++t) {
                 const int c = 8 * t + (lane >> 3), g = lane & 7;
                 const int64_t e4 = base + 4 * g;
                 float4 v;
-                v.x = e4 + 0 < n ? s_tile[c * LT + 4 * g + 0] : kPadKey;
-                v.y = e4 + 1 < n ? s_tile[c * LT + 4 * g + 1] : kPadKey;
-                v.z = e4 + 2 < n ? s_tile[c * LT + 4 * g + 2] : kPadKey;
-                v.w = e4 + 3 < n ? s_tile[c * LT + 4 * g + 3] : kPadKey;
+                v.x = s_tile[c * LT + 4 * g + 0];
+                v.y = s_tile[c * LT + 4 * g + 1];
+                v.z = s_tile[c * LT + 4 * g + 2];
+                v.w = s_tile[c * LT + 4 * g + 3];
+                if (!full) {
+                    v.x = e4 + 0 < n ? v.x : kPadKey;
+                    v.y = e4 + 1 < n ? v.y : kPadKey;
+                    v.z = e4 + 2 < n ? v.z : kPadKey;
+                    v.w = e4 + 3 < n ? v.w : kPadKey;
+                }
+                lmin[t] = fminf(lmin[t], fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
+                lmax[t] = fmaxf(lmax[t], fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
                 float4 *dst = (c < pc && e4 < n) ? reinterpret_cast<float4 *>(la.dist + s_off[c] + e4) : junk;
                 *dst = v;
             }
@@ -419,15 +431,20 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             slab(sb, base);
             base += 32 * LNW;
         }
-        // the row ranges: lanes l and l + 32 hold the same query
-        {
-            const unsigned omin = __shfl_xor(kmin, 32), omax = __shfl_xor(kmax, 32);
-            kmin = omin < kmin ? omin : kmin;
-            kmax = omax > kmax ? omax : kmax;
-            if (half == 0 && col_ok && kmin <= kmax) {
-                const int64_t qid = s_qid[col];
-                atomicMin(&la.qrange[qid], kmin);
-                atomicMax(&la.qrange[la.nq + qid], kmax);
+        // the row ranges: lanes 8c' .. 8c' + 7 share their queries; a zero
+        // minimum or maximum is widened to cover both signed zeros
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            float mn = lmin[t], mx = lmax[t];
+            for (int o = 1; o < 8; o <<= 1) {
+                mn = fminf(mn, __shfl_xor(mn, o));
+                mx = fmaxf(mx, __shfl_xor(mx, o));
+            }
+            const int c = 8 * t + (lane >> 3);
+            if ((lane & 7) == 0 && c < pc && mn <= mx) {
+                const int64_t qid = s_qid[c];
+                atomicMin(&la.qrange[qid], key_bits(mn == 0.f ? -0.f : mn));
+                atomicMax(&la.qrange[la.nq + qid], key_bits(mx == 0.f ? 0.f : mx));
             }
         }
         __syncthreads();   // the next item rewrites the query tile
@@ -576,26 +593,30 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         }
         __syncthreads();
         const int B = s_B, below = s_below, nb = s_nb;
-        if (below > 0) {
+        const bool fits = nb <= SEL_ROOM;
+        if (tid == 0) s_fill = 0;
+        __syncthreads();
+        // one pass: entries below bin B are selected; bin B goes to LDS when
+        // it fits there
+        if (below > 0 || fits) {
             for_row([&](int64_t i, unsigned u) {
-                if (u >= lo && u <= hi && (int)((u - lo) >> sh) < B) orow[atomicAdd(&s_out, 1)] = pos_of(i);
+                if (u < lo || u > hi) return;
+                const int b = (int)((u - lo) >> sh);
+                if (b < B) {
+                    orow[atomicAdd(&s_out, 1)] = pos_of(i);
+                } else if (fits && b == B) {
+                    const int slot = atomicAdd(&s_fill, 1);
+                    bkey[slot] = u;
+                    bpos[slot] = pos_of(i);
+                }
             });
         }
         need -= below;
         const unsigned blo = lo + ((unsigned)B << sh);
         const unsigned bspan = (sh >= 32) ? 0xFFFFFFFFu : ((1u << sh) - 1u);
         const unsigned bhi = (hi - blo) < bspan ? hi : blo + bspan;
-        if (nb <= SEL_ROOM) {
+        if (fits) {
             // sort bin B by (key bits, position); the first need complete the row
-            if (tid == 0) s_fill = 0;
-            __syncthreads();
-            for_row([&](int64_t i, unsigned u) {
-                if (u >= blo && u <= bhi) {
-                    const int slot = atomicAdd(&s_fill, 1);
-                    bkey[slot] = u;
-                    bpos[slot] = pos_of(i);
-                }
-            });
             __syncthreads();
             int span = 1;
             while (span < nb) span <<= 1;
