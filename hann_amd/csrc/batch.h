@@ -25,10 +25,13 @@ struct BF16Base {
 // rank bounds the query's nearest, e.g. the previous k-means assignment.
 // k_q (may be null): each query's own k, clamped to [1, k]; its row (stride
 // k) holds its k_q results, then (-1, inf). cache (may be null): the base's
-// BF16 copy, reused while ready.
+// BF16 copy, reused while ready. ws (may be null): the pass's buffers, owned
+// by the caller's stream, and then the call does not wait for the stream when
+// it can avoid it; null: the context's shared buffers, and the call returns
+// with its work complete.
 void certified_topk(Context &c, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
                     const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st,
-                    const int32_t *k_q = nullptr, BF16Base *cache = nullptr);
+                    const int32_t *k_q = nullptr, BF16Base *cache = nullptr, Workspace *ws = nullptr);
 
 // assignPoints on the BF16 path: out[i] = nearest centroid, lowest index on ties
 void assign_bf16(Context &c, const float *data, int64_t n, const float *cents, int kc, int d, const int32_t *hint,

@@ -8,14 +8,19 @@
 
 namespace hann {
 
+// bases up to this many rows: overflowed queries are ranked over every row
+// by the candidate-ranking kernel
+constexpr int64_t kRankEveryMax = 65536;
+
 // Certified BF16 path for the (squared) Euclidean metrics. Results equal the
 // exact path's: the filter keeps every row whose exact FP32 rank can be at
 // most the query's sample k-th rank, and those rows are ranked exactly.
-// Queries whose candidate list overflows are rerun on the exact path.
+// A query whose candidate list overflows is ranked over every row (small
+// bases, on the device) or rerun on the tiled exact path.
 void certified_topk(Context &c, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
                     const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st,
-                    const int32_t *k_q, BF16Base *cache) {
-    Workspace &w = c.ws_bf16;
+                    const int32_t *k_q, BF16Base *cache, Workspace *ws) {
+    Workspace &w = ws ? *ws : c.ws_bf16;
     const int dp = (int)ceil_div(dim, 32) * 32;
     const int metric = sqrt_out ? HANN_METRIC_EUCLIDEAN : HANN_METRIC_SQEUCLIDEAN;
     // BF16 copies, norms, max norm and exactness flag of the base (cached)
@@ -32,8 +37,8 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
             HANN_HIP(hipMemsetAsync(cache->stat.get(), 0, 4 * sizeof(unsigned), st));
             launch_to_bf16(base, nb, dim, dp, cache->xb.get(), cache->xn.get(), cache->xf.get(), cache->stat.get(),
                            reinterpret_cast<int *>(cache->stat.get() + 1), st);
-            // other streams may read the cache once this call returns: it
-            // synchronizes its stream before returning (the overflow check)
+            // other streams may read the cache once this call returns
+            HANN_HIP(hipStreamSynchronize(st));
             cache->ready = true;
         }
         xb = cache->xb.get();
@@ -66,13 +71,9 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
         const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
         ns = ceil_div(nb, stride);
-        std::vector<int64_t> pick((size_t)ns);
-        for (int64_t i = 0; i < ns; ++i) pick[i] = i * stride;
-        int64_t *dpick = w.get<int64_t>(7, (size_t)ns);
-        HANN_HIP(hipMemcpyAsync(dpick, pick.data(), (size_t)ns * sizeof(int64_t), hipMemcpyHostToDevice, st));
         __bf16 *sb = w.get<__bf16>(8, (size_t)ns * dp);
         float *sn = w.get<float>(9, (size_t)ns);
-        launch_gather_bf16(xb, dp, xn, dpick, ns, sb, sn, st);
+        launch_gather_bf16(xb, dp, xn, stride, ns, sb, sn, st);
         const int64_t qchunk = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(256ll << 20) / (4 * ns)));
         float *dense = w.get<float>(10, (size_t)qchunk * ns);
         for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
@@ -81,12 +82,14 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
             launch_sample_thresholds(dense, qn_, ns, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
                                      reinterpret_cast<int *>(stat + 1), dim, thr + q0, st);
         }
-        // the host copy of pick must outlive its transfer
-        HANN_HIP(hipStreamSynchronize(st));
     }
     // candidates: expected about k * nb / ns per query; room for 8x that
     const int64_t expect = std::max<int64_t>(k, ceil_div((int64_t)k * nb, ns));
     const int cap = (int)std::min<int64_t>(nb, std::max<int64_t>(256, 8 * expect));
+    // a query whose candidates overflow the room: with a small base the
+    // ranking kernel ranks every row itself (and the host does not wait for
+    // the counts); otherwise it is redone below by the tiled exact path
+    const bool rank_every = nb <= kRankEveryMax;
     // rows per workgroup: about 8 expected candidates per query and
     // workgroup, against 32 slots
     const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb / (16 * expect)));
@@ -97,17 +100,22 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
         const int64_t qn_ = std::min(chunk, nq - q0);
         launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cap, st);
-        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, cnt + q0, cand, cap, k, k_q ? k_q + q0 : nullptr,
-                               sqrt_out, out_ids + q0 * k, out_dist + q0 * k, st);
+        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, rank_every ? nb : 0, cnt + q0, cand, cap, k,
+                               k_q ? k_q + q0 : nullptr, sqrt_out, out_ids + q0 * k, out_dist + q0 * k, st);
     }
-    // overflowed queries (and queries with fewer than k rows) take the exact path
+    if (rank_every) {
+        // the shared buffers are free again only once the stream has run
+        // this far
+        if (!ws) HANN_HIP(hipStreamSynchronize(st));
+        return;
+    }
+    // overflowed queries take the exact path
     std::vector<int32_t> hc((size_t)nq);
     HANN_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)nq * sizeof(int32_t), hipMemcpyDeviceToHost, st));
     HANN_HIP(hipStreamSynchronize(st));
     std::vector<int64_t> redo;
     for (int64_t q = 0; q < nq; ++q)
         if (hc[q] > cap) redo.push_back(q);
-    c.bf16_fallback_queries += (int64_t)redo.size();
     if (redo.empty()) return;
     const int64_t nr = (int64_t)redo.size();
     int64_t *dredo = w.get<int64_t>(7, (size_t)nr);

@@ -1,6 +1,8 @@
 // kernels.h -- host-side launchers of the hann_amd device kernels.
 #pragma once
 
+#include <memory>
+
 #include "common.h"
 
 namespace hann {
@@ -17,6 +19,14 @@ struct Workspace {
         slots[slot].resize(count * sizeof(T) + 256);
         return reinterpret_cast<T *>(slots[slot].ptr);
     }
+    // a second workspace owned by this one (for a pass nested in a search)
+    Workspace &aux() {
+        if (!aux_) aux_.reset(new Workspace);
+        return *aux_;
+    }
+
+  private:
+    std::unique_ptr<Workspace> aux_;
 };
 
 int kp_for(int k);

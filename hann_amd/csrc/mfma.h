@@ -16,7 +16,7 @@ void launch_hint_rank(const float *queries, int64_t nq, int d, const float *base
 void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm_f, const unsigned *max_norm,
                        const int *inexact, int d, float *thr, hipStream_t st);
 // append every row with approx distance <= thr[q] to q's candidate list
-// (cnt[q] > cap marks an overflowed list); a workgroup covers rt_per_wg
+// (cnt[q] > cap: more candidates than room); a workgroup covers rt_per_wg
 // tiles of 128 rows
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                         const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row, int cap,
@@ -24,8 +24,8 @@ void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb
 // every approximate distance: dense[q * nb + row]
 void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                        const float *xn, float *dense, hipStream_t st);
-// out rows = X rows pick[...] with their BF16 norms
-void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t *pick, int64_t n, __bf16 *out,
+// out rows = X rows 0, stride, 2 stride, ... (n of them) with their BF16 norms
+void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, int64_t stride, int64_t n, __bf16 *out,
                         float *out_n, hipStream_t st);
 // filter thresholds from the k-th smallest approximate distance of each
 // query's ns sample rows (approx[q * ns + j])
@@ -33,9 +33,11 @@ void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t 
 void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const int32_t *k_q,
                               const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, float *thr,
                               hipStream_t st);
-// exact reference-order ranking of the candidates (queries with cnt <= cap)
-void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
-                            const int32_t *cand_row, int cap, int k, const int32_t *k_q, int sqrt_out,
-                            int64_t *out_ids, double *out_dist, hipStream_t st);
+// exact reference-order ranking of the candidates; a query whose candidates
+// overflowed (cnt > cap) ranks all nb rows instead when nb > 0, and is left
+// untouched when nb == 0
+void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
+                            const int32_t *cnt, const int32_t *cand_row, int cap, int k, const int32_t *k_q,
+                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st);
 
 }  // namespace hann

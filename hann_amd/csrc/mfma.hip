@@ -21,29 +21,57 @@ namespace dev {
 // FP32 rows -> BF16 rows padded with zeros to dp columns; norm_b = ||x_b||^2
 // of the BF16 values (FP32 sum), norm_f = ||x|| of the FP32 row; max_norm
 // (float bits, non-negative) and inexact flag gathered with atomics.
-__global__ void k_to_bf16(const float *__restrict__ x, int64_t n, int d, int dp, __bf16 *__restrict__ out,
-                          float *__restrict__ norm_b, float *__restrict__ norm_f, unsigned *__restrict__ max_norm,
-                          int *__restrict__ inexact) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const float *r = x + i * d;
-    __bf16 *o = out + i * dp;
-    float sb = 0.f, sf = 0.f;
-    int bad = 0;
-    for (int e = 0; e < dp; ++e) {
-        float v = e < d ? r[e] : 0.f;
-        __bf16 b = (__bf16)v;
-        float vb = (float)b;
-        bad |= vb != v;
-        o[e] = b;
-        sb = __builtin_fmaf(vb, vb, sb);
-        sf = __builtin_fmaf(v, v, sf);
+// one wave per row, 64 rows per workgroup; one atomic per workgroup
+constexpr int TB_ROWS = 64;
+
+__global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ x, int64_t n, int d, int dp,
+                                                 __bf16 *__restrict__ out, float *__restrict__ norm_b,
+                                                 float *__restrict__ norm_f, unsigned *__restrict__ max_norm,
+                                                 int *__restrict__ inexact) {
+    __shared__ float s_max[4];
+    __shared__ int s_bad[4];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    float wmax = 0.f;
+    int wbad = 0;
+    for (int rr = w; rr < TB_ROWS; rr += 4) {
+        const int64_t i = (int64_t)blockIdx.x * TB_ROWS + rr;
+        if (i >= n) break;
+        const float *r = x + i * d;
+        __bf16 *o = out + i * dp;
+        float sb = 0.f, sf = 0.f;
+        int bad = 0;
+        for (int e = lane; e < dp; e += 64) {
+            const float v = e < d ? r[e] : 0.f;
+            const __bf16 b = (__bf16)v;
+            const float vb = (float)b;
+            bad |= vb != v;
+            o[e] = b;
+            sb = __builtin_fmaf(vb, vb, sb);
+            sf = __builtin_fmaf(v, v, sf);
+        }
+        for (int o2 = 32; o2 > 0; o2 >>= 1) {
+            sb += __shfl_xor(sb, o2);
+            sf += __shfl_xor(sf, o2);
+            bad |= __shfl_xor(bad, o2);
+        }
+        const float nf = __builtin_sqrtf(sf);
+        if (lane == 0) {
+            norm_b[i] = sb;
+            norm_f[i] = nf;
+        }
+        wmax = fmaxf(wmax, nf);
+        wbad |= bad;
     }
-    norm_b[i] = sb;
-    const float nf = __builtin_sqrtf(sf);
-    norm_f[i] = nf;
-    atomicMax(max_norm, __float_as_uint(nf));
-    if (bad) atomicOr(inexact, 1);
+    if (lane == 0) {
+        s_max[w] = wmax;
+        s_bad[w] = wbad;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
+        atomicMax(max_norm, __float_as_uint(m));
+        if (s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]) atomicOr(inexact, 1);
+    }
 }
 
 // Per-query filter threshold from the sample's exact k-th rank tau.
@@ -94,8 +122,9 @@ constexpr int MF_SLOTS = 32;   // per-query candidate slots of a workgroup
 // 8*(r/4) + 4*(l/32) + r%4, column l%32.
 // DENSE: write every approximate distance to dense[q * ld_dense + row].
 // Otherwise rows within thr[q] are gathered per query in LDS and appended
-// to the query's global list once per workgroup; a query whose LDS slots
-// overflow gets cnt > cap, which sends it to the exact path.
+// to the query's global list once per workgroup; rows beyond the LDS slots
+// are appended one by one. cnt[q] > cap (more candidates than room) sends
+// the query to the exact path.
 template <bool DENSE>
 __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q, int64_t nq,
                                                    const __bf16 *__restrict__ X, int64_t nb, int dp,
@@ -176,7 +205,14 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
                         dense[gq * ld_dense + gr] = approx;
                     } else if (approx <= thr[gq]) {
                         const int slot = atomicAdd(&s_cc[ql], 1);
-                        if (slot < MF_SLOTS) s_cr[ql * MF_SLOTS + slot] = (int32_t)gr;
+                        if (slot < MF_SLOTS) {
+                            s_cr[ql * MF_SLOTS + slot] = (int32_t)gr;
+                        } else {
+                            // the workgroup's slots for this query are full:
+                            // straight to the query's global list
+                            const int g = atomicAdd(&cnt[gq], 1);
+                            if (g < cap) cand_row[gq * cap + g] = (int32_t)gr;
+                        }
                     }
                 }
             }
@@ -185,26 +221,21 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
     __syncthreads();
     for (int ql = tid; ql < MF_TILE; ql += 256) {
         const int64_t gq = q0 + ql;
-        const int n = s_cc[ql];
+        const int n = s_cc[ql] < MF_SLOTS ? s_cc[ql] : MF_SLOTS;
         if (gq >= nq || n == 0) continue;
-        if (n > MF_SLOTS) {
-            atomicMax(&cnt[gq], cap + 1);
-            continue;
-        }
         const int base = atomicAdd(&cnt[gq], n);
         for (int i = 0; i < n && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[ql * MF_SLOTS + i];
     }
 }
 
-// out rows b = X rows pick[b] (BF16, dp columns) and their BF16 norms
-__global__ void k_gather_bf16(const __bf16 *__restrict__ X, int dp, const float *__restrict__ xn,
-                              const int64_t *__restrict__ pick, int64_t n, __bf16 *__restrict__ out,
-                              float *__restrict__ out_n) {
+// rows 0, stride, 2 stride, ... of X (BF16, dp columns) with their BF16 norms
+__global__ void k_gather_bf16(const __bf16 *__restrict__ X, int dp, const float *__restrict__ xn, int64_t stride,
+                              int64_t n, __bf16 *__restrict__ out, float *__restrict__ out_n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * dp) return;
     const int64_t b = t / dp;
     const int c = (int)(t % dp);
-    const int64_t src = pick[b];
+    const int64_t src = b * stride;
     out[t] = X[src * dp + c];
     if (c == 0) out_n[b] = xn[src];
 }
@@ -308,28 +339,31 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
 __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict__ queries, int d,
                                                         const float *__restrict__ base,
                                                         const int32_t *__restrict__ cnt,
-                                                        const int32_t *__restrict__ cand_row, int cap, int k,
-                                                        const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
+                                                        const int32_t *__restrict__ cand_row, int cap, int64_t nb,
+                                                        int k, const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
                                                         int64_t *__restrict__ out_ids, double *__restrict__ out_dist) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     float *s_keys = reinterpret_cast<float *>(s_raw);
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + ccap);
     const int lane = threadIdx.x;
     const int64_t q = blockIdx.x;
-    const int n = cnt[q];
-    if (n > cap) return;
+    // a query whose candidates overflowed the room ranks every row (nb > 0),
+    // or is left to the caller (nb == 0)
+    const bool every = cnt[q] > cap;
+    if (every && nb == 0) return;
+    const int64_t n = every ? nb : cnt[q];
     int kq = k_q ? k_q[q] : k;
     kq = kq < 1 ? 1 : (kq > k ? k : kq);
     WaveTopK sel;
     sel.init(s_keys, s_ids, kq, ccap);
     const float *qv = queries + q * d;
-    for (int b = 0; b < n; b += kWave) {
-        const int t = b + lane;
+    for (int64_t b = 0; b < n; b += kWave) {
+        const int64_t t = b + lane;
         const bool valid = t < n;
         float key = 0.f;
         int64_t row = 0;
         if (valid) {
-            row = cand_row[q * cap + t];
+            row = every ? t : cand_row[q * cap + t];
             key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
         }
         sel.push(key, row, valid);
@@ -349,8 +383,8 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
 void launch_to_bf16(const float *x, int64_t n, int d, int dp, __bf16 *out, float *norm_b, float *norm_f,
                     unsigned *max_norm, int *inexact, hipStream_t st) {
     if (n == 0) return;
-    hipLaunchKernelGGL(dev::k_to_bf16, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, st, x, n, d, dp, out, norm_b,
-                       norm_f, max_norm, inexact);
+    hipLaunchKernelGGL(dev::k_to_bf16, dim3((unsigned)ceil_div(n, dev::TB_ROWS)), dim3(256), 0, st, x, n, d, dp, out,
+                       norm_b, norm_f, max_norm, inexact);
     HANN_LAUNCH_CHECK();
 }
 
@@ -411,10 +445,10 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
     }
 }
 
-void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, const int64_t *pick, int64_t n, __bf16 *out,
+void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, int64_t stride, int64_t n, __bf16 *out,
                         float *out_n, hipStream_t st) {
     if (n == 0) return;
-    hipLaunchKernelGGL(dev::k_gather_bf16, dim3((unsigned)ceil_div(n * dp, 256)), dim3(256), 0, st, X, dp, xn, pick,
+    hipLaunchKernelGGL(dev::k_gather_bf16, dim3((unsigned)ceil_div(n * dp, 256)), dim3(256), 0, st, X, dp, xn, stride,
                        n, out, out_n);
     HANN_LAUNCH_CHECK();
 }
@@ -428,9 +462,9 @@ void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k
     HANN_LAUNCH_CHECK();
 }
 
-void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, const int32_t *cnt,
-                            const int32_t *cand_row, int cap, int k, const int32_t *k_q, int sqrt_out,
-                            int64_t *out_ids, double *out_dist, hipStream_t st) {
+void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
+                            const int32_t *cnt, const int32_t *cand_row, int cap, int k, const int32_t *k_q,
+                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st) {
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = (size_t)ccap * 12 + 16;
@@ -439,7 +473,8 @@ void launch_rank_candidates(const float *queries, int64_t nq, int d, const float
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d, base,
-                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, k, k_q ? k_q + q0 : nullptr, ccap, sqrt_out,
+                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
+                           sqrt_out,
                            out_ids + q0 * k,
                            out_dist + q0 * k);
         HANN_LAUNCH_CHECK();

@@ -978,7 +978,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         if (p.coarse_precision == HANN_PREC_BF16)
             // identical probes and ranks from the certified BF16 pass
             certified_topk(coarse_ctx, dq, nq, d_coarse.get(), kc, d, np, nullptr, 0, probes, probe_rank, st, dnp,
-                           &coarse_bf16);
+                           &coarse_bf16, &w.aux());
         else if (p.coarse_precision == HANN_PREC_FP32)
             launch_topk_exact(HANN_METRIC_SQEUCLIDEAN, dq, nq, d_coarse.get(), kc, d, np, probes, probe_rank, w, st);
         else
@@ -1247,7 +1247,8 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     const int wv = 4;
     const size_t fshm = block_lds_bytes((int)dim, (int)m, (int)pmax, block_cap(kp, wv), f.lds_probes);
     if (fshm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major scan: the fallback scan does not fit in LDS");
-    const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus * std::max<size_t>(1, (160 * 1024) / fshm));
+    // fallbacks are rare: one workgroup per CU drains the queue
+    const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus);
     launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
 }
 

@@ -14,7 +14,6 @@ struct Context {
     hipStream_t stream = nullptr;
     Workspace ws;
     Workspace ws_bf16;   // the BF16 candidate pass
-    int64_t bf16_fallback_queries = 0;   // queries the BF16 pass sent to the exact path
     std::mutex mu;
 };
 

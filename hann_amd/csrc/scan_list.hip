@@ -118,9 +118,16 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
     const int per = (K + 1023) / 1024;
     const int b0 = t * per, b1 = b0 + per < K ? b0 + per : K;
     int32_t sa = 0, sb = 0;
-    for (int l = b0; l < b1; ++l) {
-        sa += lcnt[l];
-        sb += (lcnt[l] + LQ - 1) / LQ;
+    // 16 loads in flight per thread
+    for (int l0 = b0; l0 < b1; l0 += 16) {
+        int32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            sa += v[u];
+            sb += (v[u] + LQ - 1) / LQ;
+        }
     }
     s_a[t] = sa;
     s_b[t] = sb;
@@ -133,12 +140,20 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
         __syncthreads();
     }
     int32_t ra = s_a[t] - sa, rb = s_b[t] - sb;
-    for (int l = b0; l < b1; ++l) {
-        lstart[l] = ra;
-        ioff[l] = rb;
-        lfill[l] = 0;
-        ra += lcnt[l];
-        rb += (lcnt[l] + LQ - 1) / LQ;
+    for (int l0 = b0; l0 < b1; l0 += 16) {
+        int32_t v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            if (l0 + u < b1) {
+                lstart[l0 + u] = ra;
+                ioff[l0 + u] = rb;
+                lfill[l0 + u] = 0;
+            }
+            ra += v[u];
+            rb += (v[u] + LQ - 1) / LQ;
+        }
     }
     if (t == 1023) {
         totals[0] = s_a[1023];
