@@ -410,10 +410,10 @@ def main():
     chosen_r["chosen"] = chosen_r[chosen_name]
     path_desc = {"table-f32": "certified bf16 coarse + persistent fp32 table scan (one table per query, per-entry "
                               "list terms, autotuned waves/query) + fp32 rerank",
-                 "list-bf16": "certified bf16 coarse + list-major decode + bf16 mfma scan (table-scan threshold "
-                              "pre-pass) + fp32 rerank",
-                 "list-bf16x3": "certified bf16 coarse + list-major decode + split-bf16 mfma scan (table-scan "
-                                "threshold pre-pass) + fp32 rerank"}
+                 "list-bf16": "certified bf16 coarse + list-major decode + bf16 mfma distance of every (query, "
+                              "entry) pair + per-query range-histogram select of k*R + fp32 rerank",
+                 "list-bf16x3": "certified bf16 coarse + list-major decode + split-bf16 mfma distance of every "
+                                "(query, entry) pair + per-query range-histogram select of k*R + fp32 rerank"}
     kernel_name = {"table-f32": "k_scan_block", "list-bf16": "k_scan_list<bf16>",
                    "list-bf16x3": "k_scan_list<bf16x3>"}
 

@@ -114,6 +114,7 @@ constexpr int MF_TILE = 128;   // queries and rows per workgroup tile
 constexpr int MF_KC = 128;     // K chunk held in LDS (bf16 columns)
 constexpr int MF_PAD = 8;      // LDS row padding (bf16) against bank conflicts
 constexpr int MF_SLOTS = 32;   // per-query candidate slots of a workgroup
+constexpr int MF_OVF = 1024;   // a workgroup's shared slots for queries whose own are full
 
 // One workgroup: 128 queries x (rt_per_wg x 128) rows, 4 waves of 64 x 64
 // (2 x 2 tiles of 32 x 32) per row tile. Lane layout of
@@ -125,8 +126,11 @@ constexpr int MF_SLOTS = 32;   // per-query candidate slots of a workgroup
 // to the query's global list once per workgroup; rows beyond the LDS slots
 // are appended one by one. cnt[q] > cap (more candidates than room) sends
 // the query to the exact path.
+// the ballot bits of this lane's half of the wave
+__device__ __forceinline__ unsigned half_mask(int lane, unsigned mlo, unsigned mhi) { return lane < 32 ? mlo : mhi; }
+
 template <bool DENSE>
-__global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q, int64_t nq,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_mfma_pass(const __bf16 *__restrict__ Q, int64_t nq,
                                                    const __bf16 *__restrict__ X, int64_t nb, int dp,
                                                    const float *__restrict__ qn, const float *__restrict__ xn,
                                                    const float *__restrict__ thr, int rt_per_wg,
@@ -137,15 +141,41 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
     const int ld = kc_max + MF_PAD;
     __bf16 *sA = reinterpret_cast<__bf16 *>(s_raw);
     __bf16 *sB = sA + MF_TILE * ld;
-    int32_t *s_cc = reinterpret_cast<int32_t *>(sB + MF_TILE * ld);
-    int32_t *s_cr = s_cc + MF_TILE;
+    float *s_qn = reinterpret_cast<float *>(sB + MF_TILE * ld);   // the tile's query norms
+    float *s_thr = s_qn + MF_TILE;                                 // and thresholds (-inf past nq)
+    int32_t *s_cc = reinterpret_cast<int32_t *>(s_thr + MF_TILE);
+    int32_t *s_cr = s_cc + MF_TILE;                                // [MF_TILE][MF_SLOTS]
+    int2 *s_ov = reinterpret_cast<int2 *>(s_cr + MF_TILE * MF_SLOTS);   // [MF_OVF] (query, row)
+    int32_t *s_on = reinterpret_cast<int32_t *>(s_ov + MF_OVF);
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int wm = wave / 2, wn = wave % 2;
     const int64_t q0 = (int64_t)blockIdx.y * MF_TILE;
     const int64_t rbase = (int64_t)blockIdx.x * MF_TILE * rt_per_wg;
-    if (!DENSE)
-        for (int t = tid; t < MF_TILE; t += 256) s_cc[t] = 0;
+    for (int t = tid; t < MF_TILE; t += 256) {
+        const int64_t gq = q0 + t;
+        s_qn[t] = gq < nq ? qn[gq] : 0.f;
+        if (!DENSE) {
+            s_thr[t] = gq < nq ? thr[gq] : -__builtin_inff();
+            s_cc[t] = 0;
+        }
+    }
+    if (!DENSE && tid == 0) *s_on = 0;
+    // 128 rows (from row r0, rows past lim zero) x columns k0 .. k0 + kc of
+    // src into an LDS tile
+    auto load_tile = [&](__bf16 *dst, const __bf16 *src, int64_t r0, int64_t lim, int k0, int kc) {
+        const int vecs = kc / 8;
+        for (int v = tid; v < MF_TILE * vecs; v += 256) {
+            const int row = v / vecs, c8 = v % vecs;
+            const int64_t g = r0 + row;
+            bf16x8 a = {};
+            if (g < lim) a = *reinterpret_cast<const bf16x8 *>(src + g * dp + k0 + c8 * 8);
+            *reinterpret_cast<bf16x8 *>(&dst[row * ld + c8 * 8]) = a;
+        }
+    };
+    // with one K chunk the query tile stays in LDS for every row tile
+    const bool a_resident = dp <= MF_KC;
+    if (a_resident) load_tile(sA, Q, q0, nq, 0, dp);
 
     for (int rt = 0; rt < rt_per_wg; ++rt) {
         const int64_t r0 = rbase + (int64_t)rt * MF_TILE;
@@ -160,17 +190,9 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
 
         for (int k0 = 0; k0 < dp; k0 += MF_KC) {
             const int kc = dp - k0 < MF_KC ? dp - k0 : MF_KC;   // multiple of 16
-            const int vecs = kc / 8;
-            __syncthreads();
-            for (int v = tid; v < MF_TILE * vecs; v += 256) {
-                const int row = v / vecs, c8 = v % vecs;
-                const int64_t gq = q0 + row, gr = r0 + row;
-                bf16x8 a = {}, b = {};
-                if (gq < nq) a = *reinterpret_cast<const bf16x8 *>(Q + gq * dp + k0 + c8 * 8);
-                if (gr < nb) b = *reinterpret_cast<const bf16x8 *>(X + gr * dp + k0 + c8 * 8);
-                *reinterpret_cast<bf16x8 *>(&sA[row * ld + c8 * 8]) = a;
-                *reinterpret_cast<bf16x8 *>(&sB[row * ld + c8 * 8]) = b;
-            }
+            __syncthreads();   // the previous tile's readers are done
+            if (!a_resident) load_tile(sA, Q, q0, nq, k0, kc);
+            load_tile(sB, X, r0, nb, k0, kc);
             __syncthreads();
             const int kl = (lane / 32) * 8;
             for (int kk = 0; kk < kc; kk += 16) {
@@ -188,33 +210,68 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
                         acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
             }
         }
-        // epilogue: approximate distance; threshold test or dense store
+        // epilogue: approximate distance; threshold test or dense store.
+        // Registers 4g .. 4g + 3 of a tile are four consecutive queries; a
+        // group's norms and thresholds are read once for both row tiles.
+        int64_t grj[2];
+        bool rvj[2];
+        float xrj[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            grj[j] = r0 + wn * 64 + j * 32 + lane % 32;
+            rvj[j] = grj[j] < nb;
+            xrj[j] = rvj[j] ? xn[grj[j]] : 0.f;
+        }
 #pragma unroll
         for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                const int64_t gr = r0 + wn * 64 + j * 32 + lane % 32;
-                const float xr = gr < nb ? xn[gr] : 0.f;
+            for (int g = 0; g < 4; ++g) {
+                const int qb = wm * 64 + i * 32 + 8 * g + 4 * (lane / 32);
+                const float4 qn4 = *reinterpret_cast<const float4 *>(&s_qn[qb]);
+                const float qv[4] = {qn4.x, qn4.y, qn4.z, qn4.w};
+                float tv[4] = {0.f, 0.f, 0.f, 0.f};
+                if (!DENSE) {
+                    const float4 t4 = *reinterpret_cast<const float4 *>(&s_thr[qb]);
+                    tv[0] = t4.x;
+                    tv[1] = t4.y;
+                    tv[2] = t4.z;
+                    tv[3] = t4.w;
+                }
 #pragma unroll
-                for (int r = 0; r < 16; ++r) {
-                    const int ql = wm * 64 + i * 32 + 8 * (r / 4) + 4 * (lane / 32) + (r % 4);
-                    const int64_t gq = q0 + ql;
-                    if (gq >= nq || gr >= nb) continue;
-                    const float approx = (qn[gq] + xr) - 2.f * acc[i][j][r];
-                    if (DENSE) {
-                        dense[gq * ld_dense + gr] = approx;
-                    } else if (approx <= thr[gq]) {
-                        const int slot = atomicAdd(&s_cc[ql], 1);
-                        if (slot < MF_SLOTS) {
-                            s_cr[ql * MF_SLOTS + slot] = (int32_t)gr;
-                        } else {
-                            // the workgroup's slots for this query are full:
-                            // straight to the query's global list
-                            const int g = atomicAdd(&cnt[gq], 1);
-                            if (g < cap) cand_row[gq * cap + g] = (int32_t)gr;
+                for (int j = 0; j < 2; ++j)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int ql = qb + u;
+                        const float approx = (qv[u] + xrj[j]) - 2.f * acc[i][j][4 * g + u];
+                        if (DENSE) {
+                            if (rvj[j] && q0 + ql < nq) dense[(q0 + ql) * ld_dense + grj[j]] = approx;
+                            continue;
+                        }
+                        // thresholds past nq are -inf: only real queries pass.
+                        // Lanes 0-31 and 32-63 each test one query against 32
+                        // rows: one LDS atomic per half reserves its slots.
+                        const bool pass = rvj[j] && approx <= tv[u];
+                        const unsigned long long m = __ballot(pass);
+                        if (m == 0) continue;
+                        const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
+                        int base = 0;
+                        if ((lane & 31) == 0 && (half_mask(lane, mlo, mhi) != 0))
+                            base = atomicAdd(&s_cc[ql], __popc(half_mask(lane, mlo, mhi)));
+                        base = __shfl(base, lane & 32);
+                        const int rank = lane < 32 ? (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
+                                                   : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
+                        if (pass) {
+                            const int slot = base + rank;
+                            if (slot < MF_SLOTS) {
+                                s_cr[ql * MF_SLOTS + slot] = (int32_t)grj[j];
+                            } else {
+                                // the query's slots are full: the
+                                // workgroup's shared overflow list
+                                const int o = atomicAdd(s_on, 1);
+                                if (o < MF_OVF) s_ov[o] = make_int2(ql, (int32_t)grj[j]);
+                            }
                         }
                     }
-                }
             }
     }
     if (DENSE) return;
@@ -226,6 +283,18 @@ __global__ void __launch_bounds__(256) k_mfma_pass(const __bf16 *__restrict__ Q,
         const int base = atomicAdd(&cnt[gq], n);
         for (int i = 0; i < n && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[ql * MF_SLOTS + i];
     }
+    const int no = *s_on;
+    for (int o = tid; o < (no < MF_OVF ? no : MF_OVF); o += 256) {
+        const int2 e = s_ov[o];
+        const int64_t gq = q0 + e.x;
+        const int g = atomicAdd(&cnt[gq], 1);
+        if (g < cap) cand_row[gq * cap + g] = e.y;
+    }
+    if (no > MF_OVF)
+        // rows were lost: every query that overflowed its slots takes the
+        // exact path (a count above the room)
+        for (int ql = tid; ql < MF_TILE; ql += 256)
+            if (s_cc[ql] > MF_SLOTS && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
 }
 
 // rows 0, stride, 2 stride, ... of X (BF16, dp columns) with their BF16 norms
@@ -406,7 +475,8 @@ void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm
 
 static size_t mfma_lds(int dp, bool dense) {
     const int ld = (dp < dev::MF_KC ? dp : dev::MF_KC) + dev::MF_PAD;
-    return (size_t)2 * dev::MF_TILE * ld * 2 + (dense ? 0 : (size_t)dev::MF_TILE * (1 + dev::MF_SLOTS) * 4);
+    return (size_t)2 * dev::MF_TILE * ld * 2 + (size_t)2 * dev::MF_TILE * 4 +
+           (dense ? 0 : (size_t)dev::MF_TILE * (1 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
 }
 
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,

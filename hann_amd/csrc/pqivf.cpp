@@ -1158,11 +1158,21 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         off += (bytes + 255) & ~(size_t)255;
         return o;
     };
+    // work items: a list with p pairs and n entries has ceil(p / 32) *
+    // ceil(n / chunk) of them, at most (p / 32 + 1) * ceil(n / chunk)
+    const int64_t chunk = list_scan_chunk();
+    int64_t max_len = 0, chunks_all = 0;
+    for (int64_t v : dev_len) {
+        max_len = std::max(max_len, v);
+        chunks_all += ceil_div(v, chunk);
+    }
+    const int64_t items_max = ceil_div(pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
     const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
                  o_lfill = carve((size_t)K * 4), o_tot = carve(16), o_qtot = carve((size_t)nq * 8),
-                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_qrange = carve((size_t)nq * 8), o_ovf = carve((size_t)nq * 4),
-                 o_ctr = carve(16), o_pq = carve((size_t)pairs_max * 4), o_pqc = carve((size_t)pairs_max * 4),
-                 o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)pairs_max * 16),
+                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_qrange = carve((size_t)nq * 8),
+                 o_ovf = carve((size_t)nq * 4), o_ctr = carve(16), o_parts = carve(kListScanParts * 4), o_pq = carve((size_t)pairs_max * 4),
+                 o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
+                 o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
                  o_pkey = carve((size_t)nq * selk * 4);
     char *base = w.get<char>(17, off);
     auto at = [&](size_t o) { return base + o; };
@@ -1176,9 +1186,11 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int64_t *roff = reinterpret_cast<int64_t *>(at(o_roff));
     unsigned *qrange = reinterpret_cast<unsigned *>(at(o_qrange));
     int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
-    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [0] items taken, [1] fallbacks, [2] fallback queue
+    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [1] fallbacks, [2] fallback queue
+    int32_t *parts = reinterpret_cast<int32_t *>(at(o_parts));   // items taken per partition
     int32_t *pair_q = reinterpret_cast<int32_t *>(at(o_pq));
     float *pair_qc = reinterpret_cast<float *>(at(o_pqc));
+    float *pair_scale = reinterpret_cast<float *>(at(o_psc));
     int64_t *pair_off = reinterpret_cast<int64_t *>(at(o_poff));
     int4 *items = reinterpret_cast<int4 *>(at(o_items));
     float *pkey = reinterpret_cast<float *>(at(o_pkey));
@@ -1188,22 +1200,22 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
     const size_t shm = list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax);
-    const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(2, (160 * 1024) / shm));
-    // every (query, entry) distance of the batch: query rows, probes in
-    // order; then the junk words of the scan's masked lanes
-    const size_t junk = (size_t)grid * list_scan_threads() * 4;
-    float *dist = w.get<float>(18, (size_t)rows_max + junk);
+    const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / shm));
+    // every (query, entry) key of the batch: query rows, probes in order
+    uint16_t *dist = w.get<uint16_t>(18, (size_t)rows_max);
 
     // 1. the batch's (query, list) pairs grouped by list, as work items, and
     //    each query's row
     launch_fill_i32(lcnt, K, 0, st);
     launch_pair_count(a, nq, lcnt, qtot, qslots, st);
     launch_row_offsets(qslots, nq, roff, st);
-    launch_pair_offsets(lcnt, K, lstart, ioff, lfill, totals, st);
-    launch_pair_scatter(a, nq, probe_rank, lstart, roff, lfill, pair_q, pair_qc, pair_off, st);
-    launch_make_items(lcnt, lstart, ioff, K, items, st);
+    launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, totals, st);
+    launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, roff, lfill, pair_q, pair_qc, pair_scale,
+                        pair_off, st);
+    launch_make_items(lcnt, lstart, ioff, d_list_len.get(), K, items, st);
     // 2. the scan
     launch_fill_i32(ctr, 4, 0, st);
+    launch_fill_i32(parts, kListScanParts, 0, st);
     launch_fill_i32(reinterpret_cast<int32_t *>(qrange), nq, -1, st);
     launch_fill_i32(reinterpret_cast<int32_t *>(qrange) + nq, nq, 0, st);
     ListScanArgs la{};
@@ -1221,12 +1233,13 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.cb = d_cb.get();
     la.pair_q = pair_q;
     la.pair_qc = pair_qc;
+    la.pair_scale = pair_scale;
     la.pair_off = pair_off;
     la.items = items;
     la.n_items = totals + 1;
-    la.item_counter = ctr;
+    la.item_counter = parts;
+    la.n_parts = std::min(kListScanParts, grid);
     la.dist = dist;
-    la.junk = dist + rows_max;
     la.qrange = qrange;
     mark_kernel(0, st);
     launch_scan_list(la, x3, grid, st);
@@ -1254,9 +1267,9 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
 
 int64_t PQIVF::list_rows_bound(int64_t nq, int np) const {
     // each query's row holds at most its np longest probed lists, each
-    // rounded up to 4 slots
+    // rounded up to 8 slots
     std::vector<int64_t> len(dev_len);
-    for (int64_t &v : len) v = (v + 3) & ~(int64_t)3;
+    for (int64_t &v : len) v = (v + 7) & ~(int64_t)7;
     const size_t top = std::min(len.size(), (size_t)std::max(np, 0));
     std::nth_element(len.begin(), len.begin() + (ptrdiff_t)top, len.end(), std::greater<int64_t>());
     int64_t s = 0;

@@ -18,33 +18,40 @@ struct ListScanArgs {
     const float *cb;             // [m][P][s]
     const int32_t *pair_q;       // [pairs] query of each pair, grouped by list
     const float *pair_qc;        // [pairs] |q - c_l|^2 (the coarse rank)
-    const int64_t *pair_off;     // [pairs] where the pair's distances start in dist
-    const int4 *items;           // [n_items] (list, first pair, pairs <= 32, 0)
+    const float *pair_scale;     // [pairs] the query's key scale (a power of two)
+    const int64_t *pair_off;     // [pairs] where the pair's keys start in dist
+    const int4 *items;           // [n_items] (list, first pair, pairs <= 32, first entry of the chunk)
     const int32_t *n_items;      // device count of items
-    int32_t *item_counter;       // zeroed before the launch
-    float *dist;                 // the batch's rows: query q's distances, its probes in order, each
-                                 // probe's run padded to a multiple of 4 slots (16-byte aligned)
-    float *junk;                 // [grid * 4 waves * 64 lanes] float4 stores of masked lanes
-    unsigned *qrange;            // [2][nq] smallest / largest ordered key of each row (min set to ~0, max to 0)
+    int32_t *item_counter;       // [n_parts] zeroed before the launch
+    int n_parts;                 // item partitions (at most the grid)
+    uint16_t *dist;              // the batch's rows of 16-bit ordered keys: query q's, its probes in
+                                 // order, each probe's run padded to a multiple of 8 slots (16-byte aligned)
+    unsigned *qrange;            // [2][nq] smallest / largest key of each row (min set to ~0, max to 0)
 };
 
+// entries of a list per work item of the list-major scan
+int list_scan_chunk();
+// work-item partitions (counters) of the list-major scan
+constexpr int kListScanParts = 64;
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 // per-list pair counts, per-query entry totals and row slots (each probed
-// list's length rounded up to 4; lcnt zeroed by the caller)
+// list's length rounded up to 8; lcnt zeroed by the caller)
 void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int64_t *qtot, int64_t *qslots,
                        hipStream_t st);
 // row_off[q] = qslots[0] + ... + qslots[q - 1]
 void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hipStream_t st);
-// exclusive offsets of pairs and of 32-query items per list; totals[0] =
-// pairs, totals[1] = items; lfill is zeroed
-void launch_pair_offsets(const int32_t *lcnt, int K, int32_t *lstart, int32_t *ioff, int32_t *lfill,
-                         int32_t *totals, hipStream_t st);
-// the pairs grouped by list, with the coarse rank and the row offset of each
-void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const int32_t *lstart,
-                         const int64_t *row_off, int32_t *lfill, int32_t *pair_q, float *pair_qc, int64_t *pair_off,
-                         hipStream_t st);
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, int K, int4 *items,
-                       hipStream_t st);
+// exclusive offsets of pairs and of work items per list (ceil(pairs / 32)
+// query groups times ceil(len / chunk) entry chunks); totals[0] = pairs,
+// totals[1] = items; lfill is zeroed
+void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
+                         int32_t *lfill, int32_t *totals, hipStream_t st);
+// the pairs grouped by list, with the coarse rank, the query's key scale and
+// the row offset of each; ymax: bits of an upper bound of |y_e| over the index
+void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
+                         const int32_t *lstart, const int64_t *row_off, int32_t *lfill, int32_t *pair_q,
+                         float *pair_qc, float *pair_scale, int64_t *pair_off, hipStream_t st);
+void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, const int64_t *list_len, int K,
+                       int4 *items, hipStream_t st);
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);
@@ -56,7 +63,7 @@ int select_rows_max_probes();
 // per query, the selk = k_q * R best row entries by (key, position) as
 // positions in out_pos rows (-1 after the last); queries that need the
 // brute-force fallback go to ovf (count in *ovf_cnt, zeroed by the caller)
-void launch_select_rows(const ScanArgs &a, int64_t nq, const float *dist, const int64_t *row_off,
+void launch_select_rows(const ScanArgs &a, int64_t nq, const uint16_t *dist, const int64_t *row_off,
                         const int64_t *qtot, const int64_t *qslots, const unsigned *qrange, int64_t *out_pos,
                         int32_t *out_cnt, int out_stride, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st);
 

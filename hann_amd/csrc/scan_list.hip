@@ -6,20 +6,25 @@
 //   |q - c_l - y_e|^2 = |q - c_l|^2 + t_e - 2 <q, y_e>
 // and for a list l probed by queries Q_l the inner products are one product
 // of the decoded list [n_l x d] with Q_l^T [d x |Q_l|]. The pairs (query,
-// probed list) of a batch are grouped by list; a work item is one list with
-// up to 32 of its queries. A workgroup gathers 32 entries per wave from a
-// BF16 codebook in LDS (BF16 hi + lo for the split mode) straight into the
-// A operand of v_mfma_f32_32x32x16_bf16 and multiplies them with the item's
-// queries. Each list's codes are read once per item instead of once per
-// (query, list).
+// probed list) of a batch are grouped by list; a work item is one chunk of up
+// to 256 entries of a list with up to 32 of its queries, and belongs to one
+// wave. The wave gathers 32 entries at a time from a BF16 codebook in LDS
+// (BF16 hi + lo for the split mode) straight into the A operand of
+// v_mfma_f32_32x32x16_bf16 and multiplies them with the item's queries,
+// which stay in registers as the B operand. Each list's codes are read once
+// per item instead of once per (query, list).
 //
-// Every (query, entry) distance of the batch is written: query q owns a row
-// of its probed lists' entries, probes in order, so each pair's distances
-// land at a fixed offset and no candidate test, threshold or atomic append
-// is involved. The scan also keeps each row's smallest and largest key. A
-// select kernel then takes, per query, the selk = k * R best row entries by
-// (key, position) with a range histogram over the row (refined while the
-// boundary bin is too large to sort), and those are reranked in exact FP32.
+// Every (query, entry) distance of the batch is written as a 16-bit key:
+// query q owns a row of its probed lists' entries, probes in order, so each
+// pair's keys land at a fixed offset and no candidate test, threshold or
+// atomic append is involved. A key is the FP16 value of the distance times a
+// power of two chosen per query (so that the largest distance any of its
+// probed entries can have stays far below FP16's range), with its bits mapped
+// so that unsigned order is value order. The scan also keeps each row's
+// smallest and largest key. A select kernel then takes, per query, the
+// selk = k * R best row entries by (key, position) with a range histogram over
+// the row (refined while the boundary bin is too large to sort), and those are
+// reranked in exact FP32.
 #include "common.h"
 #include "scan_list.h"
 
@@ -31,12 +36,13 @@ namespace hann {
 namespace dev {
 
 constexpr int LQ = 32;          // queries per work item (the MFMA's N)
+constexpr int LCH = 256;        // entries per work item (a chunk of the list; a multiple of 32)
 constexpr int SEL_BINS = 2048;  // histogram bins of the row select
 constexpr int SEL_ROOM = 2048;  // boundary-bin entries the select sorts in LDS
 constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
-// row slots past a list's end: a NaN whose ordered bits (0xFFFFFFFF) exceed
-// every stored key's (keys are finite or +inf), so no key range holds it
-__device__ constexpr float kPadKey = __builtin_bit_cast(float, 0x7FFFFFFFu);
+// key of the row slots past a list's end: the ordered bits of an FP16 NaN,
+// above every stored key (stored keys are finite or +inf)
+constexpr unsigned kPadKey = 0xFFFFu;
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -50,15 +56,17 @@ __device__ __forceinline__ int clamp_nprobe(const ScanArgs &a, int64_t q) {
     return npq < 0 ? 0 : npq;
 }
 
-// ordered bits of a key: unsigned order = float order (-0 before +0); the
-// row keys are never NaN except the padding sentinel, which maps to ~0u
-__device__ __forceinline__ unsigned key_bits(float k) {
-    const unsigned u = __float_as_uint(k);
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+// 16-bit ordered key of a scaled distance: its FP16 value (round to
+// nearest even, so the map is monotone; NaN becomes +inf), bits mapped so
+// that unsigned order is value order (-0 just below +0)
+__device__ __forceinline__ unsigned key16(float v) {
+    const _Float16 h = (_Float16)fminf(v, __builtin_inff());
+    const unsigned u = (unsigned)__builtin_bit_cast(unsigned short, h);
+    return (u & 0x8000u) ? (~u & 0xFFFFu) : (u | 0x8000u);
 }
 
-// a pair's slots in its query's row: its list's length rounded up to 4
-__device__ __forceinline__ int64_t row_slots(int64_t len) { return (len + 3) & ~(int64_t)3; }
+// a pair's slots in its query's row: its list's length rounded up to 8
+__device__ __forceinline__ int64_t row_slots(int64_t len) { return (len + 7) & ~(int64_t)7; }
 
 // one wave per query: entries and row slots of its probed lists
 __global__ void __launch_bounds__(64) k_pair_count(ScanArgs a, int32_t *__restrict__ lcnt,
@@ -108,9 +116,15 @@ __global__ void __launch_bounds__(1024) k_row_offsets(const int64_t *__restrict_
     }
 }
 
+// work items of a list: query groups of 32 times entry chunks
+__device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
+    return pairs > 0 ? (int32_t)(((pairs + LQ - 1) / LQ) * ((len + LCH - 1) / LCH)) : 0;
+}
+
 // one workgroup of 1024: chunked exclusive scans of lcnt and of the item
-// counts ceil(lcnt / 32)
-__global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt, int K,
+// counts
+__global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
+                                                       const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
                                                        int32_t *__restrict__ lfill, int32_t *__restrict__ totals) {
     __shared__ int32_t s_a[1024], s_b[1024];
@@ -121,12 +135,16 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
     // 16 loads in flight per thread
     for (int l0 = b0; l0 < b1; l0 += 16) {
         int32_t v[16];
+        int64_t n[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+        for (int u = 0; u < 16; ++u) {
+            v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+            n[u] = l0 + u < b1 ? list_len[l0 + u] : 0;
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             sa += v[u];
-            sb += (v[u] + LQ - 1) / LQ;
+            sb += list_items(v[u], n[u]);
         }
     }
     s_a[t] = sa;
@@ -142,8 +160,12 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
     int32_t ra = s_a[t] - sa, rb = s_b[t] - sb;
     for (int l0 = b0; l0 < b1; l0 += 16) {
         int32_t v[16];
+        int64_t n[16];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+        for (int u = 0; u < 16; ++u) {
+            v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
+            n[u] = l0 + u < b1 ? list_len[l0 + u] : 0;
+        }
 #pragma unroll
         for (int u = 0; u < 16; ++u) {
             if (l0 + u < b1) {
@@ -152,7 +174,7 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
                 lfill[l0 + u] = 0;
             }
             ra += v[u];
-            rb += (v[u] + LQ - 1) / LQ;
+            rb += list_items(v[u], n[u]);
         }
     }
     if (t == 1023) {
@@ -162,16 +184,38 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
 }
 
 // one wave per query: its pairs (non-empty probed lists) into their lists'
-// pair ranges, each with its coarse rank and where its distances start in
-// the query's row (the probes' slot counts summed in probe order)
+// pair ranges, each with its coarse rank, the query's key scale and where
+// its keys start in the query's row (the probes' slot counts summed in
+// probe order). The scale is the power of two 2^(14 - e) with
+// (sqrt(max coarse rank) + max|y|)^2 < 2^e: no probed entry's distance
+// (nor its BF16 estimate) reaches FP16's range once scaled, and every
+// scaled distance above 2^-28 of that bound keeps FP16's full precision.
 __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *__restrict__ probe_rank,
+                                                     const unsigned *__restrict__ ymax,
                                                      const int32_t *__restrict__ lstart,
                                                      const int64_t *__restrict__ row_off, int32_t *__restrict__ lfill,
                                                      int32_t *__restrict__ pair_q, float *__restrict__ pair_qc,
-                                                     int64_t *__restrict__ pair_off) {
+                                                     float *__restrict__ pair_scale, int64_t *__restrict__ pair_off) {
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     const int npq = clamp_nprobe(a, q);
+    auto rank_of = [&](int li, int64_t l) -> float {
+        return probe_rank ? (float)probe_rank[q * a.probe_stride + li]
+                          : sqdiff_exact(PtrLoad{a.queries + q * a.d}, PtrLoad{a.coarse + l * a.d}, a.d);
+    };
+    float qcmax = 0.f;
+    for (int li = lane; li < npq; li += 64) qcmax = fmaxf(qcmax, rank_of(li, a.probes[q * a.probe_stride + li]));
+    for (int o = 32; o > 0; o >>= 1) qcmax = fmaxf(qcmax, __shfl_xor(qcmax, o));
+    const float yb = ymax ? __uint_as_float(*ymax) : 0.f;
+    const float r = __builtin_sqrtf(fmaxf(qcmax, 0.f)) + yb;
+    const float bound = r * r;
+    float scale = 1.f;
+    if (bound > 0.f && bound < __builtin_inff()) {
+        int e = 0;
+        (void)__builtin_frexpf(bound, &e);   // bound < 2^e
+        e = e < -100 ? -100 : (e > 100 ? 100 : e);
+        scale = __builtin_ldexpf(1.f, 14 - e);
+    }
     int64_t carry = row_off[q];
     for (int l0 = 0; l0 < npq; l0 += 64) {
         const int li = l0 + lane;
@@ -189,58 +233,62 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
         if (li < npq && len > 0) {
             const int pos = lstart[l] + atomicAdd(&lfill[l], 1);
             pair_q[pos] = (int32_t)q;
-            pair_qc[pos] = probe_rank ? (float)probe_rank[q * a.probe_stride + li]
-                                      : sqdiff_exact(PtrLoad{a.queries + q * a.d}, PtrLoad{a.coarse + l * a.d}, a.d);
+            pair_qc[pos] = rank_of(li, l);
+            pair_scale[pos] = scale;
             pair_off[pos] = carry + incl - slots;
         }
         carry += __shfl(incl, 63);
     }
 }
 
+// one thread per list: its query groups times its entry chunks
 __global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__restrict__ lstart,
-                             const int32_t *__restrict__ ioff, int K, int4 *__restrict__ items) {
+                             const int32_t *__restrict__ ioff, const int64_t *__restrict__ list_len, int K,
+                             int4 *__restrict__ items) {
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l >= K) return;
     const int n = lcnt[l];
+    if (n == 0) return;
+    const int nch = (int)((list_len[l] + LCH - 1) / LCH);
+    int o = ioff[l];
     for (int t = 0; t * LQ < n; ++t)
-        items[ioff[l] + t] = make_int4(l, lstart[l] + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, 0);
+        for (int c = 0; c < nch; ++c)
+            items[o++] = make_int4(l, lstart[l] + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
 }
 
-// LNW = 4 waves per workgroup; s == 4 (d = 4 M) and uint8 codes. The
-// codebook sits in LDS as BF16 (hi, and lo for the split mode) for the
-// kernel's lifetime; the MFMA's A operand of lane l for k-step j is the two
-// codewords of subquantizers 4j + 2h and 4j + 2h + 1 (h = l / 32) of entry
-// row l % 32, gathered straight from LDS into registers, and its B operand
-// (the item's queries) stays in registers for the whole item.
+// LNW waves per workgroup share the codebook in LDS; each wave runs its own
+// work items with no block barrier after the codebook is staged. s == 4
+// (d = 4 M) and uint8 codes. The codebook sits in LDS as BF16 (hi, and lo
+// for the split mode); the MFMA's A operand of lane l for k-step j is the
+// two codewords of subquantizers 4j + 2h and 4j + 2h + 1 (h = l / 32) of
+// entry l % 32 of the slab, gathered straight from LDS into registers, and
+// its B operand (column l % 32: the item's query of that rank) is loaded
+// once per item from the query vector.
 //
-// Each wave turns its 32 x 32 accumulator (entries x queries) around in an
-// LDS tile, so that its stores write each query's 32 distances as one run.
+// D register r of lane l holds entry 8 (r / 4) + 4 h + r % 4 of the slab for
+// query l % 32, so the four registers r = 4g .. 4g + 3 are four consecutive
+// entries of one query's row: each becomes one 8-byte store of four keys,
+// with no transposition. The entry terms of a slab pass through a 32-float
+// LDS slot of the wave, read back as float4 broadcasts.
 //
-// Latency: a workgroup takes its next item from the queue while it starts
-// the current one, and every thread prefetches its share of the next item
-// (its query row slice, the pair's rank and row offset, and the first code
-// slab) into registers; inside an item each wave loads its next slab's codes
-// and terms while it computes the current one.
-constexpr int LNW = 4;
-constexpr int LT = 33;   // tile row stride (floats)
+// Scheduling: items are split into partitions (item i belongs to partition
+// i mod n_parts) with a counter each, and a workgroup takes the items of
+// partition blockIdx mod n_parts, so that no single counter takes every
+// wave's atomics. Latency: the next item's index, header and pair load while
+// an item runs, and every slab's codes and terms load while the previous
+// slab computes; waves of several workgroups per CU (the codebook is the
+// only LDS) overlap each other's loads.
+constexpr int LNW = 8;
 
 template <bool X3, int M>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     extern __shared__ __align__(16) unsigned char s_raw[];
-    __shared__ int s_next, s_nitems;
-    __shared__ float s_qc[LQ];
-    __shared__ int s_qid[LQ];
-    __shared__ int64_t s_off[LQ];
-    constexpr int D = 4 * M, LD = D + 8;   // LD: Q row stride in BF16 (16-byte rows)
-    constexpr int QV = M / 8;              // float4 per thread of a 32 x D query tile (8 threads a row)
+    constexpr int D = 4 * M;
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int P = la.P;
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
     bf16x4 *sCBl = sCBh + (size_t)M * P;                          // [M][P] (split mode)
-    __bf16 *sQh = reinterpret_cast<__bf16 *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P);
-    __bf16 *sQl = sQh + LQ * LD;
-    float *s_term = reinterpret_cast<float *>(sQh + (X3 ? 2 : 1) * LQ * LD) + wave * 32;
-    float *s_tile = reinterpret_cast<float *>(sQh + (X3 ? 2 : 1) * LQ * LD) + LNW * 32 + wave * (32 * LT);
+    float *s_term = reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) + wave * 32;
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(la.codes);
 
     // the codebook, once per workgroup
@@ -261,133 +309,93 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             sCBl[t] = g;
         }
     }
-    if (tid == 0) {
-        s_nitems = *la.n_items;
-        s_next = atomicAdd(la.item_counter, 1);
-    }
     __syncthreads();
-    const int n_items = s_nitems;
+    const int n_items = *la.n_items;
     const int col = lane & 31, half = lane >> 5;
-    const int qrow = tid >> 3, qsub = tid & 7;   // this thread's slice of the query tile
-    float4 *junk = reinterpret_cast<float4 *>(la.junk) + ((int64_t)blockIdx.x * LNW + wave) * 64 + lane;
-
-    // prefetched state of the next item
-    int4 nh = make_int4(0, 0, 0, 0);
-    int nqid = -1;
-    float nqc = 0.f;
-    int64_t noff = 0;
-    float4 nqv[QV];
-    int64_t ne0 = 0, nn = 0;
-    uint32_t nw[M / 4];
-    float nt = 0.f;
-    // codes of entry b + (lane % 32) and the term of entry b + lane (lane < 32)
-    // of a list; rows past its end read the list's first entry (items only
-    // hold non-empty lists) and are discarded, so the loads are unconditional
-    auto load_codes = [&](int64_t e0, int64_t n, int64_t b, uint32_t *w, float &t) {
-        const int64_t e = b + col;
-        const uint2 *cw = reinterpret_cast<const uint2 *>(codes + (e0 + (e < n ? e : 0)) * M);
-#pragma unroll
-        for (int j = 0; j < M / 8; ++j) {
-            const uint2 v = cw[j];
-            w[2 * j] = v.x;
-            w[2 * j + 1] = v.y;
-        }
-        t = la.term[e0 + ((lane < 32 && b + lane < n) ? b + lane : 0)];
+    // work items of this workgroup's partition: part, part + nparts, ...
+    const int nparts = la.n_parts, part = (int)(blockIdx.x % (unsigned)nparts);
+    int32_t *ctr = la.item_counter + part;
+    // an item's header and this lane's pair (column col of the product)
+    struct ItemState {
+        int4 it;
+        int64_t e0, n, off;
+        int qid;
+        float qc, sc;
     };
-    auto prefetch = [&](int item) {
+    auto load_item = [&](int item, ItemState &S) {
         if (item >= n_items) return;
-        nh = la.items[item];
-        nqid = -1;
-        if (qrow < nh.z) {
-            nqid = la.pair_q[nh.y + qrow];
-            const float4 *src = reinterpret_cast<const float4 *>(la.queries + (int64_t)nqid * D) + qsub * QV;
-#pragma unroll
-            for (int v = 0; v < QV; ++v) nqv[v] = src[v];
-            if (qsub == 0) {
-                nqc = la.pair_qc[nh.y + qrow];
-                noff = la.pair_off[nh.y + qrow];
-            }
-        }
-        ne0 = la.list_start[nh.x];
-        nn = la.list_len[nh.x];
-        load_codes(ne0, nn, (int64_t)wave * 32, nw, nt);
+        S.it = la.items[item];
+        S.e0 = la.list_start[S.it.x];
+        S.n = la.list_len[S.it.x];
+        const int pidx = S.it.y + (col < S.it.z ? col : 0);
+        S.qid = la.pair_q[pidx];
+        S.qc = la.pair_qc[pidx];
+        S.sc = la.pair_scale[pidx];
+        S.off = la.pair_off[pidx];
     };
-    int item = s_next;
-    prefetch(item);
+    int j = 0;
+    if (lane == 0) j = atomicAdd(ctr, 1);
+    int item = part + __shfl(j, 0) * nparts;
+    ItemState S{}, SN{};
+    load_item(item, S);
 
-    for (;;) {
-        if (item >= n_items) break;
-        const int pc = nh.z;
-        const int64_t e0 = ne0, n = nn;
-        // the item's queries into LDS (rows >= pc zero, so that their
-        // discarded columns stay finite)
+    while (item < n_items) {
+        if (lane == 0) j = atomicAdd(ctr, 1);   // the next item, in flight with this one's loads
+        const int pc = S.it.z;
+        const int64_t e0 = S.e0, n = S.n;
+        const int64_t eb = S.it.w;
+        const int64_t ee = n < eb + LCH ? n : eb + LCH;
+        // groups of four entries starting below this are stored (the last
+        // chunk pads its query runs to a multiple of 8 slots)
+        const int64_t slot_end = ee == n ? row_slots(n) : ee;
+        const bool qv = col < pc;
+        const int qid = S.qid;
+        const float qc = S.qc, sc = S.sc;
+        const int64_t off = S.off;
+        // B operand of k-step j: the query's elements 16 j + 8 h .. + 7
+        bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
         {
-            __bf16 *dh = sQh + qrow * LD + qsub * (4 * QV);
-            __bf16 *dl = sQl + qrow * LD + qsub * (4 * QV);
+            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D);
 #pragma unroll
-            for (int v = 0; v < QV; ++v) {
-                const float4 x = qrow < pc ? nqv[v] : make_float4(0.f, 0.f, 0.f, 0.f);
-                bf16x4 h;
-                h[0] = (__bf16)x.x;
-                h[1] = (__bf16)x.y;
-                h[2] = (__bf16)x.z;
-                h[3] = (__bf16)x.w;
-                *reinterpret_cast<bf16x4 *>(dh + 4 * v) = h;
-                if (X3) {
-                    bf16x4 g;
-                    g[0] = (__bf16)(x.x - (float)h[0]);
-                    g[1] = (__bf16)(x.y - (float)h[1]);
-                    g[2] = (__bf16)(x.z - (float)h[2]);
-                    g[3] = (__bf16)(x.w - (float)h[3]);
-                    *reinterpret_cast<bf16x4 *>(dl + 4 * v) = g;
+            for (int jj = 0; jj < M / 4; ++jj) {
+                float4 x0 = qr[4 * jj + 2 * half], x1 = qr[4 * jj + 2 * half + 1];
+                if (!qv) {
+                    x0 = make_float4(0.f, 0.f, 0.f, 0.f);
+                    x1 = x0;
+                }
+                const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    bq[jj][u] = (__bf16)xs[u];
+                    if (X3) bl[jj][u] = (__bf16)(xs[u] - (float)bq[jj][u]);
                 }
             }
-            if (qsub == 0) {
-                s_qid[qrow] = qrow < pc ? nqid : -1;
-                s_qc[qrow] = qrow < pc ? nqc : 0.f;
-                s_off[qrow] = qrow < pc ? noff : 0;
+        }
+        const int next_item = part + __shfl(j, 0) * nparts;
+        load_item(next_item, SN);
+        // codes of entry b + col and its term; rows past the chunk read its
+        // last entry and are discarded, so the loads are unconditional
+        auto load_slab = [&](int64_t b, uint32_t *w, float &t) {
+            const int64_t e = b + col < ee ? b + col : ee - 1;
+            const uint2 *cw = reinterpret_cast<const uint2 *>(codes + (e0 + e) * M);
+#pragma unroll
+            for (int j = 0; j < M / 8; ++j) {
+                const uint2 v = cw[j];
+                w[2 * j] = v.x;
+                w[2 * j + 1] = v.y;
             }
-        }
-        struct Slab {
-            uint32_t w[M / 4];
-            float t;
+            t = la.term[e0 + e];
         };
-        Slab sa, sb;
-#pragma unroll
-        for (int j = 0; j < M / 4; ++j) sa.w[j] = nw[j];
-        sa.t = nt;
-        if (tid == 0) s_next = atomicAdd(la.item_counter, 1);
-        __syncthreads();
-        const int next_item = s_next;
-        prefetch(next_item);   // in flight during this item
-        // the B operands (the item's queries) for the whole item
-        bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
-#pragma unroll
-        for (int j = 0; j < M / 4; ++j) {
-            bq[j] = *reinterpret_cast<const bf16x8 *>(&sQh[col * LD + 16 * j + 8 * half]);
-            if (X3) bl[j] = *reinterpret_cast<const bf16x8 *>(&sQl[col * LD + 16 * j + 8 * half]);
-        }
-        const float qc_r = s_qc[col];
-        // this lane's queries in the store phase are 8t + lane / 8: their
-        // smallest and largest keys so far (NaN padding is ignored by min/max)
-        float lmin[4], lmax[4];
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            lmin[t] = __builtin_inff();
-            lmax[t] = -__builtin_inff();
-        }
-        // one slab: 32 entries of the list against the item's queries. Every
-        // slab issues the same memory operations (masked lanes store to the
-        // junk words), so each wait covers only what it needs.
-        auto slab = [&](const Slab &x, int64_t base) {
-            if (lane < 32) s_term[lane] = x.t;
+        unsigned kmin = 0xFFFFu, kmax = 0u;
+        auto slab = [&](const uint32_t *w, float t, int64_t base) {
+            if (half == 0) s_term[col] = t;
             f32x16 acc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 #pragma unroll
             for (int j = 0; j < M / 4; ++j) {
                 const int i0 = 4 * j + 2 * half;
-                const uint32_t c0 = (x.w[j] >> (16 * half)) & 0xFFu, c1 = (x.w[j] >> (16 * half + 8)) & 0xFFu;
+                const uint32_t c0 = (w[j] >> (16 * half)) & 0xFFu, c1 = (w[j] >> (16 * half + 8)) & 0xFFu;
                 const bf16x4 x0 = sCBh[i0 * P + c0], x1 = sCBh[(i0 + 1) * P + c1];
                 const bf16x8 a = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[j], acc, 0, 0, 0);
@@ -399,71 +407,50 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 }
             }
             wave_sync();   // s_term
-            // rows of the accumulator: entries; columns: the item's queries
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const int row = 8 * (r >> 2) + 4 * half + (r & 3);
-                // fminf maps a NaN to +inf
-                s_tile[col * LT + row] = fminf((qc_r + s_term[row]) - 2.f * acc[r], __builtin_inff());
-            }
-            wave_sync();   // the tile
-            // 16-byte runs of the row: lane -> query 8t + lane / 8, entries
-            // base + 4 (lane % 8) .. + 3; slots past the list's end get the
-            // padding sentinel (last slab only), and masked lanes store to
-            // this wave's junk
-            const bool full = base + 32 <= n;
+            for (int g = 0; g < 4; ++g) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[8 * g + 4 * half]);
+                const int64_t e4 = base + 8 * g + 4 * half;
+                const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
+                unsigned kk[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int c = 8 * t + (lane >> 3), g = lane & 7;
-                const int64_t e4 = base + 4 * g;
-                float4 v;
-                v.x = s_tile[c * LT + 4 * g + 0];
-                v.y = s_tile[c * LT + 4 * g + 1];
-                v.z = s_tile[c * LT + 4 * g + 2];
-                v.w = s_tile[c * LT + 4 * g + 3];
-                if (!full) {
-                    v.x = e4 + 0 < n ? v.x : kPadKey;
-                    v.y = e4 + 1 < n ? v.y : kPadKey;
-                    v.z = e4 + 2 < n ? v.z : kPadKey;
-                    v.w = e4 + 3 < n ? v.w : kPadKey;
+                for (int i = 0; i < 4; ++i) {
+                    const unsigned k = key16(((qc + tv[i]) - 2.f * acc[4 * g + i]) * sc);
+                    const bool ok = e4 + i < ee;
+                    kk[i] = ok ? k : kPadKey;
+                    kmin = ok ? (k < kmin ? k : kmin) : kmin;
+                    kmax = ok ? (k > kmax ? k : kmax) : kmax;
                 }
-                lmin[t] = fminf(lmin[t], fminf(fminf(v.x, v.y), fminf(v.z, v.w)));
-                lmax[t] = fmaxf(lmax[t], fmaxf(fmaxf(v.x, v.y), fmaxf(v.z, v.w)));
-                float4 *dst = (c < pc && e4 < n) ? reinterpret_cast<float4 *>(la.dist + s_off[c] + e4) : junk;
-                *dst = v;
+                if (qv && e4 < slot_end)
+                    *reinterpret_cast<uint2 *>(la.dist + off + e4) = make_uint2(kk[0] | (kk[1] << 16), kk[2] | (kk[3] << 16));
             }
-            wave_sync();   // s_term and the tile are rewritten by the next slab
+            wave_sync();   // s_term is rewritten by the next slab
         };
-        // ping-pong slab buffers: a slab's loads land in the buffer the
-        // slab after it reads, with no register copies between them
-        int64_t base = (int64_t)wave * 32;
-        while (base < n) {
-            load_codes(e0, n, base + 32 * LNW, sb.w, sb.t);
-            slab(sa, base);
-            base += 32 * LNW;
-            if (base >= n) break;
-            load_codes(e0, n, base + 32 * LNW, sa.w, sa.t);
-            slab(sb, base);
-            base += 32 * LNW;
+        // ping-pong slab buffers: a slab's loads land in the buffer the slab
+        // after it reads, with no register copies between them
+        uint32_t wa[M / 4], wb[M / 4];
+        float ta, tb;
+        load_slab(eb, wa, ta);
+        int64_t base = eb;
+        while (base < ee) {
+            load_slab(base + 32, wb, tb);
+            slab(wa, ta, base);
+            base += 32;
+            if (base >= ee) break;
+            load_slab(base + 32, wa, ta);
+            slab(wb, tb, base);
+            base += 32;
         }
-        // the row ranges: lanes 8c' .. 8c' + 7 share their queries; a zero
-        // minimum or maximum is widened to cover both signed zeros
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-            float mn = lmin[t], mx = lmax[t];
-            for (int o = 1; o < 8; o <<= 1) {
-                mn = fminf(mn, __shfl_xor(mn, o));
-                mx = fmaxf(mx, __shfl_xor(mx, o));
-            }
-            const int c = 8 * t + (lane >> 3);
-            if ((lane & 7) == 0 && c < pc && mn <= mx) {
-                const int64_t qid = s_qid[c];
-                atomicMin(&la.qrange[qid], key_bits(mn == 0.f ? -0.f : mn));
-                atomicMax(&la.qrange[la.nq + qid], key_bits(mx == 0.f ? 0.f : mx));
-            }
+        // the row's key range: lanes col and col + 32 share a query
+        const unsigned omin = __shfl_xor(kmin, 32), omax = __shfl_xor(kmax, 32);
+        kmin = omin < kmin ? omin : kmin;
+        kmax = omax > kmax ? omax : kmax;
+        if (half == 0 && qv && kmin <= kmax) {
+            atomicMin(&la.qrange[qid], kmin);
+            atomicMax(&la.qrange[la.nq + qid], kmax);
         }
-        __syncthreads();   // the next item rewrites the query tile
         item = next_item;
+        S = SN;
     }
 }
 
@@ -479,7 +466,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
 //     is resolved by a radix select over the positions.
 // Positions come from the query's probe table (row offset and first slot
 // of each probed list) by binary search.
-__global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, const float *__restrict__ dist,
+__global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, const uint16_t *__restrict__ dist,
                                                      const int64_t *__restrict__ row_off,
                                                      const int64_t *__restrict__ qtot,
                                                      const int64_t *__restrict__ qslots,
@@ -500,7 +487,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     int kq = a.k_q ? a.k_q[q] : a.k_default;
     kq = kq < 1 ? 1 : (kq > a.k_default ? a.k_default : kq);
     const int selk = kq * a.rerank;
-    const int64_t Lr = qtot[q], L = qslots[q];   // entries, row slots (a multiple of 4)
+    const int64_t Lr = qtot[q], L = qslots[q];   // entries, row slots (a multiple of 8)
     int64_t *orow = out_pos + q * out_stride;
     if (a.allow_fallback && Lr < kq) {
         // the reference's brute-force fallback: the table scan redoes it
@@ -537,7 +524,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     }
     if (tid == 0) s_out = 0;
     __syncthreads();
-    const float *row = dist + row_off[q];
+    const uint16_t *row = dist + row_off[q];
     // slot i of the row -> entry position (the last probe whose row offset is <= i)
     auto pos_of = [&](int64_t i) -> int64_t {
         int lo_j = 0, hi_j = npq - 1;
@@ -550,27 +537,29 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         }
         return pst[lo_j] + (i - poff[lo_j]);
     };
-    // fn(slot, key bits) over the row (16-byte aligned, a multiple of 4
-    // slots): four float4 loads in flight per thread
+    // fn(slot, key) over the row (16-byte aligned, a multiple of 8 slots):
+    // four 16-byte loads in flight per thread
     constexpr int SU = 4;
-    const float4 *row4 = reinterpret_cast<const float4 *>(row);
-    const int64_t L4 = L / 4;
+    const uint4 *row8 = reinterpret_cast<const uint4 *>(row);
+    const int64_t L8 = L / 8;
     auto for_row = [&](auto &&fn) {
-        for (int64_t j0 = 0; j0 < L4; j0 += 256 * SU) {
-            float4 v[SU];
+        for (int64_t j0 = 0; j0 < L8; j0 += 256 * SU) {
+            uint4 v[SU];
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
                 const int64_t j = j0 + u * 256 + tid;
-                v[u] = row4[j < L4 ? j : L4 - 1];
+                v[u] = row8[j < L8 ? j : L8 - 1];
             }
 #pragma unroll
             for (int u = 0; u < SU; ++u) {
                 const int64_t j = j0 + u * 256 + tid;
-                if (j < L4) {
-                    fn(4 * j + 0, key_bits(v[u].x));
-                    fn(4 * j + 1, key_bits(v[u].y));
-                    fn(4 * j + 2, key_bits(v[u].z));
-                    fn(4 * j + 3, key_bits(v[u].w));
+                if (j < L8) {
+                    const unsigned w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
+#pragma unroll
+                    for (int h = 0; h < 4; ++h) {
+                        fn(8 * j + 2 * h, w[h] & 0xFFFFu);
+                        fn(8 * j + 2 * h + 1, w[h] >> 16);
+                    }
                 }
             }
         }
@@ -727,27 +716,29 @@ void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hip
     HANN_LAUNCH_CHECK();
 }
 
-void launch_pair_offsets(const int32_t *lcnt, int K, int32_t *lstart, int32_t *ioff, int32_t *lfill,
-                         int32_t *totals, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_pair_offsets, dim3(1), dim3(1024), 0, st, lcnt, K, lstart, ioff, lfill, totals);
+void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
+                         int32_t *lfill, int32_t *totals, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_pair_offsets, dim3(1), dim3(1024), 0, st, lcnt, list_len, K, lstart, ioff, lfill, totals);
     HANN_LAUNCH_CHECK();
 }
 
-void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const int32_t *lstart,
-                         const int64_t *row_off, int32_t *lfill, int32_t *pair_q, float *pair_qc, int64_t *pair_off,
-                         hipStream_t st) {
+void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
+                         const int32_t *lstart, const int64_t *row_off, int32_t *lfill, int32_t *pair_q,
+                         float *pair_qc, float *pair_scale, int64_t *pair_off, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_pair_scatter, dim3((unsigned)nq), dim3(64), 0, st, a, probe_rank, lstart, row_off, lfill,
-                       pair_q, pair_qc, pair_off);
+    hipLaunchKernelGGL(dev::k_pair_scatter, dim3((unsigned)nq), dim3(64), 0, st, a, probe_rank, ymax, lstart, row_off,
+                       lfill, pair_q, pair_qc, pair_scale, pair_off);
     HANN_LAUNCH_CHECK();
 }
 
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, int K, int4 *items,
-                       hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lstart, ioff, K,
-                       items);
+void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, const int64_t *list_len, int K,
+                       int4 *items, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lstart, ioff,
+                       list_len, K, items);
     HANN_LAUNCH_CHECK();
 }
+
+int list_scan_chunk() { return dev::LCH; }
 
 bool list_scan_supported(int d, int m, int P, int code_bytes) {
     if (code_bytes != 1 || d != 4 * m) return false;
@@ -756,9 +747,9 @@ bool list_scan_supported(int d, int m, int P, int code_bytes) {
 }
 
 size_t list_scan_lds_bytes(int x3, int d, int m, int P) {
-    const int nb = x3 ? 2 : 1, ld = d + 8;
-    return (size_t)nb * m * P * 8 + (size_t)nb * dev::LQ * ld * 2 + (size_t)dev::LNW * 32 * 4 +
-           (size_t)dev::LNW * 32 * dev::LT * 4;
+    (void)d;
+    const int nb = x3 ? 2 : 1;
+    return (size_t)nb * m * P * 8 + (size_t)dev::LNW * 32 * 4;
 }
 
 int list_scan_threads() { return 64 * dev::LNW; }
@@ -797,7 +788,7 @@ void launch_scan_list(const ListScanArgs &la, int x3, int grid, hipStream_t st) 
 
 int select_rows_max_probes() { return dev::SEL_MAXP; }
 
-void launch_select_rows(const ScanArgs &a, int64_t nq, const float *dist, const int64_t *row_off,
+void launch_select_rows(const ScanArgs &a, int64_t nq, const uint16_t *dist, const int64_t *row_off,
                         const int64_t *qtot, const int64_t *qslots, const unsigned *qrange, int64_t *out_pos,
                         int32_t *out_cnt, int out_stride, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st) {
     if (nq == 0) return;
