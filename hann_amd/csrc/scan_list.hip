@@ -40,8 +40,8 @@ constexpr int LCH = 256;        // entries per work item (a chunk of the list; a
 constexpr int SEL_BINS = 2048;  // histogram bins of the row select
 constexpr int SEL_ROOM = 2048;  // boundary-bin entries the select sorts in LDS
 constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
-// key of the row slots past a list's end: the ordered bits of an FP16 NaN,
-// above every stored key (stored keys are finite or +inf)
+// key of the row slots past a list's end: the bits of an FP16 NaN, above
+// every stored key (stored keys are finite or +inf, non-negative)
 constexpr unsigned kPadKey = 0xFFFFu;
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
@@ -56,13 +56,28 @@ __device__ __forceinline__ int clamp_nprobe(const ScanArgs &a, int64_t q) {
     return npq < 0 ? 0 : npq;
 }
 
-// 16-bit ordered key of a scaled distance: its FP16 value (round to
-// nearest even, so the map is monotone; NaN becomes +inf), bits mapped so
-// that unsigned order is value order (-0 just below +0)
-__device__ __forceinline__ unsigned key16(float v) {
-    const _Float16 h = (_Float16)fminf(v, __builtin_inff());
-    const unsigned u = (unsigned)__builtin_bit_cast(unsigned short, h);
-    return (u & 0x8000u) ? (~u & 0xFFFFu) : (u | 0x8000u);
+// 16-bit keys of two scaled distances, packed: each is clamped to
+// [0, +inf] on its bits (negative values and -0 become +0, NaN becomes
+// +inf) and converted to FP16 rounding toward zero, a monotone map. The
+// FP16 bits of non-negative values order like the values as unsigned
+// integers, from 0x0000 up to +inf = 0x7C00.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float key_clamp(float v) {
+    const int b = __float_as_int(v);
+    const unsigned c = (unsigned)(b > 0 ? b : 0);
+    return __uint_as_float(c < 0x7F800000u ? c : 0x7F800000u);
+}
+__device__ __forceinline__ uint32_t key16x2(float a, float b) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(key_clamp(a), key_clamp(b)));
+}
+__device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
+}
+__device__ __forceinline__ uint32_t max_u16x2(uint32_t a, uint32_t b) {
+    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
+                                                                  __builtin_bit_cast(u16x2, b)));
 }
 
 // a pair's slots in its query's row: its list's length rounded up to 8
@@ -118,7 +133,8 @@ __global__ void __launch_bounds__(1024) k_row_offsets(const int64_t *__restrict_
 
 // work items of a list: query groups of 32 times entry chunks
 __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
-    return pairs > 0 ? (int32_t)(((pairs + LQ - 1) / LQ) * ((len + LCH - 1) / LCH)) : 0;
+    static_assert(LQ == 32 && LCH == 256, "shifts below");
+    return pairs > 0 ? ((pairs + LQ - 1) >> 5) * (int32_t)((len + LCH - 1) >> 8) : 0;
 }
 
 // one workgroup of 1024: chunked exclusive scans of lcnt and of the item
@@ -259,11 +275,17 @@ __global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__
 // LNW waves per workgroup share the codebook in LDS; each wave runs its own
 // work items with no block barrier after the codebook is staged. s == 4
 // (d = 4 M) and uint8 codes. The codebook sits in LDS as BF16 (hi, and lo
-// for the split mode); the MFMA's A operand of lane l for k-step j is the
-// two codewords of subquantizers 4j + 2h and 4j + 2h + 1 (h = l / 32) of
-// entry l % 32 of the slab, gathered straight from LDS into registers, and
-// its B operand (column l % 32: the item's query of that rank) is loaded
-// once per item from the query vector.
+// for the split mode).
+//
+// Operand layout of v_mfma_f32_32x32x16_bf16 (k-step j < M / 4, lane l,
+// h = l / 32): the contraction runs over the dimensions in the order that
+// gives half h the subquantizers h M / 2 .. (h + 1) M / 2 - 1, two per
+// k-step: k-step j, half h covers subquantizers h M / 2 + 2 j and + 1, i.e.
+// dimensions 2 h M + 8 j .. + 7. So a lane needs one contiguous half of its
+// entry's code row (M / 2 bytes, one load), and the B operand (column
+// l % 32: the item's query of that rank) is eight contiguous floats of the
+// query per k-step, loaded once per item. The A operand of lane l is the
+// two codewords of entry l % 32 of the slab, gathered straight from LDS.
 //
 // D register r of lane l holds entry 8 (r / 4) + 4 h + r % 4 of the slab for
 // query l % 32, so the four registers r = 4g .. 4g + 3 are four consecutive
@@ -275,15 +297,17 @@ __global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__
 // i mod n_parts) with a counter each, and a workgroup takes the items of
 // partition blockIdx mod n_parts, so that no single counter takes every
 // wave's atomics. Latency: the next item's index, header and pair load while
-// an item runs, and every slab's codes and terms load while the previous
-// slab computes; waves of several workgroups per CU (the codebook is the
-// only LDS) overlap each other's loads.
+// an item runs; every slab's codes and terms load two slabs ahead through a
+// ring of three register buffers, and the ring runs on into the next item's
+// first slabs; waves of several workgroups per CU (the codebook is the only
+// LDS) overlap each other's loads.
 constexpr int LNW = 8;
 
 template <bool X3, int M>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     constexpr int D = 4 * M;
+    constexpr int HW = M / 8;   // 32-bit words of half a code row
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int P = la.P;
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
@@ -315,50 +339,79 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     // work items of this workgroup's partition: part, part + nparts, ...
     const int nparts = la.n_parts, part = (int)(blockIdx.x % (unsigned)nparts);
     int32_t *ctr = la.item_counter + part;
-    // an item's header and this lane's pair (column col of the product)
+    // an item's header and this lane's pair (column col of the product);
+    // entry positions and row offsets fit in 32 bits (the host checks)
     struct ItemState {
-        int4 it;
-        int64_t e0, n, off;
-        int qid;
+        int e0, eb, ee, slot_end, ns, pc, qid, off;
         float qc, sc;
     };
-    auto load_item = [&](int item, ItemState &S) {
-        if (item >= n_items) return;
-        S.it = la.items[item];
-        S.e0 = la.list_start[S.it.x];
-        S.n = la.list_len[S.it.x];
-        const int pidx = S.it.y + (col < S.it.z ? col : 0);
+    auto load_item = [&](int item) -> ItemState {
+        ItemState S{0, 0, 0, 0, 0, 0, 0, 0, 0.f, 0.f};
+        if (item >= n_items) return S;
+        const int4 it = la.items[item];
+        S.e0 = (int)la.list_start[it.x];
+        const int n = (int)la.list_len[it.x];
+        S.eb = it.w;
+        S.ee = n < it.w + LCH ? n : it.w + LCH;
+        // groups of four entries starting below slot_end are stored (the
+        // last chunk pads its query runs to a multiple of 8 slots)
+        S.slot_end = S.ee == n ? (int)row_slots(n) : S.ee;
+        S.ns = (S.ee - it.w + 31) >> 5;
+        S.pc = it.z;
+        const int pidx = it.y + (col < it.z ? col : 0);
         S.qid = la.pair_q[pidx];
         S.qc = la.pair_qc[pidx];
         S.sc = la.pair_scale[pidx];
-        S.off = la.pair_off[pidx];
+        S.off = (int)la.pair_off[pidx];
+        return S;
+    };
+    // one slab's loads: this lane's half code row of entry b + col and
+    // that entry's term (rows past the chunk read its last entry and are
+    // discarded, so the loads are unconditional)
+    struct Slab {
+        uint32_t w[HW];
+        float t;
+    };
+    auto load_slab = [&](int e0, int b, int ee, Slab &x) {
+        const int e = b + col < ee ? b + col : ee - 1;
+        const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes + (int64_t)(e0 + e) * M + half * (M / 2));
+#pragma unroll
+        for (int u = 0; u < HW; ++u) x.w[u] = cw[u];
+        x.t = la.term[e0 + e];
     };
     int j = 0;
     if (lane == 0) j = atomicAdd(ctr, 1);
     int item = part + __shfl(j, 0) * nparts;
-    ItemState S{}, SN{};
-    load_item(item, S);
+    ItemState S = load_item(item), SN = S;
+    // slab s of the stream that runs through the current item and on into
+    // the next (past both: a discarded reload of the current item's last)
+    auto load_at = [&](int s, Slab &x) {
+        const bool cur = s < S.ns, nxt = !cur && s - S.ns < SN.ns;
+        const int e0 = nxt ? SN.e0 : S.e0;
+        const int b = cur ? S.eb + 32 * s : (nxt ? SN.eb + 32 * (s - S.ns) : S.ee - 1);
+        const int ee = nxt ? SN.ee : S.ee;
+        load_slab(e0, b, ee, x);
+    };
+    Slab b0, b1, b2;
+    if (item < n_items) {
+        load_at(0, b0);
+        load_at(1, b1);
+    }
 
     while (item < n_items) {
         if (lane == 0) j = atomicAdd(ctr, 1);   // the next item, in flight with this one's loads
-        const int pc = S.it.z;
-        const int64_t e0 = S.e0, n = S.n;
-        const int64_t eb = S.it.w;
-        const int64_t ee = n < eb + LCH ? n : eb + LCH;
-        // groups of four entries starting below this are stored (the last
-        // chunk pads its query runs to a multiple of 8 slots)
-        const int64_t slot_end = ee == n ? row_slots(n) : ee;
+        const int pc = S.pc, eb = S.eb, ee = S.ee, slot_end = S.slot_end;
         const bool qv = col < pc;
         const int qid = S.qid;
         const float qc = S.qc, sc = S.sc;
-        const int64_t off = S.off;
-        // B operand of k-step j: the query's elements 16 j + 8 h .. + 7
+        const int off = S.off;
+        // B operand of k-step jj: the query's dimensions 2 h M + 8 jj .. + 7
         bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
         {
-            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D);
+            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
 #pragma unroll
             for (int jj = 0; jj < M / 4; ++jj) {
-                float4 x0 = qr[4 * jj + 2 * half], x1 = qr[4 * jj + 2 * half + 1];
+                float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
                 if (!qv) {
                     x0 = make_float4(0.f, 0.f, 0.f, 0.f);
                     x1 = x0;
@@ -372,76 +425,89 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             }
         }
         const int next_item = part + __shfl(j, 0) * nparts;
-        load_item(next_item, SN);
-        // codes of entry b + col and its term; rows past the chunk read its
-        // last entry and are discarded, so the loads are unconditional
-        auto load_slab = [&](int64_t b, uint32_t *w, float &t) {
-            const int64_t e = b + col < ee ? b + col : ee - 1;
-            const uint2 *cw = reinterpret_cast<const uint2 *>(codes + (e0 + e) * M);
-#pragma unroll
-            for (int j = 0; j < M / 8; ++j) {
-                const uint2 v = cw[j];
-                w[2 * j] = v.x;
-                w[2 * j + 1] = v.y;
-            }
-            t = la.term[e0 + e];
-        };
-        unsigned kmin = 0xFFFFu, kmax = 0u;
-        auto slab = [&](const uint32_t *w, float t, int64_t base) {
-            if (half == 0) s_term[col] = t;
+        SN = load_item(next_item);
+        // key = ((qc + t) - 2 <q, y>) * sc, as fma(-2 sc, acc, t sc + qc sc)
+        const float qcs = qc * sc, m2sc = -2.f * sc;
+        // smallest and largest key of this lane's query, two halves each
+        uint32_t kmin2 = 0xFFFFFFFFu, kmax2 = 0u;
+        auto slab = [&](const Slab &x, int s) {
+            const int base = eb + 32 * s;
+            if (half == 0) s_term[col] = x.t;
             f32x16 acc;
 #pragma unroll
             for (int r = 0; r < 16; ++r) acc[r] = 0.f;
 #pragma unroll
-            for (int j = 0; j < M / 4; ++j) {
-                const int i0 = 4 * j + 2 * half;
-                const uint32_t c0 = (w[j] >> (16 * half)) & 0xFFu, c1 = (w[j] >> (16 * half + 8)) & 0xFFu;
+            for (int jj = 0; jj < M / 4; ++jj) {
+                const int i0 = half * (M / 2) + 2 * jj;
+                const uint32_t wv = x.w[jj >> 1] >> (16 * (jj & 1));
+                const uint32_t c0 = wv & 0xFFu, c1 = (wv >> 8) & 0xFFu;
                 const bf16x4 x0 = sCBh[i0 * P + c0], x1 = sCBh[(i0 + 1) * P + c1];
                 const bf16x8 a = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[j], acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[jj], acc, 0, 0, 0);
                 if (X3) {
                     const bf16x4 y0 = sCBl[i0 * P + c0], y1 = sCBl[(i0 + 1) * P + c1];
                     const bf16x8 al = {y0[0], y0[1], y0[2], y0[3], y1[0], y1[1], y1[2], y1[3]};
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq[j], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl[j], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq[jj], acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl[jj], acc, 0, 0, 0);
                 }
             }
             wave_sync();   // s_term
+            // every entry of the slab lies in the chunk except in its last
+            // slab (a wave-uniform test)
+            const bool full = base + 32 <= ee;
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[8 * g + 4 * half]);
-                const int64_t e4 = base + 8 * g + 4 * half;
-                const float tv[4] = {t4.x, t4.y, t4.z, t4.w};
-                unsigned kk[4];
-#pragma unroll
-                for (int i = 0; i < 4; ++i) {
-                    const unsigned k = key16(((qc + tv[i]) - 2.f * acc[4 * g + i]) * sc);
-                    const bool ok = e4 + i < ee;
-                    kk[i] = ok ? k : kPadKey;
-                    kmin = ok ? (k < kmin ? k : kmin) : kmin;
-                    kmax = ok ? (k > kmax ? k : kmax) : kmax;
+                const int e4 = base + 8 * g + 4 * half;
+                uint32_t p0 = key16x2(__builtin_fmaf(m2sc, acc[4 * g + 0], __builtin_fmaf(t4.x, sc, qcs)),
+                                      __builtin_fmaf(m2sc, acc[4 * g + 1], __builtin_fmaf(t4.y, sc, qcs)));
+                uint32_t p1 = key16x2(__builtin_fmaf(m2sc, acc[4 * g + 2], __builtin_fmaf(t4.z, sc, qcs)),
+                                      __builtin_fmaf(m2sc, acc[4 * g + 3], __builtin_fmaf(t4.w, sc, qcs)));
+                if (full) {
+                    kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
+                    kmax2 = max_u16x2(kmax2, max_u16x2(p0, p1));
+                } else {
+                    // entries past the chunk: the padding key, and no part
+                    // in the range
+                    const uint32_t v0 = (e4 + 0 < ee ? 0x0000FFFFu : 0u) | (e4 + 1 < ee ? 0xFFFF0000u : 0u);
+                    const uint32_t v1 = (e4 + 2 < ee ? 0x0000FFFFu : 0u) | (e4 + 3 < ee ? 0xFFFF0000u : 0u);
+                    p0 = (p0 & v0) | ~v0;
+                    p1 = (p1 & v1) | ~v1;
+                    kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
+                    kmax2 = max_u16x2(kmax2, max_u16x2(p0 & v0, p1 & v1));
                 }
-                if (qv && e4 < slot_end)
-                    *reinterpret_cast<uint2 *>(la.dist + off + e4) = make_uint2(kk[0] | (kk[1] << 16), kk[2] | (kk[3] << 16));
+                if (qv && e4 < slot_end) *reinterpret_cast<uint2 *>(la.dist + (uint32_t)(off + e4)) = make_uint2(p0, p1);
             }
             wave_sync();   // s_term is rewritten by the next slab
         };
-        // ping-pong slab buffers: a slab's loads land in the buffer the slab
-        // after it reads, with no register copies between them
-        uint32_t wa[M / 4], wb[M / 4];
-        float ta, tb;
-        load_slab(eb, wa, ta);
-        int64_t base = eb;
-        while (base < ee) {
-            load_slab(base + 32, wb, tb);
-            slab(wa, ta, base);
-            base += 32;
-            if (base >= ee) break;
-            load_slab(base + 32, wa, ta);
-            slab(wb, tb, base);
-            base += 32;
+        // the ring: on entry b0 holds slab 0 and b1 slab 1; each step loads
+        // slab s + 2 into the free buffer and computes slab s; on exit the
+        // next item's slabs 0 and 1 are moved into b0 and b1
+        const int ns = S.ns;
+        int s = 0;
+        for (;;) {
+            load_at(s + 2, b2);
+            slab(b0, s);
+            if (++s == ns) {
+                b0 = b1;
+                b1 = b2;
+                break;
+            }
+            load_at(s + 2, b0);
+            slab(b1, s);
+            if (++s == ns) {
+                b1 = b0;
+                b0 = b2;
+                break;
+            }
+            load_at(s + 2, b1);
+            slab(b2, s);
+            if (++s == ns) break;
         }
         // the row's key range: lanes col and col + 32 share a query
+        unsigned kmin = kmin2 & 0xFFFFu, kmax = kmax2 & 0xFFFFu;
+        kmin = (kmin2 >> 16) < kmin ? (kmin2 >> 16) : kmin;
+        kmax = (kmax2 >> 16) > kmax ? (kmax2 >> 16) : kmax;
         const unsigned omin = __shfl_xor(kmin, 32), omax = __shfl_xor(kmax, 32);
         kmin = omin < kmin ? omin : kmin;
         kmax = omax > kmax ? omax : kmax;
