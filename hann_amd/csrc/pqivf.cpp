@@ -1041,6 +1041,8 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         mode = SCAN_MODE_ADC_F32;
     }
     const bool fused = (p.fused || list_scan) && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
+    last_scan_mode = list_scan ? p.scan_mode : mode;
+    last_scan_fused = (fused && !list_scan) ? 1 : 0;
     auto scan = [&](const ScanArgs &args) {
         if (list_scan)
             run_list_scan(args, nq, x3, np, probe_rank, st, w, rows_max);
@@ -1773,6 +1775,15 @@ int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms) {
     x.resolve_timers();
     *kernel_ms = x.t_kernel_ms;
     x.t_kernel_ms = 0;
+    HANN_API_END
+}
+
+int hann_pqivf_last_scan(hann_pqivf *idx, int *mode, int *fused) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    *mode = x.last_scan_mode.load();
+    *fused = x.last_scan_fused.load();
     HANN_API_END
 }
 

@@ -117,6 +117,9 @@ struct PQIVF {
     DevBuf<float> d_term;
     DevBuf<int64_t> d_term_lists;
     DevBuf<unsigned> d_ymax_tmax;   // max |y_e|, max |t_e| (float bits) over the entries
+    // the scan the last search ran (searches may run concurrently under
+    // the shared lock, so these are atomics)
+    std::atomic<int> last_scan_mode{-1}, last_scan_fused{0};
     void refresh_terms(const std::vector<int64_t> *lists);   // null: every list
     // the coarse step's BF16 candidate pass (used under the index lock) and
     // the centroids' BF16 copy (rebuilt after a full sync)

@@ -7,7 +7,7 @@
 // and for a list l probed by queries Q_l the inner products are one product
 // of the decoded list [n_l x d] with Q_l^T [d x |Q_l|]. The pairs (query,
 // probed list) of a batch are grouped by list; a work item is one chunk of up
-// to 256 entries of a list with up to 32 of its queries, and belongs to one
+// to 1,024 entries of a list with up to 32 of its queries, and belongs to one
 // wave. The wave gathers 32 entries at a time from a BF16 codebook in LDS
 // (BF16 hi + lo for the split mode) straight into the A operand of
 // v_mfma_f32_32x32x16_bf16 and multiplies them with the item's queries,
@@ -36,10 +36,12 @@ namespace hann {
 namespace dev {
 
 constexpr int LQ = 32;          // queries per work item (the MFMA's N)
-constexpr int LCH = 256;        // entries per work item (a chunk of the list; a multiple of 32)
+constexpr int LCH_SHIFT = 10;
+constexpr int LCH = 1 << LCH_SHIFT;   // entries per work item (a chunk of the list; a multiple of 32)
 constexpr int SEL_BINS = 2048;  // histogram bins of the row select
 constexpr int SEL_ROOM = 2048;  // boundary-bin entries the select sorts in LDS
 constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
+constexpr int SEL_SAMP = 8;     // the select's sample: every 8th 128-byte block of a row
 // key of the row slots past a list's end: the bits of an FP16 NaN, above
 // every stored key (stored keys are finite or +inf, non-negative)
 constexpr unsigned kPadKey = 0xFFFFu;
@@ -133,8 +135,8 @@ __global__ void __launch_bounds__(1024) k_row_offsets(const int64_t *__restrict_
 
 // work items of a list: query groups of 32 times entry chunks
 __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
-    static_assert(LQ == 32 && LCH == 256, "shifts below");
-    return pairs > 0 ? ((pairs + LQ - 1) >> 5) * (int32_t)((len + LCH - 1) >> 8) : 0;
+    static_assert(LQ == 32 && LCH >= 32, "shifts below");
+    return pairs > 0 ? ((pairs + LQ - 1) >> 5) * (int32_t)((len + LCH - 1) >> LCH_SHIFT) : 0;
 }
 
 // one workgroup of 1024: chunked exclusive scans of lcnt and of the item
@@ -426,6 +428,10 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         }
         const int next_item = part + __shfl(j, 0) * nparts;
         SN = load_item(next_item);
+        // b1 holds stream slab 1: this item's slab 1, or, for a one-slab
+        // item, the next item's slab 0, which was not known when the ring
+        // was filled
+        if (S.ns == 1) load_at(1, b1);
         // key = ((qc + t) - 2 <q, y>) * sc, as fma(-2 sc, acc, t sc + qc sc)
         const float qcs = qc * sc, m2sc = -2.f * sc;
         // smallest and largest key of this lane's query, two halves each
@@ -480,9 +486,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             }
             wave_sync();   // s_term is rewritten by the next slab
         };
-        // the ring: on entry b0 holds slab 0 and b1 slab 1; each step loads
-        // slab s + 2 into the free buffer and computes slab s; on exit the
-        // next item's slabs 0 and 1 are moved into b0 and b1
+        // the ring: b0 holds stream slab 0 and b1 stream slab 1; each step
+        // loads slab s + 2 into the free buffer and computes slab s; on exit
+        // the next item's slabs 0 and 1 are moved into b0 and b1
         const int ns = S.ns;
         int s = 0;
         for (;;) {
@@ -546,7 +552,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     int64_t *bpos = reinterpret_cast<int64_t *>(bkey + SEL_ROOM);   // [SEL_ROOM]
     int64_t *poff = bpos + SEL_ROOM;                            // [probe_cap + 1]
     int64_t *pst = poff + a.probe_cap + 1;                      // [probe_cap]
-    __shared__ int s_out, s_fill, s_B, s_below, s_nb;
+    __shared__ int s_out, s_fill, s_B, s_below, s_nb, s_tot;
     __shared__ unsigned s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
@@ -631,6 +637,123 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         }
     };
     unsigned lo = qrange[q], hi = qrange[nq + q];
+    // sort the nb (key, position) pairs in LDS and emit the first need
+    // after the s_out already emitted
+    auto sort_emit = [&](int nb, int need) {
+        __syncthreads();
+        int span = 1;
+        while (span < nb) span <<= 1;
+        for (int t = nb + tid; t < span; t += 256) {
+            bkey[t] = 0xFFFFFFFFu;
+            bpos[t] = INT64_MAX;
+        }
+        __syncthreads();
+        for (int size = 2; size <= span; size <<= 1) {
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = tid; t < (span >> 1); t += 256) {
+                    const int i0 = 2 * t - (t & (stride - 1));
+                    const int i1 = i0 + stride;
+                    const bool up = (i0 & size) == 0;
+                    const unsigned ka = bkey[i0], kb = bkey[i1];
+                    const int64_t pa = bpos[i0], pb = bpos[i1];
+                    const bool b_less = kb < ka || (kb == ka && pb < pa);
+                    const bool a_less = ka < kb || (ka == kb && pa < pb);
+                    if (up ? b_less : a_less) {
+                        bkey[i0] = kb;
+                        bkey[i1] = ka;
+                        bpos[i0] = pb;
+                        bpos[i1] = pa;
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const int o0 = s_out;
+        for (int t = tid; t < need; t += 256) orow[o0 + t] = bpos[t];
+    };
+    // the bin of a SEL_BINS histogram over [lo, hi] (shift sh) where the
+    // cumulative count reaches need: s_B, with s_below entries before it
+    // and s_nb in it (wave 0: 32 bins a lane); s_tot = the histogram's total
+    auto find_bin = [&](unsigned need) {
+        if (wave == 0) {
+            unsigned own = 0;
+            for (int b = 0; b < SEL_BINS / 64; ++b) own += hist[lane * (SEL_BINS / 64) + b];
+            unsigned incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            const unsigned excl = incl - own;
+            if (excl < need && need <= incl) {
+                unsigned run = excl;
+                int b = lane * (SEL_BINS / 64);
+                while (run + hist[b] < need) run += hist[b++];
+                s_B = b;
+                s_below = (int)run;
+                s_nb = (int)hist[b];
+            }
+            if (lane == 63) s_tot = (int)incl;
+        }
+        __syncthreads();
+    };
+    // Sampled threshold: a histogram of every SEL_SAMP-th 128-byte block of
+    // the row puts an estimate T above the sk-th smallest key (with margin);
+    // one pass then gathers every key <= T into LDS. If at least sk keys
+    // (and at most SEL_ROOM) are <= T they hold the sk best by (key,
+    // position), which the sort picks; otherwise the exact method below
+    // runs. Either way the selection is the same.
+    if (2 * sk <= SEL_ROOM && L8 >= 4 * SEL_SAMP && lo <= hi) {
+        const unsigned width = hi - lo;
+        int sh = 0;
+        while ((width >> sh) >= (unsigned)SEL_BINS) ++sh;
+        for (int b = tid; b < SEL_BINS; b += 256) hist[b] = 0;
+        if (tid == 0) {
+            s_B = SEL_BINS - 1;
+            s_tot = 0;
+        }
+        __syncthreads();
+        // whole 128-byte blocks (8 groups), every SEL_SAMP-th block
+        for (int64_t j = (int64_t)(tid >> 3) * 8 * SEL_SAMP + (tid & 7); j < L8; j += 32 * 8 * SEL_SAMP) {
+            const uint4 v = row8[j];
+            const unsigned w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int h = 0; h < 4; ++h) {
+                const unsigned u0 = w[h] & 0xFFFFu, u1 = w[h] >> 16;
+                if (u0 >= lo && u0 <= hi) atomicAdd(&hist[(u0 - lo) >> sh], 1u);
+                if (u1 >= lo && u1 <= hi) atomicAdd(&hist[(u1 - lo) >> sh], 1u);
+            }
+        }
+        __syncthreads();
+        // the sample holds about tot / Lr of the row's entries: aim at 1.5
+        // times the sk-th's expected rank in it, plus 16
+        unsigned tot = 0;
+        for (int b = lane; b < SEL_BINS; b += 64) tot += hist[b];
+        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
+        const uint64_t want = ((uint64_t)3 * sk * tot + 2 * (uint64_t)Lr - 1) / (2 * (uint64_t)Lr) + 16;
+        find_bin((unsigned)(want < tot ? want : tot));
+        if (tot > 0) {
+            const unsigned bspan = (sh >= 32) ? 0xFFFFFFFFu : ((1u << sh) - 1u);
+            const unsigned tlo = lo + ((unsigned)s_B << sh);
+            const unsigned thr = (hi - tlo) < bspan ? hi : tlo + bspan;
+            if (tid == 0) s_fill = 0;
+            __syncthreads();
+            for_row([&](int64_t i, unsigned u) {
+                if (u > thr) return;
+                const int slot = atomicAdd(&s_fill, 1);
+                if (slot < SEL_ROOM) {
+                    bkey[slot] = u;
+                    bpos[slot] = pos_of(i);
+                }
+            });
+            __syncthreads();
+            const int nb = s_fill;
+            if (nb >= sk && nb <= SEL_ROOM) {
+                sort_emit(nb, sk);
+                return;
+            }
+        }
+        __syncthreads();
+    }
     int need = sk;
     for (;;) {
         const unsigned width = hi - lo;
@@ -642,26 +765,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
             if (u >= lo && u <= hi) atomicAdd(&hist[(u - lo) >> sh], 1u);
         });
         __syncthreads();
-        // the bin where the cumulative count reaches need (wave 0: 32 bins a lane)
-        if (wave == 0) {
-            unsigned own = 0;
-            for (int b = 0; b < SEL_BINS / 64; ++b) own += hist[lane * (SEL_BINS / 64) + b];
-            unsigned incl = own;
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            const unsigned excl = incl - own;
-            if (excl < (unsigned)need && (unsigned)need <= incl) {
-                unsigned run = excl;
-                int b = lane * (SEL_BINS / 64);
-                while (run + hist[b] < (unsigned)need) run += hist[b++];
-                s_B = b;
-                s_below = (int)run;
-                s_nb = (int)hist[b];
-            }
-        }
-        __syncthreads();
+        find_bin((unsigned)need);
         const int B = s_B, below = s_below, nb = s_nb;
         const bool fits = nb <= SEL_ROOM;
         if (tid == 0) s_fill = 0;
@@ -687,36 +791,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         const unsigned bhi = (hi - blo) < bspan ? hi : blo + bspan;
         if (fits) {
             // sort bin B by (key bits, position); the first need complete the row
-            __syncthreads();
-            int span = 1;
-            while (span < nb) span <<= 1;
-            for (int t = nb + tid; t < span; t += 256) {
-                bkey[t] = 0xFFFFFFFFu;
-                bpos[t] = INT64_MAX;
-            }
-            __syncthreads();
-            for (int size = 2; size <= span; size <<= 1) {
-                for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                    for (int t = tid; t < (span >> 1); t += 256) {
-                        const int i0 = 2 * t - (t & (stride - 1));
-                        const int i1 = i0 + stride;
-                        const bool up = (i0 & size) == 0;
-                        const unsigned ka = bkey[i0], kb = bkey[i1];
-                        const int64_t pa = bpos[i0], pb = bpos[i1];
-                        const bool b_less = kb < ka || (kb == ka && pb < pa);
-                        const bool a_less = ka < kb || (ka == kb && pa < pb);
-                        if (up ? b_less : a_less) {
-                            bkey[i0] = kb;
-                            bkey[i1] = ka;
-                            bpos[i0] = pb;
-                            bpos[i1] = pa;
-                        }
-                    }
-                    __syncthreads();
-                }
-            }
-            const int o0 = s_out;
-            for (int t = tid; t < need; t += 256) orow[o0 + t] = bpos[t];
+            sort_emit(nb, need);
             return;
         }
         if (blo == bhi) {

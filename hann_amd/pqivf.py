@@ -241,6 +241,14 @@ class Index:
                                                     ctypes.byref(n)))
         return a.value, b.value, c.value, n.value
 
+    def last_scan(self):
+        """(mode, fused) of the scan the last search ran: a SCAN_* mode (the
+        list modes only when the list-major scan ran) and whether the
+        persistent table scan ran."""
+        m, f = ctypes.c_int(), ctypes.c_int()
+        _lib.check(self._lib.hann_pqivf_last_scan(self._h, ctypes.byref(m), ctypes.byref(f)))
+        return m.value, f.value
+
     def scan_kernel_time(self):
         """Milliseconds in the main scan kernel over the profiled searches
         (call after stage_times)."""

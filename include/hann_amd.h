@@ -246,6 +246,11 @@ int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, 
  * events on the search stream; resets the total. Call after
  * hann_pqivf_stage_times, which resolves the events (build extension). */
 int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms);
+/* The scan the last search on this index ran (build extension): a
+ * HANN_SCAN_* mode (the list modes only when the list-major scan ran; it
+ * falls back to the FP32 table scan for shapes it does not cover) and
+ * whether the persistent table scan ran (fused). */
+int hann_pqivf_last_scan(hann_pqivf *idx, int *mode, int *fused);
 
 /* Bulk export of the clustered entries in list order (CSR): list_off
  * [k+1], ids [N], codes [N][m] (rows of -1: entry without codes), vecs

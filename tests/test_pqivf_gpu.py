@@ -636,11 +636,17 @@ def list_index(n=20000, coarse_k=64, seed=4):
     from hann_amd import synthetic
     pq = P()
     x, qs = synthetic.config1(n=n, q=300)
-    idx = pq.New(64, pq.WithCoarseK(coarse_k), pq.WithNumSubquantizers(8), pq.WithPQK(256),
+    # 16 subquantizers of 4 dimensions: the shape the list-major scan covers
+    idx = pq.New(64, pq.WithCoarseK(coarse_k), pq.WithNumSubquantizers(16), pq.WithPQK(256),
                  pq.WithKMeansIters(5), pq.WithSeed(seed))
     idx.bulk_add_arrays(np.arange(n), x)
     idx.Train()
     return idx, x, qs
+
+
+def ran_list_scan(idx, mode):
+    """The last search ran the list-major scan in this mode."""
+    return idx.last_scan()[0] == mode
 
 
 @pytest.mark.parametrize("mode,rerank", [(3, 4), (4, 4), (4, 2)])
@@ -657,6 +663,7 @@ def test_list_scan_matches_table_scan(hann, oracle, mode, rerank):
     nps = rng.choice([1, 4, 8, 16, 64], 300).astype(np.int32)
     a_ids, a_d, a_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_ADC_F32, rerank=rerank, fused=1)
     b_ids, b_d, b_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=rerank, fused=1)
+    assert ran_list_scan(idx, mode)
     assert np.array_equal(a_c, b_c)
     same = 0
     for qi in range(300):
@@ -680,6 +687,7 @@ def test_list_scan_small_lists(hann):
     for nprobe in (16, 64):
         a = idx.search_batch(qs, 50, nprobe=nprobe, scan_mode=_lib.SCAN_ADC_F32, rerank=4, fused=1)
         b = idx.search_batch(qs, 50, nprobe=nprobe, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
+        assert ran_list_scan(idx, _lib.SCAN_LIST_BF16X3)
         assert np.array_equal(a[2], b[2])
         agree = np.mean([len(set(a[0][q].tolist()) & set(b[0][q].tolist())) / 50 for q in range(len(qs))])
         assert agree >= 0.99, (nprobe, agree)
@@ -697,8 +705,43 @@ def test_list_scan_rows_within_selection_equal_table_scan(hann, mode):
     from hann_amd import _lib
     a = idx.search_batch(qs, ks, nprobe=2, scan_mode=_lib.SCAN_ADC_F32, rerank=16, fused=1)
     b = idx.search_batch(qs, ks, nprobe=2, scan_mode=mode, rerank=16, fused=1)
+    assert ran_list_scan(idx, mode)
     for u, v in zip(a, b):
         assert np.array_equal(u, v)
+
+
+@pytest.fixture(scope="module")
+def tiny_lists_index():
+    """200,000 vectors in 16,384 lists (about 12 entries each) and 4,096
+    queries: about 16,000 work items of one 32-entry slab, more than the
+    list-major scan has waves, so each wave runs several in a row."""
+    from hann_amd import synthetic
+    pq = P()
+    x, qs = synthetic.config1(n=200000, q=4096)
+    idx = pq.New(64, pq.WithCoarseK(16384), pq.WithNumSubquantizers(16), pq.WithPQK(256),
+                 pq.WithKMeansIters(3), pq.WithSeed(6))
+    idx.bulk_add_arrays(np.arange(len(x)), x)
+    idx.Train()
+    return idx, qs
+
+
+@pytest.mark.parametrize("mode,floor", [(3, 0.97), (4, 0.99)])
+def test_list_scan_one_slab_lists(hann, tiny_lists_index, mode, floor):
+    """Lists of a few dozen entries at most: most work items hold a single
+    32-entry slab, and a wave's consecutive items hand their prefetched
+    slabs on to each other. With k * R well below the probed entries the
+    keys decide the selection, which must agree with the FP32 table scan's
+    (up to near-ties of the BF16 products)."""
+    from hann_amd import _lib
+    idx, qs = tiny_lists_index
+    lens = [len(v) for v in P().InvertedIDs(idx).values()]
+    assert np.median(lens) <= 32
+    a = idx.search_batch(qs, 10, nprobe=32, scan_mode=_lib.SCAN_ADC_F32, rerank=2, fused=1)
+    b = idx.search_batch(qs, 10, nprobe=32, scan_mode=mode, rerank=2, fused=1)
+    assert ran_list_scan(idx, mode)
+    assert np.array_equal(a[2], b[2])
+    agree = np.mean([len(set(a[0][q].tolist()) & set(b[0][q].tolist())) / 10 for q in range(len(qs))])
+    assert agree >= floor, agree
 
 
 def test_list_scan_exact_ties_select_by_position(hann):
@@ -711,7 +754,7 @@ def test_list_scan_exact_ties_select_by_position(hann):
     x = rng.standard_normal((3000, 64)).astype(np.float32)
     dup = np.repeat(rng.standard_normal((1, 64)).astype(np.float32), 6000, axis=0)
     data = np.concatenate([x, dup])
-    idx = pq.New(64, pq.WithCoarseK(8), pq.WithNumSubquantizers(8), pq.WithPQK(64), pq.WithKMeansIters(3),
+    idx = pq.New(64, pq.WithCoarseK(8), pq.WithNumSubquantizers(16), pq.WithPQK(64), pq.WithKMeansIters(3),
                  pq.WithSeed(2))
     idx.bulk_add_arrays(np.arange(len(data)), data)
     idx.Train()
@@ -719,6 +762,7 @@ def test_list_scan_exact_ties_select_by_position(hann):
     for mode in (3, 4):
         a = idx.search_batch(qs, 10, nprobe=2, scan_mode=_lib.SCAN_ADC_F32, rerank=4, fused=1)
         b = idx.search_batch(qs, 10, nprobe=2, scan_mode=mode, rerank=4, fused=1)
+        assert ran_list_scan(idx, mode)
         for u, v in zip(a, b):
             assert np.array_equal(u, v)
 
@@ -727,13 +771,18 @@ def test_list_scan_fallback_counter(hann):
     """The brute-force fallback (fewer than k probed entries) keeps the
     reference's rule and counter under the list-major scan."""
     from hann_amd import _lib
-    idx = new_small(4, 8, 2, 2, 5)
-    for i in range(40):
-        idx.Add(i, [i, i + 1, i + 2, i + 3])
+    pq = P()
+    idx = pq.New(32, pq.WithCoarseK(4), pq.WithNumSubquantizers(8), pq.WithPQK(8), pq.WithKMeansIters(5),
+                 pq.WithSeed(42))
+    rng = np.random.default_rng(3)
+    x = (rng.standard_normal((4, 32)) * 20)[np.arange(40) % 4] + rng.standard_normal((40, 32))
+    idx.bulk_add_arrays(np.arange(40), x.astype(np.float32))
     idx.Train()
-    q = np.array([[0, 1, 2, 3]], np.float32)
-    ids, d, c = idx.search_batch(q, 40, nprobe=3, scan_mode=_lib.SCAN_LIST_BF16, rerank=2, fused=1)
-    assert idx.Stats().FallbackSearches >= 1
+    before = idx.Stats().FallbackSearches
+    ids, d, c = idx.search_batch(x[:1].astype(np.float32), 40, nprobe=1, scan_mode=_lib.SCAN_LIST_BF16, rerank=2,
+                                 fused=1)
+    assert ran_list_scan(idx, _lib.SCAN_LIST_BF16)
+    assert idx.Stats().FallbackSearches == before + 1
     assert c[0] == 40 and sorted(ids[0].tolist()) == list(range(40))
 
 
@@ -743,6 +792,7 @@ def test_list_scan_without_rerank_uses_table_scan(hann):
     idx, x, qs = list_index(n=6000, coarse_k=32, seed=2)
     a = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_ADC_F32, rerank=0, fused=1)
     b = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16, rerank=0, fused=1)
+    assert idx.last_scan() == (_lib.SCAN_ADC_F32, 1)
     for u, v in zip(a, b):
         assert np.array_equal(u, v)
 
@@ -759,6 +809,7 @@ def test_list_scan_after_mutations(hann):
     other = pq.New(64)
     other.load_bytes(idx.save_bytes())
     a = idx.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
+    assert ran_list_scan(idx, _lib.SCAN_LIST_BF16X3)
     b = other.search_batch(qs, 10, nprobe=8, scan_mode=_lib.SCAN_LIST_BF16X3, rerank=4, fused=1)
     assert np.array_equal(a[2], b[2])
     # entry positions differ between the two device layouts, which can only
