@@ -42,9 +42,6 @@ constexpr int SEL_BINS = 2048;  // histogram bins of the row select
 constexpr int SEL_ROOM = 2048;  // boundary-bin entries the select sorts in LDS
 constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
 constexpr int SEL_SAMP = 8;     // the select's sample: every 8th 128-byte block of a row
-// key of the row slots past a list's end: the bits of an FP16 NaN, above
-// every stored key (stored keys are finite or +inf, non-negative)
-constexpr unsigned kPadKey = 0xFFFFu;
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -139,65 +136,69 @@ __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
     return pairs > 0 ? ((pairs + LQ - 1) >> 5) * (int32_t)((len + LCH - 1) >> LCH_SHIFT) : 0;
 }
 
-// one workgroup of 1024: chunked exclusive scans of lcnt and of the item
-// counts
+// one workgroup of 1024: exclusive scans of lcnt and of the item counts,
+// in rounds of 1024 lists (thread t holds list r * 1024 + t of round r, all
+// rounds' loads issued first)
+constexpr int PO_ROUNDS = 16;   // lists up to 16,384 in registers; more loop over rounds of that
+
 __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
                                                        const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
                                                        int32_t *__restrict__ lfill, int32_t *__restrict__ totals) {
-    __shared__ int32_t s_a[1024], s_b[1024];
-    const int t = threadIdx.x;
-    const int per = (K + 1023) / 1024;
-    const int b0 = t * per, b1 = b0 + per < K ? b0 + per : K;
-    int32_t sa = 0, sb = 0;
-    // 16 loads in flight per thread
-    for (int l0 = b0; l0 < b1; l0 += 16) {
-        int32_t v[16];
-        int64_t n[16];
+    __shared__ int32_t s_a[16], s_b[16];
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    int32_t carry_a = 0, carry_b = 0;
+    for (int g0 = 0; g0 < K; g0 += 1024 * PO_ROUNDS) {
+        int32_t v[PO_ROUNDS], it[PO_ROUNDS];
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
-            n[u] = l0 + u < b1 ? list_len[l0 + u] : 0;
+        for (int r = 0; r < PO_ROUNDS; ++r) {
+            const int l = g0 + r * 1024 + t;
+            const int lc = l < K ? l : K - 1;
+            const int32_t c = lcnt[lc];
+            const int64_t n = list_len[lc];
+            v[r] = l < K ? c : 0;
+            it[r] = l < K ? list_items(c, n) : 0;
         }
 #pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            sa += v[u];
-            sb += list_items(v[u], n[u]);
-        }
-    }
-    s_a[t] = sa;
-    s_b[t] = sb;
-    __syncthreads();
-    for (int o = 1; o < 1024; o <<= 1) {
-        const int32_t va = t >= o ? s_a[t - o] : 0, vb = t >= o ? s_b[t - o] : 0;
-        __syncthreads();
-        s_a[t] += va;
-        s_b[t] += vb;
-        __syncthreads();
-    }
-    int32_t ra = s_a[t] - sa, rb = s_b[t] - sb;
-    for (int l0 = b0; l0 < b1; l0 += 16) {
-        int32_t v[16];
-        int64_t n[16];
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            v[u] = l0 + u < b1 ? lcnt[l0 + u] : 0;
-            n[u] = l0 + u < b1 ? list_len[l0 + u] : 0;
-        }
-#pragma unroll
-        for (int u = 0; u < 16; ++u) {
-            if (l0 + u < b1) {
-                lstart[l0 + u] = ra;
-                ioff[l0 + u] = rb;
-                lfill[l0 + u] = 0;
+        for (int r = 0; r < PO_ROUNDS; ++r) {
+            // block-wide inclusive scan of (v, it) over the round
+            int32_t a = v[r], b = it[r];
+            for (int o = 1; o < 64; o <<= 1) {
+                const int32_t xa = __shfl_up(a, o), xb = __shfl_up(b, o);
+                if (lane >= o) {
+                    a += xa;
+                    b += xb;
+                }
             }
-            ra += v[u];
-            rb += list_items(v[u], n[u]);
+            if (lane == 63) {
+                s_a[w] = a;
+                s_b[w] = b;
+            }
+            __syncthreads();
+            int32_t wa = 0, wb = 0, ta = 0, tb = 0;
+            for (int u = 0; u < 16; ++u) {
+                const int32_t ua = s_a[u], ub = s_b[u];
+                if (u < w) {
+                    wa += ua;
+                    wb += ub;
+                }
+                ta += ua;
+                tb += ub;
+            }
+            __syncthreads();
+            const int l = g0 + r * 1024 + t;
+            if (l < K) {
+                lstart[l] = carry_a + wa + a - v[r];
+                ioff[l] = carry_b + wb + b - it[r];
+                lfill[l] = 0;
+            }
+            carry_a += ta;
+            carry_b += tb;
         }
     }
-    if (t == 1023) {
-        totals[0] = s_a[1023];
-        totals[1] = s_b[1023];
+    if (t == 0) {
+        totals[0] = carry_a;
+        totals[1] = carry_b;
     }
 }
 
@@ -473,8 +474,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                     kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
                     kmax2 = max_u16x2(kmax2, max_u16x2(p0, p1));
                 } else {
-                    // entries past the chunk: the padding key, and no part
-                    // in the range
+                    // entries past the chunk: the padding key 0xFFFF (the
+                    // bits of an FP16 NaN, above every stored key, which are
+                    // non-negative and at most +inf), and no part in the range
                     const uint32_t v0 = (e4 + 0 < ee ? 0x0000FFFFu : 0u) | (e4 + 1 < ee ? 0xFFFF0000u : 0u);
                     const uint32_t v1 = (e4 + 2 < ee ? 0x0000FFFFu : 0u) | (e4 + 3 < ee ? 0xFFFF0000u : 0u);
                     p0 = (p0 & v0) | ~v0;
@@ -549,10 +551,10 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     extern __shared__ __align__(16) unsigned char s_raw[];
     unsigned *hist = reinterpret_cast<unsigned *>(s_raw);       // [SEL_BINS]
     unsigned *bkey = hist + SEL_BINS;                           // [SEL_ROOM]
-    int64_t *bpos = reinterpret_cast<int64_t *>(bkey + SEL_ROOM);   // [SEL_ROOM]
-    int64_t *poff = bpos + SEL_ROOM;                            // [probe_cap + 1]
+    int32_t *bpos = reinterpret_cast<int32_t *>(bkey + SEL_ROOM);   // [SEL_ROOM] entry positions (< 2^31)
+    int64_t *poff = reinterpret_cast<int64_t *>(bpos + SEL_ROOM);   // [probe_cap + 1]
     int64_t *pst = poff + a.probe_cap + 1;                      // [probe_cap]
-    __shared__ int s_out, s_fill, s_B, s_below, s_nb, s_tot;
+    __shared__ int s_out, s_fill, s_B, s_below, s_nb;
     __shared__ unsigned s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
@@ -645,7 +647,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         while (span < nb) span <<= 1;
         for (int t = nb + tid; t < span; t += 256) {
             bkey[t] = 0xFFFFFFFFu;
-            bpos[t] = INT64_MAX;
+            bpos[t] = INT32_MAX;
         }
         __syncthreads();
         for (int size = 2; size <= span; size <<= 1) {
@@ -655,7 +657,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
                     const int i1 = i0 + stride;
                     const bool up = (i0 & size) == 0;
                     const unsigned ka = bkey[i0], kb = bkey[i1];
-                    const int64_t pa = bpos[i0], pb = bpos[i1];
+                    const int32_t pa = bpos[i0], pb = bpos[i1];
                     const bool b_less = kb < ka || (kb == ka && pb < pa);
                     const bool a_less = ka < kb || (ka == kb && pa < pb);
                     if (up ? b_less : a_less) {
@@ -673,7 +675,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     };
     // the bin of a SEL_BINS histogram over [lo, hi] (shift sh) where the
     // cumulative count reaches need: s_B, with s_below entries before it
-    // and s_nb in it (wave 0: 32 bins a lane); s_tot = the histogram's total
+    // and s_nb in it (wave 0: 32 bins a lane)
     auto find_bin = [&](unsigned need) {
         if (wave == 0) {
             unsigned own = 0;
@@ -692,7 +694,6 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
                 s_below = (int)run;
                 s_nb = (int)hist[b];
             }
-            if (lane == 63) s_tot = (int)incl;
         }
         __syncthreads();
     };
@@ -707,10 +708,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
         int sh = 0;
         while ((width >> sh) >= (unsigned)SEL_BINS) ++sh;
         for (int b = tid; b < SEL_BINS; b += 256) hist[b] = 0;
-        if (tid == 0) {
-            s_B = SEL_BINS - 1;
-            s_tot = 0;
-        }
+        if (tid == 0) s_B = SEL_BINS - 1;
         __syncthreads();
         // whole 128-byte blocks (8 groups), every SEL_SAMP-th block
         for (int64_t j = (int64_t)(tid >> 3) * 8 * SEL_SAMP + (tid & 7); j < L8; j += 32 * 8 * SEL_SAMP) {
@@ -742,7 +740,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
                 const int slot = atomicAdd(&s_fill, 1);
                 if (slot < SEL_ROOM) {
                     bkey[slot] = u;
-                    bpos[slot] = pos_of(i);
+                    bpos[slot] = (int32_t)pos_of(i);
                 }
             });
             __syncthreads();
@@ -781,7 +779,7 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
                 } else if (fits && b == B) {
                     const int slot = atomicAdd(&s_fill, 1);
                     bkey[slot] = u;
-                    bpos[slot] = pos_of(i);
+                    bpos[slot] = (int32_t)pos_of(i);
                 }
             });
         }
@@ -934,7 +932,7 @@ void launch_select_rows(const ScanArgs &a, int64_t nq, const uint16_t *dist, con
                         int32_t *out_cnt, int out_stride, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st) {
     if (nq == 0) return;
     if (a.probe_cap > dev::SEL_MAXP) fail(HANN_ERR_UNSUPPORTED, "list-major select: too many probes per query");
-    const size_t shm = (size_t)dev::SEL_BINS * 4 + (size_t)dev::SEL_ROOM * 12 + (size_t)(2 * a.probe_cap + 1) * 8;
+    const size_t shm = (size_t)dev::SEL_BINS * 4 + (size_t)dev::SEL_ROOM * 8 + (size_t)(2 * a.probe_cap + 1) * 8;
     HANN_HIP(hipFuncSetAttribute((const void *)dev::k_select_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
                                  (int)shm));
     hipLaunchKernelGGL(dev::k_select_rows, dim3((unsigned)nq), dim3(256), shm, st, a, nq, dist, row_off, qtot, qslots,
