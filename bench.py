@@ -36,9 +36,10 @@ if ROOT not in sys.path:
 
 METRIC = "QPS at recall@10≥0.95, 10M×96 PQIVF, chosen vs plain GPU; p50/p99 ms"
 HBM_PEAK_GBS = 8000.0
-# rocprofv3 counter summary of the table scan on the 10M x 96 workload: FETCH_SIZE
-# and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB copy kernel
-COUNTERS = os.path.join(ROOT, "profiles", "k_scan_block_pqivf_10m_x96_nlist16384_counters.txt")
+# rocprofv3 counter summary of the list-major scan on the 10M x 96 workload:
+# FETCH_SIZE and WRITE_SIZE per dispatch, scaled by what they read for a 1 GiB
+# copy kernel
+COUNTERS = os.path.join(ROOT, "profiles", "k_scan_list_pqivf_10m_x96_nlist16384_counters.txt")
 
 
 def profiled_traffic(kernel):
