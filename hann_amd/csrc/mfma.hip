@@ -21,8 +21,9 @@ namespace dev {
 // FP32 rows -> BF16 rows padded with zeros to dp columns; norm_b = ||x_b||^2
 // of the BF16 values (FP32 sum), norm_f = ||x|| of the FP32 row; max_norm
 // (float bits, non-negative) and inexact flag gathered with atomics.
-// one wave per row, 64 rows per workgroup; one atomic per workgroup
-constexpr int TB_ROWS = 64;
+// one wave per row, four rows per workgroup; at most one atomic each for
+// the maximum and the flag per workgroup
+constexpr int TB_ROWS = 4;
 
 __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ x, int64_t n, int d, int dp,
                                                  __bf16 *__restrict__ out, float *__restrict__ norm_b,
@@ -68,9 +69,11 @@ __global__ void __launch_bounds__(256) k_to_bf16(const float *__restrict__ x, in
     }
     __syncthreads();
     if (threadIdx.x == 0) {
+        // atomics only when they change something (a stale read only costs
+        // an atomic that changes nothing)
         const float m = fmaxf(fmaxf(s_max[0], s_max[1]), fmaxf(s_max[2], s_max[3]));
-        atomicMax(max_norm, __float_as_uint(m));
-        if (s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]) atomicOr(inexact, 1);
+        if (__float_as_uint(m) > *(volatile unsigned *)max_norm) atomicMax(max_norm, __float_as_uint(m));
+        if ((s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]) && *(volatile int *)inexact == 0) atomicOr(inexact, 1);
     }
 }
 
@@ -318,22 +321,26 @@ __device__ __forceinline__ float key_float(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
-// Per query: A = k-th smallest approximate distance over its ns sample rows
-// (radix select, 8 bits a pass). Each of those k rows has
-//   D <= (sqrt(max(A, 0) + E2) + delta)^2  and exact rank e <= D / (1-(d+2)u)
+// Per query: A' >= the k-th smallest approximate distance over its ns
+// sample rows -- the upper edge of the bin holding the k-th in a histogram
+// of ST_BINS bins over the query's [min, max] (in the order-preserving
+// unsigned map of the floats). Each of the k rows up to the k-th has
+//   D <= (sqrt(max(A', 0) + E2) + delta)^2  and exact rank e <= D / (1-(d+2)u)
 // (the bounds of k_thresholds, read the other way), which bounds the query's
 // true k-th exact rank tau; the filter threshold is then formed from tau as
 // in k_thresholds. One workgroup of 256 threads per query.
+constexpr int ST_BINS = 2048;
+
 __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t ns, int kmax,
                                                            const int32_t *__restrict__ k_q,
                                                            const float *__restrict__ qnorm_f,
                                                            const unsigned *__restrict__ max_norm,
                                                            const int *__restrict__ inexact, int d,
                                                            float *__restrict__ thr) {
-    __shared__ unsigned hist[256];
-    __shared__ unsigned s_prefix;
-    __shared__ int s_k;
-    const int tid = threadIdx.x, lane = tid % 64;
+    __shared__ unsigned hist[ST_BINS];
+    __shared__ unsigned s_lo[4], s_hi[4];
+    __shared__ unsigned s_edge;
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int64_t q = blockIdx.x;
     int k = k_q ? k_q[q] : kmax;
     k = k < 1 ? 1 : (k > kmax ? kmax : k);
@@ -342,51 +349,57 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
         return;
     }
     const float *a = approx + q * ns;
-    unsigned prefix = 0, mask = 0;
-    int kk = k;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 24 - 8 * pass;
-        hist[tid] = 0;
-        __syncthreads();
-        for (int64_t i = tid; i < ns; i += 256) {
-            const unsigned u = float_key(a[i]);
-            if ((u & mask) == prefix) atomicAdd(&hist[(u >> shift) & 255u], 1u);
-        }
-        __syncthreads();
-        if (tid < 64) {
-            // lane l owns bins 4l .. 4l+3; find the bin holding the kk-th
-            const unsigned h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
-                           h3 = hist[4 * lane + 3];
-            const unsigned own = h0 + h1 + h2 + h3;
-            unsigned incl = own;
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            const unsigned excl = incl - own;
-            const bool here = excl < (unsigned)kk && (unsigned)kk <= incl;
-            if (here) {
-                unsigned run = excl, bin = 4 * lane;
-                const unsigned hs[4] = {h0, h1, h2, h3};
-                for (int j = 0; j < 4; ++j) {
-                    if (run + hs[j] >= (unsigned)kk) {
-                        bin = 4 * lane + j;
-                        break;
-                    }
-                    run += hs[j];
-                }
-                s_prefix = prefix | (bin << shift);
-                s_k = kk - (int)run;
-            }
-        }
-        __syncthreads();
-        prefix = s_prefix;
-        kk = s_k;
-        mask |= 255u << shift;
-        __syncthreads();
+    // the range of the query's keys
+    unsigned lo = 0xFFFFFFFFu, hi = 0u;
+    for (int64_t i = tid; i < ns; i += 256) {
+        const unsigned u = float_key(a[i]);
+        lo = u < lo ? u : lo;
+        hi = u > hi ? u : hi;
     }
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned xl = __shfl_xor(lo, o), xh = __shfl_xor(hi, o);
+        lo = xl < lo ? xl : lo;
+        hi = xh > hi ? xh : hi;
+    }
+    if (lane == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    for (int b = tid; b < ST_BINS; b += 256) hist[b] = 0;
+    __syncthreads();
+    lo = s_lo[0];
+    hi = s_hi[0];
+    for (int w = 1; w < 4; ++w) {
+        lo = s_lo[w] < lo ? s_lo[w] : lo;
+        hi = s_hi[w] > hi ? s_hi[w] : hi;
+    }
+    int sh = 0;
+    while (((hi - lo) >> sh) >= (unsigned)ST_BINS) ++sh;
+    for (int64_t i = tid; i < ns; i += 256) atomicAdd(&hist[(float_key(a[i]) - lo) >> sh], 1u);
+    __syncthreads();
+    // the bin where the cumulative count reaches k (wave 0: 32 bins a lane)
+    if (wave == 0) {
+        unsigned own = 0;
+        for (int b = 0; b < ST_BINS / 64; ++b) own += hist[lane * (ST_BINS / 64) + b];
+        unsigned incl = own;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const unsigned excl = incl - own;
+        if (excl < (unsigned)k && (unsigned)k <= incl) {
+            unsigned run = excl;
+            int b = lane * (ST_BINS / 64);
+            while (run + hist[b] < (unsigned)k) run += hist[b++];
+            // the bin's upper edge, at most hi
+            const unsigned span = sh >= 32 ? 0xFFFFFFFFu : ((1u << sh) - 1u);
+            const unsigned blo = lo + ((unsigned)b << sh);
+            s_edge = (hi - blo) < span ? hi : blo + span;
+        }
+    }
+    __syncthreads();
     if (tid != 0) return;
-    const float A = key_float(prefix);
+    const float A = key_float(s_edge);
     if (!(A < __builtin_inff())) {
         thr[q] = __builtin_inff();
         return;
