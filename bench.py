@@ -18,7 +18,11 @@ Two search paths are measured on the same index and the same batches:
           or split-BF16 MFMA
 The rerank depth of each path is the smallest of --rerank-grid whose
 recall@10 against FP64 ground truth reaches 0.95. `value` is the chosen
-path's queries per second.
+path's queries per second with the timed batches alternating over --streams
+streams (consecutive batches overlap on the device; each stream has its own
+workspace in the library); p50/p99 are per-batch latencies in that run. Stage
+and scan-kernel times (and so the roofline) come from a separate one-stream
+run of the same batches.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W] [--n N] [--batch B]
 """
@@ -292,6 +296,8 @@ def main():
     ap.add_argument("--rerank-grid", default="1,2,4,8")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--streams", type=int, default=4,
+                    help="streams the timed batches alternate over (consecutive batches overlap)")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered queries/s (0: half the chosen QPS)")
     ap.add_argument("--paths", default="", help="comma-separated subset of the chosen-path candidates")
@@ -373,9 +379,16 @@ def main():
             if rec >= 0.95:
                 break
 
-    def timed(mode, rerank, fused, coarse, profile):
+    streams = [stream] + [torch.cuda.Stream() for _ in range(max(1, a.streams) - 1)]
+
+    def timed(mode, rerank, fused, coarse, profile, nstreams):
+        """Time a.steps batches, batch i on stream i mod nstreams (consecutive
+        batches overlap on the device when nstreams > 1); per-batch latency
+        from events on the batch's own stream."""
+        ss = streams[:nstreams]
         for i in range(a.warmup):
-            run_search(idx, dev_batches[i], sh, mode, rerank, fused, coarse)
+            for s_ in ss:
+                run_search(idx, dev_batches[i], s_.cuda_stream, mode, rerank, fused, coarse)
         torch.cuda.synchronize()
         idx.set_profile(profile)
         idx.stage_times()
@@ -383,9 +396,10 @@ def main():
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(a.steps):
-            evs[i][0].record(stream)
-            run_search(idx, dev_batches[a.warmup + i], sh, mode, rerank, fused, coarse)
-            evs[i][1].record(stream)
+            s_ = ss[i % nstreams]
+            evs[i][0].record(s_)
+            run_search(idx, dev_batches[a.warmup + i], s_.cuda_stream, mode, rerank, fused, coarse)
+            evs[i][1].record(s_)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
         idx.set_profile(False)
@@ -396,12 +410,21 @@ def main():
 
     res = {}
     for name, (mode, fused, coarse) in paths.items():
-        el, lat, st, kern = timed(mode, chosen_r[name], fused, coarse, True)
+        # stage and scan-kernel times from one stream (no overlap between
+        # batches); throughput and latency with a.streams streams
+        el1, lat1, st, kern = timed(mode, chosen_r[name], fused, coarse, True, 1)
+        if a.streams > 1:
+            el, lat, _, _ = timed(mode, chosen_r[name], fused, coarse, False, a.streams)
+        else:
+            el, lat = el1, lat1
         qps = a.steps * a.batch / el
         res[name] = {"qps": qps, "ms_per_step": 1000 * el / a.steps, "p50_ms": float(np.percentile(lat, 50)),
                      "p99_ms": float(np.percentile(lat, 99)), "stages": st, "rerank": chosen_r[name],
-                     "recall": recalls[name], "kernel_ms": kern / max(st[3], 1)}
-        log(f"{name}: {qps:.0f} q/s, {1000 * el / a.steps:.2f} ms/step, stages {st}, scan kernel {kern:.3f} ms")
+                     "recall": recalls[name], "kernel_ms": kern / max(st[3], 1),
+                     "p50_ms_one_stream": float(np.percentile(lat1, 50)),
+                     "p99_ms_one_stream": float(np.percentile(lat1, 99))}
+        log(f"{name}: {qps:.0f} q/s ({a.streams} streams), {1000 * el / a.steps:.2f} ms/step, stages {st}, "
+            f"scan kernel {kern:.3f} ms")
     # chosen: the fastest candidate path at recall@10 >= 0.95 (else the most accurate)
     cands = [n for n in res if n != "plain"]
     ok = [n for n in cands if res[n]["recall"] >= 0.95]
@@ -508,11 +531,13 @@ def main():
                                f"k in {{10,50,100}}, nprobe in {{8..128}}",
                    "n": a.n, "dim": a.dim, "nlist": a.nlist, "m": a.m, "pq_k": 256, "batch": a.batch,
                    "sigma": a.sigma, "components": a.components, "train_sample": min(a.train_sample, a.n),
-                   "kmeans_iters": a.iters},
+                   "kmeans_iters": a.iters, "streams": a.streams},
         "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
         "recall_at_10": round(ch["recall"], 4),
         "chosen": {"qps": round(ch["qps"], 1), "recall_at_10": round(ch["recall"], 4), "rerank": ch["rerank"],
                    "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
+                   "p50_ms_one_stream": round(ch["p50_ms_one_stream"], 3),
+                   "p99_ms_one_stream": round(ch["p99_ms_one_stream"], 3),
                    "path": path_desc[chosen_name], "name": chosen_name,
                    "stage_ms_per_step": [round(v / max(ch["stages"][3], 1), 4) for v in ch["stages"][:3]]},
         "candidates": {n: {"qps": round(res[n]["qps"], 1), "recall_at_10": round(res[n]["recall"], 4),
@@ -522,6 +547,8 @@ def main():
                                                  for v in res[n]["stages"][:3]]} for n in cands},
         "plain": {"qps": round(pl["qps"], 1), "recall_at_10": round(pl["recall"], 4), "rerank": pl["rerank"],
                   "p50_ms": round(pl["p50_ms"], 3), "p99_ms": round(pl["p99_ms"], 3),
+                  "p50_ms_one_stream": round(pl["p50_ms_one_stream"], 3),
+                  "p99_ms_one_stream": round(pl["p99_ms_one_stream"], 3),
                   "path": "reference-order fp32 decode + fp32 rerank",
                   "stage_ms_per_step": [round(v / max(pl["stages"][3], 1), 4) for v in pl["stages"][:3]]},
         "chosen_vs_plain": round(ch["qps"] / pl["qps"], 3),

@@ -1705,9 +1705,10 @@ int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq
     if (params) p = *params;
     x.sync_device();
     hipStream_t st = stream ? (hipStream_t)stream : x.stream;
-    float *key = x.ws.get<float>(11, nq * (size_t)kmax);
+    Workspace &w = stream ? x.stream_ws(st) : x.ws;
+    float *key = w.get<float>(11, nq * (size_t)kmax);
     x.search_device(queries, (int64_t)nq, k, nprobe, kmax, max_nprobe > 0 ? max_nprobe : (int)x.nprobe, p, key, out_ids,
-                    out_counts, st);
+                    out_counts, st, -1, &w);
     launch_finalize(key, (int64_t)nq * kmax, out_dist, st);
     if (!stream) HANN_HIP(hipStreamSynchronize(st));
     HANN_API_END

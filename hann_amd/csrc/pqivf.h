@@ -96,6 +96,15 @@ struct PQIVF {
     // device mirror
     hipStream_t stream = nullptr;
     Workspace ws;
+    // the device-buffer search's workspace per caller stream, so that
+    // searches on different streams can run on the device at once (each
+    // stream's searches reuse its buffers in stream order)
+    std::map<hipStream_t, std::unique_ptr<Workspace>> ws_by_stream;
+    Workspace &stream_ws(hipStream_t s) {
+        std::unique_ptr<Workspace> &w = ws_by_stream[s];
+        if (!w) w.reset(new Workspace);
+        return *w;
+    }
     bool dirty = true;
     bool full_rebuild = true;
     // incremental mirror updates: lists have slack capacity on the device;

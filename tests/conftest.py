@@ -4,6 +4,14 @@ import sys
 import numpy as np
 import pytest
 
+try:
+    # torch first, as bench.py does: its HIP runtime is then the one the
+    # library binds to, so that tests can hand torch device buffers and
+    # streams to the library (one runtime per process)
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 if ROOT not in sys.path:
     sys.path.insert(0, ROOT)

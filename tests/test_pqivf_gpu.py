@@ -744,6 +744,37 @@ def test_list_scan_one_slab_lists(hann, tiny_lists_index, mode, floor):
     assert agree >= floor, agree
 
 
+def test_device_search_on_two_streams(hann):
+    """Device-buffer searches on two streams (each stream has its own
+    workspace in the library) overlap on the device; every batch's results
+    equal the same batch searched alone."""
+    import torch
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
+    rng = np.random.default_rng(2)
+    batches = [np.ascontiguousarray(qs[rng.permutation(len(qs))[:200]]) for _ in range(6)]
+    kw = dict(scan_mode=_lib.SCAN_LIST_BF16, rerank=4, fused=1, coarse_precision=_lib.PREC_BF16)
+    want = [idx.search_batch(b, 10, nprobe=16, **kw) for b in batches]
+    bufs = []
+    for b in batches:
+        n = len(b)
+        bufs.append((torch.from_numpy(b).cuda(), torch.full((n,), 10, dtype=torch.int32, device="cuda"),
+                     torch.full((n,), 16, dtype=torch.int32, device="cuda"),
+                     torch.empty((n, 10), dtype=torch.int64, device="cuda"),
+                     torch.empty((n, 10), dtype=torch.float32, device="cuda"),
+                     torch.empty(n, dtype=torch.int32, device="cuda")))
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for i, (q, k, npb, ids, dist, cnt) in enumerate(bufs):
+        idx.search_batch_dev(q.data_ptr(), q.shape[0], k.data_ptr(), npb.data_ptr(), 10, 16, ids.data_ptr(),
+                             dist.data_ptr(), cnt.data_ptr(), streams[i % 2].cuda_stream, **kw)
+    torch.cuda.synchronize()
+    for (q, k, npb, ids, dist, cnt), (wi, wd, wc) in zip(bufs, want):
+        assert np.array_equal(cnt.cpu().numpy(), wc)
+        assert np.array_equal(ids.cpu().numpy(), wi)
+        assert np.allclose(dist.cpu().numpy(), wd, rtol=1e-6, atol=0)
+
+
 def test_list_scan_exact_ties_select_by_position(hann):
     """Thousands of identical vectors share one key: the selection narrows
     its histogram to that key and keeps the smallest positions, as the
