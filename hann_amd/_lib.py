@@ -87,6 +87,9 @@ SIGNATURES = {
     "simd_squared_euclidean_batch": (None, [c_float_p, c_float_p, sz, sz, c_double_p]),
     "simd_manhattan_batch": (None, [c_float_p, c_float_p, sz, sz, c_double_p]),
     "simd_cosine_distance_batch": (None, [c_float_p, c_float_p, sz, sz, c_double_p]),
+    "simd_normalize": (None, [c_float_p, sz]),
+    "hann_normalize_batch": (cint, [c_float_p, sz, sz]),
+    "hann_normalize_batch_dev": (cint, [vp, sz, sz, vp]),
     "hann_distance_batch": (cint, [cint, cint, c_float_p, sz, c_float_p, sz, sz, c_double_p]),
     "hann_distance_batch_dev": (cint, [cint, cint, vp, sz, vp, sz, sz, vp, vp]),
     "hann_batch_topk": (cint, [cint, c_float_p, sz, c_float_p, sz, sz, sz, cint, sz, c_int64_p,

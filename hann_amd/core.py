@@ -1,5 +1,5 @@
 """Host-side mirror of the reference's core package (core/metric.go,
-core/distance.go) over the hann_amd C-ABI.
+core/distance.go, core/vector_ops.go) over the hann_amd C-ABI.
 
 Names, argument meaning and error messages follow the Go API, so the tests
 read like the reference's own: Metric.Distance / Rank / FromRank /
@@ -139,3 +139,30 @@ def batch_topk(metric, queries, base, k, precision=_lib.PREC_FP32, overfetch=0):
     _lib.check(_lib.load().hann_batch_topk(metric.id, _lib.fptr(queries), nq, _lib.fptr(base), base.shape[0],
                                            d, k, precision, overfetch, _lib.i64ptr(ids), _lib.dptr(dist)))
     return ids, dist
+
+
+def NormalizeVector(vec):
+    """core/vector_ops.go:16-22: scale a float32 array to unit norm in place
+    (a zero vector is left as it is; an empty one is a no-op)."""
+    if not (isinstance(vec, np.ndarray) and vec.dtype == np.float32 and vec.flags.c_contiguous):
+        raise HannError(_lib.HANN_ERR_INVALID_ARG, "NormalizeVector needs a contiguous float32 array")
+    if vec.size == 0:
+        return
+    _lib.load().simd_normalize(_lib.fptr(vec), vec.size)
+
+
+def NormalizeBatch(vecs):
+    """core/vector_ops.go:24-54: normalize every vector in place. The
+    reference runs one simd_normalize per vector on a worker pool; here all
+    vectors of one length go to the GPU in one launch."""
+    by_len = {}
+    for i, v in enumerate(vecs):
+        if not (isinstance(v, np.ndarray) and v.dtype == np.float32 and v.flags.c_contiguous):
+            raise HannError(_lib.HANN_ERR_INVALID_ARG, "NormalizeBatch needs contiguous float32 arrays")
+        if v.size:
+            by_len.setdefault(v.size, []).append(i)
+    for dim, idx in by_len.items():
+        flat = np.ascontiguousarray(np.stack([vecs[i] for i in idx]))
+        _lib.check(_lib.load().hann_normalize_batch(_lib.fptr(flat), len(idx), dim))
+        for j, i in enumerate(idx):
+            vecs[i][:] = flat[j]

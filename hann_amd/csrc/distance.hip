@@ -234,6 +234,43 @@ __global__ void __launch_bounds__(64) k_topk_merge(const float *__restrict__ par
 }  // namespace dev
 
 // ------------------------------------------------------------ launchers --
+// Normalize n rows of dim floats in place, one wave per row, in the order
+// of core/simd_kernels.inc.h:267-315 on the AVX2+FMA tier: lane c < 32 keeps
+// slot c of the four 8-lane accumulators over the 32-element blocks (slot c
+// = accumulator c / 8, lane c % 8), lanes 0-7 add the 8-wide cleanup into
+// accumulator 0, the slots combine as (acc0 + acc1) + (acc2 + acc3) per
+// lane and the lanes as ((s0+s4)+(s1+s5)) + ((s2+s6)+(s3+s7)), then a fused
+// scalar tail; norm = sqrtf(total); a zero norm leaves the row unchanged;
+// every element is divided by the norm (correctly rounded).
+__global__ void __launch_bounds__(256) k_normalize(float *__restrict__ x, int64_t n, int dim) {
+    const int lane = threadIdx.x & 63;
+    const int64_t row = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (row >= n) return;
+    float *v = x + row * dim;
+    const int lim32 = dim - dim % 32, lim8 = dim - dim % 8;
+    float acc = 0.f;
+    for (int i = 0; i < lim32; i += 32)
+        if (lane < 32) acc = __builtin_fmaf(v[i + lane], v[i + lane], acc);
+    for (int i = lim32; i < lim8; i += 8)
+        if (lane < 8) acc = __builtin_fmaf(v[i + lane], v[i + lane], acc);
+    const float a1 = __shfl(acc, (lane + 8) & 63), a2 = __shfl(acc, (lane + 16) & 63),
+                a3 = __shfl(acc, (lane + 24) & 63);
+    const float sl = (acc + a1) + (a2 + a3);             // lanes 0-7
+    const float t = sl + __shfl(sl, (lane + 4) & 63);    // lanes 0-3: s_l + s_{l+4}
+    const float u = t + __shfl(t, (lane + 1) & 63);      // lane 0: t0 + t1, lane 2: t2 + t3
+    float total = __shfl(u, 0) + __shfl(u, 2);
+    for (int i = lim8; i < dim; ++i) total = __builtin_fmaf(v[i], v[i], total);
+    const float norm = __builtin_sqrtf(total);
+    if (norm == 0.f) return;
+    for (int i = lane; i < dim; i += 64) v[i] = v[i] / norm;
+}
+
+void launch_normalize(float *x, int64_t n, int dim, hipStream_t st) {
+    if (n == 0 || dim == 0) return;
+    hipLaunchKernelGGL(k_normalize, dim3((unsigned)ceil_div(n, 4)), dim3(256), 0, st, x, n, dim);
+    HANN_LAUNCH_CHECK();
+}
+
 void launch_distance_batch(int metric, const float *queries, int64_t nq, const float *base,
                            int64_t nb, int dim, double *out, hipStream_t st) {
     if (nq == 0 || nb == 0) return;

@@ -32,6 +32,8 @@ struct Workspace {
 int kp_for(int k);
 
 // distance.hip
+// normalize n rows of dim floats in place (core/simd_kernels.inc.h:267-315)
+void launch_normalize(float *x, int64_t n, int dim, hipStream_t st);
 void launch_distance_batch(int metric, const float *queries, int64_t nq, const float *base,
                            int64_t nb, int dim, double *out, hipStream_t st);
 void launch_topk_exact(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,

@@ -180,7 +180,51 @@ void simd_cosine_distance_batch(const float *q, const float *flat, size_t dim, s
     batch_call(HANN_METRIC_COSINE, q, flat, dim, n, out);
 }
 
+// simd_normalize (core/simd_ops.h:14): in place on a host vector; a failed
+// call leaves the vector as it was and prints the reason (the reference's
+// void signature has no error channel)
+void simd_normalize(float *vec, size_t len) {
+    try {
+        if (len == 0) return;
+        Context &c = global_ctx();
+        std::lock_guard<std::mutex> lk(c.mu);
+        float *d = c.ws.get<float>(2, len);
+        h2d(d, vec, len * sizeof(float), c.stream);
+        launch_normalize(d, 1, (int)len, c.stream);
+        d2h(vec, d, len * sizeof(float), c.stream);
+        sync(c.stream);
+    } catch (const hann::Error &e) {
+        set_last_error(e.what());
+        std::fprintf(stderr, "hann_amd: normalize failed: %s\n", e.what());
+    }
+}
+
 // ------------------------------------------------------------ batch API --
+int hann_normalize_batch_dev(float *flat, size_t n, size_t dim, void *stream) {
+    HANN_API_BEGIN
+    if (n == 0 || dim == 0) return HANN_OK;
+    if (dim > (size_t)INT32_MAX) fail(HANN_ERR_INVALID_ARG, "vector length above 2^31 - 1");
+    Context &c = global_ctx();
+    hipStream_t st = stream ? (hipStream_t)stream : c.stream;
+    launch_normalize(flat, (int64_t)n, (int)dim, st);
+    if (!stream) sync(st);
+    HANN_API_END
+}
+
+int hann_normalize_batch(float *flat, size_t n, size_t dim) {
+    HANN_API_BEGIN
+    if (n == 0 || dim == 0) return HANN_OK;
+    if (dim > (size_t)INT32_MAX) fail(HANN_ERR_INVALID_ARG, "vector length above 2^31 - 1");
+    Context &c = global_ctx();
+    std::lock_guard<std::mutex> lk(c.mu);
+    float *d = c.ws.get<float>(2, n * dim);
+    h2d(d, flat, n * dim * sizeof(float), c.stream);
+    launch_normalize(d, (int64_t)n, (int)dim, c.stream);
+    d2h(flat, d, n * dim * sizeof(float), c.stream);
+    sync(c.stream);
+    HANN_API_END
+}
+
 static void check_metric(int metric) {
     if (metric < 0 || metric > 3) fail(HANN_ERR_INVALID_ARG, strf("unknown metric %d", metric));
 }

@@ -69,6 +69,19 @@ void simd_squared_euclidean_batch(const float *q, const float *flat, size_t dim,
 void simd_manhattan_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
 void simd_cosine_distance_batch(const float *q, const float *flat, size_t dim, size_t n, double *out);
 
+/* Drop-in for simd_normalize (core/simd_ops.h:14, bound by NormalizeVector,
+ * core/vector_ops.go:16-22): scales vec to unit Euclidean norm in place,
+ * bit-identical to the reference's AVX2+FMA tier
+ * (core/simd_kernels.inc.h:267-315); a zero vector is left as it is. */
+void simd_normalize(float *vec, size_t len);
+
+/* NormalizeBatch (core/vector_ops.go:24-54) for n vectors of dim floats in
+ * one flat array, in place, in one launch (build extension of the batch
+ * form: the reference loops simd_normalize over worker goroutines). The
+ * _dev form takes a device pointer and is asynchronous on `stream`. */
+int hann_normalize_batch(float *flat, size_t n, size_t dim);
+int hann_normalize_batch_dev(float *flat, size_t n, size_t dim, void *stream);
+
 /* ------------------------------------------------------ batch distance -- */
 typedef enum {
     HANN_METRIC_SQEUCLIDEAN = 0,

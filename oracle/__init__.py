@@ -38,6 +38,7 @@ _SIGS = {
     "or_rng_perm": (None, [_u64p, ctypes.c_int64, _i64p]),
     "or_stream_seed": (_u64, [_u64, _u64, _u64]),
     "or_sqdiff": (ctypes.c_float, [_fp, _fp, _sz]),
+    "or_normalize": (None, [_fp, _sz]),
     "or_euclidean": (ctypes.c_float, [_fp, _fp, _sz]),
     "or_manhattan": (ctypes.c_float, [_fp, _fp, _sz]),
     "or_cosine": (ctypes.c_float, [_fp, _fp, _sz]),
@@ -87,6 +88,13 @@ def f32(a):
 def pair(metric, a, b):
     a, b = f32(a), f32(b)
     return np.float32(lib().or_pair(metric, a.ctypes.data_as(_fp), b.ctypes.data_as(_fp), a.size))
+
+
+def normalize(vec):
+    """A normalized copy of vec (core/simd_kernels.inc.h:267-315, AVX2 tier)."""
+    v = f32(vec).copy()
+    lib().or_normalize(v.ctypes.data_as(_fp), v.size)
+    return v
 
 
 def batch(metric, q, flat, dim):

@@ -10,7 +10,7 @@
  * AVX2 tier aliases (core/simd_distance.c:139-149).
  *
  * What pins each function (tests/test_oracle.py):
- *   or_sqdiff/euclidean/manhattan/cosine, or_batch
+ *   or_sqdiff/euclidean/manhattan/cosine, or_batch, or_normalize
  *       bit-for-bit against golden vectors written by the reference's own C
  *       kernels compiled from source (tests/golden/distance_golden.npz), and
  *       against the closed forms of core/distance_test.go:14-123.
@@ -114,6 +114,32 @@ float or_sqdiff(const float *a, const float *b, size_t n) {
         sum = fmaf(d, d, sum);
     }
     return sum;
+}
+
+void or_normalize(float *v, size_t n) {
+    /* core/simd_kernels.inc.h:267-315 on the AVX2+FMA tier (the tier
+     * hann_cpu_init(2) installs, core/simd_ops.c:58-76): the squared norm
+     * in four 8-lane FMA accumulators over 32-element blocks, the 8-wide
+     * cleanup in accumulator 0, (acc0+acc1)+(acc2+acc3), the horizontal sum
+     * of core/simd_isa.h:31-41, a fused scalar tail; sqrtf; a zero norm
+     * leaves the vector as it is; every element divided by the norm. */
+    float acc[4][8];
+    memset(acc, 0, sizeof(acc));
+    size_t i = 0;
+    size_t lim4 = n - (n % 32);
+    for (; i < lim4; i += 32)
+        for (int a = 0; a < 4; a++)
+            for (int l = 0; l < 8; l++) acc[a][l] = fmaf(v[i + 8 * a + l], v[i + 8 * a + l], acc[a][l]);
+    size_t lim = n - (n % 8);
+    for (; i < lim; i += 8)
+        for (int l = 0; l < 8; l++) acc[0][l] = fmaf(v[i + l], v[i + l], acc[0][l]);
+    float s[8];
+    for (int l = 0; l < 8; l++) s[l] = (acc[0][l] + acc[1][l]) + (acc[2][l] + acc[3][l]);
+    float total = hsum8(s);
+    for (; i < n; i++) total = fmaf(v[i], v[i], total);
+    float norm = sqrtf(total);
+    if (norm == 0.0f) return;
+    for (size_t j = 0; j < n; j++) v[j] = v[j] / norm;
 }
 
 float or_euclidean(const float *a, const float *b, size_t n) {

@@ -28,6 +28,8 @@ uint64_t or_stream_seed(uint64_t seed, uint64_t epoch, uint64_t role);
 
 /* per-pair kernels, reference AVX2+FMA tier order (AVX tier for manhattan) */
 float or_sqdiff(const float *a, const float *b, size_t n);
+/* core/simd_kernels.inc.h:267-315 (AVX2 tier), in place */
+void or_normalize(float *v, size_t n);
 float or_euclidean(const float *a, const float *b, size_t n);
 float or_manhattan(const float *a, const float *b, size_t n);
 float or_cosine(const float *a, const float *b, size_t n);

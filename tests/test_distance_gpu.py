@@ -147,3 +147,45 @@ def test_batch_topk_bf16_certified_equals_exact(hann, oracle, nq, nb, dim, k, ki
     wid, wrank = oracle.batch_topk(0, q, base, k, nthreads=8)
     assert np.array_equal(ids, wid)
     assert np.array_equal(dist, np.sqrt(wrank.astype(np.float64)))
+
+
+def test_normalize_bit_exact_vs_reference(hann):
+    """simd_normalize, NormalizeVector and NormalizeBatch equal the
+    reference's AVX2 bits (core/simd_kernels.inc.h:267-315) on lengths
+    1..1,000 with every tail size, zero vectors (left as they are), small
+    and large values."""
+    import os
+    from conftest import GOLDEN
+    from hann_amd import core
+    g = np.load(os.path.join(GOLDEN, "normalize_golden.npz"))
+    off = np.concatenate([[0], np.cumsum(g["lens"])])
+    vecs, wants = [], []
+    for i in range(len(g["lens"])):
+        v = g["inputs"][off[i]:off[i + 1]].copy()
+        want = g["out_avx2"][off[i]:off[i + 1]]
+        w = v.copy()
+        core.NormalizeVector(w)
+        assert np.array_equal(w.view(np.uint32), want.view(np.uint32)), v.size
+        vecs.append(v)
+        wants.append(want)
+    core.NormalizeBatch(vecs)
+    for v, want in zip(vecs, wants):
+        assert np.array_equal(v.view(np.uint32), want.view(np.uint32)), v.size
+    empty = np.zeros(0, np.float32)
+    core.NormalizeVector(empty)
+
+
+def test_normalize_batch_device_many_rows(hann, oracle):
+    """The device batch form on 4,099 rows of 96 (one wave per row, rows past
+    a workgroup's last): every row equals the oracle's."""
+    import torch
+    from hann_amd import _lib
+    rng = np.random.default_rng(8)
+    x = (rng.standard_normal((4099, 96)) * 5).astype(np.float32)
+    x[17] = 0
+    d = torch.from_numpy(x).cuda()
+    torch.cuda.synchronize()
+    _lib.check(_lib.load().hann_normalize_batch_dev(d.data_ptr(), x.shape[0], x.shape[1], None))
+    got = d.cpu().numpy()
+    want = np.stack([oracle.normalize(r) for r in x])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))

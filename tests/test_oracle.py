@@ -333,3 +333,29 @@ def test_kmeanspp_seeds_distinct_and_deterministic(oracle):
     # every one of the 8 separated clusters gets a seed
     near = np.argmin(((a[:, None, :] - cen[None]) ** 2).sum(-1), axis=1)
     assert set(near.tolist()) == set(range(8))
+
+
+def split_norm(g):
+    off = np.concatenate([[0], np.cumsum(g["lens"])])
+    return [(int(off[i]), int(off[i + 1])) for i in range(len(g["lens"]))]
+
+
+def test_normalize_bit_exact_vs_reference_avx2(oracle):
+    """core/simd_kernels.inc.h:267-315: golden bits from the reference's own
+    C (normalize_avx2) for lengths 1..1,000, with tails of every size,
+    zero vectors (left as they are) and very small and large values."""
+    import os
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "normalize_golden.npz"))
+    for a, b in split_norm(g):
+        got = oracle.normalize(g["inputs"][a:b])
+        assert np.array_equal(got.view(np.uint32), g["out_avx2"][a:b].view(np.uint32)), (b - a)
+    # the scalar fallback tier sums in another order: the pin is tier-specific
+    assert not np.array_equal(g["out_avx2"].view(np.uint32), g["out_fallback"].view(np.uint32))
+    # zero vectors stay zero; others have unit norm to FP32 accuracy
+    for a, b in split_norm(g):
+        v = g["out_avx2"][a:b].astype(np.float64)
+        if np.all(g["inputs"][a:b] == 0):
+            assert np.all(v == 0)
+        else:
+            assert abs(np.sqrt((v * v).sum()) - 1.0) < 1e-5
