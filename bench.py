@@ -298,6 +298,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streams", type=int, default=4,
                     help="streams the timed batches alternate over (consecutive batches overlap)")
+    ap.add_argument("--graphs", type=int, default=0,
+                    help="1: the timed batches replay one search graph per stream (0: direct calls)")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--serve-rate", type=float, default=0.0, help="offered queries/s (0: half the chosen QPS)")
     ap.add_argument("--paths", default="", help="comma-separated subset of the chosen-path candidates")
@@ -381,14 +383,43 @@ def main():
 
     streams = [stream] + [torch.cuda.Stream() for _ in range(max(1, a.streams) - 1)]
 
-    def timed(mode, rerank, fused, coarse, profile, nstreams):
+    kmax_all = max(b.kmax for b in dev_batches)
+    pmax_all = max(b.pmax for b in dev_batches)
+
+    def timed(mode, rerank, fused, coarse, profile, nstreams, graphs=False):
         """Time a.steps batches, batch i on stream i mod nstreams (consecutive
         batches overlap on the device when nstreams > 1); per-batch latency
-        from events on the batch's own stream."""
+        from events on the batch's own stream. With graphs, each stream owns
+        one search graph bound to staging buffers: a batch is copied into its
+        stream's staging buffers (inside the timed region) and the graph is
+        replayed."""
         ss = streams[:nstreams]
-        for i in range(a.warmup):
+        gs = []
+        if graphs:
             for s_ in ss:
-                run_search(idx, dev_batches[i], s_.cuda_stream, mode, rerank, fused, coarse)
+                stage = DevBatch(torch, np.zeros((a.batch, a.dim), np.float32), np.full(a.batch, kmax_all, np.int32),
+                                 np.full(a.batch, pmax_all, np.int32))
+                g = idx.search_graph(stage.q.data_ptr(), stage.nq, stage.k.data_ptr(), stage.np.data_ptr(), kmax_all,
+                                     pmax_all, stage.ids.data_ptr(), stage.dist.data_ptr(), stage.cnt.data_ptr(),
+                                     s_.cuda_stream, scan_mode=mode, rerank=rerank, fused=fused,
+                                     coarse_precision=coarse)
+                gs.append((stage, g))
+            torch.cuda.synchronize()
+
+        def step(b, j, s_):
+            if not graphs:
+                run_search(idx, b, s_.cuda_stream, mode, rerank, fused, coarse)
+                return
+            stage, g = gs[j]
+            with torch.cuda.stream(s_):
+                stage.q.copy_(b.q, non_blocking=True)
+                stage.k.copy_(b.k, non_blocking=True)
+                stage.np.copy_(b.np, non_blocking=True)
+            g.launch()
+
+        for i in range(a.warmup):
+            for j, s_ in enumerate(ss):
+                step(dev_batches[i], j, s_)
         torch.cuda.synchronize()
         idx.set_profile(profile)
         idx.stage_times()
@@ -396,9 +427,10 @@ def main():
         torch.cuda.synchronize()
         t = time.perf_counter()
         for i in range(a.steps):
-            s_ = ss[i % nstreams]
+            j = i % nstreams
+            s_ = ss[j]
             evs[i][0].record(s_)
-            run_search(idx, dev_batches[a.warmup + i], s_.cuda_stream, mode, rerank, fused, coarse)
+            step(dev_batches[a.warmup + i], j, s_)
             evs[i][1].record(s_)
         torch.cuda.synchronize()
         el = time.perf_counter() - t
@@ -406,6 +438,11 @@ def main():
         st = idx.stage_times()
         kern = idx.scan_kernel_time()
         lat = np.array([s.elapsed_time(e) for s, e in evs])
+        if graphs:
+            counts = [g.counts() for _, g in gs]
+            log(f"graphs: {counts}")
+            for _, g in gs:
+                g.close()
         return el, lat, st, kern
 
     res = {}
@@ -413,8 +450,8 @@ def main():
         # stage and scan-kernel times from one stream (no overlap between
         # batches); throughput and latency with a.streams streams
         el1, lat1, st, kern = timed(mode, chosen_r[name], fused, coarse, True, 1)
-        if a.streams > 1:
-            el, lat, _, _ = timed(mode, chosen_r[name], fused, coarse, False, a.streams)
+        if a.streams > 1 or a.graphs:
+            el, lat, _, _ = timed(mode, chosen_r[name], fused, coarse, False, a.streams, a.graphs)
         else:
             el, lat = el1, lat1
         qps = a.steps * a.batch / el
@@ -531,7 +568,8 @@ def main():
                                f"k in {{10,50,100}}, nprobe in {{8..128}}",
                    "n": a.n, "dim": a.dim, "nlist": a.nlist, "m": a.m, "pq_k": 256, "batch": a.batch,
                    "sigma": a.sigma, "components": a.components, "train_sample": min(a.train_sample, a.n),
-                   "kmeans_iters": a.iters, "streams": a.streams},
+                   "kmeans_iters": a.iters, "streams": a.streams,
+                   "graphs": bool(a.graphs)},
         "p50_ms": round(ch["p50_ms"], 3), "p99_ms": round(ch["p99_ms"], 3),
         "recall_at_10": round(ch["recall"], 4),
         "chosen": {"qps": round(ch["qps"], 1), "recall_at_10": round(ch["recall"], 4), "rerank": ch["rerank"],

@@ -115,6 +115,11 @@ SIGNATURES = {
                                        c_int32_p]),
     "hann_pqivf_search_batch_dev": (cint, [vp, vp, sz, vp, vp, i32, i32, ctypes.POINTER(SearchParams),
                                            vp, vp, vp, vp]),
+    "hann_pqivf_search_graph_create": (cint, [vp, vp, sz, vp, vp, i32, i32, ctypes.POINTER(SearchParams),
+                                              vp, vp, vp, vp, ctypes.POINTER(vp)]),
+    "hann_search_graph_launch": (cint, [vp]),
+    "hann_search_graph_counts": (cint, [vp, c_int64_p, c_int64_p, c_int64_p]),
+    "hann_search_graph_destroy": (None, [vp]),
     "hann_pqivf_stats": (cint, [vp, ctypes.POINTER(IndexStats)]),
     "hann_pqivf_save": (cint, [vp, ctypes.POINTER(vp), c_size_p]),
     "hann_pqivf_load": (cint, [vp, vp, sz]),

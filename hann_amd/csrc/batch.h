@@ -33,6 +33,11 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
                     const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st,
                     const int32_t *k_q = nullptr, BF16Base *cache = nullptr, Workspace *ws = nullptr);
 
+// bases of at most this many rows: certified_topk with a workspace runs
+// without host synchronisation (an overflowed query is ranked over every row
+// on the device)
+int64_t certified_no_sync_max_rows();
+
 // assignPoints on the BF16 path: out[i] = nearest centroid, lowest index on ties
 void assign_bf16(Context &c, const float *data, int64_t n, const float *cents, int kc, int d, const int32_t *hint,
                  int32_t *out, hipStream_t st);

@@ -5,12 +5,20 @@
 
 #include "batch.h"
 #include "mfma.h"
+#include "scan_list.h"
 
 namespace hann {
 
 // bases up to this many rows: overflowed queries are ranked over every row
 // by the candidate-ranking kernel
 constexpr int64_t kRankEveryMax = 65536;
+
+int64_t certified_no_sync_max_rows() { return kRankEveryMax; }
+
+__global__ void k_copy_u32(unsigned *__restrict__ dst, const unsigned *__restrict__ src, int n) {
+    const int i = threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
 
 // Certified BF16 path for the (squared) Euclidean metrics. Results equal the
 // exact path's: the filter keeps every row whose exact FP32 rank can be at
@@ -44,7 +52,9 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         xb = cache->xb.get();
         xn = cache->xn.get();
         xf = cache->xf.get();
-        HANN_HIP(hipMemcpyAsync(stat, cache->stat.get(), 4 * sizeof(unsigned), hipMemcpyDeviceToDevice, st));
+        // a copy kernel, not a memcpy: the search may be captured in a graph
+        hipLaunchKernelGGL(k_copy_u32, dim3(1), dim3(64), 0, st, stat, cache->stat.get(), 4);
+        HANN_LAUNCH_CHECK();
     } else {
         xb = w.get<__bf16>(0, (size_t)nb * dp);
         xn = w.get<float>(1, (size_t)nb);
@@ -96,7 +106,7 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
     int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
     int32_t *cand = w.get<int32_t>(13, (size_t)chunk * cap);
-    HANN_HIP(hipMemsetAsync(cnt, 0, (size_t)nq * sizeof(int32_t), st));
+    launch_fill_i32(cnt, nq, 0, st);   // a fill kernel: the search may be captured in a graph
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
         const int64_t qn_ = std::min(chunk, nq - q0);
         launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cap, st);

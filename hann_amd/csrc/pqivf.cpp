@@ -693,6 +693,7 @@ __global__ void k_scatter_entries(int64_t n, int m, int code_bytes, int d, const
 void PQIVF::sync_device() {
     ensure_stream();
     if (!dirty) return;
+    ++mirror_epoch;
     const int s = m > 0 ? (int)(dim / m) : 0;
     code_bytes = pmax > 256 ? 2 : 1;
     auto pack_entry = [&](const ListData &L, size_t r, int64_t pos, std::vector<int64_t> &hids,
@@ -1712,6 +1713,147 @@ int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq
     launch_finalize(key, (int64_t)nq * kmax, out_dist, st);
     if (!stream) HANN_HIP(hipStreamSynchronize(st));
     HANN_API_END
+}
+
+// A device search bound to fixed buffers, replayed as one HIP graph. The
+// first launch (and the first after the index's device mirror changed) runs
+// the search eagerly, which also sizes the graph's own workspace and fills
+// the index's caches, then records the same launches into a graph; later
+// launches replay it. Shapes whose search waits on the host (every query
+// probing every list with per-query nprobe, or a certified coarse step over
+// more centroids than it ranks on the device) and profiled searches run
+// eagerly on every launch.
+struct hann_search_graph {
+    hann_pqivf *idx = nullptr;
+    const float *queries = nullptr;
+    size_t nq = 0;
+    const int32_t *k = nullptr, *nprobe = nullptr;
+    int32_t kmax = 0, max_nprobe = 0;
+    hann_search_params p{};
+    int64_t *out_ids = nullptr;
+    float *out_dist = nullptr;
+    int32_t *out_counts = nullptr;
+    hipStream_t st = nullptr;
+    bool sync_after = false;
+    hann::Workspace ws;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t epoch = ~0ull;
+    int64_t captures = 0, replays = 0, eager = 0;
+    void drop() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        exec = nullptr;
+        graph = nullptr;
+    }
+    ~hann_search_graph() { drop(); }
+
+    void search(hann::PQIVF &x) {
+        float *key = ws.get<float>(11, nq * (size_t)kmax);
+        x.search_device(queries, (int64_t)nq, k, nprobe, kmax, max_nprobe > 0 ? max_nprobe : (int)x.nprobe, p, key,
+                        out_ids, out_counts, st, -1, &ws);
+        hann::launch_finalize(key, (int64_t)nq * kmax, out_dist, st);
+    }
+
+    bool capturable(const hann::PQIVF &x) const {
+        if (x.profile) return false;
+        const int64_t K = (int64_t)x.lists.size();
+        const int64_t np = max_nprobe > 0 ? max_nprobe : (int64_t)x.nprobe;
+        if (np >= K && nprobe) return false;   // the smallest nprobe is read on the host
+        if (np < K && p.coarse_precision == HANN_PREC_BF16 && K > hann::certified_no_sync_max_rows()) return false;
+        return true;
+    }
+};
+
+int hann_pqivf_search_graph_create(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
+                                   const int32_t *nprobe, int32_t kmax, int32_t max_nprobe,
+                                   const hann_search_params *params, int64_t *out_ids, float *out_dist,
+                                   int32_t *out_counts, void *stream, hann_search_graph **out) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    if (!out) fail(HANN_ERR_INVALID_ARG, "output handle pointer is null");
+    *out = nullptr;
+    if (nq == 0) fail(HANN_ERR_INVALID_ARG, "a search graph needs at least one query");
+    if (!queries || !k || !out_ids || !out_dist || !out_counts)
+        fail(HANN_ERR_INVALID_ARG, "queries, k and the outputs must be device pointers");
+    {
+        std::unique_lock<std::shared_mutex> lk(x.mu);
+        x.check_searchable(kmax, (size_t)x.dim);
+        x.ensure_stream();
+    }
+    std::unique_ptr<hann_search_graph> g(new hann_search_graph);
+    g->idx = idx;
+    g->queries = queries;
+    g->nq = nq;
+    g->k = k;
+    g->nprobe = nprobe;
+    g->kmax = kmax;
+    g->max_nprobe = max_nprobe;
+    hann_search_default_params(&g->p);
+    if (params) g->p = *params;
+    g->out_ids = out_ids;
+    g->out_dist = out_dist;
+    g->out_counts = out_counts;
+    // the legacy default stream cannot be captured: the index's own stream
+    // runs the graph then, and the launch waits for it like the eager call
+    g->sync_after = stream == nullptr;
+    g->st = stream ? (hipStream_t)stream : x.stream;
+    *out = g.release();
+    HANN_API_END
+}
+
+int hann_search_graph_launch(hann_search_graph *g) {
+    HANN_API_BEGIN
+    if (!g || !g->idx) fail(HANN_ERR_INVALID_ARG, "search graph handle is null");
+    PQIVF &x = g->idx->ix;
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.check_searchable(g->kmax, (size_t)x.dim);
+    x.sync_device();
+    if (!g->capturable(x)) {
+        g->drop();
+        g->search(x);
+        ++g->eager;
+    } else if (g->exec && g->epoch == x.mirror_epoch) {
+        HANN_HIP(hipGraphLaunch(g->exec, g->st));
+        ++g->replays;
+    } else {
+        g->drop();
+        // this launch's results come from the eager run; the capture only
+        // records the launches (nothing runs while the stream captures)
+        g->search(x);
+        ++g->eager;
+        HANN_HIP(hipStreamBeginCapture(g->st, hipStreamCaptureModeRelaxed));
+        try {
+            g->search(x);
+        } catch (...) {
+            hipGraph_t partial = nullptr;
+            (void)hipStreamEndCapture(g->st, &partial);
+            if (partial) (void)hipGraphDestroy(partial);
+            throw;
+        }
+        HANN_HIP(hipStreamEndCapture(g->st, &g->graph));
+        HANN_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+        g->epoch = x.mirror_epoch;
+        ++g->captures;
+    }
+    if (g->sync_after) HANN_HIP(hipStreamSynchronize(g->st));
+    HANN_API_END
+}
+
+int hann_search_graph_counts(hann_search_graph *g, int64_t *captures, int64_t *replays, int64_t *eager) {
+    HANN_API_BEGIN
+    if (!g) fail(HANN_ERR_INVALID_ARG, "search graph handle is null");
+    if (captures) *captures = g->captures;
+    if (replays) *replays = g->replays;
+    if (eager) *eager = g->eager;
+    HANN_API_END
+}
+
+void hann_search_graph_destroy(hann_search_graph *g) {
+    if (!g) return;
+    // the graph's workspace may still be in use by queued work
+    if (g->st) (void)hipStreamSynchronize(g->st);
+    delete g;
 }
 
 int hann_pqivf_stats(hann_pqivf *idx, hann_index_stats *out) {

@@ -106,6 +106,9 @@ struct PQIVF {
         return *w;
     }
     bool dirty = true;
+    // bumped by every device-mirror update: captured search graphs hold
+    // buffer pointers and sizes of the mirror they saw
+    uint64_t mirror_epoch = 0;
     bool full_rebuild = true;
     // incremental mirror updates: lists have slack capacity on the device;
     // appended tails or whole changed lists are uploaded at the next sync

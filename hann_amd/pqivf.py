@@ -278,6 +278,16 @@ class Index:
                                                          ctypes.byref(p), out_ids_ptr, out_dist_ptr, out_cnt_ptr,
                                                          stream))
 
+    def search_graph(self, q_ptr, nq, k_ptr, np_ptr, kmax, max_nprobe, out_ids_ptr, out_dist_ptr, out_cnt_ptr,
+                     stream, scan_mode=_lib.SCAN_ADC_F32, rerank=0, coarse_precision=_lib.PREC_FP32, fused=0):
+        """A device search bound to these buffers, replayed as a HIP graph."""
+        p = _lib.SearchParams(scan_mode, coarse_precision, rerank, fused)
+        h = ctypes.c_void_p()
+        _lib.check(self._lib.hann_pqivf_search_graph_create(self._h, q_ptr, nq, k_ptr, np_ptr, kmax, max_nprobe,
+                                                            ctypes.byref(p), out_ids_ptr, out_dist_ptr,
+                                                            out_cnt_ptr, stream, ctypes.byref(h)))
+        return SearchGraph(self, h)
+
     # -- test hooks (pqivf/export_test.go)
     def shape(self):
         k, m, p, d = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
@@ -287,6 +297,34 @@ class Index:
 
     def trained(self):
         return bool(self._lib.hann_pqivf_is_trained(self._h))
+
+
+class SearchGraph:
+    """hann_search_graph: launch() searches the bound buffers again."""
+
+    def __init__(self, index, handle):
+        self._index = index   # the index must outlive the graph
+        self._lib = index._lib
+        self._h = handle
+
+    def launch(self):
+        _lib.check(self._lib.hann_search_graph_launch(self._h))
+
+    def counts(self):
+        c, r, e = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        _lib.check(self._lib.hann_search_graph_counts(self._h, ctypes.byref(c), ctypes.byref(r), ctypes.byref(e)))
+        return {"captures": c.value, "replays": r.value, "eager": e.value}
+
+    def close(self):
+        if self._h:
+            self._lib.hann_search_graph_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def AssignPoints(data, centroids, precision=_lib.PREC_FP32):

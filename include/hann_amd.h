@@ -231,6 +231,30 @@ int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq
                                 const hann_search_params *params, int64_t *out_ids,
                                 float *out_dist, int32_t *out_counts, void *stream);
 
+/* The device search above bound to fixed buffers and replayed as one HIP
+ * graph (build extension: the reference has no counterpart; it serves
+ * loops that search batch after batch of the same shape, writing new
+ * queries, k and nprobe into the same device buffers before each launch).
+ * create records the arguments and runs nothing. Each launch searches
+ * asynchronously on `stream` (NULL: the index's stream, and the launch
+ * waits); the first launch, and the first after a change of the index,
+ * runs the search directly and captures it, later launches replay the
+ * graph. Results equal hann_pqivf_search_batch_dev's. Profiled searches,
+ * per-query nprobe at or above the number of lists, and a BF16 coarse step
+ * over more than 65,536 centroids run directly on every launch. A graph
+ * holds its own workspace; launches of one graph are ordered on its stream,
+ * and the index must outlive it. */
+typedef struct hann_search_graph hann_search_graph;
+int hann_pqivf_search_graph_create(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
+                                   const int32_t *nprobe, int32_t kmax, int32_t max_nprobe,
+                                   const hann_search_params *params, int64_t *out_ids, float *out_dist,
+                                   int32_t *out_counts, void *stream, hann_search_graph **out);
+int hann_search_graph_launch(hann_search_graph *g);
+/* How often the graph was captured, replayed and run directly. */
+int hann_search_graph_counts(hann_search_graph *g, int64_t *captures, int64_t *replays, int64_t *eager);
+/* Waits for the graph's stream, then releases the graph and its workspace. */
+void hann_search_graph_destroy(hann_search_graph *g);
+
 /* IndexStats (core/index.go:131-139). */
 typedef struct {
     int64_t count;

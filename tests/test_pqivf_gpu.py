@@ -775,6 +775,133 @@ def test_device_search_on_two_streams(hann):
         assert np.allclose(dist.cpu().numpy(), wd, rtol=1e-6, atol=0)
 
 
+def _graph_buffers(n, kmax):
+    import torch
+    return (torch.empty((n, 64), dtype=torch.float32, device="cuda"),
+            torch.empty(n, dtype=torch.int32, device="cuda"), torch.empty(n, dtype=torch.int32, device="cuda"),
+            torch.empty((n, kmax), dtype=torch.int64, device="cuda"),
+            torch.empty((n, kmax), dtype=torch.float32, device="cuda"),
+            torch.empty(n, dtype=torch.int32, device="cuda"))
+
+
+def _check_against_eager(idx, b, ks, nps, bufs, kw):
+    wi, wd, wc = idx.search_batch(b, ks, nprobe=nps, **kw)
+    ids, dist, cnt = bufs[3], bufs[4], bufs[5]
+    assert np.array_equal(cnt.cpu().numpy(), wc)
+    got_i = ids.cpu().numpy()
+    got_d = dist.cpu().numpy()
+    for q in range(len(b)):
+        c = wc[q]
+        assert np.array_equal(got_i[q, :c], wi[q, :c])
+        assert np.allclose(got_d[q, :c], wd[q, :c], rtol=1e-6, atol=0)
+
+
+@pytest.mark.parametrize("path", ["list_bf16", "table_f32"])
+def test_search_graph_replays_match_eager(hann, path):
+    """A search bound to fixed device buffers is captured once and replayed:
+    before each launch new queries, per-query k and nprobe are written into
+    the bound buffers, eager searches on the same index and stream
+    synchronisations run between launches, and every launch's results equal
+    the eager search of the same batch."""
+    import torch
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
+    if path == "list_bf16":
+        kw = dict(scan_mode=_lib.SCAN_LIST_BF16, rerank=4, fused=1, coarse_precision=_lib.PREC_BF16)
+    else:
+        kw = dict(scan_mode=_lib.SCAN_ADC_F32, rerank=0, fused=1, coarse_precision=_lib.PREC_FP32)
+    n, kmax, pmax = 240, 10, 16
+    bufs = _graph_buffers(n, kmax)
+    q, kb, npb, ids, dist, cnt = bufs
+    stream = torch.cuda.Stream()
+    g = idx.search_graph(q.data_ptr(), n, kb.data_ptr(), npb.data_ptr(), kmax, pmax, ids.data_ptr(),
+                         dist.data_ptr(), cnt.data_ptr(), stream.cuda_stream, **kw)
+    rng = np.random.default_rng(7)
+    for it in range(5):
+        b = np.ascontiguousarray(qs[rng.permutation(len(qs))[:n]])
+        ks = rng.integers(1, kmax + 1, n).astype(np.int32)
+        ks[0] = kmax
+        nps = rng.integers(1, pmax + 1, n).astype(np.int32)
+        q.copy_(torch.from_numpy(b))
+        kb.copy_(torch.from_numpy(ks))
+        npb.copy_(torch.from_numpy(nps))
+        ids.fill_(-7)
+        torch.cuda.synchronize()
+        g.launch()
+        stream.synchronize()
+        _check_against_eager(idx, b, ks, nps, bufs, kw)
+        # other work on the index between launches
+        idx.search_batch(qs[:50], 5, nprobe=8, **kw)
+    c = g.counts()
+    assert c == {"captures": 1, "replays": 4, "eager": 1}, c
+    g.close()
+
+
+def test_search_graph_recaptures_after_update(hann):
+    """A change of the index between launches rebuilds the device mirror; the
+    next launch captures again, and results follow the changed index."""
+    import torch
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=20000, coarse_k=256, seed=9)
+    kw = dict(scan_mode=_lib.SCAN_LIST_BF16, rerank=4, fused=1, coarse_precision=_lib.PREC_BF16)
+    n, kmax, pmax = 100, 10, 16
+    bufs = _graph_buffers(n, kmax)
+    q, kb, npb, ids, dist, cnt = bufs
+    b = np.ascontiguousarray(qs[:n])
+    ks = np.full(n, kmax, np.int32)
+    nps = np.full(n, pmax, np.int32)
+    q.copy_(torch.from_numpy(b))
+    kb.copy_(torch.from_numpy(ks))
+    npb.copy_(torch.from_numpy(nps))
+    torch.cuda.synchronize()
+    # the index's own stream (the launch waits for it)
+    g = idx.search_graph(q.data_ptr(), n, kb.data_ptr(), npb.data_ptr(), kmax, pmax, ids.data_ptr(),
+                         dist.data_ptr(), cnt.data_ptr(), None, **kw)
+    g.launch()
+    g.launch()
+    _check_against_eager(idx, b, ks, nps, bufs, kw)
+    before = ids.cpu().numpy()
+    # delete every first hit: they must disappear from the next results
+    gone = np.unique(before[:, 0])
+    idx.BulkDelete(gone.tolist())
+    g.launch()
+    _check_against_eager(idx, b, ks, nps, bufs, kw)
+    assert not np.isin(ids.cpu().numpy(), gone).any()
+    g.launch()
+    _check_against_eager(idx, b, ks, nps, bufs, kw)
+    assert g.counts() == {"captures": 2, "replays": 2, "eager": 2}, g.counts()
+    g.close()
+
+
+def test_search_graph_all_lists_runs_directly(hann):
+    """Per-query nprobe covering every list makes the search read the
+    smallest nprobe on the host: such a graph runs the search directly on
+    every launch, with the same results."""
+    import torch
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=4000, coarse_k=16, seed=9)
+    kw = dict(scan_mode=_lib.SCAN_ADC_F32, rerank=0, fused=1, coarse_precision=_lib.PREC_FP32)
+    n, kmax, pmax = 64, 10, 16
+    bufs = _graph_buffers(n, kmax)
+    q, kb, npb, ids, dist, cnt = bufs
+    b = np.ascontiguousarray(qs[:n])
+    ks = np.full(n, kmax, np.int32)
+    nps = np.full(n, pmax, np.int32)
+    q.copy_(torch.from_numpy(b))
+    kb.copy_(torch.from_numpy(ks))
+    npb.copy_(torch.from_numpy(nps))
+    torch.cuda.synchronize()
+    stream = torch.cuda.Stream()
+    g = idx.search_graph(q.data_ptr(), n, kb.data_ptr(), npb.data_ptr(), kmax, pmax, ids.data_ptr(),
+                         dist.data_ptr(), cnt.data_ptr(), stream.cuda_stream, **kw)
+    for _ in range(2):
+        g.launch()
+        stream.synchronize()
+        _check_against_eager(idx, b, ks, nps, bufs, kw)
+    assert g.counts() == {"captures": 0, "replays": 0, "eager": 2}, g.counts()
+    g.close()
+
+
 def test_list_scan_exact_ties_select_by_position(hann):
     """Thousands of identical vectors share one key: the selection narrows
     its histogram to that key and keeps the smallest positions, as the
