@@ -123,6 +123,10 @@ SIGNATURES = {
     "hann_pqivf_stats": (cint, [vp, ctypes.POINTER(IndexStats)]),
     "hann_pqivf_save": (cint, [vp, ctypes.POINTER(vp), c_size_p]),
     "hann_pqivf_load": (cint, [vp, vp, sz]),
+    "hann_pqivf_gob_encode": (cint, [vp, cint, ctypes.POINTER(vp), c_size_p]),
+    "hann_pqivf_gob_decode": (cint, [vp, vp, sz]),
+    "hann_pqivf_save_gob": (cint, [vp, cint, ctypes.POINTER(vp), c_size_p]),
+    "hann_pqivf_load_gob": (cint, [vp, vp, sz]),
     "hann_free_buffer": (None, [vp]),
     "hann_pqivf_profile": (cint, [vp, cint]),
     "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),

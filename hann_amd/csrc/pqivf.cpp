@@ -1521,28 +1521,55 @@ void PQIVF::load(const void *buf, size_t len) {
         r.raw(v.data(), n * 4);
         if (!pend.count(id)) pend.put(id, v.data());
     }
-    dim = ddim;
-    coarse_k = dck;
-    m = dm;
-    pq_k = dpk;
-    iters = dit;
+    IndexState st;
+    st.dim = ddim;
+    st.coarse_k = dck;
+    st.m = dm;
+    st.pq_k = dpk;
+    st.iters = dit;
+    st.nprobe = dnp;
+    st.fallback = dfb;
+    st.trained = dtr;
+    st.has_seed = true;
+    st.seed = dseed;
+    st.train_epoch = depoch;
+    st.kc = dkc;
+    st.coarse.swap(C);
+    st.have_codebooks = dhave;
+    st.pmax = dpmax;
+    st.cb_len.swap(cbl);
+    st.codebooks.swap(CB);
+    st.lists.swap(L);
+    st.id_to_cluster.swap(i2c);
+    std::swap(st.pending, pend);
+    install(std::move(st));
+}
+
+void PQIVF::install(IndexState &&st) {
+    dim = st.dim;
+    coarse_k = st.coarse_k;
+    m = st.m;
+    pq_k = st.pq_k;
+    iters = st.iters;
     // a file without a probe count keeps the configured one, else 3
     // (pqivf/index.go:1070-1074)
-    if (dnp > 0) nprobe = dnp;
+    if (st.nprobe > 0) nprobe = st.nprobe;
     else if (nprobe <= 0) nprobe = 3;
-    fallback = dfb;
-    trained = dtr;
-    seed = dseed;
-    train_epoch = depoch;
-    kc = dkc;
-    coarse.swap(C);
-    have_codebooks = dhave;
-    pmax = dpmax;
-    cb_len.swap(cbl);
-    codebooks.swap(CB);
-    lists.swap(L);
-    id_to_cluster.swap(i2c);
-    std::swap(pending, pend);
+    fallback = st.fallback;
+    trained = st.trained;
+    if (st.has_seed) {
+        seed = st.seed;
+        train_epoch = st.train_epoch;
+    }
+    kc = st.kc;
+    coarse.swap(st.coarse);
+    have_codebooks = st.have_codebooks;
+    pmax = st.pmax;
+    cb_len.swap(st.cb_len);
+    codebooks.swap(st.codebooks);
+    lists.swap(st.lists);
+    id_to_cluster.swap(st.id_to_cluster);
+    std::swap(pending, st.pending);
     mark_dirty();
 }
 
@@ -1886,6 +1913,52 @@ int hann_pqivf_load(hann_pqivf *idx, const void *buf, size_t len) {
     IDX_CHECK();
     std::unique_lock<std::shared_mutex> lk(x.mu);
     x.load(buf, len);
+    HANN_API_END
+}
+
+namespace {
+int give_buffer(std::vector<unsigned char> &&b, void **buf, size_t *len) {
+    if (!buf || !len) fail(HANN_ERR_INVALID_ARG, "output buffer pointers are null");
+    void *p = std::malloc(b.size() ? b.size() : 1);
+    if (!p) fail(HANN_ERR_OOM, "host allocation failed");
+    std::memcpy(p, b.data(), b.size());
+    *buf = p;
+    *len = b.size();
+    return HANN_OK;
+}
+}  // namespace
+
+int hann_pqivf_gob_encode(hann_pqivf *idx, int format_version, void **buf, size_t *len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    give_buffer(x.gob_encode(format_version), buf, len);
+    HANN_API_END
+}
+
+int hann_pqivf_gob_decode(hann_pqivf *idx, const void *buf, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    if (!buf && len) fail(HANN_ERR_INVALID_ARG, "buffer is null");
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.gob_decode(buf, len);
+    HANN_API_END
+}
+
+int hann_pqivf_save_gob(hann_pqivf *idx, int format_version, void **buf, size_t *len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::shared_lock<std::shared_mutex> lk(x.mu);
+    give_buffer(x.gob_save(format_version), buf, len);
+    HANN_API_END
+}
+
+int hann_pqivf_load_gob(hann_pqivf *idx, const void *buf, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    if (!buf && len) fail(HANN_ERR_INVALID_ARG, "buffer is null");
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.gob_load(buf, len);
     HANN_API_END
 }
 

@@ -72,6 +72,23 @@ int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, i
 void assign_points(const float *data, int64_t n, const float *cents, int k, int d, int precision,
                    const int32_t *hint, int32_t *out, hipStream_t st);
 
+// A whole index state read from a file, installed at once.
+struct IndexState {
+    int64_t dim = 0, coarse_k = 0, m = 0, pq_k = 0, iters = 0, nprobe = 0;
+    bool fallback = false, trained = false;
+    bool has_seed = false;   // the file carries the seed and training epoch
+    uint64_t seed = 0, train_epoch = 0;
+    int64_t kc = 0;
+    std::vector<float> coarse;
+    bool have_codebooks = false;
+    int64_t pmax = 0;
+    std::vector<int64_t> cb_len;
+    std::vector<float> codebooks;
+    std::vector<ListData> lists;
+    std::unordered_map<int64_t, int64_t> id_to_cluster;
+    PendingStore pending;
+};
+
 struct PQIVF {
     // parameters (pqivf/index.go:36-54)
     int64_t dim = 0, coarse_k = 16, m = 1, pq_k = 16, iters = 10, nprobe = 3;
@@ -197,6 +214,13 @@ struct PQIVF {
     int64_t count() const;
     std::vector<unsigned char> save() const;
     void load(const void *buf, size_t len);
+    void install(IndexState &&st);
+    // the reference's gob format (gob.cpp): GobEncode / GobDecode and the
+    // Save / Load wrapping (pqivf/index.go:968-1101)
+    std::vector<unsigned char> gob_encode(int format_version) const;
+    void gob_decode(const void *buf, size_t len);
+    std::vector<unsigned char> gob_save(int format_version) const;
+    void gob_load(const void *buf, size_t len);
 };
 
 }  // namespace hann

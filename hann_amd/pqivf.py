@@ -223,11 +223,35 @@ class Index:
         b = ctypes.create_string_buffer(bytes(data), len(data))
         _lib.check(self._lib.hann_pqivf_load(self._h, ctypes.cast(b, ctypes.c_void_p), len(data)))
 
-    def Save(self, w):
-        w.write(self.save_bytes())
+    def _take(self, fn, *args):
+        buf = ctypes.c_void_p()
+        n = ctypes.c_size_t()
+        _lib.check(fn(self._h, *args, ctypes.byref(buf), ctypes.byref(n)))
+        try:
+            return ctypes.string_at(buf, n.value)
+        finally:
+            self._lib.hann_free_buffer(buf)
+
+    def _give(self, fn, data):
+        data = bytes(data)
+        b = ctypes.create_string_buffer(data, len(data))
+        _lib.check(fn(self._h, ctypes.cast(b, ctypes.c_void_p), len(data)))
+
+    def GobEncode(self, format_version=1):
+        """GobEncode (pqivf/index.go:991-1017): the gob stream of serializedPQIVF."""
+        return self._take(self._lib.hann_pqivf_gob_encode, format_version)
+
+    def GobDecode(self, data):
+        """GobDecode (pqivf/index.go:1020-1084)."""
+        self._give(self._lib.hann_pqivf_gob_decode, data)
+
+    def Save(self, w, format_version=1):
+        """Save (pqivf/index.go:1086-1092): the reference's gob file."""
+        w.write(self._take(self._lib.hann_pqivf_save_gob, format_version))
 
     def Load(self, r):
-        self.load_bytes(r.read())
+        """Load (pqivf/index.go:1094-1100): a gob file written by Save here or by Go hann."""
+        self._give(self._lib.hann_pqivf_load_gob, r.read())
 
     # -- measurement (build extension)
     def set_profile(self, on):

@@ -268,6 +268,25 @@ int hann_pqivf_stats(hann_pqivf *idx, hann_index_stats *out);
  * save: *buf is malloc'ed, release with hann_free_buffer. */
 int hann_pqivf_save(hann_pqivf *idx, void **buf, size_t *len);
 int hann_pqivf_load(hann_pqivf *idx, const void *buf, size_t len);
+
+/* The reference's own on-disk format, Go encoding/gob: files written by
+ * Go hann's Index.Save load here and files saved here load in Go hann.
+ * gob_encode / gob_decode replace GobEncode / GobDecode
+ * (pqivf/index.go:991-1084): one gob stream of serializedPQIVF.
+ * save_gob / load_gob replace Save / Load (pqivf/index.go:1086-1100): that
+ * stream wrapped as one value of the *Index GobEncoder type.
+ * format_version 1 writes the current fields; 0 writes the layout from
+ * before FormatVersion and NumCandidateClusters existed. Maps are written in
+ * ascending key order. Decoding checks the format version and parameters
+ * with the reference's messages (HANN_ERR_FORMAT) and also rejects what the
+ * device index cannot hold: centroids, entries or pending vectors whose
+ * length is not the dimension, codebooks not of numSubquantizers x (dim/m),
+ * codes outside their codebook, lists without a centroid. Buffers from the
+ * encoders are malloc'ed, release with hann_free_buffer. */
+int hann_pqivf_gob_encode(hann_pqivf *idx, int format_version, void **buf, size_t *len);
+int hann_pqivf_gob_decode(hann_pqivf *idx, const void *buf, size_t len);
+int hann_pqivf_save_gob(hann_pqivf *idx, int format_version, void **buf, size_t *len);
+int hann_pqivf_load_gob(hann_pqivf *idx, const void *buf, size_t len);
 void hann_free_buffer(void *buf);
 
 /* Stage timers of the search path (measurement; no counterpart in the

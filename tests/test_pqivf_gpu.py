@@ -10,6 +10,7 @@ Bars (stated per test):
     the oracle, ids equal except at near-ties inside that bound.
   * The reference's behavioural tests (pqivf/index_test.go), restated.
 """
+import io
 import json
 import os
 import struct
@@ -354,7 +355,9 @@ def test_delete_all_then_search(hann):
 
 def test_save_load_trained_same_results(hann, tmp_path):
     """pqivf/index_test.go:170-205 and the save/load differential
-    (internal/testutil/differential.go): identical distances rank by rank."""
+    (internal/testutil/differential.go): identical distances rank by rank,
+    through the reference's gob file format (Save / Load) and through the
+    library's binary format."""
     from hann_amd import synthetic
     pq = P()
     x, qs = synthetic.config1(n=3000, q=20)
@@ -371,6 +374,14 @@ def test_save_load_trained_same_results(hann, tmp_path):
     b_ids, b_d, _ = other.search_batch(qs, 10)
     assert np.array_equal(a_ids, b_ids) and np.array_equal(a_d, b_d)
     assert other.Stats().Count == 3000
+    # the reloaded index writes the same file again
+    again = io.BytesIO()
+    other.Save(again)
+    assert again.getvalue() == path.read_bytes()
+    third = pq.New(64)
+    third.load_bytes(idx.save_bytes())
+    c_ids, c_d, _ = third.search_batch(qs, 10)
+    assert np.array_equal(a_ids, c_ids) and np.array_equal(a_d, c_d)
 
 
 def test_candidate_clusters_survive_save_load(hann):
@@ -483,6 +494,14 @@ def test_reference_trained_file_loads_and_searches(hann, oracle, gob_fixture):
         got_ids, got_d, cnt = idx.search_batch(qs, 5)   # nprobe 3 (legacy default)
         want_ids, want_d, wcnt = oracle.search_many(ref, qs, 5, 3)
         assert np.array_equal(got_ids, want_ids) and np.array_equal(got_d, want_d)
+    # the file itself, through the gob reader (Load), searches the same
+    via_gob = pq.New(8, pq.WithCoarseK(3), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithKMeansIters(5))
+    with open(os.path.join(GOLDEN, "index_v1.gob"), "rb") as fh:
+        via_gob.Load(fh)
+    gi, gd, gc = via_gob.search_batch(qs, 5)
+    assert np.array_equal(gi, want_ids) and np.array_equal(gd, want_d)
+    for q in qs:
+        assert [(r.ID, r.Distance) for r in via_gob.Search(q, 5)] == [(r.ID, r.Distance) for r in idx.Search(q, 5)]
     # stored cluster and codes equal the GPU's assignment and encoding
     assign = pq.AssignPoints(ref.vecs, ref.coarse)
     cl = np.repeat(np.arange(3), sizes)
