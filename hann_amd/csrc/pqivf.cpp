@@ -1176,7 +1176,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
                  o_ovf = carve((size_t)nq * 4), o_ctr = carve(16), o_parts = carve(kListScanParts * 4), o_pq = carve((size_t)pairs_max * 4),
                  o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
                  o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
-                 o_pkey = carve((size_t)nq * selk * 4);
+                 o_pkey = carve((size_t)nq * selk * 4), o_iord = carve((size_t)list_item_order_ints() * 4);
     char *base = w.get<char>(17, off);
     auto at = [&](size_t o) { return base + o; };
     int32_t *lcnt = reinterpret_cast<int32_t *>(at(o_lcnt));
@@ -1215,7 +1215,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, totals, st);
     launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, roff, lfill, pair_q, pair_qc, pair_scale,
                         pair_off, st);
-    launch_make_items(lcnt, lstart, ioff, d_list_len.get(), K, items, st);
+    launch_make_items(lcnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(o_iord)), items, st);
     // 2. the scan
     launch_fill_i32(ctr, 4, 0, st);
     launch_fill_i32(parts, kListScanParts, 0, st);
@@ -1243,6 +1243,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.item_counter = parts;
     la.n_parts = std::min(kListScanParts, grid);
     la.dist = dist;
+    la.dist_bytes = (uint32_t)(rows_max * 2);
     la.qrange = qrange;
     mark_kernel(0, st);
     launch_scan_list(la, x3, grid, st);

@@ -196,7 +196,7 @@ struct PQIVF {
     // entries in a batch's distance rows at most: nq times the np longest lists
     int64_t list_rows_bound(int64_t nq, int np) const;
     // the list-major scan's distance rows stay within 2^31 floats (8 GiB)
-    static constexpr int64_t kListRowsMax = int64_t(1) << 31;
+    static constexpr int64_t kListRowsMax = (int64_t(1) << 31) - 64;   // 16-bit keys, below 2^32 - 64 bytes
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;

@@ -26,6 +26,7 @@ struct ListScanArgs {
     int n_parts;                 // item partitions (at most the grid)
     uint16_t *dist;              // the batch's rows of 16-bit ordered keys: query q's, its probes in
                                  // order, each probe's run padded to a multiple of 8 slots (16-byte aligned)
+    uint32_t dist_bytes;         // size of dist in bytes (below 2^32 - 64)
     unsigned *qrange;            // [2][nq] smallest / largest key of each row (min set to ~0, max to 0)
 };
 
@@ -50,8 +51,11 @@ void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, in
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
                          const int32_t *lstart, const int64_t *row_off, int32_t *lfill, int32_t *pair_q,
                          float *pair_qc, float *pair_scale, int64_t *pair_off, hipStream_t st);
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, const int64_t *list_len, int K,
+// the batch's work items, longest first; order: list_item_order_ints()
+// ints of scratch (per-bucket counts and fill counters)
+void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int64_t *list_len, int K, int32_t *order,
                        int4 *items, hipStream_t st);
+int list_item_order_ints();
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);

@@ -261,18 +261,67 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
 }
 
 // one thread per list: its query groups times its entry chunks
-__global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__restrict__ lstart,
-                             const int32_t *__restrict__ ioff, const int64_t *__restrict__ list_len, int K,
-                             int4 *__restrict__ items) {
+// Work items longest first: item weight is its slab count (32-entry slabs
+// of its list chunk, 1 .. LCH / 32), bucket 32 - slabs. Items run in bucket
+// order, so the long items start first and the short ones fill the end
+// (results do not depend on the order: every pair's keys have a fixed place).
+constexpr int kItemBuckets = LCH / 32;
+
+__device__ inline int item_bucket(int64_t len, int c) {
+    const int64_t e = len - (int64_t)c * LCH;
+    const int slabs = (int)(((e < LCH ? e : LCH) + 31) >> 5);
+    return kItemBuckets - slabs;
+}
+
+// per bucket, the number of items of the batch
+__global__ void __launch_bounds__(256) k_item_hist(const int32_t *__restrict__ lcnt,
+                                                   const int64_t *__restrict__ list_len, int K,
+                                                   int32_t *__restrict__ hist) {
+    __shared__ int32_t s_h[kItemBuckets];
+    if (threadIdx.x < kItemBuckets) s_h[threadIdx.x] = 0;
+    __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
-    if (l >= K) return;
-    const int n = lcnt[l];
-    if (n == 0) return;
-    const int nch = (int)((list_len[l] + LCH - 1) / LCH);
-    int o = ioff[l];
-    for (int t = 0; t * LQ < n; ++t)
-        for (int c = 0; c < nch; ++c)
-            items[o++] = make_int4(l, lstart[l] + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
+    if (l < K && lcnt[l] > 0) {
+        const int64_t len = list_len[l];
+        const int nqg = (lcnt[l] + LQ - 1) / LQ;
+        const int nch = (int)((len + LCH - 1) / LCH);
+        for (int c = 0; c < nch; ++c) atomicAdd(&s_h[item_bucket(len, c)], nqg);
+    }
+    __syncthreads();
+    if (threadIdx.x < kItemBuckets && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
+}
+
+// the items in bucket order: a block reserves its share of every bucket
+// with one atomic per bucket, then places its items in LDS order
+__global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ lcnt,
+                                                    const int32_t *__restrict__ lstart,
+                                                    const int64_t *__restrict__ list_len, int K,
+                                                    const int32_t *__restrict__ hist, int32_t *__restrict__ fill,
+                                                    int4 *__restrict__ items) {
+    __shared__ int32_t s_c[kItemBuckets];
+    if (threadIdx.x < kItemBuckets) s_c[threadIdx.x] = 0;
+    __syncthreads();
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = l < K ? lcnt[l] : 0;
+    const int64_t len = n > 0 ? list_len[l] : 0;
+    const int nqg = (n + LQ - 1) / LQ;
+    const int nch = (int)((len + LCH - 1) / LCH);
+    for (int c = 0; c < nch; ++c) atomicAdd(&s_c[item_bucket(len, c)], nqg);
+    __syncthreads();
+    if (threadIdx.x < kItemBuckets) {
+        const int b = threadIdx.x;
+        int off = 0;
+        for (int j = 0; j < b; ++j) off += hist[j];
+        s_c[b] = s_c[b] ? off + atomicAdd(&fill[b], s_c[b]) : 0;
+    }
+    __syncthreads();
+    for (int c = 0; c < nch; ++c) {
+        const int b = item_bucket(len, c);
+        for (int t = 0; t * LQ < n; ++t) {
+            const int o = atomicAdd(&s_c[b], 1);
+            items[o] = make_int4(l, lstart[l] + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
+        }
+    }
 }
 
 // LNW waves per workgroup share the codebook in LDS; each wave runs its own
@@ -305,6 +354,11 @@ __global__ void k_make_items(const int32_t *__restrict__ lcnt, const int32_t *__
 // first slabs; waves of several workgroups per CU (the codebook is the only
 // LDS) overlap each other's loads.
 constexpr int LNW = 8;
+
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+// a buffer offset past any key buffer (key buffers hold less than 2^32 - 64
+// bytes), for lanes whose store is to be discarded
+constexpr uint32_t kNoStore = 0xFFFFFFF0u;
 
 template <bool X3, int M>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
@@ -401,6 +455,10 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         load_at(1, b1);
     }
 
+    // the key rows as a buffer resource of dist_bytes bytes (raw, gfx9 layout)
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc(la.dist, (short)0, (int)la.dist_bytes, 0x00020000);
+
     while (item < n_items) {
         if (lane == 0) j = atomicAdd(ctr, 1);   // the next item, in flight with this one's loads
         const int pc = S.pc, eb = S.eb, ee = S.ee, slot_end = S.slot_end;
@@ -412,14 +470,15 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
         {
             const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
+            // lanes without a query take zeros through a bit mask (a branch
+            // that overwrote the loaded registers would first wait for every
+            // memory operation of the wave)
+            const uint32_t keep = qv ? 0xFFFFFFFFu : 0u;
+            auto z = [&](float v) { return __uint_as_float(__float_as_uint(v) & keep); };
 #pragma unroll
             for (int jj = 0; jj < M / 4; ++jj) {
-                float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
-                if (!qv) {
-                    x0 = make_float4(0.f, 0.f, 0.f, 0.f);
-                    x1 = x0;
-                }
-                const float xs[8] = {x0.x, x0.y, x0.z, x0.w, x1.x, x1.y, x1.z, x1.w};
+                const float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
+                const float xs[8] = {z(x0.x), z(x0.y), z(x0.z), z(x0.w), z(x1.x), z(x1.y), z(x1.z), z(x1.w)};
 #pragma unroll
                 for (int u = 0; u < 8; ++u) {
                     bq[jj][u] = (__bf16)xs[u];
@@ -484,7 +543,13 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                     kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
                     kmax2 = max_u16x2(kmax2, max_u16x2(p0 & v0, p1 & v1));
                 }
-                if (qv && e4 < slot_end) *reinterpret_cast<uint2 *>(la.dist + (uint32_t)(off + e4)) = make_uint2(p0, p1);
+                // a buffer store for every lane: a lane without a key to
+                // write takes an offset past the buffer's range, which the
+                // hardware discards. With no branch around the store the
+                // compiler counts the wave's memory operations exactly and
+                // waits only for the loads it needs, not for every store.
+                const uint32_t boff = (qv && e4 < slot_end) ? 2u * (uint32_t)(off + e4) : kNoStore;
+                __builtin_amdgcn_raw_buffer_store_b64(u32x2{p0, p1}, drs, (int)boff, 0, 0);
             }
             wave_sync();   // s_term is rewritten by the next slab
         };
@@ -870,12 +935,18 @@ void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank
     HANN_LAUNCH_CHECK();
 }
 
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int32_t *ioff, const int64_t *list_len, int K,
+void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int64_t *list_len, int K, int32_t *order,
                        int4 *items, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lstart, ioff,
-                       list_len, K, items);
+    int32_t *hist = order, *fill = order + dev::kItemBuckets;
+    launch_fill_i32(order, 2 * dev::kItemBuckets, 0, st);
+    hipLaunchKernelGGL(dev::k_item_hist, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, list_len, K, hist);
+    HANN_LAUNCH_CHECK();
+    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lstart, list_len,
+                       K, hist, fill, items);
     HANN_LAUNCH_CHECK();
 }
+
+int list_item_order_ints() { return 2 * dev::kItemBuckets; }
 
 int list_scan_chunk() { return dev::LCH; }
 
