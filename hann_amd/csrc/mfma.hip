@@ -146,8 +146,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     __bf16 *sB = sA + MF_TILE * ld;
     float *s_qn = reinterpret_cast<float *>(sB + MF_TILE * ld);   // the tile's query norms
     float *s_thr = s_qn + MF_TILE;                                 // and thresholds (-inf past nq)
-    int32_t *s_cc = reinterpret_cast<int32_t *>(s_thr + MF_TILE);
-    int32_t *s_cr = s_cc + MF_TILE;                                // [MF_TILE][MF_SLOTS]
+    // candidates: each (query, row half) has MF_SLOTS / 2 slots, written by
+    // the one wave that tests that row half for that query
+    int32_t *s_cc = reinterpret_cast<int32_t *>(s_thr + MF_TILE);   // [MF_TILE][2] counts
+    int32_t *s_cr = s_cc + 2 * MF_TILE;                              // [MF_TILE][2][MF_SLOTS / 2]
     int2 *s_ov = reinterpret_cast<int2 *>(s_cr + MF_TILE * MF_SLOTS);   // [MF_OVF] (query, row)
     int32_t *s_on = reinterpret_cast<int32_t *>(s_ov + MF_OVF);
     const int tid = threadIdx.x;
@@ -158,10 +160,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     for (int t = tid; t < MF_TILE; t += 256) {
         const int64_t gq = q0 + t;
         s_qn[t] = gq < nq ? qn[gq] : 0.f;
-        if (!DENSE) {
-            s_thr[t] = gq < nq ? thr[gq] : -__builtin_inff();
-            s_cc[t] = 0;
-        }
+        if (!DENSE) s_thr[t] = gq < nq ? thr[gq] : -__builtin_inff();
     }
     if (!DENSE && tid == 0) *s_on = 0;
     // 128 rows (from row r0, rows past lim zero) x columns k0 .. k0 + kc of
@@ -179,6 +178,16 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
     // with one K chunk the query tile stays in LDS for every row tile
     const bool a_resident = dp <= MF_KC;
     if (a_resident) load_tile(sA, Q, q0, nq, 0, dp);
+    // the running candidate count of each of the lane half's 32 queries in
+    // this wave's row half (uniform over the half)
+    constexpr int SL = MF_SLOTS / 2;
+    int cq[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cq[i][g][u] = 0;
 
     for (int rt = 0; rt < rt_per_wg; ++rt) {
         const int64_t r0 = rbase + (int64_t)rt * MF_TILE;
@@ -252,21 +261,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                         }
                         // thresholds past nq are -inf: only real queries pass.
                         // Lanes 0-31 and 32-63 each test one query against 32
-                        // rows: one LDS atomic per half reserves its slots.
+                        // rows; a passing row's slot is the half's running
+                        // count plus its rank in the half's ballot.
                         const bool pass = rvj[j] && approx <= tv[u];
                         const unsigned long long m = __ballot(pass);
                         if (m == 0) continue;
                         const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
-                        int base = 0;
-                        if ((lane & 31) == 0 && (half_mask(lane, mlo, mhi) != 0))
-                            base = atomicAdd(&s_cc[ql], __popc(half_mask(lane, mlo, mhi)));
-                        base = __shfl(base, lane & 32);
-                        const int rank = lane < 32 ? (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
-                                                   : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
                         if (pass) {
-                            const int slot = base + rank;
-                            if (slot < MF_SLOTS) {
-                                s_cr[ql * MF_SLOTS + slot] = (int32_t)grj[j];
+                            const int rank = lane < 32 ? (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
+                                                       : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
+                            const int slot = cq[i][g][u] + rank;
+                            if (slot < SL) {
+                                s_cr[(ql * 2 + wn) * SL + slot] = (int32_t)grj[j];
                             } else {
                                 // the query's slots are full: the
                                 // workgroup's shared overflow list
@@ -274,17 +280,31 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
                                 if (o < MF_OVF) s_ov[o] = make_int2(ql, (int32_t)grj[j]);
                             }
                         }
+                        cq[i][g][u] += __popc(half_mask(lane, mlo, mhi));
                     }
             }
     }
     if (DENSE) return;
+    // the counts, from the first lane of each half
+    if ((lane & 31) == 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    s_cc[(wm * 64 + i * 32 + 8 * g + 4 * (lane / 32) + u) * 2 + wn] = cq[i][g][u];
+    }
     __syncthreads();
     for (int ql = tid; ql < MF_TILE; ql += 256) {
         const int64_t gq = q0 + ql;
-        const int n = s_cc[ql] < MF_SLOTS ? s_cc[ql] : MF_SLOTS;
-        if (gq >= nq || n == 0) continue;
-        const int base = atomicAdd(&cnt[gq], n);
-        for (int i = 0; i < n && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[ql * MF_SLOTS + i];
+        const int n0 = s_cc[ql * 2] < SL ? s_cc[ql * 2] : SL;
+        const int n1 = s_cc[ql * 2 + 1] < SL ? s_cc[ql * 2 + 1] : SL;
+        if (gq >= nq || n0 + n1 == 0) continue;
+        const int base = atomicAdd(&cnt[gq], n0 + n1);
+        for (int i = 0; i < n0 && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[(ql * 2) * SL + i];
+        for (int i = 0; i < n1 && base + n0 + i < cap; ++i)
+            cand_row[gq * cap + base + n0 + i] = s_cr[(ql * 2 + 1) * SL + i];
     }
     const int no = *s_on;
     for (int o = tid; o < (no < MF_OVF ? no : MF_OVF); o += 256) {
@@ -297,7 +317,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
         // rows were lost: every query that overflowed its slots takes the
         // exact path (a count above the room)
         for (int ql = tid; ql < MF_TILE; ql += 256)
-            if (s_cc[ql] > MF_SLOTS && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
+            if ((s_cc[ql * 2] > SL || s_cc[ql * 2 + 1] > SL) && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
 }
 
 // rows 0, stride, 2 stride, ... of X (BF16, dp columns) with their BF16 norms
@@ -489,7 +509,7 @@ void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm
 static size_t mfma_lds(int dp, bool dense) {
     const int ld = (dp < dev::MF_KC ? dp : dev::MF_KC) + dev::MF_PAD;
     return (size_t)2 * dev::MF_TILE * ld * 2 + (size_t)2 * dev::MF_TILE * 4 +
-           (dense ? 0 : (size_t)dev::MF_TILE * (1 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
+           (dense ? 0 : (size_t)dev::MF_TILE * (2 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
 }
 
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
