@@ -39,7 +39,7 @@ constexpr int LQ = 32;          // queries per work item (the MFMA's N)
 constexpr int LCH_SHIFT = 10;
 constexpr int LCH = 1 << LCH_SHIFT;   // entries per work item (a chunk of the list; a multiple of 32)
 constexpr int SEL_BINS = 2048;  // histogram bins of the row select
-constexpr int SEL_ROOM = 2048;  // boundary-bin entries the select sorts in LDS
+constexpr int SEL_ROOM = 1024;  // boundary-bin entries the select sorts in LDS (8 workgroups per CU)
 constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
 constexpr int SEL_SAMP = 8;     // the select's sample: every 8th 128-byte block of a row
 
