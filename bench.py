@@ -293,7 +293,7 @@ def main():
     ap.add_argument("--iters", type=int, default=10)
     ap.add_argument("--train-sample", type=int, default=1_000_000)
     ap.add_argument("--gt-queries", type=int, default=1024)
-    ap.add_argument("--rerank-grid", default="1,2,4,8")
+    ap.add_argument("--rerank-grid", default="1,2,3,4,8")
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--streams", type=int, default=4,
