@@ -258,17 +258,21 @@ struct WaveTopK {
         tau_i = INT64_MAX;
     }
 
-    // bitonic sort of the whole buffer (unused slots padded with +inf)
+    // bitonic sort of the filled span: the first power of two >= n slots
+    // (unused ones padded with +inf); slots past it are never read (every
+    // reader stops at n)
     __device__ void sort_all() {
         const int lane = lane_id();
-        for (int s = n + lane; s < cap; s += kWave) {
+        int span = 2;
+        while (span < n && span < cap) span <<= 1;
+        for (int s = n + lane; s < span; s += kWave) {
             keys[s] = __builtin_inff();
             ids[s] = INT64_MAX;
         }
         wave_sync();
-        for (int size = 2; size <= cap; size <<= 1) {
+        for (int size = 2; size <= span; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
-                for (int t = lane; t < (cap >> 1); t += kWave) {
+                for (int t = lane; t < (span >> 1); t += kWave) {
                     int lo = 2 * t - (t & (stride - 1));
                     int hi = lo + stride;
                     bool up = (lo & size) == 0;
