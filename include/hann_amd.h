@@ -225,7 +225,11 @@ int hann_pqivf_search_batch(hann_pqivf *idx, const float *queries, size_t nq, co
                             const int32_t *nprobe, const hann_search_params *params,
                             int64_t *out_ids, double *out_dist, int32_t *out_counts);
 /* Same with device-resident queries, k, nprobe and outputs (out_dist in FP32
- * true distance here); asynchronous on `stream`. */
+ * true distance here); asynchronous on `stream`. Each distinct stream gets
+ * its own scratch workspace (so batches on different streams run on the GPU
+ * at once), sized by the largest batch it has run and held until the index
+ * is freed: callers should search from a fixed set of streams. NULL: the
+ * index's own stream and workspace, and the call waits. */
 int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq, const int32_t *k,
                                 const int32_t *nprobe, int32_t kmax, int32_t max_nprobe,
                                 const hann_search_params *params, int64_t *out_ids,
