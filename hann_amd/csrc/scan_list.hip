@@ -36,7 +36,7 @@ namespace hann {
 namespace dev {
 
 constexpr int LQ = 32;          // queries per work item (the MFMA's N)
-constexpr int LCH_SHIFT = 10;
+constexpr int LCH_SHIFT = 11;
 constexpr int LCH = 1 << LCH_SHIFT;   // entries per work item (a chunk of the list; a multiple of 32)
 constexpr int SEL_BINS = 2048;  // histogram bins of the row select
 constexpr int SEL_ROOM = 1024;  // boundary-bin entries the select sorts in LDS (8 workgroups per CU)
