@@ -33,7 +33,7 @@ struct ListScanArgs {
 // entries of a list per work item of the list-major scan
 int list_scan_chunk();
 // work-item partitions (counters) of the list-major scan
-constexpr int kListScanParts = 64;
+constexpr int kListScanParts = 256;
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 // per-list pair counts, per-query entry totals and row slots (each probed
 // list's length rounded up to 8; lcnt zeroed by the caller)
