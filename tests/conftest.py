@@ -32,7 +32,7 @@ def oracle():
 
 @pytest.fixture(scope="session")
 def hann():
-    """The product library, loaded once per session on the GPU box."""
+    """The product library, loaded once per test session on a GPU host."""
     import hann_amd
     from hann_amd import _lib
     lib = _lib.load()
