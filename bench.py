@@ -64,11 +64,7 @@ def log(*a):
 
 def make_batch(centers, sigma, nq, seed):
     from hann_amd import synthetic
-    q, _ = synthetic.normalized_mixture(nq, centers.shape[1], centers.shape[0], sigma, seed, centers=centers)
-    rng = np.random.default_rng(seed + 1)
-    k = rng.choice(np.array([10, 50, 100], np.int32), nq).astype(np.int32)
-    nprobe = rng.choice(np.array([8, 16, 32, 64, 128], np.int32), nq).astype(np.int32)
-    return q, k, nprobe
+    return synthetic.config4_batch(centers, nq, seed, sigma=sigma)
 
 
 def ground_truth(torch, lib, base_dev, x, q, k_gt=10):
@@ -234,14 +230,9 @@ def bench_train(a, torch, lib):
     """Config 3: 1,000,000 x 128 from a 1,024-component mixture (centers
     N(0, 25 I), std 1); coarse k-means K = 4,096, 20 iterations; PQ 16 x 256
     on a 65,536-residual sample; encode all. FP32 vs BF16 assignment."""
-    from hann_amd import _lib, pqivf
-    rng = np.random.default_rng(5)
-    centers = (rng.standard_normal((1024, 128)) * 5).astype(np.float32)
+    from hann_amd import _lib, pqivf, synthetic
     n = a.n_train
-    x = np.empty((n, 128), np.float32)
-    for s0 in range(0, n, 250000):
-        e = min(n, s0 + 250000)
-        x[s0:e] = centers[rng.integers(0, 1024, e - s0)] + rng.standard_normal((e - s0, 128), dtype=np.float32)
+    x, _ = synthetic.config3(n=n)
     res = {}
     lists = {}
     for name, prec in (("fp32", _lib.PREC_FP32), ("bf16", _lib.PREC_BF16)):

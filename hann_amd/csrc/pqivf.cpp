@@ -118,7 +118,21 @@ static int64_t default_num_subquantizers(int64_t dim) {
 PQIVF::PQIVF() = default;
 
 PQIVF::~PQIVF() {
+    for (auto &kv : inflight) {
+        (void)hipEventSynchronize(kv.second);
+        (void)hipEventDestroy(kv.second);
+    }
     if (stream) (void)hipStreamDestroy(stream);
+}
+
+void PQIVF::note_inflight(hipStream_t s) {
+    hipEvent_t &e = inflight[s];
+    if (!e) HANN_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    HANN_HIP(hipEventRecord(e, s));
+}
+
+void PQIVF::wait_inflight() {
+    for (auto &kv : inflight) HANN_HIP(hipEventSynchronize(kv.second));
 }
 
 void PQIVF::ensure_stream() {
@@ -693,6 +707,7 @@ __global__ void k_scatter_entries(int64_t n, int m, int code_bytes, int d, const
 void PQIVF::sync_device() {
     ensure_stream();
     if (!dirty) return;
+    wait_inflight();
     ++mirror_epoch;
     const int s = m > 0 ? (int)(dim / m) : 0;
     code_bytes = pmax > 256 ? 2 : 1;
@@ -1739,7 +1754,10 @@ int hann_pqivf_search_batch_dev(hann_pqivf *idx, const float *queries, size_t nq
     x.search_device(queries, (int64_t)nq, k, nprobe, kmax, max_nprobe > 0 ? max_nprobe : (int)x.nprobe, p, key, out_ids,
                     out_counts, st, -1, &w);
     launch_finalize(key, (int64_t)nq * kmax, out_dist, st);
-    if (!stream) HANN_HIP(hipStreamSynchronize(st));
+    if (!stream)
+        HANN_HIP(hipStreamSynchronize(st));
+    else
+        x.note_inflight(st);
     HANN_API_END
 }
 
@@ -1864,7 +1882,10 @@ int hann_search_graph_launch(hann_search_graph *g) {
         g->epoch = x.mirror_epoch;
         ++g->captures;
     }
-    if (g->sync_after) HANN_HIP(hipStreamSynchronize(g->st));
+    if (g->sync_after)
+        HANN_HIP(hipStreamSynchronize(g->st));
+    else
+        x.note_inflight(g->st);
     HANN_API_END
 }
 

@@ -122,6 +122,14 @@ struct PQIVF {
         if (!w) w.reset(new Workspace);
         return *w;
     }
+    // searches queued on callers' streams (the device-buffer search and
+    // graph launches return before their work ends): one event per stream,
+    // recorded after that stream's latest search. A sync that changes the
+    // device mirror, and freeing the index, first wait for all of them, so
+    // no search still running reads lists that are being moved.
+    std::map<hipStream_t, hipEvent_t> inflight;
+    void note_inflight(hipStream_t s);
+    void wait_inflight();
     bool dirty = true;
     // bumped by every device-mirror update: captured search graphs hold
     // buffer pointers and sizes of the mirror they saw

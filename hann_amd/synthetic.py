@@ -68,3 +68,39 @@ def normalized_mixture(n, d, components, std, seed, centers=None, chunk=1_000_00
         x /= np.linalg.norm(x, axis=1, keepdims=True)
         out[s:e] = x
     return out, centers
+
+
+def config3(n=1_000_000, d=128, components=1024, seed=5, chunk=250_000):
+    """Training at scale: a 1,024-component mixture, centers N(0, 25 I),
+    per-dimension std 1.0."""
+    rng = np.random.default_rng(seed)
+    centers = (rng.standard_normal((components, d)) * 5).astype(np.float32)
+    x = np.empty((n, d), np.float32)
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        x[s:e] = centers[rng.integers(0, components, e - s)] + rng.standard_normal((e - s, d), dtype=np.float32)
+    return x, centers
+
+
+# the batched ragged search workload: 10M x 96 L2-normalised mixture vectors
+# (4,096 components, per-dimension std 0.06), 16,384 lists, PQ 24 x 256
+CONFIG4 = {"n": 10_000_000, "dim": 96, "components": 4096, "sigma": 0.06, "nlist": 16384, "m": 24, "pq_k": 256,
+           "kmeans_iters": 10, "train_sample": 1_000_000, "pq_train_sample": 65536, "seed": 1,
+           "candidate_clusters": 8, "data_seed": 7}
+
+
+def config4(n=None):
+    """The vectors of the batched search workload (and the mixture centers)."""
+    c = CONFIG4
+    return normalized_mixture(c["n"] if n is None else n, c["dim"], c["components"], c["sigma"], seed=c["data_seed"])
+
+
+def config4_batch(centers, nq, seed, sigma=None):
+    """One query batch: queries from the same mixture, each with its own k in
+    {10, 50, 100} and nprobe in {8, 16, 32, 64, 128}."""
+    sigma = CONFIG4["sigma"] if sigma is None else sigma
+    q, _ = normalized_mixture(nq, centers.shape[1], centers.shape[0], sigma, seed, centers=centers)
+    rng = np.random.default_rng(seed + 1)
+    k = rng.choice(np.array([10, 50, 100], np.int32), nq).astype(np.int32)
+    nprobe = rng.choice(np.array([8, 16, 32, 64, 128], np.int32), nq).astype(np.int32)
+    return q, k, nprobe

@@ -55,6 +55,10 @@ _SIGS = {
                         _fp, _szp, _i32p]),
     "or_search": (_int, [_fp, _sz, _fp, _sz, _i64p, _i64p, _fp, _i32p, _u8p, _fp, _szp, _sz, _sz,
                          _int, _int, _int, _int, _i64p, _dp, ctypes.POINTER(_int)]),
+    "or_search_rerank": (_int, [_fp, _sz, _fp, _sz, _i64p, _i64p, _fp, _i32p, _u8p, _fp, _szp, _sz, _sz,
+                                _int, _int, _int, _int, _int, _i64p, _dp, ctypes.POINTER(_int), _fp]),
+    "or_search_rerank_many": (None, [_fp, _sz, _sz, _fp, _sz, _i64p, _i64p, _fp, _i32p, _u8p, _fp, _szp, _sz,
+                                     _sz, _int, _i32p, _int, _i32p, _int, _int, _int, _i64p, _dp, _i32p, _fp]),
     "or_search_many": (None, [_fp, _sz, _sz, _fp, _sz, _i64p, _i64p, _fp, _i32p, _u8p, _fp, _szp, _sz,
                               _sz, _int, _int, _int, _int, _int, _i64p, _dp, _i32p]),
 }
@@ -230,3 +234,44 @@ def search_many(ix, queries, k, nprobe, allow_fallback=True, nthreads=1, have_co
                          nprobe, 1 if allow_fallback else 0, k, nthreads, ids.ctypes.data_as(_i64p),
                          dist.ctypes.data_as(_dp), cnt.ctypes.data_as(_i32p))
     return ids, dist, cnt
+
+
+def search_rerank_many(ix, queries, k, nprobe, rerank, allow_fallback=True, nthreads=1, have_codebooks=True):
+    """or_search_rerank for every query (k and nprobe scalars or per query).
+    Returns ids [nq][kmax], distances, counts and each query's candidate
+    bound (the PQ rank of the last of its rerank*k candidates, or +inf)."""
+    queries = f32(queries)
+    nq, d = queries.shape
+    ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (nq,)), np.int32)
+    nps = np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.int32), (nq,)), np.int32)
+    kmax = int(ks.max()) if nq else 1
+    ids = np.full((nq, kmax), -1, np.int64)
+    dist = np.full((nq, kmax), np.inf, np.float64)
+    cnt = np.zeros(nq, np.int32)
+    bound = np.zeros(nq, np.float32)
+    lib().or_search_rerank_many(queries.ctypes.data_as(_fp), nq, d, ix.coarse.ctypes.data_as(_fp),
+                                ix.coarse.shape[0], ix.list_off.ctypes.data_as(_i64p), ix.ids.ctypes.data_as(_i64p),
+                                ix.vecs.ctypes.data_as(_fp), ix.codes.ctypes.data_as(_i32p),
+                                ix.has_codes.ctypes.data_as(_u8p), ix.codebooks.ctypes.data_as(_fp),
+                                ix.cb_len.ctypes.data_as(_szp), ix.codebooks.shape[1], ix.m,
+                                1 if have_codebooks else 0, nps.ctypes.data_as(_i32p), 1 if allow_fallback else 0,
+                                ks.ctypes.data_as(_i32p), kmax, int(rerank), nthreads, ids.ctypes.data_as(_i64p),
+                                dist.ctypes.data_as(_dp), cnt.ctypes.data_as(_i32p), bound.ctypes.data_as(_fp))
+    return ids, dist, cnt, bound
+
+
+class CSRIndex:
+    """The CSR view or_search reads: list_off [K+1]; per entry ids, codes
+    [N][m] (rows of -1: no codes), vectors [N][d]; coarse centroids,
+    codebooks [m][P][d/m] and their valid row counts."""
+
+    def __init__(self, list_off, ids, codes, vecs, coarse, codebooks, cb_len):
+        self.list_off = np.ascontiguousarray(list_off, np.int64)
+        self.ids = np.ascontiguousarray(ids, np.int64)
+        self.codes = np.ascontiguousarray(codes, np.int32)
+        self.m = self.codes.shape[1]
+        self.vecs = f32(vecs)
+        self.coarse = f32(coarse)
+        self.codebooks = f32(codebooks)
+        self.cb_len = np.ascontiguousarray(cb_len, np.uint64)
+        self.has_codes = (self.codes[:, 0] >= 0).astype(np.uint8) if self.codes.size else np.zeros(0, np.uint8)

@@ -566,19 +566,50 @@ static int or_cent_cmp(const void *x, const void *y) {
     return a->c < b->c ? -1 : (a->c > b->c ? 1 : 0);
 }
 
+/* The reference's rank of entry e of list c for query q: decodePQCode
+ * (678-690) then vectorAdd (705-714) and the squared distance in the
+ * reference's order, or the distance to the stored vector when the entry has
+ * no valid codes (pqivf/index.go:905-936). `approx` holds d floats. */
+static float or_entry_rank(const float *q, size_t d, const float *coarse, int64_t c, int64_t e,
+                           const float *vecs, const int32_t *codes, const uint8_t *has_codes,
+                           const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
+                           int have_codebooks, float *approx) {
+    size_t s = m ? d / m : 0;
+    int use_codes = have_codebooks && has_codes[e];
+    if (use_codes) {
+        const int32_t *ce = codes + e * m;
+        for (size_t i = 0; i < m && use_codes; i++)
+            if (ce[i] < 0 || (size_t)ce[i] >= cb_len[i]) use_codes = 0;
+        if (use_codes) {
+            const float *cent = coarse + c * d;
+            for (size_t i = 0; i < m; i++) {
+                const float *cb = codebooks + (i * pmax + (size_t)ce[i]) * s;
+                for (size_t j = 0; j < s; j++) approx[i * s + j] = cent[i * s + j] + cb[j];
+            }
+        }
+    }
+    return use_codes ? or_sqdiff(q, approx, d) : or_sqdiff(q, vecs + e * d, d);
+}
+
+/* the probed lists of q: nearestCentroids (pqivf/index.go:184-206) sorts by
+ * rank with an unstable sort; ties here go to the lower cluster index */
+static or_cent *or_probe_order(const float *q, size_t d, const float *coarse, size_t coarse_k) {
+    or_cent *cs = (or_cent *)malloc(sizeof(or_cent) * (coarse_k ? coarse_k : 1));
+    for (size_t c = 0; c < coarse_k; c++) cs[c] = (or_cent){or_sqdiff(q, coarse + c * d, d), (int64_t)c};
+    qsort(cs, coarse_k, sizeof(or_cent), or_cent_cmp);
+    return cs;
+}
+
 int or_search(const float *q, size_t d, const float *coarse, size_t coarse_k,
               const int64_t *list_off, const int64_t *ids, const float *vecs,
               const int32_t *codes, const uint8_t *has_codes,
               const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
               int have_codebooks, int nprobe, int allow_fallback, int k,
               int64_t *out_ids, double *out_dist, int *fell_back) {
-    /* pqivf/index.go:850-949. nearestCentroids (184-206) sorts by rank with
-     * an unstable sort; ties here go to the lower cluster index. */
+    /* pqivf/index.go:850-949 */
     *fell_back = 0;
     if (k <= 0) return 0;
-    or_cent *cs = (or_cent *)malloc(sizeof(or_cent) * (coarse_k ? coarse_k : 1));
-    for (size_t c = 0; c < coarse_k; c++) cs[c] = (or_cent){or_sqdiff(q, coarse + c * d, d), (int64_t)c};
-    qsort(cs, coarse_k, sizeof(or_cent), or_cent_cmp);
+    or_cent *cs = or_probe_order(q, d, coarse, coarse_k);
     size_t np = (size_t)nprobe < coarse_k ? (size_t)nprobe : coarse_k;
     size_t ncand = 0;
     for (size_t i = 0; i < np; i++) ncand += list_off[cs[i].c + 1] - list_off[cs[i].c];
@@ -586,29 +617,14 @@ int or_search(const float *q, size_t d, const float *coarse, size_t coarse_k,
     if (ncand < (size_t)k && allow_fallback) { all = 1; *fell_back = 1; }
     size_t total = all ? (size_t)list_off[coarse_k] : ncand;
     or_cand *cand = (or_cand *)malloc(sizeof(or_cand) * (total ? total : 1));
-    size_t s = m ? d / m : 0;
     float approx[4096];
     size_t nc = 0;
     size_t nl = all ? coarse_k : np;
     for (size_t li = 0; li < nl; li++) {
         int64_t c = all ? (int64_t)li : cs[li].c;
         for (int64_t e = list_off[c]; e < list_off[c + 1]; e++) {
-            float r;
-            int use_codes = have_codebooks && has_codes[e];
-            if (use_codes) {
-                /* decodePQCode (678-690) then vectorAdd (705-714) */
-                const int32_t *ce = codes + e * m;
-                for (size_t i = 0; i < m && use_codes; i++)
-                    if (ce[i] < 0 || (size_t)ce[i] >= cb_len[i]) use_codes = 0;
-                if (use_codes) {
-                    const float *cent = coarse + c * d;
-                    for (size_t i = 0; i < m; i++) {
-                        const float *cb = codebooks + (i * pmax + (size_t)ce[i]) * s;
-                        for (size_t j = 0; j < s; j++) approx[i * s + j] = cent[i * s + j] + cb[j];
-                    }
-                }
-            }
-            r = use_codes ? or_sqdiff(q, approx, d) : or_sqdiff(q, vecs + e * d, d);
+            float r = or_entry_rank(q, d, coarse, c, e, vecs, codes, has_codes, codebooks, cb_len, pmax, m,
+                                    have_codebooks, approx);
             cand[nc++] = (or_cand){r, ids[e]};
         }
     }
@@ -618,6 +634,73 @@ int or_search(const float *q, size_t d, const float *coarse, size_t coarse_k,
         out_ids[i] = cand[i].id;
         out_dist[i] = sqrt((double)cand[i].r); /* FromRank, core/metric.go:156 */
     }
+    free(cand);
+    free(cs);
+    return kk;
+}
+
+/* candidates of the rerank: (PQ rank, storage position) */
+typedef struct { float r; int64_t e; } or_pos;
+
+static int or_pos_cmp(const void *x, const void *y) {
+    const or_pos *a = (const or_pos *)x, *b = (const or_pos *)y;
+    if (a->r < b->r) return -1;
+    if (a->r > b->r) return 1;
+    return a->e < b->e ? -1 : (a->e > b->e ? 1 : 0);
+}
+
+int or_search_rerank(const float *q, size_t d, const float *coarse, size_t coarse_k,
+                     const int64_t *list_off, const int64_t *ids, const float *vecs,
+                     const int32_t *codes, const uint8_t *has_codes,
+                     const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
+                     int have_codebooks, int nprobe, int allow_fallback, int k, int rerank,
+                     int64_t *out_ids, double *out_dist, int *fell_back, float *sel_bound) {
+    /* Search (pqivf/index.go:850-949) up to its candidate ranks: the probed
+     * lists, the fallback rule (fewer than k probed entries: every list) and
+     * the reference's PQ rank of every candidate. Then the build's rerank
+     * (no counterpart in the reference): the rerank*k best candidates by
+     * (PQ rank, storage position) -- the position is the entry's index in
+     * the CSR lists, lists in ascending cluster order, each in storage
+     * order -- are ranked by the reference's distance to the stored vector
+     * (the rank of an entry without codes, pqivf/index.go:928), and the best
+     * k by (rank, id) are returned with FromRank applied. *sel_bound is the
+     * PQ rank of the last candidate kept (+inf when every candidate is
+     * kept). */
+    *fell_back = 0;
+    *sel_bound = INFINITY;
+    if (k <= 0 || rerank <= 0) return 0;
+    or_cent *cs = or_probe_order(q, d, coarse, coarse_k);
+    size_t np = (size_t)nprobe < coarse_k ? (size_t)nprobe : coarse_k;
+    size_t ncand = 0;
+    for (size_t i = 0; i < np; i++) ncand += list_off[cs[i].c + 1] - list_off[cs[i].c];
+    int all = 0;
+    if (ncand < (size_t)k && allow_fallback) { all = 1; *fell_back = 1; }
+    size_t total = all ? (size_t)list_off[coarse_k] : ncand;
+    or_pos *cand = (or_pos *)malloc(sizeof(or_pos) * (total ? total : 1));
+    float approx[4096];
+    size_t nc = 0;
+    size_t nl = all ? coarse_k : np;
+    for (size_t li = 0; li < nl; li++) {
+        int64_t c = all ? (int64_t)li : cs[li].c;
+        for (int64_t e = list_off[c]; e < list_off[c + 1]; e++) {
+            float r = or_entry_rank(q, d, coarse, c, e, vecs, codes, has_codes, codebooks, cb_len, pmax, m,
+                                    have_codebooks, approx);
+            cand[nc++] = (or_pos){r, e};
+        }
+    }
+    qsort(cand, nc, sizeof(or_pos), or_pos_cmp);
+    size_t sel = (size_t)k * (size_t)rerank;
+    if (sel < nc) *sel_bound = cand[sel - 1].r;
+    else sel = nc;
+    or_cand *rr = (or_cand *)malloc(sizeof(or_cand) * (sel ? sel : 1));
+    for (size_t i = 0; i < sel; i++) rr[i] = (or_cand){or_sqdiff(q, vecs + cand[i].e * d, d), ids[cand[i].e]};
+    qsort(rr, sel, sizeof(or_cand), or_cand_cmp);
+    int kk = (size_t)k < sel ? k : (int)sel;
+    for (int i = 0; i < kk; i++) {
+        out_ids[i] = rr[i].id;
+        out_dist[i] = sqrt((double)rr[i].r);
+    }
+    free(rr);
     free(cand);
     free(cs);
     return kk;
@@ -639,6 +722,38 @@ static void or_search_range(void *p, size_t b, size_t e) {
                                      c->cb_len, c->pmax, c->m, c->have_cb, c->nprobe, c->fb, c->k,
                                      c->out_ids + q * c->k, c->out_dist + q * c->k, &fell);
     }
+}
+
+typedef struct {
+    or_search_ctx base; int rerank; const int32_t *k_q; const int32_t *np_q; int kstride; float *bound;
+} or_rr_ctx;
+
+static void or_rr_range(void *p, size_t b, size_t e) {
+    or_rr_ctx *r = (or_rr_ctx *)p;
+    or_search_ctx *c = &r->base;
+    for (size_t q = b; q < e; q++) {
+        int fell;
+        int kq = r->k_q ? r->k_q[q] : c->k;
+        int npq = r->np_q ? r->np_q[q] : c->nprobe;
+        c->out_counts[q] = or_search_rerank(c->qs + q * c->d, c->d, c->coarse, c->coarse_k, c->list_off,
+                                            c->ids, c->vecs, c->codes, c->has_codes, c->codebooks, c->cb_len,
+                                            c->pmax, c->m, c->have_cb, npq, c->fb, kq, r->rerank,
+                                            c->out_ids + q * r->kstride, c->out_dist + q * r->kstride, &fell,
+                                            r->bound + q);
+    }
+}
+
+void or_search_rerank_many(const float *qs, size_t nq, size_t d, const float *coarse, size_t coarse_k,
+                           const int64_t *list_off, const int64_t *ids, const float *vecs,
+                           const int32_t *codes, const uint8_t *has_codes,
+                           const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
+                           int have_codebooks, const int32_t *nprobe_q, int allow_fallback, const int32_t *k_q,
+                           int kstride, int rerank, int nthreads, int64_t *out_ids, double *out_dist,
+                           int32_t *out_counts, float *out_bound) {
+    or_rr_ctx c = {{qs, d, coarse, coarse_k, list_off, ids, vecs, codes, has_codes, codebooks, cb_len, pmax, m,
+                    have_codebooks, 0, allow_fallback, kstride, out_ids, out_dist, out_counts},
+                   rerank, k_q, nprobe_q, kstride, out_bound};
+    or_parallel(nq, nthreads, or_rr_range, &c);
 }
 
 void or_search_many(const float *qs, size_t nq, size_t d, const float *coarse, size_t coarse_k,

@@ -77,6 +77,25 @@ int or_search(const float *q, size_t d, const float *coarse, size_t coarse_k,
               int have_codebooks, int nprobe, int allow_fallback, int k,
               int64_t *out_ids, double *out_dist, int *fell_back);
 
+/* Search with the build's exact rerank: the rerank*k best candidates by
+ * (the reference's PQ rank, storage position), then the best k of those by
+ * (the reference's distance to the stored vector, id). *sel_bound: the PQ
+ * rank of the last candidate kept (+inf when all are kept). */
+int or_search_rerank(const float *q, size_t d, const float *coarse, size_t coarse_k,
+                     const int64_t *list_off, const int64_t *ids, const float *vecs,
+                     const int32_t *codes, const uint8_t *has_codes,
+                     const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
+                     int have_codebooks, int nprobe, int allow_fallback, int k, int rerank,
+                     int64_t *out_ids, double *out_dist, int *fell_back, float *sel_bound);
+/* the same for nq queries with per-query k and nprobe; outputs [nq][kstride] */
+void or_search_rerank_many(const float *qs, size_t nq, size_t d, const float *coarse, size_t coarse_k,
+                           const int64_t *list_off, const int64_t *ids, const float *vecs,
+                           const int32_t *codes, const uint8_t *has_codes,
+                           const float *codebooks, const size_t *cb_len, size_t pmax, size_t m,
+                           int have_codebooks, const int32_t *nprobe_q, int allow_fallback, const int32_t *k_q,
+                           int kstride, int rerank, int nthreads, int64_t *out_ids, double *out_dist,
+                           int32_t *out_counts, float *out_bound);
+
 /* many queries, T threads, one query per thread at a time (CPU baseline) */
 void or_search_many(const float *qs, size_t nq, size_t d, const float *coarse, size_t coarse_k,
                     const int64_t *list_off, const int64_t *ids, const float *vecs,

@@ -359,3 +359,196 @@ def test_normalize_bit_exact_vs_reference_avx2(oracle):
             assert np.all(v == 0)
         else:
             assert abs(np.sqrt((v * v).sum()) - 1.0) < 1e-5
+
+
+# ------------------------------------------- build-owned random stream --
+def test_splitmix64_known_answers(oracle):
+    """The stream is splitmix64 (Steele, Lea and Flood, OOPSLA 2014): its
+    published test sequences from state 0 and from state 1234567."""
+    import ctypes
+    lib = oracle.lib()
+    st = (ctypes.c_uint64 * 1)(0)
+    assert [lib.or_rng_next(st) for _ in range(3)] == [0xE220A8397B1DCDAF, 0x6E789E6AA1B965F4, 0x06C45D188009454F]
+    st = (ctypes.c_uint64 * 1)(1234567)
+    assert [lib.or_rng_next(st) for _ in range(5)] == [6457827717110365317, 3203168211198807973,
+                                                       9817491932198370423, 4593380528125082431,
+                                                       16408922859458223821]
+
+
+def _chi2(counts, expected):
+    return float((((counts - expected) ** 2) / expected).sum())
+
+
+@pytest.mark.parametrize("n,draws", [(2, 40000), (3, 60000), (7, 70000), (10, 100000), (1000, 200000)])
+def test_rng_bounded_uniform(oracle, n, draws):
+    """Bounded draws stay in [0, n) and are uniform (chi-squared below the
+    mean plus six standard deviations of its distribution)."""
+    import ctypes
+    lib = oracle.lib()
+    st = (ctypes.c_uint64 * 1)(n * 7919)
+    v = np.array([lib.or_rng_bounded(st, n) for _ in range(draws)], np.int64)
+    assert v.min() >= 0 and v.max() < n
+    c = np.bincount(v, minlength=n).astype(np.float64)
+    dof = n - 1
+    assert _chi2(c, draws / n) < dof + 6 * np.sqrt(2 * dof)
+
+
+def test_rng_perm_uniform_over_all_permutations(oracle):
+    """rand.Perm's inside-out shuffle (pqivf/index.go:805) gives every one of
+    the 24 permutations of 4 items equally often. Drawing j from [0, i)
+    instead of [0, i] (Sattolo's cycles) reaches only 6 of them, and a
+    biased bound skews the counts: both fail the chi-squared bound."""
+    import itertools
+    perms = {p: i for i, p in enumerate(itertools.permutations(range(4)))}
+    counts = np.zeros(24)
+    state = 42
+    for _ in range(48000):
+        p, state = oracle.perm(state, 4)
+        counts[perms[tuple(p.tolist())]] += 1
+    assert np.count_nonzero(counts) == 24
+    assert _chi2(counts, 2000.0) < 23 + 6 * np.sqrt(46)
+    # position marginals at n = 10: P(perm[i] = v) = 1/10
+    marg = np.zeros((10, 10))
+    for _ in range(20000):
+        p, state = oracle.perm(state, 10)
+        marg[np.arange(10), p] += 1
+    assert _chi2(marg.ravel(), 2000.0) < 99 + 6 * np.sqrt(198)
+
+
+def test_kmeans_seeds_are_perm_rows(oracle):
+    """runKMeans seeds centroid i with data[perm[i]] for the first k entries
+    of one permutation of the rows (pqivf/index.go:804-810): with no Lloyd
+    iteration the result is exactly those rows."""
+    rng = np.random.default_rng(31)
+    data = rng.standard_normal((500, 6)).astype(np.float32)
+    state = oracle.stream_seed(9, 0, 0)
+    got, st_after = oracle.kmeans(data, 37, 0, state)
+    p, st_perm = oracle.perm(state, 500)
+    assert np.array_equal(got, data[p[:37]])
+    assert st_after == st_perm
+    # k > n: every row, in permutation order
+    got, _ = oracle.kmeans(data[:20], 37, 0, state)
+    p, _ = oracle.perm(state, 20)
+    assert np.array_equal(got, data[:20][p])
+
+
+def test_train_stream_layout(oracle):
+    """Train draws the coarse k-means from stream role 0 and subquantizer i's
+    k-means from role 1 + i of (seed, epoch), over the residuals in list
+    order (pqivf/index.go:520-551 builds dataPerSub that way)."""
+    rng = np.random.default_rng(32)
+    x = (rng.uniform(-10, 10, (6, 8))[rng.integers(0, 6, 600)] + rng.standard_normal((600, 8))).astype(np.float32)
+    ids = np.arange(600, dtype=np.int64)
+    ix = oracle.train(ids, x, 6, 2, 8, 4, seed=77, epoch=3)
+    coarse, _ = oracle.kmeans(x, 6, 4, oracle.stream_seed(77, 3, 0))
+    assert np.array_equal(ix.coarse, coarse)
+    order = np.argsort(ix.assign, kind="stable")
+    res = x[order] - coarse[ix.assign[order]]
+    for i in range(2):
+        cb, _ = oracle.kmeans(np.ascontiguousarray(res[:, 4 * i:4 * i + 4]), 8, 4, oracle.stream_seed(77, 3, 1 + i))
+        assert np.array_equal(ix.codebooks[i, :len(cb)], cb)
+    # distinct roles and epochs give distinct streams
+    seeds = {oracle.stream_seed(77, e, r) for e in range(4) for r in range(40)}
+    assert len(seeds) == 160
+
+
+# ------------------------------------------------ search + exact rerank --
+def _rerank_fixture(oracle, n=3000, d=16, K=12, m=4, P=32, seed=3):
+    rng = np.random.default_rng(seed)
+    cent = (rng.random((K, d)) * 20 - 10).astype(np.float32)
+    x = (cent[rng.integers(0, K, n)] + rng.standard_normal((n, d))).astype(np.float32)
+    ids = rng.permutation(10 * n)[:n].astype(np.int64)
+    ix = oracle.train(ids, x, K, m, P, 5, seed=seed)
+    q = (x[rng.integers(0, n, 40)] + rng.standard_normal((40, d)).astype(np.float32) * 0.3).astype(np.float32)
+    return ix, q
+
+
+def _probed(ix, q, nprobe):
+    d64 = ((ix.coarse.astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+    return np.argsort(d64, kind="stable")[:nprobe]
+
+
+def _pq_ranks64(ix, q, entries):
+    """FP64 distance from q to the float32 reconstruction of each entry."""
+    lists = np.repeat(np.arange(len(ix.list_off) - 1), np.diff(ix.list_off))
+    m = ix.m
+    s = q.size // m
+    rec = np.concatenate([ix.codebooks[i][ix.codes[entries, i]] for i in range(m)], axis=1)
+    rec = (ix.coarse[lists[entries]] + rec).astype(np.float32)
+    return ((rec.astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+
+
+def test_search_rerank_all_candidates_is_exact_topk(oracle):
+    """When rerank * k covers every probed entry the result is the exact
+    top-k of the probed lists' stored vectors: FP64 brute force, ids equal
+    except at near-ties, distances the reference-order FP32 ones."""
+    ix, qs = _rerank_fixture(oracle)
+    for qi, q in enumerate(qs):
+        probed = _probed(ix, q, 2)
+        ents = np.concatenate([np.arange(ix.list_off[c], ix.list_off[c + 1]) for c in probed])
+        k = 10
+        R = (len(ents) + k - 1) // k + 1
+        ids, dist, cnt, bound = oracle.search_rerank_many(ix, q[None], k, 2, R)
+        assert cnt[0] == k and np.isinf(bound[0])
+        d64 = ((ix.vecs[ents].astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+        order = np.lexsort((ix.ids[ents], d64))[:k]
+        want = ix.ids[ents][order]
+        tie = np.sort(d64)
+        for r in range(k):
+            if ids[0, r] != want[r]:
+                assert abs(tie[r] - tie[min(r + 1, len(tie) - 1)]) < 1e-5 * (1 + tie[r]), qi
+        for r in range(k):
+            e = ents[ix.ids[ents] == ids[0, r]][0]
+            assert dist[0, r] == np.sqrt(np.float64(oracle.pair(0, q, ix.vecs[e])))
+
+
+@pytest.mark.parametrize("k,R,nprobe", [(5, 1, 3), (10, 3, 2), (20, 2, 4), (7, 4, 12)])
+def test_search_rerank_candidates_are_pq_topk(oracle, k, R, nprobe):
+    """The candidates are the R*k best probed entries by (PQ rank, storage
+    position) -- small codebooks make many code rows identical, so exact PQ
+    ties are common and the position decides them -- and the result is the
+    best k of those by (the stored vector's rank, id). PQ ranks are the
+    reference-order FP32 distance to the float32 reconstruction, within
+    rounding of the FP64 one; R = 1 returns the set Search returns."""
+    ix, qs = _rerank_fixture(oracle, seed=4)
+    ids, dist, cnt, bound = oracle.search_rerank_many(ix, qs, k, nprobe, R)
+    pid, _, _ = oracle.search_many(ix, qs, k, nprobe)
+    lists = np.repeat(np.arange(len(ix.list_off) - 1), np.diff(ix.list_off))
+    for qi, q in enumerate(qs):
+        probed = _probed(ix, q, nprobe)
+        ents = np.concatenate([np.arange(ix.list_off[c], ix.list_off[c + 1]) for c in probed])
+        rec = np.concatenate([ix.codebooks[i][ix.codes[ents, i]] for i in range(ix.m)], axis=1)
+        rec = (ix.coarse[lists[ents]] + rec).astype(np.float32)
+        pq32 = np.array([oracle.pair(0, q, r) for r in rec], np.float32)
+        pq64 = ((rec.astype(np.float64) - q.astype(np.float64)) ** 2).sum(1)
+        assert np.all(np.abs(pq32 - pq64) <= 1e-5 * (1 + pq64))
+        sel = min(k * R, len(ents))
+        cand = ents[np.lexsort((ents, pq32))[:sel]]
+        if k * R < len(ents):
+            assert bound[qi] == np.sort(pq32)[sel - 1]
+        else:
+            assert np.isinf(bound[qi])
+        ex = np.array([oracle.pair(0, q, ix.vecs[e]) for e in cand], np.float32)
+        best = np.lexsort((ix.ids[cand], ex))[:k]
+        assert cnt[qi] == min(k, sel)
+        assert ids[qi, :cnt[qi]].tolist() == ix.ids[cand][best].tolist()
+        assert np.array_equal(dist[qi, :cnt[qi]], np.sqrt(ex[best].astype(np.float64)))
+        if R == 1:
+            assert set(ids[qi, :cnt[qi]].tolist()) == set(pid[qi, :cnt[qi]].tolist())
+
+
+def test_search_rerank_fallback_rule(oracle):
+    """Fewer than k probed entries: every list is searched
+    (pqivf/index.go:892-899) and its rerank candidates come from all lists."""
+    ix, qs = _rerank_fixture(oracle, n=400, K=12, seed=5)
+    smallest = int(np.argmin(np.diff(ix.list_off)))
+    q = ix.coarse[smallest][None] + np.float32(0.001)
+    k = int(np.diff(ix.list_off)[smallest]) + 5
+    ids, dist, cnt, bound = oracle.search_rerank_many(ix, q, k, 1, 2)
+    assert cnt[0] == k
+    want = np.lexsort((ix.ids, ((ix.vecs.astype(np.float64) - q[0].astype(np.float64)) ** 2).sum(1)))
+    # the R * k candidates are drawn from every list, so the nearest stored
+    # vectors overall are found
+    assert set(ids[0, :5].tolist()) <= set(ix.ids[want[:2 * k]].tolist())
+    ids2, _, cnt2, _ = oracle.search_rerank_many(ix, q, k, 1, 2, allow_fallback=False)
+    assert cnt2[0] == k - 5

@@ -833,6 +833,46 @@ def test_device_search_on_two_streams(hann):
         assert np.allclose(dist.cpu().numpy(), wd, rtol=1e-6, atol=0)
 
 
+def test_mutation_waits_for_searches_in_flight(hann):
+    """A device-buffer search returns with its work still queued on the
+    caller's stream. Mutations right after it (no host synchronisation),
+    then a search on another stream that moves the device lists: the first
+    search's results are those of the index before the mutations, and the
+    second's those after them."""
+    import torch
+    from hann_amd import _lib
+    idx, x, qs = list_index(n=20000, coarse_k=64, seed=12)
+    kw = dict(scan_mode=_lib.SCAN_EXACT, rerank=2)
+    q = np.ascontiguousarray(np.concatenate([qs] * 4)[:1000])
+    before = idx.search_batch(q, 50, nprobe=64, **kw)
+    n = q.shape[0]
+
+    def bufs():
+        return (torch.from_numpy(q).cuda(), torch.full((n,), 50, dtype=torch.int32, device="cuda"),
+                torch.full((n,), 64, dtype=torch.int32, device="cuda"),
+                torch.empty((n, 50), dtype=torch.int64, device="cuda"),
+                torch.empty((n, 50), dtype=torch.float32, device="cuda"),
+                torch.empty(n, dtype=torch.int32, device="cuda"))
+    a, b = bufs(), bufs()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    # the exact scan over every list takes milliseconds: the mutations and
+    # the next sync are issued while it runs
+    idx.search_batch_dev(a[0].data_ptr(), n, a[1].data_ptr(), a[2].data_ptr(), 50, 64, a[3].data_ptr(),
+                         a[4].data_ptr(), a[5].data_ptr(), s1.cuda_stream, **kw)
+    gone = np.arange(0, 20000, 3)
+    idx.BulkDelete(gone)
+    idx.bulk_add_arrays(np.arange(50000, 50100), x[gone[:100]] + np.float32(0.5))
+    idx.search_batch_dev(b[0].data_ptr(), n, b[1].data_ptr(), b[2].data_ptr(), 50, 64, b[3].data_ptr(),
+                         b[4].data_ptr(), b[5].data_ptr(), s2.cuda_stream, **kw)
+    torch.cuda.synchronize()
+    after = idx.search_batch(q, 50, nprobe=64, **kw)
+    assert np.array_equal(a[5].cpu().numpy(), before[2])
+    assert np.array_equal(a[3].cpu().numpy(), before[0])
+    assert np.array_equal(b[3].cpu().numpy(), after[0])
+    assert not np.isin(after[0], gone).any()
+
+
 def _graph_buffers(n, kmax):
     import torch
     return (torch.empty((n, 64), dtype=torch.float32, device="cuda"),
