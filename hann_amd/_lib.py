@@ -95,6 +95,7 @@ SIGNATURES = {
     "hann_batch_topk": (cint, [cint, c_float_p, sz, c_float_p, sz, sz, sz, cint, sz, c_int64_p,
                                c_double_p]),
     "hann_batch_topk_dev": (cint, [cint, vp, sz, vp, sz, sz, sz, cint, sz, vp, vp, vp]),
+    "hann_batch_topk_noalloc_dev": (cint, [cint, vp, sz, vp, sz, sz, sz, vp, vp, vp]),
     "hann_assign_points": (cint, [c_float_p, sz, sz, c_float_p, sz, cint, c_int32_p]),
     "hann_assign_points_dev": (cint, [vp, sz, sz, vp, sz, cint, vp, vp]),
     "hann_kmeans": (cint, [c_float_p, sz, sz, sz, cint, u64, cint, cint, c_float_p, c_size_p]),

@@ -2,6 +2,8 @@
 //
 // All FP32 kernels here reproduce the reference's AVX2+FMA arithmetic
 // (core/simd_kernels.inc.h) bit for bit; see common.h for the order.
+#include <mutex>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -289,28 +291,41 @@ int kp_for(int k) {
     return kp;
 }
 
-template <int KP>
-static void topk_launch(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
-                        int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws,
-                        hipStream_t st) {
+// the tiled kernel holds 16 queries' selection buffers; it covers the
+// (squared) Euclidean metrics up to KP 256
+static bool topk_tiled(int metric, int kp) {
     const int rank_metric = metric == HANN_METRIC_EUCLIDEAN ? HANN_METRIC_SQEUCLIDEAN : metric;
-    // the LDS-tiled kernel holds 16 queries' selection buffers; it covers
-    // the (squared) Euclidean metrics up to KP 256
-    const bool tiled = rank_metric == HANN_METRIC_SQEUCLIDEAN && KP <= 256;
-    const int64_t qt = tiled ? dev::TK_QT : 1;
+    return rank_metric == HANN_METRIC_SQEUCLIDEAN && kp <= 256;
+}
+
+// Row ranges of the exact top-k: enough that the grid holds a few thousand
+// workgroups, at most max_split (>= 1) ranges of a multiple of 64 rows.
+static void topk_plan(int metric, int kp, int64_t nq, int64_t nb, int64_t max_split, int64_t &nsplit,
+                      int64_t &rows_per_split) {
+    const int64_t qt = topk_tiled(metric, kp) ? dev::TK_QT : 1;
     const int64_t qtiles = ceil_div(nq, qt);
-    // split the rows so that the grid holds a few thousand workgroups
-    int64_t nsplit = ceil_div(4096, qtiles);
-    const int64_t max_split = ceil_div(nb, 4 * 64);
+    nsplit = ceil_div(4096, qtiles);
+    const int64_t fill = ceil_div(nb, 4 * 64);
+    if (nsplit > fill) nsplit = fill;
     if (nsplit > max_split) nsplit = max_split;
     if (nsplit < 1) nsplit = 1;
     if (nsplit > 65535) nsplit = 65535;
-    int64_t rows_per_split = ceil_div(ceil_div(nb, nsplit), 64) * 64;
+    rows_per_split = ceil_div(ceil_div(nb, nsplit), 64) * 64;
     if (rows_per_split < 64) rows_per_split = 64;
     nsplit = ceil_div(nb, rows_per_split);
     if (nsplit < 1) nsplit = 1;
-    float *pkey = ws.get<float>(14, (size_t)(nq * nsplit * k));
-    int64_t *pid = ws.get<int64_t>(15, (size_t)(nq * nsplit * k));
+}
+
+// the partial lists of every (query, row range), then their merge; pkey and
+// pid hold nq * nsplit * k entries each
+template <int KP>
+static void topk_run(int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
+                     int64_t nsplit, int64_t rows_per_split, float *pkey, int64_t *pid, int64_t *out_ids,
+                     double *out_dist, hipStream_t st) {
+    const int rank_metric = metric == HANN_METRIC_EUCLIDEAN ? HANN_METRIC_SQEUCLIDEAN : metric;
+    const bool tiled = topk_tiled(metric, KP);
+    const int64_t qt = tiled ? dev::TK_QT : 1;
+    const int64_t qtiles = ceil_div(nq, qt);
     for (int64_t qt0 = 0; qt0 < qtiles; qt0 += 65535) {
         const int64_t qtn = qtiles - qt0 < 65535 ? qtiles - qt0 : 65535;
         const int64_t qb = qt0 * qt;
@@ -339,19 +354,78 @@ static void topk_launch(int metric, const float *queries, int64_t nq, const floa
     }
 }
 
+static void topk_dispatch(int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim,
+                          int k, int64_t nsplit, int64_t rows_per_split, float *pkey, int64_t *pid,
+                          int64_t *out_ids, double *out_dist, hipStream_t st) {
+    switch (kp_for(k)) {
+    case 64: topk_run<64>(metric, queries, nq, base, nb, dim, k, nsplit, rows_per_split, pkey, pid, out_ids, out_dist, st); break;
+    case 128: topk_run<128>(metric, queries, nq, base, nb, dim, k, nsplit, rows_per_split, pkey, pid, out_ids, out_dist, st); break;
+    case 256: topk_run<256>(metric, queries, nq, base, nb, dim, k, nsplit, rows_per_split, pkey, pid, out_ids, out_dist, st); break;
+    case 512: topk_run<512>(metric, queries, nq, base, nb, dim, k, nsplit, rows_per_split, pkey, pid, out_ids, out_dist, st); break;
+    default:
+        fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
+    }
+}
+
 void launch_topk_exact(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
                        int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws,
                        hipStream_t st) {
     if (nq == 0) return;
-    int kp = kp_for(k);
-    switch (kp) {
-    case 64: topk_launch<64>(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st); break;
-    case 128: topk_launch<128>(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st); break;
-    case 256: topk_launch<256>(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st); break;
-    case 512: topk_launch<512>(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st); break;
-    default:
-        fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
+    const int kp = kp_for(k);
+    if (kp > 512) fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
+    int64_t nsplit = 0, rows = 0;
+    topk_plan(metric, kp, nq, nb, 65535, nsplit, rows);
+    float *pkey = ws.get<float>(14, (size_t)(nq * nsplit * k));
+    int64_t *pid = ws.get<int64_t>(15, (size_t)(nq * nsplit * k));
+    topk_dispatch(metric, queries, nq, base, nb, dim, k, nsplit, rows, pkey, pid, out_ids, out_dist, st);
+}
+
+// ------------------------------------------------- allocation-free top-k --
+// The partial lists of hann_batch_topk_noalloc_dev live in this array: one
+// copy per GPU, held until the process ends, shared by every call on that
+// GPU (calls that overlap on one GPU race on it).
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static std::mutex g_scratch_mu;
+constexpr int kMaxDevices = 64;
+static void *g_scratch_dev[kMaxDevices];   // scratch's device address on each GPU
+
+// the device address of `scratch` on the current device, looked up once per
+// device with that device current
+void *noalloc_scratch_current() {
+    int dev = -1;
+    HANN_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) fail(HANN_ERR_DEVICE, strf("device %d is out of the supported range", dev));
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            fail(HANN_ERR_DEVICE, strf("hipGetSymbolAddress(scratch) failed on device %d: %s", dev,
+                                       hipGetErrorString(e)));
+        g_scratch_dev[dev] = ptr;
     }
+    return g_scratch_dev[dev];
+}
+
+void launch_topk_exact_noalloc(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
+                               int dim, int k, int64_t *out_ids, double *out_dist, hipStream_t st) {
+    if (nq == 0) return;
+    const int kp = kp_for(k);
+    if (kp > 512) fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
+    // 12 bytes per partial entry (a float key and an int64 row)
+    const int64_t per_split = nq * (int64_t)k * 12;
+    const int64_t max_split = (int64_t)kNoAllocScratchBytes / per_split;
+    if (max_split < 1)
+        fail(HANN_ERR_INVALID_ARG,
+             strf("allocation-free top-k: nq * k * 12 = %lld bytes exceeds its scratch of %zu bytes",
+                  (long long)per_split, kNoAllocScratchBytes));
+    int64_t nsplit = 0, rows = 0;
+    topk_plan(metric, kp, nq, nb, max_split, nsplit, rows);
+    char *base_ptr = static_cast<char *>(noalloc_scratch_current());
+    int64_t *pid = reinterpret_cast<int64_t *>(base_ptr);
+    float *pkey = reinterpret_cast<float *>(base_ptr + nq * nsplit * k * 8);
+    topk_dispatch(metric, queries, nq, base, nb, dim, k, nsplit, rows, pkey, pid, out_ids, out_dist, st);
 }
 
 }  // namespace hann

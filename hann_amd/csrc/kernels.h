@@ -39,5 +39,13 @@ void launch_distance_batch(int metric, const float *queries, int64_t nq, const f
 void launch_topk_exact(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
                        int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws,
                        hipStream_t st);
+// The same without allocating: the partial lists live in a static device
+// array of kNoAllocScratchBytes per GPU (nq * k * 12 bytes must fit).
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+void launch_topk_exact_noalloc(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
+                               int dim, int k, int64_t *out_ids, double *out_dist, hipStream_t st);
+// the static array's device address on the current device (looked up once
+// per device)
+void *noalloc_scratch_current();
 
 }  // namespace hann

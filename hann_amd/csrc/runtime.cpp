@@ -111,6 +111,8 @@ int hann_init(int device) {
     HANN_API_BEGIN
     select_device(device);
     (void)global_ctx();
+    // the allocation-free top-k's scratch address on this device
+    (void)noalloc_scratch_current();
     HANN_API_END
 }
 
@@ -276,6 +278,22 @@ int hann_batch_topk_dev(int metric, const float *queries, size_t nq, const float
     std::unique_lock<std::mutex> lk(c.mu);
     batch_topk(c, metric, queries, (int64_t)nq, base, (int64_t)nb, (int)dim, (int)k, precision,
                (int)overfetch, out_ids, out_dist, st);
+    if (!stream) sync(st);
+    HANN_API_END
+}
+
+int hann_batch_topk_noalloc_dev(int metric, const float *queries, size_t nq, const float *base, size_t nb,
+                                size_t dim, size_t k, int64_t *out_ids, double *out_dist, void *stream) {
+    HANN_API_BEGIN
+    check_metric(metric);
+    if (dim == 0) fail(HANN_ERR_INVALID_ARG, "query must not be empty");
+    if (k == 0) fail(HANN_ERR_INVALID_ARG, "k must be positive");
+    if (nq == 0) return HANN_OK;
+    ensure_device();
+    // NULL: the device's null stream, and the call waits
+    hipStream_t st = (hipStream_t)stream;
+    launch_topk_exact_noalloc(metric, queries, (int64_t)nq, base, (int64_t)nb, (int)dim, (int)k, out_ids, out_dist,
+                              st);
     if (!stream) sync(st);
     HANN_API_END
 }

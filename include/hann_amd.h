@@ -118,6 +118,20 @@ int hann_batch_topk_dev(int metric, const float *queries, size_t nq, const float
                         size_t dim, size_t k, int precision, size_t overfetch, int64_t *out_ids,
                         double *out_dist, void *stream);
 
+/* The FP32 exact form of hann_batch_topk_dev without any allocation: no
+ * counterpart in the reference. It is for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). Its kernels keep their per-(query, row
+ * range) partial lists in a static device array of 32 MiB per GPU, held
+ * until the process ends, so nq * k * 12 bytes must be at most 32 MiB
+ * (HANN_ERR_INVALID_ARG otherwise). Results are identical to
+ * hann_batch_topk_dev with HANN_PREC_FP32. Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (on different streams or from
+ * different threads) race and must be ordered by the caller. stream NULL:
+ * the device's null stream, and the call waits. */
+int hann_batch_topk_noalloc_dev(int metric, const float *queries, size_t nq, const float *base, size_t nb,
+                                size_t dim, size_t k, int64_t *out_ids, double *out_dist, void *stream);
+
 /* ------------------------------------------------------------- k-means --
  * assignPoints (pqivf/index.go:738-789): out[i] = nearest centroid of row i,
  * lowest index on a tie. precision HANN_PREC_BF16 uses an MFMA pass and
