@@ -1042,11 +1042,12 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
                       reinterpret_cast<uintptr_t>(dq) % 16 != 0 || np > select_rows_max_probes() ||
                       !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
-                      list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax) > 160 * 1024 ||
+                      list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2) > 160 * 1024 ||
+                      select_records_lds_bytes(list_select_regions(np)) > 160 * 1024 ||
                       (rows_max = list_rows_bound(nq, np)) > kListRowsMax)) {
         // the list-major scan needs a rerank, codes for every entry, a probe
-        // row per query, a supported shape and room for the batch's
-        // distance rows; otherwise the persistent table scan runs
+        // row per query, a supported shape and room for the batch's rows and
+        // a query's probed list chunks; otherwise the persistent table scan runs
         list_scan = false;
         mode = SCAN_MODE_ADC_F32;
     }
@@ -1187,7 +1188,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     const int64_t items_max = ceil_div(pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
     const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
                  o_lfill = carve((size_t)K * 4), o_tot = carve(16), o_qtot = carve((size_t)nq * 8),
-                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_qrange = carve((size_t)nq * 8),
+                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_theta = carve((size_t)nq * 8),
                  o_ovf = carve((size_t)nq * 4), o_ctr = carve(16), o_parts = carve(kListScanParts * 4), o_pq = carve((size_t)pairs_max * 4),
                  o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
                  o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
@@ -1202,7 +1203,6 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int64_t *qtot = reinterpret_cast<int64_t *>(at(o_qtot));
     int64_t *qslots = reinterpret_cast<int64_t *>(at(o_qslots));
     int64_t *roff = reinterpret_cast<int64_t *>(at(o_roff));
-    unsigned *qrange = reinterpret_cast<unsigned *>(at(o_qrange));
     int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
     int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [1] fallbacks, [2] fallback queue
     int32_t *parts = reinterpret_cast<int32_t *>(at(o_parts));   // items taken per partition
@@ -1217,10 +1217,13 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         HANN_HIP(hipGetDevice(&dev));
         HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    const size_t shm = list_scan_lds_bytes(x3, (int)dim, (int)m, (int)pmax);
+    const size_t shm = list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2);
     const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / shm));
-    // every (query, entry) key of the batch: query rows, probes in order
-    uint16_t *dist = w.get<uint16_t>(18, (size_t)rows_max);
+    // one 32-bit slot per row slot: the sample pass's 16-bit keys, then the
+    // filter pass's records; and a record count per (chunk, half) region
+    uint32_t *rec = w.get<uint32_t>(18, (size_t)rows_max);
+    uint16_t *rcnt = w.get<uint16_t>(19, (size_t)rows_max / 4 + 8);
+    uint32_t *theta = reinterpret_cast<uint32_t *>(at(o_theta));   // [nq]
 
     // 1. the batch's (query, list) pairs grouped by list, as work items, and
     //    each query's row
@@ -1231,11 +1234,6 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, roff, lfill, pair_q, pair_qc, pair_scale,
                         pair_off, st);
     launch_make_items(lcnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(o_iord)), items, st);
-    // 2. the scan
-    launch_fill_i32(ctr, 4, 0, st);
-    launch_fill_i32(parts, kListScanParts, 0, st);
-    launch_fill_i32(reinterpret_cast<int32_t *>(qrange), nq, -1, st);
-    launch_fill_i32(reinterpret_cast<int32_t *>(qrange) + nq, nq, 0, st);
     ListScanArgs la{};
     la.queries = a.queries;
     la.d = (int)dim;
@@ -1257,15 +1255,28 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.n_items = totals + 1;
     la.item_counter = parts;
     la.n_parts = std::min(kListScanParts, grid);
-    la.dist = dist;
+    la.dist = reinterpret_cast<uint16_t *>(rec);
     la.dist_bytes = (uint32_t)(rows_max * 2);
-    la.qrange = qrange;
+    la.theta = theta;
+    la.rec = rec;
+    la.cnt = rcnt;
+    la.cnt_bytes = (uint32_t)((rows_max / 4 + 8) * 2);
+    // 2. the sample pass: keys of each list's first slabs; then each
+    //    query's bound theta, the (k R)-th smallest of its sample's keys
+    launch_fill_i32(ctr, 4, 0, st);
+    launch_fill_i32(parts, kListScanParts, 0, st);
+    launch_scan_list_kind(la, x3, 1, grid, st);
+    launch_theta(a, nq, reinterpret_cast<const uint16_t *>(rec), roff, qtot, theta, st);
+    // 3. the filter pass: every (query, entry) key, and the records of those
+    //    <= theta compacted per (chunk, half) region
+    launch_fill_i32(parts, kListScanParts, 0, st);
     mark_kernel(0, st);
-    launch_scan_list(la, x3, grid, st);
+    launch_scan_list_kind(la, x3, 2, grid, st);
     mark_kernel(1, st);
-    // 3. the selk best row entries per query, as the rerank's rows
-    launch_select_rows(a, nq, dist, roff, qtot, qslots, qrange, a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
-    // 4. brute-force fallbacks: the table scan over all lists (the
+    // 4. the selk best records per query, as the rerank's rows
+    const int max_regions = list_select_regions(np);
+    launch_select_records(a, nq, rec, rcnt, roff, qtot, a.out_id, a.out_cnt, selk, max_regions, ovf, ctr + 1, st);
+    // 5. brute-force fallbacks: the table scan over all lists (the
     //    reference's rule, counter included)
     FusedArgs f{};
     f.term = d_term.get();
@@ -1282,6 +1293,13 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     // fallbacks are rare: one workgroup per CU drains the queue
     const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus);
     launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
+}
+
+int PQIVF::list_select_regions(int np) const {
+    // two halves of every chunk of np probed lists at most
+    int64_t max_ch = 1;
+    for (int64_t v : dev_len) max_ch = std::max(max_ch, ceil_div(v, (int64_t)list_scan_chunk()));
+    return (int)std::min<int64_t>(2 * (int64_t)std::max(np, 1) * max_ch, 1 << 20);
 }
 
 int64_t PQIVF::list_rows_bound(int64_t nq, int np) const {

@@ -203,6 +203,8 @@ struct PQIVF {
                        Workspace &w, int64_t rows_max);
     // entries in a batch's distance rows at most: nq times the np longest lists
     int64_t list_rows_bound(int64_t nq, int np) const;
+    // regions (chunk halves) of a query's records in the list-major select at most
+    int list_select_regions(int np) const;
     // the list-major scan's distance rows stay within 2^31 floats (8 GiB)
     static constexpr int64_t kListRowsMax = (int64_t(1) << 31) - 64;   // 16-bit keys, below 2^32 - 64 bytes
     // stage timers (HIP events on the search stream), accumulated when enabled

@@ -24,10 +24,15 @@ struct ListScanArgs {
     const int32_t *n_items;      // device count of items
     int32_t *item_counter;       // [n_parts] zeroed before the launch
     int n_parts;                 // item partitions (at most the grid)
-    uint16_t *dist;              // the batch's rows of 16-bit ordered keys: query q's, its probes in
-                                 // order, each probe's run padded to a multiple of 8 slots (16-byte aligned)
+    uint16_t *dist;              // the sample pass's 16-bit keys in the batch's rows: query q's, its
+                                 // probes in order, each probe's run padded to a multiple of 8 slots
     uint32_t dist_bytes;         // size of dist in bytes (below 2^32 - 64)
-    unsigned *qrange;            // [2][nq] smallest / largest key of each row (min set to ~0, max to 0)
+    // the filter pass
+    const uint32_t *theta;       // [nq] keys <= theta[q] survive
+    uint32_t *rec;               // records (key << 16 | entry in chunk), compacted per (pair, chunk, half)
+                                 // region of the row slots
+    uint16_t *cnt;               // a region's record count at (row slot of its chunk start) / 4 + half
+    uint32_t cnt_bytes;          // size of cnt in bytes (below 2^32 - 64)
 };
 
 // entries of a list per work item of the list-major scan
@@ -59,16 +64,22 @@ int list_item_order_ints();
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);
-size_t list_scan_lds_bytes(int x3, int d, int m, int P);
-int list_scan_threads();
-void launch_scan_list(const ListScanArgs &la, int x3, int grid, hipStream_t st);
-// the select kernel keeps a query's probe table in LDS up to this many probes
+// kind 1: the sample pass's keys (the first slabs of each list); 2: the
+// filter pass's records
+void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st);
+size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
+// per query, the bound theta of the filter pass from the sample's keys
+void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
+                  uint32_t *theta, hipStream_t st);
+// per query, the selk = k_q * R best entries by (key, entry position) from the
+// filter pass's records, as entry positions (-1 after the last); the
+// brute-force fallbacks go to ovf. max_regions: at least twice the probed
+// list chunks of any query.
+size_t select_records_lds_bytes(int max_regions);
+void launch_select_records(const ScanArgs &a, int64_t nq, const uint32_t *rec, const uint16_t *cnt,
+                           const int64_t *row_off, const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt,
+                           int out_stride, int max_regions, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st);
+// probes per query the threshold and select kernels take at most
 int select_rows_max_probes();
-// per query, the selk = k_q * R best row entries by (key, position) as
-// positions in out_pos rows (-1 after the last); queries that need the
-// brute-force fallback go to ovf (count in *ovf_cnt, zeroed by the caller)
-void launch_select_rows(const ScanArgs &a, int64_t nq, const uint16_t *dist, const int64_t *row_off,
-                        const int64_t *qtot, const int64_t *qslots, const unsigned *qrange, int64_t *out_pos,
-                        int32_t *out_cnt, int out_stride, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st);
 
 }  // namespace hann

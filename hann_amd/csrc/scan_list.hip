@@ -7,24 +7,30 @@
 // and for a list l probed by queries Q_l the inner products are one product
 // of the decoded list [n_l x d] with Q_l^T [d x |Q_l|]. The pairs (query,
 // probed list) of a batch are grouped by list; a work item is one chunk of up
-// to 1,024 entries of a list with up to 32 of its queries, and belongs to one
+// to 2,048 entries of a list with up to 32 of its queries, and belongs to one
 // wave. The wave gathers 32 entries at a time from a BF16 codebook in LDS
 // (BF16 hi + lo for the split mode) straight into the A operand of
 // v_mfma_f32_32x32x16_bf16 and multiplies them with the item's queries,
-// which stay in registers as the B operand. Each list's codes are read once
-// per item instead of once per (query, list).
+// which stay in registers as the B operand (scaled by -2 and the query's
+// key scale); the accumulator starts at the entry's and the pair's terms, so
+// it ends at the scaled distance. Each list's codes are read once per item
+// instead of once per (query, list).
 //
-// Every (query, entry) distance of the batch is written as a 16-bit key:
-// query q owns a row of its probed lists' entries, probes in order, so each
-// pair's keys land at a fixed offset and no candidate test, threshold or
-// atomic append is involved. A key is the FP16 value of the distance times a
-// power of two chosen per query (so that the largest distance any of its
-// probed entries can have stays far below FP16's range), with its bits mapped
-// so that unsigned order is value order. The scan also keeps each row's
-// smallest and largest key. A select kernel then takes, per query, the
-// selk = k * R best row entries by (key, position) with a range histogram over
-// the row (refined while the boundary bin is too large to sort), and those are
-// reranked in exact FP32.
+// A key is the FP16 value of the distance times a power of two chosen per
+// query (so that no probed entry's distance reaches FP16's range), with its
+// bits mapped so that unsigned order is value order. A query's rerank
+// candidates are its selk = k * R best probed entries by (key, entry
+// position). They are found without writing every key:
+//  1. the sample pass scores the first slabs of every list (about an
+//     eighth of the entries) and writes their keys into the query's row;
+//  2. k_theta takes, per query, a bound theta >= the selk-th smallest sample
+//     key -- at least selk entries have keys <= theta, and the selk best do;
+//  3. the filter pass scores every entry again, with bit-identical keys, and
+//     writes only the (key, entry) records with key <= theta, each lane into
+//     its own (query, chunk, half) region of the row, four at a time;
+//  4. k_select_records picks the selk best records per query.
+// The selection is the same as over every key; those are reranked in exact
+// FP32.
 #include "common.h"
 #include "scan_list.h"
 
@@ -38,10 +44,7 @@ namespace dev {
 constexpr int LQ = 32;          // queries per work item (the MFMA's N)
 constexpr int LCH_SHIFT = 11;
 constexpr int LCH = 1 << LCH_SHIFT;   // entries per work item (a chunk of the list; a multiple of 32)
-constexpr int SEL_BINS = 2048;  // histogram bins of the row select
-constexpr int SEL_ROOM = 1024;  // boundary-bin entries the select sorts in LDS (8 workgroups per CU)
-constexpr int SEL_MAXP = 2048;  // probes per query in the select's LDS table
-constexpr int SEL_SAMP = 8;     // the select's sample: every 8th 128-byte block of a row
+constexpr int SEL_MAXP = 2048;  // probes per query the threshold and select kernels take
 
 __global__ void k_fill_i32(int32_t *__restrict__ p, int64_t n, int32_t v) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -55,27 +58,35 @@ __device__ __forceinline__ int clamp_nprobe(const ScanArgs &a, int64_t q) {
     return npq < 0 ? 0 : npq;
 }
 
-// 16-bit keys of two scaled distances, packed: each is clamped to
-// [0, +inf] on its bits (negative values and -0 become +0, NaN becomes
-// +inf) and converted to FP16 rounding toward zero, a monotone map. The
-// FP16 bits of non-negative values order like the values as unsigned
-// integers, from 0x0000 up to +inf = 0x7C00.
-typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+// 16-bit keys of two scaled distances, packed: each is clamped on its bits
+// as a signed integer to [0, +inf] (negative values, -0 and negative NaNs
+// become +0, positive NaNs +inf) and converted to FP16 rounding toward zero.
+// The key is thus a non-decreasing function of the value's bits read as a
+// signed integer, from 0x0000 up to +inf = 0x7C00, and the FP16 bits order
+// like the values as unsigned integers.
 typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float key_clamp(float v) {
     const int b = __float_as_int(v);
-    const unsigned c = (unsigned)(b > 0 ? b : 0);
-    return __uint_as_float(c < 0x7F800000u ? c : 0x7F800000u);
+    return __int_as_float(min(max(b, 0), 0x7F800000));
 }
 __device__ __forceinline__ uint32_t key16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(key_clamp(a), key_clamp(b)));
 }
+// the key of v in the upper 16 bits (the lower 16 are zero)
+__device__ __forceinline__ uint32_t key16_hi(float v) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(0.f, key_clamp(v)));
+}
+// the largest signed bit pattern b with key(b) <= theta: key(b) <= theta
+// exactly when (int)b <= key_bound(theta). Below +inf, theta + 1 is a
+// representable FP16 value V, RTZ(v) <= theta exactly when v < V, and
+// positive floats order like their bits; from +inf up every key passes.
+__device__ __forceinline__ int key_bound(uint32_t theta) {
+    if (theta >= 0x7C00u) return 0x7FFFFFFF;
+    const float v = (float)__builtin_bit_cast(_Float16, (unsigned short)(theta + 1u));
+    return __float_as_int(v) - 1;
+}
 __device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
     return __builtin_bit_cast(uint32_t, __builtin_elementwise_min(__builtin_bit_cast(u16x2, a),
-                                                                  __builtin_bit_cast(u16x2, b)));
-}
-__device__ __forceinline__ uint32_t max_u16x2(uint32_t a, uint32_t b) {
-    return __builtin_bit_cast(uint32_t, __builtin_elementwise_max(__builtin_bit_cast(u16x2, a),
                                                                   __builtin_bit_cast(u16x2, b)));
 }
 
@@ -341,9 +352,17 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
 //
 // D register r of lane l holds entry 8 (r / 4) + 4 h + r % 4 of the slab for
 // query l % 32, so the four registers r = 4g .. 4g + 3 are four consecutive
-// entries of one query's row: each becomes one 8-byte store of four keys,
-// with no transposition. The entry terms of a slab pass through a 32-float
-// LDS slot of the wave, read back as float4 broadcasts.
+// entries of one query's row: in the sample pass each becomes one 8-byte
+// store of four keys, with no transposition. The entry terms of a slab pass
+// through a 32-float LDS slot of the wave, read back as float4 broadcasts.
+//
+// The filter pass tests a register's value against the lane's bound on the
+// value bits (no key conversion), takes the wave's ballot and, where any lane
+// passes, appends the passing lanes' records with their row positions to a
+// ring of the wave in LDS at (fill + lanes before it); full blocks of 64 are
+// written out by the whole wave, at the start of a slab before its loads
+// (so the compiler's count of memory operations in flight never has to
+// include a store issued after the loads it waits for).
 //
 // Scheduling: items are split into partitions (item i belongs to partition
 // i mod n_parts) with a counter each, and a workgroup takes the items of
@@ -355,12 +374,26 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
 // LDS) overlap each other's loads.
 constexpr int LNW = 8;
 
+// what a launch of the list-major scan writes
+enum { KIND_SAMPLE = 1, KIND_FILTER = 2 };
+// the sample pass scores the first ceil(slabs / SAMP_STRIDE) 32-entry slabs of each list
+constexpr int SAMP_STRIDE = 8;
+__device__ __forceinline__ int64_t sample_slots(int64_t len) {
+    const int64_t slabs = (len + 31) >> 5;
+    const int64_t s = 32 * ((slabs + SAMP_STRIDE - 1) / SAMP_STRIDE);
+    const int64_t r = row_slots(len);
+    return s < r ? s : r;
+}
+// the filter pass's ring of (record, row position) pairs per wave; it holds
+// fewer than 64 between slabs, and a register adds at most 64
+constexpr int RING = 256;
+
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 // a buffer offset past any key buffer (key buffers hold less than 2^32 - 64
 // bytes), for lanes whose store is to be discarded
 constexpr uint32_t kNoStore = 0xFFFFFFF0u;
 
-template <bool X3, int M>
+template <bool X3, int M, int KIND>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     constexpr int D = 4 * M;
@@ -370,6 +403,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
     bf16x4 *sCBl = sCBh + (size_t)M * P;                          // [M][P] (split mode)
     float *s_term = reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) + wave * 32;
+    // filter pass: the wave's ring of pending records
+    u32x2 *s_ring = reinterpret_cast<u32x2 *>(reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) +
+                                              LNW * 32) + (size_t)wave * RING;
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(la.codes);
 
     // the codebook, once per workgroup
@@ -413,7 +449,19 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // groups of four entries starting below slot_end are stored (the
         // last chunk pads its query runs to a multiple of 8 slots)
         S.slot_end = S.ee == n ? (int)row_slots(n) : S.ee;
-        S.ns = (S.ee - it.w + 31) >> 5;
+        if (KIND == KIND_SAMPLE) {
+            // the sample: chunk 0's first slabs; other chunks score one
+            // empty slab and store nothing
+            const int ss = (int)sample_slots(n);
+            if (it.w == 0) {
+                S.ee = S.ee < ss ? S.ee : ss;
+                S.slot_end = ss;
+            } else {
+                S.ee = it.w;
+                S.slot_end = it.w;
+            }
+        }
+        S.ns = S.ee > it.w ? (S.ee - it.w + 31) >> 5 : 1;
         S.pc = it.z;
         const int pidx = it.y + (col < it.z ? col : 0);
         S.qid = la.pair_q[pidx];
@@ -423,8 +471,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         return S;
     };
     // one slab's loads: this lane's half code row of entry b + col and
-    // that entry's term (rows past the chunk read its last entry and are
-    // discarded, so the loads are unconditional)
+    // that entry's term (rows past the chunk read its last entry, or the
+    // entry before an empty chunk, and are discarded, so the loads are
+    // unconditional)
     struct Slab {
         uint32_t w[HW];
         float t;
@@ -436,9 +485,12 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         for (int u = 0; u < HW; ++u) x.w[u] = cw[u];
         x.t = la.term[e0 + e];
     };
+    // item indices are wave-uniform (taken from lane 0 into a scalar), so
+    // item headers load through the scalar unit and the slab loop's
+    // control flow is uniform
     int j = 0;
     if (lane == 0) j = atomicAdd(ctr, 1);
-    int item = part + __shfl(j, 0) * nparts;
+    int item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
     ItemState S = load_item(item), SN = S;
     // slab s of the stream that runs through the current item and on into
     // the next (past both: a discarded reload of the current item's last)
@@ -458,15 +510,39 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     // the key rows as a buffer resource of dist_bytes bytes (raw, gfx9 layout)
     const __amdgpu_buffer_rsrc_t drs =
         __builtin_amdgcn_make_buffer_rsrc(la.dist, (short)0, (int)la.dist_bytes, 0x00020000);
+    // the filter pass's region counts as a buffer resource of cnt_bytes bytes
+    const __amdgpu_buffer_rsrc_t crs =
+        __builtin_amdgcn_make_buffer_rsrc(la.cnt, (short)0, (int)la.cnt_bytes, 0x00020000);
+    // the ring's pending records are [head, fill) (wave-uniform counters)
+    uint32_t fill = 0, head = 0;
+    // the whole wave writes n <= 64 pending records (lane i the i-th)
+    auto flush = [&](uint32_t n) {
+        wave_sync();   // the ring's appends by other lanes
+        const u32x2 v = s_ring[(head + (uint32_t)lane) & (RING - 1)];
+        if ((uint32_t)lane < n) la.rec[v.y] = v.x;
+        head = (uint32_t)__builtin_amdgcn_readfirstlane((int)(head + n));
+        wave_sync();   // the slots are read before they are appended to again
+    };
 
     while (item < n_items) {
         if (lane == 0) j = atomicAdd(ctr, 1);   // the next item, in flight with this one's loads
-        const int pc = S.pc, eb = S.eb, ee = S.ee, slot_end = S.slot_end;
-        const bool qv = col < pc;
+        const int eb = S.eb, ee = S.ee, slot_end = S.slot_end;
+        const bool qv = col < S.pc;
         const int qid = S.qid;
-        const float qc = S.qc, sc = S.sc;
+        const float sc = S.sc, qcs = S.qc * S.sc;
         const int off = S.off;
+        // filter pass: values with key <= theta survive, i.e. those whose bits
+        // are <= tb as a signed integer, into this lane's region of the
+        // chunk's row slots (its half: entries with e % 8 in [4h, 4h + 4))
+        const int tb = (KIND == KIND_FILTER && qv) ? key_bound(la.theta[qid]) : 0;
+        const uint64_t qmask = __builtin_amdgcn_ballot_w64(qv);
+        const int hcap = (slot_end - eb) >> 1;
+        const uint32_t rbase = (uint32_t)(off + eb + half * hcap);   // a multiple of 4
+        uint32_t dptr = rbase;   // this lane's next record
         // B operand of k-step jj: the query's dimensions 2 h M + 8 jj .. + 7
+        // times -2 sc (a power of two, so the BF16 rounding is scaled
+        // exactly); the accumulator starts at (t + |q - c|^2) sc, so it ends
+        // at the scaled key ((qc + t) - 2 <q, y>) sc
         bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
         {
             const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
@@ -474,7 +550,8 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             // that overwrote the loaded registers would first wait for every
             // memory operation of the wave)
             const uint32_t keep = qv ? 0xFFFFFFFFu : 0u;
-            auto z = [&](float v) { return __uint_as_float(__float_as_uint(v) & keep); };
+            const float m2sc = -2.f * sc;
+            auto z = [&](float v) { return __uint_as_float(__float_as_uint(v * m2sc) & keep); };
 #pragma unroll
             for (int jj = 0; jj < M / 4; ++jj) {
                 const float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
@@ -486,22 +563,53 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 }
             }
         }
-        const int next_item = part + __shfl(j, 0) * nparts;
+        const int next_item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
         SN = load_item(next_item);
         // b1 holds stream slab 1: this item's slab 1, or, for a one-slab
         // item, the next item's slab 0, which was not known when the ring
         // was filled
         if (S.ns == 1) load_at(1, b1);
-        // key = ((qc + t) - 2 <q, y>) * sc, as fma(-2 sc, acc, t sc + qc sc)
-        const float qcs = qc * sc, m2sc = -2.f * sc;
-        // smallest and largest key of this lane's query, two halves each
-        uint32_t kmin2 = 0xFFFFFFFFu, kmax2 = 0u;
+        // the filter pass's survivors of a slab: register r holds entry
+        // 8 (r / 4) + 4 h + r % 4 of the slab; CHECK: the slab runs past the
+        // chunk's end
+        auto survivors = [&](const f32x16 &acc, int base, auto check) {
+            // entry in chunk of register 0, with bits 0, 1, 3 and 4 free
+            const uint32_t e0r = (uint32_t)(base - eb) + 4u * (uint32_t)half;
+#pragma unroll
+            for (int r = 0; r < 16; ++r) {
+                const uint32_t cr = 8u * (uint32_t)(r >> 2) + (uint32_t)(r & 3);
+                // the wave's mask of passing lanes (compares straight into
+                // a lane mask: 41 and 40 are signed <= and <)
+                uint64_t mk = __builtin_amdgcn_sicmp(__float_as_int(acc[r]), tb, 41) & qmask;
+                if (decltype(check)::value) mk &= __builtin_amdgcn_sicmp((int)(e0r + cr), ee - eb, 40);
+                if (mk) {
+                    // room for 64 more (taken only when a slab has many)
+                    if (fill - head > (uint32_t)(RING - 64)) flush(64);
+                    if (__builtin_amdgcn_inverse_ballot_w64(mk)) {
+                        const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                            (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, fill));
+                        s_ring[pos & (RING - 1)] = u32x2{key16_hi(acc[r]) | e0r | cr, dptr};
+                        ++dptr;
+                    }
+                    fill = (uint32_t)__builtin_amdgcn_readfirstlane((int)(fill + (uint32_t)__builtin_popcountll(mk)));
+                }
+            }
+        };
         auto slab = [&](const Slab &x, int s) {
             const int base = eb + 32 * s;
             if (half == 0) s_term[col] = x.t;
+            wave_sync();   // s_term
+            // accumulator register r of lane l: entry 8 (r / 4) + 4 h + r % 4
+            // of the slab, for query l % 32
             f32x16 acc;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) acc[r] = 0.f;
+            for (int g = 0; g < 4; ++g) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[8 * g + 4 * half]);
+                acc[4 * g + 0] = __builtin_fmaf(t4.x, sc, qcs);
+                acc[4 * g + 1] = __builtin_fmaf(t4.y, sc, qcs);
+                acc[4 * g + 2] = __builtin_fmaf(t4.z, sc, qcs);
+                acc[4 * g + 3] = __builtin_fmaf(t4.w, sc, qcs);
+            }
 #pragma unroll
             for (int jj = 0; jj < M / 4; ++jj) {
                 const int i0 = half * (M / 2) + 2 * jj;
@@ -517,41 +625,47 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                     acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl[jj], acc, 0, 0, 0);
                 }
             }
-            wave_sync();   // s_term
             // every entry of the slab lies in the chunk except in its last
             // slab (a wave-uniform test)
             const bool full = base + 32 <= ee;
+            if (KIND == KIND_SAMPLE) {
+                // the 16-bit keys, two per word: word 2 g + v holds entries
+                // 8 g + 4 h + 2 v and + 1
+                uint32_t pk[8];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[8 * g + 4 * half]);
-                const int e4 = base + 8 * g + 4 * half;
-                uint32_t p0 = key16x2(__builtin_fmaf(m2sc, acc[4 * g + 0], __builtin_fmaf(t4.x, sc, qcs)),
-                                      __builtin_fmaf(m2sc, acc[4 * g + 1], __builtin_fmaf(t4.y, sc, qcs)));
-                uint32_t p1 = key16x2(__builtin_fmaf(m2sc, acc[4 * g + 2], __builtin_fmaf(t4.z, sc, qcs)),
-                                      __builtin_fmaf(m2sc, acc[4 * g + 3], __builtin_fmaf(t4.w, sc, qcs)));
-                if (full) {
-                    kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
-                    kmax2 = max_u16x2(kmax2, max_u16x2(p0, p1));
-                } else {
-                    // entries past the chunk: the padding key 0xFFFF (the
-                    // bits of an FP16 NaN, above every stored key, which are
-                    // non-negative and at most +inf), and no part in the range
-                    const uint32_t v0 = (e4 + 0 < ee ? 0x0000FFFFu : 0u) | (e4 + 1 < ee ? 0xFFFF0000u : 0u);
-                    const uint32_t v1 = (e4 + 2 < ee ? 0x0000FFFFu : 0u) | (e4 + 3 < ee ? 0xFFFF0000u : 0u);
-                    p0 = (p0 & v0) | ~v0;
-                    p1 = (p1 & v1) | ~v1;
-                    kmin2 = min_u16x2(kmin2, min_u16x2(p0, p1));
-                    kmax2 = max_u16x2(kmax2, max_u16x2(p0 & v0, p1 & v1));
+                for (int v = 0; v < 8; ++v) pk[v] = key16x2(acc[2 * v], acc[2 * v + 1]);
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int e4 = base + 8 * g + 4 * half;
+                    uint32_t p0 = pk[2 * g], p1 = pk[2 * g + 1];
+                    if (!full) {
+                        // entries past the chunk: the padding key 0xFFFF,
+                        // above every key (keys are at most +inf = 0x7C00)
+                        const uint32_t v0 = (e4 + 0 < ee ? 0x0000FFFFu : 0u) | (e4 + 1 < ee ? 0xFFFF0000u : 0u);
+                        const uint32_t v1 = (e4 + 2 < ee ? 0x0000FFFFu : 0u) | (e4 + 3 < ee ? 0xFFFF0000u : 0u);
+                        p0 = (p0 & v0) | ~v0;
+                        p1 = (p1 & v1) | ~v1;
+                    }
+                    // a buffer store for every lane: a lane without a key to
+                    // write takes an offset past the buffer's range, which the
+                    // hardware discards. With no branch around the store the
+                    // compiler counts the wave's memory operations exactly and
+                    // waits only for the loads it needs, not for every store.
+                    const uint32_t boff = (qv && e4 < slot_end) ? 2u * (uint32_t)(off + e4) : kNoStore;
+                    __builtin_amdgcn_raw_buffer_store_b64(u32x2{p0, p1}, drs, (int)boff, 0, 0);
                 }
-                // a buffer store for every lane: a lane without a key to
-                // write takes an offset past the buffer's range, which the
-                // hardware discards. With no branch around the store the
-                // compiler counts the wave's memory operations exactly and
-                // waits only for the loads it needs, not for every store.
-                const uint32_t boff = (qv && e4 < slot_end) ? 2u * (uint32_t)(off + e4) : kNoStore;
-                __builtin_amdgcn_raw_buffer_store_b64(u32x2{p0, p1}, drs, (int)boff, 0, 0);
+            } else if (full) {
+                survivors(acc, base, std::false_type{});
+            } else {
+                survivors(acc, base, std::true_type{});
             }
             wave_sync();   // s_term is rewritten by the next slab
+        };
+        // filter pass: full blocks of pending records, written before the
+        // next slab's loads are issued
+        auto drain = [&]() {
+            if (KIND == KIND_FILTER)
+                while (fill - head >= 64u) flush(64);
         };
         // the ring: b0 holds stream slab 0 and b1 stream slab 1; each step
         // loads slab s + 2 into the free buffer and computes slab s; on exit
@@ -559,6 +673,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         const int ns = S.ns;
         int s = 0;
         for (;;) {
+            drain();
             load_at(s + 2, b2);
             slab(b0, s);
             if (++s == ns) {
@@ -566,6 +681,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 b1 = b2;
                 break;
             }
+            drain();
             load_at(s + 2, b0);
             slab(b1, s);
             if (++s == ns) {
@@ -573,63 +689,200 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 b0 = b2;
                 break;
             }
+            drain();
             load_at(s + 2, b1);
             slab(b2, s);
             if (++s == ns) break;
         }
-        // the row's key range: lanes col and col + 32 share a query
-        unsigned kmin = kmin2 & 0xFFFFu, kmax = kmax2 & 0xFFFFu;
-        kmin = (kmin2 >> 16) < kmin ? (kmin2 >> 16) : kmin;
-        kmax = (kmax2 >> 16) > kmax ? (kmax2 >> 16) : kmax;
-        const unsigned omin = __shfl_xor(kmin, 32), omax = __shfl_xor(kmax, 32);
-        kmin = omin < kmin ? omin : kmin;
-        kmax = omax > kmax ? omax : kmax;
-        if (half == 0 && qv && kmin <= kmax) {
-            atomicMin(&la.qrange[qid], kmin);
-            atomicMax(&la.qrange[la.nq + qid], kmax);
+        if (KIND == KIND_FILTER) {
+            // the region's record count at (chunk start) / 4 + half, one
+            // buffer store for every lane (lanes without a query discard it)
+            const uint32_t coff = qv ? 2u * ((uint32_t)(off + eb) / 4u + (uint32_t)half) : kNoStore;
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(dptr - rbase), crs, (int)coff, 0, 0);
         }
         item = next_item;
         S = SN;
     }
+    // the last pending records
+    if (KIND == KIND_FILTER)
+        while (fill != head) flush(fill - head < 64u ? fill - head : 64u);
 }
 
-// One workgroup of 256 per query: the sk = min(k_q * R, row length) best
-// entries of the query's row by (key bits, position).
-//  1. A histogram of SEL_BINS bins over the row's key range (from the scan)
-//     finds the bin B where the count reaches sk; entries below B are all
-//     selected.
-//  2. If bin B holds at most SEL_ROOM entries they are sorted in LDS by
-//     (key bits, position) and the first ones complete the selection;
-//     otherwise the range narrows to bin B and step 1 repeats.
-//  3. A bin of one key value with more than SEL_ROOM entries (exact ties)
-//     is resolved by a radix select over the positions.
-// Positions come from the query's probe table (row offset and first slot
-// of each probed list) by binary search.
-__global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, const uint16_t *__restrict__ dist,
-                                                     const int64_t *__restrict__ row_off,
-                                                     const int64_t *__restrict__ qtot,
-                                                     const int64_t *__restrict__ qslots,
-                                                     const unsigned *__restrict__ qrange,
-                                                     int64_t *__restrict__ out_pos, int32_t *__restrict__ out_cnt,
-                                                     int out_stride, int32_t *__restrict__ ovf,
-                                                     int32_t *__restrict__ ovf_cnt) {
+// Selection helpers of the threshold and select kernels: histograms over
+// key bits, and the bin where a running count reaches a rank.
+// the bin of hist[0 .. nb) where the count reaches need: s_bin, with s_below
+// entries before it (wave 0: a block of nb / 64 bins a lane, then the bins
+// of the block that holds it, one a lane; nb / 64 <= 64)
+__device__ __forceinline__ void bins_find(const unsigned *hist, int nb, unsigned need, int wave, int lane, int *s_bin,
+                                          unsigned *s_below) {
+    if (wave == 0) {
+        const int per = nb / 64;
+        unsigned own = 0;
+        for (int b = 0; b < per; ++b) own += hist[lane * per + b];
+        unsigned incl = own;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const uint64_t hit = __ballot(incl >= need);
+        const int blk = hit ? __builtin_ctzll(hit) : 63;
+        const unsigned before = __shfl(incl - own, blk);
+        const unsigned h = lane < per ? hist[blk * per + lane] : 0u;
+        unsigned ih = h;
+        for (int o = 1; o < 64; o <<= 1) {
+            const unsigned v = __shfl_up(ih, o);
+            if (lane >= o) ih += v;
+        }
+        const uint64_t hit2 = __ballot(lane < per && before + ih >= need);
+        const int b = hit2 ? __builtin_ctzll(hit2) : per - 1;
+        if (lane == b) {
+            *s_bin = blk * per + b;
+            *s_below = before + ih - h;
+        }
+    }
+    __syncthreads();
+}
+
+constexpr int QB = 8;   // loads in flight per thread in the select's first pass
+
+// The 4,096-bin histogram over key >> 4 (FP16 keys: 64 bins a binade).
+constexpr int QH_BINS = 4096;
+
+// One workgroup of 256 per query: theta[q] bounds the selk-th smallest key
+// of the query's sample (the first sample_slots of each probed list's run in
+// its row, written by the sample pass) from above: the last key of the
+// 4,096-bin histogram bin (key >> 4) where the sample's count reaches selk.
+// 0xFFFF (every key) when the sample holds fewer than selk keys or the
+// query takes the brute-force fallback. The filter pass recomputes the
+// sample's keys bit for bit, so at least selk entries of the query have
+// keys <= theta[q], and every entry of its selk best (by key, then entry
+// position) does.
+__global__ void __launch_bounds__(256) k_theta(ScanArgs a, int64_t nq, const uint16_t *__restrict__ keys,
+                                               const int64_t *__restrict__ row_off, const int64_t *__restrict__ qtot,
+                                               uint32_t *__restrict__ theta) {
     extern __shared__ __align__(16) unsigned char s_raw[];
-    unsigned *hist = reinterpret_cast<unsigned *>(s_raw);       // [SEL_BINS]
-    unsigned *bkey = hist + SEL_BINS;                           // [SEL_ROOM]
-    int32_t *bpos = reinterpret_cast<int32_t *>(bkey + SEL_ROOM);   // [SEL_ROOM] entry positions (< 2^31)
-    int64_t *poff = reinterpret_cast<int64_t *>(bpos + SEL_ROOM);   // [probe_cap + 1]
-    int64_t *pst = poff + a.probe_cap + 1;                      // [probe_cap]
-    __shared__ int s_out, s_fill, s_B, s_below, s_nb;
-    __shared__ unsigned s_prefix;
+    int64_t *s_poff = reinterpret_cast<int64_t *>(s_raw);               // [npq] row offset of each probe's run
+    int32_t *s_ss = reinterpret_cast<int32_t *>(s_poff + a.probe_cap);   // [npq] sample slots of each probe
+    unsigned *hist = reinterpret_cast<unsigned *>(s_ss + a.probe_cap + (a.probe_cap & 1));   // [QH_BINS]
+    __shared__ int s_bin;
+    __shared__ unsigned s_below, s_cnt;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int64_t q = blockIdx.x;
+    int kq = a.k_q ? a.k_q[q] : a.k_default;
+    kq = kq < 1 ? 1 : (kq > a.k_default ? a.k_default : kq);
+    const unsigned selk = (unsigned)(kq * a.rerank);
+    if (a.allow_fallback && qtot[q] < kq) {
+        if (tid == 0) theta[q] = 0xFFFFu;
+        return;
+    }
+    const int npq = clamp_nprobe(a, q);
+    if (wave == 0) {
+        int64_t cr = 0;
+        for (int l0 = 0; l0 < npq; l0 += 64) {
+            const int li = l0 + lane;
+            int64_t len = 0;
+            if (li < npq) len = a.list_len[a.probes[q * a.probe_stride + li]];
+            const int64_t rs = row_slots(len);
+            int64_t ir = rs;
+            for (int o = 1; o < 64; o <<= 1) {
+                const int64_t vr = __shfl_up(ir, o);
+                if (lane >= o) ir += vr;
+            }
+            if (li < npq) {
+                s_poff[li] = cr + ir - rs;
+                s_ss[li] = (int32_t)sample_slots(len);
+            }
+            cr += __shfl(ir, 63);
+        }
+    }
+    for (int b = tid; b < QH_BINS; b += 256) hist[b] = 0;
+    if (tid == 0) s_cnt = 0;
+    __syncthreads();
+    // each wave takes whole probes, four at a time, lanes over the keys of
+    // a probe's sample (at most 2 x 64 for lists up to 4,096 entries; longer
+    // ones loop)
+    const uint16_t *row = keys + row_off[q];
+    unsigned c = 0;
+    for (int p0 = wave * 4; p0 < npq; p0 += 16) {
+        unsigned k[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u;
+            const bool ok = p < npq;
+            const int ss = ok ? s_ss[p] : 0;
+            const int64_t b = ok ? s_poff[p] : 0;
+            k[2 * u] = lane < ss ? (unsigned)row[b + lane] : 0xFFFFu;
+            k[2 * u + 1] = lane + 64 < ss ? (unsigned)row[b + lane + 64] : 0xFFFFu;
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (k[u] != 0xFFFFu) {
+                atomicAdd(&hist[k[u] >> 4], 1u);
+                ++c;
+            }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int p = p0 + u;
+            if (p >= npq) continue;
+            const int ss = s_ss[p];
+            const int64_t b = s_poff[p];
+            for (int i = 128 + lane; i < ss; i += 64) {
+                const unsigned kk = row[b + i];
+                if (kk != 0xFFFFu) {
+                    atomicAdd(&hist[kk >> 4], 1u);
+                    ++c;
+                }
+            }
+        }
+    }
+    atomicAdd(&s_cnt, c);
+    __syncthreads();
+    if (s_cnt < selk) {
+        if (tid == 0) theta[q] = 0xFFFFu;
+        return;
+    }
+    bins_find(hist, QH_BINS, selk, wave, lane, &s_bin, &s_below);
+    if (tid == 0) theta[q] = ((unsigned)s_bin << 4) | 15u;
+}
+
+constexpr int SELR_ROOM = 1024;    // records of the boundary bin the select keeps in LDS
+constexpr int SELR_RANK = 512;     // boundary bins up to this size are ranked by counting
+constexpr int SELR_CACHE = 2048;   // records a query keeps in LDS (more are read again from the row)
+constexpr int SELR_BINS = 2048;    // histogram over key >> 5
+
+// One workgroup of 256 per query: the sk = min(k_q * R, probed entries)
+// best entries of the query by (key, entry position), from the records the
+// filter pass kept (every entry with key <= theta[q]; the sk best are among
+// them). A 2,048-bin histogram over key >> 5 gives the bin B where the count
+// reaches sk; records below B are selected, and those in B are ranked by
+// (key, entry position) -- by counting for up to SELR_RANK of them, else
+// by a sort, or for more than SELR_ROOM by the exact boundary key and a
+// radix select over positions. Output: entry positions for the rerank.
+// Queries that take the brute-force fallback go to ovf.
+__global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, const uint32_t *__restrict__ rec,
+                                                        const uint16_t *__restrict__ cnt,
+                                                        const int64_t *__restrict__ row_off,
+                                                        const int64_t *__restrict__ qtot,
+                                                        int64_t *__restrict__ out_pos, int32_t *__restrict__ out_cnt,
+                                                        int out_stride, int max_regions, int32_t *__restrict__ ovf,
+                                                        int32_t *__restrict__ ovf_cnt) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    uint64_t *cache = reinterpret_cast<uint64_t *>(s_raw);              // [SELR_CACHE] (key << 32 | entry)
+    uint64_t *tie = cache + SELR_CACHE;                                  // [SELR_ROOM] the boundary bin's records
+    int64_t *r_ent = reinterpret_cast<int64_t *>(tie + SELR_ROOM);      // [R] entry position of the chunk start
+    unsigned *hist = reinterpret_cast<unsigned *>(r_ent + max_regions);  // [SELR_BINS]
+    uint32_t *r_pref = hist + SELR_BINS;                                 // [R + 1] record prefix
+    uint32_t *r_start = r_pref + max_regions + 1;                        // [R] region start in the row
+    __shared__ int s_bin, s_out, s_fill, s_nreg;
+    __shared__ unsigned s_below, s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
     int kq = a.k_q ? a.k_q[q] : a.k_default;
     kq = kq < 1 ? 1 : (kq > a.k_default ? a.k_default : kq);
     const int selk = kq * a.rerank;
-    const int64_t Lr = qtot[q], L = qslots[q];   // entries, row slots (a multiple of 8)
+    const int64_t Lr = qtot[q];
     int64_t *orow = out_pos + q * out_stride;
     if (a.allow_fallback && Lr < kq) {
-        // the reference's brute-force fallback: the table scan redoes it
         if (tid == 0) ovf[atomicAdd(ovf_cnt, 1)] = (int32_t)q;
         return;
     }
@@ -638,265 +891,216 @@ __global__ void __launch_bounds__(256) k_select_rows(ScanArgs a, int64_t nq, con
     if (tid == 0) out_cnt[q] = sk;
     if (sk == 0) return;
     const int npq = clamp_nprobe(a, q);
+    const int64_t rq = row_off[q];
+    // the regions: two halves of every chunk of every probed list, in probe
+    // order (wave 0: a lane per probe, rounds of 64)
     if (wave == 0) {
-        int64_t carry = 0;
+        int64_t cr = 0;
+        int creg = 0;
         for (int l0 = 0; l0 < npq; l0 += 64) {
             const int li = l0 + lane;
             int64_t len = 0, first = 0;
             if (li < npq) {
                 const int64_t l = a.probes[q * a.probe_stride + li];
-                len = row_slots(a.list_len[l]);
+                len = a.list_len[l];
                 first = a.list_start[l];
             }
-            int64_t incl = len;
+            const int64_t rs = row_slots(len);
+            const int nr = len > 0 ? 2 * (int)((len + LCH - 1) / LCH) : 0;
+            int64_t ir = rs;
+            int inr = nr;
             for (int o = 1; o < 64; o <<= 1) {
-                const int64_t v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            if (li < npq) {
-                poff[li] = carry + incl - len;
-                pst[li] = first;
-            }
-            carry += __shfl(incl, 63);
-        }
-        if (lane == 0) poff[npq] = carry;
-    }
-    if (tid == 0) s_out = 0;
-    __syncthreads();
-    const uint16_t *row = dist + row_off[q];
-    // slot i of the row -> entry position (the last probe whose row offset is <= i)
-    auto pos_of = [&](int64_t i) -> int64_t {
-        int lo_j = 0, hi_j = npq - 1;
-        while (lo_j < hi_j) {
-            const int mid = (lo_j + hi_j + 1) >> 1;
-            if (poff[mid] <= i)
-                lo_j = mid;
-            else
-                hi_j = mid - 1;
-        }
-        return pst[lo_j] + (i - poff[lo_j]);
-    };
-    // fn(slot, key) over the row (16-byte aligned, a multiple of 8 slots):
-    // four 16-byte loads in flight per thread
-    constexpr int SU = 4;
-    const uint4 *row8 = reinterpret_cast<const uint4 *>(row);
-    const int64_t L8 = L / 8;
-    auto for_row = [&](auto &&fn) {
-        for (int64_t j0 = 0; j0 < L8; j0 += 256 * SU) {
-            uint4 v[SU];
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t j = j0 + u * 256 + tid;
-                v[u] = row8[j < L8 ? j : L8 - 1];
-            }
-#pragma unroll
-            for (int u = 0; u < SU; ++u) {
-                const int64_t j = j0 + u * 256 + tid;
-                if (j < L8) {
-                    const unsigned w[4] = {v[u].x, v[u].y, v[u].z, v[u].w};
-#pragma unroll
-                    for (int h = 0; h < 4; ++h) {
-                        fn(8 * j + 2 * h, w[h] & 0xFFFFu);
-                        fn(8 * j + 2 * h + 1, w[h] >> 16);
-                    }
+                const int64_t vr = __shfl_up(ir, o);
+                const int vn = __shfl_up(inr, o);
+                if (lane >= o) {
+                    ir += vr;
+                    inr += vn;
                 }
             }
+            const int64_t poff = cr + ir - rs;
+            int r = creg + inr - nr;
+            for (int c = 0; 2 * c < nr; ++c, r += 2) {
+                const int64_t cs = poff + (int64_t)c * LCH;
+                const int64_t slots = (2 * c + 2 == nr) ? rs - (int64_t)c * LCH : LCH;
+                const uint32_t c0 = cnt[(uint64_t)(rq + cs) / 4], c1 = cnt[(uint64_t)(rq + cs) / 4 + 1];
+                if (r + 1 < max_regions) {
+                    r_ent[r] = r_ent[r + 1] = first + (int64_t)c * LCH;
+                    r_start[r] = (uint32_t)cs;
+                    r_start[r + 1] = (uint32_t)(cs + (slots >> 1));
+                    r_pref[r] = c0;
+                    r_pref[r + 1] = c1;
+                }
+            }
+            cr += __shfl(ir, 63);
+            creg += __shfl(inr, 63);
         }
+        if (lane == 0) s_nreg = creg < max_regions ? creg : max_regions;
+    }
+    for (int b = tid; b < SELR_BINS; b += 256) hist[b] = 0;
+    __syncthreads();
+    const int R = s_nreg;
+    // exclusive prefix of the region counts (wave 0: consecutive runs of
+    // regions a lane, then a scan over the lanes)
+    if (wave == 0) {
+        const int per = (R + 63) / 64, r0 = lane * per, r1 = r0 + per < R ? r0 + per : R;
+        uint32_t own = 0;
+        for (int r = r0; r < r1; ++r) own += r_pref[r];
+        uint32_t incl = own;
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        uint32_t run = incl - own;
+        for (int r = r0; r < r1; ++r) {
+            const uint32_t c = r_pref[r];
+            r_pref[r] = run;
+            run += c;
+        }
+        if (lane == 63) r_pref[R] = incl;
+        if (lane == 0) {
+            s_out = 0;
+            s_fill = 0;
+        }
+    }
+    __syncthreads();
+    const uint32_t T = r_pref[R];
+    const uint32_t *rrow = rec + rq;
+    // record t as (key << 32 | entry position): its region by binary search
+    auto rec_at = [&](uint32_t t) -> uint64_t {
+        int lo = 0, hi = R - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (r_pref[mid] <= t)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        const uint32_t v = rrow[r_start[lo] + (t - r_pref[lo])];
+        return ((uint64_t)(v >> 16) << 32) | (uint64_t)(r_ent[lo] + (v & 0xFFFFu));
     };
-    unsigned lo = qrange[q], hi = qrange[nq + q];
-    // sort the nb (key, position) pairs in LDS and emit the first need
-    // after the s_out already emitted
-    auto sort_emit = [&](int nb, int need) {
-        __syncthreads();
+    // pass 0: the records (eight loads in flight a thread) into the cache
+    // and the histogram
+    for (uint32_t t0 = tid; t0 < T; t0 += 256 * QB) {
+        uint64_t v[QB];
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+            const uint32_t t = t0 + 256 * u;
+            v[u] = t < T ? rec_at(t) : 0ull;
+        }
+#pragma unroll
+        for (int u = 0; u < QB; ++u) {
+            const uint32_t t = t0 + 256 * u;
+            if (t >= T) continue;
+            if (t < (uint32_t)SELR_CACHE) cache[t] = v[u];
+            atomicAdd(&hist[(unsigned)(v[u] >> 37)], 1u);
+        }
+    }
+    __syncthreads();
+    auto for_rec = [&](auto &&fn) {
+        for (uint32_t t = tid; t < T; t += 256) fn(t < (uint32_t)SELR_CACHE ? cache[t] : rec_at(t));
+    };
+    if (T <= (uint32_t)sk) {
+        // every record is selected (T < sk cannot happen: at least
+        // min(probed entries, selk) keys are <= theta)
+        for_rec([&](uint64_t v) { orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v; });
+        if (tid == 0) out_cnt[q] = (int32_t)T;
+        return;
+    }
+    bins_find(hist, SELR_BINS, (unsigned)sk, wave, lane, &s_bin, &s_below);
+    const unsigned B = (unsigned)s_bin;
+    const int need = sk - (int)s_below;   // records of bin B to take, 1 .. hist[B]
+    const int nb = (int)hist[B];
+    const bool room = nb <= SELR_ROOM;
+    // pass 1: records below bin B are selected; bin B's go to LDS
+    for_rec([&](uint64_t v) {
+        const unsigned b = (unsigned)(v >> 37);
+        if (b < B)
+            orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
+        else if (b == B && room)
+            tie[atomicAdd(&s_fill, 1)] = v;
+    });
+    __syncthreads();
+    if (room && nb <= SELR_RANK) {
+        // a record of bin B is taken when fewer than need of the others
+        // come before it by (key, entry position)
+        const int o0 = s_out;
+        for (int i = tid; i < nb; i += 256) {
+            const uint64_t vi = tie[i];
+            int rank = 0;
+            for (int j2 = 0; j2 < nb; ++j2) rank += tie[j2] < vi ? 1 : 0;
+            if (rank < need) orow[o0 + rank] = (int64_t)(uint32_t)vi;
+        }
+        return;
+    }
+    if (room) {
+        // sort bin B by (key, entry position); the first need complete the selection
         int span = 1;
         while (span < nb) span <<= 1;
-        for (int t = nb + tid; t < span; t += 256) {
-            bkey[t] = 0xFFFFFFFFu;
-            bpos[t] = INT32_MAX;
-        }
+        for (int t = nb + tid; t < span; t += 256) tie[t] = ~0ull;
         __syncthreads();
         for (int size = 2; size <= span; size <<= 1) {
             for (int stride = size >> 1; stride > 0; stride >>= 1) {
                 for (int t = tid; t < (span >> 1); t += 256) {
-                    const int i0 = 2 * t - (t & (stride - 1));
-                    const int i1 = i0 + stride;
+                    const int i0 = 2 * t - (t & (stride - 1)), i1 = i0 + stride;
                     const bool up = (i0 & size) == 0;
-                    const unsigned ka = bkey[i0], kb = bkey[i1];
-                    const int32_t pa = bpos[i0], pb = bpos[i1];
-                    const bool b_less = kb < ka || (kb == ka && pb < pa);
-                    const bool a_less = ka < kb || (ka == kb && pa < pb);
-                    if (up ? b_less : a_less) {
-                        bkey[i0] = kb;
-                        bkey[i1] = ka;
-                        bpos[i0] = pb;
-                        bpos[i1] = pa;
+                    const uint64_t va = tie[i0], vb = tie[i1];
+                    if (up ? vb < va : va < vb) {
+                        tie[i0] = vb;
+                        tie[i1] = va;
                     }
                 }
                 __syncthreads();
             }
         }
         const int o0 = s_out;
-        for (int t = tid; t < need; t += 256) orow[o0 + t] = bpos[t];
-    };
-    // the bin of a SEL_BINS histogram over [lo, hi] (shift sh) where the
-    // cumulative count reaches need: s_B, with s_below entries before it
-    // and s_nb in it (wave 0: 32 bins a lane)
-    auto find_bin = [&](unsigned need) {
-        if (wave == 0) {
-            unsigned own = 0;
-            for (int b = 0; b < SEL_BINS / 64; ++b) own += hist[lane * (SEL_BINS / 64) + b];
-            unsigned incl = own;
-            for (int o = 1; o < 64; o <<= 1) {
-                const unsigned v = __shfl_up(incl, o);
-                if (lane >= o) incl += v;
-            }
-            const unsigned excl = incl - own;
-            if (excl < need && need <= incl) {
-                unsigned run = excl;
-                int b = lane * (SEL_BINS / 64);
-                while (run + hist[b] < need) run += hist[b++];
-                s_B = b;
-                s_below = (int)run;
-                s_nb = (int)hist[b];
-            }
-        }
-        __syncthreads();
-    };
-    // Sampled threshold: a histogram of every SEL_SAMP-th 128-byte block of
-    // the row puts an estimate T above the sk-th smallest key (with margin);
-    // one pass then gathers every key <= T into LDS. If at least sk keys
-    // (and at most SEL_ROOM) are <= T they hold the sk best by (key,
-    // position), which the sort picks; otherwise the exact method below
-    // runs. Either way the selection is the same.
-    if (2 * sk <= SEL_ROOM && L8 >= 4 * SEL_SAMP && lo <= hi) {
-        const unsigned width = hi - lo;
-        int sh = 0;
-        while ((width >> sh) >= (unsigned)SEL_BINS) ++sh;
-        for (int b = tid; b < SEL_BINS; b += 256) hist[b] = 0;
-        if (tid == 0) s_B = SEL_BINS - 1;
-        __syncthreads();
-        // whole 128-byte blocks (8 groups), every SEL_SAMP-th block
-        for (int64_t j = (int64_t)(tid >> 3) * 8 * SEL_SAMP + (tid & 7); j < L8; j += 32 * 8 * SEL_SAMP) {
-            const uint4 v = row8[j];
-            const unsigned w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int h = 0; h < 4; ++h) {
-                const unsigned u0 = w[h] & 0xFFFFu, u1 = w[h] >> 16;
-                if (u0 >= lo && u0 <= hi) atomicAdd(&hist[(u0 - lo) >> sh], 1u);
-                if (u1 >= lo && u1 <= hi) atomicAdd(&hist[(u1 - lo) >> sh], 1u);
-            }
-        }
-        __syncthreads();
-        // the sample holds about tot / Lr of the row's entries: aim at 1.5
-        // times the sk-th's expected rank in it, plus 16
-        unsigned tot = 0;
-        for (int b = lane; b < SEL_BINS; b += 64) tot += hist[b];
-        for (int o = 32; o > 0; o >>= 1) tot += __shfl_xor(tot, o);
-        const uint64_t want = ((uint64_t)3 * sk * tot + 2 * (uint64_t)Lr - 1) / (2 * (uint64_t)Lr) + 16;
-        find_bin((unsigned)(want < tot ? want : tot));
-        if (tot > 0) {
-            const unsigned bspan = (sh >= 32) ? 0xFFFFFFFFu : ((1u << sh) - 1u);
-            const unsigned tlo = lo + ((unsigned)s_B << sh);
-            const unsigned thr = (hi - tlo) < bspan ? hi : tlo + bspan;
-            if (tid == 0) s_fill = 0;
-            __syncthreads();
-            for_row([&](int64_t i, unsigned u) {
-                if (u > thr) return;
-                const int slot = atomicAdd(&s_fill, 1);
-                if (slot < SEL_ROOM) {
-                    bkey[slot] = u;
-                    bpos[slot] = (int32_t)pos_of(i);
-                }
-            });
-            __syncthreads();
-            const int nb = s_fill;
-            if (nb >= sk && nb <= SEL_ROOM) {
-                sort_emit(nb, sk);
-                return;
-            }
-        }
-        __syncthreads();
+        for (int t = tid; t < need; t += 256) orow[o0 + t] = (int64_t)(uint32_t)tie[t];
+        return;
     }
-    int need = sk;
-    for (;;) {
-        const unsigned width = hi - lo;
-        int sh = 0;
-        while ((width >> sh) >= (unsigned)SEL_BINS) ++sh;
-        for (int b = tid; b < SEL_BINS; b += 256) hist[b] = 0;
+    // more than SELR_ROOM records in bin B: its exact boundary key K (a
+    // histogram of its 32 key values), the keys below K in it, then a radix
+    // select over the entry positions of the records with key K
+    __syncthreads();
+    if (tid < 64) hist[tid] = 0;
+    __syncthreads();
+    for_rec([&](uint64_t v) {
+        if ((unsigned)(v >> 37) == B) atomicAdd(&hist[(unsigned)(v >> 32) & 31u], 1u);
+    });
+    __syncthreads();
+    bins_find(hist, 64, (unsigned)need, wave, lane, &s_bin, &s_below);
+    const unsigned K = (B << 5) | (unsigned)s_bin;
+    int kk = need - (int)s_below;
+    for_rec([&](uint64_t v) {
+        const unsigned k = (unsigned)(v >> 32);
+        if ((k >> 5) == B && k < K) orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
+    });
+    unsigned prefix = 0, mask = 0;
+    for (int pass = 0; pass < 4; ++pass) {
+        const int shift = 24 - 8 * pass;
         __syncthreads();
-        for_row([&](int64_t, unsigned u) {
-            if (u >= lo && u <= hi) atomicAdd(&hist[(u - lo) >> sh], 1u);
+        for (int b = tid; b < 256; b += 256) hist[b] = 0;
+        __syncthreads();
+        for_rec([&](uint64_t v) {
+            if ((unsigned)(v >> 32) != K) return;
+            const unsigned p = (unsigned)v;
+            if ((p & mask) == prefix) atomicAdd(&hist[(p >> shift) & 255u], 1u);
         });
         __syncthreads();
-        find_bin((unsigned)need);
-        const int B = s_B, below = s_below, nb = s_nb;
-        const bool fits = nb <= SEL_ROOM;
-        if (tid == 0) s_fill = 0;
+        if (tid == 0) {
+            unsigned run = 0;
+            int b = 0;
+            while (run + hist[b] < (unsigned)kk) run += hist[b++];
+            s_prefix = prefix | ((unsigned)b << shift);
+            s_below = run;
+        }
         __syncthreads();
-        // one pass: entries below bin B are selected; bin B goes to LDS when
-        // it fits there
-        if (below > 0 || fits) {
-            for_row([&](int64_t i, unsigned u) {
-                if (u < lo || u > hi) return;
-                const int b = (int)((u - lo) >> sh);
-                if (b < B) {
-                    orow[atomicAdd(&s_out, 1)] = pos_of(i);
-                } else if (fits && b == B) {
-                    const int slot = atomicAdd(&s_fill, 1);
-                    bkey[slot] = u;
-                    bpos[slot] = (int32_t)pos_of(i);
-                }
-            });
-        }
-        need -= below;
-        const unsigned blo = lo + ((unsigned)B << sh);
-        const unsigned bspan = (sh >= 32) ? 0xFFFFFFFFu : ((1u << sh) - 1u);
-        const unsigned bhi = (hi - blo) < bspan ? hi : blo + bspan;
-        if (fits) {
-            // sort bin B by (key bits, position); the first need complete the row
-            sort_emit(nb, need);
-            return;
-        }
-        if (blo == bhi) {
-            // more than SEL_ROOM entries share one key: the need smallest
-            // positions among them (positions < 2^31: 8 bits a pass)
-            unsigned prefix = 0, mask = 0;
-            int kk = need;
-            for (int pass = 0; pass < 4; ++pass) {
-                const int shift = 24 - 8 * pass;
-                for (int b = tid; b < 256; b += 256) hist[b] = 0;
-                __syncthreads();
-                for_row([&](int64_t i, unsigned u) {
-                    if (u != blo) return;
-                    const unsigned p = (unsigned)pos_of(i);
-                    if ((p & mask) == prefix) atomicAdd(&hist[(p >> shift) & 255u], 1u);
-                });
-                __syncthreads();
-                if (tid == 0) {
-                    unsigned run = 0;
-                    int b = 0;
-                    while (run + hist[b] < (unsigned)kk) run += hist[b++];
-                    s_prefix = prefix | ((unsigned)b << shift);
-                    s_below = (int)run;
-                }
-                __syncthreads();
-                prefix = s_prefix;
-                kk -= s_below;
-                mask |= 255u << shift;
-                __syncthreads();
-            }
-            // positions are distinct: exactly need of them are <= prefix
-            for_row([&](int64_t i, unsigned u) {
-                if (u != blo) return;
-                const int64_t p = pos_of(i);
-                if ((unsigned)p <= prefix) orow[atomicAdd(&s_out, 1)] = p;
-            });
-            return;
-        }
-        lo = blo;
-        hi = bhi;
-        __syncthreads();
+        prefix = s_prefix;
+        kk -= (int)s_below;
+        mask |= 255u << shift;
     }
+    // entry positions are distinct: exactly the remaining ones are <= prefix
+    for_rec([&](uint64_t v) {
+        if ((unsigned)(v >> 32) == K && (unsigned)v <= prefix) orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
+    });
 }
 
 }  // namespace dev
@@ -953,37 +1157,34 @@ int list_scan_chunk() { return dev::LCH; }
 bool list_scan_supported(int d, int m, int P, int code_bytes) {
     if (code_bytes != 1 || d != 4 * m) return false;
     if (m != 8 && m != 16 && m != 24 && m != 32 && m != 48 && m != 64) return false;
-    return list_scan_lds_bytes(1, d, m, P) <= 160 * 1024;
+    return list_scan_lds_bytes_kind(0, m, P, dev::KIND_FILTER) <= 160 * 1024;
 }
 
-size_t list_scan_lds_bytes(int x3, int d, int m, int P) {
-    (void)d;
-    const int nb = x3 ? 2 : 1;
-    return (size_t)nb * m * P * 8 + (size_t)dev::LNW * 32 * 4;
-}
-
-int list_scan_threads() { return 64 * dev::LNW; }
-
-void launch_scan_list(const ListScanArgs &la, int x3, int grid, hipStream_t st) {
-    const size_t shm = list_scan_lds_bytes(x3, la.d, la.m, la.P);
+void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st) {
+    const size_t shm = list_scan_lds_bytes_kind(x3, la.m, la.P, kind);
     if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("list-major scan needs %zu bytes of LDS", shm));
-#define HANN_LIST(X, MM)                                                                                 \
-    do {                                                                                                 \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_list<X, MM>,                              \
-                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));             \
-        hipLaunchKernelGGL((dev::k_scan_list<X, MM>), dim3((unsigned)grid), dim3(64 * dev::LNW), shm, st, \
-                           la);                                                                          \
-        HANN_LAUNCH_CHECK();                                                                             \
+#define HANN_LIST(X, MM, KK)                                                                                 \
+    do {                                                                                                     \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_list<X, MM, KK>,                              \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                 \
+        hipLaunchKernelGGL((dev::k_scan_list<X, MM, KK>), dim3((unsigned)grid), dim3(64 * dev::LNW), shm, st, \
+                           la);                                                                              \
+        HANN_LAUNCH_CHECK();                                                                                 \
+    } while (0)
+#define HANN_LIST_K(X, MM)                                                             \
+    do {                                                                               \
+        if (kind == dev::KIND_SAMPLE) HANN_LIST(X, MM, dev::KIND_SAMPLE);              \
+        else HANN_LIST(X, MM, dev::KIND_FILTER);                                       \
     } while (0)
 #define HANN_LIST_M(X)                                                            \
     do {                                                                          \
         switch (la.m) {                                                           \
-        case 8: HANN_LIST(X, 8); break;                                           \
-        case 16: HANN_LIST(X, 16); break;                                         \
-        case 24: HANN_LIST(X, 24); break;                                         \
-        case 32: HANN_LIST(X, 32); break;                                         \
-        case 48: HANN_LIST(X, 48); break;                                         \
-        case 64: HANN_LIST(X, 64); break;                                         \
+        case 8: HANN_LIST_K(X, 8); break;                                         \
+        case 16: HANN_LIST_K(X, 16); break;                                       \
+        case 24: HANN_LIST_K(X, 24); break;                                       \
+        case 32: HANN_LIST_K(X, 32); break;                                       \
+        case 48: HANN_LIST_K(X, 48); break;                                       \
+        case 64: HANN_LIST_K(X, 64); break;                                       \
         default: fail(HANN_ERR_UNSUPPORTED, "list-major scan: unsupported m");    \
         }                                                                         \
     } while (0)
@@ -993,22 +1194,44 @@ void launch_scan_list(const ListScanArgs &la, int x3, int grid, hipStream_t st) 
     else
         HANN_LIST_M(false);
 #undef HANN_LIST_M
+#undef HANN_LIST_K
 #undef HANN_LIST
 }
 
-int select_rows_max_probes() { return dev::SEL_MAXP; }
+size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind) {
+    const int nb = x3 ? 2 : 1;
+    size_t b = (size_t)nb * m * P * 8 + (size_t)dev::LNW * 32 * 4;
+    if (kind == dev::KIND_FILTER) b += (size_t)dev::LNW * dev::RING * 8;
+    return b;
+}
 
-void launch_select_rows(const ScanArgs &a, int64_t nq, const uint16_t *dist, const int64_t *row_off,
-                        const int64_t *qtot, const int64_t *qslots, const unsigned *qrange, int64_t *out_pos,
-                        int32_t *out_cnt, int out_stride, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st) {
+void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
+                  uint32_t *theta, hipStream_t st) {
     if (nq == 0) return;
-    if (a.probe_cap > dev::SEL_MAXP) fail(HANN_ERR_UNSUPPORTED, "list-major select: too many probes per query");
-    const size_t shm = (size_t)dev::SEL_BINS * 4 + (size_t)dev::SEL_ROOM * 8 + (size_t)(2 * a.probe_cap + 1) * 8;
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_select_rows, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
-    hipLaunchKernelGGL(dev::k_select_rows, dim3((unsigned)nq), dim3(256), shm, st, a, nq, dist, row_off, qtot, qslots,
-                       qrange, out_pos, out_cnt, out_stride, ovf, ovf_cnt);
+    const size_t shm = (size_t)a.probe_cap * 8 + (size_t)(a.probe_cap + 1) * 4 + 4 + (size_t)dev::QH_BINS * 4;
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_theta, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    hipLaunchKernelGGL(dev::k_theta, dim3((unsigned)nq), dim3(256), shm, st, a, nq, keys, row_off, qtot, theta);
     HANN_LAUNCH_CHECK();
 }
+
+size_t select_records_lds_bytes(int max_regions) {
+    return (size_t)(dev::SELR_CACHE + dev::SELR_ROOM) * 8 + (size_t)max_regions * 8 + (size_t)dev::SELR_BINS * 4 +
+           (size_t)(2 * max_regions + 1) * 4;
+}
+
+void launch_select_records(const ScanArgs &a, int64_t nq, const uint32_t *rec, const uint16_t *cnt,
+                           const int64_t *row_off, const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt,
+                           int out_stride, int max_regions, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st) {
+    if (nq == 0) return;
+    const size_t shm = select_records_lds_bytes(max_regions);
+    if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major select: too many probed list chunks per query");
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_select_records, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 (int)shm));
+    hipLaunchKernelGGL(dev::k_select_records, dim3((unsigned)nq), dim3(256), shm, st, a, nq, rec, cnt, row_off, qtot,
+                       out_pos, out_cnt, out_stride, max_regions, ovf, ovf_cnt);
+    HANN_LAUNCH_CHECK();
+}
+
+int select_rows_max_probes() { return dev::SEL_MAXP; }
 
 }  // namespace hann
