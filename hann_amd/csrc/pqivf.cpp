@@ -1187,11 +1187,14 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     }
     const int64_t items_max = ceil_div(pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
     const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
-                 o_lfill = carve((size_t)K * 4), o_tot = carve(16), o_qtot = carve((size_t)nq * 8),
+                 o_lfill = carve((size_t)K * 4), o_lscnt = carve((size_t)K * 4), o_lbfill = carve((size_t)K * 4),
+                 o_qsnp = carve((size_t)nq * 4), o_qsamp = carve((size_t)nq * 8), o_tot = carve(16),
+                 o_qtot = carve((size_t)nq * 8),
                  o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_theta = carve((size_t)nq * 8),
                  o_ovf = carve((size_t)nq * 4), o_ctr = carve(16), o_parts = carve(kListScanParts * 4), o_pq = carve((size_t)pairs_max * 4),
                  o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
                  o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
+                 o_items_s = carve((size_t)items_max * 16),
                  o_pkey = carve((size_t)nq * selk * 4), o_iord = carve((size_t)list_item_order_ints() * 4);
     char *base = w.get<char>(17, off);
     auto at = [&](size_t o) { return base + o; };
@@ -1199,6 +1202,10 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int32_t *lstart = reinterpret_cast<int32_t *>(at(o_lstart));
     int32_t *ioff = reinterpret_cast<int32_t *>(at(o_ioff));
     int32_t *lfill = reinterpret_cast<int32_t *>(at(o_lfill));
+    int32_t *lscnt = reinterpret_cast<int32_t *>(at(o_lscnt));   // sample pairs per list
+    int32_t *lbfill = reinterpret_cast<int32_t *>(at(o_lbfill));
+    int32_t *qsamp_np = reinterpret_cast<int32_t *>(at(o_qsnp));   // probes in a query's sample
+    int64_t *qsamp = reinterpret_cast<int64_t *>(at(o_qsamp));     // row slots of a query's sample
     int32_t *totals = reinterpret_cast<int32_t *>(at(o_tot));
     int64_t *qtot = reinterpret_cast<int64_t *>(at(o_qtot));
     int64_t *qslots = reinterpret_cast<int64_t *>(at(o_qslots));
@@ -1211,6 +1218,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     float *pair_scale = reinterpret_cast<float *>(at(o_psc));
     int64_t *pair_off = reinterpret_cast<int64_t *>(at(o_poff));
     int4 *items = reinterpret_cast<int4 *>(at(o_items));
+    int4 *items_s = reinterpret_cast<int4 *>(at(o_items_s));   // the sample pass's items
     float *pkey = reinterpret_cast<float *>(at(o_pkey));
     if (num_cus == 0) {
         int dev = 0;
@@ -1225,15 +1233,17 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     uint16_t *rcnt = w.get<uint16_t>(19, (size_t)rows_max / 4 + 8);
     uint32_t *theta = reinterpret_cast<uint32_t *>(at(o_theta));   // [nq]
 
-    // 1. the batch's (query, list) pairs grouped by list, as work items, and
-    //    each query's row
+    // 1. the batch's (query, list) pairs grouped by list, as work items (and
+    //    the sample pass's over the sample pairs), and each query's row
     launch_fill_i32(lcnt, K, 0, st);
-    launch_pair_count(a, nq, lcnt, qtot, qslots, st);
+    launch_fill_i32(lscnt, K, 0, st);
+    launch_pair_count(a, nq, lcnt, lscnt, qtot, qslots, qsamp_np, qsamp, st);
     launch_row_offsets(qslots, nq, roff, st);
-    launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, totals, st);
-    launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, roff, lfill, pair_q, pair_qc, pair_scale,
-                        pair_off, st);
-    launch_make_items(lcnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(o_iord)), items, st);
+    launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, lbfill, totals, st);
+    launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, lcnt, qsamp_np, roff, lfill, lbfill, pair_q,
+                        pair_qc, pair_scale, pair_off, st);
+    launch_make_items(lcnt, lscnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(o_iord)), items, items_s,
+                      totals + 2, st);
     ListScanArgs la{};
     la.queries = a.queries;
     la.d = (int)dim;
@@ -1261,12 +1271,17 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.rec = rec;
     la.cnt = rcnt;
     la.cnt_bytes = (uint32_t)((rows_max / 4 + 8) * 2);
-    // 2. the sample pass: keys of each list's first slabs; then each
-    //    query's bound theta, the (k R)-th smallest of its sample's keys
+    // 2. the sample pass: every key of each query's nearest probed lists;
+    //    then each query's bound theta, the (k R)-th smallest of them
     launch_fill_i32(ctr, 4, 0, st);
     launch_fill_i32(parts, kListScanParts, 0, st);
-    launch_scan_list_kind(la, x3, 1, grid, st);
-    launch_theta(a, nq, reinterpret_cast<const uint16_t *>(rec), roff, qtot, theta, st);
+    {
+        ListScanArgs ls = la;
+        ls.items = items_s;
+        ls.n_items = totals + 2;
+        launch_scan_list_kind(ls, x3, 1, grid, st);
+    }
+    launch_theta(a, nq, reinterpret_cast<const uint16_t *>(rec), roff, qtot, qsamp, theta, st);
     // 3. the filter pass: every (query, entry) key, and the records of those
     //    <= theta compacted per (chunk, half) region
     launch_fill_i32(parts, kListScanParts, 0, st);

@@ -40,37 +40,44 @@ int list_scan_chunk();
 // work-item partitions (counters) of the list-major scan
 constexpr int kListScanParts = 256;
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
-// per-list pair counts, per-query entry totals and row slots (each probed
-// list's length rounded up to 8; lcnt zeroed by the caller)
-void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int64_t *qtot, int64_t *qslots,
-                       hipStream_t st);
+// per-list pair counts (lcnt) and sample pair counts (lscnt), per-query
+// entry totals and row slots (each probed list's length rounded up to 8),
+// and the sample's probes and row slots (its nearest probed lists: at least
+// two, and until they hold 4 k R entries); lcnt, lscnt zeroed by the caller
+void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot, int64_t *qslots,
+                       int32_t *qsamp_np, int64_t *qsamp_slots, hipStream_t st);
 // row_off[q] = qslots[0] + ... + qslots[q - 1]
 void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hipStream_t st);
 // exclusive offsets of pairs and of work items per list (ceil(pairs / 32)
 // query groups times ceil(len / chunk) entry chunks); totals[0] = pairs,
-// totals[1] = items; lfill is zeroed
+// totals[1] = items, totals[2] = 0 (then the sample pass's items); lfill and
+// lbfill are zeroed
 void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
-                         int32_t *lfill, int32_t *totals, hipStream_t st);
-// the pairs grouped by list, with the coarse rank, the query's key scale and
-// the row offset of each; ymax: bits of an upper bound of |y_e| over the index
+                         int32_t *lfill, int32_t *lbfill, int32_t *totals, hipStream_t st);
+// the pairs grouped by list (each list's sample pairs first), with the
+// coarse rank, the query's key scale and the row offset of each; ymax: bits
+// of an upper bound of |y_e| over the index
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
-                         const int32_t *lstart, const int64_t *row_off, int32_t *lfill, int32_t *pair_q,
-                         float *pair_qc, float *pair_scale, int64_t *pair_off, hipStream_t st);
-// the batch's work items, longest first; order: list_item_order_ints()
-// ints of scratch (per-bucket counts and fill counters)
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int64_t *list_len, int K, int32_t *order,
-                       int4 *items, hipStream_t st);
+                         const int32_t *lstart, const int32_t *lcnt, const int32_t *qsamp_np, const int64_t *row_off,
+                         int32_t *lfill, int32_t *lbfill, int32_t *pair_q, float *pair_qc, float *pair_scale,
+                         int64_t *pair_off, hipStream_t st);
+// the batch's work items and the sample pass's (over each list's sample
+// pairs; their count is added to *n_samp), longest first; order:
+// list_item_order_ints() ints of scratch (per-bucket counts and fill counters)
+void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t *lstart, const int64_t *list_len,
+                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st);
 int list_item_order_ints();
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);
-// kind 1: the sample pass's keys (the first slabs of each list); 2: the
-// filter pass's records
+// kind 1: the sample pass's keys (every key of the sample pairs, in the
+// rows); 2: the filter pass's records
 void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st);
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
-// per query, the bound theta of the filter pass from the sample's keys
+// per query, the bound theta of the filter pass: the (k R)-th smallest key of
+// its sample (the first qsamp_slots of its row)
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
-                  uint32_t *theta, hipStream_t st);
+                  const int64_t *qsamp_slots, uint32_t *theta, hipStream_t st);
 // per query, the selk = k_q * R best entries by (key, entry position) from the
 // filter pass's records, as entry positions (-1 after the last); the
 // brute-force fallbacks go to ovf. max_regions: at least twice the probed

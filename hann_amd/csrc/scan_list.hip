@@ -21,10 +21,11 @@
 // bits mapped so that unsigned order is value order. A query's rerank
 // candidates are its selk = k * R best probed entries by (key, entry
 // position). They are found without writing every key:
-//  1. the sample pass scores the first slabs of every list (about an
-//     eighth of the entries) and writes their keys into the query's row;
-//  2. k_theta takes, per query, a bound theta >= the selk-th smallest sample
-//     key -- at least selk entries have keys <= theta, and the selk best do;
+//  1. the sample pass scores each query's nearest probed lists (at least
+//     two, and until they hold 4 selk entries) and writes their keys into
+//     the query's row, whose prefix they are;
+//  2. k_theta takes, per query, theta = the selk-th smallest sample key --
+//     at least selk entries have keys <= theta, and the selk best do;
 //  3. the filter pass scores every entry again, with bit-identical keys, and
 //     writes only the (key, entry) records with key <= theta, each lane into
 //     its own (query, chunk, half) region of the row, four at a time;
@@ -93,27 +94,67 @@ __device__ __forceinline__ uint32_t min_u16x2(uint32_t a, uint32_t b) {
 // a pair's slots in its query's row: its list's length rounded up to 8
 __device__ __forceinline__ int64_t row_slots(int64_t len) { return (len + 7) & ~(int64_t)7; }
 
-// one wave per query: entries and row slots of its probed lists
+// The sample of a query: every entry of its nearest probed lists -- the
+// probes of rank < SAMP_MIN_PROBES, and on until the lists before the next
+// probe hold SAMP_MULT * k * R entries. The nearest lists hold most of a
+// query's best entries, so the (k R)-th smallest key of such a sample lies
+// close to that of all its probed entries.
+constexpr int SAMP_MIN_PROBES = 2;
+constexpr int SAMP_MULT = 4;
+
+// a query's rerank depth k * R (k clamped to [1, the batch's k])
+__device__ __forceinline__ unsigned query_selk(const ScanArgs &a, int64_t q) {
+    int kq = a.k_q ? a.k_q[q] : a.k_default;
+    kq = kq < 1 ? 1 : (kq > a.k_default ? a.k_default : kq);
+    return (unsigned)(kq * a.rerank);
+}
+
+// one wave per query: entries and row slots of its probed lists, the pairs
+// of every list and of its sample, and the sample's probes (a prefix of the
+// probe order, so its keys are a prefix of the query's row) and row slots
 __global__ void __launch_bounds__(64) k_pair_count(ScanArgs a, int32_t *__restrict__ lcnt,
-                                                   int64_t *__restrict__ qtot, int64_t *__restrict__ qslots) {
+                                                   int32_t *__restrict__ lscnt, int64_t *__restrict__ qtot,
+                                                   int64_t *__restrict__ qslots, int32_t *__restrict__ qsamp_np,
+                                                   int64_t *__restrict__ qsamp_slots) {
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     const int npq = clamp_nprobe(a, q);
-    int64_t tot = 0, slots = 0;
-    for (int li = lane; li < npq; li += 64) {
-        const int64_t l = a.probes[q * a.probe_stride + li];
-        const int64_t len = a.list_len[l];
-        tot += len;
-        slots += row_slots(len);
-        if (len > 0) atomicAdd(&lcnt[l], 1);
-    }
-    for (int o = 32; o > 0; o >>= 1) {
-        tot += __shfl_xor(tot, o);
-        slots += __shfl_xor(slots, o);
+    const int64_t target = (int64_t)SAMP_MULT * query_selk(a, q);
+    int64_t tot = 0, slots = 0, ssl = 0;
+    int snp = 0;
+    for (int l0 = 0; l0 < npq; l0 += 64) {
+        const int li = l0 + lane;
+        int64_t l = 0, len = 0;
+        if (li < npq) {
+            l = a.probes[q * a.probe_stride + li];
+            len = a.list_len[l];
+        }
+        int64_t incl = len;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int64_t v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        // sampled: a low rank, or fewer than target entries before it
+        const bool smp = li < npq && (li < SAMP_MIN_PROBES || tot + incl - len < target);
+        if (li < npq && len > 0) {
+            atomicAdd(&lcnt[l], 1);
+            if (smp) atomicAdd(&lscnt[l], 1);
+        }
+        int64_t rs = row_slots(len), ss = smp ? rs : 0;
+        for (int o = 32; o > 0; o >>= 1) {
+            rs += __shfl_xor(rs, o);
+            ss += __shfl_xor(ss, o);
+        }
+        snp += __popcll(__ballot(smp));
+        tot += __shfl(incl, 63);
+        slots += rs;
+        ssl += ss;
     }
     if (lane == 0) {
         qtot[q] = tot;
         qslots[q] = slots;
+        qsamp_np[q] = snp;
+        qsamp_slots[q] = ssl;
     }
 }
 
@@ -155,7 +196,8 @@ constexpr int PO_ROUNDS = 16;   // lists up to 16,384 in registers; more loop ov
 __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
                                                        const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
-                                                       int32_t *__restrict__ lfill, int32_t *__restrict__ totals) {
+                                                       int32_t *__restrict__ lfill, int32_t *__restrict__ lbfill,
+                                                       int32_t *__restrict__ totals) {
     __shared__ int32_t s_a[16], s_b[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     int32_t carry_a = 0, carry_b = 0;
@@ -202,6 +244,7 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
                 lstart[l] = carry_a + wa + a - v[r];
                 ioff[l] = carry_b + wb + b - it[r];
                 lfill[l] = 0;
+                lbfill[l] = 0;
             }
             carry_a += ta;
             carry_b += tb;
@@ -210,20 +253,25 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
     if (t == 0) {
         totals[0] = carry_a;
         totals[1] = carry_b;
+        totals[2] = 0;   // the sample pass's items, counted by k_item_hist
     }
 }
 
 // one wave per query: its pairs (non-empty probed lists) into their lists'
-// pair ranges, each with its coarse rank, the query's key scale and where
-// its keys start in the query's row (the probes' slot counts summed in
-// probe order). The scale is the power of two 2^(14 - e) with
+// pair ranges -- the sample's pairs from the front, the others from the
+// back -- each with its coarse rank, the query's key scale and where its
+// keys start in the query's row (the probes' slot counts summed in probe
+// order). The scale is the power of two 2^(14 - e) with
 // (sqrt(max coarse rank) + max|y|)^2 < 2^e: no probed entry's distance
 // (nor its BF16 estimate) reaches FP16's range once scaled, and every
 // scaled distance above 2^-28 of that bound keeps FP16's full precision.
 __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *__restrict__ probe_rank,
                                                      const unsigned *__restrict__ ymax,
                                                      const int32_t *__restrict__ lstart,
+                                                     const int32_t *__restrict__ lcnt,
+                                                     const int32_t *__restrict__ qsamp_np,
                                                      const int64_t *__restrict__ row_off, int32_t *__restrict__ lfill,
+                                                     int32_t *__restrict__ lbfill,
                                                      int32_t *__restrict__ pair_q, float *__restrict__ pair_qc,
                                                      float *__restrict__ pair_scale, int64_t *__restrict__ pair_off) {
     const int64_t q = blockIdx.x;
@@ -247,6 +295,7 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
         scale = __builtin_ldexpf(1.f, 14 - e);
     }
     int64_t carry = row_off[q];
+    const int snp = qsamp_np[q];
     for (int l0 = 0; l0 < npq; l0 += 64) {
         const int li = l0 + lane;
         int64_t l = 0, len = 0;
@@ -261,7 +310,8 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
             if (lane >= o) incl += v;
         }
         if (li < npq && len > 0) {
-            const int pos = lstart[l] + atomicAdd(&lfill[l], 1);
+            const int pos = li < snp ? lstart[l] + atomicAdd(&lfill[l], 1)
+                                     : lstart[l] + lcnt[l] - 1 - atomicAdd(&lbfill[l], 1);
             pair_q[pos] = (int32_t)q;
             pair_qc[pos] = rank_of(li, l);
             pair_scale[pos] = scale;
@@ -284,53 +334,80 @@ __device__ inline int item_bucket(int64_t len, int c) {
     return kItemBuckets - slabs;
 }
 
-// per bucket, the number of items of the batch
+// per bucket, the number of items of the batch (hist) and of the sample
+// pass (hist_s: items over each list's sample pairs, which lead its pair
+// range; their total is added to *n_samp)
 __global__ void __launch_bounds__(256) k_item_hist(const int32_t *__restrict__ lcnt,
+                                                   const int32_t *__restrict__ lscnt,
                                                    const int64_t *__restrict__ list_len, int K,
-                                                   int32_t *__restrict__ hist) {
-    __shared__ int32_t s_h[kItemBuckets];
-    if (threadIdx.x < kItemBuckets) s_h[threadIdx.x] = 0;
+                                                   int32_t *__restrict__ hist, int32_t *__restrict__ hist_s,
+                                                   int32_t *__restrict__ n_samp) {
+    __shared__ int32_t s_h[2 * kItemBuckets];
+    if (threadIdx.x < 2 * kItemBuckets) s_h[threadIdx.x] = 0;
     __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     if (l < K && lcnt[l] > 0) {
         const int64_t len = list_len[l];
-        const int nqg = (lcnt[l] + LQ - 1) / LQ;
+        const int nqg = (lcnt[l] + LQ - 1) / LQ, nsg = (lscnt[l] + LQ - 1) / LQ;
         const int nch = (int)((len + LCH - 1) / LCH);
-        for (int c = 0; c < nch; ++c) atomicAdd(&s_h[item_bucket(len, c)], nqg);
+        for (int c = 0; c < nch; ++c) {
+            atomicAdd(&s_h[item_bucket(len, c)], nqg);
+            if (nsg) atomicAdd(&s_h[kItemBuckets + item_bucket(len, c)], nsg);
+        }
     }
     __syncthreads();
-    if (threadIdx.x < kItemBuckets && s_h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], s_h[threadIdx.x]);
+    const int t = threadIdx.x;
+    if (t < 2 * kItemBuckets && s_h[t]) atomicAdd(t < kItemBuckets ? &hist[t] : &hist_s[t - kItemBuckets], s_h[t]);
+    if (t == 0) {
+        int tot = 0;
+        for (int j = 0; j < kItemBuckets; ++j) tot += s_h[kItemBuckets + j];
+        if (tot) atomicAdd(n_samp, tot);
+    }
 }
 
 // the items in bucket order: a block reserves its share of every bucket
-// with one atomic per bucket, then places its items in LDS order
+// with one atomic per bucket, then places its items in LDS order (the
+// batch's items, and the sample pass's in the same way)
 __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ lcnt,
+                                                    const int32_t *__restrict__ lscnt,
                                                     const int32_t *__restrict__ lstart,
                                                     const int64_t *__restrict__ list_len, int K,
                                                     const int32_t *__restrict__ hist, int32_t *__restrict__ fill,
-                                                    int4 *__restrict__ items) {
-    __shared__ int32_t s_c[kItemBuckets];
-    if (threadIdx.x < kItemBuckets) s_c[threadIdx.x] = 0;
+                                                    int4 *__restrict__ items, const int32_t *__restrict__ hist_s,
+                                                    int32_t *__restrict__ fill_s, int4 *__restrict__ items_s) {
+    __shared__ int32_t s_c[2 * kItemBuckets];
+    if (threadIdx.x < 2 * kItemBuckets) s_c[threadIdx.x] = 0;
     __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = l < K ? lcnt[l] : 0;
+    const int ns = n > 0 ? lscnt[l] : 0;
     const int64_t len = n > 0 ? list_len[l] : 0;
-    const int nqg = (n + LQ - 1) / LQ;
+    const int nqg = (n + LQ - 1) / LQ, nsg = (ns + LQ - 1) / LQ;
     const int nch = (int)((len + LCH - 1) / LCH);
-    for (int c = 0; c < nch; ++c) atomicAdd(&s_c[item_bucket(len, c)], nqg);
-    __syncthreads();
-    if (threadIdx.x < kItemBuckets) {
-        const int b = threadIdx.x;
-        int off = 0;
-        for (int j = 0; j < b; ++j) off += hist[j];
-        s_c[b] = s_c[b] ? off + atomicAdd(&fill[b], s_c[b]) : 0;
+    for (int c = 0; c < nch; ++c) {
+        atomicAdd(&s_c[item_bucket(len, c)], nqg);
+        if (nsg) atomicAdd(&s_c[kItemBuckets + item_bucket(len, c)], nsg);
     }
     __syncthreads();
+    if (threadIdx.x < 2 * kItemBuckets) {
+        const int set = threadIdx.x / kItemBuckets, b = threadIdx.x % kItemBuckets;
+        const int32_t *h = set ? hist_s : hist;
+        int off = 0;
+        for (int j = 0; j < b; ++j) off += h[j];
+        const int v = s_c[threadIdx.x];
+        s_c[threadIdx.x] = v ? off + atomicAdd(set ? &fill_s[b] : &fill[b], v) : 0;
+    }
+    __syncthreads();
+    const int ls = n > 0 ? lstart[l] : 0;
     for (int c = 0; c < nch; ++c) {
         const int b = item_bucket(len, c);
         for (int t = 0; t * LQ < n; ++t) {
             const int o = atomicAdd(&s_c[b], 1);
-            items[o] = make_int4(l, lstart[l] + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
+            items[o] = make_int4(l, ls + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
+        }
+        for (int t = 0; t * LQ < ns; ++t) {
+            const int o = atomicAdd(&s_c[kItemBuckets + b], 1);
+            items_s[o] = make_int4(l, ls + t * LQ, ns - t * LQ < LQ ? ns - t * LQ : LQ, c * LCH);
         }
     }
 }
@@ -376,14 +453,6 @@ constexpr int LNW = 8;
 
 // what a launch of the list-major scan writes
 enum { KIND_SAMPLE = 1, KIND_FILTER = 2 };
-// the sample pass scores the first ceil(slabs / SAMP_STRIDE) 32-entry slabs of each list
-constexpr int SAMP_STRIDE = 8;
-__device__ __forceinline__ int64_t sample_slots(int64_t len) {
-    const int64_t slabs = (len + 31) >> 5;
-    const int64_t s = 32 * ((slabs + SAMP_STRIDE - 1) / SAMP_STRIDE);
-    const int64_t r = row_slots(len);
-    return s < r ? s : r;
-}
 // the filter pass's ring of (record, row position) pairs per wave; it holds
 // fewer than 64 between slabs, and a register adds at most 64
 constexpr int RING = 256;
@@ -449,18 +518,6 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // groups of four entries starting below slot_end are stored (the
         // last chunk pads its query runs to a multiple of 8 slots)
         S.slot_end = S.ee == n ? (int)row_slots(n) : S.ee;
-        if (KIND == KIND_SAMPLE) {
-            // the sample: chunk 0's first slabs; other chunks score one
-            // empty slab and store nothing
-            const int ss = (int)sample_slots(n);
-            if (it.w == 0) {
-                S.ee = S.ee < ss ? S.ee : ss;
-                S.slot_end = ss;
-            } else {
-                S.ee = it.w;
-                S.slot_end = it.w;
-            }
-        }
         S.ns = S.ee > it.w ? (S.ee - it.w + 31) >> 5 : 1;
         S.pc = it.z;
         const int pidx = it.y + (col < it.z ? col : 0);
@@ -745,95 +802,57 @@ __device__ __forceinline__ void bins_find(const unsigned *hist, int nb, unsigned
 
 constexpr int QB = 8;   // loads in flight per thread in the select's first pass
 
-// The 4,096-bin histogram over key >> 4 (FP16 keys: 64 bins a binade).
-constexpr int QH_BINS = 4096;
+// the threshold's first histogram: 2,048 bins over key >> 5
+constexpr int TH_BINS = 2048;
 
-// One workgroup of 256 per query: theta[q] bounds the selk-th smallest key
-// of the query's sample (the first sample_slots of each probed list's run in
-// its row, written by the sample pass) from above: the last key of the
-// 4,096-bin histogram bin (key >> 4) where the sample's count reaches selk.
-// 0xFFFF (every key) when the sample holds fewer than selk keys or the
-// query takes the brute-force fallback. The filter pass recomputes the
-// sample's keys bit for bit, so at least selk entries of the query have
-// keys <= theta[q], and every entry of its selk best (by key, then entry
-// position) does.
+// One workgroup of 256 per query: theta[q] = the selk-th smallest key of the
+// query's sample, the prefix of its row of qsamp_slots[q] slots that the
+// sample pass wrote (pad slots hold 0xFFFF, above every key): a histogram
+// over key >> 5, then one over the low 5 bits of the bin where the count
+// reaches selk. 0xFFFF (every key) when the sample holds fewer than selk
+// keys or the query takes the brute-force fallback. The filter pass
+// recomputes the sample's keys bit for bit, so at least selk entries of the
+// query have keys <= theta[q], and every entry of its selk best (by key,
+// then entry position) does.
 __global__ void __launch_bounds__(256) k_theta(ScanArgs a, int64_t nq, const uint16_t *__restrict__ keys,
                                                const int64_t *__restrict__ row_off, const int64_t *__restrict__ qtot,
+                                               const int64_t *__restrict__ qsamp_slots,
                                                uint32_t *__restrict__ theta) {
-    extern __shared__ __align__(16) unsigned char s_raw[];
-    int64_t *s_poff = reinterpret_cast<int64_t *>(s_raw);               // [npq] row offset of each probe's run
-    int32_t *s_ss = reinterpret_cast<int32_t *>(s_poff + a.probe_cap);   // [npq] sample slots of each probe
-    unsigned *hist = reinterpret_cast<unsigned *>(s_ss + a.probe_cap + (a.probe_cap & 1));   // [QH_BINS]
+    __shared__ unsigned hist[TH_BINS];
+    __shared__ unsigned h2[32];
     __shared__ int s_bin;
     __shared__ unsigned s_below, s_cnt;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
-    int kq = a.k_q ? a.k_q[q] : a.k_default;
-    kq = kq < 1 ? 1 : (kq > a.k_default ? a.k_default : kq);
-    const unsigned selk = (unsigned)(kq * a.rerank);
-    if (a.allow_fallback && qtot[q] < kq) {
+    const unsigned selk = query_selk(a, q);
+    if (a.allow_fallback && qtot[q] < (int64_t)(selk / (unsigned)a.rerank)) {
         if (tid == 0) theta[q] = 0xFFFFu;
         return;
     }
-    const int npq = clamp_nprobe(a, q);
-    if (wave == 0) {
-        int64_t cr = 0;
-        for (int l0 = 0; l0 < npq; l0 += 64) {
-            const int li = l0 + lane;
-            int64_t len = 0;
-            if (li < npq) len = a.list_len[a.probes[q * a.probe_stride + li]];
-            const int64_t rs = row_slots(len);
-            int64_t ir = rs;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int64_t vr = __shfl_up(ir, o);
-                if (lane >= o) ir += vr;
-            }
-            if (li < npq) {
-                s_poff[li] = cr + ir - rs;
-                s_ss[li] = (int32_t)sample_slots(len);
-            }
-            cr += __shfl(ir, 63);
-        }
-    }
-    for (int b = tid; b < QH_BINS; b += 256) hist[b] = 0;
+    for (int b = tid; b < TH_BINS; b += 256) hist[b] = 0;
+    if (tid < 32) h2[tid] = 0;
     if (tid == 0) s_cnt = 0;
     __syncthreads();
-    // each wave takes whole probes, four at a time, lanes over the keys of
-    // a probe's sample (at most 2 x 64 for lists up to 4,096 entries; longer
-    // ones loop)
-    const uint16_t *row = keys + row_off[q];
+    // rows start at multiples of 8 slots, and the sample is whole lists' runs
+    const uint4 *row = reinterpret_cast<const uint4 *>(keys + row_off[q]);
+    const int64_t n8 = qsamp_slots[q] >> 3;
+    auto each_key = [&](const uint4 &v, auto f) {
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f((w[u >> 1] >> (16 * (u & 1))) & 0xFFFFu);
+    };
     unsigned c = 0;
-    for (int p0 = wave * 4; p0 < npq; p0 += 16) {
-        unsigned k[8];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int p = p0 + u;
-            const bool ok = p < npq;
-            const int ss = ok ? s_ss[p] : 0;
-            const int64_t b = ok ? s_poff[p] : 0;
-            k[2 * u] = lane < ss ? (unsigned)row[b + lane] : 0xFFFFu;
-            k[2 * u + 1] = lane + 64 < ss ? (unsigned)row[b + lane + 64] : 0xFFFFu;
-        }
-#pragma unroll
-        for (int u = 0; u < 8; ++u)
-            if (k[u] != 0xFFFFu) {
-                atomicAdd(&hist[k[u] >> 4], 1u);
+    for (int64_t i = tid; i < n8; i += 512) {
+        const uint4 v0 = row[i];
+        const uint4 v1 = i + 256 < n8 ? row[i + 256] : make_uint4(~0u, ~0u, ~0u, ~0u);
+        auto add = [&](uint32_t k) {
+            if (k != 0xFFFFu) {
+                atomicAdd(&hist[k >> 5], 1u);
                 ++c;
             }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int p = p0 + u;
-            if (p >= npq) continue;
-            const int ss = s_ss[p];
-            const int64_t b = s_poff[p];
-            for (int i = 128 + lane; i < ss; i += 64) {
-                const unsigned kk = row[b + i];
-                if (kk != 0xFFFFu) {
-                    atomicAdd(&hist[kk >> 4], 1u);
-                    ++c;
-                }
-            }
-        }
+        };
+        each_key(v0, add);
+        each_key(v1, add);
     }
     atomicAdd(&s_cnt, c);
     __syncthreads();
@@ -841,8 +860,23 @@ __global__ void __launch_bounds__(256) k_theta(ScanArgs a, int64_t nq, const uin
         if (tid == 0) theta[q] = 0xFFFFu;
         return;
     }
-    bins_find(hist, QH_BINS, selk, wave, lane, &s_bin, &s_below);
-    if (tid == 0) theta[q] = ((unsigned)s_bin << 4) | 15u;
+    bins_find(hist, TH_BINS, selk, wave, lane, &s_bin, &s_below);
+    const uint32_t bin = (uint32_t)s_bin;
+    for (int64_t i = tid; i < n8; i += 256)
+        each_key(row[i], [&](uint32_t k) {
+            if (k >> 5 == bin) atomicAdd(&h2[k & 31u], 1u);
+        });
+    __syncthreads();
+    if (wave == 0) {
+        const unsigned h = lane < 32 ? h2[lane] : 0u;
+        unsigned incl = h;
+        for (int o = 1; o < 32; o <<= 1) {
+            const unsigned v = __shfl_up(incl, o);
+            if (lane >= o) incl += v;
+        }
+        const uint64_t hit = __ballot(lane < 32 && s_below + incl >= selk);
+        if (lane == 0) theta[q] = (bin << 5) | (uint32_t)(hit ? __builtin_ctzll(hit) : 31);
+    }
 }
 
 constexpr int SELR_ROOM = 1024;    // records of the boundary bin the select keeps in LDS
@@ -1111,10 +1145,11 @@ void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st) {
     HANN_LAUNCH_CHECK();
 }
 
-void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int64_t *qtot, int64_t *qslots,
-                       hipStream_t st) {
+void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot, int64_t *qslots,
+                       int32_t *qsamp_np, int64_t *qsamp_slots, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_pair_count, dim3((unsigned)nq), dim3(64), 0, st, a, lcnt, qtot, qslots);
+    hipLaunchKernelGGL(dev::k_pair_count, dim3((unsigned)nq), dim3(64), 0, st, a, lcnt, lscnt, qtot, qslots, qsamp_np,
+                       qsamp_slots);
     HANN_LAUNCH_CHECK();
 }
 
@@ -1125,32 +1160,36 @@ void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hip
 }
 
 void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
-                         int32_t *lfill, int32_t *totals, hipStream_t st) {
-    hipLaunchKernelGGL(dev::k_pair_offsets, dim3(1), dim3(1024), 0, st, lcnt, list_len, K, lstart, ioff, lfill, totals);
+                         int32_t *lfill, int32_t *lbfill, int32_t *totals, hipStream_t st) {
+    hipLaunchKernelGGL(dev::k_pair_offsets, dim3(1), dim3(1024), 0, st, lcnt, list_len, K, lstart, ioff, lfill, lbfill,
+                       totals);
     HANN_LAUNCH_CHECK();
 }
 
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
-                         const int32_t *lstart, const int64_t *row_off, int32_t *lfill, int32_t *pair_q,
-                         float *pair_qc, float *pair_scale, int64_t *pair_off, hipStream_t st) {
+                         const int32_t *lstart, const int32_t *lcnt, const int32_t *qsamp_np, const int64_t *row_off,
+                         int32_t *lfill, int32_t *lbfill, int32_t *pair_q, float *pair_qc, float *pair_scale,
+                         int64_t *pair_off, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_pair_scatter, dim3((unsigned)nq), dim3(64), 0, st, a, probe_rank, ymax, lstart, row_off,
-                       lfill, pair_q, pair_qc, pair_scale, pair_off);
+    hipLaunchKernelGGL(dev::k_pair_scatter, dim3((unsigned)nq), dim3(64), 0, st, a, probe_rank, ymax, lstart, lcnt,
+                       qsamp_np, row_off, lfill, lbfill, pair_q, pair_qc, pair_scale, pair_off);
     HANN_LAUNCH_CHECK();
 }
 
-void launch_make_items(const int32_t *lcnt, const int32_t *lstart, const int64_t *list_len, int K, int32_t *order,
-                       int4 *items, hipStream_t st) {
-    int32_t *hist = order, *fill = order + dev::kItemBuckets;
-    launch_fill_i32(order, 2 * dev::kItemBuckets, 0, st);
-    hipLaunchKernelGGL(dev::k_item_hist, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, list_len, K, hist);
+void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t *lstart, const int64_t *list_len,
+                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st) {
+    constexpr int B = dev::kItemBuckets;
+    int32_t *hist = order, *fill = order + B, *hist_s = order + 2 * B, *fill_s = order + 3 * B;
+    launch_fill_i32(order, 4 * B, 0, st);
+    hipLaunchKernelGGL(dev::k_item_hist, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lscnt, list_len, K,
+                       hist, hist_s, n_samp);
     HANN_LAUNCH_CHECK();
-    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lstart, list_len,
-                       K, hist, fill, items);
+    hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lscnt, lstart,
+                       list_len, K, hist, fill, items, hist_s, fill_s, items_s);
     HANN_LAUNCH_CHECK();
 }
 
-int list_item_order_ints() { return 2 * dev::kItemBuckets; }
+int list_item_order_ints() { return 4 * dev::kItemBuckets; }
 
 int list_scan_chunk() { return dev::LCH; }
 
@@ -1206,11 +1245,10 @@ size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind) {
 }
 
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
-                  uint32_t *theta, hipStream_t st) {
+                  const int64_t *qsamp_slots, uint32_t *theta, hipStream_t st) {
     if (nq == 0) return;
-    const size_t shm = (size_t)a.probe_cap * 8 + (size_t)(a.probe_cap + 1) * 4 + 4 + (size_t)dev::QH_BINS * 4;
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_theta, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
-    hipLaunchKernelGGL(dev::k_theta, dim3((unsigned)nq), dim3(256), shm, st, a, nq, keys, row_off, qtot, theta);
+    hipLaunchKernelGGL(dev::k_theta, dim3((unsigned)nq), dim3(256), 0, st, a, nq, keys, row_off, qtot, qsamp_slots,
+                       theta);
     HANN_LAUNCH_CHECK();
 }
 
