@@ -1186,16 +1186,20 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         chunks_all += ceil_div(v, chunk);
     }
     const int64_t items_max = ceil_div(pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
-    const size_t o_lcnt = carve((size_t)K * 4), o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
-                 o_lfill = carve((size_t)K * 4), o_lscnt = carve((size_t)K * 4), o_lbfill = carve((size_t)K * 4),
+    // the buffers that start at zero first, cleared by one fill
+    const size_t o_lcnt = carve((size_t)K * 4), o_lscnt = carve((size_t)K * 4),
+                 o_iord = carve((size_t)list_item_order_ints() * 4), o_ctr = carve(16),
+                 o_parts = carve(kListScanParts * 4), o_parts_f = carve(kListScanParts * 4), o_zero_end = off;
+    const size_t o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
+                 o_lfill = carve((size_t)K * 4), o_lbfill = carve((size_t)K * 4),
                  o_qsnp = carve((size_t)nq * 4), o_qsamp = carve((size_t)nq * 8), o_tot = carve(16),
                  o_qtot = carve((size_t)nq * 8),
                  o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_theta = carve((size_t)nq * 8),
-                 o_ovf = carve((size_t)nq * 4), o_ctr = carve(16), o_parts = carve(kListScanParts * 4), o_pq = carve((size_t)pairs_max * 4),
+                 o_ovf = carve((size_t)nq * 4), o_pq = carve((size_t)pairs_max * 4),
                  o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
                  o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
                  o_items_s = carve((size_t)items_max * 16),
-                 o_pkey = carve((size_t)nq * selk * 4), o_iord = carve((size_t)list_item_order_ints() * 4);
+                 o_pkey = carve((size_t)nq * selk * 4);
     char *base = w.get<char>(17, off);
     auto at = [&](size_t o) { return base + o; };
     int32_t *lcnt = reinterpret_cast<int32_t *>(at(o_lcnt));
@@ -1212,7 +1216,8 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int64_t *roff = reinterpret_cast<int64_t *>(at(o_roff));
     int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
     int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [1] fallbacks, [2] fallback queue
-    int32_t *parts = reinterpret_cast<int32_t *>(at(o_parts));   // items taken per partition
+    int32_t *parts = reinterpret_cast<int32_t *>(at(o_parts));   // items taken per partition (sample pass)
+    int32_t *parts_f = reinterpret_cast<int32_t *>(at(o_parts_f));   // (filter pass)
     int32_t *pair_q = reinterpret_cast<int32_t *>(at(o_pq));
     float *pair_qc = reinterpret_cast<float *>(at(o_pqc));
     float *pair_scale = reinterpret_cast<float *>(at(o_psc));
@@ -1235,8 +1240,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
 
     // 1. the batch's (query, list) pairs grouped by list, as work items (and
     //    the sample pass's over the sample pairs), and each query's row
-    launch_fill_i32(lcnt, K, 0, st);
-    launch_fill_i32(lscnt, K, 0, st);
+    launch_fill_i32(lcnt, (int64_t)(o_zero_end - o_lcnt) / 4, 0, st);
     launch_pair_count(a, nq, lcnt, lscnt, qtot, qslots, qsamp_np, qsamp, st);
     launch_row_offsets(qslots, nq, roff, st);
     launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, lbfill, totals, st);
@@ -1273,8 +1277,6 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.cnt_bytes = (uint32_t)((rows_max / 4 + 8) * 2);
     // 2. the sample pass: every key of each query's nearest probed lists;
     //    then each query's bound theta, the (k R)-th smallest of them
-    launch_fill_i32(ctr, 4, 0, st);
-    launch_fill_i32(parts, kListScanParts, 0, st);
     {
         ListScanArgs ls = la;
         ls.items = items_s;
@@ -1284,7 +1286,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     launch_theta(a, nq, reinterpret_cast<const uint16_t *>(rec), roff, qtot, qsamp, theta, st);
     // 3. the filter pass: every (query, entry) key, and the records of those
     //    <= theta compacted per (chunk, half) region
-    launch_fill_i32(parts, kListScanParts, 0, st);
+    la.item_counter = parts_f;
     mark_kernel(0, st);
     launch_scan_list_kind(la, x3, 2, grid, st);
     mark_kernel(1, st);

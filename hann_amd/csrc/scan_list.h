@@ -63,7 +63,8 @@ void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank
                          int64_t *pair_off, hipStream_t st);
 // the batch's work items and the sample pass's (over each list's sample
 // pairs; their count is added to *n_samp), longest first; order:
-// list_item_order_ints() ints of scratch (per-bucket counts and fill counters)
+// list_item_order_ints() ints of scratch (per-bucket counts and fill counters),
+// zeroed by the caller
 void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t *lstart, const int64_t *list_len,
                        int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st);
 int list_item_order_ints();

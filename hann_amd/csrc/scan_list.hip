@@ -189,9 +189,9 @@ __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
 }
 
 // one workgroup of 1024: exclusive scans of lcnt and of the item counts,
-// in rounds of 1024 lists (thread t holds list r * 1024 + t of round r, all
-// rounds' loads issued first)
-constexpr int PO_ROUNDS = 16;   // lists up to 16,384 in registers; more loop over rounds of that
+// in spans of 1024 * PO_PER lists (thread t sums lists t * PO_PER ..
+// + PO_PER - 1 of a span on its own; one block scan of the sums a span)
+constexpr int PO_PER = 16;
 
 __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
                                                        const int64_t *__restrict__ list_len, int K,
@@ -201,11 +201,12 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
     __shared__ int32_t s_a[16], s_b[16];
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     int32_t carry_a = 0, carry_b = 0;
-    for (int g0 = 0; g0 < K; g0 += 1024 * PO_ROUNDS) {
-        int32_t v[PO_ROUNDS], it[PO_ROUNDS];
+    for (int g0 = 0; g0 < K; g0 += 1024 * PO_PER) {
+        const int base = g0 + t * PO_PER;
+        int32_t v[PO_PER], it[PO_PER], sa = 0, sb = 0;
 #pragma unroll
-        for (int r = 0; r < PO_ROUNDS; ++r) {
-            const int l = g0 + r * 1024 + t;
+        for (int r = 0; r < PO_PER; ++r) {
+            const int l = base + r;
             const int lc = l < K ? l : K - 1;
             const int32_t c = lcnt[lc];
             const int64_t n = list_len[lc];
@@ -213,42 +214,50 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
             it[r] = l < K ? list_items(c, n) : 0;
         }
 #pragma unroll
-        for (int r = 0; r < PO_ROUNDS; ++r) {
-            // block-wide inclusive scan of (v, it) over the round
-            int32_t a = v[r], b = it[r];
-            for (int o = 1; o < 64; o <<= 1) {
-                const int32_t xa = __shfl_up(a, o), xb = __shfl_up(b, o);
-                if (lane >= o) {
-                    a += xa;
-                    b += xb;
-                }
+        for (int r = 0; r < PO_PER; ++r) {
+            sa += v[r];
+            sb += it[r];
+        }
+        // block-wide exclusive scan of (sa, sb)
+        int32_t a = sa, b = sb;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t xa = __shfl_up(a, o), xb = __shfl_up(b, o);
+            if (lane >= o) {
+                a += xa;
+                b += xb;
             }
-            if (lane == 63) {
-                s_a[w] = a;
-                s_b[w] = b;
+        }
+        if (lane == 63) {
+            s_a[w] = a;
+            s_b[w] = b;
+        }
+        __syncthreads();
+        int32_t wa = 0, wb = 0, ta = 0, tb = 0;
+        for (int u = 0; u < 16; ++u) {
+            const int32_t ua = s_a[u], ub = s_b[u];
+            if (u < w) {
+                wa += ua;
+                wb += ub;
             }
-            __syncthreads();
-            int32_t wa = 0, wb = 0, ta = 0, tb = 0;
-            for (int u = 0; u < 16; ++u) {
-                const int32_t ua = s_a[u], ub = s_b[u];
-                if (u < w) {
-                    wa += ua;
-                    wb += ub;
-                }
-                ta += ua;
-                tb += ub;
-            }
-            __syncthreads();
-            const int l = g0 + r * 1024 + t;
+            ta += ua;
+            tb += ub;
+        }
+        __syncthreads();
+        int32_t ra = carry_a + wa + a - sa, rb = carry_b + wb + b - sb;
+#pragma unroll
+        for (int r = 0; r < PO_PER; ++r) {
+            const int l = base + r;
             if (l < K) {
-                lstart[l] = carry_a + wa + a - v[r];
-                ioff[l] = carry_b + wb + b - it[r];
+                lstart[l] = ra;
+                ioff[l] = rb;
                 lfill[l] = 0;
                 lbfill[l] = 0;
             }
-            carry_a += ta;
-            carry_b += tb;
+            ra += v[r];
+            rb += it[r];
         }
+        carry_a += ta;
+        carry_b += tb;
     }
     if (t == 0) {
         totals[0] = carry_a;
@@ -1180,7 +1189,6 @@ void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t 
                        int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st) {
     constexpr int B = dev::kItemBuckets;
     int32_t *hist = order, *fill = order + B, *hist_s = order + 2 * B, *fill_s = order + 3 * B;
-    launch_fill_i32(order, 4 * B, 0, st);
     hipLaunchKernelGGL(dev::k_item_hist, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lscnt, list_len, K,
                        hist, hist_s, n_samp);
     HANN_LAUNCH_CHECK();
