@@ -33,6 +33,11 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
                     const int32_t *hint, int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st,
                     const int32_t *k_q = nullptr, BF16Base *cache = nullptr, Workspace *ws = nullptr);
 
+// an upper bound of the workspace bytes certified_topk takes for this shape
+// (cached: the base's BF16 copy lives in a cache, not in the workspace),
+// before an overflowed query's exact rerun
+size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool cached);
+
 // bases of at most this many rows: certified_topk with a workspace runs
 // without host synchronisation (an overflowed query is ranked over every row
 // on the device)

@@ -20,6 +20,37 @@ __global__ void k_copy_u32(unsigned *__restrict__ dst, const unsigned *__restric
     if (i < n) dst[i] = src[i];
 }
 
+size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool cached) {
+    // the buffers certified_topk takes, by the same sizing rules (256 bytes
+    // of slack each, as the workspace adds)
+    const int64_t dp = ceil_div(dim, 32) * 32;
+    size_t b = 0;
+    auto add = [&](int64_t bytes) { b += (size_t)bytes + 256; };
+    add(16);
+    if (!cached) {
+        add(nb * dp * 2);
+        add(nb * 4);
+        add(nb * 4);
+    }
+    add(nq * dp * 2);
+    add(nq * 4);
+    add(nq * 4);
+    add(nq * 4);
+    const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
+    const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+    const int64_t ns = ceil_div(nb, stride);
+    add(ns * dp * 2);
+    add(ns * 4);
+    const int64_t qchunk = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(256ll << 20) / (4 * ns)));
+    add(std::max<int64_t>(qchunk * ns * 4, nq * 8));
+    const int64_t expect = std::max<int64_t>(k, ceil_div((int64_t)k * nb, ns));
+    const int64_t cap = std::min<int64_t>(nb, std::max<int64_t>(256, 8 * expect));
+    const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
+    add(nq * 4);
+    add(chunk * cap * 4);
+    return b;
+}
+
 // Certified BF16 path for the (squared) Euclidean metrics. Results equal the
 // exact path's: the filter keeps every row whose exact FP32 rank can be at
 // most the query's sample k-th rank, and those rows are ranked exactly.

@@ -367,6 +367,14 @@ static void topk_dispatch(int metric, const float *queries, int64_t nq, const fl
     }
 }
 
+size_t topk_exact_scratch_bytes(int metric, int64_t nq, int64_t nb, int k) {
+    const int kp = kp_for(k);
+    if (nq == 0 || kp > 512) return 0;
+    int64_t nsplit = 0, rows = 0;
+    topk_plan(metric, kp, nq, nb, 65535, nsplit, rows);
+    return (size_t)(nq * nsplit * k) * 12 + 512;
+}
+
 void launch_topk_exact(int metric, const float *queries, int64_t nq, const float *base, int64_t nb,
                        int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws,
                        hipStream_t st) {

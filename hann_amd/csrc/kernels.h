@@ -24,12 +24,20 @@ struct Workspace {
         if (!aux_) aux_.reset(new Workspace);
         return *aux_;
     }
+    // device bytes held (this workspace's slots and its nested one's)
+    size_t bytes() const {
+        size_t b = 0;
+        for (const auto &s : slots) b += s.n;
+        return b + (aux_ ? aux_->bytes() : 0);
+    }
 
   private:
     std::unique_ptr<Workspace> aux_;
 };
 
 int kp_for(int k);
+// an upper bound of the workspace bytes launch_topk_exact takes for this shape
+size_t topk_exact_scratch_bytes(int metric, int64_t nq, int64_t nb, int k);
 
 // distance.hip
 // normalize n rows of dim floats in place (core/simd_kernels.inc.h:267-315)

@@ -1038,13 +1038,13 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         fail(HANN_ERR_UNSUPPORTED, strf("scan mode %d is not supported", mode));
     bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
     const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
-    int64_t rows_max = 0;
+    PQIVF::ListPlan lplan{};
     if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
                       reinterpret_cast<uintptr_t>(dq) % 16 != 0 || np > select_rows_max_probes() ||
                       !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
                       list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2) > 160 * 1024 ||
-                      select_records_lds_bytes(list_select_regions(np)) > 160 * 1024 ||
-                      (rows_max = list_rows_bound(nq, np)) > kListRowsMax)) {
+                      select_records_lds_bytes() > 160 * 1024 ||
+                      !list_plan_fits(lplan = list_plan(nq, np, R > 0 ? kmax * R : kmax), nq))) {
         // the list-major scan needs a rerank, codes for every entry, a probe
         // row per query, a supported shape and room for the batch's rows and
         // a query's probed list chunks; otherwise the persistent table scan runs
@@ -1062,7 +1062,7 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     last_scan_fused = (fused && !list_scan) ? 1 : 0;
     auto scan = [&](const ScanArgs &args) {
         if (list_scan)
-            run_list_scan(args, nq, x3, np, probe_rank, st, w, rows_max);
+            run_list_scan(args, nq, x3, np, probe_rank, st, w, lplan);
         else if (fused)
             run_fused_scan(args, nq, np, probe_rank, st, w);
         else {
@@ -1163,20 +1163,50 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
     if (main_scan) mark_kernel(1, st);
 }
 
-void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank,
-                          hipStream_t st, Workspace &w, int64_t rows_max) {
-    const int K = (int)lists.size();
-    const int selk = a.out_stride;
-    const int np = a.probe_cap;
-    const int kp = a.cap / 2;
-    const int64_t pairs_max = nq * (int64_t)np;
-    // arena (slot 17) for the batch's small buffers
+// The list-major scan's batch buffers: the arena's layout (one workspace
+// slot, the buffers that start at zero first so one fill clears them), the
+// record room per query and the sample rows' size.
+struct ListArena {
     size_t off = 0;
-    auto carve = [&](size_t bytes) {
+    size_t o_lcnt, o_lscnt, o_iord, o_ctr, o_parts, o_parts_f, o_qcnt, o_zero_end, o_lstart, o_ioff, o_lfill, o_lbfill,
+        o_qsnp, o_qsamp, o_tot, o_qtot, o_soff, o_theta, o_ovf, o_pq, o_pqc, o_psc, o_poff, o_items, o_items_s, o_pkey;
+    size_t carve(size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~(size_t)255;
         return o;
-    };
+    }
+    ListArena(int64_t K, int64_t nq, int64_t pairs_max, int64_t items_max, int selk) {
+        o_lcnt = carve((size_t)K * 4);
+        o_lscnt = carve((size_t)K * 4);
+        o_iord = carve((size_t)list_item_order_ints() * 4);
+        o_ctr = carve(16);
+        o_parts = carve(kListScanParts * 4);
+        o_parts_f = carve(kListScanParts * 4);
+        o_qcnt = carve((size_t)nq * 4);
+        o_zero_end = off;
+        o_lstart = carve((size_t)K * 4);
+        o_ioff = carve((size_t)K * 4);
+        o_lfill = carve((size_t)K * 4);
+        o_lbfill = carve((size_t)K * 4);
+        o_qsnp = carve((size_t)nq * 4);
+        o_qsamp = carve((size_t)nq * 8);
+        o_tot = carve(16);
+        o_qtot = carve((size_t)nq * 8);
+        o_soff = carve((size_t)nq * 8);
+        o_theta = carve((size_t)nq * 8);
+        o_ovf = carve((size_t)nq * 4);
+        o_pq = carve((size_t)pairs_max * 4);
+        o_pqc = carve((size_t)pairs_max * 4);
+        o_psc = carve((size_t)pairs_max * 4);
+        o_poff = carve((size_t)pairs_max * 8);
+        o_items = carve((size_t)items_max * 16);
+        o_items_s = carve((size_t)items_max * 16);
+        o_pkey = carve((size_t)nq * selk * 4);
+    }
+};
+
+PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk) const {
+    ListPlan L;
     // work items: a list with p pairs and n entries has ceil(p / 32) *
     // ceil(n / chunk) of them, at most (p / 32 + 1) * ceil(n / chunk)
     const int64_t chunk = list_scan_chunk();
@@ -1185,46 +1215,50 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         max_len = std::max(max_len, v);
         chunks_all += ceil_div(v, chunk);
     }
-    const int64_t items_max = ceil_div(pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
-    // the buffers that start at zero first, cleared by one fill
-    const size_t o_lcnt = carve((size_t)K * 4), o_lscnt = carve((size_t)K * 4),
-                 o_iord = carve((size_t)list_item_order_ints() * 4), o_ctr = carve(16),
-                 o_parts = carve(kListScanParts * 4), o_parts_f = carve(kListScanParts * 4), o_zero_end = off;
-    const size_t o_lstart = carve((size_t)K * 4), o_ioff = carve((size_t)K * 4),
-                 o_lfill = carve((size_t)K * 4), o_lbfill = carve((size_t)K * 4),
-                 o_qsnp = carve((size_t)nq * 4), o_qsamp = carve((size_t)nq * 8), o_tot = carve(16),
-                 o_qtot = carve((size_t)nq * 8),
-                 o_qslots = carve((size_t)nq * 8), o_roff = carve((size_t)nq * 8), o_theta = carve((size_t)nq * 8),
-                 o_ovf = carve((size_t)nq * 4), o_pq = carve((size_t)pairs_max * 4),
-                 o_pqc = carve((size_t)pairs_max * 4), o_psc = carve((size_t)pairs_max * 4),
-                 o_poff = carve((size_t)pairs_max * 8), o_items = carve((size_t)items_max * 16),
-                 o_items_s = carve((size_t)items_max * 16),
-                 o_pkey = carve((size_t)nq * selk * 4);
-    char *base = w.get<char>(17, off);
+    L.pairs_max = nq * (int64_t)np;
+    L.items_max = ceil_div(L.pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
+    // a query's sample: at least two probes, then on while fewer than
+    // 4 selk entries come before: at most max(4 selk + max_len, 2 max_len)
+    // entries, each probe's run rounded up to 8 slots
+    L.sample_slots = nq * (std::max<int64_t>(4 * (int64_t)selk + max_len, 2 * max_len) + 7 * (int64_t)np);
+    // record room: the filter pass keeps about 1.5 selk records a query
+    // (at most every probed entry); a query with more takes the table scan
+    L.capq = std::max<int64_t>(8 * (int64_t)selk, 256);
+    L.arena = ListArena((int64_t)lists.size(), nq, L.pairs_max, L.items_max, selk).off;
+    return L;
+}
+
+void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank,
+                          hipStream_t st, Workspace &w, const ListPlan &L) {
+    const int K = (int)lists.size();
+    const int selk = a.out_stride;
+    const int kp = a.cap / 2;
+    const ListArena A(K, nq, L.pairs_max, L.items_max, selk);
+    char *base = w.get<char>(17, A.off);
     auto at = [&](size_t o) { return base + o; };
-    int32_t *lcnt = reinterpret_cast<int32_t *>(at(o_lcnt));
-    int32_t *lstart = reinterpret_cast<int32_t *>(at(o_lstart));
-    int32_t *ioff = reinterpret_cast<int32_t *>(at(o_ioff));
-    int32_t *lfill = reinterpret_cast<int32_t *>(at(o_lfill));
-    int32_t *lscnt = reinterpret_cast<int32_t *>(at(o_lscnt));   // sample pairs per list
-    int32_t *lbfill = reinterpret_cast<int32_t *>(at(o_lbfill));
-    int32_t *qsamp_np = reinterpret_cast<int32_t *>(at(o_qsnp));   // probes in a query's sample
-    int64_t *qsamp = reinterpret_cast<int64_t *>(at(o_qsamp));     // row slots of a query's sample
-    int32_t *totals = reinterpret_cast<int32_t *>(at(o_tot));
-    int64_t *qtot = reinterpret_cast<int64_t *>(at(o_qtot));
-    int64_t *qslots = reinterpret_cast<int64_t *>(at(o_qslots));
-    int64_t *roff = reinterpret_cast<int64_t *>(at(o_roff));
-    int32_t *ovf = reinterpret_cast<int32_t *>(at(o_ovf));
-    int32_t *ctr = reinterpret_cast<int32_t *>(at(o_ctr));   // [1] fallbacks, [2] fallback queue
-    int32_t *parts = reinterpret_cast<int32_t *>(at(o_parts));   // items taken per partition (sample pass)
-    int32_t *parts_f = reinterpret_cast<int32_t *>(at(o_parts_f));   // (filter pass)
-    int32_t *pair_q = reinterpret_cast<int32_t *>(at(o_pq));
-    float *pair_qc = reinterpret_cast<float *>(at(o_pqc));
-    float *pair_scale = reinterpret_cast<float *>(at(o_psc));
-    int64_t *pair_off = reinterpret_cast<int64_t *>(at(o_poff));
-    int4 *items = reinterpret_cast<int4 *>(at(o_items));
-    int4 *items_s = reinterpret_cast<int4 *>(at(o_items_s));   // the sample pass's items
-    float *pkey = reinterpret_cast<float *>(at(o_pkey));
+    int32_t *lcnt = reinterpret_cast<int32_t *>(at(A.o_lcnt));
+    int32_t *lstart = reinterpret_cast<int32_t *>(at(A.o_lstart));
+    int32_t *ioff = reinterpret_cast<int32_t *>(at(A.o_ioff));
+    int32_t *lfill = reinterpret_cast<int32_t *>(at(A.o_lfill));
+    int32_t *lscnt = reinterpret_cast<int32_t *>(at(A.o_lscnt));   // sample pairs per list
+    int32_t *lbfill = reinterpret_cast<int32_t *>(at(A.o_lbfill));
+    int32_t *qsamp_np = reinterpret_cast<int32_t *>(at(A.o_qsnp));   // probes in a query's sample
+    int64_t *qsamp = reinterpret_cast<int64_t *>(at(A.o_qsamp));     // row slots of a query's sample
+    int32_t *totals = reinterpret_cast<int32_t *>(at(A.o_tot));
+    int64_t *qtot = reinterpret_cast<int64_t *>(at(A.o_qtot));
+    int64_t *soff = reinterpret_cast<int64_t *>(at(A.o_soff));   // sample row starts
+    uint32_t *qcnt = reinterpret_cast<uint32_t *>(at(A.o_qcnt));   // records per query
+    int32_t *ovf = reinterpret_cast<int32_t *>(at(A.o_ovf));
+    int32_t *ctr = reinterpret_cast<int32_t *>(at(A.o_ctr));   // [1] fallbacks, [2] fallback queue
+    int32_t *parts = reinterpret_cast<int32_t *>(at(A.o_parts));   // items taken per partition (sample pass)
+    int32_t *parts_f = reinterpret_cast<int32_t *>(at(A.o_parts_f));   // (filter pass)
+    int32_t *pair_q = reinterpret_cast<int32_t *>(at(A.o_pq));
+    float *pair_qc = reinterpret_cast<float *>(at(A.o_pqc));
+    float *pair_scale = reinterpret_cast<float *>(at(A.o_psc));
+    int64_t *pair_off = reinterpret_cast<int64_t *>(at(A.o_poff));
+    int4 *items = reinterpret_cast<int4 *>(at(A.o_items));
+    int4 *items_s = reinterpret_cast<int4 *>(at(A.o_items_s));   // the sample pass's items
+    float *pkey = reinterpret_cast<float *>(at(A.o_pkey));
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
@@ -1232,22 +1266,21 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     }
     const size_t shm = list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2);
     const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / shm));
-    // one 32-bit slot per row slot: the sample pass's 16-bit keys, then the
-    // filter pass's records; and a record count per (chunk, half) region
-    uint32_t *rec = w.get<uint32_t>(18, (size_t)rows_max);
-    uint16_t *rcnt = w.get<uint16_t>(19, (size_t)rows_max / 4 + 8);
-    uint32_t *theta = reinterpret_cast<uint32_t *>(at(o_theta));   // [nq]
+    // the filter pass's records, capq a query; the sample pass's keys
+    uint64_t *rec = w.get<uint64_t>(18, (size_t)(nq * L.capq));
+    uint16_t *srow = w.get<uint16_t>(19, (size_t)L.sample_slots);
+    uint32_t *theta = reinterpret_cast<uint32_t *>(at(A.o_theta));   // [nq]
 
     // 1. the batch's (query, list) pairs grouped by list, as work items (and
-    //    the sample pass's over the sample pairs), and each query's row
-    launch_fill_i32(lcnt, (int64_t)(o_zero_end - o_lcnt) / 4, 0, st);
-    launch_pair_count(a, nq, lcnt, lscnt, qtot, qslots, qsamp_np, qsamp, st);
-    launch_row_offsets(qslots, nq, roff, st);
+    //    the sample pass's over the sample pairs), and each query's sample row
+    launch_fill_i32(lcnt, (int64_t)(A.o_zero_end - A.o_lcnt) / 4, 0, st);
+    launch_pair_count(a, nq, lcnt, lscnt, qtot, qsamp_np, qsamp, st);
+    launch_row_offsets(qsamp, nq, soff, st);
     launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, lbfill, totals, st);
-    launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, lcnt, qsamp_np, roff, lfill, lbfill, pair_q,
+    launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, lcnt, qsamp_np, soff, lfill, lbfill, pair_q,
                         pair_qc, pair_scale, pair_off, st);
-    launch_make_items(lcnt, lscnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(o_iord)), items, items_s,
-                      totals + 2, st);
+    launch_make_items(lcnt, lscnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(A.o_iord)), items,
+                      items_s, totals + 2, st);
     ListScanArgs la{};
     la.queries = a.queries;
     la.d = (int)dim;
@@ -1269,12 +1302,12 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.n_items = totals + 1;
     la.item_counter = parts;
     la.n_parts = std::min(kListScanParts, grid);
-    la.dist = reinterpret_cast<uint16_t *>(rec);
-    la.dist_bytes = (uint32_t)(rows_max * 2);
+    la.dist = srow;
+    la.dist_bytes = (uint32_t)(L.sample_slots * 2);
     la.theta = theta;
     la.rec = rec;
-    la.cnt = rcnt;
-    la.cnt_bytes = (uint32_t)((rows_max / 4 + 8) * 2);
+    la.qcnt = qcnt;
+    la.capq = (uint32_t)L.capq;
     // 2. the sample pass: every key of each query's nearest probed lists;
     //    then each query's bound theta, the (k R)-th smallest of them
     {
@@ -1283,18 +1316,18 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         ls.n_items = totals + 2;
         launch_scan_list_kind(ls, x3, 1, grid, st);
     }
-    launch_theta(a, nq, reinterpret_cast<const uint16_t *>(rec), roff, qtot, qsamp, theta, st);
+    launch_theta(a, nq, srow, soff, qtot, qsamp, theta, st);
     // 3. the filter pass: every (query, entry) key, and the records of those
-    //    <= theta compacted per (chunk, half) region
+    //    <= theta in the query's record room
     la.item_counter = parts_f;
     mark_kernel(0, st);
     launch_scan_list_kind(la, x3, 2, grid, st);
     mark_kernel(1, st);
     // 4. the selk best records per query, as the rerank's rows
-    const int max_regions = list_select_regions(np);
-    launch_select_records(a, nq, rec, rcnt, roff, qtot, a.out_id, a.out_cnt, selk, max_regions, ovf, ctr + 1, st);
-    // 5. brute-force fallbacks: the table scan over all lists (the
-    //    reference's rule, counter included)
+    launch_select_records(a, nq, rec, qcnt, (uint32_t)L.capq, qtot, a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
+    // 5. the queries the select passed on: the table scan ranks them (over
+    //    all lists where the reference's brute-force rule applies, counter
+    //    included; over their probes when their records overflowed)
     FusedArgs f{};
     f.term = d_term.get();
     f.probe_rank = probe_rank;
@@ -1312,23 +1345,11 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
 }
 
-int PQIVF::list_select_regions(int np) const {
-    // two halves of every chunk of np probed lists at most
-    int64_t max_ch = 1;
-    for (int64_t v : dev_len) max_ch = std::max(max_ch, ceil_div(v, (int64_t)list_scan_chunk()));
-    return (int)std::min<int64_t>(2 * (int64_t)std::max(np, 1) * max_ch, 1 << 20);
-}
-
-int64_t PQIVF::list_rows_bound(int64_t nq, int np) const {
-    // each query's row holds at most its np longest probed lists, each
-    // rounded up to 8 slots
-    std::vector<int64_t> len(dev_len);
-    for (int64_t &v : len) v = (v + 7) & ~(int64_t)7;
-    const size_t top = std::min(len.size(), (size_t)std::max(np, 0));
-    std::nth_element(len.begin(), len.begin() + (ptrdiff_t)top, len.end(), std::greater<int64_t>());
-    int64_t s = 0;
-    for (size_t i = 0; i < top; ++i) s += len[i];
-    return nq * s;
+bool PQIVF::list_plan_fits(const ListPlan &L, int64_t nq) {
+    // 16-bit sample keys addressed below 2^32 - 64 bytes; record rooms and
+    // entry positions addressed in 32 bits
+    return L.sample_slots * 2 < (int64_t(1) << 32) - 64 && nq * L.capq < (int64_t(1) << 31) &&
+           L.pairs_max < (int64_t(1) << 31);
 }
 
 void PQIVF::resolve_timers() {

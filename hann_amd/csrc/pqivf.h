@@ -198,15 +198,18 @@ struct PQIVF {
     int64_t fallback_count();
     void run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double *probe_rank, hipStream_t st,
                         Workspace &w, bool main_scan = true);
+    // the list-major scan's batch buffers for nq queries probing np lists
+    // with rerank depth selk: pairs, work items, sample-row slots, record
+    // room per query, and the arena's bytes
+    struct ListPlan {
+        int64_t pairs_max = 0, items_max = 0, sample_slots = 0, capq = 0;
+        size_t arena = 0;
+    };
+    ListPlan list_plan(int64_t nq, int np, int selk) const;
+    static bool list_plan_fits(const ListPlan &L, int64_t nq);
     // list-major MFMA scan; a.out_* are the candidate rows of the rerank
     void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
-                       Workspace &w, int64_t rows_max);
-    // entries in a batch's distance rows at most: nq times the np longest lists
-    int64_t list_rows_bound(int64_t nq, int np) const;
-    // regions (chunk halves) of a query's records in the list-major select at most
-    int list_select_regions(int np) const;
-    // the list-major scan's distance rows stay within 2^31 floats (8 GiB)
-    static constexpr int64_t kListRowsMax = (int64_t(1) << 31) - 64;   // 16-bit keys, below 2^32 - 64 bytes
+                       Workspace &w, const ListPlan &L);
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;

@@ -24,15 +24,15 @@ struct ListScanArgs {
     const int32_t *n_items;      // device count of items
     int32_t *item_counter;       // [n_parts] zeroed before the launch
     int n_parts;                 // item partitions (at most the grid)
-    uint16_t *dist;              // the sample pass's 16-bit keys in the batch's rows: query q's, its
-                                 // probes in order, each probe's run padded to a multiple of 8 slots
+    uint16_t *dist;              // the sample pass's 16-bit keys in the batch's sample rows: query q's
+                                 // sample probes in order, each probe's run padded to a multiple of 8 slots
     uint32_t dist_bytes;         // size of dist in bytes (below 2^32 - 64)
     // the filter pass
     const uint32_t *theta;       // [nq] keys <= theta[q] survive
-    uint32_t *rec;               // records (key << 16 | entry in chunk), compacted per (pair, chunk, half)
-                                 // region of the row slots
-    uint16_t *cnt;               // a region's record count at (row slot of its chunk start) / 4 + half
-    uint32_t cnt_bytes;          // size of cnt in bytes (below 2^32 - 64)
+    uint64_t *rec;               // [nq][capq] each query's records (key << 32 | entry position), in the
+                                 // order they were written; the first capq of them are kept
+    uint32_t *qcnt;              // [nq] records written per query (zeroed before the launch)
+    uint32_t capq;               // record room per query
 };
 
 // entries of a list per work item of the list-major scan
@@ -41,12 +41,12 @@ int list_scan_chunk();
 constexpr int kListScanParts = 256;
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 // per-list pair counts (lcnt) and sample pair counts (lscnt), per-query
-// entry totals and row slots (each probed list's length rounded up to 8),
-// and the sample's probes and row slots (its nearest probed lists: at least
-// two, and until they hold 4 k R entries); lcnt, lscnt zeroed by the caller
-void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot, int64_t *qslots,
+// entry totals, and the sample's probes and row slots (its nearest probed
+// lists: at least two, and until they hold 4 k R entries; each probed
+// list's run is its length rounded up to 8); lcnt, lscnt zeroed by the caller
+void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot,
                        int32_t *qsamp_np, int64_t *qsamp_slots, hipStream_t st);
-// row_off[q] = qslots[0] + ... + qslots[q - 1]
+// the sample rows' starts: row_off[q] = qslots[0] + ... + qslots[q - 1]
 void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hipStream_t st);
 // exclusive offsets of pairs and of work items per list (ceil(pairs / 32)
 // query groups times ceil(len / chunk) entry chunks); totals[0] = pairs,
@@ -55,8 +55,9 @@ void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hip
 void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
                          int32_t *lfill, int32_t *lbfill, int32_t *totals, hipStream_t st);
 // the pairs grouped by list (each list's sample pairs first), with the
-// coarse rank, the query's key scale and the row offset of each; ymax: bits
-// of an upper bound of |y_e| over the index
+// coarse rank, the query's key scale and, for a sample pair, where its keys
+// start in the sample rows (0 for the others); ymax: bits of an upper bound
+// of |y_e| over the index
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
                          const int32_t *lstart, const int32_t *lcnt, const int32_t *qsamp_np, const int64_t *row_off,
                          int32_t *lfill, int32_t *lbfill, int32_t *pair_q, float *pair_qc, float *pair_scale,
@@ -76,17 +77,17 @@ bool list_scan_supported(int d, int m, int P, int code_bytes);
 void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st);
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
 // per query, the bound theta of the filter pass: the (k R)-th smallest key of
-// its sample (the first qsamp_slots of its row)
+// its sample (qsamp_slots keys from row_off[q] of the sample rows)
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
                   const int64_t *qsamp_slots, uint32_t *theta, hipStream_t st);
 // per query, the selk = k_q * R best entries by (key, entry position) from the
-// filter pass's records, as entry positions (-1 after the last); the
-// brute-force fallbacks go to ovf. max_regions: at least twice the probed
-// list chunks of any query.
-size_t select_records_lds_bytes(int max_regions);
-void launch_select_records(const ScanArgs &a, int64_t nq, const uint32_t *rec, const uint16_t *cnt,
-                           const int64_t *row_off, const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt,
-                           int out_stride, int max_regions, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st);
+// filter pass's records, as entry positions (-1 after the last); queries
+// that take the brute-force fallback, and those with more than capq
+// records, go to ovf (the table scan then ranks them)
+size_t select_records_lds_bytes();
+void launch_select_records(const ScanArgs &a, int64_t nq, const uint64_t *rec, const uint32_t *qcnt, uint32_t capq,
+                           const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt, int out_stride, int32_t *ovf,
+                           int32_t *ovf_cnt, hipStream_t st);
 // probes per query the threshold and select kernels take at most
 int select_rows_max_probes();
 

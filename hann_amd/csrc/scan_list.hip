@@ -23,15 +23,16 @@
 // position). They are found without writing every key:
 //  1. the sample pass scores each query's nearest probed lists (at least
 //     two, and until they hold 4 selk entries) and writes their keys into
-//     the query's row, whose prefix they are;
+//     the query's sample row;
 //  2. k_theta takes, per query, theta = the selk-th smallest sample key --
 //     at least selk entries have keys <= theta, and the selk best do;
 //  3. the filter pass scores every entry again, with bit-identical keys, and
-//     writes only the (key, entry) records with key <= theta, each lane into
-//     its own (query, chunk, half) region of the row, four at a time;
+//     writes only the (key, entry position) records with key <= theta into
+//     the query's record room (about 1.5 selk of them);
 //  4. k_select_records picks the selk best records per query.
 // The selection is the same as over every key; those are reranked in exact
-// FP32.
+// FP32. A query whose records overflow its room (8 selk, at least 256) is
+// ranked by the table scan instead.
 #include "common.h"
 #include "scan_list.h"
 
@@ -73,9 +74,9 @@ __device__ __forceinline__ float key_clamp(float v) {
 __device__ __forceinline__ uint32_t key16x2(float a, float b) {
     return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(key_clamp(a), key_clamp(b)));
 }
-// the key of v in the upper 16 bits (the lower 16 are zero)
-__device__ __forceinline__ uint32_t key16_hi(float v) {
-    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(0.f, key_clamp(v)));
+// the key of v (in the lower 16 bits; the upper 16 are zero)
+__device__ __forceinline__ uint32_t key16(float v) {
+    return __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(key_clamp(v), 0.f));
 }
 // the largest signed bit pattern b with key(b) <= theta: key(b) <= theta
 // exactly when (int)b <= key_bound(theta). Below +inf, theta + 1 is a
@@ -109,18 +110,18 @@ __device__ __forceinline__ unsigned query_selk(const ScanArgs &a, int64_t q) {
     return (unsigned)(kq * a.rerank);
 }
 
-// one wave per query: entries and row slots of its probed lists, the pairs
-// of every list and of its sample, and the sample's probes (a prefix of the
-// probe order, so its keys are a prefix of the query's row) and row slots
+// one wave per query: entries of its probed lists, the pairs of every list
+// and of its sample, and the sample's probes (a prefix of the probe order)
+// and row slots
 __global__ void __launch_bounds__(64) k_pair_count(ScanArgs a, int32_t *__restrict__ lcnt,
                                                    int32_t *__restrict__ lscnt, int64_t *__restrict__ qtot,
-                                                   int64_t *__restrict__ qslots, int32_t *__restrict__ qsamp_np,
+                                                   int32_t *__restrict__ qsamp_np,
                                                    int64_t *__restrict__ qsamp_slots) {
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     const int npq = clamp_nprobe(a, q);
     const int64_t target = (int64_t)SAMP_MULT * query_selk(a, q);
-    int64_t tot = 0, slots = 0, ssl = 0;
+    int64_t tot = 0, ssl = 0;
     int snp = 0;
     for (int l0 = 0; l0 < npq; l0 += 64) {
         const int li = l0 + lane;
@@ -140,19 +141,14 @@ __global__ void __launch_bounds__(64) k_pair_count(ScanArgs a, int32_t *__restri
             atomicAdd(&lcnt[l], 1);
             if (smp) atomicAdd(&lscnt[l], 1);
         }
-        int64_t rs = row_slots(len), ss = smp ? rs : 0;
-        for (int o = 32; o > 0; o >>= 1) {
-            rs += __shfl_xor(rs, o);
-            ss += __shfl_xor(ss, o);
-        }
+        int64_t ss = smp ? row_slots(len) : 0;
+        for (int o = 32; o > 0; o >>= 1) ss += __shfl_xor(ss, o);
         snp += __popcll(__ballot(smp));
         tot += __shfl(incl, 63);
-        slots += rs;
         ssl += ss;
     }
     if (lane == 0) {
         qtot[q] = tot;
-        qslots[q] = slots;
         qsamp_np[q] = snp;
         qsamp_slots[q] = ssl;
     }
@@ -189,75 +185,102 @@ __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
 }
 
 // one workgroup of 1024: exclusive scans of lcnt and of the item counts,
-// in spans of 1024 * PO_PER lists (thread t sums lists t * PO_PER ..
-// + PO_PER - 1 of a span on its own; one block scan of the sums a span)
-constexpr int PO_PER = 16;
+// in spans of 16 rounds of 1024 lists (thread t holds list r * 1024 + t of
+// round r; all loads issued first): the waves scan all rounds at once, one
+// wave turns the waves' round totals into prefixes, and every list's offset
+// follows, with two block barriers a span
+constexpr int PO_ROUNDS = 16;   // lists up to 16,384 a span
 
 __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
                                                        const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
                                                        int32_t *__restrict__ lfill, int32_t *__restrict__ lbfill,
                                                        int32_t *__restrict__ totals) {
-    __shared__ int32_t s_a[16], s_b[16];
+    // [wave][round] wave totals, then their exclusive prefixes over the
+    // waves plus the rounds before (the span's carry included)
+    __shared__ int32_t s_a[16][PO_ROUNDS], s_b[16][PO_ROUNDS];
+    __shared__ int32_t s_ca, s_cb;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     int32_t carry_a = 0, carry_b = 0;
-    for (int g0 = 0; g0 < K; g0 += 1024 * PO_PER) {
-        const int base = g0 + t * PO_PER;
-        int32_t v[PO_PER], it[PO_PER], sa = 0, sb = 0;
+    for (int g0 = 0; g0 < K; g0 += 1024 * PO_ROUNDS) {
+        int32_t v[PO_ROUNDS], it[PO_ROUNDS], a[PO_ROUNDS], b[PO_ROUNDS];
 #pragma unroll
-        for (int r = 0; r < PO_PER; ++r) {
-            const int l = base + r;
+        for (int r = 0; r < PO_ROUNDS; ++r) {
+            const int l = g0 + r * 1024 + t;
             const int lc = l < K ? l : K - 1;
             const int32_t c = lcnt[lc];
             const int64_t n = list_len[lc];
             v[r] = l < K ? c : 0;
             it[r] = l < K ? list_items(c, n) : 0;
+            a[r] = v[r];
+            b[r] = it[r];
         }
-#pragma unroll
-        for (int r = 0; r < PO_PER; ++r) {
-            sa += v[r];
-            sb += it[r];
-        }
-        // block-wide exclusive scan of (sa, sb)
-        int32_t a = sa, b = sb;
+        // inclusive scans over the wave's lanes, every round
         for (int o = 1; o < 64; o <<= 1) {
-            const int32_t xa = __shfl_up(a, o), xb = __shfl_up(b, o);
-            if (lane >= o) {
-                a += xa;
-                b += xb;
+#pragma unroll
+            for (int r = 0; r < PO_ROUNDS; ++r) {
+                const int32_t xa = __shfl_up(a[r], o), xb = __shfl_up(b[r], o);
+                if (lane >= o) {
+                    a[r] += xa;
+                    b[r] += xb;
+                }
             }
         }
         if (lane == 63) {
-            s_a[w] = a;
-            s_b[w] = b;
-        }
-        __syncthreads();
-        int32_t wa = 0, wb = 0, ta = 0, tb = 0;
-        for (int u = 0; u < 16; ++u) {
-            const int32_t ua = s_a[u], ub = s_b[u];
-            if (u < w) {
-                wa += ua;
-                wb += ub;
-            }
-            ta += ua;
-            tb += ub;
-        }
-        __syncthreads();
-        int32_t ra = carry_a + wa + a - sa, rb = carry_b + wb + b - sb;
 #pragma unroll
-        for (int r = 0; r < PO_PER; ++r) {
-            const int l = base + r;
+            for (int r = 0; r < PO_ROUNDS; ++r) {
+                s_a[w][r] = a[r];
+                s_b[w][r] = b[r];
+            }
+        }
+        __syncthreads();
+        if (w == 0) {
+            // lane r < PO_ROUNDS: round r's prefixes over the waves, then
+            // the rounds' totals scanned over the lanes
+            int32_t ra = 0, rb = 0;
+            if (lane < PO_ROUNDS) {
+                for (int u = 0; u < 16; ++u) {
+                    const int32_t xa = s_a[u][lane], xb = s_b[u][lane];
+                    s_a[u][lane] = ra;
+                    s_b[u][lane] = rb;
+                    ra += xa;
+                    rb += xb;
+                }
+            }
+            int32_t ia = ra, ib = rb;
+            for (int o = 1; o < PO_ROUNDS; o <<= 1) {
+                const int32_t xa = __shfl_up(ia, o), xb = __shfl_up(ib, o);
+                if (lane >= o) {
+                    ia += xa;
+                    ib += xb;
+                }
+            }
+            if (lane < PO_ROUNDS) {
+                const int32_t ea = carry_a + ia - ra, eb = carry_b + ib - rb;
+                for (int u = 0; u < 16; ++u) {
+                    s_a[u][lane] += ea;
+                    s_b[u][lane] += eb;
+                }
+            }
+            if (lane == PO_ROUNDS - 1) {
+                s_ca = carry_a + ia;
+                s_cb = carry_b + ib;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < PO_ROUNDS; ++r) {
+            const int l = g0 + r * 1024 + t;
             if (l < K) {
-                lstart[l] = ra;
-                ioff[l] = rb;
+                lstart[l] = s_a[w][r] + a[r] - v[r];
+                ioff[l] = s_b[w][r] + b[r] - it[r];
                 lfill[l] = 0;
                 lbfill[l] = 0;
             }
-            ra += v[r];
-            rb += it[r];
         }
-        carry_a += ta;
-        carry_b += tb;
+        carry_a = s_ca;
+        carry_b = s_cb;
+        __syncthreads();   // s_a, s_b are rewritten by the next span
     }
     if (t == 0) {
         totals[0] = carry_a;
@@ -268,9 +291,9 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
 
 // one wave per query: its pairs (non-empty probed lists) into their lists'
 // pair ranges -- the sample's pairs from the front, the others from the
-// back -- each with its coarse rank, the query's key scale and where its
-// keys start in the query's row (the probes' slot counts summed in probe
-// order). The scale is the power of two 2^(14 - e) with
+// back -- each with its coarse rank, the query's key scale and, for a
+// sample pair, where its keys start in the query's sample row (the sample
+// probes' slot counts summed in probe order; 0 for the other pairs). The scale is the power of two 2^(14 - e) with
 // (sqrt(max coarse rank) + max|y|)^2 < 2^e: no probed entry's distance
 // (nor its BF16 estimate) reaches FP16's range once scaled, and every
 // scaled distance above 2^-28 of that bound keeps FP16's full precision.
@@ -324,7 +347,7 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
             pair_q[pos] = (int32_t)q;
             pair_qc[pos] = rank_of(li, l);
             pair_scale[pos] = scale;
-            pair_off[pos] = carry + incl - slots;
+            pair_off[pos] = li < snp ? carry + incl - slots : 0;
         }
         carry += __shfl(incl, 63);
     }
@@ -462,8 +485,9 @@ constexpr int LNW = 8;
 
 // what a launch of the list-major scan writes
 enum { KIND_SAMPLE = 1, KIND_FILTER = 2 };
-// the filter pass's ring of (record, row position) pairs per wave; it holds
-// fewer than 64 between slabs, and a register adds at most 64
+// the filter pass's ring of pending records per wave, (key << 32 | entry
+// position) and query; it holds fewer than 64 between slabs, and a
+// register adds at most 64
 constexpr int RING = 256;
 
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
@@ -482,8 +506,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     bf16x4 *sCBl = sCBh + (size_t)M * P;                          // [M][P] (split mode)
     float *s_term = reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) + wave * 32;
     // filter pass: the wave's ring of pending records
-    u32x2 *s_ring = reinterpret_cast<u32x2 *>(reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) +
-                                              LNW * 32) + (size_t)wave * RING;
+    uint64_t *s_ring = reinterpret_cast<uint64_t *>(reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) +
+                                                    LNW * 32) + (size_t)wave * RING;
+    uint32_t *s_ringq = reinterpret_cast<uint32_t *>(s_ring + (size_t)(LNW - wave) * RING) + (size_t)wave * RING;
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(la.codes);
 
     // the codebook, once per workgroup
@@ -576,16 +601,36 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     // the key rows as a buffer resource of dist_bytes bytes (raw, gfx9 layout)
     const __amdgpu_buffer_rsrc_t drs =
         __builtin_amdgcn_make_buffer_rsrc(la.dist, (short)0, (int)la.dist_bytes, 0x00020000);
-    // the filter pass's region counts as a buffer resource of cnt_bytes bytes
-    const __amdgpu_buffer_rsrc_t crs =
-        __builtin_amdgcn_make_buffer_rsrc(la.cnt, (short)0, (int)la.cnt_bytes, 0x00020000);
     // the ring's pending records are [head, fill) (wave-uniform counters)
     uint32_t fill = 0, head = 0;
-    // the whole wave writes n <= 64 pending records (lane i the i-th)
+    // the whole wave writes n <= 64 pending records (lane i the i-th), each
+    // at the next place of its query's record room (kept while in the room).
+    // Lanes are grouped by query first, so that one atomic a query reserves
+    // the places of all its records (device-wide atomics on one address
+    // are served one at a time).
     auto flush = [&](uint32_t n) {
         wave_sync();   // the ring's appends by other lanes
-        const u32x2 v = s_ring[(head + (uint32_t)lane) & (RING - 1)];
-        if ((uint32_t)lane < n) la.rec[v.y] = v.x;
+        const uint32_t slot = (head + (uint32_t)lane) & (RING - 1);
+        const uint64_t v = s_ring[slot];
+        const uint32_t vq = s_ringq[slot];
+        const bool has = (uint32_t)lane < n;
+        int lead = 0;
+        uint32_t rank = 0, gcnt = 0;
+        for (uint64_t todo = __builtin_amdgcn_ballot_w64(has); todo;) {
+            const int l0 = __builtin_ctzll(todo);
+            const uint32_t ql = (uint32_t)__builtin_amdgcn_readlane((int)vq, l0);
+            const uint64_t grp = __builtin_amdgcn_ballot_w64(vq == ql) & todo;
+            if (__builtin_amdgcn_inverse_ballot_w64(grp)) {
+                lead = l0;
+                rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u));
+            }
+            if (lane == l0) gcnt = (uint32_t)__builtin_popcountll(grp);
+            todo &= ~grp;
+        }
+        uint32_t p0 = 0;
+        if (has && lane == lead) p0 = atomicAdd(&la.qcnt[vq], gcnt);
+        const uint32_t p = (uint32_t)__shfl((int)p0, lead) + rank;
+        if (has && p < la.capq) la.rec[(uint64_t)vq * la.capq + p] = v;
         head = (uint32_t)__builtin_amdgcn_readfirstlane((int)(head + n));
         wave_sync();   // the slots are read before they are appended to again
     };
@@ -598,13 +643,10 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         const float sc = S.sc, qcs = S.qc * S.sc;
         const int off = S.off;
         // filter pass: values with key <= theta survive, i.e. those whose bits
-        // are <= tb as a signed integer, into this lane's region of the
-        // chunk's row slots (its half: entries with e % 8 in [4h, 4h + 4))
+        // are <= tb as a signed integer
         const int tb = (KIND == KIND_FILTER && qv) ? key_bound(la.theta[qid]) : 0;
         const uint64_t qmask = __builtin_amdgcn_ballot_w64(qv);
-        const int hcap = (slot_end - eb) >> 1;
-        const uint32_t rbase = (uint32_t)(off + eb + half * hcap);   // a multiple of 4
-        uint32_t dptr = rbase;   // this lane's next record
+        const int e0 = S.e0;
         // B operand of k-step jj: the query's dimensions 2 h M + 8 jj .. + 7
         // times -2 sc (a power of two, so the BF16 rounding is scaled
         // exactly); the accumulator starts at (t + |q - c|^2) sc, so it ends
@@ -639,8 +681,10 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // 8 (r / 4) + 4 h + r % 4 of the slab; CHECK: the slab runs past the
         // chunk's end
         auto survivors = [&](const f32x16 &acc, int base, auto check) {
-            // entry in chunk of register 0, with bits 0, 1, 3 and 4 free
+            // entry in chunk of register 0, with bits 0, 1, 3 and 4 free, and
+            // its entry position
             const uint32_t e0r = (uint32_t)(base - eb) + 4u * (uint32_t)half;
+            const uint32_t p0r = (uint32_t)(e0 + base) + 4u * (uint32_t)half;
 #pragma unroll
             for (int r = 0; r < 16; ++r) {
                 const uint32_t cr = 8u * (uint32_t)(r >> 2) + (uint32_t)(r & 3);
@@ -654,8 +698,8 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                     if (__builtin_amdgcn_inverse_ballot_w64(mk)) {
                         const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, fill));
-                        s_ring[pos & (RING - 1)] = u32x2{key16_hi(acc[r]) | e0r | cr, dptr};
-                        ++dptr;
+                        s_ring[pos & (RING - 1)] = ((uint64_t)key16(acc[r]) << 32) | (p0r + cr);
+                        s_ringq[pos & (RING - 1)] = (uint32_t)qid;
                     }
                     fill = (uint32_t)__builtin_amdgcn_readfirstlane((int)(fill + (uint32_t)__builtin_popcountll(mk)));
                 }
@@ -759,12 +803,6 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             load_at(s + 2, b1);
             slab(b2, s);
             if (++s == ns) break;
-        }
-        if (KIND == KIND_FILTER) {
-            // the region's record count at (chunk start) / 4 + half, one
-            // buffer store for every lane (lanes without a query discard it)
-            const uint32_t coff = qv ? 2u * ((uint32_t)(off + eb) / 4u + (uint32_t)half) : kNoStore;
-            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(dptr - rbase), crs, (int)coff, 0, 0);
         }
         item = next_item;
         S = SN;
@@ -896,27 +934,24 @@ constexpr int SELR_BINS = 2048;    // histogram over key >> 5
 // One workgroup of 256 per query: the sk = min(k_q * R, probed entries)
 // best entries of the query by (key, entry position), from the records the
 // filter pass kept (every entry with key <= theta[q]; the sk best are among
-// them). A 2,048-bin histogram over key >> 5 gives the bin B where the count
-// reaches sk; records below B are selected, and those in B are ranked by
-// (key, entry position) -- by counting for up to SELR_RANK of them, else
-// by a sort, or for more than SELR_ROOM by the exact boundary key and a
-// radix select over positions. Output: entry positions for the rerank.
-// Queries that take the brute-force fallback go to ovf.
-__global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, const uint32_t *__restrict__ rec,
-                                                        const uint16_t *__restrict__ cnt,
-                                                        const int64_t *__restrict__ row_off,
+// them), held contiguously in the query's record room. A 2,048-bin
+// histogram over key >> 5 gives the bin B where the count reaches sk;
+// records below B are selected, and those in B are ranked by (key, entry
+// position) -- by counting for up to SELR_RANK of them, else by a sort, or
+// for more than SELR_ROOM by the exact boundary key and a radix select over
+// positions. Output: entry positions for the rerank. Queries that take the
+// brute-force fallback, and those whose records overflowed their room, go
+// to ovf.
+__global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, const uint64_t *__restrict__ rec,
+                                                        const uint32_t *__restrict__ qcnt, uint32_t capq,
                                                         const int64_t *__restrict__ qtot,
                                                         int64_t *__restrict__ out_pos, int32_t *__restrict__ out_cnt,
-                                                        int out_stride, int max_regions, int32_t *__restrict__ ovf,
+                                                        int out_stride, int32_t *__restrict__ ovf,
                                                         int32_t *__restrict__ ovf_cnt) {
-    extern __shared__ __align__(16) unsigned char s_raw[];
-    uint64_t *cache = reinterpret_cast<uint64_t *>(s_raw);              // [SELR_CACHE] (key << 32 | entry)
-    uint64_t *tie = cache + SELR_CACHE;                                  // [SELR_ROOM] the boundary bin's records
-    int64_t *r_ent = reinterpret_cast<int64_t *>(tie + SELR_ROOM);      // [R] entry position of the chunk start
-    unsigned *hist = reinterpret_cast<unsigned *>(r_ent + max_regions);  // [SELR_BINS]
-    uint32_t *r_pref = hist + SELR_BINS;                                 // [R + 1] record prefix
-    uint32_t *r_start = r_pref + max_regions + 1;                        // [R] region start in the row
-    __shared__ int s_bin, s_out, s_fill, s_nreg;
+    __shared__ uint64_t cache[SELR_CACHE];   // (key << 32 | entry position)
+    __shared__ uint64_t tie[SELR_ROOM];      // the boundary bin's records
+    __shared__ unsigned hist[SELR_BINS];
+    __shared__ int s_bin, s_out, s_fill;
     __shared__ unsigned s_below, s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
@@ -925,7 +960,8 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
     const int selk = kq * a.rerank;
     const int64_t Lr = qtot[q];
     int64_t *orow = out_pos + q * out_stride;
-    if (a.allow_fallback && Lr < kq) {
+    const uint32_t T = qcnt[q];
+    if ((a.allow_fallback && Lr < kq) || T > capq) {
         if (tid == 0) ovf[atomicAdd(ovf_cnt, 1)] = (int32_t)q;
         return;
     }
@@ -933,94 +969,14 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
     for (int t = sk + tid; t < out_stride; t += 256) orow[t] = -1;
     if (tid == 0) out_cnt[q] = sk;
     if (sk == 0) return;
-    const int npq = clamp_nprobe(a, q);
-    const int64_t rq = row_off[q];
-    // the regions: two halves of every chunk of every probed list, in probe
-    // order (wave 0: a lane per probe, rounds of 64)
-    if (wave == 0) {
-        int64_t cr = 0;
-        int creg = 0;
-        for (int l0 = 0; l0 < npq; l0 += 64) {
-            const int li = l0 + lane;
-            int64_t len = 0, first = 0;
-            if (li < npq) {
-                const int64_t l = a.probes[q * a.probe_stride + li];
-                len = a.list_len[l];
-                first = a.list_start[l];
-            }
-            const int64_t rs = row_slots(len);
-            const int nr = len > 0 ? 2 * (int)((len + LCH - 1) / LCH) : 0;
-            int64_t ir = rs;
-            int inr = nr;
-            for (int o = 1; o < 64; o <<= 1) {
-                const int64_t vr = __shfl_up(ir, o);
-                const int vn = __shfl_up(inr, o);
-                if (lane >= o) {
-                    ir += vr;
-                    inr += vn;
-                }
-            }
-            const int64_t poff = cr + ir - rs;
-            int r = creg + inr - nr;
-            for (int c = 0; 2 * c < nr; ++c, r += 2) {
-                const int64_t cs = poff + (int64_t)c * LCH;
-                const int64_t slots = (2 * c + 2 == nr) ? rs - (int64_t)c * LCH : LCH;
-                const uint32_t c0 = cnt[(uint64_t)(rq + cs) / 4], c1 = cnt[(uint64_t)(rq + cs) / 4 + 1];
-                if (r + 1 < max_regions) {
-                    r_ent[r] = r_ent[r + 1] = first + (int64_t)c * LCH;
-                    r_start[r] = (uint32_t)cs;
-                    r_start[r + 1] = (uint32_t)(cs + (slots >> 1));
-                    r_pref[r] = c0;
-                    r_pref[r + 1] = c1;
-                }
-            }
-            cr += __shfl(ir, 63);
-            creg += __shfl(inr, 63);
-        }
-        if (lane == 0) s_nreg = creg < max_regions ? creg : max_regions;
-    }
     for (int b = tid; b < SELR_BINS; b += 256) hist[b] = 0;
-    __syncthreads();
-    const int R = s_nreg;
-    // exclusive prefix of the region counts (wave 0: consecutive runs of
-    // regions a lane, then a scan over the lanes)
-    if (wave == 0) {
-        const int per = (R + 63) / 64, r0 = lane * per, r1 = r0 + per < R ? r0 + per : R;
-        uint32_t own = 0;
-        for (int r = r0; r < r1; ++r) own += r_pref[r];
-        uint32_t incl = own;
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t v = __shfl_up(incl, o);
-            if (lane >= o) incl += v;
-        }
-        uint32_t run = incl - own;
-        for (int r = r0; r < r1; ++r) {
-            const uint32_t c = r_pref[r];
-            r_pref[r] = run;
-            run += c;
-        }
-        if (lane == 63) r_pref[R] = incl;
-        if (lane == 0) {
-            s_out = 0;
-            s_fill = 0;
-        }
+    if (tid == 0) {
+        s_out = 0;
+        s_fill = 0;
     }
     __syncthreads();
-    const uint32_t T = r_pref[R];
-    const uint32_t *rrow = rec + rq;
-    // record t as (key << 32 | entry position): its region by binary search
-    auto rec_at = [&](uint32_t t) -> uint64_t {
-        int lo = 0, hi = R - 1;
-        while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (r_pref[mid] <= t)
-                lo = mid;
-            else
-                hi = mid - 1;
-        }
-        const uint32_t v = rrow[r_start[lo] + (t - r_pref[lo])];
-        return ((uint64_t)(v >> 16) << 32) | (uint64_t)(r_ent[lo] + (v & 0xFFFFu));
-    };
+    const uint64_t *rrow = rec + (uint64_t)q * capq;
+    auto rec_at = [&](uint32_t t) -> uint64_t { return rrow[t]; };
     // pass 0: the records (eight loads in flight a thread) into the cache
     // and the histogram
     for (uint32_t t0 = tid; t0 < T; t0 += 256 * QB) {
@@ -1154,10 +1110,10 @@ void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st) {
     HANN_LAUNCH_CHECK();
 }
 
-void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot, int64_t *qslots,
+void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot,
                        int32_t *qsamp_np, int64_t *qsamp_slots, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_pair_count, dim3((unsigned)nq), dim3(64), 0, st, a, lcnt, lscnt, qtot, qslots, qsamp_np,
+    hipLaunchKernelGGL(dev::k_pair_count, dim3((unsigned)nq), dim3(64), 0, st, a, lcnt, lscnt, qtot, qsamp_np,
                        qsamp_slots);
     HANN_LAUNCH_CHECK();
 }
@@ -1248,7 +1204,7 @@ void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, h
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind) {
     const int nb = x3 ? 2 : 1;
     size_t b = (size_t)nb * m * P * 8 + (size_t)dev::LNW * 32 * 4;
-    if (kind == dev::KIND_FILTER) b += (size_t)dev::LNW * dev::RING * 8;
+    if (kind == dev::KIND_FILTER) b += (size_t)dev::LNW * dev::RING * 12;
     return b;
 }
 
@@ -1260,21 +1216,16 @@ void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int
     HANN_LAUNCH_CHECK();
 }
 
-size_t select_records_lds_bytes(int max_regions) {
-    return (size_t)(dev::SELR_CACHE + dev::SELR_ROOM) * 8 + (size_t)max_regions * 8 + (size_t)dev::SELR_BINS * 4 +
-           (size_t)(2 * max_regions + 1) * 4;
+size_t select_records_lds_bytes() {
+    return (size_t)(dev::SELR_CACHE + dev::SELR_ROOM) * 8 + (size_t)dev::SELR_BINS * 4 + 64;
 }
 
-void launch_select_records(const ScanArgs &a, int64_t nq, const uint32_t *rec, const uint16_t *cnt,
-                           const int64_t *row_off, const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt,
-                           int out_stride, int max_regions, int32_t *ovf, int32_t *ovf_cnt, hipStream_t st) {
+void launch_select_records(const ScanArgs &a, int64_t nq, const uint64_t *rec, const uint32_t *qcnt, uint32_t capq,
+                           const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt, int out_stride, int32_t *ovf,
+                           int32_t *ovf_cnt, hipStream_t st) {
     if (nq == 0) return;
-    const size_t shm = select_records_lds_bytes(max_regions);
-    if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, "list-major select: too many probed list chunks per query");
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_select_records, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
-    hipLaunchKernelGGL(dev::k_select_records, dim3((unsigned)nq), dim3(256), shm, st, a, nq, rec, cnt, row_off, qtot,
-                       out_pos, out_cnt, out_stride, max_regions, ovf, ovf_cnt);
+    hipLaunchKernelGGL(dev::k_select_records, dim3((unsigned)nq), dim3(256), 0, st, a, nq, rec, qcnt, capq, qtot,
+                       out_pos, out_cnt, out_stride, ovf, ovf_cnt);
     HANN_LAUNCH_CHECK();
 }
 
