@@ -65,7 +65,8 @@ class ServerOptions(ctypes.Structure):
 
 class ServerCounters(ctypes.Structure):
     _fields_ = [("completed_queries", i64), ("completed_requests", i64), ("batches", i64), ("mutations", i64),
-                ("queue_depth", i64), ("in_flight", i64), ("budget_in_use", i64), ("budget_peak", i64)]
+                ("queue_depth", i64), ("in_flight", i64), ("budget_in_use", i64), ("budget_peak", i64),
+                ("held_peak", i64), ("scratch_over_bound", i64)]
 
 
 class IndexStats(ctypes.Structure):

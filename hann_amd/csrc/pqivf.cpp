@@ -1039,17 +1039,37 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
     const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
     PQIVF::ListPlan lplan{};
-    if (list_scan && (R == 0 || !have_codebooks || any_uncoded || probe_stride == 0 || (int64_t)d_ids.n >= INT32_MAX ||
-                      reinterpret_cast<uintptr_t>(dq) % 16 != 0 || np > select_rows_max_probes() ||
-                      !list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
-                      list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2) > 160 * 1024 ||
-                      select_records_lds_bytes() > 160 * 1024 ||
-                      !list_plan_fits(lplan = list_plan(nq, np, R > 0 ? kmax * R : kmax), nq))) {
+    refresh_shape();
+    if (list_scan) {
         // the list-major scan needs a rerank, codes for every entry, a probe
-        // row per query, a supported shape and room for the batch's rows and
-        // a query's probed list chunks; otherwise the persistent table scan runs
-        list_scan = false;
-        mode = SCAN_MODE_ADC_F32;
+        // row per query, a supported shape and room for the batch's
+        // buffers; otherwise the persistent table scan runs, and the first
+        // time a reason holds for this index it is said on stderr
+        const char *why = nullptr;
+        if (R == 0)
+            why = "it needs a rerank depth R > 0";
+        else if (!have_codebooks || any_uncoded)
+            why = "it needs PQ codes for every entry";
+        else if (probe_stride == 0)
+            why = "every query probes every list";
+        else if ((int64_t)d_ids.n >= INT32_MAX || np > select_rows_max_probes())
+            why = "the index or nprobe is too large for it";
+        else if (reinterpret_cast<uintptr_t>(dq) % 16 != 0)
+            why = "the queries are not 16-byte aligned";
+        else if (!list_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) ||
+                 list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2) > 160 * 1024 ||
+                 select_records_lds_bytes() > 160 * 1024)
+            why = "it covers 4-dimensional subvectors (d = 4 m, m in {8,16,24,32,48,64}) with uint8 codes";
+        else if (!list_plan_fits(lplan = list_plan(nq, np, R > 0 ? kmax * R : kmax), nq))
+            why = "the batch's buffers exceed its 32-bit addressing";
+        if (why) {
+            list_scan = false;
+            mode = SCAN_MODE_ADC_F32;
+            std::lock_guard<std::mutex> g(note_mu);
+            if (list_notes.insert(why).second)
+                std::fprintf(stderr, "hann_amd: the list-major scan was asked for but the table scan ran: %s (d %lld, m %lld)\n",
+                             why, (long long)dim, (long long)m);
+        }
     }
     if (list_scan) mode = SCAN_MODE_ADC_F32;   // the fallback scans
     if (!have_codebooks) {
@@ -1205,16 +1225,26 @@ struct ListArena {
     }
 };
 
-PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk) const {
-    ListPlan L;
-    // work items: a list with p pairs and n entries has ceil(p / 32) *
-    // ceil(n / chunk) of them, at most (p / 32 + 1) * ceil(n / chunk)
+void PQIVF::refresh_shape() const {
+    if (shape_epoch.load(std::memory_order_acquire) == mirror_epoch) return;
     const int64_t chunk = list_scan_chunk();
     int64_t max_len = 0, chunks_all = 0;
     for (int64_t v : dev_len) {
         max_len = std::max(max_len, v);
         chunks_all += ceil_div(v, chunk);
     }
+    shape_max_len.store(max_len, std::memory_order_relaxed);
+    shape_chunks.store(chunks_all, std::memory_order_relaxed);
+    shape_epoch.store(mirror_epoch, std::memory_order_release);
+}
+
+PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk) const {
+    ListPlan L;
+    // work items: a list with p pairs and n entries has ceil(p / 32) *
+    // ceil(n / chunk) of them, at most (p / 32 + 1) * ceil(n / chunk)
+    const int64_t chunk = list_scan_chunk();
+    const int64_t max_len = shape_max_len.load(std::memory_order_relaxed);
+    const int64_t chunks_all = shape_chunks.load(std::memory_order_relaxed);
     L.pairs_max = nq * (int64_t)np;
     L.items_max = ceil_div(L.pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
     // a query's sample: at least two probes, then on while fewer than
@@ -1343,6 +1373,41 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     // fallbacks are rare: one workgroup per CU drains the queue
     const int fgrid = (int)std::min<int64_t>(nq, (int64_t)num_cus);
     launch_scan_block(fa, f, nq, wv, kp, code_bytes, maxp, fgrid, st);
+}
+
+int64_t PQIVF::search_scratch_bytes(int64_t nq, int kmax, int pmax_probe, const hann_search_params &p) const {
+    if (nq <= 0) return 0;
+    const int64_t K = (int64_t)lists.size();
+    const int np = (int)std::max<int64_t>(1, std::min<int64_t>(pmax_probe, K));
+    const int R = p.rerank > 0 ? p.rerank : 0;
+    const int selk = R > 0 ? kmax * R : kmax;
+    int64_t b = 0;
+    auto add = [&](int64_t bytes) { b += bytes + 256; };   // the workspace's slack a slot
+    add(4);   // device_min
+    if (np < K) {
+        add(nq * np * 8);   // probes
+        add(nq * np * 8);   // probe ranks
+        if (p.coarse_precision == HANN_PREC_BF16)
+            b += (int64_t)certified_topk_scratch_bytes(nq, kc, (int)dim, np, true);
+        else
+            b += (int64_t)topk_exact_scratch_bytes(HANN_METRIC_SQEUCLIDEAN, nq, kc, np);
+    }
+    if (R > 0) {
+        add(nq * selk * 4);   // candidate keys
+        add(nq * selk * 8);   // candidate positions
+        add(nq * 4);
+    }
+    // the table scan (also the list-major scan's fallback)
+    add(nq * 4);
+    add(4);
+    if (nq * m * pmax * 4 <= (int64_t(512) << 20)) add(nq * m * pmax * 4);
+    if (p.scan_mode == SCAN_MODE_LIST_BF16 || p.scan_mode == SCAN_MODE_LIST_BF16X3) {
+        const ListPlan L = list_plan(nq, np, selk);
+        add((int64_t)L.arena);
+        add(nq * L.capq * 8);
+        add(L.sample_slots * 2);
+    }
+    return b;
 }
 
 bool PQIVF::list_plan_fits(const ListPlan &L, int64_t nq) {

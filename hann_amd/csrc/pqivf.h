@@ -4,6 +4,8 @@
 #include <array>
 #include <atomic>
 #include <map>
+#include <mutex>
+#include <set>
 #include <shared_mutex>
 #include <string>
 #include <unordered_map>
@@ -206,6 +208,20 @@ struct PQIVF {
         size_t arena = 0;
     };
     ListPlan list_plan(int64_t nq, int np, int selk) const;
+    // the device lists' longest length and chunk count, refreshed from the
+    // mirror when it changed (callers hold the index lock); readable without
+    // it, then as of the last refresh
+    mutable std::atomic<uint64_t> shape_epoch{~0ull};
+    mutable std::atomic<int64_t> shape_max_len{0}, shape_chunks{0};
+    void refresh_shape() const;
+    // reasons the list-major scan was declined, each said once on stderr
+    std::mutex note_mu;
+    std::set<std::string> list_notes;
+    // an upper bound of the device workspace bytes search_device takes for
+    // nq queries, k up to kmax and nprobe up to pmax_probe (every slot of a
+    // grow-only workspace counted at the size this shape asks of it; list
+    // lengths as of the last refresh_shape)
+    int64_t search_scratch_bytes(int64_t nq, int kmax, int pmax_probe, const hann_search_params &p) const;
     static bool list_plan_fits(const ListPlan &L, int64_t nq);
     // list-major MFMA scan; a.out_* are the candidate rows of the rerank
     void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,

@@ -68,8 +68,14 @@ struct Slot {
     std::vector<Request *> reqs;
     int64_t nq = 0;
     int kmax = 0;
-    int64_t bytes = 0;
+    int64_t bytes = 0;       // the batch in flight's charge
+    int64_t max_bound = 0;   // the largest scratch bound charged on this slot
     bool busy = false;
+    // device bytes the slot holds: its workspace (grow-only) and buffers
+    int64_t held() const {
+        return (int64_t)ws.bytes() + (int64_t)(d_q.n * sizeof(float) + d_key.n * sizeof(float) +
+                                               (d_k.n + d_np.n + d_cnt.n) * sizeof(int32_t) + d_id.n * sizeof(int64_t));
+    }
 };
 
 struct Server {
@@ -83,16 +89,22 @@ struct Server {
     uint64_t next_ticket = 1;
     bool stop = false;
     int64_t budget_in_use = 0, budget_peak = 0;
+    int64_t held_peak = 0;            // device bytes all slots held, at most
+    int64_t scratch_over_bound = 0;   // workspace bytes above the charged bound (0 when the bound holds)
     int64_t completed_queries = 0, completed_requests = 0, batches = 0, mutations = 0;
     int in_flight = 0;
     std::thread worker, completer;
     std::deque<Slot *> launched;   // in launch order
     std::string fatal;
 
+    // a batch's charge against the budget: its query, parameter and result
+    // buffers, and the search's workspace bound for its shape (list lengths
+    // as of the last refresh of the index's shape)
     int64_t batch_bytes(int64_t nq, int kmax, int pmax) const {
-        const int64_t d = ix->dim;
-        const int R = opt.params.rerank > 0 ? opt.params.rerank : 1;
-        return nq * (d * 4 + 8 + (int64_t)kmax * 12 + (int64_t)kmax * R * 12 + (int64_t)pmax * 16 + 16) + (1 << 20);
+        return staging_bytes(nq, kmax) + ix->search_scratch_bytes(nq, kmax, pmax, opt.params);
+    }
+    int64_t staging_bytes(int64_t nq, int kmax) const {
+        return nq * (ix->dim * 4 + 12 + (int64_t)kmax * 12) + 4 * 256;
     }
 
     void ensure_slot_capacity(Slot &s, int64_t nq, int kmax) {
@@ -136,6 +148,7 @@ struct Server {
         s.nq = nq;
         s.kmax = kmax;
         s.bytes = batch_bytes(nq, kmax, pmax);
+        s.max_bound = std::max(s.max_bound, s.bytes - staging_bytes(nq, kmax));
         s.busy = true;
         budget_in_use += s.bytes;
         budget_peak = std::max(budget_peak, budget_in_use);
@@ -192,6 +205,9 @@ struct Server {
                 ix->bulk_add(r->ids.data(), r->vecs.data(), nullptr, r->ids.size());
             else
                 ix->bulk_delete(r->ids.data(), r->ids.size());
+            // later batches are charged for the lists as they now are
+            ix->sync_device();
+            ix->refresh_shape();
         } catch (const Error &e) {
             r->status = e.code;
             r->error = e.what();
@@ -292,6 +308,10 @@ struct Server {
             s->reqs.clear();
             s->busy = false;
             budget_in_use -= s->bytes;
+            scratch_over_bound = std::max(scratch_over_bound, (int64_t)s->ws.bytes() - s->max_bound);
+            int64_t held = 0;
+            for (auto &sl : slots) held += sl->held();
+            held_peak = std::max(held_peak, held);
             --in_flight;
             cv_slot.notify_all();
             cv_done.notify_all();
@@ -337,6 +357,7 @@ int hann_server_create(hann_pqivf *idx, const hann_server_options *opts, hann_se
         if (!s.ix->trained) fail(HANN_ERR_NOT_TRAINED, "PQIVF index is not trained. Call Train() after adding data before searching");
         s.ix->ensure_stream();
         s.ix->sync_device();
+        s.ix->refresh_shape();
     }
     for (int i = 0; i < s.opt.slots; ++i) {
         auto sl = std::make_unique<Slot>();
@@ -475,6 +496,8 @@ int hann_server_stats(hann_server *srv, hann_server_counters *out) {
     out->in_flight = s.in_flight;
     out->budget_in_use = s.budget_in_use;
     out->budget_peak = s.budget_peak;
+    out->held_peak = s.held_peak;
+    out->scratch_over_bound = s.scratch_over_bound;
     HANN_API_END
 }
 

@@ -353,6 +353,10 @@ typedef struct {
 typedef struct {
     int64_t completed_queries, completed_requests, batches, mutations;
     int64_t queue_depth, in_flight, budget_in_use, budget_peak;
+    /* device bytes the slots' workspaces and buffers held at most (they are
+     * grow-only), and how far a slot's workspace ever exceeded the largest
+     * bound charged on it (0: every charge bounded the memory it used) */
+    int64_t held_peak, scratch_over_bound;
 } hann_server_counters;
 
 void hann_server_default_options(hann_server_options *o);

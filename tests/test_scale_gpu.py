@@ -268,6 +268,9 @@ def test_c5_serving_alone_equals_mixed_with_mutations(hann, c4):
                 assert not np.isin(res[("probe_out", r)][0][:, 0], adds[r][0]).any()
     assert st["mutations"] == 2 * sum(1 for p in plan if p[0] in ("add", "del"))
     assert st["batches"] < 2 * n_req
+    # every batch's charge bounded the device memory its slot's search took
+    assert st["scratch_over_bound"] == 0, st
+    assert 0 < st["budget_peak"] <= 8 << 30, st
 
 
 # ------------------------------------------------------------------ C2 --

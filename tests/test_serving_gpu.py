@@ -20,7 +20,7 @@ def index(hann):
     return idx, x
 
 
-@pytest.mark.parametrize("mode,rerank", [(2, 0), (1, 2), (0, 0)])
+@pytest.mark.parametrize("mode,rerank", [(2, 0), (1, 2), (0, 0), (3, 2), (4, 2)])
 def test_alone_equals_mixed(hann, index, mode, rerank):
     from hann_amd import serving, synthetic
     idx, x = index
@@ -53,6 +53,7 @@ def test_alone_equals_mixed(hann, index, mode, rerank):
         assert np.array_equal(a[0], b[0]), info
         assert np.array_equal(a[1], b[1]), info
     assert st["batches"] < 2 * n_req   # the mixed phase did batch requests together
+    assert st["scratch_over_bound"] == 0, st   # each charge bounded its slot's device memory
     # and both equal the direct batched search
     ids, dist, cnt = idx.search_batch(qs[:off[n_req]], ks[:off[n_req]], nprobe=nps[:off[n_req]], scan_mode=mode,
                                       rerank=rerank, fused=1 if mode else 0)
