@@ -133,6 +133,7 @@ SIGNATURES = {
     "hann_pqivf_profile": (cint, [vp, cint]),
     "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),
     "hann_pqivf_scan_kernel_time": (cint, [vp, c_double_p]),
+    "hann_pqivf_scan_kernel_times": (cint, [vp, c_double_p, c_double_p]),
     "hann_pqivf_last_scan": (cint, [vp, c_int32_p, c_int32_p]),
     "hann_pqivf_export_lists": (cint, [vp, c_int64_p, c_int64_p, c_int32_p, c_float_p]),
     "hann_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),

@@ -954,14 +954,13 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     // (pqivf/index.go:184-206). When every query probes every list no
     // selection is needed; otherwise the per-query prefix of the sorted
     // probe list is each query's set.
-    std::array<hipEvent_t, 6> ev{};
+    std::array<hipEvent_t, 8> ev{};
     if (profile) {
         for (auto &e : ev) HANN_HIP(hipEventCreate(&e));
         HANN_HIP(hipEventRecord(ev[0], st));
-        // [4], [5] start out recorded so that a search without a scan
-        // kernel measures zero
-        HANN_HIP(hipEventRecord(ev[4], st));
-        HANN_HIP(hipEventRecord(ev[5], st));
+        // [4] .. [7] start out recorded so that a search without a scan
+        // kernel (or without a sample pass) measures zero
+        for (int i = 4; i < 8; ++i) HANN_HIP(hipEventRecord(ev[i], st));
         kmark = &ev[4];
     }
     struct KmarkReset {
@@ -1344,7 +1343,9 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         ListScanArgs ls = la;
         ls.items = items_s;
         ls.n_items = totals + 2;
+        mark_kernel(2, st);
         launch_scan_list_kind(ls, x3, 1, grid, st);
+        mark_kernel(3, st);
     }
     launch_theta(a, nq, srow, soff, qtot, qsamp, theta, st);
     // 3. the filter pass: every (query, entry) key, and the records of those
@@ -1424,12 +1425,14 @@ void PQIVF::resolve_timers() {
         HANN_HIP(hipEventElapsedTime(&a, ev[0], ev[1]));
         HANN_HIP(hipEventElapsedTime(&b, ev[1], ev[2]));
         HANN_HIP(hipEventElapsedTime(&c, ev[2], ev[3]));
-        float kt = 0;
+        float kt = 0, ks = 0;
         HANN_HIP(hipEventElapsedTime(&kt, ev[4], ev[5]));
+        HANN_HIP(hipEventElapsedTime(&ks, ev[6], ev[7]));
         t_coarse_ms += a;
         t_scan_ms += b;
         t_rerank_ms += c;
         t_kernel_ms += kt;
+        t_sample_ms += ks;
         ++t_calls;
         for (auto &e : ev) (void)hipEventDestroy(e);
     }
@@ -2134,6 +2137,17 @@ int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms) {
     x.resolve_timers();
     *kernel_ms = x.t_kernel_ms;
     x.t_kernel_ms = 0;
+    HANN_API_END
+}
+
+int hann_pqivf_scan_kernel_times(hann_pqivf *idx, double *filter_ms, double *sample_ms) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    x.resolve_timers();
+    *filter_ms = x.t_kernel_ms;
+    *sample_ms = x.t_sample_ms;
+    x.t_kernel_ms = x.t_sample_ms = 0;
     HANN_API_END
 }
 

@@ -231,10 +231,11 @@ struct PQIVF {
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;
     int64_t t_calls = 0;
     // per search: stage boundaries [0..3]; [4], [5] bracket the main scan
-    // kernel's launch (k_scan_list or k_scan_block or k_scan) when recorded
-    std::vector<std::array<hipEvent_t, 6>> t_events;   // pending, resolved by stage_times()
-    hipEvent_t *kmark = nullptr;   // the current search's [4], [5] while it launches
-    double t_kernel_ms = 0;
+    // kernel's launch (k_scan_list's filter pass, k_scan_block or k_scan)
+    // and [6], [7] the list-major scan's sample pass, when recorded
+    std::vector<std::array<hipEvent_t, 8>> t_events;   // pending, resolved by stage_times()
+    hipEvent_t *kmark = nullptr;   // the current search's [4] .. [7] while it launches
+    double t_kernel_ms = 0, t_sample_ms = 0;
     int64_t t_kernel_calls = 0;
     void mark_kernel(int which, hipStream_t st) {
         if (kmark) HANN_HIP(hipEventRecord(kmark[which], st));

@@ -280,6 +280,14 @@ class Index:
         _lib.check(self._lib.hann_pqivf_scan_kernel_time(self._h, ctypes.byref(v)))
         return v.value
 
+    def scan_kernel_times(self):
+        """(filter_ms, sample_ms): milliseconds in the main scan kernel (the
+        list-major scan's filter pass) and in the list-major sample pass over
+        the profiled searches (call after stage_times)."""
+        f, s = ctypes.c_double(), ctypes.c_double()
+        _lib.check(self._lib.hann_pqivf_scan_kernel_times(self._h, ctypes.byref(f), ctypes.byref(s)))
+        return f.value, s.value
+
     def export_lists(self, with_vectors=True):
         """(list_off [k+1], ids [N], codes [N][m], vecs [N][dim] or None)"""
         k, m, _, d = self.shape()

@@ -320,6 +320,10 @@ int hann_pqivf_stage_times(hann_pqivf *idx, double *coarse_ms, double *scan_ms, 
  * events on the search stream; resets the total. Call after
  * hann_pqivf_stage_times, which resolves the events (build extension). */
 int hann_pqivf_scan_kernel_time(hann_pqivf *idx, double *kernel_ms);
+/* The same main-kernel total (the list-major scan's filter pass) and the
+ * list-major scan's sample pass, from HIP events on the search stream;
+ * resets both totals (build extension). */
+int hann_pqivf_scan_kernel_times(hann_pqivf *idx, double *filter_ms, double *sample_ms);
 /* The scan the last search on this index ran (build extension): a
  * HANN_SCAN_* mode (the list modes only when the list-major scan ran; it
  * falls back to the FP32 table scan for shapes it does not cover) and
