@@ -41,10 +41,19 @@ struct ScanArgs {
     int32_t *out_cnt;            // [nq]
     int32_t *fell_back;          // [nq] or null
     unsigned long long *fb_counter;  // searches that fell back (device counter) or null
+    // null, or k_scan's block b ranks query q_order[order_base + b] when
+    // order_base + b < *q_order_n (and returns otherwise)
+    const int32_t *q_order;
+    const int32_t *q_order_n;
+    int order_base;
 };
 
 size_t scan_lds_bytes(int mode, int d, int m, int pmax_cb, int cap);
 void launch_scan(int mode, int code_bytes, const ScanArgs &a, int64_t nq, hipStream_t st);
+// k_scan over the queries listed in order[0 .. *order_n) (at most nq_max of
+// them; the count is read on the device)
+void launch_scan_ordered(int mode, int code_bytes, const ScanArgs &a, int64_t nq_max, const int32_t *order,
+                         const int32_t *order_n, hipStream_t st);
 // k_q values are clamped to [1, kmax]
 void launch_rerank(const float *queries, int d, int64_t nq, const int64_t *cand_pos, const int32_t *cand_cnt,
                    int cand_stride, const int32_t *k_q, int k_default, int kmax, const float *vecs,

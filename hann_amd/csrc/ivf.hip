@@ -24,7 +24,12 @@ template <int MODE, typename CodeT>
 __global__ void __launch_bounds__(64) k_scan(ScanArgs a) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     const int lane = threadIdx.x;
-    const int64_t q = blockIdx.x;
+    int64_t q = blockIdx.x;
+    if (a.q_order) {
+        const int b = a.order_base + (int)blockIdx.x;
+        if (b >= *a.q_order_n) return;
+        q = a.q_order[b];
+    }
     const int d = a.d, m = a.m, s = a.s, P = a.pmax_cb;
     float *s_q = reinterpret_cast<float *>(s_raw);
     float *s_c = s_q + d;
@@ -230,6 +235,39 @@ void launch_scan(int mode, int code_bytes, const ScanArgs &a, int64_t nq, hipStr
         else HANN_SCAN_LAUNCH(SCAN_MODE_ADC_F16, uint16_t);
     }
 #undef HANN_SCAN_LAUNCH
+}
+
+void launch_scan_ordered(int mode, int code_bytes, const ScanArgs &a, int64_t nq_max, const int32_t *order,
+                         const int32_t *order_n, hipStream_t st) {
+    if (nq_max == 0) return;
+    const size_t shm = scan_lds_bytes(mode, a.d, a.m, a.pmax_cb, a.cap);
+    if (shm > 160 * 1024)
+        fail(HANN_ERR_UNSUPPORTED, strf("scan needs %zu bytes of LDS (d=%d, m=%d, pq_k=%d, k=%d); the limit is 160 KiB",
+                                        shm, a.d, a.m, a.pmax_cb, a.cap / 2));
+#define HANN_SCAN_ORDERED(MODE, CT)                                                                     \
+    do {                                                                                                \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan<MODE, CT>,                               \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));            \
+        for (int64_t b0 = 0; b0 < nq_max; b0 += 65535) {                                                \
+            ScanArgs b = a;                                                                             \
+            b.q_order = order;                                                                          \
+            b.q_order_n = order_n;                                                                      \
+            b.order_base = (int)b0;                                                                     \
+            const int64_t bn = nq_max - b0 < 65535 ? nq_max - b0 : 65535;                               \
+            hipLaunchKernelGGL((dev::k_scan<MODE, CT>), dim3((unsigned)bn), dim3(64), shm, st, b);       \
+            HANN_LAUNCH_CHECK();                                                                        \
+        }                                                                                               \
+    } while (0)
+    if (code_bytes == 1) {
+        if (mode == SCAN_MODE_EXACT) HANN_SCAN_ORDERED(SCAN_MODE_EXACT, uint8_t);
+        else if (mode == SCAN_MODE_ADC_F32) HANN_SCAN_ORDERED(SCAN_MODE_ADC_F32, uint8_t);
+        else HANN_SCAN_ORDERED(SCAN_MODE_ADC_F16, uint8_t);
+    } else {
+        if (mode == SCAN_MODE_EXACT) HANN_SCAN_ORDERED(SCAN_MODE_EXACT, uint16_t);
+        else if (mode == SCAN_MODE_ADC_F32) HANN_SCAN_ORDERED(SCAN_MODE_ADC_F32, uint16_t);
+        else HANN_SCAN_ORDERED(SCAN_MODE_ADC_F16, uint16_t);
+    }
+#undef HANN_SCAN_ORDERED
 }
 
 void launch_rerank(const float *queries, int d, int64_t nq, const int64_t *cand_pos, const int32_t *cand_cnt,

@@ -1076,12 +1076,21 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
         a.any_uncoded = 1;
         mode = SCAN_MODE_ADC_F32;
     }
+    // the plain path (the exact mode with a rerank) runs the list-major
+    // exact scan where it covers the shape: the same distances bit for bit
+    // and the same candidate rows; otherwise the per-query exact scan
+    const bool exact_list = mode == SCAN_MODE_EXACT && R > 0 && have_codebooks && !any_uncoded && probe_stride > 0 &&
+                            (int64_t)d_ids.n < INT32_MAX && np <= select_rows_max_probes() &&
+                            exact_scan_supported((int)dim, (int)m, (int)pmax, code_bytes) &&
+                            select_records_lds_bytes() <= 160 * 1024 &&
+                            scan_lds_bytes(SCAN_MODE_EXACT, d, (int)m, (int)pmax, cap) <= 160 * 1024 &&
+                            list_plan_fits(lplan = list_plan(nq, np, kmax * R, true), nq);
     const bool fused = (p.fused || list_scan) && have_codebooks && (mode == SCAN_MODE_ADC_F32 || mode == SCAN_MODE_ADC_F16);
     last_scan_mode = list_scan ? p.scan_mode : mode;
-    last_scan_fused = (fused && !list_scan) ? 1 : 0;
+    last_scan_fused = exact_list ? 2 : ((fused && !list_scan) ? 1 : 0);
     auto scan = [&](const ScanArgs &args) {
-        if (list_scan)
-            run_list_scan(args, nq, x3, np, probe_rank, st, w, lplan);
+        if (list_scan || exact_list)
+            run_list_scan(args, nq, exact_list ? 2 : x3, np, probe_rank, st, w, lplan);
         else if (fused)
             run_fused_scan(args, nq, np, probe_rank, st, w);
         else {
@@ -1188,7 +1197,8 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
 struct ListArena {
     size_t off = 0;
     size_t o_lcnt, o_lscnt, o_iord, o_ctr, o_parts, o_parts_f, o_qcnt, o_zero_end, o_lstart, o_ioff, o_lfill, o_lbfill,
-        o_qsnp, o_qsamp, o_tot, o_qtot, o_soff, o_theta, o_ovf, o_pq, o_pqc, o_psc, o_poff, o_items, o_items_s, o_pkey;
+        o_qsnp, o_qsamp, o_tot, o_qtot, o_soff, o_theta, o_qscale, o_ovf, o_pq, o_pqc, o_psc, o_poff, o_items,
+        o_items_s, o_pkey;
     size_t carve(size_t bytes) {
         const size_t o = off;
         off += (bytes + 255) & ~(size_t)255;
@@ -1213,6 +1223,7 @@ struct ListArena {
         o_qtot = carve((size_t)nq * 8);
         o_soff = carve((size_t)nq * 8);
         o_theta = carve((size_t)nq * 8);
+        o_qscale = carve((size_t)nq * 4);
         o_ovf = carve((size_t)nq * 4);
         o_pq = carve((size_t)pairs_max * 4);
         o_pqc = carve((size_t)pairs_max * 4);
@@ -1226,26 +1237,31 @@ struct ListArena {
 
 void PQIVF::refresh_shape() const {
     if (shape_epoch.load(std::memory_order_acquire) == mirror_epoch) return;
-    const int64_t chunk = list_scan_chunk();
-    int64_t max_len = 0, chunks_all = 0;
+    const int64_t chunk = int64_t(1) << kListItems.ch, chunk_x = int64_t(1) << kExactItems.ch;
+    int64_t max_len = 0, chunks_all = 0, chunks_x = 0;
     for (int64_t v : dev_len) {
         max_len = std::max(max_len, v);
         chunks_all += ceil_div(v, chunk);
+        chunks_x += ceil_div(v, chunk_x);
     }
     shape_max_len.store(max_len, std::memory_order_relaxed);
     shape_chunks.store(chunks_all, std::memory_order_relaxed);
+    shape_chunks_x.store(chunks_x, std::memory_order_relaxed);
     shape_epoch.store(mirror_epoch, std::memory_order_release);
 }
 
-PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk) const {
+PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk, bool exact) const {
     ListPlan L;
-    // work items: a list with p pairs and n entries has ceil(p / 32) *
-    // ceil(n / chunk) of them, at most (p / 32 + 1) * ceil(n / chunk)
-    const int64_t chunk = list_scan_chunk();
+    // work items: a list with p pairs and n entries has ceil(p / G) *
+    // ceil(n / C) of them (G queries, C entries an item), at most
+    // (p / G + 1) * ceil(n / C)
+    const ItemShape sh = exact ? kExactItems : kListItems;
+    const int64_t chunk = int64_t(1) << sh.ch, group = int64_t(1) << sh.qg;
     const int64_t max_len = shape_max_len.load(std::memory_order_relaxed);
-    const int64_t chunks_all = shape_chunks.load(std::memory_order_relaxed);
+    const int64_t chunks_all = (exact ? shape_chunks_x : shape_chunks).load(std::memory_order_relaxed);
+    L.exact = exact;
     L.pairs_max = nq * (int64_t)np;
-    L.items_max = ceil_div(L.pairs_max, 32) * ceil_div(max_len, chunk) + chunks_all + 1;
+    L.items_max = ceil_div(L.pairs_max, group) * ceil_div(max_len, chunk) + chunks_all + 1;
     // a query's sample: at least two probes, then on while fewer than
     // 4 selk entries come before: at most max(4 selk + max_len, 2 max_len)
     // entries, each probe's run rounded up to 8 slots
@@ -1288,12 +1304,16 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     int4 *items = reinterpret_cast<int4 *>(at(A.o_items));
     int4 *items_s = reinterpret_cast<int4 *>(at(A.o_items_s));   // the sample pass's items
     float *pkey = reinterpret_cast<float *>(at(A.o_pkey));
+    const bool exact = x3 == 2;
+    float *q_scale = exact ? reinterpret_cast<float *>(at(A.o_qscale)) : nullptr;   // key scale per query
+    const ItemShape sh = exact ? kExactItems : kListItems;
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
         HANN_HIP(hipDeviceGetAttribute(&num_cus, hipDeviceAttributeMultiprocessorCount, dev));
     }
-    const size_t shm = list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2);
+    const size_t shm = exact ? exact_scan_lds_bytes((int)dim, (int)m, (int)pmax)
+                             : list_scan_lds_bytes_kind(x3, (int)m, (int)pmax, 2);
     const int grid = num_cus * (int)std::max<size_t>(1, std::min<size_t>(4, (160 * 1024) / shm));
     // the filter pass's records, capq a query; the sample pass's keys
     uint64_t *rec = w.get<uint64_t>(18, (size_t)(nq * L.capq));
@@ -1305,11 +1325,11 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     launch_fill_i32(lcnt, (int64_t)(A.o_zero_end - A.o_lcnt) / 4, 0, st);
     launch_pair_count(a, nq, lcnt, lscnt, qtot, qsamp_np, qsamp, st);
     launch_row_offsets(qsamp, nq, soff, st);
-    launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, lbfill, totals, st);
+    launch_pair_offsets(lcnt, d_list_len.get(), K, lstart, ioff, lfill, lbfill, totals, sh, st);
     launch_pair_scatter(a, nq, probe_rank, d_ymax_tmax.get(), lstart, lcnt, qsamp_np, soff, lfill, lbfill, pair_q,
-                        pair_qc, pair_scale, pair_off, st);
+                        pair_qc, pair_scale, pair_off, q_scale, st);
     launch_make_items(lcnt, lscnt, lstart, d_list_len.get(), K, reinterpret_cast<int32_t *>(at(A.o_iord)), items,
-                      items_s, totals + 2, st);
+                      items_s, totals + 2, sh, st);
     ListScanArgs la{};
     la.queries = a.queries;
     la.d = (int)dim;
@@ -1337,6 +1357,8 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.rec = rec;
     la.qcnt = qcnt;
     la.capq = (uint32_t)L.capq;
+    la.coarse = d_coarse.get();
+    la.chunk = 1 << sh.ch;
     // 2. the sample pass: every key of each query's nearest probed lists;
     //    then each query's bound theta, the (k R)-th smallest of them
     {
@@ -1344,7 +1366,10 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
         ls.items = items_s;
         ls.n_items = totals + 2;
         mark_kernel(2, st);
-        launch_scan_list_kind(ls, x3, 1, grid, st);
+        if (exact)
+            launch_scan_exact_kind(ls, 1, grid, st);
+        else
+            launch_scan_list_kind(ls, x3, 1, grid, st);
         mark_kernel(3, st);
     }
     launch_theta(a, nq, srow, soff, qtot, qsamp, theta, st);
@@ -1352,13 +1377,24 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     //    <= theta in the query's record room
     la.item_counter = parts_f;
     mark_kernel(0, st);
-    launch_scan_list_kind(la, x3, 2, grid, st);
+    if (exact)
+        launch_scan_exact_kind(la, 2, grid, st);
+    else
+        launch_scan_list_kind(la, x3, 2, grid, st);
     mark_kernel(1, st);
     // 4. the selk best records per query, as the rerank's rows
-    launch_select_records(a, nq, rec, qcnt, (uint32_t)L.capq, qtot, a.out_id, a.out_cnt, selk, ovf, ctr + 1, st);
+    launch_select_records(a, nq, rec, qcnt, (uint32_t)L.capq, qtot, a.out_id, a.out_cnt, selk, ovf, ctr + 1, q_scale,
+                          st);
     // 5. the queries the select passed on: the table scan ranks them (over
     //    all lists where the reference's brute-force rule applies, counter
     //    included; over their probes when their records overflowed)
+    if (exact) {
+        // the exact scan's passed-on queries: the per-query exact scan
+        ScanArgs fa = a;
+        fa.out_key = pkey;
+        launch_scan_ordered(SCAN_MODE_EXACT, code_bytes, fa, nq, ovf, ctr + 1, st);
+        return;
+    }
     FusedArgs f{};
     f.term = d_term.get();
     f.probe_rank = probe_rank;
@@ -1402,8 +1438,9 @@ int64_t PQIVF::search_scratch_bytes(int64_t nq, int kmax, int pmax_probe, const 
     add(nq * 4);
     add(4);
     if (nq * m * pmax * 4 <= (int64_t(512) << 20)) add(nq * m * pmax * 4);
-    if (p.scan_mode == SCAN_MODE_LIST_BF16 || p.scan_mode == SCAN_MODE_LIST_BF16X3) {
-        const ListPlan L = list_plan(nq, np, selk);
+    const bool lm = p.scan_mode == SCAN_MODE_LIST_BF16 || p.scan_mode == SCAN_MODE_LIST_BF16X3;
+    if (lm || (p.scan_mode == SCAN_MODE_EXACT && R > 0)) {
+        const ListPlan L = list_plan(nq, np, selk, !lm);
         add((int64_t)L.arena);
         add(nq * L.capq * 8);
         add(L.sample_slots * 2);

@@ -206,13 +206,14 @@ struct PQIVF {
     struct ListPlan {
         int64_t pairs_max = 0, items_max = 0, sample_slots = 0, capq = 0;
         size_t arena = 0;
+        bool exact = false;   // the exact scan's work items
     };
-    ListPlan list_plan(int64_t nq, int np, int selk) const;
+    ListPlan list_plan(int64_t nq, int np, int selk, bool exact = false) const;
     // the device lists' longest length and chunk count, refreshed from the
     // mirror when it changed (callers hold the index lock); readable without
     // it, then as of the last refresh
     mutable std::atomic<uint64_t> shape_epoch{~0ull};
-    mutable std::atomic<int64_t> shape_max_len{0}, shape_chunks{0};
+    mutable std::atomic<int64_t> shape_max_len{0}, shape_chunks{0}, shape_chunks_x{0};
     void refresh_shape() const;
     // reasons the list-major scan was declined, each said once on stderr
     std::mutex note_mu;
@@ -223,7 +224,8 @@ struct PQIVF {
     // lengths as of the last refresh_shape)
     int64_t search_scratch_bytes(int64_t nq, int kmax, int pmax_probe, const hann_search_params &p) const;
     static bool list_plan_fits(const ListPlan &L, int64_t nq);
-    // list-major MFMA scan; a.out_* are the candidate rows of the rerank
+    // list-major scan (x3: 0 BF16, 1 split BF16 MFMA, 2 exact FP32); a.out_*
+    // are the candidate rows of the rerank
     void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
                        Workspace &w, const ListPlan &L);
     // stage timers (HIP events on the search stream), accumulated when enabled

@@ -33,8 +33,19 @@ struct ListScanArgs {
                                  // order they were written; the first capq of them are kept
     uint32_t *qcnt;              // [nq] records written per query (zeroed before the launch)
     uint32_t capq;               // record room per query
+    // the exact scan
+    const float *coarse;         // [K][d] coarse centroids
+    int chunk;                   // entries per work item (a multiple of 64)
 };
 
+// work items: up to 2^qg queries of a list times a chunk of up to 2^ch of
+// its entries (qg <= 5, 6 <= ch <= 11)
+struct ItemShape {
+    int qg, ch;
+};
+// the MFMA scan's items (32 queries x 2,048 entries) and the exact scan's
+constexpr ItemShape kListItems{5, 11};
+constexpr ItemShape kExactItems{3, 8};
 // entries of a list per work item of the list-major scan
 int list_scan_chunk();
 // work-item partitions (counters) of the list-major scan
@@ -53,21 +64,22 @@ void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hip
 // totals[1] = items, totals[2] = 0 (then the sample pass's items); lfill and
 // lbfill are zeroed
 void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
-                         int32_t *lfill, int32_t *lbfill, int32_t *totals, hipStream_t st);
+                         int32_t *lfill, int32_t *lbfill, int32_t *totals, ItemShape sh, hipStream_t st);
 // the pairs grouped by list (each list's sample pairs first), with the
 // coarse rank, the query's key scale and, for a sample pair, where its keys
 // start in the sample rows (0 for the others); ymax: bits of an upper bound
-// of |y_e| over the index
+// of |y_e| over the index; q_scale (or null): each query's key scale
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
                          const int32_t *lstart, const int32_t *lcnt, const int32_t *qsamp_np, const int64_t *row_off,
                          int32_t *lfill, int32_t *lbfill, int32_t *pair_q, float *pair_qc, float *pair_scale,
-                         int64_t *pair_off, hipStream_t st);
+                         int64_t *pair_off, float *q_scale, hipStream_t st);
 // the batch's work items and the sample pass's (over each list's sample
 // pairs; their count is added to *n_samp), longest first; order:
 // list_item_order_ints() ints of scratch (per-bucket counts and fill counters),
 // zeroed by the caller
 void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t *lstart, const int64_t *list_len,
-                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st);
+                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, ItemShape sh,
+                       hipStream_t st);
 int list_item_order_ints();
 // the list-major scan covers s = d/m = 4, m in {8, 16, 24, 32, 48, 64},
 // uint8 codes, and a BF16 codebook that fits in LDS
@@ -76,6 +88,14 @@ bool list_scan_supported(int d, int m, int P, int code_bytes);
 // rows); 2: the filter pass's records
 void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st);
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
+// The list-major exact scan: the reference's decode-and-measure distance
+// (pqivf/index.go:905-936) of every (query, probed entry) pair in FP32, bit
+// for bit, with the same sample / bound / filter / select passes; records
+// hold the exact distance. d a multiple of 32, s in {4, 8}, uint8 codes and
+// an FP32 codebook that fits in LDS.
+bool exact_scan_supported(int d, int m, int P, int code_bytes);
+size_t exact_scan_lds_bytes(int d, int m, int P);
+void launch_scan_exact_kind(const ListScanArgs &la, int kind, int grid, hipStream_t st);
 // per query, the bound theta of the filter pass: the (k R)-th smallest key of
 // its sample (qsamp_slots keys from row_off[q] of the sample rows)
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
@@ -83,11 +103,13 @@ void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int
 // per query, the selk = k_q * R best entries by (key, entry position) from the
 // filter pass's records, as entry positions (-1 after the last); queries
 // that take the brute-force fallback, and those with more than capq
-// records, go to ovf (the table scan then ranks them)
+// records, go to ovf (a per-query scan then ranks them). q_scale null: the
+// records hold 16-bit keys; else exact FP32 distances, binned by the 16-bit
+// key of the distance times q_scale[q]
 size_t select_records_lds_bytes();
 void launch_select_records(const ScanArgs &a, int64_t nq, const uint64_t *rec, const uint32_t *qcnt, uint32_t capq,
                            const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt, int out_stride, int32_t *ovf,
-                           int32_t *ovf_cnt, hipStream_t st);
+                           int32_t *ovf_cnt, const float *q_scale, hipStream_t st);
 // probes per query the threshold and select kernels take at most
 int select_rows_max_probes();
 

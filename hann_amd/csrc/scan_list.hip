@@ -178,10 +178,10 @@ __global__ void __launch_bounds__(1024) k_row_offsets(const int64_t *__restrict_
     }
 }
 
-// work items of a list: query groups of 32 times entry chunks
-__device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len) {
-    static_assert(LQ == 32 && LCH >= 32, "shifts below");
-    return pairs > 0 ? ((pairs + LQ - 1) >> 5) * (int32_t)((len + LCH - 1) >> LCH_SHIFT) : 0;
+// work items of a list: query groups of 2^qg times entry chunks of 2^ch
+// (the list-major MFMA scan takes 32 x 2,048, the exact scan smaller items)
+__device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len, ItemShape sh) {
+    return pairs > 0 ? ((pairs + (1 << sh.qg) - 1) >> sh.qg) * (int32_t)((len + (1 << sh.ch) - 1) >> sh.ch) : 0;
 }
 
 // one workgroup of 1024: exclusive scans of lcnt and of the item counts,
@@ -195,7 +195,7 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
                                                        const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
                                                        int32_t *__restrict__ lfill, int32_t *__restrict__ lbfill,
-                                                       int32_t *__restrict__ totals) {
+                                                       int32_t *__restrict__ totals, ItemShape sh) {
     // [wave][round] wave totals, then their exclusive prefixes over the
     // waves plus the rounds before (the span's carry included)
     __shared__ int32_t s_a[16][PO_ROUNDS], s_b[16][PO_ROUNDS];
@@ -211,7 +211,7 @@ __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict
             const int32_t c = lcnt[lc];
             const int64_t n = list_len[lc];
             v[r] = l < K ? c : 0;
-            it[r] = l < K ? list_items(c, n) : 0;
+            it[r] = l < K ? list_items(c, n, sh) : 0;
             a[r] = v[r];
             b[r] = it[r];
         }
@@ -305,7 +305,8 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
                                                      const int64_t *__restrict__ row_off, int32_t *__restrict__ lfill,
                                                      int32_t *__restrict__ lbfill,
                                                      int32_t *__restrict__ pair_q, float *__restrict__ pair_qc,
-                                                     float *__restrict__ pair_scale, int64_t *__restrict__ pair_off) {
+                                                     float *__restrict__ pair_scale, int64_t *__restrict__ pair_off,
+                                                     float *__restrict__ q_scale) {
     const int64_t q = blockIdx.x;
     const int lane = threadIdx.x;
     const int npq = clamp_nprobe(a, q);
@@ -326,6 +327,7 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
         e = e < -100 ? -100 : (e > 100 ? 100 : e);
         scale = __builtin_ldexpf(1.f, 14 - e);
     }
+    if (q_scale && lane == 0) q_scale[q] = scale;
     int64_t carry = row_off[q];
     const int snp = qsamp_np[q];
     for (int l0 = 0; l0 < npq; l0 += 64) {
@@ -360,9 +362,10 @@ __global__ void __launch_bounds__(64) k_pair_scatter(ScanArgs a, const double *_
 // (results do not depend on the order: every pair's keys have a fixed place).
 constexpr int kItemBuckets = LCH / 32;
 
-__device__ inline int item_bucket(int64_t len, int c) {
-    const int64_t e = len - (int64_t)c * LCH;
-    const int slabs = (int)(((e < LCH ? e : LCH) + 31) >> 5);
+__device__ inline int item_bucket(int64_t len, int c, ItemShape sh) {
+    const int64_t ch = (int64_t)1 << sh.ch;
+    const int64_t e = len - (int64_t)c * ch;
+    const int slabs = (int)(((e < ch ? e : ch) + 31) >> 5);
     return kItemBuckets - slabs;
 }
 
@@ -373,18 +376,20 @@ __global__ void __launch_bounds__(256) k_item_hist(const int32_t *__restrict__ l
                                                    const int32_t *__restrict__ lscnt,
                                                    const int64_t *__restrict__ list_len, int K,
                                                    int32_t *__restrict__ hist, int32_t *__restrict__ hist_s,
-                                                   int32_t *__restrict__ n_samp) {
+                                                   int32_t *__restrict__ n_samp, ItemShape sh) {
     __shared__ int32_t s_h[2 * kItemBuckets];
     if (threadIdx.x < 2 * kItemBuckets) s_h[threadIdx.x] = 0;
     __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gq = 1 << sh.qg;
+    const int64_t ch = (int64_t)1 << sh.ch;
     if (l < K && lcnt[l] > 0) {
         const int64_t len = list_len[l];
-        const int nqg = (lcnt[l] + LQ - 1) / LQ, nsg = (lscnt[l] + LQ - 1) / LQ;
-        const int nch = (int)((len + LCH - 1) / LCH);
+        const int nqg = (lcnt[l] + gq - 1) / gq, nsg = (lscnt[l] + gq - 1) / gq;
+        const int nch = (int)((len + ch - 1) / ch);
         for (int c = 0; c < nch; ++c) {
-            atomicAdd(&s_h[item_bucket(len, c)], nqg);
-            if (nsg) atomicAdd(&s_h[kItemBuckets + item_bucket(len, c)], nsg);
+            atomicAdd(&s_h[item_bucket(len, c, sh)], nqg);
+            if (nsg) atomicAdd(&s_h[kItemBuckets + item_bucket(len, c, sh)], nsg);
         }
     }
     __syncthreads();
@@ -406,19 +411,22 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
                                                     const int64_t *__restrict__ list_len, int K,
                                                     const int32_t *__restrict__ hist, int32_t *__restrict__ fill,
                                                     int4 *__restrict__ items, const int32_t *__restrict__ hist_s,
-                                                    int32_t *__restrict__ fill_s, int4 *__restrict__ items_s) {
+                                                    int32_t *__restrict__ fill_s, int4 *__restrict__ items_s,
+                                                    ItemShape sh) {
     __shared__ int32_t s_c[2 * kItemBuckets];
     if (threadIdx.x < 2 * kItemBuckets) s_c[threadIdx.x] = 0;
     __syncthreads();
     const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gq = 1 << sh.qg;
+    const int64_t ch = (int64_t)1 << sh.ch;
     const int n = l < K ? lcnt[l] : 0;
     const int ns = n > 0 ? lscnt[l] : 0;
     const int64_t len = n > 0 ? list_len[l] : 0;
-    const int nqg = (n + LQ - 1) / LQ, nsg = (ns + LQ - 1) / LQ;
-    const int nch = (int)((len + LCH - 1) / LCH);
+    const int nqg = (n + gq - 1) / gq, nsg = (ns + gq - 1) / gq;
+    const int nch = (int)((len + ch - 1) / ch);
     for (int c = 0; c < nch; ++c) {
-        atomicAdd(&s_c[item_bucket(len, c)], nqg);
-        if (nsg) atomicAdd(&s_c[kItemBuckets + item_bucket(len, c)], nsg);
+        atomicAdd(&s_c[item_bucket(len, c, sh)], nqg);
+        if (nsg) atomicAdd(&s_c[kItemBuckets + item_bucket(len, c, sh)], nsg);
     }
     __syncthreads();
     if (threadIdx.x < 2 * kItemBuckets) {
@@ -432,14 +440,14 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
     __syncthreads();
     const int ls = n > 0 ? lstart[l] : 0;
     for (int c = 0; c < nch; ++c) {
-        const int b = item_bucket(len, c);
-        for (int t = 0; t * LQ < n; ++t) {
+        const int b = item_bucket(len, c, sh);
+        for (int t = 0; t * gq < n; ++t) {
             const int o = atomicAdd(&s_c[b], 1);
-            items[o] = make_int4(l, ls + t * LQ, n - t * LQ < LQ ? n - t * LQ : LQ, c * LCH);
+            items[o] = make_int4(l, ls + t * gq, n - t * gq < gq ? n - t * gq : gq, (int)(c * ch));
         }
-        for (int t = 0; t * LQ < ns; ++t) {
+        for (int t = 0; t * gq < ns; ++t) {
             const int o = atomicAdd(&s_c[kItemBuckets + b], 1);
-            items_s[o] = make_int4(l, ls + t * LQ, ns - t * LQ < LQ ? ns - t * LQ : LQ, c * LCH);
+            items_s[o] = make_int4(l, ls + t * gq, ns - t * gq < gq ? ns - t * gq : gq, (int)(c * ch));
         }
     }
 }
@@ -812,6 +820,181 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         while (fill != head) flush(fill - head < 64u ? fill - head : 64u);
 }
 
+// The list-major exact scan. A work item is a chunk of up to la.chunk
+// entries of one list with up to 8 of its queries, and belongs to one wave;
+// the workgroup's waves share the FP32 codebook in LDS. Lane l takes entry
+// b + l of each 64-entry slab of the chunk and decodes it once into
+// registers as decodePQCode + vectorAdd do (pqivf/index.go:678-714): x[t] =
+// c[t] + cb_i[code_i][t] in float32. Then, for each query of the item, the
+// squared distance in the reference's AVX2 order (32 accumulator slots,
+// blocks of 32 elements, d = q - x rounded, fused d * d + acc; slots
+// combined (acc0 + acc1) + (acc2 + acc3) per lane, then the horizontal sum),
+// bit-identical to the reference and to k_scan's exact mode. Query values
+// are wave-uniform and load through the scalar unit (they are the VALU
+// instructions' scalar operands).
+// The sample pass stores each sample pair's 16-bit keys (of the distance
+// times the query's scale) in its run of the query's sample row; the filter
+// pass keeps the records (distance bits << 32 | entry position) whose
+// scaled distance has a key <= theta, through the same per-wave ring as the
+// MFMA scan.
+constexpr int XNW = 8;
+
+// wave-uniform reads of buffers the kernel does not write, through the
+// constant address space, so that they load through the scalar unit
+typedef const __attribute__((address_space(4))) float cfloat;
+__device__ __forceinline__ cfloat *as_const(const float *p) {
+    return (cfloat *)(const __attribute__((address_space(4))) void *)(uintptr_t)p;
+}
+
+template <int D, int S, int KIND>
+__global__ void __launch_bounds__(64 * XNW) k_scan_exact(ListScanArgs la) {
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    constexpr int M = D / S;
+    constexpr int NB = D / 32;
+    static_assert(D % 32 == 0 && (S == 4 || S == 8) && M % 4 == 0, "shape");
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int P = la.P;
+    float *sCB = reinterpret_cast<float *>(s_raw);   // [M][P][S]
+    uint64_t *ring0 = reinterpret_cast<uint64_t *>(sCB + (size_t)M * P * S);
+    uint64_t *s_ring = ring0 + (size_t)wave * RING;
+    uint32_t *s_ringq = reinterpret_cast<uint32_t *>(ring0 + (size_t)XNW * RING) + (size_t)wave * RING;
+    const uint8_t *codes = reinterpret_cast<const uint8_t *>(la.codes);
+    {
+        const float4 *src = reinterpret_cast<const float4 *>(la.cb);
+        float4 *dst = reinterpret_cast<float4 *>(sCB);
+        for (int t = tid; t < M * P * S / 4; t += 64 * XNW) dst[t] = src[t];
+    }
+    __syncthreads();
+    const int n_items = *la.n_items;
+    const int nparts = la.n_parts, part = (int)(blockIdx.x % (unsigned)nparts);
+    int32_t *ctr = la.item_counter + part;
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc(la.dist, (short)0, (int)la.dist_bytes, 0x00020000);
+    uint32_t fill = 0, head = 0;
+    // the whole wave writes n <= 64 pending records (lane i the i-th), each
+    // at the next place of its query's record room, one atomic a query
+    auto flush = [&](uint32_t n) {
+        wave_sync();
+        const uint32_t slot = (head + (uint32_t)lane) & (RING - 1);
+        const uint64_t v = s_ring[slot];
+        const uint32_t vq = s_ringq[slot];
+        const bool has = (uint32_t)lane < n;
+        int lead = 0;
+        uint32_t rank = 0, gcnt = 0;
+        for (uint64_t todo = __builtin_amdgcn_ballot_w64(has); todo;) {
+            const int l0 = __builtin_ctzll(todo);
+            const uint32_t ql = (uint32_t)__builtin_amdgcn_readlane((int)vq, l0);
+            const uint64_t grp = __builtin_amdgcn_ballot_w64(vq == ql) & todo;
+            if (__builtin_amdgcn_inverse_ballot_w64(grp)) {
+                lead = l0;
+                rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u));
+            }
+            if (lane == l0) gcnt = (uint32_t)__builtin_popcountll(grp);
+            todo &= ~grp;
+        }
+        uint32_t p0 = 0;
+        if (has && lane == lead) p0 = atomicAdd(&la.qcnt[vq], gcnt);
+        const uint32_t pp = (uint32_t)__shfl((int)p0, lead) + rank;
+        if (has && pp < la.capq) la.rec[(uint64_t)vq * la.capq + pp] = v;
+        head = (uint32_t)__builtin_amdgcn_readfirstlane((int)(head + n));
+        wave_sync();
+    };
+    int j = 0;
+    if (lane == 0) j = atomicAdd(ctr, 1);
+    int item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
+    while (item < n_items) {
+        if (lane == 0) j = atomicAdd(ctr, 1);
+        const int4 it = la.items[item];
+        const int l = __builtin_amdgcn_readfirstlane(it.x);
+        const int pc = __builtin_amdgcn_readfirstlane(it.z);
+        const int eb = __builtin_amdgcn_readfirstlane(it.w);
+        const int e0 = (int)la.list_start[l];
+        const int n = (int)la.list_len[l];
+        const int ee = n < eb + la.chunk ? n : eb + la.chunk;
+        const int slot_end = ee == n ? (int)row_slots(n) : ee;
+        // lane jq < pc holds the item's query jq: its id, key scale, and its
+        // run's start in the sample row or its bound
+        const int pidx = it.y + (lane < pc ? lane : 0);
+        const int vq = la.pair_q[pidx];
+        const float vsc = la.pair_scale[pidx];
+        const int vaux = KIND == KIND_SAMPLE ? (int)la.pair_off[pidx] : key_bound(la.theta[vq]);
+        const float *cent = la.coarse + (int64_t)l * D;
+        for (int b = eb; b < ee; b += 64) {
+            if (KIND == KIND_FILTER)
+                while (fill - head >= 64u) flush(64);
+            const int e = b + lane;
+            const bool valid = e < ee;
+            const int er = valid ? e : ee - 1;
+            float x[D];
+            {
+                const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes + (int64_t)(e0 + er) * M);
+                uint32_t w[M / 4];
+#pragma unroll
+                for (int u = 0; u < M / 4; ++u) w[u] = cw[u];
+#pragma unroll
+                for (int i = 0; i < M; ++i) {
+                    const uint32_t c = (w[i >> 2] >> (8 * (i & 3))) & 0xFFu;
+                    const float4 *cb = reinterpret_cast<const float4 *>(sCB + ((size_t)i * P + c) * S);
+#pragma unroll
+                    for (int h = 0; h < S / 4; ++h) {
+                        const float4 v = cb[h];
+                        const int t = i * S + 4 * h;
+                        x[t + 0] = cent[t + 0] + v.x;
+                        x[t + 1] = cent[t + 1] + v.y;
+                        x[t + 2] = cent[t + 2] + v.z;
+                        x[t + 3] = cent[t + 3] + v.w;
+                    }
+                }
+            }
+            for (int jq = 0; jq < pc; ++jq) {
+                const int qid = __builtin_amdgcn_readlane(vq, jq);
+                const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vsc), jq));
+                const int aux = __builtin_amdgcn_readlane(vaux, jq);
+                cfloat *qp = as_const(la.queries + (int64_t)qid * D);
+                float acc[32];
+#pragma unroll
+                for (int c = 0; c < 32; ++c) acc[c] = 0.f;
+#pragma unroll
+                for (int blk = 0; blk < NB; ++blk) {
+#pragma unroll
+                    for (int c = 0; c < 32; ++c) {
+                        const float dd = qp[32 * blk + c] - x[32 * blk + c];
+                        acc[c] = __builtin_fmaf(dd, dd, acc[c]);
+                    }
+                }
+                float s8[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s8[u] = (acc[u] + acc[8 + u]) + (acc[16 + u] + acc[24 + u]);
+                const float key = hsum8(s8);
+                const float ks = key * sc;
+                if (KIND == KIND_SAMPLE) {
+                    // entries past the chunk up to the run's slot end take
+                    // the padding key 0xFFFF; other lanes' stores are discarded
+                    const uint32_t k16 = valid ? key16(ks) : 0xFFFFu;
+                    const uint32_t boff = e < slot_end ? 2u * (uint32_t)(aux + e) : kNoStore;
+                    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)k16, drs, (int)boff, 0, 0);
+                } else {
+                    const uint64_t mk = __builtin_amdgcn_ballot_w64(valid && __float_as_int(ks) <= aux);
+                    if (mk) {
+                        if (fill - head > (uint32_t)(RING - 64)) flush(64);
+                        if (__builtin_amdgcn_inverse_ballot_w64(mk)) {
+                            const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, fill));
+                            s_ring[pos & (RING - 1)] = ((uint64_t)__float_as_uint(key) << 32) | (uint32_t)(e0 + e);
+                            s_ringq[pos & (RING - 1)] = (uint32_t)qid;
+                        }
+                        fill = (uint32_t)__builtin_amdgcn_readfirstlane(
+                            (int)(fill + (uint32_t)__builtin_popcountll(mk)));
+                    }
+                }
+            }
+        }
+        item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
+    }
+    if (KIND == KIND_FILTER)
+        while (fill != head) flush(fill - head < 64u ? fill - head : 64u);
+}
+
 // Selection helpers of the threshold and select kernels: histograms over
 // key bits, and the bin where a running count reaches a rank.
 // the bin of hist[0 .. nb) where the count reaches need: s_bin, with s_below
@@ -934,25 +1117,30 @@ constexpr int SELR_BINS = 2048;    // histogram over key >> 5
 // One workgroup of 256 per query: the sk = min(k_q * R, probed entries)
 // best entries of the query by (key, entry position), from the records the
 // filter pass kept (every entry with key <= theta[q]; the sk best are among
-// them), held contiguously in the query's record room. A 2,048-bin
-// histogram over key >> 5 gives the bin B where the count reaches sk;
-// records below B are selected, and those in B are ranked by (key, entry
-// position) -- by counting for up to SELR_RANK of them, else by a sort, or
-// for more than SELR_ROOM by the exact boundary key and a radix select over
-// positions. Output: entry positions for the rerank. Queries that take the
-// brute-force fallback, and those whose records overflowed their room, go
-// to ovf.
+// them), held contiguously in the query's record room. A record is
+// (key << 32 | entry position): the 16-bit key of the MFMA scan, or, with
+// q_scale, the exact FP32 distance's bits (positive floats order like their
+// bits), binned by the 16-bit key of the distance times the query's scale
+// (a non-decreasing function of it). A 2,048-bin histogram over key >> 5
+// gives the bin B where the count reaches sk; records below B are selected,
+// and those in B are ranked by their whole value -- by counting for up to
+// SELR_RANK of them, else by a sort, or for more than SELR_ROOM by a radix
+// select over the 64-bit value. Output: entry positions for the rerank.
+// Queries that take the brute-force fallback, and those whose records
+// overflowed their room, go to ovf.
 __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, const uint64_t *__restrict__ rec,
                                                         const uint32_t *__restrict__ qcnt, uint32_t capq,
                                                         const int64_t *__restrict__ qtot,
                                                         int64_t *__restrict__ out_pos, int32_t *__restrict__ out_cnt,
                                                         int out_stride, int32_t *__restrict__ ovf,
-                                                        int32_t *__restrict__ ovf_cnt) {
+                                                        int32_t *__restrict__ ovf_cnt,
+                                                        const float *__restrict__ q_scale) {
     __shared__ uint64_t cache[SELR_CACHE];   // (key << 32 | entry position)
     __shared__ uint64_t tie[SELR_ROOM];      // the boundary bin's records
     __shared__ unsigned hist[SELR_BINS];
     __shared__ int s_bin, s_out, s_fill;
-    __shared__ unsigned s_below, s_prefix;
+    __shared__ unsigned s_below;
+    __shared__ uint64_t s_prefix;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int64_t q = blockIdx.x;
     int kq = a.k_q ? a.k_q[q] : a.k_default;
@@ -969,6 +1157,11 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
     for (int t = sk + tid; t < out_stride; t += 256) orow[t] = -1;
     if (tid == 0) out_cnt[q] = sk;
     if (sk == 0) return;
+    const float qs = q_scale ? q_scale[q] : 0.f;
+    auto bin_of = [&](uint64_t v) -> unsigned {
+        if (!q_scale) return (unsigned)(v >> 37);
+        return key16(__uint_as_float((uint32_t)(v >> 32)) * qs) >> 5;
+    };
     for (int b = tid; b < SELR_BINS; b += 256) hist[b] = 0;
     if (tid == 0) {
         s_out = 0;
@@ -991,7 +1184,7 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
             const uint32_t t = t0 + 256 * u;
             if (t >= T) continue;
             if (t < (uint32_t)SELR_CACHE) cache[t] = v[u];
-            atomicAdd(&hist[(unsigned)(v[u] >> 37)], 1u);
+            atomicAdd(&hist[bin_of(v[u])], 1u);
         }
     }
     __syncthreads();
@@ -1012,7 +1205,7 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
     const bool room = nb <= SELR_ROOM;
     // pass 1: records below bin B are selected; bin B's go to LDS
     for_rec([&](uint64_t v) {
-        const unsigned b = (unsigned)(v >> 37);
+        const unsigned b = bin_of(v);
         if (b < B)
             orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
         else if (b == B && room)
@@ -1055,50 +1248,35 @@ __global__ void __launch_bounds__(256) k_select_records(ScanArgs a, int64_t nq, 
         for (int t = tid; t < need; t += 256) orow[o0 + t] = (int64_t)(uint32_t)tie[t];
         return;
     }
-    // more than SELR_ROOM records in bin B: its exact boundary key K (a
-    // histogram of its 32 key values), the keys below K in it, then a radix
-    // select over the entry positions of the records with key K
-    __syncthreads();
-    if (tid < 64) hist[tid] = 0;
-    __syncthreads();
-    for_rec([&](uint64_t v) {
-        if ((unsigned)(v >> 37) == B) atomicAdd(&hist[(unsigned)(v >> 32) & 31u], 1u);
-    });
-    __syncthreads();
-    bins_find(hist, 64, (unsigned)need, wave, lane, &s_bin, &s_below);
-    const unsigned K = (B << 5) | (unsigned)s_bin;
-    int kk = need - (int)s_below;
-    for_rec([&](uint64_t v) {
-        const unsigned k = (unsigned)(v >> 32);
-        if ((k >> 5) == B && k < K) orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
-    });
-    unsigned prefix = 0, mask = 0;
-    for (int pass = 0; pass < 4; ++pass) {
-        const int shift = 24 - 8 * pass;
+    // more than SELR_ROOM records in bin B: the need-th smallest value V of
+    // the bin by a radix select over the 64-bit record value, eight bits a
+    // pass from the top (values are distinct: entry positions are), then
+    // every record of the bin up to V
+    uint64_t prefix = 0, mask = 0;
+    int kk = need;
+    for (int pass = 0; pass < 8; ++pass) {
+        const int shift = 56 - 8 * pass;
         __syncthreads();
-        for (int b = tid; b < 256; b += 256) hist[b] = 0;
+        hist[tid] = 0;
         __syncthreads();
         for_rec([&](uint64_t v) {
-            if ((unsigned)(v >> 32) != K) return;
-            const unsigned p = (unsigned)v;
-            if ((p & mask) == prefix) atomicAdd(&hist[(p >> shift) & 255u], 1u);
+            if (bin_of(v) == B && (v & mask) == prefix) atomicAdd(&hist[(unsigned)(v >> shift) & 255u], 1u);
         });
         __syncthreads();
         if (tid == 0) {
             unsigned run = 0;
             int b = 0;
             while (run + hist[b] < (unsigned)kk) run += hist[b++];
-            s_prefix = prefix | ((unsigned)b << shift);
+            s_prefix = prefix | ((uint64_t)b << shift);
             s_below = run;
         }
         __syncthreads();
         prefix = s_prefix;
         kk -= (int)s_below;
-        mask |= 255u << shift;
+        mask |= (uint64_t)255 << shift;
     }
-    // entry positions are distinct: exactly the remaining ones are <= prefix
     for_rec([&](uint64_t v) {
-        if ((unsigned)(v >> 32) == K && (unsigned)v <= prefix) orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
+        if (bin_of(v) == B && v <= prefix) orow[atomicAdd(&s_out, 1)] = (int64_t)(uint32_t)v;
     });
 }
 
@@ -1125,31 +1303,34 @@ void launch_row_offsets(const int64_t *qslots, int64_t nq, int64_t *row_off, hip
 }
 
 void launch_pair_offsets(const int32_t *lcnt, const int64_t *list_len, int K, int32_t *lstart, int32_t *ioff,
-                         int32_t *lfill, int32_t *lbfill, int32_t *totals, hipStream_t st) {
+                         int32_t *lfill, int32_t *lbfill, int32_t *totals, ItemShape sh, hipStream_t st) {
     hipLaunchKernelGGL(dev::k_pair_offsets, dim3(1), dim3(1024), 0, st, lcnt, list_len, K, lstart, ioff, lfill, lbfill,
-                       totals);
+                       totals, sh);
     HANN_LAUNCH_CHECK();
 }
 
 void launch_pair_scatter(const ScanArgs &a, int64_t nq, const double *probe_rank, const unsigned *ymax,
                          const int32_t *lstart, const int32_t *lcnt, const int32_t *qsamp_np, const int64_t *row_off,
                          int32_t *lfill, int32_t *lbfill, int32_t *pair_q, float *pair_qc, float *pair_scale,
-                         int64_t *pair_off, hipStream_t st) {
+                         int64_t *pair_off, float *q_scale, hipStream_t st) {
     if (nq == 0) return;
     hipLaunchKernelGGL(dev::k_pair_scatter, dim3((unsigned)nq), dim3(64), 0, st, a, probe_rank, ymax, lstart, lcnt,
-                       qsamp_np, row_off, lfill, lbfill, pair_q, pair_qc, pair_scale, pair_off);
+                       qsamp_np, row_off, lfill, lbfill, pair_q, pair_qc, pair_scale, pair_off, q_scale);
     HANN_LAUNCH_CHECK();
 }
 
 void launch_make_items(const int32_t *lcnt, const int32_t *lscnt, const int32_t *lstart, const int64_t *list_len,
-                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, hipStream_t st) {
+                       int K, int32_t *order, int4 *items, int4 *items_s, int32_t *n_samp, ItemShape sh,
+                       hipStream_t st) {
+    if (sh.qg < 0 || sh.qg > 5 || sh.ch < 6 || sh.ch > dev::LCH_SHIFT)
+        fail(HANN_ERR_UNSUPPORTED, "list-major scan: unsupported work-item shape");
     constexpr int B = dev::kItemBuckets;
     int32_t *hist = order, *fill = order + B, *hist_s = order + 2 * B, *fill_s = order + 3 * B;
     hipLaunchKernelGGL(dev::k_item_hist, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lscnt, list_len, K,
-                       hist, hist_s, n_samp);
+                       hist, hist_s, n_samp, sh);
     HANN_LAUNCH_CHECK();
     hipLaunchKernelGGL(dev::k_make_items, dim3((unsigned)ceil_div(K, 256)), dim3(256), 0, st, lcnt, lscnt, lstart,
-                       list_len, K, hist, fill, items, hist_s, fill_s, items_s);
+                       list_len, K, hist, fill, items, hist_s, fill_s, items_s, sh);
     HANN_LAUNCH_CHECK();
 }
 
@@ -1208,6 +1389,46 @@ size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind) {
     return b;
 }
 
+bool exact_scan_supported(int d, int m, int P, int code_bytes) {
+    if (code_bytes != 1 || m <= 0 || d % m != 0) return false;
+    const int s = d / m;
+    if (s != 4 && s != 8) return false;
+    if (d != 64 && d != 96 && d != 128) return false;
+    return exact_scan_lds_bytes(d, m, P) <= 160 * 1024;
+}
+
+size_t exact_scan_lds_bytes(int d, int m, int P) {
+    return (size_t)m * P * (d / m) * 4 + (size_t)dev::XNW * dev::RING * 12;
+}
+
+void launch_scan_exact_kind(const ListScanArgs &la, int kind, int grid, hipStream_t st) {
+    if (!exact_scan_supported(la.d, la.m, la.P, 1) || la.chunk % 64 != 0)
+        fail(HANN_ERR_UNSUPPORTED, "exact list scan: unsupported shape");
+    const size_t shm = exact_scan_lds_bytes(la.d, la.m, la.P);
+#define HANN_EXACT(DD, SS, KK)                                                                               \
+    do {                                                                                                     \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_exact<DD, SS, KK>,                            \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                 \
+        hipLaunchKernelGGL((dev::k_scan_exact<DD, SS, KK>), dim3((unsigned)grid), dim3(64 * dev::XNW), shm,  \
+                           st, la);                                                                          \
+        HANN_LAUNCH_CHECK();                                                                                 \
+    } while (0)
+#define HANN_EXACT_K(DD, SS)                                                        \
+    do {                                                                            \
+        if (kind == dev::KIND_SAMPLE) HANN_EXACT(DD, SS, dev::KIND_SAMPLE);         \
+        else HANN_EXACT(DD, SS, dev::KIND_FILTER);                                  \
+    } while (0)
+    const int s = la.d / la.m;
+    if (la.d == 64 && s == 4) HANN_EXACT_K(64, 4);
+    else if (la.d == 64) HANN_EXACT_K(64, 8);
+    else if (la.d == 96 && s == 4) HANN_EXACT_K(96, 4);
+    else if (la.d == 96) HANN_EXACT_K(96, 8);
+    else if (la.d == 128 && s == 4) HANN_EXACT_K(128, 4);
+    else HANN_EXACT_K(128, 8);
+#undef HANN_EXACT_K
+#undef HANN_EXACT
+}
+
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,
                   const int64_t *qsamp_slots, uint32_t *theta, hipStream_t st) {
     if (nq == 0) return;
@@ -1222,10 +1443,10 @@ size_t select_records_lds_bytes() {
 
 void launch_select_records(const ScanArgs &a, int64_t nq, const uint64_t *rec, const uint32_t *qcnt, uint32_t capq,
                            const int64_t *qtot, int64_t *out_pos, int32_t *out_cnt, int out_stride, int32_t *ovf,
-                           int32_t *ovf_cnt, hipStream_t st) {
+                           int32_t *ovf_cnt, const float *q_scale, hipStream_t st) {
     if (nq == 0) return;
     hipLaunchKernelGGL(dev::k_select_records, dim3((unsigned)nq), dim3(256), 0, st, a, nq, rec, qcnt, capq, qtot,
-                       out_pos, out_cnt, out_stride, ovf, ovf_cnt);
+                       out_pos, out_cnt, out_stride, ovf, ovf_cnt, q_scale);
     HANN_LAUNCH_CHECK();
 }
 

@@ -327,7 +327,9 @@ int hann_pqivf_scan_kernel_times(hann_pqivf *idx, double *filter_ms, double *sam
 /* The scan the last search on this index ran (build extension): a
  * HANN_SCAN_* mode (the list modes only when the list-major scan ran; it
  * falls back to the FP32 table scan for shapes it does not cover) and
- * whether the persistent table scan ran (fused). */
+ * fused: 1 when the persistent table scan ran, 2 when the exact mode ran
+ * the list-major exact scan (a rerank, d in {64, 96, 128}, d / m in {4, 8}),
+ * 0 otherwise. */
 int hann_pqivf_last_scan(hann_pqivf *idx, int *mode, int *fused);
 
 /* Bulk export of the clustered entries in list order (CSR): list_off
