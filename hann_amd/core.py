@@ -141,6 +141,42 @@ def batch_topk(metric, queries, base, k, precision=_lib.PREC_FP32, overfetch=0):
     return ids, dist
 
 
+def ExhaustiveFallback(metric, query, node_ids, node_vecs, k, candidate_ids=()):
+    """The exhaustive fallback of HNSW's Search (hnsw/index.go:1198-1319) on
+    the GPU batch top-k: the k nearest nodes by metric.Rank that are not
+    already graph candidates, from the nodes taken in ascending id order.
+
+    Returns (ids, ranks): at most k ids with their rank values (FromRank not
+    applied: the reference merges them with the graph candidates and sorts
+    by (rank, id) before FromRank), ordered by (rank, id). The reference
+    keeps them in per-worker heaps whose choice among equal ranks at the
+    k-th place is unspecified; here equal ranks go by id. Rows of the base
+    are the nodes in ascending id order, so the top-k's (rank, row) order is
+    (rank, id): the k + |candidates| best rows hold the k best
+    non-candidates."""
+    q = _f32(query).reshape(1, -1)
+    ids = np.ascontiguousarray(node_ids, np.int64)
+    vecs = _f32(node_vecs)
+    if vecs.ndim != 2 or vecs.shape[0] != ids.size or vecs.shape[1] != q.shape[1]:
+        raise HannError(_lib.HANN_ERR_INVALID_ARG, "node vectors must be [len(node_ids)][dim]")
+    order = np.argsort(ids, kind="stable")
+    ids_sorted, rows = ids[order], np.ascontiguousarray(vecs[order])
+    cand = np.asarray(list(candidate_ids), np.int64)
+    n = ids.size - int(np.isin(cand, ids).sum()) if cand.size else ids.size
+    if k <= 0 or n <= 0:
+        return np.zeros(0, np.int64), np.zeros(0, np.float64)
+    kk = min(ids.size, k + cand.size)
+    # Rank: the metric's rank metric (squared Euclidean for Euclidean,
+    # core/metric.go:155-158), so the values are ranks
+    rank_metric = {_lib.METRIC_EUCLIDEAN: SquaredEuclidean}.get(metric.id, metric)
+    rid, rank = batch_topk(rank_metric, q, rows, kk)
+    rid, rank = rid[0], rank[0]
+    got = ids_sorted[rid[rid >= 0]]
+    rank = rank[rid >= 0]
+    keep = ~np.isin(got, cand) if cand.size else np.ones(got.size, bool)
+    return got[keep][:k], rank[keep][:k]
+
+
 def NormalizeVector(vec):
     """core/vector_ops.go:16-22: scale a float32 array to unit norm in place
     (a zero vector is left as it is; an empty one is a no-op)."""
