@@ -699,18 +699,24 @@ void PQIVF::gob_decode(const void *buf, size_t len) {
             if ((int64_t)E.vec.size() != ddim)
                 fail(HANN_ERR_FORMAT, strf("entry %lld has %zu values, expected %lld", (long long)E.id, E.vec.size(),
                                            (long long)ddim));
-            if (!E.codes.empty() && (int64_t)E.codes.size() != dm)
-                fail(HANN_ERR_FORMAT, strf("entry %lld has %zu codes, expected %lld", (long long)E.id, E.codes.size(),
-                                           (long long)dm));
             L.ids[r] = E.id;
             std::copy(E.vec.begin(), E.vec.end(), L.vecs.begin() + r * (size_t)ddim);
-            L.has_codes[r] = E.codes.empty() ? 0 : 1;
+            // Search decodes an entry's codes only when it holds m of them
+            // and every one lies in its codebook; otherwise it ranks the
+            // entry by its vector (pqivf/index.go:906-928, decodePQCode
+            // 678-690). Such entries are kept without codes. A negative code
+            // passes decodePQCode's bound check and then indexes out of
+            // range, so a file holding one is rejected.
+            bool coded = (int64_t)E.codes.size() == dm && have_cb;
             for (size_t j = 0; j < E.codes.size(); ++j) {
-                if (E.codes[j] < 0 || E.codes[j] >= S.cb_len[j])
-                    fail(HANN_ERR_FORMAT, strf("entry %lld has code %lld outside codebook %zu", (long long)E.id,
-                                               (long long)E.codes[j], j));
-                L.codes[r * (size_t)dm + j] = (int32_t)E.codes[j];
+                if (E.codes[j] < 0)
+                    fail(HANN_ERR_FORMAT, strf("entry %lld has negative code %lld", (long long)E.id,
+                                               (long long)E.codes[j]));
+                if (j >= (size_t)dm || E.codes[j] >= S.cb_len[j]) coded = false;
             }
+            L.has_codes[r] = coded ? 1 : 0;
+            if (coded)
+                for (size_t j = 0; j < E.codes.size(); ++j) L.codes[r * (size_t)dm + j] = (int32_t)E.codes[j];
         }
     }
     // idToCluster rebuilt from the lists (pqivf/index.go:1076-1081)

@@ -54,3 +54,29 @@ def gob_fixture():
 
 def bits_to_f32(a):
     return np.array(a, np.uint32).view(np.float32)
+
+
+def gob_code_byte(raw, load, codes_of):
+    """The offset of a byte of tests/golden/index_v1.gob that holds one
+    entry's PQ code: codes there are 0..3, each one byte (gob writes int v >= 0
+    as the unsigned 2v), so setting such a byte to 0x0A makes that code 5,
+    past its 4-row codebook. Found by trial: the first offset whose change
+    loads and leaves exactly one entry without codes and every other entry's
+    codes as they were. load(bytes) -> index or raises; codes_of(index) ->
+    {list: [[codes]...]} with rows of -1 for entries without codes."""
+    base = codes_of(load(bytes(raw)))
+    for p in range(len(raw)):
+        if raw[p] not in (0, 2, 4, 6):
+            continue
+        b = bytearray(raw)
+        b[p] = 0x0A
+        try:
+            got = codes_of(load(bytes(b)))
+        except Exception:
+            continue
+        diff = [(c, r) for c in base for r in range(len(base[c])) if got[c][r] != base[c][r]]
+        if len(diff) == 1:
+            c, r = diff[0]
+            if all(v == -1 for v in got[c][r]):
+                return p, c, r
+    raise AssertionError("no code byte found")

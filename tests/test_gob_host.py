@@ -246,3 +246,34 @@ def test_decode_rejects_what_the_device_index_cannot_hold():
     with pytest.raises(HannError, match="coarse centroid of 8 values in an index of dimension 4") as e:
         pqivf.New(8).GobDecode(bytes(good))
     assert e.value.code == _lib.HANN_ERR_FORMAT
+
+
+def _load_raw(raw):
+    idx = pqivf.New(8, pqivf.WithCoarseK(3), pqivf.WithNumSubquantizers(2), pqivf.WithPQK(4),
+                    pqivf.WithKMeansIters(5))
+    idx.Load(io.BytesIO(raw))
+    return idx
+
+
+def _codes_of(idx):
+    return {c: v.tolist() for c, v in pqivf.InvertedCodes(idx).items()}
+
+
+def test_uncodable_entry_loads_without_codes():
+    """Search decodes an entry's codes only when each lies in its codebook
+    (pqivf/index.go:906-928, decodePQCode 678-690); otherwise it ranks the
+    entry by its vector. A file whose entry holds a code past its codebook
+    loads, with that entry kept without codes and everything else as in the
+    file; a negative code, which the reference indexes out of range with, is
+    rejected."""
+    from conftest import gob_code_byte
+    raw = gob_file()
+    p, c, r = gob_code_byte(raw, _load_raw, _codes_of)
+    b = bytearray(raw)
+    b[p] = 0x0A
+    idx = _load_raw(bytes(b))
+    assert idx.Stats().Count == 29
+    assert pqivf.InvertedIDs(idx) == pqivf.InvertedIDs(_load_raw(raw))
+    b[p] = 0x01   # gob's int -1
+    with pytest.raises(HannError, match="negative code"):
+        _load_raw(bytes(b))

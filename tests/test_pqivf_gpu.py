@@ -510,6 +510,46 @@ def test_reference_trained_file_loads_and_searches(hann, oracle, gob_fixture):
         assert pq.EncodeVector(idx, ref.vecs[j], int(cl[j])) == e["Codes"]
 
 
+def test_file_with_uncodable_entry_searches_like_oracle(hann, oracle):
+    """Go hann's index_v1.gob with one entry's code set past its codebook:
+    the entry loads without codes and Search ranks it by its vector, as
+    pqivf/index.go:906-928 does, for queries near it and elsewhere."""
+    from conftest import gob_code_byte
+    pq = P()
+
+    def load(raw):
+        ix = pq.New(8, pq.WithCoarseK(3), pq.WithNumSubquantizers(2), pq.WithPQK(4), pq.WithKMeansIters(5))
+        ix.Load(io.BytesIO(raw))
+        return ix
+
+    def codes_of(ix):
+        return {c: v.tolist() for c, v in pq.InvertedCodes(ix).items()}
+    with open(os.path.join(GOLDEN, "index_v1.gob"), "rb") as fh:
+        raw = fh.read()
+    p, c, r = gob_code_byte(raw, load, codes_of)
+    b = bytearray(raw)
+    b[p] = 0x0A
+    idx = load(bytes(b))
+    off, ids, codes, vecs = idx.export_lists(with_vectors=True)
+    cb, cbl = pq.Codebooks(idx)
+    ix = oracle.CSRIndex(off, ids, codes, vecs, pq.CoarseCentroids(idx), cb, cbl)
+    assert int(ix.has_codes.sum()) == 28
+    rng = np.random.default_rng(12)
+    e = int(off[c] + r)
+    qs = np.concatenate([vecs[e:e + 1] + np.float32(0.01), vecs[rng.integers(0, 29, 5)]]).astype(np.float32)
+    qs += (rng.standard_normal(qs.shape) * 0.05).astype(np.float32)
+    got_ids, got_d, cnt = idx.search_batch(qs, 5, nprobe=3, scan_mode=_lib_mod().SCAN_EXACT)
+    want_ids, want_d, wcnt = oracle.search_many(ix, qs, 5, 3)
+    assert np.array_equal(cnt, wcnt)
+    assert np.array_equal(got_ids, want_ids) and np.array_equal(got_d, want_d)
+    assert ids[e] in got_ids[0].tolist()
+
+
+def _lib_mod():
+    from hann_amd import _lib
+    return _lib
+
+
 # ------------------------------------------------------ BF16 assignment --
 @pytest.mark.parametrize("n,d,k,spread", [(5000, 128, 256, 5.0), (3000, 96, 1000, 1.0), (2000, 64, 64, 20.0),
                                           (700, 33, 70, 3.0), (4096, 128, 4096, 5.0)])
