@@ -655,6 +655,7 @@ def main():
         # batches, MALL flushed before each); throughput and latency with
         # a.streams streams
         el1, lat1, st, kern = timed(mode, chosen_r[name], fused, coarse, True, 1, flush_between=True)
+        tuned = idx.last_tune()
         el, lat, _, _ = timed(mode, chosen_r[name], fused, coarse, False, a.streams, a.graphs)
         qps = a.steps * a.batch / el
         calls = max(st[3], 1)
@@ -662,6 +663,7 @@ def main():
                      "p99_ms": float(np.percentile(lat, 99)), "stages": st, "rerank": chosen_r[name],
                      "recall": rec_ho[name][0], "recall_se": rec_ho[name][1], "recall_sel": rec_sel[name],
                      "err": rec_ho[name][2], "kernel_ms": kern[0] / calls, "sample_ms": kern[1] / calls,
+                     "tuned": tuned,
                      "p50_ms_one_stream": float(np.percentile(lat1, 50)),
                      "p99_ms_one_stream": float(np.percentile(lat1, 99))}
         log(f"{name}: {qps:.0f} q/s ({a.streams} streams, {len(idxs)} index copies), {1000 * el / a.steps:.2f} "
@@ -807,7 +809,7 @@ def main():
         "candidates": {n: {"qps": round(res[n]["qps"], 1), "recall_at_10": round(res[n]["recall"], 4),
                            "recall_at_10_se": round(res[n]["recall_se"], 4), "rerank": res[n]["rerank"],
                            "p99_ms": round(res[n]["p99_ms"], 3), "scan_kernel_ms": round(res[n]["kernel_ms"], 4),
-                           "sample_pass_ms": round(res[n]["sample_ms"], 4),
+                           "sample_pass_ms": round(res[n]["sample_ms"], 4), "autotuner": res[n]["tuned"],
                            "stage_ms_per_step": stages(res[n])} for n in cands},
         "plain": {"qps": round(pl["qps"], 1), "recall_at_10": round(pl["recall"], 4),
                   "recall_at_10_se": round(pl["recall_se"], 4), "rerank": pl["rerank"],

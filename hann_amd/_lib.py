@@ -56,7 +56,7 @@ class PQIVFOptions(ctypes.Structure):
 
 
 class SearchParams(ctypes.Structure):
-    _fields_ = [("scan_mode", i32), ("coarse_precision", i32), ("rerank", i32), ("fused", i32)]
+    _fields_ = [("scan_mode", i32), ("coarse_precision", i32), ("rerank", i32), ("fused", i32), ("list_config", i32)]
 
 
 class ServerOptions(ctypes.Structure):
@@ -134,6 +134,7 @@ SIGNATURES = {
     "hann_pqivf_stage_times": (cint, [vp, c_double_p, c_double_p, c_double_p, c_int64_p]),
     "hann_pqivf_scan_kernel_time": (cint, [vp, c_double_p]),
     "hann_pqivf_scan_kernel_times": (cint, [vp, c_double_p, c_double_p]),
+    "hann_pqivf_last_tune": (cint, [vp, ctypes.c_char_p, ctypes.c_size_t]),
     "hann_pqivf_last_scan": (cint, [vp, c_int32_p, c_int32_p]),
     "hann_pqivf_export_lists": (cint, [vp, c_int64_p, c_int64_p, c_int32_p, c_float_p]),
     "hann_server_default_options": (None, [ctypes.POINTER(ServerOptions)]),

@@ -1089,8 +1089,16 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     last_scan_mode = list_scan ? p.scan_mode : mode;
     last_scan_fused = exact_list ? 2 : ((fused && !list_scan) ? 1 : 0);
     auto scan = [&](const ScanArgs &args) {
-        if (list_scan || exact_list)
-            run_list_scan(args, nq, exact_list ? 2 : x3, np, probe_rank, st, w, lplan);
+        if (list_scan || exact_list) {
+            const int xm = exact_list ? 2 : x3;
+            const int nc = xm == 2 ? (int)(sizeof(kExactTunes) / sizeof(kExactTunes[0]))
+                                   : (int)(sizeof(kListTunes) / sizeof(kListTunes[0]));
+            if (p.list_config < 0 || p.list_config > nc)
+                fail(HANN_ERR_INVALID_ARG, strf("list_config %d is outside 0 .. %d", p.list_config, nc));
+            const ListTune t = p.list_config > 0 ? (xm == 2 ? kExactTunes : kListTunes)[p.list_config - 1]
+                                                 : list_tuned(args, nq, xm, np, probe_rank, st, w, lplan);
+            run_list_scan(args, nq, xm, np, probe_rank, st, w, lplan, t);
+        }
         else if (fused)
             run_fused_scan(args, nq, np, probe_rank, st, w);
         else {
@@ -1164,13 +1172,16 @@ void PQIVF::run_fused_scan(const ScanArgs &a, int64_t nq, int maxp, const double
         wpq = it->second;
     } else {
         float best = 1e30f;
+        // trial runs count no brute-force fallbacks: the run below does
+        ScanArgs ta = a;
+        ta.fb_counter = nullptr;
         for (int wv : {2, 4, 8}) {
             if (!fits(wv)) continue;
             hipEvent_t e0, e1;
             HANN_HIP(hipEventCreate(&e0));
             HANN_HIP(hipEventCreate(&e1));
             HANN_HIP(hipEventRecord(e0, st));
-            launch_scan_block(a, f, nq, wv, kp, code_bytes, maxp, grid_for(wv), st);
+            launch_scan_block(ta, f, nq, wv, kp, code_bytes, maxp, grid_for(wv), st);
             HANN_HIP(hipEventRecord(e1, st));
             HANN_HIP(hipEventSynchronize(e1));
             float ms = 0;
@@ -1237,16 +1248,13 @@ struct ListArena {
 
 void PQIVF::refresh_shape() const {
     if (shape_epoch.load(std::memory_order_acquire) == mirror_epoch) return;
-    const int64_t chunk = int64_t(1) << kListItems.ch, chunk_x = int64_t(1) << kExactItems.ch;
-    int64_t max_len = 0, chunks_all = 0, chunks_x = 0;
+    int64_t max_len = 0, chunks[6] = {0, 0, 0, 0, 0, 0};
     for (int64_t v : dev_len) {
         max_len = std::max(max_len, v);
-        chunks_all += ceil_div(v, chunk);
-        chunks_x += ceil_div(v, chunk_x);
+        for (int c = 0; c < 6; ++c) chunks[c] += ceil_div(v, int64_t(1) << (6 + c));
     }
     shape_max_len.store(max_len, std::memory_order_relaxed);
-    shape_chunks.store(chunks_all, std::memory_order_relaxed);
-    shape_chunks_x.store(chunks_x, std::memory_order_relaxed);
+    for (int c = 0; c < 6; ++c) shape_chunks[c].store(chunks[c], std::memory_order_relaxed);
     shape_epoch.store(mirror_epoch, std::memory_order_release);
 }
 
@@ -1255,10 +1263,11 @@ PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk, bool exact) const
     // work items: a list with p pairs and n entries has ceil(p / G) *
     // ceil(n / C) of them (G queries, C entries an item), at most
     // (p / G + 1) * ceil(n / C)
-    const ItemShape sh = exact ? kExactItems : kListItems;
+    // sized for the smallest chunk and query group the tuner may pick
+    const ItemShape sh = exact ? kExactItemsMin : kListItemsMin;
     const int64_t chunk = int64_t(1) << sh.ch, group = int64_t(1) << sh.qg;
     const int64_t max_len = shape_max_len.load(std::memory_order_relaxed);
-    const int64_t chunks_all = (exact ? shape_chunks_x : shape_chunks).load(std::memory_order_relaxed);
+    const int64_t chunks_all = shape_chunks[sh.ch - 6].load(std::memory_order_relaxed);
     L.exact = exact;
     L.pairs_max = nq * (int64_t)np;
     L.items_max = ceil_div(L.pairs_max, group) * ceil_div(max_len, chunk) + chunks_all + 1;
@@ -1273,8 +1282,50 @@ PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk, bool exact) const
     return L;
 }
 
+ListTune PQIVF::list_tuned(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
+                           Workspace &w, const ListPlan &L) {
+    const uint64_t key = ((uint64_t)x3 << 40) ^ ((uint64_t)a.out_stride << 8) ^
+                         (uint64_t)(maxp >= 64 ? 2 : maxp >= 16 ? 1 : 0);
+    auto it = tuned_list.find(key);
+    if (it != tuned_list.end()) return it->second;
+    const ListTune *cands = x3 == 2 ? kExactTunes : kListTunes;
+    const int nc = x3 == 2 ? (int)(sizeof(kExactTunes) / sizeof(kExactTunes[0]))
+                           : (int)(sizeof(kListTunes) / sizeof(kListTunes[0]));
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HANN_HIP(hipStreamIsCapturing(st, &cs));
+    if (cs != hipStreamCaptureStatusNone) return cands[0];
+    // trial runs count no brute-force fallbacks: the caller's run does
+    ScanArgs ta = a;
+    ta.fb_counter = nullptr;
+    // one untimed run first (the kernels' first launches load code)
+    run_list_scan(ta, nq, x3, maxp, probe_rank, st, w, L, cands[0]);
+    float best = 1e30f;
+    int bi = 0;
+    for (int i = 0; i < nc; ++i) {
+        hipEvent_t e0, e1;
+        HANN_HIP(hipEventCreate(&e0));
+        HANN_HIP(hipEventCreate(&e1));
+        HANN_HIP(hipEventRecord(e0, st));
+        run_list_scan(ta, nq, x3, maxp, probe_rank, st, w, L, cands[i]);
+        HANN_HIP(hipEventRecord(e1, st));
+        HANN_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        HANN_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (ms < best) {
+            best = ms;
+            bi = i;
+        }
+    }
+    tuned_list[key] = cands[bi];
+    last_tune = strf("list-major %s scan tuned: %d queries x %d entries per item, %d partitions (%.3f ms)",
+                     x3 == 2 ? "exact" : "MFMA", 1 << cands[bi].sh.qg, 1 << cands[bi].sh.ch, cands[bi].parts, best);
+    return cands[bi];
+}
+
 void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank,
-                          hipStream_t st, Workspace &w, const ListPlan &L) {
+                          hipStream_t st, Workspace &w, const ListPlan &L, const ListTune &t) {
     const int K = (int)lists.size();
     const int selk = a.out_stride;
     const int kp = a.cap / 2;
@@ -1306,7 +1357,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     float *pkey = reinterpret_cast<float *>(at(A.o_pkey));
     const bool exact = x3 == 2;
     float *q_scale = exact ? reinterpret_cast<float *>(at(A.o_qscale)) : nullptr;   // key scale per query
-    const ItemShape sh = exact ? kExactItems : kListItems;
+    const ItemShape sh = t.sh;
     if (num_cus == 0) {
         int dev = 0;
         HANN_HIP(hipGetDevice(&dev));
@@ -1350,7 +1401,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.items = items;
     la.n_items = totals + 1;
     la.item_counter = parts;
-    la.n_parts = std::min(kListScanParts, grid);
+    la.n_parts = std::min(std::min(t.parts, kListScanParts), grid);
     la.dist = srow;
     la.dist_bytes = (uint32_t)(L.sample_slots * 2);
     la.theta = theta;
@@ -2185,6 +2236,17 @@ int hann_pqivf_scan_kernel_times(hann_pqivf *idx, double *filter_ms, double *sam
     *filter_ms = x.t_kernel_ms;
     *sample_ms = x.t_sample_ms;
     x.t_kernel_ms = x.t_sample_ms = 0;
+    HANN_API_END
+}
+
+int hann_pqivf_last_tune(hann_pqivf *idx, char *buf, size_t len) {
+    HANN_API_BEGIN
+    IDX_CHECK();
+    std::unique_lock<std::shared_mutex> lk(x.mu);
+    if (!buf || len == 0) fail(HANN_ERR_INVALID_ARG, "buffer is null or empty");
+    const size_t n = std::min(len - 1, x.last_tune.size());
+    std::memcpy(buf, x.last_tune.data(), n);
+    buf[n] = 0;
     HANN_API_END
 }
 

@@ -15,6 +15,7 @@
 #include "ivf.h"
 #include "kmeans.h"
 #include "runtime.h"
+#include "scan_list.h"
 
 namespace hann {
 
@@ -165,6 +166,10 @@ struct PQIVF {
     Context coarse_ctx;
     BF16Base coarse_bf16;
     std::map<uint64_t, int> tuned_wpq;   // persistent scan waves per query; key: (cap, maxp bucket)
+    // list-major scans' work items and partitions; key: (scan, selection
+    // size, nprobe bucket). Searches hold the index lock exclusively, so the
+    // maps need no lock of their own.
+    std::map<uint64_t, ListTune> tuned_list;
     int num_cus = 0;
     std::string last_tune;
 
@@ -213,7 +218,8 @@ struct PQIVF {
     // mirror when it changed (callers hold the index lock); readable without
     // it, then as of the last refresh
     mutable std::atomic<uint64_t> shape_epoch{~0ull};
-    mutable std::atomic<int64_t> shape_max_len{0}, shape_chunks{0}, shape_chunks_x{0};
+    // Σ ceil(len / 2^c) over the device lists for c = 6 .. 11
+    mutable std::atomic<int64_t> shape_max_len{0}, shape_chunks[6]{};
     void refresh_shape() const;
     // reasons the list-major scan was declined, each said once on stderr
     std::mutex note_mu;
@@ -227,7 +233,13 @@ struct PQIVF {
     // list-major scan (x3: 0 BF16, 1 split BF16 MFMA, 2 exact FP32); a.out_*
     // are the candidate rows of the rerank
     void run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
-                       Workspace &w, const ListPlan &L);
+                       Workspace &w, const ListPlan &L, const ListTune &t);
+    // the configuration of the list-major scan for this batch shape: tuned
+    // on its first batch (every candidate timed once on the search stream,
+    // the fastest kept; results do not depend on it), the default while the
+    // stream is being captured
+    ListTune list_tuned(const ScanArgs &a, int64_t nq, int x3, int maxp, const double *probe_rank, hipStream_t st,
+                        Workspace &w, const ListPlan &L);
     // stage timers (HIP events on the search stream), accumulated when enabled
     bool profile = false;
     double t_coarse_ms = 0, t_scan_ms = 0, t_rerank_ms = 0;

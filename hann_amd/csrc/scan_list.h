@@ -33,9 +33,8 @@ struct ListScanArgs {
                                  // order they were written; the first capq of them are kept
     uint32_t *qcnt;              // [nq] records written per query (zeroed before the launch)
     uint32_t capq;               // record room per query
-    // the exact scan
-    const float *coarse;         // [K][d] coarse centroids
-    int chunk;                   // entries per work item (a multiple of 64)
+    const float *coarse;         // [K][d] coarse centroids (the exact scan)
+    int chunk;                   // entries per work item (a multiple of 64; at most 2,048)
 };
 
 // work items: up to 2^qg queries of a list times a chunk of up to 2^ch of
@@ -44,8 +43,20 @@ struct ItemShape {
     int qg, ch;
 };
 // the MFMA scan's items (32 queries x 2,048 entries) and the exact scan's
+// (8 x 256) when untuned
 constexpr ItemShape kListItems{5, 11};
 constexpr ItemShape kExactItems{3, 8};
+// a list-major scan's configuration: its work items and item partitions
+struct ListTune {
+    ItemShape sh;
+    int parts;
+};
+// the configurations the autotuner times (the first is the default); the
+// smallest chunk and query group among them size the batch's item buffers
+constexpr ListTune kListTunes[] = {{{5, 11}, 256}, {{5, 10}, 256}, {{5, 11}, 128}, {{5, 10}, 128}};
+constexpr ListTune kExactTunes[] = {{{3, 8}, 256}, {{2, 8}, 256}, {{3, 7}, 256}, {{3, 9}, 256}, {{4, 8}, 256}};
+constexpr ItemShape kListItemsMin{5, 10};
+constexpr ItemShape kExactItemsMin{2, 7};
 // entries of a list per work item of the list-major scan
 int list_scan_chunk();
 // work-item partitions (counters) of the list-major scan

@@ -556,7 +556,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         S.e0 = (int)la.list_start[it.x];
         const int n = (int)la.list_len[it.x];
         S.eb = it.w;
-        S.ee = n < it.w + LCH ? n : it.w + LCH;
+        S.ee = n < it.w + la.chunk ? n : it.w + la.chunk;
         // groups of four entries starting below slot_end are stored (the
         // last chunk pads its query runs to a multiple of 8 slots)
         S.slot_end = S.ee == n ? (int)row_slots(n) : S.ee;

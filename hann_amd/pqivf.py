@@ -186,15 +186,16 @@ class Index:
         return [Neighbor(int(ids[i]), float(dist[i])) for i in range(cnt.value)]
 
     def search_batch(self, queries, k, nprobe=None, scan_mode=_lib.SCAN_EXACT, rerank=0,
-                     coarse_precision=_lib.PREC_FP32, fused=0):
-        """Batched search (build extension): k and nprobe scalars or per-query."""
+                     coarse_precision=_lib.PREC_FP32, fused=0, list_config=0):
+        """Batched search (build extension): k and nprobe scalars or per-query;
+        list_config pins a list-major scan configuration (0: tuned)."""
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
         ks = np.ascontiguousarray(np.broadcast_to(np.asarray(k, np.int32), (nq,)), np.int32)
         kmax = int(ks.max()) if nq else 1
         nps = None if nprobe is None else np.ascontiguousarray(np.broadcast_to(np.asarray(nprobe, np.int32), (nq,)),
                                                                 np.int32)
-        p = _lib.SearchParams(scan_mode, coarse_precision, rerank, fused)
+        p = _lib.SearchParams(scan_mode, coarse_precision, rerank, fused, list_config)
         ids = np.full((nq, kmax), -1, np.int64)
         dist = np.full((nq, kmax), np.inf, np.float64)
         cnt = np.zeros(nq, np.int32)
@@ -279,6 +280,12 @@ class Index:
         v = ctypes.c_double()
         _lib.check(self._lib.hann_pqivf_scan_kernel_time(self._h, ctypes.byref(v)))
         return v.value
+
+    def last_tune(self):
+        """The autotuner's last choice on this index, as text."""
+        buf = ctypes.create_string_buffer(512)
+        _lib.check(self._lib.hann_pqivf_last_tune(self._h, buf, 512))
+        return buf.value.decode()
 
     def scan_kernel_times(self):
         """(filter_ms, sample_ms): milliseconds in the main scan kernel (the

@@ -222,6 +222,9 @@ typedef struct {
     int32_t fused;            /* 1 (ADC modes): the persistent scan with one FP32 table per
                                  query and a per-entry list term; scan_mode ADC_F32 and
                                  ADC_F16 both take it */
+    int32_t list_config;      /* the list-major scans' work items and partitions: 0 the
+                                 autotuner's choice for the batch shape, i > 0 the tuner's
+                                 i-th candidate (results are identical for every choice) */
 } hann_search_params;
 
 void hann_search_default_params(hann_search_params *p);
@@ -331,6 +334,13 @@ int hann_pqivf_scan_kernel_times(hann_pqivf *idx, double *filter_ms, double *sam
  * the list-major exact scan (a rerank, d in {64, 96, 128}, d / m in {4, 8}),
  * 0 otherwise. */
 int hann_pqivf_last_scan(hann_pqivf *idx, int *mode, int *fused);
+/* The autotuner's last choice on this index as text (build extension): the
+ * persistent table scan's waves per query, or a list-major scan's work items
+ * and partitions, with the time that won; empty before any tuning. The
+ * tuners time each candidate once on the first batch of a shape (selection
+ * size k * R, nprobe bucket) and keep the fastest; results do not depend on
+ * the choice. */
+int hann_pqivf_last_tune(hann_pqivf *idx, char *buf, size_t len);
 
 /* Bulk export of the clustered entries in list order (CSR): list_off
  * [k+1], ids [N], codes [N][m] (rows of -1: entry without codes), vecs

@@ -89,22 +89,31 @@ def test_exact_list_scan_bf16_coarse_identical(hann, oracle):
     _same(a, oracle.search_rerank_many(ix, qs, ks, nps, 3, nthreads=8))
 
 
-def test_exact_list_scan_fallback_counter(hann, oracle):
+@pytest.mark.parametrize("path", ["exact", "table", "bf16"])
+def test_fallback_counter_counts_each_search_once(hann, oracle, path):
     """A query whose probed lists hold fewer than k entries takes the
-    reference's brute-force fallback (every list), counter included, through
-    the per-query exact scan; the others stay on the list scan."""
+    reference's brute-force fallback (every list), through the per-query
+    exact scan (plain path) or the table scan; the counter counts it once
+    per search, also on a batch shape's first search, where the autotuner
+    runs the scan once per candidate. The plain path's results equal the
+    oracle's."""
     from hann_amd import _lib
     rng = np.random.default_rng(3)
     x = ((rng.standard_normal((4, 64)) * 20)[np.arange(400) % 4] + rng.standard_normal((400, 64))).astype(np.float32)
-    idx = _index(x, 4, 8, pq_k=16, iters=5)
+    idx = _index(x, 4, 16, pq_k=16, iters=5)
     ix = _csr(oracle, idx)
-    before = idx.Stats().FallbackSearches
     qs = x[:6] + np.float32(0.01)
     ks = np.array([150, 10, 150, 10, 10, 10], np.int32)
-    got = idx.search_batch(qs, ks, nprobe=1, scan_mode=_lib.SCAN_EXACT, rerank=2)
-    assert idx.last_scan() == (_lib.SCAN_EXACT, 2)
-    assert idx.Stats().FallbackSearches == before + 2
-    _same(got, oracle.search_rerank_many(ix, qs, ks, np.ones(6, np.int32), 2, nthreads=8))
+    kw = {"exact": dict(scan_mode=_lib.SCAN_EXACT, rerank=2),
+          "table": dict(scan_mode=_lib.SCAN_ADC_F32, rerank=2, fused=1),
+          "bf16": dict(scan_mode=_lib.SCAN_LIST_BF16, rerank=2, fused=1)}[path]
+    for rep in range(2):
+        before = idx.Stats().FallbackSearches
+        got = idx.search_batch(qs, ks, nprobe=1, **kw)
+        assert idx.Stats().FallbackSearches == before + 2, rep
+        if path == "exact":
+            assert idx.last_scan() == (_lib.SCAN_EXACT, 2)
+            _same(got, oracle.search_rerank_many(ix, qs, ks, np.ones(6, np.int32), 2, nthreads=8))
 
 
 def test_exact_list_scan_record_overflow(hann, oracle):
@@ -153,3 +162,36 @@ def test_record_select_radix_boundary(hann, oracle, mode):
         assert idx.last_scan()[0] == _lib.SCAN_LIST_BF16
         for u, w in zip(a, b):
             assert np.array_equal(u, w)
+
+
+def test_every_tuner_candidate_gives_identical_results(hann, oracle):
+    """The autotuner may pick any of its candidates (work items of other
+    chunk sizes and query groups, other partition counts): each gives the
+    same results — the exact scan's equal the oracle's, and the BF16 scan's
+    are identical across candidates. A search with list_config 0 tunes on
+    its first batch of a shape and reports the choice."""
+    from hann_amd import _lib
+    x, c = _mixture(20000, 96, 64, seed=21)
+    idx = _index(x, 128, 24)
+    ix = _csr(oracle, idx)
+    rng = np.random.default_rng(22)
+    qs = (c[rng.integers(0, 64, 160)] + rng.standard_normal((160, 96))).astype(np.float32)
+    ks = rng.choice(np.array([10, 50, 100], np.int32), 160)
+    nps = rng.choice(np.array([8, 16, 32, 64], np.int32), 160)
+    want = oracle.search_rerank_many(ix, qs, ks, nps, 3, nthreads=8)
+    for cfg in range(1, 6):
+        got = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_EXACT, rerank=3, list_config=cfg)
+        assert idx.last_scan() == (_lib.SCAN_EXACT, 2)
+        _same(got, want)
+    first = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_LIST_BF16, rerank=3, fused=1, list_config=1)
+    for cfg in range(2, 5):
+        got = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_LIST_BF16, rerank=3, fused=1, list_config=cfg)
+        assert idx.last_scan()[0] == _lib.SCAN_LIST_BF16
+        for u, v in zip(first, got):
+            assert np.array_equal(u, v)
+    tuned = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_LIST_BF16, rerank=3, fused=1)
+    assert "list-major MFMA scan tuned" in idx.last_tune()
+    for u, v in zip(first, tuned):
+        assert np.array_equal(u, v)
+    with pytest.raises(Exception, match="list_config"):
+        idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_LIST_BF16, rerank=3, fused=1, list_config=9)
