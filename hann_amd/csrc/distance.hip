@@ -6,6 +6,7 @@
 
 #include "common.h"
 #include "kernels.h"
+#include "scan_list.h"
 
 namespace hann {
 namespace dev {
@@ -370,6 +371,16 @@ static void topk_dispatch(int metric, const float *queries, int64_t nq, const fl
 size_t topk_exact_scratch_bytes(int metric, int64_t nq, int64_t nb, int k) {
     const int kp = kp_for(k);
     if (nq == 0 || kp > 512) return 0;
+    // dim is not known here, and a grow-only workspace keeps the slots of
+    // both paths once batches of different shapes have taken each: both
+    if (topk_rows_supported(metric, nb, 64, k))
+        return topk_rows_scratch_bytes(nq, nb, k) + topk_exact_tiled_scratch_bytes(metric, nq, nb, k);
+    return topk_exact_tiled_scratch_bytes(metric, nq, nb, k);
+}
+
+size_t topk_exact_tiled_scratch_bytes(int metric, int64_t nq, int64_t nb, int k) {
+    const int kp = kp_for(k);
+    if (nq == 0 || kp > 512) return 0;
     int64_t nsplit = 0, rows = 0;
     topk_plan(metric, kp, nq, nb, 65535, nsplit, rows);
     return (size_t)(nq * nsplit * k) * 12 + 512;
@@ -381,6 +392,12 @@ void launch_topk_exact(int metric, const float *queries, int64_t nq, const float
     if (nq == 0) return;
     const int kp = kp_for(k);
     if (kp > 512) fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
+    if (topk_rows_supported(metric, nb, dim, k)) {
+        // bases of up to 65,536 rows (the coarse step's centroids): sample,
+        // bound, filter and select over rows held in registers
+        launch_topk_rows(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st);
+        return;
+    }
     int64_t nsplit = 0, rows = 0;
     topk_plan(metric, kp, nq, nb, 65535, nsplit, rows);
     float *pkey = ws.get<float>(14, (size_t)(nq * nsplit * k));

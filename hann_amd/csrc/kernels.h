@@ -10,8 +10,9 @@ namespace hann {
 // Grow-only device scratch, in numbered slots, owned by one stream's user.
 // Slot map: 0-4 and 10-11 host-facing staging of the entry points, 5-9 the
 // search pipeline, 12-13 the persistent scan's queue, 14-15
-// launch_topk_exact's partial lists, 16 device_min, 17-19 the list-major
-// scan, 20 the batch's query tables.
+// launch_topk_exact's partial lists (14 also its row path's selection), 16
+// device_min, 17-19 the list-major scan, 20 the batch's query tables, 21-23
+// launch_topk_exact's row path.
 struct Workspace {
     DevBuf<char> slots[24];
     template <typename T>
@@ -38,6 +39,8 @@ struct Workspace {
 int kp_for(int k);
 // an upper bound of the workspace bytes launch_topk_exact takes for this shape
 size_t topk_exact_scratch_bytes(int metric, int64_t nq, int64_t nb, int k);
+// the same for its tiled kernel alone
+size_t topk_exact_tiled_scratch_bytes(int metric, int64_t nq, int64_t nb, int k);
 
 // distance.hip
 // normalize n rows of dim floats in place (core/simd_kernels.inc.h:267-315)

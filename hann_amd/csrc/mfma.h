@@ -39,5 +39,9 @@ void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
                             const int32_t *cnt, const int32_t *cand_row, int cap, int k, const int32_t *k_q,
                             int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st);
+// the same over int64 candidate rows (rows[q * cap + i], i < cnt[q])
+void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *base, int64_t nb, const int32_t *cnt,
+                        const int64_t *rows, int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids,
+                        double *out_dist, hipStream_t st);
 
 }  // namespace hann

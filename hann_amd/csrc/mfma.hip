@@ -438,10 +438,11 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
 // Exact reference-order rank of every candidate, then the k best by
 // (rank, row). One wave per query. Queries whose candidate list overflowed
 // write nothing (the host reruns them on the exact path).
+template <typename CT>
 __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict__ queries, int d,
                                                         const float *__restrict__ base,
                                                         const int32_t *__restrict__ cnt,
-                                                        const int32_t *__restrict__ cand_row, int cap, int64_t nb,
+                                                        const CT *__restrict__ cand_row, int cap, int64_t nb,
                                                         int k, const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
                                                         int64_t *__restrict__ out_ids, double *__restrict__ out_dist) {
     extern __shared__ __align__(16) unsigned char s_raw[];
@@ -465,7 +466,7 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
         float key = 0.f;
         int64_t row = 0;
         if (valid) {
-            row = every ? t : cand_row[q * cap + t];
+            row = every ? t : (int64_t)cand_row[q * cap + t];
             key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
         }
         sel.push(key, row, valid);
@@ -571,15 +572,30 @@ void launch_rank_candidates(const float *queries, int64_t nq, int d, const float
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = (size_t)ccap * 12 + 16;
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 (int)shm));
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int32_t>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
-        hipLaunchKernelGGL(dev::k_rank_candidates, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d, base,
-                           cnt + q0, cand_row + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
-                           sqrt_out,
-                           out_ids + q0 * k,
-                           out_dist + q0 * k);
+        hipLaunchKernelGGL(dev::k_rank_candidates<int32_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d,
+                           base, cnt + q0, cand_row + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
+                           sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *base, int64_t nb, const int32_t *cnt,
+                        const int64_t *rows, int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids,
+                        double *out_dist, hipStream_t st) {
+    if (nq == 0) return;
+    const int ccap = 2 * kp_for(k);
+    const size_t shm = (size_t)ccap * 12 + 16;
+    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int64_t>,
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
+        const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
+        hipLaunchKernelGGL(dev::k_rank_candidates<int64_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d,
+                           d, base, cnt + q0, rows + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
+                           sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
     }
 }

@@ -2,6 +2,7 @@
 #pragma once
 
 #include "ivf.h"
+#include "kernels.h"
 
 namespace hann {
 
@@ -107,6 +108,15 @@ size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
 bool exact_scan_supported(int d, int m, int P, int code_bytes);
 size_t exact_scan_lds_bytes(int d, int m, int P);
 void launch_scan_exact_kind(const ListScanArgs &la, int kind, int grid, hipStream_t st);
+// The exact FP32 top-k of dense rows by sample, bound, filter and select
+// (the same passes, records holding the exact rank), then ranked and
+// ordered by (rank, row): identical results to launch_topk_exact's tiled
+// kernel. (Squared) Euclidean, d in {64, 96, 128}, k <= 512, nb <= 65,536 and
+// at least 16 k (the coarse step's centroids). Workspace slots 14, 21-23.
+bool topk_rows_supported(int metric, int64_t nb, int dim, int k);
+size_t topk_rows_scratch_bytes(int64_t nq, int64_t nb, int k);
+void launch_topk_rows(int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
+                      int64_t *out_ids, double *out_dist, Workspace &ws, hipStream_t st);
 // per query, the bound theta of the filter pass: the (k R)-th smallest key of
 // its sample (qsamp_slots keys from row_off[q] of the sample rows)
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,

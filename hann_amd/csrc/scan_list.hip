@@ -34,6 +34,7 @@
 // FP32. A query whose records overflow its room (8 selk, at least 256) is
 // ranked by the table scan instead.
 #include "common.h"
+#include "mfma.h"
 #include "scan_list.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
@@ -846,11 +847,33 @@ __device__ __forceinline__ cfloat *as_const(const float *p) {
     return (cfloat *)(const __attribute__((address_space(4))) void *)(uintptr_t)p;
 }
 
+// squared distance of a lane's row x and a wave-uniform query q in the
+// reference's AVX2 order (32 slots, d = q - x rounded, fused d * d + acc,
+// (acc0 + acc1) + (acc2 + acc3) per lane, the horizontal sum); D a multiple
+// of 32, q's values scalar operands
+template <int D>
+__device__ __forceinline__ float sqdiff_scalar_q(const float (&x)[D], cfloat *qp) {
+    float acc[32];
+#pragma unroll
+    for (int c = 0; c < 32; ++c) acc[c] = 0.f;
+#pragma unroll
+    for (int blk = 0; blk < D / 32; ++blk) {
+#pragma unroll
+        for (int c = 0; c < 32; ++c) {
+            const float dd = qp[32 * blk + c] - x[32 * blk + c];
+            acc[c] = __builtin_fmaf(dd, dd, acc[c]);
+        }
+    }
+    float s8[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) s8[u] = (acc[u] + acc[8 + u]) + (acc[16 + u] + acc[24 + u]);
+    return hsum8(s8);
+}
+
 template <int D, int S, int KIND>
 __global__ void __launch_bounds__(64 * XNW) k_scan_exact(ListScanArgs la) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     constexpr int M = D / S;
-    constexpr int NB = D / 32;
     static_assert(D % 32 == 0 && (S == 4 || S == 8) && M % 4 == 0, "shape");
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int P = la.P;
@@ -950,22 +973,7 @@ __global__ void __launch_bounds__(64 * XNW) k_scan_exact(ListScanArgs la) {
                 const int qid = __builtin_amdgcn_readlane(vq, jq);
                 const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vsc), jq));
                 const int aux = __builtin_amdgcn_readlane(vaux, jq);
-                cfloat *qp = as_const(la.queries + (int64_t)qid * D);
-                float acc[32];
-#pragma unroll
-                for (int c = 0; c < 32; ++c) acc[c] = 0.f;
-#pragma unroll
-                for (int blk = 0; blk < NB; ++blk) {
-#pragma unroll
-                    for (int c = 0; c < 32; ++c) {
-                        const float dd = qp[32 * blk + c] - x[32 * blk + c];
-                        acc[c] = __builtin_fmaf(dd, dd, acc[c]);
-                    }
-                }
-                float s8[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) s8[u] = (acc[u] + acc[8 + u]) + (acc[16 + u] + acc[24 + u]);
-                const float key = hsum8(s8);
+                const float key = sqdiff_scalar_q<D>(x, as_const(la.queries + (int64_t)qid * D));
                 const float ks = key * sc;
                 if (KIND == KIND_SAMPLE) {
                     // entries past the chunk up to the run's slot end take
@@ -993,6 +1001,183 @@ __global__ void __launch_bounds__(64 * XNW) k_scan_exact(ListScanArgs la) {
     }
     if (KIND == KIND_FILTER)
         while (fill != head) flush(fill - head < 64u ? fill - head : 64u);
+}
+
+// ------------------------------------------------- exact top-k over rows --
+// The exact FP32 top-k of a dense row matrix (the plain path's coarse step:
+// every query's nearest centroids) in the same four steps as the list scans:
+// a sample pass over every stride-th row writes each query's 16-bit keys of
+// (distance times its scale) into its sample row; k_theta takes the k-th
+// smallest; a filter pass over every row keeps the records (distance bits <<
+// 32 | row) whose keys are at most that; the exact-record select picks the k
+// best by (distance, row); then the rank kernel orders them (a query whose
+// records overflowed its room ranks every row). Work item i is rows chunk
+// i / ngroups times query group i % ngroups, so that the items in flight at
+// once share their rows in L2. Lane l takes row b + l of each 64-row slab
+// into registers; the distance of each of the item's queries, in the
+// reference's order with the query's values as scalar operands, is the
+// reference-order rank bit for bit.
+struct TopkRowsArgs {
+    const float *queries;     // [nq][D]
+    const float *base;        // [nb][D]
+    int64_t nq, nb;
+    const float *q_scale;     // [nq] key scale (a power of two)
+    int64_t stride, ns, ns_pad;   // sample: rows r * stride, r < ns; rows of ns_pad slots
+    uint16_t *skeys;          // [nq][ns_pad]
+    uint32_t skey_bytes;      // below 2^32 - 64
+    const uint32_t *theta;    // [nq]
+    uint64_t *rec;            // [nq][capq]
+    uint32_t *qcnt;           // [nq], zeroed
+    uint32_t capq;
+    int qg, chunk;            // queries per item (<= 64), rows per item (a multiple of 64)
+    int64_t ngroups;
+    int n_items;
+    int32_t *item_counter;    // [n_parts], zeroed
+    int n_parts;
+};
+
+template <int D, int KIND>
+__global__ void __launch_bounds__(64 * XNW) k_topk_rows(TopkRowsArgs t) {
+    __shared__ uint64_t s_ring0[XNW * RING];
+    __shared__ uint32_t s_ringq0[XNW * RING];
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    uint64_t *s_ring = s_ring0 + (size_t)wave * RING;
+    uint32_t *s_ringq = s_ringq0 + (size_t)wave * RING;
+    const int nparts = t.n_parts, part = (int)(blockIdx.x % (unsigned)nparts);
+    int32_t *ctr = t.item_counter + part;
+    const __amdgpu_buffer_rsrc_t drs =
+        __builtin_amdgcn_make_buffer_rsrc(t.skeys, (short)0, (int)t.skey_bytes, 0x00020000);
+    const int64_t nrows = KIND == KIND_SAMPLE ? t.ns_pad : t.nb;
+    uint32_t fill = 0, head = 0;
+    auto flush = [&](uint32_t n) {
+        wave_sync();
+        const uint32_t slot = (head + (uint32_t)lane) & (RING - 1);
+        const uint64_t v = s_ring[slot];
+        const uint32_t vq = s_ringq[slot];
+        const bool has = (uint32_t)lane < n;
+        int lead = 0;
+        uint32_t rank = 0, gcnt = 0;
+        for (uint64_t todo = __builtin_amdgcn_ballot_w64(has); todo;) {
+            const int l0 = __builtin_ctzll(todo);
+            const uint32_t ql = (uint32_t)__builtin_amdgcn_readlane((int)vq, l0);
+            const uint64_t grp = __builtin_amdgcn_ballot_w64(vq == ql) & todo;
+            if (__builtin_amdgcn_inverse_ballot_w64(grp)) {
+                lead = l0;
+                rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(grp >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)grp, 0u));
+            }
+            if (lane == l0) gcnt = (uint32_t)__builtin_popcountll(grp);
+            todo &= ~grp;
+        }
+        uint32_t p0 = 0;
+        if (has && lane == lead) p0 = atomicAdd(&t.qcnt[vq], gcnt);
+        const uint32_t pp = (uint32_t)__shfl((int)p0, lead) + rank;
+        if (has && pp < t.capq) t.rec[(uint64_t)vq * t.capq + pp] = v;
+        head = (uint32_t)__builtin_amdgcn_readfirstlane((int)(head + n));
+        wave_sync();
+    };
+    int j = 0;
+    if (lane == 0) j = atomicAdd(ctr, 1);
+    int item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
+    while (item < t.n_items) {
+        if (lane == 0) j = atomicAdd(ctr, 1);
+        const int64_t c = item / t.ngroups, g = item % t.ngroups;
+        const int64_t q0 = g * t.qg;
+        const int pc = (int)(t.nq - q0 < t.qg ? t.nq - q0 : t.qg);
+        const int64_t rb = c * t.chunk, re = rb + t.chunk < nrows ? rb + t.chunk : nrows;
+        // lane jq < pc holds query q0 + jq: its scale and bound
+        const int vq = (int)(q0 + (lane < pc ? lane : 0));
+        const float vsc = t.q_scale[vq];
+        const int vaux = KIND == KIND_SAMPLE ? 0 : key_bound(t.theta[vq]);
+        for (int64_t b = rb; b < re; b += 64) {
+            if (KIND == KIND_FILTER)
+                while (fill - head >= 64u) flush(64);
+            const int64_t r = b + lane;
+            const bool valid = KIND == KIND_SAMPLE ? r < t.ns : r < re;
+            const int64_t row = KIND == KIND_SAMPLE ? (r < t.ns ? r : t.ns - 1) * t.stride : (r < re ? r : re - 1);
+            float x[D];
+            {
+                const float4 *xr = reinterpret_cast<const float4 *>(t.base + row * D);
+#pragma unroll
+                for (int u = 0; u < D / 4; ++u) {
+                    const float4 v = xr[u];
+                    x[4 * u + 0] = v.x;
+                    x[4 * u + 1] = v.y;
+                    x[4 * u + 2] = v.z;
+                    x[4 * u + 3] = v.w;
+                }
+            }
+            for (int jq = 0; jq < pc; ++jq) {
+                const int qid = __builtin_amdgcn_readlane(vq, jq);
+                const float sc = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(vsc), jq));
+                const float key = sqdiff_scalar_q<D>(x, as_const(t.queries + (int64_t)qid * D));
+                const float ks = key * sc;
+                if (KIND == KIND_SAMPLE) {
+                    // slots past the sample hold the padding key 0xFFFF
+                    const uint32_t k16 = valid ? key16(ks) : 0xFFFFu;
+                    const uint32_t boff = r < re ? 2u * (uint32_t)((int64_t)qid * t.ns_pad + r) : kNoStore;
+                    __builtin_amdgcn_raw_buffer_store_b16((unsigned short)k16, drs, (int)boff, 0, 0);
+                } else {
+                    const int aux = __builtin_amdgcn_readlane(vaux, jq);
+                    const uint64_t mk = __builtin_amdgcn_ballot_w64(valid && __float_as_int(ks) <= aux);
+                    if (mk) {
+                        if (fill - head > (uint32_t)(RING - 64)) flush(64);
+                        if (__builtin_amdgcn_inverse_ballot_w64(mk)) {
+                            const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
+                                (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, fill));
+                            s_ring[pos & (RING - 1)] = ((uint64_t)__float_as_uint(key) << 32) | (uint32_t)row;
+                            s_ringq[pos & (RING - 1)] = (uint32_t)qid;
+                        }
+                        fill = (uint32_t)__builtin_amdgcn_readfirstlane(
+                            (int)(fill + (uint32_t)__builtin_popcountll(mk)));
+                    }
+                }
+            }
+        }
+        item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
+    }
+    if (KIND == KIND_FILTER)
+        while (fill != head) flush(fill - head < 64u ? fill - head : 64u);
+}
+
+// the largest row norm of the base (as float bits; *out zeroed before)
+__global__ void __launch_bounds__(256) k_row_norm_max(const float *__restrict__ base, int64_t nb, int d,
+                                                      unsigned *__restrict__ out) {
+    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    float m = 0.f;
+    if (r < nb) {
+        float s = 0.f;
+        for (int e = 0; e < d; ++e) s = __builtin_fmaf(base[r * d + e], base[r * d + e], s);
+        m = __builtin_sqrtf(s) * (1.f + 1.f / 1024.f);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+}
+
+// per query: the key scale 2^(14 - e) with (|q| + max |x|)^2 < 2^e, and the
+// select's per-query arrays (row offsets and slot counts of the sample rows,
+// the row count)
+__global__ void __launch_bounds__(256) k_topk_rows_prep(const float *__restrict__ queries, int64_t nq, int d,
+                                                        const unsigned *__restrict__ max_norm, int64_t ns_pad,
+                                                        int64_t nb, float *__restrict__ q_scale,
+                                                        int64_t *__restrict__ row_off, int64_t *__restrict__ qslots,
+                                                        int64_t *__restrict__ qtot) {
+    const int64_t q = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (q >= nq) return;
+    float s = 0.f;
+    for (int e = 0; e < d; ++e) s = __builtin_fmaf(queries[q * d + e], queries[q * d + e], s);
+    const float r = __builtin_sqrtf(s) * (1.f + 1.f / 1024.f) + __uint_as_float(*max_norm);
+    const float bound = r * r;
+    float scale = 1.f;
+    if (bound > 0.f && bound < __builtin_inff()) {
+        int e = 0;
+        (void)__builtin_frexpf(bound, &e);
+        e = e < -100 ? -100 : (e > 100 ? 100 : e);
+        scale = __builtin_ldexpf(1.f, 14 - e);
+    }
+    q_scale[q] = scale;
+    row_off[q] = q * ns_pad;
+    qslots[q] = ns_pad;
+    qtot[q] = nb;
 }
 
 // Selection helpers of the threshold and select kernels: histograms over
@@ -1448,6 +1633,142 @@ void launch_select_records(const ScanArgs &a, int64_t nq, const uint64_t *rec, c
     hipLaunchKernelGGL(dev::k_select_records, dim3((unsigned)nq), dim3(256), 0, st, a, nq, rec, qcnt, capq, qtot,
                        out_pos, out_cnt, out_stride, ovf, ovf_cnt, q_scale);
     HANN_LAUNCH_CHECK();
+}
+
+// ---------------------------------------------------- exact top-k rows --
+namespace {
+struct RowsPlan {
+    int64_t stride = 1, ns = 0, ns_pad = 0, capq = 0, qc = 0;
+};
+RowsPlan rows_plan(int64_t nq, int64_t nb, int k) {
+    RowsPlan P;
+    // a sample of about a quarter of the rows (at least 8 k, at most 65,536):
+    // a query's k-th sample key bounds about its (k stride)-th distance
+    const int64_t want = std::max<int64_t>(8 * (int64_t)k, std::min<int64_t>(nb / 4, 65536));
+    P.stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+    P.ns = ceil_div(nb, P.stride);
+    P.ns_pad = (P.ns + 7) & ~(int64_t)7;
+    P.capq = std::max<int64_t>(512, 4 * (int64_t)k * P.stride);
+    // queries per round: sample rows and record rooms within 512 MB
+    const int64_t per = P.ns_pad * 2 + P.capq * 8 + (int64_t)k * 8 + 64;
+    P.qc = std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t(512) << 20) / per));
+    P.qc = std::min<int64_t>(P.qc, ((int64_t(1) << 32) - 64) / (2 * P.ns_pad));
+    return P;
+}
+}  // namespace
+
+bool topk_rows_supported(int metric, int64_t nb, int dim, int k) {
+    return (metric == HANN_METRIC_SQEUCLIDEAN || metric == HANN_METRIC_EUCLIDEAN) &&
+           (dim == 64 || dim == 96 || dim == 128) && k >= 1 && kp_for(k) <= 512 && nb <= 65536 &&
+           nb >= 16 * (int64_t)k;
+}
+
+size_t topk_rows_scratch_bytes(int64_t nq, int64_t nb, int k) {
+    const RowsPlan P = rows_plan(nq, nb, k);
+    return (size_t)P.qc * (size_t)(P.ns_pad * 2 + P.capq * 8 + (int64_t)k * 8 + 64) + 8192 + 3 * 256;
+}
+
+void launch_topk_rows(int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
+                      int64_t *out_ids, double *out_dist, Workspace &ws, hipStream_t st) {
+    if (nq == 0) return;
+    if (!topk_rows_supported(metric, nb, dim, k)) fail(HANN_ERR_UNSUPPORTED, "exact top-k rows: unsupported shape");
+    const RowsPlan P = rows_plan(nq, nb, k);
+    const int64_t qc = P.qc;
+    // arena: the zeroed block (record counts, counters, partitions, the
+    // norm bound), then the per-query arrays
+    size_t off = 0;
+    auto carve = [&](size_t b) {
+        const size_t o = off;
+        off += (b + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_qcnt = carve((size_t)qc * 4), o_ctr = carve(16), o_ps = carve(kListScanParts * 4),
+                 o_pf = carve(kListScanParts * 4), o_norm = carve(16), o_zero_end = off;
+    const size_t o_scale = carve((size_t)qc * 4), o_theta = carve((size_t)qc * 4), o_roff = carve((size_t)qc * 8),
+                 o_qsl = carve((size_t)qc * 8), o_qtot = carve((size_t)qc * 8), o_cnt = carve((size_t)qc * 4),
+                 o_ovf = carve((size_t)qc * 4);
+    char *A = ws.get<char>(21, off);
+    uint32_t *qcnt = reinterpret_cast<uint32_t *>(A + o_qcnt);
+    int32_t *ctr = reinterpret_cast<int32_t *>(A + o_ctr);
+    int32_t *parts_s = reinterpret_cast<int32_t *>(A + o_ps);
+    int32_t *parts_f = reinterpret_cast<int32_t *>(A + o_pf);
+    unsigned *norm = reinterpret_cast<unsigned *>(A + o_norm);
+    float *q_scale = reinterpret_cast<float *>(A + o_scale);
+    uint32_t *theta = reinterpret_cast<uint32_t *>(A + o_theta);
+    int64_t *row_off = reinterpret_cast<int64_t *>(A + o_roff);
+    int64_t *qslots = reinterpret_cast<int64_t *>(A + o_qsl);
+    int64_t *qtot = reinterpret_cast<int64_t *>(A + o_qtot);
+    int32_t *cnt = reinterpret_cast<int32_t *>(A + o_cnt);
+    int32_t *ovf = reinterpret_cast<int32_t *>(A + o_ovf);
+    uint16_t *skeys = ws.get<uint16_t>(22, (size_t)(qc * P.ns_pad));
+    uint64_t *rec = ws.get<uint64_t>(23, (size_t)(qc * P.capq));
+    int64_t *pos = ws.get<int64_t>(14, (size_t)(qc * k));
+    int dev = 0, cus = 0;
+    HANN_HIP(hipGetDevice(&dev));
+    HANN_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    const int grid = cus * 2;
+    for (int64_t q0 = 0; q0 < nq; q0 += qc) {
+        const int64_t qn = std::min(qc, nq - q0);
+        const float *qs = queries + q0 * dim;
+        launch_fill_i32(reinterpret_cast<int32_t *>(A), (int64_t)(o_zero_end / 4), 0, st);
+        hipLaunchKernelGGL(dev::k_row_norm_max, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, base, nb, dim,
+                           norm);
+        HANN_LAUNCH_CHECK();
+        hipLaunchKernelGGL(dev::k_topk_rows_prep, dim3((unsigned)ceil_div(qn, 256)), dim3(256), 0, st, qs, qn, dim,
+                           norm, P.ns_pad, nb, q_scale, row_off, qslots, qtot);
+        HANN_LAUNCH_CHECK();
+        dev::TopkRowsArgs t{};
+        t.queries = qs;
+        t.base = base;
+        t.nq = qn;
+        t.nb = nb;
+        t.q_scale = q_scale;
+        t.stride = P.stride;
+        t.ns = P.ns;
+        t.ns_pad = P.ns_pad;
+        t.skeys = skeys;
+        t.skey_bytes = (uint32_t)(qn * P.ns_pad * 2);
+        t.theta = theta;
+        t.rec = rec;
+        t.qcnt = qcnt;
+        t.capq = (uint32_t)P.capq;
+        t.qg = 8;
+        t.chunk = 256;
+        t.ngroups = ceil_div(qn, (int64_t)t.qg);
+        t.n_parts = std::min(kListScanParts, grid);
+#define HANN_ROWS(DD, KK, NR, CTR)                                                                   \
+    do {                                                                                             \
+        t.n_items = (int)(ceil_div((int64_t)(NR), (int64_t)t.chunk) * t.ngroups);                    \
+        t.item_counter = CTR;                                                                        \
+        hipLaunchKernelGGL((dev::k_topk_rows<DD, KK>), dim3((unsigned)grid), dim3(64 * dev::XNW), 0, \
+                           st, t);                                                                   \
+        HANN_LAUNCH_CHECK();                                                                         \
+    } while (0)
+#define HANN_ROWS_D(KK, NR, CTR)                             \
+    do {                                                     \
+        if (dim == 64) HANN_ROWS(64, KK, NR, CTR);           \
+        else if (dim == 96) HANN_ROWS(96, KK, NR, CTR);      \
+        else HANN_ROWS(128, KK, NR, CTR);                    \
+    } while (0)
+        // the sample pass, each query's bound, the filter pass
+        HANN_ROWS_D(dev::KIND_SAMPLE, P.ns_pad, parts_s);
+        ScanArgs a{};
+        a.k_q = nullptr;
+        a.k_default = k;
+        a.rerank = 1;
+        a.allow_fallback = 0;
+        launch_theta(a, qn, skeys, row_off, qtot, qslots, theta, st);
+        HANN_ROWS_D(dev::KIND_FILTER, nb, parts_f);
+#undef HANN_ROWS_D
+#undef HANN_ROWS
+        // the k best records by (distance, row); a query whose records
+        // overflowed keeps the count INT32_MAX, and the rank kernel ranks
+        // every row for it
+        launch_fill_i32(cnt, qn, INT32_MAX, st);
+        launch_select_records(a, qn, rec, qcnt, (uint32_t)P.capq, qtot, pos, cnt, k, ovf, ctr + 1, q_scale, st);
+        launch_rank_rows64(qs, qn, dim, base, nb, cnt, pos, k, k, nullptr,
+                           metric == HANN_METRIC_EUCLIDEAN ? 1 : 0, out_ids + q0 * k, out_dist + q0 * k, st);
+    }
 }
 
 int select_rows_max_probes() { return dev::SEL_MAXP; }

@@ -69,7 +69,8 @@ struct Slot {
     int64_t nq = 0;
     int kmax = 0;
     int64_t bytes = 0;       // the batch in flight's charge
-    int64_t max_bound = 0;   // the largest scratch bound charged on this slot
+    int64_t bound = 0;       // its scratch bound (the charge less the staging)
+    int64_t ws_before = 0;   // the slot's workspace bytes when the batch started
     bool busy = false;
     // device bytes the slot holds: its workspace (grow-only) and buffers
     int64_t held() const {
@@ -90,7 +91,7 @@ struct Server {
     bool stop = false;
     int64_t budget_in_use = 0, budget_peak = 0;
     int64_t held_peak = 0;            // device bytes all slots held, at most
-    int64_t scratch_over_bound = 0;   // workspace bytes above the charged bound (0 when the bound holds)
+    int64_t scratch_over_bound = 0;   // most a batch grew its slot's workspace past its scratch bound (0: the bounds held)
     int64_t completed_queries = 0, completed_requests = 0, batches = 0, mutations = 0;
     int in_flight = 0;
     std::thread worker, completer;
@@ -148,7 +149,8 @@ struct Server {
         s.nq = nq;
         s.kmax = kmax;
         s.bytes = batch_bytes(nq, kmax, pmax);
-        s.max_bound = std::max(s.max_bound, s.bytes - staging_bytes(nq, kmax));
+        s.bound = s.bytes - staging_bytes(nq, kmax);
+        s.ws_before = (int64_t)s.ws.bytes();
         s.busy = true;
         budget_in_use += s.bytes;
         budget_peak = std::max(budget_peak, budget_in_use);
@@ -308,7 +310,9 @@ struct Server {
             s->reqs.clear();
             s->busy = false;
             budget_in_use -= s->bytes;
-            scratch_over_bound = std::max(scratch_over_bound, (int64_t)s->ws.bytes() - s->max_bound);
+            // the workspace only grows, each slot to the largest need of
+            // any batch it ran: what one batch adds is at most its bound
+            scratch_over_bound = std::max(scratch_over_bound, (int64_t)s->ws.bytes() - s->ws_before - s->bound);
             int64_t held = 0;
             for (auto &sl : slots) held += sl->held();
             held_peak = std::max(held_peak, held);

@@ -370,8 +370,9 @@ typedef struct {
     int64_t completed_queries, completed_requests, batches, mutations;
     int64_t queue_depth, in_flight, budget_in_use, budget_peak;
     /* device bytes the slots' workspaces and buffers held at most (they are
-     * grow-only), and how far a slot's workspace ever exceeded the largest
-     * bound charged on it (0: every charge bounded the memory it used) */
+     * grow-only), and the most any batch grew its slot's workspace past the
+     * scratch bound it was charged (0: every charge bounded the device
+     * memory its search added) */
     int64_t held_peak, scratch_over_bound;
 } hann_server_counters;
 

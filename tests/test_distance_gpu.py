@@ -94,6 +94,37 @@ def test_batch_topk_exact_matches_oracle(hann, oracle, nq, nb, dim, k):
         assert np.all(ids[:, kk:] == -1) and np.all(np.isinf(dist[:, kk:]))
 
 
+@pytest.mark.parametrize("nq,nb,dim,k,kind", [(500, 16384, 96, 128, "real"), (300, 20000, 128, 100, "int"),
+                                              (64, 8192, 64, 512, "real"), (40, 5000, 64, 10, "dup"),
+                                              (400, 65536, 96, 8, "real"), (33, 4000, 128, 1, "int")])
+def test_batch_topk_rows_path_matches_oracle(hann, oracle, nq, nb, dim, k, kind):
+    """The exact top-k's row path (bases of up to 65,536 rows, d in {64, 96,
+    128}: a sample pass, each query's bound, a filter pass, the exact-record
+    select and the rank kernel): ids and ranks identical to the oracle's,
+    ties by row; with three distinct rows the records of every query overflow
+    their room and every row is ranked."""
+    from hann_amd import core
+    rng = np.random.default_rng(nq + nb + dim + k)
+    if kind == "dup":
+        rows = rng.standard_normal((3, dim)).astype(np.float32)
+        base = rows[rng.integers(0, 3, nb)]
+        q = rows[rng.integers(0, 3, nq)] + np.float32(0.01)
+    elif kind == "int":
+        lam = rng.uniform(10, 30, dim)
+        base = np.minimum(rng.poisson(lam, (nb, dim)), 255).astype(np.float32)
+        q = np.minimum(rng.poisson(lam, (nq, dim)), 255).astype(np.float32)
+    else:
+        cen = rng.standard_normal((64, dim)).astype(np.float32) * 3
+        base = (cen[rng.integers(0, 64, nb)] + rng.standard_normal((nb, dim))).astype(np.float32)
+        q = (cen[rng.integers(0, 64, nq)] + rng.standard_normal((nq, dim))).astype(np.float32)
+    for metric, sq in ((core.SquaredEuclidean, True), (core.Euclidean, False)):
+        ids, dist = core.batch_topk(metric, q, base, k)
+        wid, wrank = oracle.batch_topk(0, q, base, k, nthreads=8)
+        assert np.array_equal(ids, wid)
+        want = wrank.astype(np.float64)
+        assert np.array_equal(dist, want if sq else np.sqrt(want))
+
+
 def test_batch_topk_integer_ties_ordered_by_row(hann, oracle):
     """Config-2 style integer data: huge tie groups, ordered by (rank, row)."""
     from hann_amd import core
