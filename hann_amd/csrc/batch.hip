@@ -1,5 +1,6 @@
 // batch.hip -- batch top-k dispatch: exact FP32, or the certified BF16
 // MFMA candidate pass followed by an exact FP32 ranking.
+#include <cmath>
 #include <algorithm>
 #include <vector>
 
@@ -14,6 +15,20 @@ namespace hann {
 constexpr int64_t kRankEveryMax = 65536;
 
 int64_t certified_no_sync_max_rows() { return kRankEveryMax; }
+
+// The row sample of the certified pass: every stride-th row. A query's
+// threshold bounds about its (k stride)-th distance, so the candidates it
+// ranks exactly number about k nb / ns; the sample costs nq ns distances
+// (an MFMA pass that writes them, then a per-query threshold pass that reads
+// them), a candidate about eight times one sample distance (a random row
+// read and its exact rank). The total is least near ns = sqrt(8 k nb): 4,096
+// rows for 16,384 centroids and k = 128, about 28,000 for a million rows and
+// k = 100. At least 4 k sample rows.
+static int64_t sample_stride(int64_t nb, int k) {
+    const double ns = std::sqrt(8.0 * (double)k * (double)nb);
+    const int64_t want = std::max<int64_t>(4 * (int64_t)k, (int64_t)ns);
+    return std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+}
 
 __global__ void k_copy_u32(unsigned *__restrict__ dst, const unsigned *__restrict__ src, int n) {
     const int i = threadIdx.x;
@@ -36,8 +51,7 @@ size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool
     add(nq * 4);
     add(nq * 4);
     add(nq * 4);
-    const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
-    const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+    const int64_t stride = sample_stride(nb, k);
     const int64_t ns = ceil_div(nb, stride);
     add(ns * dp * 2);
     add(ns * 4);
@@ -109,8 +123,7 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         launch_thresholds(srank, 1, (int)nq, qf, stat, reinterpret_cast<int *>(stat + 1), dim, thr, st);
         ns = std::max<int64_t>(1, nb / 64);   // sizes the candidate room only
     } else {
-        const int64_t want = std::max<int64_t>(4 * (int64_t)k, ceil_div(nb * (int64_t)k, 512));
-        const int64_t stride = std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
+        const int64_t stride = sample_stride(nb, k);
         ns = ceil_div(nb, stride);
         __bf16 *sb = w.get<__bf16>(8, (size_t)ns * dp);
         float *sn = w.get<float>(9, (size_t)ns);

@@ -3,9 +3,11 @@
 // All FP32 kernels here reproduce the reference's AVX2+FMA arithmetic
 // (core/simd_kernels.inc.h) bit for bit; see common.h for the order.
 #include <mutex>
+#include <vector>
 
 #include "common.h"
 #include "kernels.h"
+#include "kmeans.h"
 #include "scan_list.h"
 
 namespace hann {
@@ -393,16 +395,49 @@ void launch_topk_exact(int metric, const float *queries, int64_t nq, const float
     const int kp = kp_for(k);
     if (kp > 512) fail(HANN_ERR_UNSUPPORTED, strf("top-k with k=%d exceeds the supported maximum of 512", k));
     if (topk_rows_supported(metric, nb, dim, k)) {
-        // bases of up to 65,536 rows (the coarse step's centroids): sample,
-        // bound, filter and select over rows held in registers
-        launch_topk_rows(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st);
-        return;
+        // sample, bound, filter and select over rows held in registers; past
+        // 65,536 rows its overflow check waits on the host, which a stream
+        // under capture cannot do: the tiled kernel then
+        hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+        if (nb > 65536) HANN_HIP(hipStreamIsCapturing(st, &cs));
+        if (cs == hipStreamCaptureStatusNone) {
+            launch_topk_rows(metric, queries, nq, base, nb, dim, k, out_ids, out_dist, ws, st);
+            return;
+        }
     }
     int64_t nsplit = 0, rows = 0;
     topk_plan(metric, kp, nq, nb, 65535, nsplit, rows);
     float *pkey = ws.get<float>(14, (size_t)(nq * nsplit * k));
     int64_t *pid = ws.get<int64_t>(15, (size_t)(nq * nsplit * k));
     topk_dispatch(metric, queries, nq, base, nb, dim, k, nsplit, rows, pkey, pid, out_ids, out_dist, st);
+}
+
+void topk_tiled_redo(int metric, const float *queries, const std::vector<int64_t> &redo, const float *base,
+                     int64_t nb, int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws, hipStream_t st) {
+    (void)ws;
+    const int64_t nr = (int64_t)redo.size();
+    if (nr == 0) return;
+    const int kp = kp_for(k);
+    DevBuf<int64_t> dredo((size_t)nr);
+    DevBuf<float> rq((size_t)(nr * dim));
+    DevBuf<int64_t> rid((size_t)(nr * k));
+    DevBuf<double> rdist((size_t)(nr * k));
+    HANN_HIP(hipMemcpyAsync(dredo.get(), redo.data(), (size_t)nr * sizeof(int64_t), hipMemcpyHostToDevice, st));
+    launch_gather_rows(queries, dim, dredo.get(), nr, rq.get(), st);
+    int64_t nsplit = 0, rows = 0;
+    topk_plan(metric, kp, nr, nb, 65535, nsplit, rows);
+    DevBuf<float> pkey((size_t)(nr * nsplit * k));
+    DevBuf<int64_t> pid((size_t)(nr * nsplit * k));
+    topk_dispatch(metric, rq.get(), nr, base, nb, dim, k, nsplit, rows, pkey.get(), pid.get(), rid.get(), rdist.get(),
+                  st);
+    for (int64_t i = 0; i < nr; ++i) {
+        HANN_HIP(hipMemcpyAsync(out_ids + redo[i] * k, rid.get() + i * k, (size_t)k * sizeof(int64_t),
+                                hipMemcpyDeviceToDevice, st));
+        HANN_HIP(hipMemcpyAsync(out_dist + redo[i] * k, rdist.get() + i * k, (size_t)k * sizeof(double),
+                                hipMemcpyDeviceToDevice, st));
+    }
+    // the local buffers are freed on return
+    HANN_HIP(hipStreamSynchronize(st));
 }
 
 // ------------------------------------------------- allocation-free top-k --

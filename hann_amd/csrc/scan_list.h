@@ -1,6 +1,8 @@
 // scan_list.h -- list-major decode + MFMA scan (scan_list.hip).
 #pragma once
 
+#include <vector>
+
 #include "ivf.h"
 #include "kernels.h"
 
@@ -111,12 +113,18 @@ void launch_scan_exact_kind(const ListScanArgs &la, int kind, int grid, hipStrea
 // The exact FP32 top-k of dense rows by sample, bound, filter and select
 // (the same passes, records holding the exact rank), then ranked and
 // ordered by (rank, row): identical results to launch_topk_exact's tiled
-// kernel. (Squared) Euclidean, d in {64, 96, 128}, k <= 512, nb <= 65,536 and
-// at least 16 k (the coarse step's centroids). Workspace slots 14, 21-23.
+// kernel. (Squared) Euclidean, d in {64, 96, 128}, k <= 512, nb at least
+// 16 k. A query whose records overflow its room is ranked over every row on
+// the device (nb <= 65,536) or by the tiled kernel after a host check of the
+// counts. Workspace slots 14, 21-23.
 bool topk_rows_supported(int metric, int64_t nb, int dim, int k);
 size_t topk_rows_scratch_bytes(int64_t nq, int64_t nb, int k);
 void launch_topk_rows(int metric, const float *queries, int64_t nq, const float *base, int64_t nb, int dim, int k,
                       int64_t *out_ids, double *out_dist, Workspace &ws, hipStream_t st);
+// the tiled kernel over the listed queries (global indices), results
+// scattered into their rows of out_ids / out_dist; waits for the stream
+void topk_tiled_redo(int metric, const float *queries, const std::vector<int64_t> &redo, const float *base,
+                     int64_t nb, int dim, int k, int64_t *out_ids, double *out_dist, Workspace &ws, hipStream_t st);
 // per query, the bound theta of the filter pass: the (k R)-th smallest key of
 // its sample (qsamp_slots keys from row_off[q] of the sample rows)
 void launch_theta(const ScanArgs &a, int64_t nq, const uint16_t *keys, const int64_t *row_off, const int64_t *qtot,

@@ -1659,9 +1659,13 @@ RowsPlan rows_plan(int64_t nq, int64_t nb, int k) {
 
 bool topk_rows_supported(int metric, int64_t nb, int dim, int k) {
     return (metric == HANN_METRIC_SQEUCLIDEAN || metric == HANN_METRIC_EUCLIDEAN) &&
-           (dim == 64 || dim == 96 || dim == 128) && k >= 1 && kp_for(k) <= 512 && nb <= 65536 &&
+           (dim == 64 || dim == 96 || dim == 128) && k >= 1 && kp_for(k) <= 512 && nb < (int64_t(1) << 31) &&
            nb >= 16 * (int64_t)k;
 }
+
+// bases up to this many rows rank an overflowed query's every row on the
+// device; larger ones hand such queries to the tiled kernel from the host
+constexpr int64_t kRowsRankEveryMax = 65536;
 
 size_t topk_rows_scratch_bytes(int64_t nq, int64_t nb, int k) {
     const RowsPlan P = rows_plan(nq, nb, k);
@@ -1766,8 +1770,19 @@ void launch_topk_rows(int metric, const float *queries, int64_t nq, const float 
         // every row for it
         launch_fill_i32(cnt, qn, INT32_MAX, st);
         launch_select_records(a, qn, rec, qcnt, (uint32_t)P.capq, qtot, pos, cnt, k, ovf, ctr + 1, q_scale, st);
-        launch_rank_rows64(qs, qn, dim, base, nb, cnt, pos, k, k, nullptr,
+        const bool every = nb <= kRowsRankEveryMax;
+        launch_rank_rows64(qs, qn, dim, base, every ? nb : 0, cnt, pos, k, k, nullptr,
                            metric == HANN_METRIC_EUCLIDEAN ? 1 : 0, out_ids + q0 * k, out_dist + q0 * k, st);
+        if (every) continue;
+        // a large base: the queries whose records overflowed (left untouched
+        // by the rank kernel) take the tiled kernel
+        std::vector<int32_t> hc((size_t)qn);
+        HANN_HIP(hipMemcpyAsync(hc.data(), cnt, (size_t)qn * sizeof(int32_t), hipMemcpyDeviceToHost, st));
+        HANN_HIP(hipStreamSynchronize(st));
+        std::vector<int64_t> redo;
+        for (int64_t q = 0; q < qn; ++q)
+            if (hc[q] > k) redo.push_back(q0 + q);
+        if (!redo.empty()) topk_tiled_redo(metric, queries, redo, base, nb, dim, k, out_ids, out_dist, ws, st);
     }
 }
 

@@ -96,13 +96,16 @@ def test_batch_topk_exact_matches_oracle(hann, oracle, nq, nb, dim, k):
 
 @pytest.mark.parametrize("nq,nb,dim,k,kind", [(500, 16384, 96, 128, "real"), (300, 20000, 128, 100, "int"),
                                               (64, 8192, 64, 512, "real"), (40, 5000, 64, 10, "dup"),
-                                              (400, 65536, 96, 8, "real"), (33, 4000, 128, 1, "int")])
+                                              (400, 65536, 96, 8, "real"), (33, 4000, 128, 1, "int"),
+                                              (200, 150000, 128, 10, "int"), (24, 70000, 64, 10, "dup"),
+                                              (120, 100000, 96, 100, "real")])
 def test_batch_topk_rows_path_matches_oracle(hann, oracle, nq, nb, dim, k, kind):
-    """The exact top-k's row path (bases of up to 65,536 rows, d in {64, 96,
-    128}: a sample pass, each query's bound, a filter pass, the exact-record
-    select and the rank kernel): ids and ranks identical to the oracle's,
-    ties by row; with three distinct rows the records of every query overflow
-    their room and every row is ranked."""
+    """The exact top-k's row path (d in {64, 96, 128}: a sample pass, each
+    query's bound, a filter pass, the exact-record select and the rank
+    kernel): ids and ranks identical to the oracle's, ties by row. With three
+    distinct rows the records of every query overflow their room: up to
+    65,536 rows the rank kernel ranks every row for them, past that the tiled
+    kernel reruns them."""
     from hann_amd import core
     rng = np.random.default_rng(nq + nb + dim + k)
     if kind == "dup":
