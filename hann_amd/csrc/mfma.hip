@@ -320,6 +320,205 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if ((s_cc[ql * 2] > SL || s_cc[ql * 2 + 1] > SL) && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
 }
 
+// The same pass for dp = 16 KS <= 128, with each wave's query fragments in
+// registers and the next row tile's rows fetched into registers while the
+// current tile is multiplied and tested (LDS holds one row tile, so two
+// workgroups share a CU). Grid: x = query tiles, y = row groups, so the
+// workgroups in flight share their row tiles in L2.
+template <bool DENSE, int KS>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict__ X, int64_t nb,
+              const float *__restrict__ qn, const float *__restrict__ xn, const float *__restrict__ thr, int rt_per_wg,
+              int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row, int cap, float *__restrict__ dense,
+              int64_t ld_dense, int64_t rg0) {
+    constexpr int DP = KS * 16;
+    constexpr int LD = DP + MF_PAD;
+    constexpr int VECS = DP / 8;   // 16-byte vectors per row
+    constexpr int NV = MF_TILE * VECS / 256;
+    static_assert(NV * 256 == MF_TILE * VECS, "whole vectors per thread");
+    extern __shared__ __align__(16) unsigned char s_raw[];
+    __bf16 *sB = reinterpret_cast<__bf16 *>(s_raw);
+    float *s_xn = reinterpret_cast<float *>(sB + MF_TILE * LD);   // the row tile's norms
+    float *s_qn = s_xn + MF_TILE;                                  // the query tile's norms
+    float *s_thr = s_qn + MF_TILE;                                 // and thresholds (-inf past nq)
+    int32_t *s_cc = reinterpret_cast<int32_t *>(s_thr + MF_TILE);  // [MF_TILE][2] counts
+    int32_t *s_cr = s_cc + 2 * MF_TILE;                             // [MF_TILE][2][MF_SLOTS / 2]
+    int2 *s_ov = reinterpret_cast<int2 *>(s_cr + MF_TILE * MF_SLOTS);   // [MF_OVF] (query, row)
+    int32_t *s_on = reinterpret_cast<int32_t *>(s_ov + MF_OVF);
+    const int tid = threadIdx.x;
+    const int wave = tid / 64, lane = tid % 64;
+    const int wm = wave / 2, wn = wave % 2;
+    const int kl = (lane / 32) * 8;
+    const int64_t q0 = (int64_t)blockIdx.x * MF_TILE;
+    const int64_t rbase = ((int64_t)blockIdx.y + rg0) * MF_TILE * rt_per_wg;
+    for (int t = tid; t < MF_TILE; t += 256) {
+        const int64_t gq = q0 + t;
+        s_qn[t] = gq < nq ? qn[gq] : 0.f;
+        if (!DENSE) s_thr[t] = gq < nq ? thr[gq] : -__builtin_inff();
+    }
+    if (!DENSE && tid == 0) *s_on = 0;
+    // this wave's query fragments: A lane l holds query row l % 32 of each
+    // 32-row tile, columns 16 ks + kl .. + 7
+    bf16x8 a[2][KS];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int64_t gq = q0 + wm * 64 + i * 32 + lane % 32;
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            bf16x8 v = {};
+            if (gq < nq) v = *reinterpret_cast<const bf16x8 *>(Q + gq * DP + ks * 16 + kl);
+            a[i][ks] = v;
+        }
+    }
+    const int64_t left = nb - rbase;   // > 0: the grid covers nb
+    const int ntiles = (int)(left < (int64_t)rt_per_wg * MF_TILE ? (left + MF_TILE - 1) / MF_TILE : rt_per_wg);
+    // vector v = tid + 256 n of a tile is row v / VECS, columns 8 (v % VECS)
+    // .. + 7: a tile's rows are contiguous in X
+    bf16x8 breg[NV];
+    float xreg = 0.f;
+    auto fetch = [&](int64_t r0) {
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            const int v = tid + 256 * n;
+            bf16x8 b = {};
+            if (r0 + v / VECS < nb) b = *reinterpret_cast<const bf16x8 *>(X + r0 * DP + (int64_t)v * 8);
+            breg[n] = b;
+        }
+        if (tid < MF_TILE) xreg = r0 + tid < nb ? xn[r0 + tid] : 0.f;
+    };
+    fetch(rbase);
+    constexpr int SL = MF_SLOTS / 2;
+    // the running candidate counts of each query in this wave's row half,
+    // wave-uniform: lanes 0-31's count in the low 16 bits, lanes 32-63's in
+    // the high 16 (at most 64 rt_per_wg rows each)
+    unsigned cq[2][4][4];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int g = 0; g < 4; ++g)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) cq[i][g][u] = 0;
+
+    for (int rt = 0; rt < ntiles; ++rt) {
+        const int64_t r0 = rbase + (int64_t)rt * MF_TILE;
+        __syncthreads();   // the previous tile's readers are done
+#pragma unroll
+        for (int n = 0; n < NV; ++n) {
+            const int v = tid + 256 * n;
+            *reinterpret_cast<bf16x8 *>(&sB[(v / VECS) * LD + (v % VECS) * 8]) = breg[n];
+        }
+        if (tid < MF_TILE) s_xn[tid] = xreg;
+        __syncthreads();
+        if (rt + 1 < ntiles) fetch(r0 + MF_TILE);
+        // one 32-row half of the wave's 64 rows at a time: its fragments,
+        // the products with both 32-query tiles, then their test
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            const int rl = wn * 64 + j * 32 + lane % 32;
+            const int64_t gr = r0 + rl;
+            const bool rv = gr < nb;
+            const float xr = s_xn[rl];
+            f32x16 acc[2];
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int r = 0; r < 16; ++r) acc[i][r] = 0.f;
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks) {
+                const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[rl * LD + ks * 16 + kl]);
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][ks], b, acc[i], 0, 0, 0);
+            }
+            // the epilogue of k_mfma_pass: approximate distance, then the
+            // threshold test (ballot, slot by rank in the lane half) or the
+            // dense store
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const int qb = wm * 64 + i * 32 + 8 * g + 4 * (lane / 32);
+                    const float4 qn4 = *reinterpret_cast<const float4 *>(&s_qn[qb]);
+                    const float qv[4] = {qn4.x, qn4.y, qn4.z, qn4.w};
+                    float tv[4] = {0.f, 0.f, 0.f, 0.f};
+                    if (!DENSE) {
+                        const float4 t4 = *reinterpret_cast<const float4 *>(&s_thr[qb]);
+                        tv[0] = t4.x;
+                        tv[1] = t4.y;
+                        tv[2] = t4.z;
+                        tv[3] = t4.w;
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const float approx = (qv[u] + xr) - 2.f * acc[i][4 * g + u];
+                        if (DENSE) {
+                            int qbo = qb;
+                            asm volatile("" : "+v"(qbo));
+                            const int ql = qbo + u;
+                            if (rv && q0 + ql < nq) dense[(q0 + ql) * ld_dense + gr] = approx;
+                            continue;
+                        }
+                        const bool pass = rv && approx <= tv[u];
+                        const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                        if (m == 0) continue;
+                        // the rare path: the query index is computed here
+                        // from an opaque copy of the lane's query base, so
+                        // that the 32 per-element indices and addresses are
+                        // not kept in registers across the loop
+                        int qbo = wm * 64 + 4 * (lane / 32);
+                        asm volatile("" : "+v"(qbo));
+                        const int ql = qbo + i * 32 + 8 * g + u;
+                        const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
+                        if (pass) {
+                            const int rank = lane < 32 ? (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
+                                                       : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
+                            const int slot = (int)(lane < 32 ? cq[i][g][u] & 0xffffu : cq[i][g][u] >> 16) + rank;
+                            if (slot < SL) {
+                                s_cr[(ql * 2 + wn) * SL + slot] = (int32_t)gr;
+                            } else {
+                                const int o = atomicAdd(s_on, 1);
+                                if (o < MF_OVF) s_ov[o] = make_int2(ql, (int32_t)gr);
+                            }
+                        }
+                        cq[i][g][u] += (unsigned)__popc(mlo) + ((unsigned)__popc(mhi) << 16);
+                    }
+                }
+        }
+    }
+    if (DENSE) return;
+    if ((lane & 31) == 0) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int g = 0; g < 4; ++g)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    s_cc[(wm * 64 + i * 32 + 8 * g + 4 * (lane / 32) + u) * 2 + wn] =
+                        (int)(lane < 32 ? cq[i][g][u] & 0xffffu : cq[i][g][u] >> 16);
+    }
+    __syncthreads();
+    for (int ql = tid; ql < MF_TILE; ql += 256) {
+        const int64_t gq = q0 + ql;
+        const int n0 = s_cc[ql * 2] < SL ? s_cc[ql * 2] : SL;
+        const int n1 = s_cc[ql * 2 + 1] < SL ? s_cc[ql * 2 + 1] : SL;
+        if (gq >= nq || n0 + n1 == 0) continue;
+        const int base = atomicAdd(&cnt[gq], n0 + n1);
+        for (int i = 0; i < n0 && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[(ql * 2) * SL + i];
+        for (int i = 0; i < n1 && base + n0 + i < cap; ++i)
+            cand_row[gq * cap + base + n0 + i] = s_cr[(ql * 2 + 1) * SL + i];
+    }
+    const int no = *s_on;
+    for (int o = tid; o < (no < MF_OVF ? no : MF_OVF); o += 256) {
+        const int2 e = s_ov[o];
+        const int64_t gq = q0 + e.x;
+        if (gq >= nq) continue;
+        const int g = atomicAdd(&cnt[gq], 1);
+        if (g < cap) cand_row[gq * cap + g] = e.y;
+    }
+    if (no > MF_OVF)
+        for (int ql = tid; ql < MF_TILE; ql += 256)
+            if ((s_cc[ql * 2] > SL || s_cc[ql * 2 + 1] > SL) && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
+}
+
 // rows 0, stride, 2 stride, ... of X (BF16, dp columns) with their BF16 norms
 __global__ void k_gather_bf16(const __bf16 *__restrict__ X, int dp, const float *__restrict__ xn, int64_t stride,
                               int64_t n, __bf16 *__restrict__ out, float *__restrict__ out_n) {
@@ -513,11 +712,50 @@ static size_t mfma_lds(int dp, bool dense) {
            (dense ? 0 : (size_t)dev::MF_TILE * (2 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
 }
 
+static size_t mfma_lds_r(int dp, bool dense) {
+    return (size_t)dev::MF_TILE * (dp + dev::MF_PAD) * 2 + (size_t)3 * dev::MF_TILE * 4 +
+           (dense ? 0 : (size_t)dev::MF_TILE * (2 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
+}
+
+// dp <= 128: the register-resident kernel, grid (query tiles, row groups)
+template <bool DENSE>
+static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
+                          const float *xn, const float *thr, int rt, int32_t *cnt, int32_t *cand_row, int cap,
+                          float *dense, int64_t ld_dense, hipStream_t st) {
+    const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt), qt = ceil_div(nq, dev::MF_TILE);
+    if (qt > 0x7fffffff) fail(HANN_ERR_UNSUPPORTED, "BF16 candidate pass: too many queries");
+    const size_t shm = mfma_lds_r(dp, DENSE);
+    for (int64_t y0 = 0; y0 < rg; y0 += 65535) {
+        const int64_t yn = rg - y0 < 65535 ? rg - y0 : 65535;
+        const dim3 grid((unsigned)qt, (unsigned)yn);
+#define HANN_PASS_R(KS)                                                                                         \
+    do {                                                                                                        \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass_r<DENSE, KS>,                               \
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                    \
+        hipLaunchKernelGGL((dev::k_mfma_pass_r<DENSE, KS>), grid, dim3(256), shm, st, Q, nq, X, nb, qn, xn,     \
+                           thr, rt, cnt, cand_row, cap, dense, ld_dense, y0);                                   \
+        HANN_LAUNCH_CHECK();                                                                                    \
+    } while (0)
+        switch (dp) {
+        case 32: HANN_PASS_R(2); break;
+        case 64: HANN_PASS_R(4); break;
+        case 96: HANN_PASS_R(6); break;
+        case 128: HANN_PASS_R(8); break;
+        default: fail(HANN_ERR_UNSUPPORTED, strf("BF16 candidate pass: padded width %d", dp));
+        }
+#undef HANN_PASS_R
+    }
+}
+
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                         const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row, int cap,
                         hipStream_t st) {
     if (nq == 0 || nb == 0) return;
     if (rt_per_wg < 1) rt_per_wg = 1;
+    if (dp <= dev::MF_KC) {
+        launch_pass_r<false>(Q, nq, X, nb, dp, qn, xn, thr, rt_per_wg, cnt, cand_row, cap, nullptr, 0, st);
+        return;
+    }
     const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt_per_wg), qt = ceil_div(nq, dev::MF_TILE);
     const size_t shm = mfma_lds(dp, false);
     HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass<false>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -536,6 +774,10 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
                        const float *xn, float *dense, hipStream_t st) {
     if (nq == 0 || nb == 0) return;
     const int rt = 4;
+    if (dp <= dev::MF_KC) {
+        launch_pass_r<true>(Q, nq, X, nb, dp, qn, xn, nullptr, rt, nullptr, nullptr, 0, dense, nb, st);
+        return;
+    }
     const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt), qt = ceil_div(nq, dev::MF_TILE);
     const size_t shm = mfma_lds(dp, true);
     HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass<true>, hipFuncAttributeMaxDynamicSharedMemorySize,
