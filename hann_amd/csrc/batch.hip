@@ -62,6 +62,10 @@ size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool
     const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
     add(nq * 4);
     add(chunk * cap * 4);
+    if (mfma_filter_writes_approx((int)dp)) {
+        add(chunk * cap * 4);   // the candidates' approximate distances
+        add(nq * 4);            // and the second thresholds
+    }
     return b;
 }
 
@@ -133,7 +137,7 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
             const int64_t qn_ = std::min(qchunk, nq - q0);
             launch_mfma_dense(qb + q0 * dp, qn_, sb, ns, dp, qn + q0, sn, dense, st);
-            launch_sample_thresholds(dense, qn_, ns, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
+            launch_sample_thresholds(dense, qn_, ns, nullptr, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
                                      reinterpret_cast<int *>(stat + 1), dim, thr + q0, st);
         }
     }
@@ -144,18 +148,31 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     // ranking kernel ranks every row itself (and the host does not wait for
     // the counts); otherwise it is redone below by the tiled exact path
     const bool rank_every = nb <= kRankEveryMax;
-    // rows per workgroup: about 8 expected candidates per query and
-    // workgroup, against 32 slots
-    const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb / (16 * expect)));
+    // rows per workgroup: expected candidates per query and workgroup about
+    // a quarter of its LDS slots
+    const int slots = mfma_filter_slots(dp);
+    const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb * (slots / 4) / (128 * expect)));
     const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
     int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
     int32_t *cand = w.get<int32_t>(13, (size_t)chunk * cap);
+    // with the candidates' approximate distances, the k-th smallest of them
+    // gives a second, tighter threshold (by the same bound), and only the
+    // candidates within it are ranked; not after a hint row's threshold,
+    // which is already an exact distance's
+    const bool refine = mfma_filter_writes_approx(dp) && !(hint && k == 1);
+    float *cand_ap = mfma_filter_writes_approx(dp) ? w.get<float>(16, (size_t)chunk * cap) : nullptr;
+    float *thr2 = refine ? w.get<float>(17, (size_t)nq) : nullptr;
     launch_fill_i32(cnt, nq, 0, st);   // a fill kernel: the search may be captured in a graph
     for (int64_t q0 = 0; q0 < nq; q0 += chunk) {
         const int64_t qn_ = std::min(chunk, nq - q0);
-        launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cap, st);
-        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, rank_every ? nb : 0, cnt + q0, cand, cap, k,
-                               k_q ? k_q + q0 : nullptr, sqrt_out, out_ids + q0 * k, out_dist + q0 * k, st);
+        launch_mfma_filter(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, thr + q0, rt, cnt + q0, cand, cand_ap, cap, st);
+        if (refine)
+            launch_sample_thresholds(cand_ap, qn_, cap, cnt + q0, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
+                                     reinterpret_cast<int *>(stat + 1), dim, thr2 + q0, st);
+        launch_rank_candidates(queries + q0 * dim, qn_, dim, base, rank_every ? nb : 0, cnt + q0, cand,
+                               refine ? cand_ap : nullptr, refine ? thr2 + q0 : nullptr, cap, k,
+                               k_q ? k_q + q0 : nullptr, sqrt_out,
+                               out_ids + q0 * k, out_dist + q0 * k, st);
     }
     if (rank_every) {
         // the shared buffers are free again only once the stream has run

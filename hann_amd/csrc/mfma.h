@@ -17,10 +17,14 @@ void launch_thresholds(const double *tau_rank, int k, int nq, const float *qnorm
                        const int *inexact, int d, float *thr, hipStream_t st);
 // append every row with approx distance <= thr[q] to q's candidate list
 // (cnt[q] > cap: more candidates than room); a workgroup covers rt_per_wg
-// tiles of 128 rows
+// tiles of 128 rows and holds mfma_filter_slots(dp) LDS slots per query.
+// Where mfma_filter_writes_approx(dp) (dp <= 128), each candidate's
+// approximate distance goes to cand_ap (same layout as cand_row)
+bool mfma_filter_writes_approx(int dp);
+int mfma_filter_slots(int dp);
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                        const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row, int cap,
-                        hipStream_t st);
+                        const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row,
+                        float *cand_ap, int cap, hipStream_t st);
 // every approximate distance: dense[q * nb + row]
 void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                        const float *xn, float *dense, hipStream_t st);
@@ -28,17 +32,21 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
 void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, int64_t stride, int64_t n, __bf16 *out,
                         float *out_n, hipStream_t st);
 // filter thresholds from the k-th smallest approximate distance of each
-// query's ns sample rows (approx[q * ns + j])
+// query's ns sample rows (approx[q * ns + j]), or, with cnt, of its first
+// cnt[q] values (its candidates; +inf when fewer than k or more than ns)
 // (k_q: each query's own k, clamped to [1, k], or null)
-void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const int32_t *k_q,
-                              const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, float *thr,
-                              hipStream_t st);
+void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, const int32_t *cnt, int k,
+                              const int32_t *k_q, const float *qnorm_f, const unsigned *max_norm, const int *inexact,
+                              int d, float *thr, hipStream_t st);
 // exact reference-order ranking of the candidates; a query whose candidates
 // overflowed (cnt > cap) ranks all nb rows instead when nb > 0, and is left
 // untouched when nb == 0
+// cand_ap and thr2 (or null): only candidates with cand_ap <= thr2[q] are
+// ranked
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
-                            const int32_t *cnt, const int32_t *cand_row, int cap, int k, const int32_t *k_q,
-                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st);
+                            const int32_t *cnt, const int32_t *cand_row, const float *cand_ap, const float *thr2,
+                            int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids, double *out_dist,
+                            hipStream_t st);
 // the same over int64 candidate rows (rows[q * cap + i], i < cnt[q])
 void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *base, int64_t nb, const int32_t *cnt,
                         const int64_t *rows, int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids,

@@ -324,13 +324,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // registers and the next row tile's rows fetched into registers while the
 // current tile is multiplied and tested (LDS holds one row tile, so two
 // workgroups share a CU). Grid: x = query tiles, y = row groups, so the
-// workgroups in flight share their row tiles in L2.
+// workgroups in flight share their row tiles in L2. Each candidate is
+// written with its approximate distance (cand_ap), from which a tighter
+// threshold follows; a (query, row half) has MF_SLOTS_R / 2 LDS slots.
+constexpr int MF_SLOTS_R = 16;
+constexpr int MF_OVF_R = 512;
 template <bool DENSE, int KS>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict__ X, int64_t nb,
               const float *__restrict__ qn, const float *__restrict__ xn, const float *__restrict__ thr, int rt_per_wg,
-              int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row, int cap, float *__restrict__ dense,
-              int64_t ld_dense, int64_t rg0) {
+              int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row, float *__restrict__ cand_ap, int cap,
+              float *__restrict__ dense, int64_t ld_dense, int64_t rg0) {
     constexpr int DP = KS * 16;
     constexpr int LD = DP + MF_PAD;
     constexpr int VECS = DP / 8;   // 16-byte vectors per row
@@ -342,9 +346,9 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
     float *s_qn = s_xn + MF_TILE;                                  // the query tile's norms
     float *s_thr = s_qn + MF_TILE;                                 // and thresholds (-inf past nq)
     int32_t *s_cc = reinterpret_cast<int32_t *>(s_thr + MF_TILE);  // [MF_TILE][2] counts
-    int32_t *s_cr = s_cc + 2 * MF_TILE;                             // [MF_TILE][2][MF_SLOTS / 2]
-    int2 *s_ov = reinterpret_cast<int2 *>(s_cr + MF_TILE * MF_SLOTS);   // [MF_OVF] (query, row)
-    int32_t *s_on = reinterpret_cast<int32_t *>(s_ov + MF_OVF);
+    int2 *s_cr = reinterpret_cast<int2 *>(s_cc + 2 * MF_TILE);      // [MF_TILE][2][MF_SLOTS_R / 2] (row, approx)
+    int4 *s_ov = reinterpret_cast<int4 *>(s_cr + MF_TILE * MF_SLOTS_R);   // [MF_OVF_R] (query, row, approx)
+    int32_t *s_on = reinterpret_cast<int32_t *>(s_ov + MF_OVF_R);
     const int tid = threadIdx.x;
     const int wave = tid / 64, lane = tid % 64;
     const int wm = wave / 2, wn = wave % 2;
@@ -387,7 +391,7 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
         if (tid < MF_TILE) xreg = r0 + tid < nb ? xn[r0 + tid] : 0.f;
     };
     fetch(rbase);
-    constexpr int SL = MF_SLOTS / 2;
+    constexpr int SL = MF_SLOTS_R / 2;
     // the running candidate counts of each query in this wave's row half,
     // wave-uniform: lanes 0-31's count in the low 16 bits, lanes 32-63's in
     // the high 16 (at most 64 rt_per_wg rows each)
@@ -473,10 +477,10 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
                                                        : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
                             const int slot = (int)(lane < 32 ? cq[i][g][u] & 0xffffu : cq[i][g][u] >> 16) + rank;
                             if (slot < SL) {
-                                s_cr[(ql * 2 + wn) * SL + slot] = (int32_t)gr;
+                                s_cr[(ql * 2 + wn) * SL + slot] = make_int2((int32_t)gr, __float_as_int(approx));
                             } else {
                                 const int o = atomicAdd(s_on, 1);
-                                if (o < MF_OVF) s_ov[o] = make_int2(ql, (int32_t)gr);
+                                if (o < MF_OVF_R) s_ov[o] = make_int4(ql, (int32_t)gr, __float_as_int(approx), 0);
                             }
                         }
                         cq[i][g][u] += (unsigned)__popc(mlo) + ((unsigned)__popc(mhi) << 16);
@@ -502,19 +506,24 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
         const int n1 = s_cc[ql * 2 + 1] < SL ? s_cc[ql * 2 + 1] : SL;
         if (gq >= nq || n0 + n1 == 0) continue;
         const int base = atomicAdd(&cnt[gq], n0 + n1);
-        for (int i = 0; i < n0 && base + i < cap; ++i) cand_row[gq * cap + base + i] = s_cr[(ql * 2) * SL + i];
-        for (int i = 0; i < n1 && base + n0 + i < cap; ++i)
-            cand_row[gq * cap + base + n0 + i] = s_cr[(ql * 2 + 1) * SL + i];
+        for (int i = 0; i < n0 + n1 && base + i < cap; ++i) {
+            const int2 e = s_cr[(ql * 2 + (i < n0 ? 0 : 1)) * SL + (i < n0 ? i : i - n0)];
+            cand_row[gq * cap + base + i] = e.x;
+            cand_ap[gq * cap + base + i] = __int_as_float(e.y);
+        }
     }
     const int no = *s_on;
-    for (int o = tid; o < (no < MF_OVF ? no : MF_OVF); o += 256) {
-        const int2 e = s_ov[o];
+    for (int o = tid; o < (no < MF_OVF_R ? no : MF_OVF_R); o += 256) {
+        const int4 e = s_ov[o];
         const int64_t gq = q0 + e.x;
         if (gq >= nq) continue;
         const int g = atomicAdd(&cnt[gq], 1);
-        if (g < cap) cand_row[gq * cap + g] = e.y;
+        if (g < cap) {
+            cand_row[gq * cap + g] = e.y;
+            cand_ap[gq * cap + g] = __int_as_float(e.z);
+        }
     }
-    if (no > MF_OVF)
+    if (no > MF_OVF_R)
         for (int ql = tid; ql < MF_TILE; ql += 256)
             if ((s_cc[ql * 2] > SL || s_cc[ql * 2 + 1] > SL) && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
 }
@@ -550,7 +559,10 @@ __device__ __forceinline__ float key_float(unsigned k) {
 // in k_thresholds. One workgroup of 256 threads per query.
 constexpr int ST_BINS = 2048;
 
-__global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t ns, int kmax,
+// cnt non-null: query q's values are its first cnt[q] of ns (its
+// candidates); fewer than k of them, or more than ns, give no bound (+inf)
+__global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t stride,
+                                                           const int32_t *__restrict__ cnt, int kmax,
                                                            const int32_t *__restrict__ k_q,
                                                            const float *__restrict__ qnorm_f,
                                                            const unsigned *__restrict__ max_norm,
@@ -563,11 +575,12 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
     const int64_t q = blockIdx.x;
     int k = k_q ? k_q[q] : kmax;
     k = k < 1 ? 1 : (k > kmax ? kmax : k);
-    if (k > ns) {
+    const int64_t ns = cnt ? (int64_t)cnt[q] : stride;
+    if (k > ns || ns > stride) {
         if (tid == 0) thr[q] = __builtin_inff();
         return;
     }
-    const float *a = approx + q * ns;
+    const float *a = approx + q * stride;
     // the range of the query's keys
     unsigned lo = 0xFFFFFFFFu, hi = 0u;
     for (int64_t i = tid; i < ns; i += 256) {
@@ -636,17 +649,22 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
 
 // Exact reference-order rank of every candidate, then the k best by
 // (rank, row). One wave per query. Queries whose candidate list overflowed
-// write nothing (the host reruns them on the exact path).
+// write nothing (the host reruns them on the exact path). With cand_ap, only
+// the candidates whose approximate distance is within thr2[q] are ranked:
+// they are packed 64 at a time through LDS, so every lane ranks a row.
 template <typename CT>
 __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict__ queries, int d,
                                                         const float *__restrict__ base,
                                                         const int32_t *__restrict__ cnt,
-                                                        const CT *__restrict__ cand_row, int cap, int64_t nb,
+                                                        const CT *__restrict__ cand_row,
+                                                        const float *__restrict__ cand_ap,
+                                                        const float *__restrict__ thr2, int cap, int64_t nb,
                                                         int k, const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
                                                         int64_t *__restrict__ out_ids, double *__restrict__ out_dist) {
     extern __shared__ __align__(16) unsigned char s_raw[];
     float *s_keys = reinterpret_cast<float *>(s_raw);
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + ccap);
+    int64_t *s_buf = s_ids + ccap;   // [2 kWave] rows waiting to be ranked
     const int lane = threadIdx.x;
     const int64_t q = blockIdx.x;
     // a query whose candidates overflowed the room ranks every row (nb > 0),
@@ -659,16 +677,40 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
     WaveTopK sel;
     sel.init(s_keys, s_ids, kq, ccap);
     const float *qv = queries + q * d;
-    for (int64_t b = 0; b < n; b += kWave) {
-        const int64_t t = b + lane;
-        const bool valid = t < n;
+    auto rank = [&](int64_t row, bool valid) {
         float key = 0.f;
-        int64_t row = 0;
-        if (valid) {
-            row = every ? t : (int64_t)cand_row[q * cap + t];
-            key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
-        }
+        if (valid) key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
         sel.push(key, row, valid);
+    };
+    if (every || !cand_ap) {
+        for (int64_t b = 0; b < n; b += kWave) {
+            const int64_t t = b + lane;
+            const bool valid = t < n;
+            rank(valid ? (every ? t : (int64_t)cand_row[q * cap + t]) : 0, valid);
+        }
+    } else {
+        const float t2 = thr2[q];
+        int nbuf = 0;   // wave-uniform
+        for (int64_t b = 0; b < n; b += kWave) {
+            const int64_t t = b + lane;
+            const bool ok = t < n && cand_ap[q * cap + t] <= t2;
+            const unsigned long long m = __builtin_amdgcn_ballot_w64(ok);
+            if (ok) {
+                const int at = nbuf + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                                      __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                s_buf[at] = (int64_t)cand_row[q * cap + t];
+            }
+            nbuf += __popcll(m);
+            __syncthreads();
+            if (nbuf >= kWave) {
+                rank(s_buf[lane], true);
+                __syncthreads();   // every lane has read its row
+                if (lane < nbuf - kWave) s_buf[lane] = s_buf[kWave + lane];
+                nbuf -= kWave;
+                __syncthreads();
+            }
+        }
+        if (nbuf > 0) rank(lane < nbuf ? s_buf[lane] : 0, lane < nbuf);
     }
     int got = sel.finish();
     got = got < kq ? got : kq;
@@ -714,14 +756,16 @@ static size_t mfma_lds(int dp, bool dense) {
 
 static size_t mfma_lds_r(int dp, bool dense) {
     return (size_t)dev::MF_TILE * (dp + dev::MF_PAD) * 2 + (size_t)3 * dev::MF_TILE * 4 +
-           (dense ? 0 : (size_t)dev::MF_TILE * (2 + dev::MF_SLOTS) * 4 + (size_t)dev::MF_OVF * 8 + 16);
+           (dense ? 0
+                  : (size_t)dev::MF_TILE * 2 * 4 + (size_t)dev::MF_TILE * dev::MF_SLOTS_R * 8 +
+                        (size_t)dev::MF_OVF_R * 16 + 16);
 }
 
 // dp <= 128: the register-resident kernel, grid (query tiles, row groups)
 template <bool DENSE>
 static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                          const float *xn, const float *thr, int rt, int32_t *cnt, int32_t *cand_row, int cap,
-                          float *dense, int64_t ld_dense, hipStream_t st) {
+                          const float *xn, const float *thr, int rt, int32_t *cnt, int32_t *cand_row,
+                          float *cand_ap, int cap, float *dense, int64_t ld_dense, hipStream_t st) {
     const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt), qt = ceil_div(nq, dev::MF_TILE);
     if (qt > 0x7fffffff) fail(HANN_ERR_UNSUPPORTED, "BF16 candidate pass: too many queries");
     const size_t shm = mfma_lds_r(dp, DENSE);
@@ -733,7 +777,7 @@ static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t 
         HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass_r<DENSE, KS>,                               \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                    \
         hipLaunchKernelGGL((dev::k_mfma_pass_r<DENSE, KS>), grid, dim3(256), shm, st, Q, nq, X, nb, qn, xn,     \
-                           thr, rt, cnt, cand_row, cap, dense, ld_dense, y0);                                   \
+                           thr, rt, cnt, cand_row, cand_ap, cap, dense, ld_dense, y0);                          \
         HANN_LAUNCH_CHECK();                                                                                    \
     } while (0)
         switch (dp) {
@@ -747,13 +791,18 @@ static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t 
     }
 }
 
+bool mfma_filter_writes_approx(int dp) { return dp <= dev::MF_KC; }
+
+int mfma_filter_slots(int dp) { return dp <= dev::MF_KC ? dev::MF_SLOTS_R : dev::MF_SLOTS; }
+
 void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                        const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row, int cap,
-                        hipStream_t st) {
+                        const float *xn, const float *thr, int rt_per_wg, int32_t *cnt, int32_t *cand_row,
+                        float *cand_ap, int cap, hipStream_t st) {
     if (nq == 0 || nb == 0) return;
     if (rt_per_wg < 1) rt_per_wg = 1;
     if (dp <= dev::MF_KC) {
-        launch_pass_r<false>(Q, nq, X, nb, dp, qn, xn, thr, rt_per_wg, cnt, cand_row, cap, nullptr, 0, st);
+        if (!cand_ap) fail(HANN_ERR_INVALID_ARG, "BF16 candidate pass: no room for the approximate distances");
+        launch_pass_r<false>(Q, nq, X, nb, dp, qn, xn, thr, rt_per_wg, cnt, cand_row, cand_ap, cap, nullptr, 0, st);
         return;
     }
     const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt_per_wg), qt = ceil_div(nq, dev::MF_TILE);
@@ -775,7 +824,7 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
     if (nq == 0 || nb == 0) return;
     const int rt = 4;
     if (dp <= dev::MF_KC) {
-        launch_pass_r<true>(Q, nq, X, nb, dp, qn, xn, nullptr, rt, nullptr, nullptr, 0, dense, nb, st);
+        launch_pass_r<true>(Q, nq, X, nb, dp, qn, xn, nullptr, rt, nullptr, nullptr, nullptr, 0, dense, nb, st);
         return;
     }
     const int64_t rg = ceil_div(nb, (int64_t)dev::MF_TILE * rt), qt = ceil_div(nq, dev::MF_TILE);
@@ -799,28 +848,32 @@ void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, int64_t stride
     HANN_LAUNCH_CHECK();
 }
 
-void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, int k, const int32_t *k_q,
-                              const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, float *thr,
-                              hipStream_t st) {
+void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, const int32_t *cnt, int k,
+                              const int32_t *k_q, const float *qnorm_f, const unsigned *max_norm, const int *inexact,
+                              int d, float *thr, hipStream_t st) {
     if (nq == 0) return;
-    hipLaunchKernelGGL(dev::k_sample_thresholds, dim3((unsigned)nq), dim3(256), 0, st, approx, ns, k, k_q, qnorm_f,
-                       max_norm, inexact, d, thr);
+    hipLaunchKernelGGL(dev::k_sample_thresholds, dim3((unsigned)nq), dim3(256), 0, st, approx, ns, cnt, k, k_q,
+                       qnorm_f, max_norm, inexact, d, thr);
     HANN_LAUNCH_CHECK();
 }
 
+static size_t rank_lds(int ccap) { return (size_t)ccap * 12 + (size_t)2 * dev::kWave * 8 + 16; }
+
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
-                            const int32_t *cnt, const int32_t *cand_row, int cap, int k, const int32_t *k_q,
-                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st) {
+                            const int32_t *cnt, const int32_t *cand_row, const float *cand_ap, const float *thr2,
+                            int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids, double *out_dist,
+                            hipStream_t st) {
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
-    const size_t shm = (size_t)ccap * 12 + 16;
+    const size_t shm = rank_lds(ccap);
     HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int32_t>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates<int32_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d,
-                           base, cnt + q0, cand_row + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
-                           sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
+                           base, cnt + q0, cand_row + q0 * (int64_t)cap, cand_ap ? cand_ap + q0 * (int64_t)cap : nullptr,
+                           thr2 ? thr2 + q0 : nullptr, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap, sqrt_out,
+                           out_ids + q0 * k, out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
     }
 }
@@ -830,13 +883,14 @@ void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *ba
                         double *out_dist, hipStream_t st) {
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
-    const size_t shm = (size_t)ccap * 12 + 16;
+    const size_t shm = rank_lds(ccap);
     HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int64_t>,
                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates<int64_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d,
-                           d, base, cnt + q0, rows + q0 * (int64_t)cap, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap,
+                           d, base, cnt + q0, rows + q0 * (int64_t)cap, nullptr, nullptr, cap, nb, k,
+                           k_q ? k_q + q0 : nullptr, ccap,
                            sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
     }
