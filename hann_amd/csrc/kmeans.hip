@@ -613,21 +613,45 @@ void launch_gather_rows(const float *src, int d, const int64_t *idx, int64_t n, 
 namespace dev {
 constexpr int PP_BLOCK = 256;
 
-// D[i] = min(D[i], |x_i - c|^2) (first: D[i] = |x_i - c|^2); per-block sum
-// of D in FP64 (fixed tree order)
+// D[i] = min(D[i], e(x_i, c)) in the reference FP32 order, A[i] = the index
+// of the center that gave D[i] (first: D[i] = e(x_i, c_0), A[i] = 0); per
+// block the FP64 tree sum of D, rewritten only where some D changed. A point
+// is skipped when |c_A - c|^2 >= 4 D margin (cd[A], FP64): by the triangle
+// inequality its distance to c is then at least sqrt(D) with room for the
+// FP32 rounding of both distances, so min(D, e) = D. The skip changes no
+// value, only which distances are computed.
 template <bool FIRST>
 __global__ void __launch_bounds__(PP_BLOCK) k_pp_update(const float *__restrict__ data, int64_t n, int64_t row_stride,
-                                                        int d, const float *__restrict__ c, float *__restrict__ D,
+                                                        int d, const float *__restrict__ c, int ci,
+                                                        const double *__restrict__ cd, double margin,
+                                                        float *__restrict__ D, int32_t *__restrict__ A,
                                                         double *__restrict__ bsum) {
     __shared__ double s_sum[PP_BLOCK];
     const int64_t i = (int64_t)blockIdx.x * PP_BLOCK + threadIdx.x;
     double v = 0.0;
+    int changed = 0;
     if (i < n) {
-        const float e = sane_key(sqdiff_exact(PtrLoad{data + i * row_stride}, PtrLoad{c}, d));
-        const float dv = FIRST ? e : (e < D[i] ? e : D[i]);
-        D[i] = dv;
-        v = dv;
+        if (FIRST) {
+            const float e = sane_key(sqdiff_exact(PtrLoad{data + i * row_stride}, PtrLoad{c}, d));
+            D[i] = e;
+            A[i] = ci;
+            v = e;
+            changed = 1;
+        } else {
+            float dv = D[i];
+            if (!(cd[A[i]] >= 4.0 * (double)dv * margin)) {
+                const float e = sane_key(sqdiff_exact(PtrLoad{data + i * row_stride}, PtrLoad{c}, d));
+                if (e < dv) {
+                    dv = e;
+                    D[i] = e;
+                    A[i] = ci;
+                    changed = 1;
+                }
+            }
+            v = dv;
+        }
     }
+    if (!__syncthreads_or(changed)) return;
     s_sum[threadIdx.x] = v;
     __syncthreads();
     for (int o = PP_BLOCK / 2; o > 0; o >>= 1) {
@@ -637,14 +661,35 @@ __global__ void __launch_bounds__(PP_BLOCK) k_pp_update(const float *__restrict_
     if (threadIdx.x == 0) bsum[blockIdx.x] = s_sum[0];
 }
 
+// cd[i] = |c_i - c_j|^2 in FP64 for the centers i < j (one wave each)
+__global__ void __launch_bounds__(256) k_pp_center_dist(const float *__restrict__ cents, int d, int j,
+                                                        double *__restrict__ cd) {
+    const int w = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    if (w >= j) return;
+    const float *a = cents + (int64_t)w * d, *b = cents + (int64_t)j * d;
+    double s = 0.0;
+    for (int e = lane; e < d; e += 64) {
+        const double t = (double)a[e] - (double)b[e];
+        s += t * t;
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (lane == 0) cd[w] = s;
+}
+
 // one workgroup: the row whose D-prefix first exceeds u * total; copy it to
-// the next center slot
+// the next center slot. The chunk that holds the target is walked block by
+// block (block sums, then, in the block that crosses, its rows one by one
+// in order); the rows of that block are first staged in LDS by the whole
+// workgroup, so the walk reads no global memory.
 __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsum, int64_t nblocks,
                                                   const float *__restrict__ D, int64_t n, const double *__restrict__ u,
                                                   const float *__restrict__ data, int64_t row_stride, int d,
                                                   float *__restrict__ cent_out) {
     __shared__ double s_tot[1024];
-    __shared__ int64_t s_pick;
+    __shared__ float s_d[PP_BLOCK];
+    __shared__ double s_run;
+    __shared__ int64_t s_pick, s_last, s_blk;
+    __shared__ int s_owner;
     const int t = threadIdx.x;
     // per-thread chunk sums in block order, then an inclusive scan over threads
     const int64_t per = (nblocks + 1023) / 1024;
@@ -660,7 +705,11 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
         __syncthreads();
     }
     const double total = s_tot[1023];
-    if (t == 0) s_pick = -1;
+    if (t == 0) {
+        s_pick = -1;
+        s_last = -1;
+        s_owner = -1;
+    }
     __syncthreads();
     if (!(total > 0.0)) {
         if (t == 0) {
@@ -670,28 +719,54 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
     } else {
         const double target = *u * total;
         // [previous inclusive sum, own inclusive sum): disjoint intervals,
-        // so exactly one thread holds the target
+        // so at most one thread holds the target
         const double excl = t > 0 ? s_tot[t - 1] : 0.0;
         if (excl <= target && target < s_tot[t]) {
-            // this thread's chunk holds the crossing: walk its blocks, then the rows
-            double run = excl;
-            int64_t pick = -1, last_pos = -1;
-            for (int64_t b = b0; b < b1 && pick < 0; ++b) {
-                if (run + bsum[b] <= target) {
-                    run += bsum[b];
-                    continue;
+            s_owner = t;
+            s_run = excl;
+            s_blk = b0;
+        }
+        __syncthreads();
+        const int ot = s_owner;
+        if (ot >= 0) {
+            const int64_t ob1 = ot * per + per < nblocks ? ot * per + per : nblocks;
+            for (;;) {
+                if (t == ot) {
+                    // the next block whose sum carries the running sum past the target
+                    double run = s_run;
+                    int64_t b = s_blk;
+                    while (b < ob1 && run + bsum[b] <= target) run += bsum[b++];
+                    s_run = run;
+                    s_blk = b;
                 }
-                const int64_t r1 = (b + 1) * PP_BLOCK < n ? (b + 1) * PP_BLOCK : n;
-                for (int64_t r = b * PP_BLOCK; r < r1; ++r) {
-                    if (D[r] > 0.f) last_pos = r;
-                    run += (double)D[r];
-                    if (run > target && D[r] > 0.f) {
-                        pick = r;
-                        break;
+                __syncthreads();
+                const int64_t b = s_blk;
+                if (b >= ob1) break;
+                const int64_t r0 = b * PP_BLOCK;
+                if (t < PP_BLOCK) s_d[t] = r0 + t < n ? D[r0 + t] : 0.f;
+                __syncthreads();
+                if (t == ot) {
+                    double run = s_run;
+                    int64_t pick = -1, last = s_last;
+                    const int64_t rows = n - r0 < PP_BLOCK ? n - r0 : PP_BLOCK;
+                    for (int r = 0; r < (int)rows; ++r) {
+                        const float dr = s_d[r];
+                        if (dr > 0.f) last = r0 + r;
+                        run += (double)dr;
+                        if (run > target && dr > 0.f) {
+                            pick = r0 + r;
+                            break;
+                        }
                     }
+                    s_run = run;
+                    s_last = last;
+                    s_pick = pick;
+                    s_blk = b + 1;
                 }
+                __syncthreads();
+                if (s_pick >= 0) break;
             }
-            s_pick = pick >= 0 ? pick : last_pos;
+            if (t == ot && s_pick < 0) s_pick = s_last;
         }
     }
     __syncthreads();
@@ -717,21 +792,32 @@ void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, in
     if (k <= 0 || n <= 0) return;
     const int64_t nb = ceil_div(n, (int64_t)PP_BLOCK);
     DevBuf<float> D((size_t)n);
+    DevBuf<int32_t> A((size_t)n);
     DevBuf<double> bsum((size_t)nb);
+    DevBuf<double> cd((size_t)k);
+    // room over the FP32 rounding of the distances (at most (d + 2) u of
+    // each), with a wide margin
+    const double margin = 1.0 + 8.0 * (double)(d + 2) / 16777216.0 + 1e-5;
     // the first center
     HANN_HIP(hipMemcpy2DAsync(cents, d * sizeof(float), data + first * row_stride, d * sizeof(float),
                               d * sizeof(float), 1, hipMemcpyDeviceToDevice, st));
     for (int j = 1; j < k; ++j) {
         if (j == 1)
-            hipLaunchKernelGGL(dev::k_pp_update<true>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride, d,
-                               cents, D.get(), bsum.get());
+            hipLaunchKernelGGL(dev::k_pp_update<true>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride,
+                               d, cents, 0, nullptr, margin, D.get(), A.get(), bsum.get());
         else
-            hipLaunchKernelGGL(dev::k_pp_update<false>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride, d,
-                               cents + (int64_t)(j - 1) * d, D.get(), bsum.get());
+            hipLaunchKernelGGL(dev::k_pp_update<false>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n,
+                               row_stride, d, cents + (int64_t)(j - 1) * d, j - 1, cd.get(), margin, D.get(), A.get(),
+                               bsum.get());
         HANN_LAUNCH_CHECK();
         hipLaunchKernelGGL(dev::k_pp_pick, dim3(1), dim3(1024), 0, st, bsum.get(), nb, D.get(), n, u_dev + (j - 1), data,
                            row_stride, d, cents + (int64_t)j * d);
         HANN_LAUNCH_CHECK();
+        if (j + 1 < k) {
+            hipLaunchKernelGGL(dev::k_pp_center_dist, dim3((unsigned)ceil_div((int64_t)j, (int64_t)4)), dim3(256), 0,
+                               st, cents, d, j, cd.get());
+            HANN_LAUNCH_CHECK();
+        }
     }
     HANN_HIP(hipStreamSynchronize(st));
 }

@@ -1140,17 +1140,22 @@ __global__ void __launch_bounds__(64 * XNW) k_topk_rows(TopkRowsArgs t) {
 }
 
 // the largest row norm of the base (as float bits; *out zeroed before)
+// one wave per row in turn (lanes read consecutive columns), 64 rows per
+// workgroup; the norms are widened by 2^-10, which covers their FP32 error
+constexpr int NORM_ROWS = 64;
 __global__ void __launch_bounds__(256) k_row_norm_max(const float *__restrict__ base, int64_t nb, int d,
                                                       unsigned *__restrict__ out) {
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     float m = 0.f;
-    if (r < nb) {
+    for (int i = wave; i < NORM_ROWS; i += 4) {
+        const int64_t r = (int64_t)blockIdx.x * NORM_ROWS + i;
+        if (r >= nb) break;
         float s = 0.f;
-        for (int e = 0; e < d; ++e) s = __builtin_fmaf(base[r * d + e], base[r * d + e], s);
-        m = __builtin_sqrtf(s) * (1.f + 1.f / 1024.f);
+        for (int e = lane; e < d; e += 64) s = __builtin_fmaf(base[r * d + e], base[r * d + e], s);
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+        m = fmaxf(m, __builtin_sqrtf(s) * (1.f + 1.f / 1024.f));
     }
-    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, __float_as_uint(m));
+    if (lane == 0) atomicMax(out, __float_as_uint(m));
 }
 
 // per query: the key scale 2^(14 - e) with (|q| + max |x|)^2 < 2^e, and the
@@ -1714,10 +1719,13 @@ void launch_topk_rows(int metric, const float *queries, int64_t nq, const float 
     for (int64_t q0 = 0; q0 < nq; q0 += qc) {
         const int64_t qn = std::min(qc, nq - q0);
         const float *qs = queries + q0 * dim;
-        launch_fill_i32(reinterpret_cast<int32_t *>(A), (int64_t)(o_zero_end / 4), 0, st);
-        hipLaunchKernelGGL(dev::k_row_norm_max, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, st, base, nb, dim,
-                           norm);
-        HANN_LAUNCH_CHECK();
+        // the zeroed block (the norm bound once, in the first round)
+        launch_fill_i32(reinterpret_cast<int32_t *>(A), (int64_t)((q0 == 0 ? o_zero_end : o_norm) / 4), 0, st);
+        if (q0 == 0) {
+            hipLaunchKernelGGL(dev::k_row_norm_max, dim3((unsigned)ceil_div(nb, (int64_t)dev::NORM_ROWS)), dim3(256),
+                               0, st, base, nb, dim, norm);
+            HANN_LAUNCH_CHECK();
+        }
         hipLaunchKernelGGL(dev::k_topk_rows_prep, dim3((unsigned)ceil_div(qn, 256)), dim3(256), 0, st, qs, qn, dim,
                            norm, P.ns_pad, nb, q_scale, row_off, qslots, qtot);
         HANN_LAUNCH_CHECK();

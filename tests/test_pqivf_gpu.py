@@ -1118,15 +1118,32 @@ def test_list_scan_after_mutations(hann):
 
 # ------------------------------------------- k-means++ (build extension) --
 @pytest.mark.parametrize("n,d,k,iters", [(3000, 16, 32, 0), (3000, 16, 32, 3), (20000, 64, 100, 2),
-                                         (700, 8, 256, 2), (300, 4, 300, 1), (5000, 128, 64, 1)])
+                                         (700, 8, 256, 2), (300, 4, 300, 1), (5000, 128, 64, 1),
+                                         (60000, 128, 600, 0)])
 def test_kmeanspp_matches_oracle(hann, oracle, n, d, k, iters):
     """k-means++ seeding on the device equals the oracle's restatement of its
-    order of operations bit for bit, seeds and Lloyd iterations alike."""
+    order of operations bit for bit, seeds and Lloyd iterations alike. The
+    device skips the distances that the triangle inequality over the centers
+    shows cannot lower a point's D (most of them, once the centers cover the
+    mixture's components), which must change nothing."""
     rng = np.random.default_rng(n + d + k)
     cen = rng.uniform(-10, 10, (max(k // 3, 1), d)).astype(np.float32)
     x = (cen[rng.integers(0, len(cen), n)] + rng.standard_normal((n, d))).astype(np.float32)
     got = P().KMeans(x, k, iters, 7 + n, init=1)
     want, _ = oracle.kmeans_init(x, k, iters, oracle.stream_seed(7 + n, 0, 0), init=1, nthreads=8)
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+def test_kmeanspp_duplicate_rows_matches_oracle(hann, oracle):
+    """Many rows repeated exactly (D reaches 0 for them, and their blocks'
+    sums stop changing) and a few far outliers: the seeds still equal the
+    oracle's."""
+    rng = np.random.default_rng(12)
+    base = rng.standard_normal((40, 32)).astype(np.float32) * 5
+    x = base[rng.integers(0, 40, 30000)].copy()
+    x[rng.integers(0, 30000, 50)] += np.float32(300.0)
+    got = P().KMeans(x, 120, 0, 99, init=1)
+    want, _ = oracle.kmeans_init(x, 120, 0, oracle.stream_seed(99, 0, 0), init=1, nthreads=8)
     assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
 
 
