@@ -34,12 +34,14 @@ void launch_residuals(const float *vecs, int d, const float *coarse, const int32
                       int64_t n, float *out, hipStream_t st);
 void launch_gather_rows(const float *src, int d, const int64_t *idx, int64_t n, float *out, hipStream_t st);
 
-// k-means++ seeding of one group (columns [0, d) of rows row_stride floats
-// apart) on the device: cents[0] = row first; then for j = 1 .. k-1 the row
-// where the running prefix of D^2 (squared distance to the nearest chosen
-// center, reference FP32 order; prefix in FP64, fixed order) first exceeds
-// u[j-1] times its total (a uniform row when the total is 0).
-void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, int k, int64_t first,
-                     const double *u_dev, float *cents, hipStream_t st);
+// k-means++ seeding of `groups` groups side by side on the device (group g:
+// columns [g d, g d + d) of rows row_stride floats apart, its centers at
+// cents + g k d, its draws at u_dev + g (k - 1)): cents[0] = row first[g];
+// then for j = 1 .. k-1 the row where the running prefix of D^2 (squared
+// distance to the nearest chosen center, reference FP32 order; prefix in
+// FP64, fixed order) first exceeds u[j-1] times its total (a uniform row
+// when the total is 0).
+void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, int k, int groups,
+                     const int64_t *first, const double *u_dev, float *cents, hipStream_t st);
 
 }  // namespace hann

@@ -620,13 +620,23 @@ constexpr int PP_BLOCK = 256;
 // inequality its distance to c is then at least sqrt(D) with room for the
 // FP32 rounding of both distances, so min(D, e) = D. The skip changes no
 // value, only which distances are computed.
+// Groups (blockIdx.y; the subspaces of PQ training) run side by side: group
+// g's rows start at data + g d, its centers at cents + g k d, and its D, A,
+// block sums and center distances at g n, g n, g nblocks and g k.
 template <bool FIRST>
 __global__ void __launch_bounds__(PP_BLOCK) k_pp_update(const float *__restrict__ data, int64_t n, int64_t row_stride,
-                                                        int d, const float *__restrict__ c, int ci,
-                                                        const double *__restrict__ cd, double margin,
-                                                        float *__restrict__ D, int32_t *__restrict__ A,
-                                                        double *__restrict__ bsum) {
+                                                        int d, const float *__restrict__ cents, int k, int ci,
+                                                        const double *__restrict__ cd_all, double margin,
+                                                        float *__restrict__ D_all, int32_t *__restrict__ A_all,
+                                                        double *__restrict__ bsum_all) {
     __shared__ double s_sum[PP_BLOCK];
+    const int64_t g = blockIdx.y;
+    data += g * d;
+    const float *c = cents + (g * k + ci) * d;
+    const double *cd = cd_all + g * k;
+    float *D = D_all + g * n;
+    int32_t *A = A_all + g * n;
+    double *bsum = bsum_all + g * (int64_t)gridDim.x;
     const int64_t i = (int64_t)blockIdx.x * PP_BLOCK + threadIdx.x;
     double v = 0.0;
     int changed = 0;
@@ -661,11 +671,14 @@ __global__ void __launch_bounds__(PP_BLOCK) k_pp_update(const float *__restrict_
     if (threadIdx.x == 0) bsum[blockIdx.x] = s_sum[0];
 }
 
-// cd[i] = |c_i - c_j|^2 in FP64 for the centers i < j (one wave each)
-__global__ void __launch_bounds__(256) k_pp_center_dist(const float *__restrict__ cents, int d, int j,
-                                                        double *__restrict__ cd) {
+// cd[i] = |c_i - c_j|^2 in FP64 for the centers i < j (one wave each; group
+// blockIdx.y)
+__global__ void __launch_bounds__(256) k_pp_center_dist(const float *__restrict__ cents_all, int d, int k, int j,
+                                                        double *__restrict__ cd_all) {
     const int w = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (w >= j) return;
+    const float *cents = cents_all + (int64_t)blockIdx.y * k * d;
+    double *cd = cd_all + (int64_t)blockIdx.y * k;
     const float *a = cents + (int64_t)w * d, *b = cents + (int64_t)j * d;
     double s = 0.0;
     for (int e = lane; e < d; e += 64) {
@@ -681,10 +694,18 @@ __global__ void __launch_bounds__(256) k_pp_center_dist(const float *__restrict_
 // block (block sums, then, in the block that crosses, its rows one by one
 // in order); the rows of that block are first staged in LDS by the whole
 // workgroup, so the walk reads no global memory.
-__global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsum, int64_t nblocks,
-                                                  const float *__restrict__ D, int64_t n, const double *__restrict__ u,
-                                                  const float *__restrict__ data, int64_t row_stride, int d,
-                                                  float *__restrict__ cent_out) {
+// (one workgroup per group: blockIdx.x; u holds k - 1 draws per group)
+__global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsum_all, int64_t nblocks,
+                                                  const float *__restrict__ D_all, int64_t n,
+                                                  const double *__restrict__ u_all, const float *__restrict__ data_all,
+                                                  int64_t row_stride, int d, int k, int j,
+                                                  float *__restrict__ cents_all) {
+    const int64_t g = blockIdx.x;
+    const double *bsum = bsum_all + g * nblocks;
+    const float *D = D_all + g * n;
+    const double *u = u_all + g * (k - 1) + (j - 1);
+    const float *data = data_all + g * d;
+    float *cent_out = cents_all + (g * k + j) * d;
     __shared__ double s_tot[1024];
     __shared__ float s_d[PP_BLOCK];
     __shared__ double s_run;
@@ -786,36 +807,39 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
 
 }  // namespace dev
 
-void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, int k, int64_t first,
-                     const double *u_dev, float *cents, hipStream_t st) {
+void launch_kmeanspp(const float *data, int64_t n, int64_t row_stride, int d, int k, int groups,
+                     const int64_t *first, const double *u_dev, float *cents, hipStream_t st) {
     using dev::PP_BLOCK;
-    if (k <= 0 || n <= 0) return;
+    if (k <= 0 || n <= 0 || groups <= 0) return;
     const int64_t nb = ceil_div(n, (int64_t)PP_BLOCK);
-    DevBuf<float> D((size_t)n);
-    DevBuf<int32_t> A((size_t)n);
-    DevBuf<double> bsum((size_t)nb);
-    DevBuf<double> cd((size_t)k);
+    if (groups > 65535 || nb > 0x7fffffff) fail(HANN_ERR_UNSUPPORTED, "k-means++: too many groups or rows");
+    DevBuf<float> D((size_t)groups * n);
+    DevBuf<int32_t> A((size_t)groups * n);
+    DevBuf<double> bsum((size_t)groups * nb);
+    DevBuf<double> cd((size_t)groups * k);
     // room over the FP32 rounding of the distances (at most (d + 2) u of
     // each), with a wide margin
     const double margin = 1.0 + 8.0 * (double)(d + 2) / 16777216.0 + 1e-5;
-    // the first center
-    HANN_HIP(hipMemcpy2DAsync(cents, d * sizeof(float), data + first * row_stride, d * sizeof(float),
-                              d * sizeof(float), 1, hipMemcpyDeviceToDevice, st));
+    // the first centers
+    for (int g = 0; g < groups; ++g)
+        HANN_HIP(hipMemcpy2DAsync(cents + (int64_t)g * k * d, d * sizeof(float),
+                                  data + first[g] * row_stride + (int64_t)g * d, d * sizeof(float), d * sizeof(float),
+                                  1, hipMemcpyDeviceToDevice, st));
+    const dim3 ugrid((unsigned)nb, (unsigned)groups);
     for (int j = 1; j < k; ++j) {
         if (j == 1)
-            hipLaunchKernelGGL(dev::k_pp_update<true>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n, row_stride,
-                               d, cents, 0, nullptr, margin, D.get(), A.get(), bsum.get());
+            hipLaunchKernelGGL(dev::k_pp_update<true>, ugrid, dim3(PP_BLOCK), 0, st, data, n, row_stride, d, cents, k,
+                               0, nullptr, margin, D.get(), A.get(), bsum.get());
         else
-            hipLaunchKernelGGL(dev::k_pp_update<false>, dim3((unsigned)nb), dim3(PP_BLOCK), 0, st, data, n,
-                               row_stride, d, cents + (int64_t)(j - 1) * d, j - 1, cd.get(), margin, D.get(), A.get(),
-                               bsum.get());
+            hipLaunchKernelGGL(dev::k_pp_update<false>, ugrid, dim3(PP_BLOCK), 0, st, data, n, row_stride, d, cents,
+                               k, j - 1, cd.get(), margin, D.get(), A.get(), bsum.get());
         HANN_LAUNCH_CHECK();
-        hipLaunchKernelGGL(dev::k_pp_pick, dim3(1), dim3(1024), 0, st, bsum.get(), nb, D.get(), n, u_dev + (j - 1), data,
-                           row_stride, d, cents + (int64_t)j * d);
+        hipLaunchKernelGGL(dev::k_pp_pick, dim3((unsigned)groups), dim3(1024), 0, st, bsum.get(), nb, D.get(), n, u_dev,
+                           data, row_stride, d, k, j, cents);
         HANN_LAUNCH_CHECK();
         if (j + 1 < k) {
-            hipLaunchKernelGGL(dev::k_pp_center_dist, dim3((unsigned)ceil_div((int64_t)j, (int64_t)4)), dim3(256), 0,
-                               st, cents, d, j, cd.get());
+            hipLaunchKernelGGL(dev::k_pp_center_dist, dim3((unsigned)ceil_div((int64_t)j, (int64_t)4), (unsigned)groups),
+                               dim3(256), 0, st, cents, d, k, j, cd.get());
             HANN_LAUNCH_CHECK();
         }
     }

@@ -48,16 +48,18 @@ int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, i
     if (keff <= 0) return 0;
     if (init_mode == 1) {
         // k-means++ (build extension): per group, a uniform first row and
-        // keff - 1 uniform draws for the D^2 sampling, from the group's stream
-        std::vector<double> u((size_t)keff);
-        DevBuf<double> du((size_t)keff);
+        // keff - 1 uniform draws for the D^2 sampling, from the group's
+        // stream; the groups' seedings run side by side
+        std::vector<double> u((size_t)groups * (keff - 1) + 1);
+        std::vector<int64_t> first((size_t)groups);
         for (int g = 0; g < groups; ++g) {
-            const int64_t first = (int64_t)rng_bounded(states[g], (uint64_t)n);
-            for (int64_t j = 0; j + 1 < keff; ++j) u[j] = (double)(rng_next(states[g]) >> 11) * (1.0 / 9007199254740992.0);
-            HANN_HIP(hipMemcpyAsync(du.get(), u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, st));
-            launch_kmeanspp(data + (int64_t)g * dsub, n, row_stride, dsub, (int)keff, first, du.get(),
-                            cents + (int64_t)g * keff * dsub, st);
+            first[g] = (int64_t)rng_bounded(states[g], (uint64_t)n);
+            for (int64_t j = 0; j + 1 < keff; ++j)
+                u[(size_t)g * (keff - 1) + j] = (double)(rng_next(states[g]) >> 11) * (1.0 / 9007199254740992.0);
         }
+        DevBuf<double> du(u.size());
+        HANN_HIP(hipMemcpyAsync(du.get(), u.data(), u.size() * sizeof(double), hipMemcpyHostToDevice, st));
+        launch_kmeanspp(data, n, row_stride, dsub, (int)keff, groups, first.data(), du.get(), cents, st);
     } else {
         // seeds: the first keff entries of a random permutation (pqivf/index.go:804-810)
         std::vector<int64_t> init((size_t)groups * keff);

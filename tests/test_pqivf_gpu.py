@@ -1159,5 +1159,17 @@ def test_train_with_kmeanspp(hann, oracle):
     idx.Train()
     want, _ = oracle.kmeans_init(x, 32, 4, oracle.stream_seed(21, 0, 0), init=1, nthreads=8)
     assert np.array_equal(pq.CoarseCentroids(idx).view(np.uint32), want.view(np.uint32))
+    # the 8 subspaces' codebooks (seeded side by side on the device) are the
+    # oracle's k-means++ runs over the residuals' sub-vectors in list order
+    # (by cluster, then id: pqivf/index.go:520-541), each from its
+    # subspace's stream
+    a = oracle.assign(x, want, nthreads=8)
+    res = (x - want[a])[np.argsort(a, kind="stable")]
+    cb, cbl = pq.Codebooks(idx)
+    for i in range(8):
+        wi, _ = oracle.kmeans_init(np.ascontiguousarray(res[:, 8 * i:8 * i + 8]), 64, 4,
+                                   oracle.stream_seed(21, 0, 1 + i), init=1, nthreads=8)
+        assert cbl[i] == len(wi)
+        assert np.array_equal(cb[i, :len(wi)].view(np.uint32), wi.view(np.uint32)), i
     with pytest.raises(Exception):
         pq.New(64, pq.WithKMeansInit(7))
