@@ -149,9 +149,10 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     // the counts); otherwise it is redone below by the tiled exact path
     const bool rank_every = nb <= kRankEveryMax;
     // rows per workgroup: expected candidates per query and workgroup about
-    // a quarter of its LDS slots
+    // half its LDS slots (the rest, and the workgroup's shared overflow
+    // list, absorb the spread)
     const int slots = mfma_filter_slots(dp);
-    const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb * (slots / 4) / (128 * expect)));
+    const int rt = (int)std::max<int64_t>(1, std::min<int64_t>(64, nb * (slots / 2) / (128 * expect)));
     const int64_t chunk = std::max<int64_t>(128, std::min<int64_t>(nq, (int64_t)(512ll << 20) / (4ll * cap)));
     int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
     int32_t *cand = w.get<int32_t>(13, (size_t)chunk * cap);
