@@ -436,10 +436,20 @@ void PQIVF::reserve_slack(ListData &L, int64_t len) const {
     L.has_codes.reserve(cap);
 }
 
+// threads for host-side copies of a train's vectors
+static size_t host_threads(int64_t work_rows) {
+    const size_t hw = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    return (size_t)std::max<int64_t>(1, std::min<int64_t>((int64_t)hw, work_rows / 65536));
+}
+
 void PQIVF::train() {
     // Train (pqivf/index.go:451-605)
     std::vector<int64_t> all_ids;
     std::vector<const float *> all_vecs;
+    size_t total = pending.size();
+    for (const ListData &L : lists) total += L.ids.size();
+    all_ids.reserve(total);
+    all_vecs.reserve(total);
     for (const ListData &L : lists)
         for (size_t r = 0; r < L.ids.size(); ++r) {
             all_ids.push_back(L.ids[r]);
@@ -453,21 +463,54 @@ void PQIVF::train() {
     if (n < coarse_k)
         fail(HANN_ERR_EMPTY, strf("not enough vectors (%lld) to train coarse quantizer with %lld clusters", (long long)n,
                                   (long long)coarse_k));
-    std::vector<size_t> order((size_t)n);
-    std::iota(order.begin(), order.end(), 0);
-    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return all_ids[a] < all_ids[b]; });
-    std::vector<int64_t> ids((size_t)n);
-    std::vector<float> X((size_t)n * dim);
-    for (int64_t i = 0; i < n; ++i) {
-        ids[i] = all_ids[order[i]];
-        std::memcpy(&X[(size_t)i * dim], all_vecs[order[i]], dim * sizeof(float));
+    // ascending id order (ids are distinct); vectors added in id order need
+    // no sort. The vectors stay where they are (the old lists and the
+    // pending store, both alive until the new state is committed): src(i)
+    // is the i-th vector in id order
+    std::vector<int64_t> order;
+    if (!std::is_sorted(all_ids.begin(), all_ids.end())) {
+        std::vector<std::pair<int64_t, int64_t>> key((size_t)n);
+        for (int64_t i = 0; i < n; ++i) key[i] = {all_ids[i], i};
+        std::sort(key.begin(), key.end());
+        order.resize((size_t)n);
+        for (int64_t i = 0; i < n; ++i) order[i] = key[i].second;
     }
-    all_vecs.clear();
+    auto src = [&](int64_t i) { return all_vecs[order.empty() ? i : order[i]]; };
+    std::vector<int64_t> ids((size_t)n);
+    for (int64_t i = 0; i < n; ++i) ids[i] = all_ids[order.empty() ? i : order[i]];
     ensure_stream();
     const uint64_t epoch = train_epoch;
     const int d = (int)dim;
-    DevBuf<float> dX(X.size());
-    HANN_HIP(hipMemcpyAsync(dX.get(), X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+    DevBuf<float> dX((size_t)n * dim);
+    {
+        // rows that sit one after another in host memory go up in one copy;
+        // many short runs are packed on the host first (in parallel)
+        std::vector<std::pair<int64_t, int64_t>> runs;   // (first row, rows)
+        for (int64_t i = 0; i < n;) {
+            int64_t j = i + 1;
+            while (j < n && src(j) == src(j - 1) + dim) ++j;
+            runs.push_back({i, j - i});
+            i = j;
+            if (runs.size() > 4096) break;
+        }
+        if (runs.size() <= 4096) {
+            for (const auto &r : runs)
+                HANN_HIP(hipMemcpyAsync(dX.get() + r.first * dim, src(r.first), (size_t)r.second * dim * sizeof(float),
+                                        hipMemcpyHostToDevice, stream));
+        } else {
+            std::vector<float> X((size_t)n * dim);
+            const size_t nt = host_threads(n);
+            std::vector<std::thread> pool;
+            for (size_t t = 0; t < nt; ++t)
+                pool.emplace_back([&, t] {
+                    for (int64_t i = (int64_t)t * n / (int64_t)nt; i < (int64_t)(t + 1) * n / (int64_t)nt; ++i)
+                        std::memcpy(&X[(size_t)i * dim], src(i), dim * sizeof(float));
+                });
+            for (auto &th : pool) th.join();
+            HANN_HIP(hipMemcpyAsync(dX.get(), X.data(), X.size() * sizeof(float), hipMemcpyHostToDevice, stream));
+            HANN_HIP(hipStreamSynchronize(stream));
+        }
+    }
 
     // coarse quantizer, optionally on a seeded sample (build extension)
     KMeansScratch sc;
@@ -562,21 +605,53 @@ void PQIVF::train() {
     pmax = pk;
     cb_len = cbl;
     have_codebooks = true;
-    lists.assign((size_t)kc, ListData());
-    id_to_cluster.clear();
-    id_to_cluster.reserve((size_t)n);
-    std::vector<int64_t> cnt((size_t)kc, 0);
-    for (int64_t i = 0; i < n; ++i) cnt[A[i]]++;
-    for (int64_t c = 0; c < kc; ++c) reserve_slack(lists[c], cnt[c]);
-    for (int64_t i = 0; i < n; ++i) {
-        ListData &L = lists[A[i]];
-        L.exists = true;
-        L.ids.push_back(ids[i]);
-        L.vecs.insert(L.vecs.end(), &X[(size_t)i * dim], &X[(size_t)(i + 1) * dim]);
-        L.codes.insert(L.codes.end(), &codes[(size_t)i * m], &codes[(size_t)(i + 1) * m]);
-        L.has_codes.push_back(1);
-        id_to_cluster[ids[i]] = A[i];
+    // the new lists: each list's members in ascending id order, built by
+    // threads over ranges of lists while one more fills the id map (the
+    // vectors are read from their old places, freed after the swap)
+    std::vector<ListData> fresh((size_t)kc);
+    std::vector<int64_t> off((size_t)kc + 1, 0);
+    for (int64_t i = 0; i < n; ++i) off[(size_t)A[i] + 1]++;
+    for (int64_t c = 0; c < kc; ++c) off[c + 1] += off[c];
+    std::vector<int64_t> members((size_t)n);
+    {
+        std::vector<int64_t> pos(off.begin(), off.end() - 1);
+        for (int64_t i = 0; i < n; ++i) members[pos[A[i]]++] = i;
     }
+    {
+        const size_t nt = host_threads(n);
+        std::vector<std::thread> pool;
+        pool.emplace_back([&] {
+            id_to_cluster.clear();
+            id_to_cluster.reserve((size_t)n);
+            for (int64_t i = 0; i < n; ++i) id_to_cluster[ids[i]] = A[i];
+        });
+        // list ranges of about n / nt members each
+        std::vector<int64_t> cut{0};
+        for (size_t t = 1; t < nt; ++t) {
+            const int64_t want = (int64_t)t * n / (int64_t)nt;
+            cut.push_back(std::lower_bound(off.begin(), off.end(), want) - off.begin());
+        }
+        cut.push_back(kc);
+        for (size_t t = 0; t < nt; ++t)
+            pool.emplace_back([&, t] {
+                for (int64_t c = cut[t]; c < cut[t + 1]; ++c) {
+                    ListData &L = fresh[c];
+                    const int64_t len = off[c + 1] - off[c];
+                    reserve_slack(L, len);
+                    for (int64_t e = off[c]; e < off[c + 1]; ++e) {
+                        const int64_t i = members[e];
+                        L.exists = true;
+                        L.ids.push_back(ids[i]);
+                        const float *v = src(i);
+                        L.vecs.insert(L.vecs.end(), v, v + dim);
+                        L.codes.insert(L.codes.end(), &codes[(size_t)i * m], &codes[(size_t)(i + 1) * m]);
+                        L.has_codes.push_back(1);
+                    }
+                }
+            });
+        for (auto &th : pool) th.join();
+    }
+    lists.swap(fresh);
     pending.clear();
     trained = true;
     mark_dirty();

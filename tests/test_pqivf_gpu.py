@@ -130,6 +130,26 @@ def test_train_shuffled_ids_and_small_dims(hann, oracle):
     check_train_parity(idx, ref)
 
 
+def test_retrain_from_lists_bit_exact(hann, oracle):
+    """A second Train reads its vectors back from the trained lists, where
+    consecutive ids are scattered over thousands of places (they are packed
+    on the host by threads before the upload): bit-exact to the oracle's
+    Train of the same vectors at epoch 1. The first Train, over vectors added
+    in shuffled id order, takes the sort."""
+    rng = np.random.default_rng(23)
+    x = (rng.uniform(-8, 8, (40, 16))[rng.integers(0, 40, 20000)] + rng.standard_normal((20000, 16))).astype(np.float32)
+    ids = rng.permutation(60000)[:20000].astype(np.int64)
+    pq = P()
+    idx = pq.New(16, pq.WithCoarseK(32), pq.WithNumSubquantizers(4), pq.WithPQK(16), pq.WithKMeansIters(4),
+                 pq.WithSeed(9))
+    idx.bulk_add_arrays(ids, x)
+    idx.Train()
+    check_train_parity(idx, oracle.train(ids, x, 32, 4, 16, 4, seed=9, epoch=0, nthreads=8))
+    idx.Train()
+    check_train_parity(idx, oracle.train(ids, x, 32, 4, 16, 4, seed=9, epoch=1, nthreads=8))
+    assert idx.Stats().Count == 20000
+
+
 def test_train_wide_subspaces(hann, oracle):
     """d=128, m=2: 64-dim sub-spaces take the tiled assignment path."""
     rng = np.random.default_rng(22)
