@@ -421,6 +421,9 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
             const int rl = wn * 64 + j * 32 + lane % 32;
             const int64_t gr = r0 + rl;
             const bool rv = gr < nb;
+            // the lanes whose row exists (a wave-uniform mask, ANDed into each
+            // compare's, so no per-lane boolean is kept per element)
+            const unsigned long long rvm = __builtin_amdgcn_ballot_w64(rv);
             const float xr = s_xn[rl];
             f32x16 acc[2];
 #pragma unroll
@@ -461,8 +464,8 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
                             if (rv && q0 + ql < nq) dense[(q0 + ql) * ld_dense + gr] = approx;
                             continue;
                         }
-                        const bool pass = rv && approx <= tv[u];
-                        const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+                        // the compare goes straight into a lane mask
+                        const unsigned long long m = __builtin_amdgcn_ballot_w64(approx <= tv[u]) & rvm;
                         if (m == 0) continue;
                         // the rare path: the query index is computed here
                         // from an opaque copy of the lane's query base, so
@@ -472,9 +475,12 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
                         asm volatile("" : "+v"(qbo));
                         const int ql = qbo + i * 32 + 8 * g + u;
                         const unsigned mlo = (unsigned)m, mhi = (unsigned)(m >> 32);
-                        if (pass) {
-                            const int rank = lane < 32 ? (int)__builtin_amdgcn_mbcnt_lo(mlo, 0u)
-                                                       : (int)__builtin_amdgcn_mbcnt_hi(mhi, 0u);
+                        if (__builtin_amdgcn_inverse_ballot_w64(m)) {
+                            // rank among the passing lanes of the lane's half:
+                            // mbcnt over both words counts all of the low
+                            // half's for an upper lane, so subtract them
+                            const unsigned below = __builtin_amdgcn_mbcnt_hi(mhi, __builtin_amdgcn_mbcnt_lo(mlo, 0u));
+                            const int rank = (int)(lane < 32 ? below : below - (unsigned)__popc(mlo));
                             const int slot = (int)(lane < 32 ? cq[i][g][u] & 0xffffu : cq[i][g][u] >> 16) + rank;
                             if (slot < SL) {
                                 s_cr[(ql * 2 + wn) * SL + slot] = make_int2((int32_t)gr, __float_as_int(approx));
