@@ -612,6 +612,7 @@ void launch_gather_rows(const float *src, int d, const int64_t *idx, int64_t n, 
 // ---------------------------------------------------------- k-means++ --
 namespace dev {
 constexpr int PP_BLOCK = 256;
+constexpr int PP_BS = 4;   // block sums per pick thread held in LDS (32 KB)
 
 // D[i] = min(D[i], e(x_i, c)) in the reference FP32 order, A[i] = the index
 // of the center that gave D[i] (first: D[i] = e(x_i, c_0), A[i] = 0); per
@@ -707,6 +708,7 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
     const float *data = data_all + g * d;
     float *cent_out = cents_all + (g * k + j) * d;
     __shared__ double s_tot[1024];
+    __shared__ double s_bs[1024 * PP_BS];   // the block sums (when at most PP_BS a thread)
     __shared__ float s_d[PP_BLOCK];
     __shared__ double s_run;
     __shared__ int64_t s_pick, s_last, s_blk;
@@ -715,8 +717,13 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
     // per-thread chunk sums in block order, then an inclusive scan over threads
     const int64_t per = (nblocks + 1023) / 1024;
     const int64_t b0 = t * per, b1 = b0 + per < nblocks ? b0 + per : nblocks;
+    const bool in_lds = per <= PP_BS;
     double mine = 0.0;
-    for (int64_t b = b0; b < b1; ++b) mine += bsum[b];
+    for (int64_t b = b0; b < b1; ++b) {
+        const double v = bsum[b];
+        if (in_lds) s_bs[b] = v;
+        mine += v;
+    }
     s_tot[t] = mine;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
@@ -756,7 +763,10 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
                     // the next block whose sum carries the running sum past the target
                     double run = s_run;
                     int64_t b = s_blk;
-                    while (b < ob1 && run + bsum[b] <= target) run += bsum[b++];
+                    if (in_lds)
+                        while (b < ob1 && run + s_bs[b] <= target) run += s_bs[b++];
+                    else
+                        while (b < ob1 && run + bsum[b] <= target) run += bsum[b++];
                     s_run = run;
                     s_blk = b;
                 }
@@ -769,14 +779,25 @@ __global__ void __launch_bounds__(1024) k_pp_pick(const double *__restrict__ bsu
                 if (t == ot) {
                     double run = s_run;
                     int64_t pick = -1, last = s_last;
-                    const int64_t rows = n - r0 < PP_BLOCK ? n - r0 : PP_BLOCK;
-                    for (int r = 0; r < (int)rows; ++r) {
-                        const float dr = s_d[r];
-                        if (dr > 0.f) last = r0 + r;
-                        run += (double)dr;
-                        if (run > target && dr > 0.f) {
-                            pick = r0 + r;
-                            break;
+                    const int rows = (int)(n - r0 < PP_BLOCK ? n - r0 : PP_BLOCK);
+                    // in row order; eight rows' values read ahead of their
+                    // dependent sums (s_d holds PP_BLOCK values)
+                    bool done = false;
+                    for (int rb = 0; rb < rows && !done; rb += 8) {
+                        float dv[8];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) dv[q] = s_d[rb + q];
+#pragma unroll
+                        for (int q = 0; q < 8; ++q) {
+                            const int r = rb + q;
+                            if (done || r >= rows) continue;
+                            const float dr = dv[q];
+                            if (dr > 0.f) last = r0 + r;
+                            run += (double)dr;
+                            if (run > target && dr > 0.f) {
+                                pick = r0 + r;
+                                done = true;
+                            }
                         }
                     }
                     s_run = run;
