@@ -43,7 +43,7 @@ void assign_points(const float *data, int64_t n, const float *cents, int k, int 
 
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
                    std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st, int precision,
-                   int init_mode) {
+                   int init_mode, int32_t *assign_out) {
     const int64_t keff = n < k ? n : k;
     if (keff <= 0) return 0;
     if (init_mode == 1) {
@@ -74,7 +74,11 @@ int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, i
         HANN_HIP(hipStreamSynchronize(st));
     }
     sc.reserve(n, (int)keff, groups);
-    DevBuf<int32_t> assign((size_t)groups * n);
+    DevBuf<int32_t> own_assign(assign_out ? 0 : (size_t)groups * n);
+    struct {
+        int32_t *p;
+        int32_t *get() const { return p; }
+    } assign{assign_out ? assign_out : own_assign.get()};
     std::vector<int64_t> offs((size_t)groups * (keff + 1));
     std::vector<int64_t> reseed((size_t)groups * keff);
     const bool bf16 = precision == HANN_PREC_BF16 && groups == 1 && dsub >= 32 && row_stride == dsub;
@@ -533,14 +537,18 @@ void PQIVF::train() {
     }
     DevBuf<float> dC((size_t)coarse_k * dim);
     std::vector<uint64_t> st0{stream_seed(seed, epoch, 0)};
+    // k-means over every vector leaves its last assignment in dA, a bound
+    // for the final one (the certified BF16 assignment's hint)
+    DevBuf<int32_t> dA((size_t)n);
+    const bool hinted = ntrain == n && iters > 0;
     const int64_t new_kc =
         run_kmeans(train_src, ntrain, dim, d, (int)coarse_k, (int)iters, 1, st0, dC.get(), sc, stream, train_precision,
-                   kmeans_init);
+                   kmeans_init, hinted ? dA.get() : nullptr);
     dS.release();
 
     // final assignment of every vector; lists in ascending id order
-    DevBuf<int32_t> dA((size_t)n);
-    assign_points(dX.get(), n, dC.get(), (int)new_kc, d, train_precision, nullptr, dA.get(), stream);
+    assign_points(dX.get(), n, dC.get(), (int)new_kc, d, train_precision, hinted ? dA.get() : nullptr, dA.get(),
+                  stream);
     std::vector<int32_t> A((size_t)n);
     HANN_HIP(hipMemcpyAsync(A.data(), dA.get(), A.size() * sizeof(int32_t), hipMemcpyDeviceToHost, stream));
 

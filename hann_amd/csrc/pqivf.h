@@ -67,10 +67,11 @@ struct PendingStore {
 
 // runKMeans over `groups` problems on device data; returns min(k, n)
 // precision HANN_PREC_BF16 (single group, dsub >= 32): certified BF16
-// assignment, identical results
+// assignment, identical results. assign_out (or null): [groups][n], holds
+// the last Lloyd iteration's assignment (made before its update) on return
 int64_t run_kmeans(const float *data, int64_t n, int64_t row_stride, int dsub, int k, int iters, int groups,
                    std::vector<uint64_t> &states, float *cents, KMeansScratch &sc, hipStream_t st,
-                   int precision = HANN_PREC_FP32, int init_mode = 0);
+                   int precision = HANN_PREC_FP32, int init_mode = 0, int32_t *assign_out = nullptr);
 // assignPoints in either precision (BF16 needs a contiguous [n][d] layout)
 void assign_points(const float *data, int64_t n, const float *cents, int k, int d, int precision,
                    const int32_t *hint, int32_t *out, hipStream_t st);
