@@ -551,7 +551,8 @@ def main():
     paths = {"plain": (_lib.SCAN_EXACT, 0, _lib.PREC_FP32),
              "table-f32": (_lib.SCAN_ADC_F32, 1, _lib.PREC_BF16),
              "list-bf16": (_lib.SCAN_LIST_BF16, 1, _lib.PREC_BF16),
-             "list-bf16x3": (_lib.SCAN_LIST_BF16X3, 1, _lib.PREC_BF16)}
+             "list-bf16x3": (_lib.SCAN_LIST_BF16X3, 1, _lib.PREC_BF16),
+             "list-fp8": (_lib.SCAN_LIST_FP8, 1, _lib.PREC_BF16)}
     if a.paths:
         keep = set(a.paths.split(",")) | {"plain"}
         paths = {k: v for k, v in paths.items() if k in keep}
@@ -688,9 +689,12 @@ def main():
                               "the bound, a per-query select of k*R + fp32 rerank",
                  "list-bf16x3": "certified bf16 coarse + list-major decode + split-bf16 mfma: a sample pass over "
                                 "each query's nearest lists bounds its k*R-th key, a filter pass keeps the records "
-                                "under the bound, a per-query select of k*R + fp32 rerank"}
+                                "under the bound, a per-query select of k*R + fp32 rerank",
+                 "list-fp8": "certified bf16 coarse + list-major decode + fp8 (e4m3) mfma: a sample pass over "
+                             "each query's nearest lists bounds its k*R-th key, a filter pass keeps the records "
+                             "under the bound, a per-query select of k*R + fp32 rerank"}
     kernel_name = {"table-f32": "k_scan_block", "list-bf16": "k_scan_list<bf16,filter>",
-                   "list-bf16x3": "k_scan_list<bf16x3,filter>"}
+                   "list-bf16x3": "k_scan_list<bf16x3,filter>", "list-fp8": "k_scan_list<fp8,filter>"}
 
     # roofline of the dominant kernel: the chosen path's list scan (filter
     # pass; the sample pass beside it)

@@ -22,7 +22,7 @@ HANN_ERR_UNSUPPORTED = 9
 
 METRIC_SQEUCLIDEAN, METRIC_EUCLIDEAN, METRIC_MANHATTAN, METRIC_COSINE = 0, 1, 2, 3
 PREC_FP32, PREC_BF16 = 0, 1
-SCAN_EXACT, SCAN_ADC_F32, SCAN_ADC_F16, SCAN_LIST_BF16, SCAN_LIST_BF16X3 = 0, 1, 2, 3, 4
+SCAN_EXACT, SCAN_ADC_F32, SCAN_ADC_F16, SCAN_LIST_BF16, SCAN_LIST_BF16X3, SCAN_LIST_FP8 = 0, 1, 2, 3, 4, 5
 
 
 class HannError(Exception):

@@ -8,7 +8,14 @@
 
 namespace hann {
 
-enum { SCAN_MODE_EXACT = 0, SCAN_MODE_ADC_F32 = 1, SCAN_MODE_ADC_F16 = 2, SCAN_MODE_LIST_BF16 = 3, SCAN_MODE_LIST_BF16X3 = 4 };
+enum {
+    SCAN_MODE_EXACT = 0,
+    SCAN_MODE_ADC_F32 = 1,
+    SCAN_MODE_ADC_F16 = 2,
+    SCAN_MODE_LIST_BF16 = 3,
+    SCAN_MODE_LIST_BF16X3 = 4,
+    SCAN_MODE_LIST_FP8 = 5
+};
 
 // Arguments of k_scan (passed by value).
 struct ScanArgs {

@@ -976,6 +976,13 @@ void PQIVF::sync_device() {
     up(d_coarse, coarse);
     up(d_cb, codebooks);
     up(d_cb_len, cb_len);
+    // the FP8 list scan's codebook scale: the largest power of two that
+    // keeps every codeword value within 224 (half of e4m3's largest)
+    {
+        float mx = 0.f;
+        for (float v : codebooks) mx = std::max(mx, std::fabs(v));
+        cb_fp8_scale = mx > 0.f && std::isfinite(mx) ? std::exp2(std::floor(std::log2(224.f / mx))) : 1.f;
+    }
     (void)s;
     coarse_bf16.ready = false;
     d_term.resize((size_t)N + 1);
@@ -1118,10 +1125,12 @@ void PQIVF::search_device(const float *dq, int64_t nq, const int32_t *dk, const 
     a.fell_back = nullptr;
     if (profile) HANN_HIP(hipEventRecord(ev[1], st));
     int mode = p.scan_mode;
-    if (mode < SCAN_MODE_EXACT || mode > SCAN_MODE_LIST_BF16X3)
+    if (mode < SCAN_MODE_EXACT || mode > SCAN_MODE_LIST_FP8)
         fail(HANN_ERR_UNSUPPORTED, strf("scan mode %d is not supported", mode));
-    bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3;
-    const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : 0;
+    bool list_scan = mode == SCAN_MODE_LIST_BF16 || mode == SCAN_MODE_LIST_BF16X3 || mode == SCAN_MODE_LIST_FP8;
+    // the list-major scan's operands: 0 BF16, 1 split BF16, 3 FP8 (2 is the
+    // exact scan)
+    const int x3 = mode == SCAN_MODE_LIST_BF16X3 ? 1 : (mode == SCAN_MODE_LIST_FP8 ? 3 : 0);
     PQIVF::ListPlan lplan{};
     refresh_shape();
     if (list_scan) {
@@ -1495,6 +1504,7 @@ void PQIVF::run_list_scan(const ScanArgs &a, int64_t nq, int x3, int maxp, const
     la.capq = (uint32_t)L.capq;
     la.coarse = d_coarse.get();
     la.chunk = 1 << sh.ch;
+    la.cb_scale = cb_fp8_scale;
     // 2. the sample pass: every key of each query's nearest probed lists;
     //    then each query's bound theta, the (k R)-th smallest of them
     {
@@ -1574,7 +1584,8 @@ int64_t PQIVF::search_scratch_bytes(int64_t nq, int kmax, int pmax_probe, const 
     add(nq * 4);
     add(4);
     if (nq * m * pmax * 4 <= (int64_t(512) << 20)) add(nq * m * pmax * 4);
-    const bool lm = p.scan_mode == SCAN_MODE_LIST_BF16 || p.scan_mode == SCAN_MODE_LIST_BF16X3;
+    const bool lm = p.scan_mode == SCAN_MODE_LIST_BF16 || p.scan_mode == SCAN_MODE_LIST_BF16X3 ||
+                    p.scan_mode == SCAN_MODE_LIST_FP8;
     if (lm || (p.scan_mode == SCAN_MODE_EXACT && R > 0)) {
         const ListPlan L = list_plan(nq, np, selk, !lm);
         add((int64_t)L.arena);

@@ -147,6 +147,7 @@ struct PQIVF {
     std::vector<int64_t> touched_lists;
     int64_t incremental_syncs = 0, full_syncs = 0;
     bool any_uncoded = false;
+    float cb_fp8_scale = 1.f;   // FP8 list scan: the codebook scale (a power of two)
     int code_bytes = 1;
     int64_t n_entries = 0;
     DevBuf<int64_t> d_list_start, d_list_len, d_ids, d_cb_len, all_lists;

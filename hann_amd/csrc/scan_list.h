@@ -38,6 +38,7 @@ struct ListScanArgs {
     uint32_t capq;               // record room per query
     const float *coarse;         // [K][d] coarse centroids (the exact scan)
     int chunk;                   // entries per work item (a multiple of 64; at most 2,048)
+    float cb_scale;              // FP8 mode: the codebook's scale (a power of two)
 };
 
 // work items: up to 2^qg queries of a list times a chunk of up to 2^ch of
@@ -99,7 +100,8 @@ int list_item_order_ints();
 // uint8 codes, and a BF16 codebook that fits in LDS
 bool list_scan_supported(int d, int m, int P, int code_bytes);
 // kind 1: the sample pass's keys (every key of the sample pairs, in the
-// rows); 2: the filter pass's records
+// rows); 2: the filter pass's records. x3: the operands, 0 BF16, 1 split
+// BF16, 3 FP8 (e4m3; the codebook scaled by la.cb_scale)
 void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, hipStream_t st);
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind);
 // The list-major exact scan: the reference's decode-and-measure distance

@@ -504,8 +504,20 @@ typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 // bytes), for lanes whose store is to be discarded
 constexpr uint32_t kNoStore = 0xFFFFFFF0u;
 
-template <bool X3, int M, int KIND>
+// F8: FP8 (e4m3) operands and v_mfma_f32_32x32x16_fp8_fp8: the codebook
+// scaled by la.cb_scale (a power of two), the query by a power of two of its
+// own that keeps its B operand within FP8's range (the key scale of the
+// query then includes it)
+__device__ __forceinline__ float fp8_clamp(float v) { return fminf(fmaxf(v, -448.f), 448.f); }
+__device__ __forceinline__ uint32_t pack4_fp8(float a, float b, float c, float d) {
+    int p = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(a), fp8_clamp(b), 0, false);
+    p = __builtin_amdgcn_cvt_pk_fp8_f32(fp8_clamp(c), fp8_clamp(d), p, true);
+    return (uint32_t)p;
+}
+
+template <bool X3, int M, int KIND, bool F8>
 __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
+    static_assert(!(X3 && F8), "one operand mode");
     extern __shared__ __align__(16) unsigned char s_raw[];
     constexpr int D = 4 * M;
     constexpr int HW = M / 8;   // 32-bit words of half a code row
@@ -513,9 +525,11 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     const int P = la.P;
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
     bf16x4 *sCBl = sCBh + (size_t)M * P;                          // [M][P] (split mode)
-    float *s_term = reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) + wave * 32;
+    uint32_t *sCB8 = reinterpret_cast<uint32_t *>(s_raw);         // [M][P] (FP8 mode)
+    const size_t cb_bytes = F8 ? (size_t)M * P * 4 : (X3 ? 2 : 1) * (size_t)M * P * 8;
+    float *s_term = reinterpret_cast<float *>(s_raw + cb_bytes) + wave * 32;
     // filter pass: the wave's ring of pending records
-    uint64_t *s_ring = reinterpret_cast<uint64_t *>(reinterpret_cast<float *>(sCBh + (X3 ? 2 : 1) * (size_t)M * P) +
+    uint64_t *s_ring = reinterpret_cast<uint64_t *>(reinterpret_cast<float *>(s_raw + cb_bytes) +
                                                     LNW * 32) + (size_t)wave * RING;
     uint32_t *s_ringq = reinterpret_cast<uint32_t *>(s_ring + (size_t)(LNW - wave) * RING) + (size_t)wave * RING;
     const uint8_t *codes = reinterpret_cast<const uint8_t *>(la.codes);
@@ -523,6 +537,11 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     // the codebook, once per workgroup
     for (int t = tid; t < M * P; t += 64 * LNW) {
         const float4 w = *reinterpret_cast<const float4 *>(la.cb + (size_t)t * 4);
+        if (F8) {
+            const float a = la.cb_scale;
+            sCB8[t] = pack4_fp8(w.x * a, w.y * a, w.z * a, w.w * a);
+            continue;
+        }
         bf16x4 h;
         h[0] = (__bf16)w.x;
         h[1] = (__bf16)w.y;
@@ -649,7 +668,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         const int eb = S.eb, ee = S.ee, slot_end = S.slot_end;
         const bool qv = col < S.pc;
         const int qid = S.qid;
-        const float sc = S.sc, qcs = S.qc * S.sc;
+        float sc = S.sc;
         const int off = S.off;
         // filter pass: values with key <= theta survive, i.e. those whose bits
         // are <= tb as a signed integer
@@ -661,7 +680,32 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // exactly); the accumulator starts at (t + |q - c|^2) sc, so it ends
         // at the scaled key ((qc + t) - 2 <q, y>) sc
         bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
-        {
+        long bq8[F8 ? M / 4 : 1];
+        if (F8) {
+            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
+            float mx = 0.f;
+#pragma unroll
+            for (int jj = 0; jj < M / 2; ++jj) {
+                const float4 v = qr[jj];
+                mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
+            }
+            // the query's largest |value| over both halves; its scale: the
+            // largest power of two k <= 1 with 2 sc k max|q| / cb_scale <= 224
+            mx = fmaxf(mx, __shfl_xor(mx, 32));
+            const float lim = 224.f * la.cb_scale / (2.f * sc * mx);
+            const float kap = mx > 0.f ? fminf(1.f, __builtin_exp2f(__builtin_floorf(__builtin_log2f(lim)))) : 1.f;
+            sc *= kap;
+            const uint32_t keep = qv ? 0xFFFFFFFFu : 0u;
+            const float m2 = -2.f * sc / la.cb_scale;
+            auto z = [&](float v) { return __uint_as_float(__float_as_uint(v * m2) & keep); };
+#pragma unroll
+            for (int jj = 0; jj < M / 4; ++jj) {
+                const float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
+                const uint32_t lo = pack4_fp8(z(x0.x), z(x0.y), z(x0.z), z(x0.w));
+                const uint32_t hi = pack4_fp8(z(x1.x), z(x1.y), z(x1.z), z(x1.w));
+                bq8[jj] = (long)((uint64_t)lo | ((uint64_t)hi << 32));
+            }
+        } else {
             const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
             // lanes without a query take zeros through a bit mask (a branch
             // that overwrote the loaded registers would first wait for every
@@ -680,6 +724,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 }
             }
         }
+        const float qcs = S.qc * sc;
         const int next_item = part + __builtin_amdgcn_readfirstlane(j) * nparts;
         SN = load_item(next_item);
         // b1 holds stream slab 1: this item's slab 1, or, for a one-slab
@@ -734,6 +779,11 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                 const int i0 = half * (M / 2) + 2 * jj;
                 const uint32_t wv = x.w[jj >> 1] >> (16 * (jj & 1));
                 const uint32_t c0 = wv & 0xFFu, c1 = (wv >> 8) & 0xFFu;
+                if (F8) {
+                    const uint64_t a8 = (uint64_t)sCB8[i0 * P + c0] | ((uint64_t)sCB8[(i0 + 1) * P + c1] << 32);
+                    acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8((long)a8, bq8[jj], acc, 0, 0, 0);
+                    continue;
+                }
                 const bf16x4 x0 = sCBh[i0 * P + c0], x1 = sCBh[(i0 + 1) * P + c1];
                 const bf16x8 a = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
                 acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[jj], acc, 0, 0, 0);
@@ -1539,10 +1589,11 @@ void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, h
     if (shm > 160 * 1024) fail(HANN_ERR_UNSUPPORTED, strf("list-major scan needs %zu bytes of LDS", shm));
 #define HANN_LIST(X, MM, KK)                                                                                 \
     do {                                                                                                     \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_list<X, MM, KK>,                              \
+        constexpr bool F8_ = (X) == 2, X3_ = (X) == 1;                                                       \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_scan_list<X3_, MM, KK, F8_>,                       \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                 \
-        hipLaunchKernelGGL((dev::k_scan_list<X, MM, KK>), dim3((unsigned)grid), dim3(64 * dev::LNW), shm, st, \
-                           la);                                                                              \
+        hipLaunchKernelGGL((dev::k_scan_list<X3_, MM, KK, F8_>), dim3((unsigned)grid), dim3(64 * dev::LNW),  \
+                           shm, st, la);                                                                     \
         HANN_LAUNCH_CHECK();                                                                                 \
     } while (0)
 #define HANN_LIST_K(X, MM)                                                             \
@@ -1563,18 +1614,22 @@ void launch_scan_list_kind(const ListScanArgs &la, int x3, int kind, int grid, h
         }                                                                         \
     } while (0)
     if (la.d != 4 * la.m) fail(HANN_ERR_UNSUPPORTED, "list-major scan needs 4-dimensional subvectors");
-    if (x3)
-        HANN_LIST_M(true);
+    // x3: 0 BF16, 1 split BF16, 3 FP8
+    if (x3 == 1)
+        HANN_LIST_M(1);
+    else if (x3 == 3)
+        HANN_LIST_M(2);
     else
-        HANN_LIST_M(false);
+        HANN_LIST_M(0);
 #undef HANN_LIST_M
 #undef HANN_LIST_K
 #undef HANN_LIST
 }
 
 size_t list_scan_lds_bytes_kind(int x3, int m, int P, int kind) {
-    const int nb = x3 ? 2 : 1;
-    size_t b = (size_t)nb * m * P * 8 + (size_t)dev::LNW * 32 * 4;
+    // the codebook: BF16 (8 bytes a codeword), split BF16 (16), FP8 (4)
+    const size_t per = x3 == 1 ? 16 : (x3 == 3 ? 4 : 8);
+    size_t b = per * m * P + (size_t)dev::LNW * 32 * 4;
     if (kind == dev::KIND_FILTER) b += (size_t)dev::LNW * dev::RING * 12;
     return b;
 }

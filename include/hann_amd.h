@@ -209,8 +209,10 @@ typedef enum {
                                   (BF16 operands, FP32 accumulation); needs rerank > 0,
                                   otherwise (and for indexes with entries lacking codes)
                                   the persistent FP32 table scan runs */
-    HANN_SCAN_LIST_BF16X3 = 4  /* the same with split BF16 operands (hi + lo, three
+    HANN_SCAN_LIST_BF16X3 = 4, /* the same with split BF16 operands (hi + lo, three
                                   products): inner products to about FP32 accuracy */
+    HANN_SCAN_LIST_FP8 = 5     /* the same with FP8 (e4m3) operands: the codebook and
+                                  each query scaled by powers of two into FP8's range */
 } hann_scan_mode;
 
 typedef struct {

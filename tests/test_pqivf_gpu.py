@@ -758,14 +758,15 @@ def test_list_scan_matches_table_scan(hann, oracle, mode, rerank):
     assert np.mean(both) >= 0.95
 
 
-@pytest.mark.parametrize("mode", [3, 4])
+@pytest.mark.parametrize("mode", [3, 4, 5])
 def test_list_scan_bench_shape(hann, oracle, mode):
     """The list-major scan at the benchmark's shape (d = 96, m = 24, PQ 256,
     L2-normalised mixture data, k in {10, 50, 100}, R = 3): the kernel
     instantiation the headline figure runs. Returned distances are the exact
     reference-order ones, counts equal the table scan's, and the results
     agree with the FP32 table scan + rerank (>= 97 % of queries identical
-    for split BF16, >= 95 % overlap for BF16)."""
+    for split BF16, >= 95 % overlap for BF16, >= 97 % for FP8, whose
+    operands carry 3 mantissa bits)."""
     from hann_amd import _lib, synthetic
     pq = P()
     x, centers = synthetic.normalized_mixture(30000, 96, 512, 0.06, seed=11)
@@ -792,9 +793,10 @@ def test_list_scan_bench_shape(hann, oracle, mode):
         assert np.all(np.diff(b_d[qi, :kq]) >= 0)
         same += int(np.array_equal(a_ids[qi, :kq], b_ids[qi, :kq]))
     both = [len(set(a_ids[q, :ks[q]].tolist()) & set(b_ids[q, :ks[q]].tolist())) / ks[q] for q in range(len(qs))]
+    print(f"mode {mode}: {same} of {len(qs)} queries identical, mean overlap {np.mean(both):.4f}")
     if mode == 4:
         assert same >= 0.97 * len(qs), same
-    assert np.mean(both) >= 0.95, np.mean(both)
+    assert np.mean(both) >= (0.97 if mode == 5 else 0.95), np.mean(both)
 
 
 def test_list_scan_small_lists(hann):
