@@ -1365,9 +1365,10 @@ PQIVF::ListPlan PQIVF::list_plan(int64_t nq, int np, int selk, bool exact) const
     L.exact = exact;
     L.pairs_max = nq * (int64_t)np;
     L.items_max = ceil_div(L.pairs_max, group) * ceil_div(max_len, chunk) + chunks_all + 1;
-    // a query's sample: at least two probes, then on while fewer than
-    // 4 selk entries come before: at most max(4 selk + max_len, 2 max_len)
-    // entries, each probe's run rounded up to 8 slots
+    // a query's sample: at least one probe, then on while fewer than
+    // 4 selk entries come before: at most 4 selk + max_len entries (the
+    // bound below also covers a two-probe minimum), each probe's run
+    // rounded up to 8 slots
     L.sample_slots = nq * (std::max<int64_t>(4 * (int64_t)selk + max_len, 2 * max_len) + 7 * (int64_t)np);
     // record room: the filter pass keeps about 1.5 selk records a query
     // (at most every probed entry); a query with more takes the table scan

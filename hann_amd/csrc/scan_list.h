@@ -68,7 +68,7 @@ constexpr int kListScanParts = 256;
 void launch_fill_i32(int32_t *p, int64_t n, int32_t v, hipStream_t st);
 // per-list pair counts (lcnt) and sample pair counts (lscnt), per-query
 // entry totals, and the sample's probes and row slots (its nearest probed
-// lists: at least two, and until they hold 4 k R entries; each probed
+// lists: at least one, and until they hold 4 k R entries; each probed
 // list's run is its length rounded up to 8); lcnt, lscnt zeroed by the caller
 void launch_pair_count(const ScanArgs &a, int64_t nq, int32_t *lcnt, int32_t *lscnt, int64_t *qtot,
                        int32_t *qsamp_np, int64_t *qsamp_slots, hipStream_t st);
