@@ -170,12 +170,9 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         if (refine)
             launch_sample_thresholds(cand_ap, qn_, cap, cnt + q0, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
                                      reinterpret_cast<int *>(stat + 1), dim, thr2 + q0, st);
-        // (after a hint row, k = 1: the rank kernel takes the least
-        // candidate's threshold itself)
-        const bool refine_min = cand_ap && !refine && k == 1;
         launch_rank_candidates(queries + q0 * dim, qn_, dim, base, rank_every ? nb : 0, cnt + q0, cand,
-                               refine || refine_min ? cand_ap : nullptr, refine ? thr2 + q0 : nullptr,
-                               refine_min ? qf + q0 : nullptr, stat, cap, k, k_q ? k_q + q0 : nullptr, sqrt_out,
+                               refine ? cand_ap : nullptr, refine ? thr2 + q0 : nullptr, cap, k,
+                               k_q ? k_q + q0 : nullptr, sqrt_out,
                                out_ids + q0 * k, out_dist + q0 * k, st);
     }
     if (rank_every) {

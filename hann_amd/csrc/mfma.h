@@ -42,13 +42,11 @@ void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, const
 // overflowed (cnt > cap) ranks all nb rows instead when nb > 0, and is left
 // untouched when nb == 0
 // cand_ap and thr2 (or null): only candidates with cand_ap <= thr2[q] are
-// ranked; cand_ap with qnorm_min and stat (the max norm bits and the
-// inexact flag) instead of thr2, for k = 1: the threshold of the least
-// cand_ap of each query
+// ranked
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
                             const int32_t *cnt, const int32_t *cand_row, const float *cand_ap, const float *thr2,
-                            const float *qnorm_min, const unsigned *stat, int cap, int k, const int32_t *k_q,
-                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st);
+                            int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids, double *out_dist,
+                            hipStream_t st);
 // the same over int64 candidate rows (rows[q * cap + i], i < cnt[q])
 void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *base, int64_t nb, const int32_t *cnt,
                         const int64_t *rows, int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids,

@@ -565,23 +565,6 @@ __device__ __forceinline__ float key_float(unsigned k) {
 // in k_thresholds. One workgroup of 256 threads per query.
 constexpr int ST_BINS = 2048;
 
-// The filter threshold from A, an upper bound of a query's k-th smallest
-// approximate distance over some rows: A bounds the k-th exact rank tau
-// through the error bounds, and every row whose exact rank is at most tau
-// has an approximate distance within the result (see k_thresholds).
-__device__ inline float thr_from_approx(float A, float qnorm, float max_norm, int inexact, int d) {
-    if (!(A < __builtin_inff())) return __builtin_inff();
-    const double u = 1.0 / 16777216.0;
-    const double qa = qnorm, qb = max_norm;
-    const double delta = inexact ? (qa + qb) / 512.0 : 0.0;
-    const double e2 = 2.0 * (d + 2) * u * (qa + qb) * (qa + qb) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
-    const double s0 = __builtin_sqrt((A > 0.f ? (double)A : 0.0) + e2) + delta;
-    const double tau = s0 * s0 / (1.0 - (d + 2) * u) * (1.0 + 1.0 / 1048576.0);
-    const double tp = tau / (1.0 - (d + 2) * u);
-    const double s1 = __builtin_sqrt(tp) + delta;
-    return (float)((s1 * s1 + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
-}
-
 // cnt non-null: query q's values are its first cnt[q] of ns (its
 // candidates); fewer than k of them, or more than ns, give no bound (+inf)
 __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restrict__ approx, int64_t stride,
@@ -659,7 +642,15 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
         thr[q] = __builtin_inff();
         return;
     }
-    thr[q] = thr_from_approx(A, qnorm_f[q], __uint_as_float(*max_norm), *inexact, d);
+    const double u = 1.0 / 16777216.0;
+    const double qa = qnorm_f[q], qb = (double)__uint_as_float(*max_norm);
+    const double delta = *inexact ? (qa + qb) / 512.0 : 0.0;
+    const double e2 = 2.0 * (d + 2) * u * (qa + qb) * (qa + qb) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
+    const double s0 = __builtin_sqrt((A > 0.f ? (double)A : 0.0) + e2) + delta;
+    const double tau = s0 * s0 / (1.0 - (d + 2) * u) * (1.0 + 1.0 / 1048576.0);
+    const double tp = tau / (1.0 - (d + 2) * u);
+    const double s1 = __builtin_sqrt(tp) + delta;
+    thr[q] = (float)((s1 * s1 + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
 }
 
 // Exact reference-order rank of every candidate, then the k best by
@@ -673,9 +664,7 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
                                                         const int32_t *__restrict__ cnt,
                                                         const CT *__restrict__ cand_row,
                                                         const float *__restrict__ cand_ap,
-                                                        const float *__restrict__ thr2,
-                                                        const float *__restrict__ qnorm_min,
-                                                        const unsigned *__restrict__ stat, int cap, int64_t nb,
+                                                        const float *__restrict__ thr2, int cap, int64_t nb,
                                                         int k, const int32_t *__restrict__ k_q, int ccap, int sqrt_out,
                                                         int64_t *__restrict__ out_ids, double *__restrict__ out_dist) {
     extern __shared__ __align__(16) unsigned char s_raw[];
@@ -699,24 +688,14 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
         if (valid) key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
         sel.push(key, row, valid);
     };
-    if (every || !cand_ap || (!thr2 && !qnorm_min)) {
+    if (every || !cand_ap) {
         for (int64_t b = 0; b < n; b += kWave) {
             const int64_t t = b + lane;
             const bool valid = t < n;
             rank(valid ? (every ? t : (int64_t)cand_row[q * cap + t]) : 0, valid);
         }
     } else {
-        float t2;
-        if (thr2) {
-            t2 = thr2[q];
-        } else {
-            // k = 1 after a hint row: the threshold of the candidates' least
-            // approximate distance (the same bound as the sample's)
-            float amin = __builtin_inff();
-            for (int64_t t = lane; t < n; t += kWave) amin = fminf(amin, cand_ap[q * cap + t]);
-            for (int o = 32; o > 0; o >>= 1) amin = fminf(amin, __shfl_xor(amin, o));
-            t2 = thr_from_approx(amin, qnorm_min[q], __uint_as_float(stat[0]), (int)stat[1], d);
-        }
+        const float t2 = thr2[q];
         int nbuf = 0;   // wave-uniform
         for (int64_t b = 0; b < n; b += kWave) {
             const int64_t t = b + lane;
@@ -888,8 +867,8 @@ static size_t rank_lds(int ccap) { return (size_t)ccap * 12 + (size_t)2 * dev::k
 
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
                             const int32_t *cnt, const int32_t *cand_row, const float *cand_ap, const float *thr2,
-                            const float *qnorm_min, const unsigned *stat, int cap, int k, const int32_t *k_q,
-                            int sqrt_out, int64_t *out_ids, double *out_dist, hipStream_t st) {
+                            int cap, int k, const int32_t *k_q, int sqrt_out, int64_t *out_ids, double *out_dist,
+                            hipStream_t st) {
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = rank_lds(ccap);
@@ -899,8 +878,8 @@ void launch_rank_candidates(const float *queries, int64_t nq, int d, const float
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates<int32_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d,
                            base, cnt + q0, cand_row + q0 * (int64_t)cap, cand_ap ? cand_ap + q0 * (int64_t)cap : nullptr,
-                           thr2 ? thr2 + q0 : nullptr, qnorm_min ? qnorm_min + q0 : nullptr, stat, cap, nb, k,
-                           k_q ? k_q + q0 : nullptr, ccap, sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
+                           thr2 ? thr2 + q0 : nullptr, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap, sqrt_out,
+                           out_ids + q0 * k, out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
     }
 }
@@ -916,7 +895,7 @@ void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *ba
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_rank_candidates<int64_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d,
-                           d, base, cnt + q0, rows + q0 * (int64_t)cap, nullptr, nullptr, nullptr, nullptr, cap, nb, k,
+                           d, base, cnt + q0, rows + q0 * (int64_t)cap, nullptr, nullptr, cap, nb, k,
                            k_q ? k_q + q0 : nullptr, ccap,
                            sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
         HANN_LAUNCH_CHECK();
