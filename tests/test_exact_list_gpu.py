@@ -70,6 +70,27 @@ def test_exact_list_scan_equals_oracle(hann, oracle, d, m):
         _same(got, oracle.search_rerank_many(ix, qs, ks, nps, R, nthreads=8))
 
 
+@pytest.mark.parametrize("d,m", [(40, 8), (45, 9), (130, 10), (20, 4), (200, 8)])
+def test_plain_path_rerank_any_width_equals_oracle(hann, oracle, d, m):
+    """Widths the list-major exact scan does not take (the per-query exact
+    scan ranks the candidates): the rerank's two-rows-a-step kernel covers
+    32 <= d < 160 -- d = 40 exercises its 8-wide cleanup slots, 45 the
+    cleanup and a 5-element scalar tail, 130 four blocks and a tail -- and
+    the one-row-per-lane kernel the others (20, 200). Ids, distances and
+    counts identical to or_search_rerank."""
+    from hann_amd import _lib
+    x, c = _mixture(6000, d, 24, seed=d)
+    idx = _index(x, 32, m, pq_k=64, iters=4)
+    ix = _csr(oracle, idx)
+    rng = np.random.default_rng(d)
+    qs = (c[rng.integers(0, 24, 64)] + rng.standard_normal((64, d))).astype(np.float32)
+    ks = rng.choice(np.array([1, 10, 50], np.int32), 64)
+    nps = rng.choice(np.array([1, 2, 8], np.int32), 64)
+    for R in (3, 1):
+        got = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_EXACT, rerank=R)
+        _same(got, oracle.search_rerank_many(ix, qs, ks, nps, R, nthreads=8))
+
+
 def test_exact_list_scan_bf16_coarse_identical(hann, oracle):
     """The certified BF16 coarse step under the exact list scan: identical to
     the FP32 coarse step and to the oracle."""
