@@ -541,6 +541,8 @@ def main():
         worst = 0.0
         for i in range(0, ho_q.shape[0], 7):
             c = int(cnt[i])
+            if c == 0:
+                continue
             d64 = np.sqrt(((x[ids[i, :c]].astype(np.float64) - ho_q[i].astype(np.float64)) ** 2).sum(1))
             worst = max(worst, float(np.max(np.abs(dist[i, :c] - d64) / np.maximum(d64, 1e-30))))
         return rec, se, worst
