@@ -30,6 +30,20 @@ static int64_t sample_stride(int64_t nb, int k) {
     return std::max<int64_t>(1, nb / std::max<int64_t>(1, want));
 }
 
+// The dense form of the certified pass, for bases of at most kRankEveryMax
+// rows (a multiple of 8) and widths up to 128 (a padded width the register-resident MFMA
+// kernel takes): every (query, row) approximate distance is written as a
+// 16-bit key, and each query's threshold comes from the k-th smallest of all
+// its keys, not of a sample, so no second threshold and no candidate-slot
+// sizing is needed; queries run in chunks of at most 256 MB of keys.
+static bool dense_path(int64_t nb, int dp, bool hinted) {
+    return !hinted && nb <= kRankEveryMax && nb <= dense_select_max_rows() && nb % 8 == 0 && dp <= 128;
+}
+static int64_t dense_qchunk(int64_t nq, int64_t nb) {
+    return std::max<int64_t>(1, std::min<int64_t>(nq, (int64_t)(256ll << 20) / (2 * nb)));
+}
+static int dense_cap(int64_t nb, int k) { return (int)std::min<int64_t>(nb, std::max<int64_t>(256, 8ll * k)); }
+
 __global__ void k_copy_u32(unsigned *__restrict__ dst, const unsigned *__restrict__ src, int n) {
     const int i = threadIdx.x;
     if (i < n) dst[i] = src[i];
@@ -65,6 +79,13 @@ size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool
     if (mfma_filter_writes_approx((int)dp)) {
         add(chunk * cap * 4);   // the candidates' approximate distances
         add(nq * 4);            // and the second thresholds
+    }
+    if (dense_path(nb, (int)dp, false)) {
+        // the dense form: its keys in slot 10 and its candidates in slot 13
+        // can be larger than the sampled form's buffers there
+        const int64_t qc = dense_qchunk(nq, nb);
+        b += (size_t)std::max<int64_t>(0, qc * nb * 2 - std::max<int64_t>(qchunk * ns * 4, nq * 8));
+        b += (size_t)std::max<int64_t>(0, qc * (int64_t)dense_cap(nb, k) * 4 - chunk * cap * 4);
     }
     return b;
 }
@@ -116,6 +137,26 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
     float *qn = w.get<float>(5, (size_t)nq);
     float *qf = w.get<float>(6, (size_t)nq);
     launch_to_bf16(queries, nq, dim, dp, qb, qn, qf, stat + 2, reinterpret_cast<int *>(stat + 1), st);
+    if (dense_path(nb, dp, hint && k == 1)) {
+        const int64_t qchunk = dense_qchunk(nq, nb);
+        const int cap = dense_cap(nb, k);
+        uint16_t *keys = w.get<uint16_t>(10, (size_t)(qchunk * nb));
+        int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
+        int32_t *cand = w.get<int32_t>(13, (size_t)qchunk * cap);
+        for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
+            const int64_t qn_ = std::min(qchunk, nq - q0);
+            launch_mfma_keys(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, keys, st);
+            launch_dense_select(keys, qn_, nb, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
+                                reinterpret_cast<int *>(stat + 1), dim, cap, cnt + q0, cand, st);
+            // a query with more than cap candidates is ranked over every row
+            launch_rank_candidates(queries + q0 * dim, qn_, dim, base, nb, cnt + q0, cand, nullptr, nullptr, cap, k,
+                                   k_q ? k_q + q0 : nullptr, sqrt_out, out_ids + q0 * k, out_dist + q0 * k, st);
+        }
+        // the shared buffers are free again only once the stream has run
+        // this far
+        if (!ws) HANN_HIP(hipStreamSynchronize(st));
+        return;
+    }
     // filter thresholds: from the hint row's exact rank (k == 1), or from
     // the k-th smallest approximate distance over a row sample (every
     // stride-th row), which bounds the query's k-th exact rank

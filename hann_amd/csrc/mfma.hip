@@ -320,6 +320,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
             if ((s_cc[ql * 2] > SL || s_cc[ql * 2 + 1] > SL) && q0 + ql < nq) atomicMax(&cnt[q0 + ql], cap + 1);
 }
 
+// The 16-bit key of an approximate distance: the top 16 bits of its FP32
+// value, a non-decreasing function of it (negative values key 0, NaN the key
+// of +inf). approx_key_upper(K) is the largest float whose key is K.
+__device__ __forceinline__ uint16_t approx_key16(float a) {
+    if (a != a) return 0x7F80u;
+    return a > 0.f ? (uint16_t)(__float_as_uint(a) >> 16) : (uint16_t)0u;
+}
+__device__ __forceinline__ float approx_key_upper(unsigned key) {
+    return key >= 0x7F80u ? __builtin_inff() : __uint_as_float((key << 16) | 0xFFFFu);
+}
+
 // The same pass for dp = 16 KS <= 128, with each wave's query fragments in
 // registers and the next row tile's rows fetched into registers while the
 // current tile is multiplied and tested (LDS holds one row tile, so two
@@ -329,7 +340,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
 // threshold follows; a (query, row half) has MF_SLOTS_R / 2 LDS slots.
 constexpr int MF_SLOTS_R = 16;
 constexpr int MF_OVF_R = 512;
-template <bool DENSE, int KS>
+template <bool DENSE, int KS, bool KEY16 = false>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict__ X, int64_t nb,
               const float *__restrict__ qn, const float *__restrict__ xn, const float *__restrict__ thr, int rt_per_wg,
@@ -434,7 +445,43 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
             for (int ks = 0; ks < KS; ++ks) {
                 const bf16x8 b = *reinterpret_cast<const bf16x8 *>(&sB[rl * LD + ks * 16 + kl]);
 #pragma unroll
-                for (int i = 0; i < 2; ++i) acc[i] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][ks], b, acc[i], 0, 0, 0);
+                for (int i = 0; i < 2; ++i)
+                    acc[i] = KEY16 ? __builtin_amdgcn_mfma_f32_32x32x16_bf16(b, a[i][ks], acc[i], 0, 0, 0)
+                                   : __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[i][ks], b, acc[i], 0, 0, 0);
+            }
+            if (KEY16) {
+                // operands swapped: register 4 g + u of lane l holds row
+                // 8 g + 4 h + u of this 32-row half for query l % 32 of query
+                // tile i, so a lane's four keys of a group are consecutive
+                // rows of one query: one 8-byte store (ld_dense a multiple
+                // of 4)
+#pragma unroll
+                for (int i = 0; i < 2; ++i) {
+                    const int ql = wm * 64 + i * 32 + lane % 32;
+                    const float qv = s_qn[ql];
+                    const int64_t gq = q0 + ql;
+#pragma unroll
+                    for (int g = 0; g < 4; ++g) {
+                        const int rl0 = wn * 64 + j * 32 + 8 * g + 4 * (lane / 32);
+                        const float4 x4 = *reinterpret_cast<const float4 *>(&s_xn[rl0]);
+                        const float xs[4] = {x4.x, x4.y, x4.z, x4.w};
+                        unsigned kk[4];
+#pragma unroll
+                        for (int u = 0; u < 4; ++u) kk[u] = approx_key16((qv + xs[u]) - 2.f * acc[i][4 * g + u]);
+                        const int64_t gr0 = r0 + rl0;
+                        if (gq < nq) {
+                            uint16_t *dst = reinterpret_cast<uint16_t *>(dense) + gq * ld_dense + gr0;
+                            if (gr0 + 3 < nb) {
+                                *reinterpret_cast<uint2 *>(dst) = make_uint2(kk[0] | (kk[1] << 16), kk[2] | (kk[3] << 16));
+                            } else {
+#pragma unroll
+                                for (int u = 0; u < 4; ++u)
+                                    if (gr0 + u < nb) dst[u] = (uint16_t)kk[u];
+                            }
+                        }
+                    }
+                }
+                continue;
             }
             // the epilogue of k_mfma_pass: approximate distance, then the
             // threshold test (ballot, slot by rank in the lane half) or the
@@ -555,6 +602,25 @@ __device__ __forceinline__ float key_float(unsigned k) {
     return __uint_as_float((k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k);
 }
 
+// The filter threshold from A >= the k-th smallest approximate distance of a
+// query (qa = its FP32 norm, qb = the base's largest, inexact: some input is
+// not a BF16 value): each of the k rows up to the k-th has
+//   D <= (sqrt(max(A, 0) + E2) + delta)^2  and exact rank e <= D / (1-(d+2)u)
+// (the bounds of k_thresholds, read the other way), which bounds the query's
+// true k-th exact rank tau; the threshold is formed from tau as in
+// k_thresholds. A = +inf (or NaN) gives +inf.
+__device__ __forceinline__ float threshold_from_kth(float A, double qa, double qb, int inexact, int d) {
+    if (!(A < __builtin_inff())) return __builtin_inff();
+    const double u = 1.0 / 16777216.0;
+    const double delta = inexact ? (qa + qb) / 512.0 : 0.0;
+    const double e2 = 2.0 * (d + 2) * u * (qa + qb) * (qa + qb) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
+    const double s0 = __builtin_sqrt((A > 0.f ? (double)A : 0.0) + e2) + delta;
+    const double tau = s0 * s0 / (1.0 - (d + 2) * u) * (1.0 + 1.0 / 1048576.0);
+    const double tp = tau / (1.0 - (d + 2) * u);
+    const double s1 = __builtin_sqrt(tp) + delta;
+    return (float)((s1 * s1 + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
+}
+
 // Per query: A' >= the k-th smallest approximate distance over its ns
 // sample rows -- the upper edge of the bin holding the k-th in a histogram
 // of ST_BINS bins over the query's [min, max] (in the order-preserving
@@ -637,20 +703,116 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
     }
     __syncthreads();
     if (tid != 0) return;
-    const float A = key_float(s_edge);
-    if (!(A < __builtin_inff())) {
-        thr[q] = __builtin_inff();
-        return;
+    thr[q] = threshold_from_kth(key_float(s_edge), qnorm_f[q], __uint_as_float(*max_norm), *inexact, d);
+}
+
+// Per query, over all its nb rows: the k-th smallest of its approximate-
+// distance keys (keys[q * nb + row], approx_key16), A = the largest float
+// with that key (at least the k-th smallest approximate distance), the filter
+// threshold T from A as in k_sample_thresholds, and the candidates: every
+// row whose key is at most T's (a superset of the rows with approximate
+// distance <= T, by the keys' monotonicity), written to cand_row[q * cap +
+// i] for i < cap; cnt[q] = their number (more than cap: the caller ranks
+// every row). The keys of one query span few values: a 2,048-bin histogram
+// over their [min, max] finds the k-th. One workgroup of 256 threads per
+// query; its keys load eight at a time (nb a multiple of 8), once a pass
+// (the second and third passes mostly from the caches).
+__global__ void __launch_bounds__(256) k_dense_select(const uint16_t *__restrict__ keys, int64_t nb, int kmax,
+                                                      const int32_t *__restrict__ k_q,
+                                                      const float *__restrict__ qnorm_f,
+                                                      const unsigned *__restrict__ max_norm,
+                                                      const int *__restrict__ inexact, int d, int cap,
+                                                      int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row) {
+    __shared__ unsigned hist[ST_BINS];
+    __shared__ unsigned s_lo[4], s_hi[4];
+    __shared__ unsigned s_kt, s_n;
+    const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
+    const int64_t q = blockIdx.x;
+    int k = k_q ? k_q[q] : kmax;
+    k = k < 1 ? 1 : (k > kmax ? kmax : k);
+    const int nv = (int)(nb / 8);
+    const uint4 *kv = reinterpret_cast<const uint4 *>(keys + q * nb);
+    // f(row, key) over the query's keys, each thread its own rows
+    auto for_keys = [&](auto f) {
+        for (int v = tid; v < nv; v += 256) {
+            const uint4 w = kv[v];
+            const unsigned ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int j = 0; j < 8; ++j) f((int64_t)8 * v + j, (ws[j >> 1] >> (16 * (j & 1))) & 0xFFFFu, true);
+        }
+    };
+    unsigned lo = 0xFFFFu, hi = 0u;
+    for_keys([&](int64_t, unsigned key, bool) {
+        lo = key < lo ? key : lo;
+        hi = key > hi ? key : hi;
+    });
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned xl = __shfl_xor(lo, o), xh = __shfl_xor(hi, o);
+        lo = xl < lo ? xl : lo;
+        hi = xh > hi ? xh : hi;
     }
-    const double u = 1.0 / 16777216.0;
-    const double qa = qnorm_f[q], qb = (double)__uint_as_float(*max_norm);
-    const double delta = *inexact ? (qa + qb) / 512.0 : 0.0;
-    const double e2 = 2.0 * (d + 2) * u * (qa + qb) * (qa + qb) * (1.0 + 1.0 / 256.0) * (1.0 + 1.0 / 256.0);
-    const double s0 = __builtin_sqrt((A > 0.f ? (double)A : 0.0) + e2) + delta;
-    const double tau = s0 * s0 / (1.0 - (d + 2) * u) * (1.0 + 1.0 / 1048576.0);
-    const double tp = tau / (1.0 - (d + 2) * u);
-    const double s1 = __builtin_sqrt(tp) + delta;
-    thr[q] = (float)((s1 * s1 + e2) * (1.0 + 1.0 / 1048576.0) + 1e-30);
+    if (lane == 0) {
+        s_lo[wave] = lo;
+        s_hi[wave] = hi;
+    }
+    for (int b = tid; b < ST_BINS; b += 256) hist[b] = 0;
+    if (tid == 0) {
+        s_n = 0;
+        s_kt = 0xFFFFu;   // fewer than k rows: every row is a candidate
+    }
+    __syncthreads();
+    lo = s_lo[0];
+    hi = s_hi[0];
+    for (int w = 1; w < 4; ++w) {
+        lo = s_lo[w] < lo ? s_lo[w] : lo;
+        hi = s_hi[w] > hi ? s_hi[w] : hi;
+    }
+    int sh = 0;
+    while (((hi - lo) >> sh) >= (unsigned)ST_BINS) ++sh;
+    if ((int64_t)k <= nb) {
+        for_keys([&](int64_t, unsigned key, bool) { atomicAdd(&hist[(key - lo) >> sh], 1u); });
+        __syncthreads();
+        // the bin where the cumulative count reaches k (wave 0: 32 bins a
+        // lane), its largest key, the float bound A and the threshold's key
+        if (wave == 0) {
+            unsigned own = 0;
+            for (int b = 0; b < ST_BINS / 64; ++b) own += hist[lane * (ST_BINS / 64) + b];
+            unsigned incl = own;
+            for (int o = 1; o < 64; o <<= 1) {
+                const unsigned v = __shfl_up(incl, o);
+                if (lane >= o) incl += v;
+            }
+            const unsigned excl = incl - own;
+            if (excl < (unsigned)k && (unsigned)k <= incl) {
+                unsigned run = excl;
+                int b = lane * (ST_BINS / 64);
+                while (run + hist[b] < (unsigned)k) run += hist[b++];
+                const unsigned top = lo + (((unsigned)b + 1u) << sh) - 1u;
+                const float A = approx_key_upper(top < hi ? top : hi);
+                const float T = threshold_from_kth(A, qnorm_f[q], __uint_as_float(*max_norm), *inexact, d);
+                s_kt = approx_key16(T);
+            }
+        }
+    }
+    __syncthreads();
+    const unsigned kt = s_kt;
+    // the candidates, compacted per wave with one LDS atomic per wave and key
+    // position
+    for_keys([&](int64_t r, unsigned key, bool) {
+        const bool pass = key <= kt;
+        const unsigned long long m = __builtin_amdgcn_ballot_w64(pass);
+        if (m == 0) return;
+        unsigned base = 0;
+        if (lane == (int)__builtin_ctzll(m)) base = atomicAdd(&s_n, (unsigned)__popcll(m));
+        base = (unsigned)__shfl((int)base, (int)__builtin_ctzll(m));
+        if (pass) {
+            const unsigned pos = base + (unsigned)__builtin_amdgcn_mbcnt_hi(
+                                            (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+            if (pos < (unsigned)cap) cand_row[q * cap + pos] = (int32_t)r;
+        }
+    });
+    __syncthreads();
+    if (tid == 0) cnt[q] = (int32_t)s_n;
 }
 
 // Exact reference-order rank of every candidate, then the k best by
@@ -768,7 +930,7 @@ static size_t mfma_lds_r(int dp, bool dense) {
 }
 
 // dp <= 128: the register-resident kernel, grid (query tiles, row groups)
-template <bool DENSE>
+template <bool DENSE, bool KEY16 = false>
 static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                           const float *xn, const float *thr, int rt, int32_t *cnt, int32_t *cand_row,
                           float *cand_ap, int cap, float *dense, int64_t ld_dense, hipStream_t st) {
@@ -780,9 +942,9 @@ static void launch_pass_r(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t 
         const dim3 grid((unsigned)qt, (unsigned)yn);
 #define HANN_PASS_R(KS)                                                                                         \
     do {                                                                                                        \
-        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass_r<DENSE, KS>,                               \
+        HANN_HIP(hipFuncSetAttribute((const void *)dev::k_mfma_pass_r<DENSE, KS, KEY16>,                        \
                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));                    \
-        hipLaunchKernelGGL((dev::k_mfma_pass_r<DENSE, KS>), grid, dim3(256), shm, st, Q, nq, X, nb, qn, xn,     \
+        hipLaunchKernelGGL((dev::k_mfma_pass_r<DENSE, KS, KEY16>), grid, dim3(256), shm, st, Q, nq, X, nb, qn, xn, \
                            thr, rt, cnt, cand_row, cand_ap, cap, dense, ld_dense, y0);                          \
         HANN_LAUNCH_CHECK();                                                                                    \
     } while (0)
@@ -842,6 +1004,33 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
         hipLaunchKernelGGL(dev::k_mfma_pass<true>, dim3((unsigned)rg, (unsigned)yn), dim3(256), shm, st,
                            Q + y0 * dev::MF_TILE * dp, nq - y0 * dev::MF_TILE, X, nb, dp, qn + y0 * dev::MF_TILE, xn,
                            nullptr, rt, nullptr, nullptr, 0, dense + y0 * dev::MF_TILE * nb, nb);
+        HANN_LAUNCH_CHECK();
+    }
+}
+
+void launch_mfma_keys(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
+                      const float *xn, uint16_t *keys, hipStream_t st) {
+    if (nq == 0 || nb == 0) return;
+    if (dp > dev::MF_KC || nb % 4 != 0)
+        fail(HANN_ERR_UNSUPPORTED, strf("BF16 key pass: padded width %d, %lld rows", dp, (long long)nb));
+    launch_pass_r<true, true>(Q, nq, X, nb, dp, qn, xn, nullptr, 4, nullptr, nullptr, nullptr, 0,
+                              reinterpret_cast<float *>(keys), nb, st);
+}
+
+int64_t dense_select_max_rows() { return 65536; }
+
+void launch_dense_select(const uint16_t *keys, int64_t nq, int64_t nb, int k, const int32_t *k_q,
+                         const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, int cap,
+                         int32_t *cnt, int32_t *cand_row, hipStream_t st) {
+    if (nq == 0) return;
+    if (nb % 8 != 0 || nb > dense_select_max_rows())
+        fail(HANN_ERR_UNSUPPORTED, strf("dense select: %lld rows (a multiple of 8 up to %lld)", (long long)nb,
+                                        (long long)dense_select_max_rows()));
+    for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
+        const int64_t qn_ = nq - q0 < 65535 ? nq - q0 : 65535;
+        hipLaunchKernelGGL(dev::k_dense_select, dim3((unsigned)qn_), dim3(256), 0, st, keys + q0 * nb, nb, k,
+                           k_q ? k_q + q0 : nullptr, qnorm_f + q0, max_norm, inexact, d, cap, cnt + q0,
+                           cand_row + q0 * (int64_t)cap);
         HANN_LAUNCH_CHECK();
     }
 }

@@ -157,10 +157,14 @@ def test_batch_topk_other_metrics(hann, oracle, mid):
 @pytest.mark.parametrize("nq,nb,dim,k,kind", [(300, 20000, 128, 100, "int"), (130, 50000, 96, 10, "real"),
                                               (64, 4096, 100, 50, "real"), (5, 300, 32, 10, "real"),
                                               (200, 30000, 784, 20, "real"), (1000, 8000, 64, 1, "int"),
-                                              (50, 5000, 64, 10, "dup"), (700, 16384, 96, 128, "real")])
+                                              (50, 5000, 64, 10, "dup"), (700, 16384, 96, 128, "real"),
+                                              (200, 20001, 96, 50, "real"), (100, 70000, 64, 20, "int")])
 def test_batch_topk_bf16_certified_equals_exact(hann, oracle, nq, nb, dim, k, kind):
     """BF16 MFMA candidate pass + exact FP32 ranking: ids and distances are
-    identical to the exact path (the filter threshold is a proven bound)."""
+    identical to the exact path (the filter threshold is a proven bound).
+    Bases of at most 65,536 rows, a multiple of 8, and widths up to 128 take
+    the dense form (every pair's key, each query's k-th smallest); the others
+    (300 and 20,001 rows, width 784, 70,000 rows) the sampled form."""
     from hann_amd import _lib, core
     rng = np.random.default_rng(nq + nb + dim)
     if kind == "dup":
