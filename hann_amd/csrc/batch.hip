@@ -86,6 +86,7 @@ size_t certified_topk_scratch_bytes(int64_t nq, int64_t nb, int dim, int k, bool
         const int64_t qc = dense_qchunk(nq, nb);
         b += (size_t)std::max<int64_t>(0, qc * nb * 2 - std::max<int64_t>(qchunk * ns * 4, nq * 8));
         b += (size_t)std::max<int64_t>(0, qc * (int64_t)dense_cap(nb, k) * 4 - chunk * cap * 4);
+        b += (size_t)std::max<int64_t>(0, qc * 8 - nq * 4);   // slot 11: the key ranges
     }
     return b;
 }
@@ -143,10 +144,11 @@ void certified_topk(Context &c, const float *queries, int64_t nq, const float *b
         uint16_t *keys = w.get<uint16_t>(10, (size_t)(qchunk * nb));
         int32_t *cnt = w.get<int32_t>(12, (size_t)nq);
         int32_t *cand = w.get<int32_t>(13, (size_t)qchunk * cap);
+        unsigned *krange = w.get<unsigned>(11, (size_t)(2 * qchunk));   // each query's least and largest key
         for (int64_t q0 = 0; q0 < nq; q0 += qchunk) {
             const int64_t qn_ = std::min(qchunk, nq - q0);
-            launch_mfma_keys(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, keys, st);
-            launch_dense_select(keys, qn_, nb, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
+            launch_mfma_keys(qb + q0 * dp, qn_, xb, nb, dp, qn + q0, xn, keys, krange, krange + qchunk, st);
+            launch_dense_select(keys, krange, krange + qchunk, qn_, nb, k, k_q ? k_q + q0 : nullptr, qf + q0, stat,
                                 reinterpret_cast<int *>(stat + 1), dim, cap, cnt + q0, cand, st);
             // a query with more than cap candidates is ranked over every row
             launch_rank_candidates(queries + q0 * dim, qn_, dim, base, nb, cnt + q0, cand, nullptr, nullptr, cap, k,

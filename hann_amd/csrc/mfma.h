@@ -29,17 +29,18 @@ void launch_mfma_filter(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb
 void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
                        const float *xn, float *dense, hipStream_t st);
 // every approximate distance's 16-bit key (its top 16 bits; negative
-// values 0): keys[q * nb + row]; dp <= 128
+// values 0): keys[q * nb + row], and each query's least and largest key
+// (qlo[q], qhi[q]); dp <= 128, nb a multiple of 4
 void launch_mfma_keys(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                      const float *xn, uint16_t *keys, hipStream_t st);
+                      const float *xn, uint16_t *keys, unsigned *qlo, unsigned *qhi, hipStream_t st);
 // per query, from its nb keys: the certified threshold of its k-th
 // smallest and every row whose key is within it (cand_row[q * cap + i],
 // cnt[q] of them; more than cap: the caller ranks every row)
 // (nb a multiple of 8, at most dense_select_max_rows())
 int64_t dense_select_max_rows();
-void launch_dense_select(const uint16_t *keys, int64_t nq, int64_t nb, int k, const int32_t *k_q,
-                         const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, int cap,
-                         int32_t *cnt, int32_t *cand_row, hipStream_t st);
+void launch_dense_select(const uint16_t *keys, const unsigned *qlo, const unsigned *qhi, int64_t nq, int64_t nb,
+                         int k, const int32_t *k_q, const float *qnorm_f, const unsigned *max_norm,
+                         const int *inexact, int d, int cap, int32_t *cnt, int32_t *cand_row, hipStream_t st);
 // out rows = X rows 0, stride, 2 stride, ... (n of them) with their BF16 norms
 void launch_gather_bf16(const __bf16 *X, int dp, const float *xn, int64_t stride, int64_t n, __bf16 *out,
                         float *out_n, hipStream_t st);

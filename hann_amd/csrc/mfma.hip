@@ -11,6 +11,7 @@
 // result equals the exact path's.
 #include "common.h"
 #include "mfma.h"
+#include "scan_list.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
@@ -402,6 +403,9 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
         if (tid < MF_TILE) xreg = r0 + tid < nb ? xn[r0 + tid] : 0.f;
     };
     fetch(rbase);
+    // KEY16: this lane's least and largest key of each query tile i (its
+    // query's valid rows), folded into the query's range at the end
+    unsigned kmin[2] = {0xFFFFu, 0xFFFFu}, kmax[2] = {0u, 0u};
     constexpr int SL = MF_SLOTS_R / 2;
     // the running candidate counts of each query in this wave's row half,
     // wave-uniform: lanes 0-31's count in the low 16 bits, lanes 32-63's in
@@ -469,6 +473,12 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
 #pragma unroll
                         for (int u = 0; u < 4; ++u) kk[u] = approx_key16((qv + xs[u]) - 2.f * acc[i][4 * g + u]);
                         const int64_t gr0 = r0 + rl0;
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+                            if (gr0 + u < nb) {
+                                kmin[i] = kk[u] < kmin[i] ? kk[u] : kmin[i];
+                                kmax[i] = kk[u] > kmax[i] ? kk[u] : kmax[i];
+                            }
                         if (gq < nq) {
                             uint16_t *dst = reinterpret_cast<uint16_t *>(dense) + gq * ld_dense + gr0;
                             if (gr0 + 3 < nb) {
@@ -540,6 +550,21 @@ k_mfma_pass_r(const __bf16 *__restrict__ Q, int64_t nq, const __bf16 *__restrict
                     }
                 }
         }
+    }
+    if (KEY16) {
+        // each query's key range: lanes l and l + 32 hold the same query;
+        // cnt[q] takes the least key, cand_row[q] the largest
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            const unsigned mn = __shfl_xor(kmin[i], 32), mx = __shfl_xor(kmax[i], 32);
+            const unsigned lo = mn < kmin[i] ? mn : kmin[i], hi = mx > kmax[i] ? mx : kmax[i];
+            const int64_t gq = q0 + wm * 64 + i * 32 + lane % 32;
+            if (lane < 32 && gq < nq) {
+                atomicMin(reinterpret_cast<unsigned *>(cnt) + gq, lo);
+                atomicMax(reinterpret_cast<unsigned *>(cand_row) + gq, hi);
+            }
+        }
+        return;
     }
     if (DENSE) return;
     if ((lane & 31) == 0) {
@@ -714,17 +739,19 @@ __global__ void __launch_bounds__(256) k_sample_thresholds(const float *__restri
 // distance <= T, by the keys' monotonicity), written to cand_row[q * cap +
 // i] for i < cap; cnt[q] = their number (more than cap: the caller ranks
 // every row). The keys of one query span few values: a 2,048-bin histogram
-// over their [min, max] finds the k-th. One workgroup of 256 threads per
-// query; its keys load eight at a time (nb a multiple of 8), once a pass
-// (the second and third passes mostly from the caches).
+// over their [min, max] (qlo[q], qhi[q], from the key pass) finds the k-th.
+// One workgroup of 256 threads per query; its keys load eight at a time (nb
+// a multiple of 8), once for the histogram and once for the candidates
+// (mostly from the caches).
 __global__ void __launch_bounds__(256) k_dense_select(const uint16_t *__restrict__ keys, int64_t nb, int kmax,
+                                                      const unsigned *__restrict__ qlo,
+                                                      const unsigned *__restrict__ qhi,
                                                       const int32_t *__restrict__ k_q,
                                                       const float *__restrict__ qnorm_f,
                                                       const unsigned *__restrict__ max_norm,
                                                       const int *__restrict__ inexact, int d, int cap,
                                                       int32_t *__restrict__ cnt, int32_t *__restrict__ cand_row) {
     __shared__ unsigned hist[ST_BINS];
-    __shared__ unsigned s_lo[4], s_hi[4];
     __shared__ unsigned s_kt, s_n;
     const int tid = threadIdx.x, lane = tid % 64, wave = tid / 64;
     const int64_t q = blockIdx.x;
@@ -741,32 +768,14 @@ __global__ void __launch_bounds__(256) k_dense_select(const uint16_t *__restrict
             for (int j = 0; j < 8; ++j) f((int64_t)8 * v + j, (ws[j >> 1] >> (16 * (j & 1))) & 0xFFFFu, true);
         }
     };
-    unsigned lo = 0xFFFFu, hi = 0u;
-    for_keys([&](int64_t, unsigned key, bool) {
-        lo = key < lo ? key : lo;
-        hi = key > hi ? key : hi;
-    });
-    for (int o = 32; o > 0; o >>= 1) {
-        const unsigned xl = __shfl_xor(lo, o), xh = __shfl_xor(hi, o);
-        lo = xl < lo ? xl : lo;
-        hi = xh > hi ? xh : hi;
-    }
-    if (lane == 0) {
-        s_lo[wave] = lo;
-        s_hi[wave] = hi;
-    }
+    // the key range the key pass recorded
+    const unsigned lo = qlo[q], hi = qhi[q] > lo ? qhi[q] : lo;
     for (int b = tid; b < ST_BINS; b += 256) hist[b] = 0;
     if (tid == 0) {
         s_n = 0;
         s_kt = 0xFFFFu;   // fewer than k rows: every row is a candidate
     }
     __syncthreads();
-    lo = s_lo[0];
-    hi = s_hi[0];
-    for (int w = 1; w < 4; ++w) {
-        lo = s_lo[w] < lo ? s_lo[w] : lo;
-        hi = s_hi[w] > hi ? s_hi[w] : hi;
-    }
     int sh = 0;
     while (((hi - lo) >> sh) >= (unsigned)ST_BINS) ++sh;
     if ((int64_t)k <= nb) {
@@ -1009,19 +1018,22 @@ void launch_mfma_dense(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb,
 }
 
 void launch_mfma_keys(const __bf16 *Q, int64_t nq, const __bf16 *X, int64_t nb, int dp, const float *qn,
-                      const float *xn, uint16_t *keys, hipStream_t st) {
+                      const float *xn, uint16_t *keys, unsigned *qlo, unsigned *qhi, hipStream_t st) {
     if (nq == 0 || nb == 0) return;
     if (dp > dev::MF_KC || nb % 4 != 0)
         fail(HANN_ERR_UNSUPPORTED, strf("BF16 key pass: padded width %d, %lld rows", dp, (long long)nb));
-    launch_pass_r<true, true>(Q, nq, X, nb, dp, qn, xn, nullptr, 4, nullptr, nullptr, nullptr, 0,
-                              reinterpret_cast<float *>(keys), nb, st);
+    // fill kernels, not memsets: the search may be captured in a graph
+    launch_fill_i32(reinterpret_cast<int32_t *>(qlo), nq, 0xFFFF, st);
+    launch_fill_i32(reinterpret_cast<int32_t *>(qhi), nq, 0, st);
+    launch_pass_r<true, true>(Q, nq, X, nb, dp, qn, xn, nullptr, 4, reinterpret_cast<int32_t *>(qlo),
+                              reinterpret_cast<int32_t *>(qhi), nullptr, 0, reinterpret_cast<float *>(keys), nb, st);
 }
 
 int64_t dense_select_max_rows() { return 65536; }
 
-void launch_dense_select(const uint16_t *keys, int64_t nq, int64_t nb, int k, const int32_t *k_q,
-                         const float *qnorm_f, const unsigned *max_norm, const int *inexact, int d, int cap,
-                         int32_t *cnt, int32_t *cand_row, hipStream_t st) {
+void launch_dense_select(const uint16_t *keys, const unsigned *qlo, const unsigned *qhi, int64_t nq, int64_t nb,
+                         int k, const int32_t *k_q, const float *qnorm_f, const unsigned *max_norm,
+                         const int *inexact, int d, int cap, int32_t *cnt, int32_t *cand_row, hipStream_t st) {
     if (nq == 0) return;
     if (nb % 8 != 0 || nb > dense_select_max_rows())
         fail(HANN_ERR_UNSUPPORTED, strf("dense select: %lld rows (a multiple of 8 up to %lld)", (long long)nb,
@@ -1029,7 +1041,7 @@ void launch_dense_select(const uint16_t *keys, int64_t nq, int64_t nb, int k, co
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn_ = nq - q0 < 65535 ? nq - q0 : 65535;
         hipLaunchKernelGGL(dev::k_dense_select, dim3((unsigned)qn_), dim3(256), 0, st, keys + q0 * nb, nb, k,
-                           k_q ? k_q + q0 : nullptr, qnorm_f + q0, max_norm, inexact, d, cap, cnt + q0,
+                           qlo + q0, qhi + q0, k_q ? k_q + q0 : nullptr, qnorm_f + q0, max_norm, inexact, d, cap, cnt + q0,
                            cand_row + q0 * (int64_t)cap);
         HANN_LAUNCH_CHECK();
     }
