@@ -187,103 +187,129 @@ __device__ __forceinline__ int32_t list_items(int32_t pairs, int64_t len, ItemSh
     return pairs > 0 ? ((pairs + (1 << sh.qg) - 1) >> sh.qg) * (int32_t)((len + (1 << sh.ch) - 1) >> sh.ch) : 0;
 }
 
-// one workgroup of 1024: exclusive scans of lcnt and of the item counts,
-// in spans of 16 rounds of 1024 lists (thread t holds list r * 1024 + t of
-// round r; all loads issued first): the waves scan all rounds at once, one
-// wave turns the waves' round totals into prefixes, and every list's offset
-// follows, with two block barriers a span
-constexpr int PO_ROUNDS = 16;   // lists up to 16,384 a span
+// one workgroup of 1024: exclusive scans of lcnt and of the item counts, in
+// spans of 16,384 lists. Thread t holds the 16 consecutive lists 16 t .. of
+// the span (vector loads and stores), sums them itself, and one block scan
+// of the 1,024 thread sums (wave shuffles, then the 16 wave totals by one
+// wave) gives each thread its offset, with two block barriers a span
+constexpr int PO_PER = 16;   // lists per thread
 
 __global__ void __launch_bounds__(1024) k_pair_offsets(const int32_t *__restrict__ lcnt,
                                                        const int64_t *__restrict__ list_len, int K,
                                                        int32_t *__restrict__ lstart, int32_t *__restrict__ ioff,
                                                        int32_t *__restrict__ lfill, int32_t *__restrict__ lbfill,
                                                        int32_t *__restrict__ totals, ItemShape sh) {
-    // [wave][round] wave totals, then their exclusive prefixes over the
-    // waves plus the rounds before (the span's carry included)
-    __shared__ int32_t s_a[16][PO_ROUNDS], s_b[16][PO_ROUNDS];
+    // the waves' totals, then their exclusive prefixes (the span's carry
+    // included)
+    __shared__ int32_t s_a[16], s_b[16];
     __shared__ int32_t s_ca, s_cb;
     const int t = threadIdx.x, lane = t & 63, w = t >> 6;
     int32_t carry_a = 0, carry_b = 0;
-    for (int g0 = 0; g0 < K; g0 += 1024 * PO_ROUNDS) {
-        int32_t v[PO_ROUNDS], it[PO_ROUNDS], a[PO_ROUNDS], b[PO_ROUNDS];
+    for (int g0 = 0; g0 < K; g0 += 1024 * PO_PER) {
+        const int l0 = g0 + t * PO_PER;
+        int32_t v[PO_PER], it[PO_PER];
+        const bool whole = l0 + PO_PER <= K;
+        if (whole) {
+            // 64-byte aligned runs (buffers from the allocator, l0 a
+            // multiple of 16)
+            const int4 *c4 = reinterpret_cast<const int4 *>(lcnt + l0);
+            const longlong2 *n2 = reinterpret_cast<const longlong2 *>(list_len + l0);
 #pragma unroll
-        for (int r = 0; r < PO_ROUNDS; ++r) {
-            const int l = g0 + r * 1024 + t;
-            const int lc = l < K ? l : K - 1;
-            const int32_t c = lcnt[lc];
-            const int64_t n = list_len[lc];
-            v[r] = l < K ? c : 0;
-            it[r] = l < K ? list_items(c, n, sh) : 0;
-            a[r] = v[r];
-            b[r] = it[r];
+            for (int u = 0; u < PO_PER / 4; ++u) {
+                const int4 c = c4[u];
+                v[4 * u] = c.x;
+                v[4 * u + 1] = c.y;
+                v[4 * u + 2] = c.z;
+                v[4 * u + 3] = c.w;
+            }
+#pragma unroll
+            for (int u = 0; u < PO_PER / 2; ++u) {
+                const longlong2 n = n2[u];
+                it[2 * u] = list_items(v[2 * u], n.x, sh);
+                it[2 * u + 1] = list_items(v[2 * u + 1], n.y, sh);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PO_PER; ++r) {
+                const int l = l0 + r;
+                v[r] = l < K ? lcnt[l] : 0;
+                it[r] = l < K ? list_items(v[r], list_len[l], sh) : 0;
+            }
         }
-        // inclusive scans over the wave's lanes, every round
-        for (int o = 1; o < 64; o <<= 1) {
+        int32_t sa = 0, sb = 0;
 #pragma unroll
-            for (int r = 0; r < PO_ROUNDS; ++r) {
-                const int32_t xa = __shfl_up(a[r], o), xb = __shfl_up(b[r], o);
-                if (lane >= o) {
-                    a[r] += xa;
-                    b[r] += xb;
-                }
+        for (int r = 0; r < PO_PER; ++r) {
+            sa += v[r];
+            sb += it[r];
+        }
+        // inclusive scans of the thread sums over the wave's lanes
+        int32_t ia = sa, ib = sb;
+        for (int o = 1; o < 64; o <<= 1) {
+            const int32_t xa = __shfl_up(ia, o), xb = __shfl_up(ib, o);
+            if (lane >= o) {
+                ia += xa;
+                ib += xb;
             }
         }
         if (lane == 63) {
-#pragma unroll
-            for (int r = 0; r < PO_ROUNDS; ++r) {
-                s_a[w][r] = a[r];
-                s_b[w][r] = b[r];
-            }
+            s_a[w] = ia;
+            s_b[w] = ib;
         }
         __syncthreads();
         if (w == 0) {
-            // lane r < PO_ROUNDS: round r's prefixes over the waves, then
-            // the rounds' totals scanned over the lanes
-            int32_t ra = 0, rb = 0;
-            if (lane < PO_ROUNDS) {
-                for (int u = 0; u < 16; ++u) {
-                    const int32_t xa = s_a[u][lane], xb = s_b[u][lane];
-                    s_a[u][lane] = ra;
-                    s_b[u][lane] = rb;
-                    ra += xa;
-                    rb += xb;
-                }
-            }
-            int32_t ia = ra, ib = rb;
-            for (int o = 1; o < PO_ROUNDS; o <<= 1) {
-                const int32_t xa = __shfl_up(ia, o), xb = __shfl_up(ib, o);
+            // lanes < 16: the waves' totals scanned, made exclusive, plus the
+            // carry of the spans before
+            const int32_t ra = lane < 16 ? s_a[lane] : 0, rb = lane < 16 ? s_b[lane] : 0;
+            int32_t ja = ra, jb = rb;
+            for (int o = 1; o < 16; o <<= 1) {
+                const int32_t xa = __shfl_up(ja, o), xb = __shfl_up(jb, o);
                 if (lane >= o) {
-                    ia += xa;
-                    ib += xb;
+                    ja += xa;
+                    jb += xb;
                 }
             }
-            if (lane < PO_ROUNDS) {
-                const int32_t ea = carry_a + ia - ra, eb = carry_b + ib - rb;
-                for (int u = 0; u < 16; ++u) {
-                    s_a[u][lane] += ea;
-                    s_b[u][lane] += eb;
-                }
+            if (lane < 16) {
+                s_a[lane] = carry_a + ja - ra;
+                s_b[lane] = carry_b + jb - rb;
             }
-            if (lane == PO_ROUNDS - 1) {
-                s_ca = carry_a + ia;
-                s_cb = carry_b + ib;
+            if (lane == 15) {
+                s_ca = carry_a + ja;
+                s_cb = carry_b + jb;
             }
         }
         __syncthreads();
+        int32_t oa = s_a[w] + ia - sa, ob = s_b[w] + ib - sb;
+        int32_t xs[PO_PER], ys[PO_PER];
 #pragma unroll
-        for (int r = 0; r < PO_ROUNDS; ++r) {
-            const int l = g0 + r * 1024 + t;
-            if (l < K) {
-                lstart[l] = s_a[w][r] + a[r] - v[r];
-                ioff[l] = s_b[w][r] + b[r] - it[r];
-                lfill[l] = 0;
-                lbfill[l] = 0;
+        for (int r = 0; r < PO_PER; ++r) {
+            xs[r] = oa;
+            ys[r] = ob;
+            oa += v[r];
+            ob += it[r];
+        }
+        if (whole) {
+#pragma unroll
+            for (int u = 0; u < PO_PER / 4; ++u) {
+                reinterpret_cast<int4 *>(lstart + l0)[u] = make_int4(xs[4 * u], xs[4 * u + 1], xs[4 * u + 2], xs[4 * u + 3]);
+                reinterpret_cast<int4 *>(ioff + l0)[u] = make_int4(ys[4 * u], ys[4 * u + 1], ys[4 * u + 2], ys[4 * u + 3]);
+                reinterpret_cast<int4 *>(lfill + l0)[u] = make_int4(0, 0, 0, 0);
+                reinterpret_cast<int4 *>(lbfill + l0)[u] = make_int4(0, 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int r = 0; r < PO_PER; ++r) {
+                const int l = l0 + r;
+                if (l < K) {
+                    lstart[l] = xs[r];
+                    ioff[l] = ys[r];
+                    lfill[l] = 0;
+                    lbfill[l] = 0;
+                }
             }
         }
         carry_a = s_ca;
         carry_b = s_cb;
-        __syncthreads();   // s_a, s_b are rewritten by the next span
+        __syncthreads();   // s_a, s_b, s_ca, s_cb are rewritten by the next span
     }
     if (t == 0) {
         totals[0] = carry_a;
