@@ -829,7 +829,14 @@ __global__ void __launch_bounds__(256) k_dense_select(const uint16_t *__restrict
 // write nothing (the host reruns them on the exact path). With cand_ap, only
 // the candidates whose approximate distance is within thr2[q] are ranked:
 // they are packed 64 at a time through LDS, so every lane ranks a row.
-template <typename CT>
+// NB > 0 (d = 32 NB, 16-byte aligned rows): each row is ranked by a quad of
+// lanes, lane j of the quad holding accumulator slots 8 j .. 8 j + 7 of the
+// reference's 32-slot order (elements 8 j + i + 32 b, fused in block order),
+// so the quad's loads of a block cover one 128-byte line; the quad then
+// combines the slots as the reference does, (acc[l] + acc[8 + l]) +
+// (acc[16 + l] + acc[24 + l]) by two quad-permute moves, and lane 0 takes
+// the horizontal sum: the same bits as sqdiff_exact, 16 rows a step.
+template <typename CT, int NB>
 __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict__ queries, int d,
                                                         const float *__restrict__ base,
                                                         const int32_t *__restrict__ cnt,
@@ -842,6 +849,7 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
     float *s_keys = reinterpret_cast<float *>(s_raw);
     int64_t *s_ids = reinterpret_cast<int64_t *>(s_keys + ccap);
     int64_t *s_buf = s_ids + ccap;   // [2 kWave] rows waiting to be ranked
+    float *s_d = reinterpret_cast<float *>(s_buf + 2 * kWave);   // [kWave] distances (NB > 0)
     const int lane = threadIdx.x;
     const int64_t q = blockIdx.x;
     // a query whose candidates overflowed the room ranks every row (nb > 0),
@@ -854,9 +862,65 @@ __global__ void __launch_bounds__(64) k_rank_candidates(const float *__restrict_
     WaveTopK sel;
     sel.init(s_keys, s_ids, kq, ccap);
     const float *qv = queries + q * d;
+    // NB > 0: this lane's query values, elements 8 j + i + 32 b
+    float qr[NB > 0 ? NB : 1][8];
+#pragma unroll
+    for (int b = 0; b < NB; ++b)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) qr[b][i] = qv[32 * b + 8 * (lane & 3) + i];
+    // the valid lanes of every call are a prefix of the wave (lane 0 valid)
     auto rank = [&](int64_t row, bool valid) {
         float key = 0.f;
-        if (valid) key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
+        if constexpr (NB > 0) {
+            const int64_t row0 = (int64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)row) |
+                                 ((int64_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)row >> 32)) << 32);
+            const int64_t rv = valid ? row : row0;
+            const int j = lane & 3, qd = lane >> 2;
+            // step s: the quad ranks the row of lane 16 s + qd
+            float4 x[4][2 * NB];
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                const int src = 16 * st + qd;
+                const int64_t rr = (int64_t)(uint32_t)__builtin_amdgcn_ds_bpermute(4 * src, (int)(uint32_t)rv) |
+                                   ((int64_t)__builtin_amdgcn_ds_bpermute(4 * src, (int)(uint32_t)((uint64_t)rv >> 32)) << 32);
+                const float4 *xr = reinterpret_cast<const float4 *>(base + rr * d + 8 * j);
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    x[st][2 * b] = xr[8 * b];
+                    x[st][2 * b + 1] = xr[8 * b + 1];
+                }
+            }
+#pragma unroll
+            for (int st = 0; st < 4; ++st) {
+                float acc[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) acc[i] = 0.f;
+#pragma unroll
+                for (int b = 0; b < NB; ++b) {
+                    const float4 lo = x[st][2 * b], hi = x[st][2 * b + 1];
+                    const float xs[8] = {lo.x, lo.y, lo.z, lo.w, hi.x, hi.y, hi.z, hi.w};
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) {
+                        const float dd = qr[b][i] - xs[i];
+                        acc[i] = __builtin_fmaf(dd, dd, acc[i]);
+                    }
+                }
+                float s8[8];
+#pragma unroll
+                for (int i = 0; i < 8; ++i) {
+                    // quad_perm [1,0,3,2], then [2,3,0,1]
+                    const float t = acc[i] + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(acc[i]), 0xB1, 0xF, 0xF, false));
+                    s8[i] = t + __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(t), 0x4E, 0xF, 0xF, false));
+                }
+                const float dist = hsum8(s8);
+                if (j == 0) s_d[16 * st + qd] = dist;
+            }
+            wave_sync();
+            key = s_d[lane];
+            wave_sync();   // read before the next call writes
+        } else {
+            if (valid) key = sqdiff_exact(PtrLoad{qv}, PtrLoad{base + row * d}, d);
+        }
         sel.push(key, row, valid);
     };
     if (every || !cand_ap) {
@@ -1064,7 +1128,22 @@ void launch_sample_thresholds(const float *approx, int64_t nq, int64_t ns, const
     HANN_LAUNCH_CHECK();
 }
 
-static size_t rank_lds(int ccap) { return (size_t)ccap * 12 + (size_t)2 * dev::kWave * 8 + 16; }
+static size_t rank_lds(int ccap) { return (size_t)ccap * 12 + (size_t)2 * dev::kWave * 8 + (size_t)dev::kWave * 4 + 16; }
+
+// the rank kernel for rows of d floats at base: rows ranked by quads for
+// d in {32, 64, 96} with 16-byte aligned rows, one per lane otherwise (at
+// d = 128 the quads' row registers halve the waves a SIMD holds, and the
+// k-means assignment and the 1M-row top-k measured no faster)
+template <typename CT>
+static decltype(&dev::k_rank_candidates<CT, 0>) rank_kernel(int d, const float *base) {
+    if (d % 32 != 0 || ((uintptr_t)base & 15) != 0) return dev::k_rank_candidates<CT, 0>;
+    switch (d / 32) {
+    case 1: return dev::k_rank_candidates<CT, 1>;
+    case 2: return dev::k_rank_candidates<CT, 2>;
+    case 3: return dev::k_rank_candidates<CT, 3>;
+    default: return dev::k_rank_candidates<CT, 0>;
+    }
+}
 
 void launch_rank_candidates(const float *queries, int64_t nq, int d, const float *base, int64_t nb,
                             const int32_t *cnt, const int32_t *cand_row, const float *cand_ap, const float *thr2,
@@ -1073,11 +1152,11 @@ void launch_rank_candidates(const float *queries, int64_t nq, int d, const float
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = rank_lds(ccap);
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int32_t>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const auto kern = rank_kernel<int32_t>(d, base);
+    HANN_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
-        hipLaunchKernelGGL(dev::k_rank_candidates<int32_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d,
+        hipLaunchKernelGGL(kern, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d, d,
                            base, cnt + q0, cand_row + q0 * (int64_t)cap, cand_ap ? cand_ap + q0 * (int64_t)cap : nullptr,
                            thr2 ? thr2 + q0 : nullptr, cap, nb, k, k_q ? k_q + q0 : nullptr, ccap, sqrt_out,
                            out_ids + q0 * k, out_dist + q0 * k);
@@ -1091,11 +1170,11 @@ void launch_rank_rows64(const float *queries, int64_t nq, int d, const float *ba
     if (nq == 0) return;
     const int ccap = 2 * kp_for(k);
     const size_t shm = rank_lds(ccap);
-    HANN_HIP(hipFuncSetAttribute((const void *)dev::k_rank_candidates<int64_t>,
-                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
+    const auto kern = rank_kernel<int64_t>(d, base);
+    HANN_HIP(hipFuncSetAttribute((const void *)kern, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shm));
     for (int64_t q0 = 0; q0 < nq; q0 += 65535) {
         const int64_t qn = nq - q0 < 65535 ? nq - q0 : 65535;
-        hipLaunchKernelGGL(dev::k_rank_candidates<int64_t>, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d,
+        hipLaunchKernelGGL(kern, dim3((unsigned)qn), dim3(64), shm, st, queries + q0 * d,
                            d, base, cnt + q0, rows + q0 * (int64_t)cap, nullptr, nullptr, cap, nb, k,
                            k_q ? k_q + q0 : nullptr, ccap,
                            sqrt_out, out_ids + q0 * k, out_dist + q0 * k);
