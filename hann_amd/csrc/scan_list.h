@@ -46,9 +46,9 @@ struct ListScanArgs {
 struct ItemShape {
     int qg, ch;
 };
-// the MFMA scan's items (32 queries x 2,048 entries) and the exact scan's
+// the MFMA scan's items (16 queries x 2,048 entries) and the exact scan's
 // (8 x 256) when untuned
-constexpr ItemShape kListItems{5, 11};
+constexpr ItemShape kListItems{4, 11};
 constexpr ItemShape kExactItems{3, 8};
 // a list-major scan's configuration: its work items and item partitions
 struct ListTune {
@@ -57,9 +57,9 @@ struct ListTune {
 };
 // the configurations the autotuner times (the first is the default); the
 // smallest chunk and query group among them size the batch's item buffers
-constexpr ListTune kListTunes[] = {{{5, 11}, 256}, {{5, 10}, 256}, {{5, 11}, 128}, {{5, 10}, 128}};
+constexpr ListTune kListTunes[] = {{{4, 11}, 256}, {{4, 10}, 256}, {{4, 11}, 128}, {{4, 10}, 128}};
 constexpr ListTune kExactTunes[] = {{{3, 8}, 256}, {{2, 8}, 256}, {{3, 7}, 256}, {{3, 9}, 256}, {{4, 8}, 256}};
-constexpr ItemShape kListItemsMin{5, 10};
+constexpr ItemShape kListItemsMin{4, 10};
 constexpr ItemShape kExactItemsMin{2, 7};
 // entries of a list per work item of the list-major scan
 int list_scan_chunk();

@@ -7,10 +7,10 @@
 // and for a list l probed by queries Q_l the inner products are one product
 // of the decoded list [n_l x d] with Q_l^T [d x |Q_l|]. The pairs (query,
 // probed list) of a batch are grouped by list; a work item is one chunk of up
-// to 2,048 entries of a list with up to 32 of its queries, and belongs to one
+// to 2,048 entries of a list with up to 16 of its queries, and belongs to one
 // wave. The wave gathers 32 entries at a time from a BF16 codebook in LDS
-// (BF16 hi + lo for the split mode) straight into the A operand of
-// v_mfma_f32_32x32x16_bf16 and multiplies them with the item's queries,
+// (BF16 hi + lo for the split mode) straight into the A operands of two
+// v_mfma_f32_16x16x32_bf16 tiles and multiplies them with the item's queries,
 // which stay in registers as the B operand (scaled by -2 and the query's
 // key scale); the accumulator starts at the entry's and the pair's terms, so
 // it ends at the scaled distance. Each list's codes are read once per item
@@ -40,11 +40,12 @@
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 namespace hann {
 namespace dev {
 
-constexpr int LQ = 32;          // queries per work item (the MFMA's N)
+constexpr int LQ = 16;          // queries per work item (the MFMA's N)
 constexpr int LCH_SHIFT = 11;
 constexpr int LCH = 1 << LCH_SHIFT;   // entries per work item (a chunk of the list; a multiple of 32)
 constexpr int SEL_MAXP = 2048;  // probes per query the threshold and select kernels take
@@ -486,21 +487,26 @@ __global__ void __launch_bounds__(256) k_make_items(const int32_t *__restrict__ 
 // (d = 4 M) and uint8 codes. The codebook sits in LDS as BF16 (hi, and lo
 // for the split mode).
 //
-// Operand layout of v_mfma_f32_32x32x16_bf16 (k-step j < M / 4, lane l,
-// h = l / 32): the contraction runs over the dimensions in the order that
-// gives half h the subquantizers h M / 2 .. (h + 1) M / 2 - 1, two per
-// k-step: k-step j, half h covers subquantizers h M / 2 + 2 j and + 1, i.e.
-// dimensions 2 h M + 8 j .. + 7. So a lane needs one contiguous half of its
-// entry's code row (M / 2 bytes, one load), and the B operand (column
-// l % 32: the item's query of that rank) is eight contiguous floats of the
-// query per k-step, loaded once per item. The A operand of lane l is the
-// two codewords of entry l % 32 of the slab, gathered straight from LDS.
+// A slab is 32 entries, two tiles of 16, times the item's up to 16 queries
+// (a list is probed by 12.6 queries on average at the benchmark's shape, so
+// a 16-query tile wastes less than a 32-query one). Operand layout of
+// v_mfma_f32_16x16x32_bf16 (k-block j < M / 8 of 32 dimensions, lane l,
+// group g = l / 16): the contraction runs over the dimensions in the order
+// that gives group g the subquantizers g M / 4 .. (g + 1) M / 4 - 1, two per
+// k-block: k-block j, group g covers subquantizers g M / 4 + 2 j and + 1,
+// i.e. dimensions g M + 8 j .. + 7. So a lane needs one run of M / 4
+// contiguous bytes of each of its two entries' code rows (from the
+// dword-aligned words that hold it), and the B operand (column l % 16: the
+// item's query of that rank) is eight contiguous floats of the query per
+// k-block, loaded once per item. The A operand of lane l for tile t is the
+// two codewords of entry 16 t + l % 16 of the slab, gathered straight from
+// LDS.
 //
-// D register r of lane l holds entry 8 (r / 4) + 4 h + r % 4 of the slab for
-// query l % 32, so the four registers r = 4g .. 4g + 3 are four consecutive
-// entries of one query's row: in the sample pass each becomes one 8-byte
-// store of four keys, with no transposition. The entry terms of a slab pass
-// through a 32-float LDS slot of the wave, read back as float4 broadcasts.
+// D register i of tile t of lane l holds entry 16 t + 4 g + i of the slab for
+// query l % 16, so a tile's four registers are four consecutive entries of
+// one query's row: in the sample pass each tile becomes one 8-byte store of
+// four keys, with no transposition. The entry terms of a slab pass through a
+// 32-float LDS slot of the wave, read back as float4 broadcasts.
 //
 // The filter pass tests a register's value against the lane's bound on the
 // value bits (no key conversion), takes the wave's ballot and, where any lane
@@ -548,7 +554,9 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     static_assert(!(X3 && F8), "one operand mode");
     extern __shared__ __align__(16) unsigned char s_raw[];
     constexpr int D = 4 * M;
-    constexpr int HW = M / 8;   // 32-bit words of half a code row
+    constexpr int KB = M / 8;          // k-blocks of 32 dimensions (MFMAs per tile)
+    constexpr int RUN = M / 4;         // code bytes of a lane group's run
+    constexpr int ND = (RUN + (M % 16 ? 2 : 0) + 3) / 4;   // dwords that hold the run
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int P = la.P;
     bf16x4 *sCBh = reinterpret_cast<bf16x4 *>(s_raw);            // [M][P]
@@ -587,7 +595,11 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
     }
     __syncthreads();
     const int n_items = *la.n_items;
-    const int col = lane & 31, half = lane >> 5;
+    // lane: column (query rank in the item) col, k-group grp; the group's run
+    // of RUN code bytes starts at byte sb of an entry's code row, in the
+    // dword-aligned words loaded from sb & ~3, shifted by sb & 3 bytes
+    const int col = lane & 15, grp = lane >> 4;
+    const int sb = RUN * grp, sbo = sb & 3;
     // work items of this workgroup's partition: part, part + nparts, ...
     const int nparts = la.n_parts, part = (int)(blockIdx.x % (unsigned)nparts);
     int32_t *ctr = la.item_counter + part;
@@ -617,20 +629,23 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         S.off = (int)la.pair_off[pidx];
         return S;
     };
-    // one slab's loads: this lane's half code row of entry b + col and
-    // that entry's term (rows past the chunk read its last entry, or the
-    // entry before an empty chunk, and are discarded, so the loads are
-    // unconditional)
+    // one slab's loads: this lane group's code run of entries b + col and
+    // b + 16 + col (tiles 0 and 1) and their terms (rows past the chunk
+    // read its last entry, or the entry before an empty chunk, and are
+    // discarded, so the loads are unconditional)
     struct Slab {
-        uint32_t w[HW];
-        float t;
+        uint32_t w[2][ND];
+        float t[2];
     };
     auto load_slab = [&](int e0, int b, int ee, Slab &x) {
-        const int e = b + col < ee ? b + col : ee - 1;
-        const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes + (int64_t)(e0 + e) * M + half * (M / 2));
 #pragma unroll
-        for (int u = 0; u < HW; ++u) x.w[u] = cw[u];
-        x.t = la.term[e0 + e];
+        for (int t = 0; t < 2; ++t) {
+            const int e = b + 16 * t + col < ee ? b + 16 * t + col : ee - 1;
+            const uint32_t *cw = reinterpret_cast<const uint32_t *>(codes + (int64_t)(e0 + e) * M + (sb & ~3));
+#pragma unroll
+            for (int u = 0; u < ND; ++u) x.w[t][u] = cw[u];
+            x.t[t] = la.term[e0 + e];
+        }
     };
     // item indices are wave-uniform (taken from lane 0 into a scalar), so
     // item headers load through the scalar unit and the slab loop's
@@ -707,18 +722,19 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // times -2 sc (a power of two, so the BF16 rounding is scaled
         // exactly); the accumulator starts at (t + |q - c|^2) sc, so it ends
         // at the scaled key ((qc + t) - 2 <q, y>) sc
-        bf16x8 bq[M / 4], bl[X3 ? M / 4 : 1];
-        long bq8[F8 ? M / 4 : 1];
+        bf16x8 bq[KB], bl[X3 ? KB : 1];
+        long bq8[F8 ? KB : 1];
         if (F8) {
-            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
+            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + grp * (M / 4);
             float mx = 0.f;
 #pragma unroll
-            for (int jj = 0; jj < M / 2; ++jj) {
+            for (int jj = 0; jj < M / 4; ++jj) {
                 const float4 v = qr[jj];
                 mx = fmaxf(mx, fmaxf(fmaxf(fabsf(v.x), fabsf(v.y)), fmaxf(fabsf(v.z), fabsf(v.w))));
             }
-            // the query's largest |value| over both halves; its scale: the
-            // largest power of two k <= 1 with 2 sc k max|q| / cb_scale <= 224
+            // the query's largest |value| over the four groups; its scale:
+            // the largest power of two k <= 1 with 2 sc k max|q| / cb_scale <= 224
+            mx = fmaxf(mx, __shfl_xor(mx, 16));
             mx = fmaxf(mx, __shfl_xor(mx, 32));
             const float lim = 224.f * la.cb_scale / (2.f * sc * mx);
             const float kap = mx > 0.f ? fminf(1.f, __builtin_exp2f(__builtin_floorf(__builtin_log2f(lim)))) : 1.f;
@@ -727,14 +743,14 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             const float m2 = -2.f * sc / la.cb_scale;
             auto z = [&](float v) { return __uint_as_float(__float_as_uint(v * m2) & keep); };
 #pragma unroll
-            for (int jj = 0; jj < M / 4; ++jj) {
+            for (int jj = 0; jj < KB; ++jj) {
                 const float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
                 const uint32_t lo = pack4_fp8(z(x0.x), z(x0.y), z(x0.z), z(x0.w));
                 const uint32_t hi = pack4_fp8(z(x1.x), z(x1.y), z(x1.z), z(x1.w));
                 bq8[jj] = (long)((uint64_t)lo | ((uint64_t)hi << 32));
             }
         } else {
-            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + half * (M / 2);
+            const float4 *qr = reinterpret_cast<const float4 *>(la.queries + (int64_t)qid * D) + grp * (M / 4);
             // lanes without a query take zeros through a bit mask (a branch
             // that overwrote the loaded registers would first wait for every
             // memory operation of the wave)
@@ -742,7 +758,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
             const float m2sc = -2.f * sc;
             auto z = [&](float v) { return __uint_as_float(__float_as_uint(v * m2sc) & keep); };
 #pragma unroll
-            for (int jj = 0; jj < M / 4; ++jj) {
+            for (int jj = 0; jj < KB; ++jj) {
                 const float4 x0 = qr[2 * jj], x1 = qr[2 * jj + 1];
                 const float xs[8] = {z(x0.x), z(x0.y), z(x0.z), z(x0.w), z(x1.x), z(x1.y), z(x1.z), z(x1.w)};
 #pragma unroll
@@ -759,20 +775,20 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         // item, the next item's slab 0, which was not known when the ring
         // was filled
         if (S.ns == 1) load_at(1, b1);
-        // the filter pass's survivors of a slab: register r holds entry
-        // 8 (r / 4) + 4 h + r % 4 of the slab; CHECK: the slab runs past the
+        // the filter pass's survivors of a slab: register i of tile t holds
+        // entry 16 t + 4 grp + i of the slab; CHECK: the slab runs past the
         // chunk's end
-        auto survivors = [&](const f32x16 &acc, int base, auto check) {
-            // entry in chunk of register 0, with bits 0, 1, 3 and 4 free, and
-            // its entry position
-            const uint32_t e0r = (uint32_t)(base - eb) + 4u * (uint32_t)half;
-            const uint32_t p0r = (uint32_t)(e0 + base) + 4u * (uint32_t)half;
+        auto survivors = [&](const f32x4 (&acc)[2], int base, auto check) {
+            // entry in chunk of tile 0's register 0, and its entry position
+            const uint32_t e0r = (uint32_t)(base - eb) + 4u * (uint32_t)grp;
+            const uint32_t p0r = (uint32_t)(e0 + base) + 4u * (uint32_t)grp;
 #pragma unroll
-            for (int r = 0; r < 16; ++r) {
-                const uint32_t cr = 8u * (uint32_t)(r >> 2) + (uint32_t)(r & 3);
+            for (int r = 0; r < 8; ++r) {
+                const float av = acc[r >> 2][r & 3];
+                const uint32_t cr = 16u * (uint32_t)(r >> 2) + (uint32_t)(r & 3);
                 // the wave's mask of passing lanes (compares straight into
                 // a lane mask: 41 and 40 are signed <= and <)
-                uint64_t mk = __builtin_amdgcn_sicmp(__float_as_int(acc[r]), tb, 41) & qmask;
+                uint64_t mk = __builtin_amdgcn_sicmp(__float_as_int(av), tb, 41) & qmask;
                 if (decltype(check)::value) mk &= __builtin_amdgcn_sicmp((int)(e0r + cr), ee - eb, 40);
                 if (mk) {
                     // room for 64 more (taken only when a slab has many)
@@ -780,7 +796,7 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
                     if (__builtin_amdgcn_inverse_ballot_w64(mk)) {
                         const uint32_t pos = __builtin_amdgcn_mbcnt_hi(
                             (uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, fill));
-                        s_ring[pos & (RING - 1)] = ((uint64_t)key16(acc[r]) << 32) | (p0r + cr);
+                        s_ring[pos & (RING - 1)] = ((uint64_t)key16(av) << 32) | (p0r + cr);
                         s_ringq[pos & (RING - 1)] = (uint32_t)qid;
                     }
                     fill = (uint32_t)__builtin_amdgcn_readfirstlane((int)(fill + (uint32_t)__builtin_popcountll(mk)));
@@ -789,52 +805,61 @@ __global__ void __launch_bounds__(64 * LNW) k_scan_list(ListScanArgs la) {
         };
         auto slab = [&](const Slab &x, int s) {
             const int base = eb + 32 * s;
-            if (half == 0) s_term[col] = x.t;
+            if (grp == 0) {
+                s_term[col] = x.t[0];
+                s_term[16 + col] = x.t[1];
+            }
             wave_sync();   // s_term
-            // accumulator register r of lane l: entry 8 (r / 4) + 4 h + r % 4
-            // of the slab, for query l % 32
-            f32x16 acc;
+            // register i of tile t of lane l: entry 16 t + 4 (l / 16) + i of
+            // the slab, for query l % 16
+            f32x4 acc[2];
 #pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[8 * g + 4 * half]);
-                acc[4 * g + 0] = __builtin_fmaf(t4.x, sc, qcs);
-                acc[4 * g + 1] = __builtin_fmaf(t4.y, sc, qcs);
-                acc[4 * g + 2] = __builtin_fmaf(t4.z, sc, qcs);
-                acc[4 * g + 3] = __builtin_fmaf(t4.w, sc, qcs);
+            for (int t = 0; t < 2; ++t) {
+                const float4 t4 = *reinterpret_cast<const float4 *>(&s_term[16 * t + 4 * grp]);
+                acc[t][0] = __builtin_fmaf(t4.x, sc, qcs);
+                acc[t][1] = __builtin_fmaf(t4.y, sc, qcs);
+                acc[t][2] = __builtin_fmaf(t4.z, sc, qcs);
+                acc[t][3] = __builtin_fmaf(t4.w, sc, qcs);
             }
 #pragma unroll
-            for (int jj = 0; jj < M / 4; ++jj) {
-                const int i0 = half * (M / 2) + 2 * jj;
-                const uint32_t wv = x.w[jj >> 1] >> (16 * (jj & 1));
-                const uint32_t c0 = wv & 0xFFu, c1 = (wv >> 8) & 0xFFu;
-                if (F8) {
-                    const uint64_t a8 = (uint64_t)sCB8[i0 * P + c0] | ((uint64_t)sCB8[(i0 + 1) * P + c1] << 32);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_fp8_fp8((long)a8, bq8[jj], acc, 0, 0, 0);
-                    continue;
-                }
-                const bf16x4 x0 = sCBh[i0 * P + c0], x1 = sCBh[(i0 + 1) * P + c1];
-                const bf16x8 a = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
-                acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bq[jj], acc, 0, 0, 0);
-                if (X3) {
-                    const bf16x4 y0 = sCBl[i0 * P + c0], y1 = sCBl[(i0 + 1) * P + c1];
-                    const bf16x8 al = {y0[0], y0[1], y0[2], y0[3], y1[0], y1[1], y1[2], y1[3]};
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(al, bq[jj], acc, 0, 0, 0);
-                    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bl[jj], acc, 0, 0, 0);
+            for (int t = 0; t < 2; ++t) {
+                // the run's bytes from byte 0: words of four codes
+                uint32_t r[ND];
+#pragma unroll
+                for (int u = 0; u < ND; ++u)
+                    r[u] = u + 1 < ND ? __builtin_amdgcn_alignbyte(x.w[t][u + 1], x.w[t][u], (uint32_t)sbo)
+                                      : x.w[t][u] >> (8 * sbo);
+#pragma unroll
+                for (int jj = 0; jj < KB; ++jj) {
+                    const int i0 = RUN * grp + 2 * jj;
+                    const uint32_t wv = r[jj >> 1] >> (16 * (jj & 1));
+                    const uint32_t c0 = wv & 0xFFu, c1 = (wv >> 8) & 0xFFu;
+                    if (F8) {
+                        const uint64_t a8 = (uint64_t)sCB8[i0 * P + c0] | ((uint64_t)sCB8[(i0 + 1) * P + c1] << 32);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_fp8_fp8((long)a8, bq8[jj], acc[t], 0, 0, 0);
+                        continue;
+                    }
+                    const bf16x4 x0 = sCBh[i0 * P + c0], x1 = sCBh[(i0 + 1) * P + c1];
+                    const bf16x8 a = {x0[0], x0[1], x0[2], x0[3], x1[0], x1[1], x1[2], x1[3]};
+                    acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bq[jj], acc[t], 0, 0, 0);
+                    if (X3) {
+                        const bf16x4 y0 = sCBl[i0 * P + c0], y1 = sCBl[(i0 + 1) * P + c1];
+                        const bf16x8 al = {y0[0], y0[1], y0[2], y0[3], y1[0], y1[1], y1[2], y1[3]};
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bq[jj], acc[t], 0, 0, 0);
+                        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bl[jj], acc[t], 0, 0, 0);
+                    }
                 }
             }
             // every entry of the slab lies in the chunk except in its last
             // slab (a wave-uniform test)
             const bool full = base + 32 <= ee;
             if (KIND == KIND_SAMPLE) {
-                // the 16-bit keys, two per word: word 2 g + v holds entries
-                // 8 g + 4 h + 2 v and + 1
-                uint32_t pk[8];
+                // the 16-bit keys, two per word: tile g's words hold entries
+                // 16 g + 4 grp + (0, 1) and (2, 3)
 #pragma unroll
-                for (int v = 0; v < 8; ++v) pk[v] = key16x2(acc[2 * v], acc[2 * v + 1]);
-#pragma unroll
-                for (int g = 0; g < 4; ++g) {
-                    const int e4 = base + 8 * g + 4 * half;
-                    uint32_t p0 = pk[2 * g], p1 = pk[2 * g + 1];
+                for (int g = 0; g < 2; ++g) {
+                    const int e4 = base + 16 * g + 4 * grp;
+                    uint32_t p0 = key16x2(acc[g][0], acc[g][1]), p1 = key16x2(acc[g][2], acc[g][3]);
                     if (!full) {
                         // entries past the chunk: the padding key 0xFFFF,
                         // above every key (keys are at most +inf = 0x7C00)
