@@ -813,6 +813,51 @@ def test_list_scan_small_lists(hann):
         assert agree >= 0.99, (nprobe, agree)
 
 
+@pytest.mark.parametrize("d,m", [(32, 8), (128, 32), (192, 48), (256, 64)])
+@pytest.mark.parametrize("mode", [3, 4, 5])
+def test_list_scan_other_widths(hann, oracle, d, m, mode):
+    """The list-major MFMA scan at the other widths it takes (each lane group
+    reads a run of m/4 code bytes per entry: 2, 8, 12 and 16 bytes, the
+    m = 8 runs from odd byte offsets as at m = 24): returned distances are
+    the exact reference-order ones, counts equal the FP32 table scan's, and
+    the results agree with it (>= 97 % of queries identical for split BF16,
+    >= 95 % overlap for BF16; >= 90 % for FP8, whose 3 mantissa bits order
+    the candidates of this wide, unnormalised mixture less well: 0.938 at
+    d = 192). Split BF16 at m >= 48 needs more LDS than a CU has and takes
+    the table scan."""
+    from hann_amd import _lib
+    pq = P()
+    rng = np.random.default_rng(d + m)
+    cen = rng.standard_normal((24, d)).astype(np.float32) * 3
+    x = (cen[rng.integers(0, 24, 8000)] + rng.standard_normal((8000, d))).astype(np.float32)
+    qs = (cen[rng.integers(0, 24, 160)] + rng.standard_normal((160, d))).astype(np.float32)
+    idx = pq.New(d, pq.WithCoarseK(32), pq.WithNumSubquantizers(m), pq.WithPQK(256), pq.WithKMeansIters(4),
+                 pq.WithSeed(d))
+    idx.bulk_add_arrays(np.arange(8000), x)
+    idx.Train()
+    ks = rng.choice([1, 10, 50], 160).astype(np.int32)
+    nps = rng.choice([1, 4, 8, 16], 160).astype(np.int32)
+    a_ids, a_d, a_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=_lib.SCAN_ADC_F32, rerank=4, fused=1)
+    b_ids, b_d, b_c = idx.search_batch(qs, ks, nprobe=nps, scan_mode=mode, rerank=4, fused=1)
+    if mode == 4 and m >= 48:
+        assert idx.last_scan()[0] == _lib.SCAN_ADC_F32
+        return
+    assert ran_list_scan(idx, mode)
+    assert np.array_equal(a_c, b_c)
+    same = 0
+    for qi in range(160):
+        kq = int(ks[qi])
+        for r in range(0, kq, max(1, kq // 4)):
+            j = int(b_ids[qi, r])
+            assert b_d[qi, r] == np.sqrt(np.float64(oracle.pair(0, qs[qi], x[j])))
+        assert np.all(np.diff(b_d[qi, :kq]) >= 0)
+        same += int(np.array_equal(a_ids[qi, :kq], b_ids[qi, :kq]))
+    both = [len(set(a_ids[q, :ks[q]].tolist()) & set(b_ids[q, :ks[q]].tolist())) / ks[q] for q in range(160)]
+    if mode == 4:
+        assert same >= 155, same
+    assert np.mean(both) >= (0.90 if mode == 5 else 0.95), np.mean(both)
+
+
 @pytest.mark.parametrize("mode", [3, 4])
 def test_list_scan_rows_within_selection_equal_table_scan(hann, mode):
     """When k * R covers every probed entry, both scans rerank every entry
