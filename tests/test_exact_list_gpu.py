@@ -73,11 +73,11 @@ def test_exact_list_scan_equals_oracle(hann, oracle, d, m):
 @pytest.mark.parametrize("d,m", [(40, 8), (45, 9), (130, 10), (20, 4), (200, 8)])
 def test_plain_path_rerank_any_width_equals_oracle(hann, oracle, d, m):
     """Widths the list-major exact scan does not take (the per-query exact
-    scan ranks the candidates): the rerank's two-rows-a-step kernel covers
-    32 <= d < 160 -- d = 40 exercises its 8-wide cleanup slots, 45 the
-    cleanup and a 5-element scalar tail, 130 four blocks and a tail -- and
-    the one-row-per-lane kernel the others (20, 200). Ids, distances and
-    counts identical to or_search_rerank."""
+    scan ranks the candidates), through the rerank's reference-order
+    distance: d = 40 exercises the 8-wide cleanup slots, 45 the cleanup and
+    a 5-element scalar tail, 130 four blocks and a tail, 20 no full block,
+    200 six blocks and a cleanup. Ids, distances and counts identical to
+    or_search_rerank."""
     from hann_amd import _lib
     x, c = _mixture(6000, d, 24, seed=d)
     idx = _index(x, 32, m, pq_k=64, iters=4)
