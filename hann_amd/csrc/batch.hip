@@ -31,11 +31,12 @@ static int64_t sample_stride(int64_t nb, int k) {
 }
 
 // The dense form of the certified pass, for bases of at most kRankEveryMax
-// rows (a multiple of 8) and widths up to 128 (a padded width the register-resident MFMA
-// kernel takes): every (query, row) approximate distance is written as a
-// 16-bit key, and each query's threshold comes from the k-th smallest of all
-// its keys, not of a sample, so no second threshold and no candidate-slot
-// sizing is needed; queries run in chunks of at most 256 MB of keys.
+// rows (a multiple of 8) and widths up to 128 (a padded width the
+// register-resident MFMA kernel takes): every (query, row) approximate
+// distance is written as a 16-bit key, and each query's threshold comes from
+// the k-th smallest of all its keys, not of a sample, so no second threshold
+// and no candidate-slot sizing is needed; queries run in chunks of at most
+// 256 MB of keys.
 static bool dense_path(int64_t nb, int dp, bool hinted) {
     return !hinted && nb <= kRankEveryMax && nb <= dense_select_max_rows() && nb % 8 == 0 && dp <= 128;
 }
