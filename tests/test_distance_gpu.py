@@ -158,13 +158,16 @@ def test_batch_topk_other_metrics(hann, oracle, mid):
                                               (64, 4096, 100, 50, "real"), (5, 300, 32, 10, "real"),
                                               (200, 30000, 784, 20, "real"), (1000, 8000, 64, 1, "int"),
                                               (50, 5000, 64, 10, "dup"), (700, 16384, 96, 128, "real"),
-                                              (200, 20001, 96, 50, "real"), (100, 70000, 64, 20, "int")])
+                                              (200, 20001, 96, 50, "real"), (100, 70000, 64, 20, "int"),
+                                              (7, 16, 32, 20, "real"), (9, 64, 96, 64, "dup")])
 def test_batch_topk_bf16_certified_equals_exact(hann, oracle, nq, nb, dim, k, kind):
     """BF16 MFMA candidate pass + exact FP32 ranking: ids and distances are
     identical to the exact path (the filter threshold is a proven bound).
     Bases of at most 65,536 rows, a multiple of 8, and widths up to 128 take
     the dense form (every pair's key, each query's k-th smallest); the others
-    (300 and 20,001 rows, width 784, 70,000 rows) the sampled form."""
+    (300 and 20,001 rows, width 784, 70,000 rows) the sampled form. Also k
+    above the base's size (every row a candidate) and k equal to it on
+    three distinct rows."""
     from hann_amd import _lib, core
     rng = np.random.default_rng(nq + nb + dim)
     if kind == "dup":
